@@ -1,0 +1,78 @@
+"""ctypes binding of ``liblighthit_amd.so`` (the gfx950 C-ABI).
+
+The library is loaded from the package directory, never from a cache, and a
+missing library is an error: there is no CPU fallback for any entry point.
+Device buffers are torch tensors on a ROCm device; only their addresses cross
+the boundary.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIBRARY = Path(__file__).resolve().parent / "liblighthit_amd.so"
+
+LH_OK, LH_ERR_ARGUMENT, LH_ERR_HIP = 0, 1, 2
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_f = ctypes.c_float
+_i64 = ctypes.c_int64
+
+#: Entry points and their argument types, as declared in include/lighthit_amd.h.
+SIGNATURES = {
+    "lh_last_error": (ctypes.c_char_p, []),
+    "lh_clear_error": (None, []),
+    "lh_abi_version": (_i, []),
+    "lh_readout_bins": (_i, [_p, _p, _p, _p, _i, _i, _i, _p, _p]),
+}
+
+_lib = None
+
+
+def load():
+    """The loaded library (raises ``RuntimeError`` when it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIBRARY.exists():
+            raise RuntimeError(
+                f"{LIBRARY} is missing: build it with `python -m lighthit_amd.build` "
+                "(lighthit_amd has no CPU fallback)")
+        lib = ctypes.CDLL(str(LIBRARY))
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = lib
+    return _lib
+
+
+def call(name, *args):
+    """Call entry point ``name``; map a failed status to the reference's exceptions."""
+    lib = load()
+    status = getattr(lib, name)(*args)
+    if status != LH_OK:
+        message = lib.lh_last_error().decode(errors="replace")
+        lib.lh_clear_error()
+        if status == LH_ERR_ARGUMENT:
+            raise ValueError(message)
+        raise RuntimeError(message)
+
+
+def ptr(tensor):
+    """Device address of a torch tensor (None -> NULL)."""
+    if tensor is None:
+        return None
+    if not tensor.is_cuda:
+        raise ValueError("lighthit_amd entry points take device tensors")
+    if not tensor.is_contiguous():
+        raise ValueError("lighthit_amd entry points take contiguous tensors")
+    return ctypes.c_void_p(tensor.data_ptr())
+
+
+def stream_handle(stream=None):
+    """hipStream_t of ``stream`` (default: torch's current stream)."""
+    import torch
+    stream = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(stream.cuda_stream)

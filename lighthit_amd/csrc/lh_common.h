@@ -1,0 +1,46 @@
+// Shared definitions of the lighthit_amd C-ABI library (gfx950 only).
+//
+// Every entry point returns an lh_status; on failure the message of the
+// calling thread is available from lh_last_error(). Device pointers are
+// owned by the caller; no entry point allocates device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/lighthit_amd.h"
+
+namespace lh {
+
+// Records a printf-style message for lh_last_error() of this thread.
+void set_error(const char* fmt, ...);
+
+inline int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s launch failed: %s", what, hipGetErrorString(e));
+        return LH_ERR_HIP;
+    }
+    return LH_OK;
+}
+
+}  // namespace lh
+
+#define LH_REQUIRE(cond, ...)                                   \
+    do {                                                        \
+        if (!(cond)) {                                          \
+            lh::set_error(__VA_ARGS__);                         \
+            return LH_ERR_ARGUMENT;                             \
+        }                                                       \
+    } while (0)
+
+#define LH_HIP_CALL(call)                                                     \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            lh::set_error("%s failed: %s", #call, hipGetErrorString(e_));     \
+            return LH_ERR_HIP;                                                \
+        }                                                                     \
+    } while (0)
