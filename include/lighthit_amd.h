@@ -56,6 +56,60 @@ int lh_readout_bins(const float* basis, const double* omega, const void* spectru
                     const double* origins, int n_bins, int n_freq, int n_obs,
                     float* bins, void* stream);
 
+/*
+ * Stage (b): order >=2 response of OMs to a fused multi-band axial source,
+ * applied with the band-folded kernel pack.
+ *
+ * Replaces apply_multi_spectral(bundle, sources, receivers_m, look_directions,
+ * alpha, out_scale, radial_window_m=...) (_shower_basis.py:1126-1198) with its
+ * compiled core _apply_multi_spectral (_shower_basis.py:895-1050), as called by
+ * apply_folded (shower/ge2.py:99-119) for the predicted, exact zero-frequency
+ * and full-frequency >=2 charges of the shower route (shower/api.py:347-352,
+ * 405-406, 434-435). Every band's channels keep their own emission-time
+ * reference (the reference rounds band-aligned channels instead); the result
+ * is the absolute-time spectrum (frequency, OM).
+ *
+ * Pairs of the pack are the canonical (detector degree, |mu|) list of the
+ * reference's CouplingTable (directional.py:161-169): lambda = 0..A, mu =
+ * 0..lambda. Built for acceptance degree A in {2, 3} and azimuthal degree
+ * min(source azimuthal degree, pack degree) in 0..4.
+ */
+typedef struct lh_ge2_pack {
+    const void* coefficients;   /* (bands, intervals, degree+1, pairs, 4, freq_stride)
+                                   complex64, cubic powers t^3..t^0 in log r */
+    const double* log_r;        /* (intervals+1) */
+    const double* absorption;   /* (bands) common attenuation [1/m] */
+    const double* speed;        /* (bands) flight speed [m/ns] */
+    const double* omega;        /* (freq_stride) [rad/ns] */
+    int bands, intervals, degree, acceptance_degree, freq_stride;
+    int flight;                 /* 1: radial splines carry the flight phase */
+} lh_ge2_pack;
+
+typedef struct lh_ge2_source {
+    const double* points_m;      /* (points, 3) in the shower frame (first, second, axis) */
+    const void* channels_data;   /* (bands, points, channels, freq_stride) complex64 */
+    const double* reference_ns;  /* (bands) emission-time reference of each band */
+    int points, channels, degree, azimuthal_degree, freq_stride;
+} lh_ge2_source;
+
+typedef struct lh_ge2_receivers {
+    const double* positions;     /* (count, 3) in the shower frame */
+    const double* looks;         /* (count, 3) unit look directions in the shower frame */
+    const double* out_scale;     /* (bands, count) detector scale (OM area) */
+    int count;
+} lh_ge2_receivers;
+
+/*
+ * Descriptors are host structs holding device pointers. alpha (A+1) double
+ * are the acceptance Legendre coefficients. The first `freq` frequencies are
+ * evaluated (freq = 1: the zero-frequency charge). Points at a distance
+ * outside [radial_low, radial_high] are skipped (radial_window_m).
+ * out (freq, count) complex64.
+ */
+int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source,
+                 const lh_ge2_receivers* receivers, const double* alpha, int freq,
+                 double radial_low, double radial_high, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,18 +40,25 @@ def load():
                 f"{LIBRARY} is missing: build it with `python -m lighthit_amd.build` "
                 "(lighthit_amd has no CPU fallback)")
         lib = ctypes.CDLL(str(LIBRARY))
-        for name, (restype, argtypes) in SIGNATURES.items():
+        for name in ("lh_last_error", "lh_clear_error", "lh_abi_version"):
             fn = getattr(lib, name)
-            fn.restype = restype
-            fn.argtypes = argtypes
+            fn.restype, fn.argtypes = SIGNATURES[name]
         _lib = lib
     return _lib
+
+
+_bound = set()
 
 
 def call(name, *args):
     """Call entry point ``name``; map a failed status to the reference's exceptions."""
     lib = load()
-    status = getattr(lib, name)(*args)
+    fn = getattr(lib, name)
+    if name not in _bound:
+        restype, argtypes = SIGNATURES[name]
+        fn.restype, fn.argtypes = restype, argtypes
+        _bound.add(name)
+    status = fn(*args)
     if status != LH_OK:
         message = lib.lh_last_error().decode(errors="replace")
         lib.lh_clear_error()

@@ -1,0 +1,687 @@
+// Stage (b): order >=2 apply of the band-folded RTE kernel pack.
+//
+// Equation (*) of the reference (directional.py:17-37) as fused in
+// _apply_multi_spectral (_shower_basis.py:895-1050): for every OM d and
+// source point p inside the radial window,
+//
+//   out[w, d] += sum_band e^{i w ref_band} scale(d,p,band) e^{i w r/v_band}
+//                sum_l sum_a rot_l[a](p, band, w) cs_l[a](d,p, band, w)
+//   rot_l[a] = sum_b E_l[a, b](d,p) ch[band, p, l-block + b, w]
+//   cs_l[a]  = sum_pairs (alpha Y)(d,p) cubic(coef[band, j(d,p), l, pair, :, w], t(d,p))
+//
+// Two launch shapes share the geometry code:
+//   * wide (many frequencies): one workgroup per OM. Its threads first build,
+//     in FP64, the per-(OM, point) record (Wigner rotation rows E_l of every
+//     degree, detector harmonics times acceptance, spline interval and
+//     offset, attenuation) of a chunk of points into LDS; then each thread
+//     owns one (band, frequency) lane and sums the chunk's points, reading
+//     the records as LDS broadcasts and the pack and channels with coalesced
+//     loads along the frequency axis. Bands are combined in a fixed order at
+//     the end, so the result does not depend on scheduling.
+//   * narrow (one frequency, the zero-frequency charge): one thread per
+//     (OM, point) advances the Wigner recurrences degree by degree in
+//     registers; the workgroup reduces its points in a fixed tree order.
+// The heavy contraction is FP32; geometry, spline offset and the flight
+// phase w r / v (up to ~10^3 rad) are formed in FP64 and reduced modulo 2 pi.
+#include "lh_common.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxBands = 9;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+struct Ge2Params {
+    const float2* coef;        // (bands, intervals, degree+1, pairs, 4, cstride)
+    const double* log_r;       // (intervals+1)
+    const double* absorption;  // (bands)
+    const double* speed;       // (bands)
+    const double* omega;       // (nfreq..)
+    const double* alpha;       // (A+1)
+    const double* points;      // (npoints, 3) frame coordinates
+    const float2* channels;    // (bands, npoints, nchan, chstride)
+    const double* band_ref;    // (bands) emission-time reference of each band [ns]
+    const double* receivers;   // (nrec, 3) frame coordinates
+    const double* looks;       // (nrec, 3) frame coordinates
+    const double* out_scale;   // (bands, nrec)
+    float2* out;               // (nfreq, nrec)
+    int bands, intervals, degree, cstride, flight;
+    int npoints, nchan, src_az, chstride;
+    int nrec, nfreq;
+    double r_low, r_high;
+    int chunk, record_words;
+};
+
+__device__ __forceinline__ double parity(int n) { return (n & 1) ? -1.0 : 1.0; }
+
+__device__ __forceinline__ double ipow(double x, int k) {
+    double r = 1.0;
+    for (int i = 0; i < k; ++i) r *= x;
+    return r;
+}
+
+// sqrt(binom(2j, j - nu)), exact in double for the small degrees used here.
+__device__ __forceinline__ double start_norm(int j, int nu) {
+    double b = 1.0;
+    const int k = j - nu;
+    for (int i = 1; i <= k; ++i) b = b * (2 * j - k + i) / i;
+    return sqrt(b);
+}
+
+// d^j_(mu nu) at j = max(|mu|, |nu|) (_directional_numba.py:36-49).
+__device__ double start_value(int j, int mu, int nu, double half_cos, double half_sin) {
+    double sign = 1.0;
+    if (abs(nu) == j && abs(mu) != j) {
+        const int tmp = mu;
+        mu = nu;
+        nu = tmp;
+        sign *= parity(mu - nu);
+    }
+    if (mu == -j) {
+        mu = j;
+        nu = -nu;
+        sign *= parity(j + nu);
+    }
+    return sign * start_norm(j, nu) * ipow(half_cos, j + nu) * ipow(-half_sin, j - nu);
+}
+
+// Real orthonormal harmonics of a vector up to degree A, flat l*l+l+m
+// (_directional_numba.py:91-126).
+template <int A>
+__device__ void harmonics(double x, double y, double z, double (&out)[(A + 1) * (A + 1)]) {
+    const double radius = sqrt(x * x + y * y + z * z);
+    const double mu = z / radius;
+    const double transverse = sqrt(x * x + y * y);
+    const double sine = transverse / radius;
+    const double cp = transverse == 0.0 ? 1.0 : x / transverse;
+    const double sp = transverse == 0.0 ? 0.0 : y / transverse;
+    double cm = 1.0, sm = 0.0;
+    double diagonal = 0.28209479177387814347403972578039;  // 1/sqrt(4 pi)
+#pragma unroll
+    for (int m = 0; m <= A; ++m) {
+        if (m) {
+            diagonal *= sqrt((2.0 * m + 1) / (2.0 * m)) * sine;
+            const double ncm = cm * cp - sm * sp;
+            const double nsm = sm * cp + cm * sp;
+            cm = ncm;
+            sm = nsm;
+        }
+        double v0 = diagonal, v1 = 0.0;
+#pragma unroll
+        for (int ell = m; ell <= A; ++ell) {
+            double value;
+            if (ell == m) {
+                value = v0;
+            } else if (ell == m + 1) {
+                value = sqrt(2.0 * m + 3) * mu * v0;
+                v1 = value;
+            } else {
+                const double aa = sqrt((4.0 * ell * ell - 1) / ((double)ell * ell - m * m));
+                const double bb = sqrt(((2.0 * ell + 1) * ((double)(ell - 1) * (ell - 1) - m * m))
+                                       / ((2.0 * ell - 3) * ((double)ell * ell - m * m)));
+                value = aa * mu * v1 - bb * v0;
+                v0 = v1;
+                v1 = value;
+            }
+            const int centre = ell * ell + ell;
+            if (m == 0) {
+                out[centre] = value;
+            } else {
+                out[centre - m] = 1.4142135623730950488 * value * sm;
+                out[centre + m] = 1.4142135623730950488 * value * cm;
+            }
+        }
+    }
+}
+
+// Wigner d^l_(mu nu)(theta) for mu = n in [0, M], nu = +-p with p in [0, A],
+// advanced one degree at a time (the l-recurrence of _wigner_into,
+// _directional_numba.py:52-88), and the real rotation rows E_l built from them
+// (_shower_basis.py:1006-1029).
+template <int A, int M>
+struct Rotation {
+    double cosine, half_cos, half_sin;
+    double cosn[M + 1], sinn[M + 1];
+    double cur[A + 1][M + 1][2], prev[A + 1][M + 1][2];
+
+    __device__ void init(double c, double phi) {
+        cosine = c;
+        half_cos = sqrt((1.0 + c) / 2.0);
+        half_sin = sqrt((1.0 - c) / 2.0);
+#pragma unroll
+        for (int n = 0; n <= M; ++n) {
+            cosn[n] = cos(n * phi);
+            sinn[n] = sin(n * phi);
+        }
+#pragma unroll
+        for (int p = 0; p <= A; ++p)
+#pragma unroll
+            for (int n = 0; n <= M; ++n)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) cur[p][n][s] = prev[p][n][s] = 0.0;
+    }
+
+    // Bring every recurrence to degree l (call with l = 0, 1, 2, ...).
+    __device__ void step(int l) {
+#pragma unroll
+        for (int p = 0; p <= A; ++p)
+#pragma unroll
+            for (int n = 0; n <= M; ++n)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    if (p == 0 && s == 1) continue;
+                    const int nu = s ? -p : p;
+                    const int mu = n;
+                    const int start = p > n ? p : n;
+                    if (l < start) {
+                        cur[p][n][s] = 0.0;
+                    } else if (l == start) {
+                        cur[p][n][s] = start_value(start, mu, nu, half_cos, half_sin);
+                        prev[p][n][s] = 0.0;
+                    } else {
+                        const int ell = l - 1;
+                        double value;
+                        if (ell == 0) {
+                            value = cosine * cur[p][n][s];
+                        } else {
+                            double lower = (double)(ell * ell - mu * mu) * (double)(ell * ell - nu * nu);
+                            lower = lower > 0.0 ? sqrt(lower) : 0.0;
+                            const double upper = sqrt((double)((ell + 1) * (ell + 1) - mu * mu)
+                                                      * (double)((ell + 1) * (ell + 1) - nu * nu));
+                            value = ((2 * ell + 1) * (ell * (ell + 1) * cosine - mu * nu) * cur[p][n][s]
+                                     - (ell + 1) * lower * prev[p][n][s]) / (ell * upper);
+                        }
+                        prev[p][n][s] = cur[p][n][s];
+                        cur[p][n][s] = value;
+                    }
+                }
+    }
+
+    // E_l[a][b], a = A + nu, b = M + m (after step(l)).
+    __device__ void rows(int l, float (&E)[2 * A + 1][2 * M + 1]) const {
+        const double root2 = 1.4142135623730950488;
+#pragma unroll
+        for (int a = 0; a < 2 * A + 1; ++a)
+#pragma unroll
+            for (int b = 0; b < 2 * M + 1; ++b) E[a][b] = 0.f;
+        const int top_n = l < M ? l : M;
+        const int top_p = l < A ? l : A;
+        E[A][M] = (float)cur[0][0][0];
+#pragma unroll
+        for (int n = 1; n <= M; ++n) {
+            if (n > top_n) continue;
+            const double base = root2 * parity(n) * cur[0][n][0];
+            E[A][M + n] = (float)(base * cosn[n]);
+            E[A][M - n] = (float)(base * sinn[n]);
+        }
+#pragma unroll
+        for (int p = 1; p <= A; ++p) {
+            if (p > top_p) continue;
+            E[A + p][M] = (float)(root2 * parity(p) * cur[p][0][0]);
+#pragma unroll
+            for (int n = 1; n <= M; ++n) {
+                if (n > top_n) continue;
+                const double cross = parity(p) * cur[p][n][0];
+                const double straight = cur[p][n][1];
+                const double cc = parity(n) * (cross + straight);
+                const double ss = parity(n) * (cross - straight);
+                E[A + p][M + n] = (float)(cc * cosn[n]);
+                E[A + p][M - n] = (float)(cc * sinn[n]);
+                E[A - p][M + n] = (float)(-ss * sinn[n]);
+                E[A - p][M - n] = (float)(ss * cosn[n]);
+            }
+        }
+    }
+};
+
+// Geometry of one (OM, point) pair in FP64. Returns false outside the window.
+template <int A, int M>
+struct PairGeometry {
+    double radius, t;
+    int j;
+    double cosine, phi;
+    float pos[(A + 1) * (A + 2) / 2], neg[(A + 1) * (A + 2) / 2];
+
+    __device__ bool compute(const Ge2Params& P, int d, int p) {
+        const double dx = P.receivers[3 * d + 0] - P.points[3 * p + 0];
+        const double dy = P.receivers[3 * d + 1] - P.points[3 * p + 1];
+        const double dz = P.receivers[3 * d + 2] - P.points[3 * p + 2];
+        radius = sqrt(dx * dx + dy * dy + dz * dz);
+        if (radius < P.r_low || radius > P.r_high) return false;
+        cosine = dz / radius;
+        cosine = cosine > 1.0 ? 1.0 : (cosine < -1.0 ? -1.0 : cosine);
+        phi = atan2(dy, dx);
+        const double transverse = sqrt(dx * dx + dy * dy);
+        const double sine = transverse / radius;
+        const double cp = transverse == 0.0 ? 1.0 : dx / transverse;
+        const double sp = transverse == 0.0 ? 0.0 : dy / transverse;
+        const double lx = P.looks[3 * d + 0], ly = P.looks[3 * d + 1], lz = P.looks[3 * d + 2];
+        const double a1 = cosine * cp * lx + cosine * sp * ly - sine * lz;
+        const double a2 = -sp * lx + cp * ly;
+        const double a3 = sine * cp * lx + sine * sp * ly + cosine * lz;
+        double Y[(A + 1) * (A + 1)];
+        harmonics<A>(a1, a2, a3, Y);
+        int k = 0;
+#pragma unroll
+        for (int lam = 0; lam <= A; ++lam)
+#pragma unroll
+            for (int mu = 0; mu <= lam; ++mu, ++k) {
+                const int c = lam * lam + lam;
+                pos[k] = (float)(P.alpha[lam] * Y[c + mu]);
+                neg[k] = mu ? (float)(P.alpha[lam] * Y[c - mu]) : 0.f;
+            }
+        const double logr = log(radius);
+        // searchsorted(log_r, logr, 'right') - 1, clamped to the intervals
+        int lo = 0, hi = P.intervals + 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (P.log_r[mid] <= logr) lo = mid + 1; else hi = mid;
+        }
+        j = lo - 1;
+        j = j < 0 ? 0 : (j > P.intervals - 1 ? P.intervals - 1 : j);
+        t = logr - P.log_r[j];
+        return true;
+    }
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 phase_of(double angle) {
+    const double reduced = angle - kTwoPi * rint(angle * (1.0 / kTwoPi));
+    float s, c;
+    sincosf((float)reduced, &s, &c);
+    return make_float2(c, s);
+}
+
+// Record layout (32-bit words) of one (OM, point) in LDS for the wide kernel.
+template <int A, int M>
+struct Record {
+    static constexpr int NA = 2 * A + 1, NM = 2 * M + 1, NPAIR = (A + 1) * (A + 2) / 2;
+    static __host__ __device__ int words(int degree, int bands) {
+        // valid, j, t, (pad), radius (2 words), G, scale, E; padded to 4 words
+        const int w = 6 + 2 * NPAIR + bands + (degree + 1) * NA * NM;
+        return (w + 3) & ~3;
+    }
+};
+
+template <int A, int M>
+__global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
+    using R = Record<A, M>;
+    constexpr int NA = R::NA, NM = R::NM, NPAIR = R::NPAIR;
+    extern __shared__ float lds[];
+    const int d = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lanes = P.bands * P.nfreq;
+    // Per-(band, frequency) accumulators, each owned by one thread, then the
+    // chunk's point records.
+    float2* acc = reinterpret_cast<float2*>(lds);
+    float* smem = lds + ((2 * lanes + 3) & ~3);
+    for (int lw = tid; lw < lanes; lw += kThreads) acc[lw] = make_float2(0.f, 0.f);
+
+    for (int base = 0; base < P.npoints; base += P.chunk) {
+        const int count = min(P.chunk, P.npoints - base);
+        __syncthreads();
+        for (int i = tid; i < count; i += kThreads) {
+            float* rec = smem + (size_t)i * P.record_words;
+            int* irec = reinterpret_cast<int*>(rec);
+            PairGeometry<A, M> g;
+            if (!g.compute(P, d, base + i)) {
+                irec[0] = 0;
+                continue;
+            }
+            irec[0] = 1;
+            irec[1] = g.j;
+            rec[2] = (float)g.t;
+            reinterpret_cast<double*>(rec)[2] = g.radius;  // words 4-5... see below
+#pragma unroll
+            for (int k = 0; k < NPAIR; ++k) {
+                rec[6 + 2 * k] = g.pos[k];
+                rec[7 + 2 * k] = g.neg[k];
+            }
+            float* scale = rec + 6 + 2 * NPAIR;
+            for (int lam = 0; lam < P.bands; ++lam)
+                scale[lam] = (float)(P.out_scale[(size_t)lam * P.nrec + d]
+                                     * exp(-P.absorption[lam] * g.radius)
+                                     / (4.0 * 3.14159265358979323846 * g.radius * g.radius));
+            float* E = scale + P.bands;
+            Rotation<A, M> rot;
+            rot.init(g.cosine, g.phi);
+            for (int l = 0; l <= P.degree; ++l) {
+                rot.step(l);
+                float El[NA][NM];
+                rot.rows(l, El);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int b = 0; b < NM; ++b) E[(l * NA + a) * NM + b] = El[a][b];
+            }
+        }
+        __syncthreads();
+
+        for (int lw = tid; lw < lanes; lw += kThreads) {
+            const int lam = lw / P.nfreq;
+            const int w = lw - lam * P.nfreq;
+            const double inv_speed_w = P.flight ? P.omega[w] / P.speed[lam] : 0.0;
+            const size_t cl = (size_t)NPAIR * 4 * P.cstride;  // per degree
+            float2 total = acc[lw];
+            for (int i = 0; i < count; ++i) {
+                const float* rec = smem + (size_t)i * P.record_words;
+                const int* irec = reinterpret_cast<const int*>(rec);
+                if (irec[0] == 0) continue;
+                const int j = irec[1];
+                const float t = rec[2];
+                const double radius = reinterpret_cast<const double*>(rec)[2];
+                const float* G = rec + 6;
+                const float scale = rec[6 + 2 * NPAIR + lam];
+                const float* E = rec + 6 + 2 * NPAIR + P.bands;
+                const float2* cbase = P.coef + ((size_t)lam * P.intervals + j) * (P.degree + 1) * cl + w;
+                const float2* chbase = P.channels + ((size_t)lam * P.npoints + base + i) * P.nchan * P.chstride + w;
+                float2 resp = make_float2(0.f, 0.f);
+                int starts = 0;
+                for (int l = 0; l <= P.degree; ++l) {
+                    const int top_p = l < A ? l : A;
+                    const int keep = l < M ? l : M;
+                    const int centre = starts + keep;
+                    starts += 2 * (l < P.src_az ? l : P.src_az) + 1;
+                    float2 chv[NM];
+#pragma unroll
+                    for (int b = 0; b < NM; ++b) {
+                        const int m = b - M;
+                        chv[b] = (m >= -keep && m <= keep)
+                                     ? chbase[(size_t)(centre + m) * P.chstride]
+                                     : make_float2(0.f, 0.f);
+                    }
+                    float2 cs[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) cs[a] = make_float2(0.f, 0.f);
+                    const float2* cdeg = cbase + (size_t)l * cl;
+                    int k = 0;
+#pragma unroll
+                    for (int dl = 0; dl <= A; ++dl)
+#pragma unroll
+                        for (int mu = 0; mu <= dl; ++mu, ++k) {
+                            if (mu > top_p) continue;
+                            const float2* c = cdeg + (size_t)k * 4 * P.cstride;
+                            const float2 c0 = c[0], c1 = c[P.cstride], c2 = c[2 * P.cstride],
+                                         c3 = c[3 * P.cstride];
+                            float2 radial;
+                            radial.x = ((c0.x * t + c1.x) * t + c2.x) * t + c3.x;
+                            radial.y = ((c0.y * t + c1.y) * t + c2.y) * t + c3.y;
+                            const float pos = G[2 * k], neg = G[2 * k + 1];
+                            cs[A + mu].x += pos * radial.x;
+                            cs[A + mu].y += pos * radial.y;
+                            if (mu) {
+                                cs[A - mu].x += neg * radial.x;
+                                cs[A - mu].y += neg * radial.y;
+                            }
+                        }
+                    const float* El = E + l * NA * NM;
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) {
+                        const int nu = a - A;
+                        if (nu < -top_p || nu > top_p) continue;
+                        float2 r = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int b = 0; b < NM; ++b) {
+                            const float e = El[a * NM + b];
+                            r.x += e * chv[b].x;
+                            r.y += e * chv[b].y;
+                        }
+                        const float2 v = cmul(r, cs[a]);
+                        resp.x += v.x;
+                        resp.y += v.y;
+                    }
+                }
+                float2 sp = make_float2(scale, 0.f);
+                if (P.flight) {
+                    const float2 ph = phase_of(inv_speed_w * radius);
+                    sp = make_float2(scale * ph.x, scale * ph.y);
+                }
+                const float2 v = cmul(resp, sp);
+                total.x += v.x;
+                total.y += v.y;
+            }
+            acc[lw] = total;
+        }
+    }
+
+    // Combine the bands of every frequency in a fixed order.
+    __syncthreads();
+    float2* red = acc;
+    for (int lw = tid; lw < lanes; lw += kThreads) {
+        const int lam = lw / P.nfreq;
+        const int w = lw - lam * P.nfreq;
+        red[lw] = cmul(acc[lw], phase_of(P.omega[w] * P.band_ref[lam]));
+    }
+    __syncthreads();
+    for (int w = tid; w < P.nfreq; w += kThreads) {
+        float2 sum = make_float2(0.f, 0.f);
+        for (int lam = 0; lam < P.bands; ++lam) {
+            sum.x += red[lam * P.nfreq + w].x;
+            sum.y += red[lam * P.nfreq + w].y;
+        }
+        P.out[(size_t)w * P.nrec + d] = sum;
+    }
+}
+
+// One frequency: one thread per (OM, point); fixed-order tree reduction.
+template <int A, int M>
+__global__ __launch_bounds__(kThreads) void ge2_narrow(Ge2Params P, int w) {
+    constexpr int NA = 2 * A + 1, NM = 2 * M + 1;
+    __shared__ float2 red[kMaxBands][kThreads];
+    const int d = blockIdx.x;
+    const int tid = threadIdx.x;
+    float2 acc[kMaxBands];
+#pragma unroll
+    for (int b = 0; b < kMaxBands; ++b) acc[b] = make_float2(0.f, 0.f);
+
+    for (int p = tid; p < P.npoints; p += kThreads) {
+        PairGeometry<A, M> g;
+        if (!g.compute(P, d, p)) continue;
+        const float t = (float)g.t;
+        Rotation<A, M> rot;
+        rot.init(g.cosine, g.phi);
+        float2 resp[kMaxBands];
+#pragma unroll
+        for (int b = 0; b < kMaxBands; ++b) resp[b] = make_float2(0.f, 0.f);
+        const size_t cl = (size_t)((A + 1) * (A + 2) / 2) * 4 * P.cstride;
+        int starts = 0;
+        for (int l = 0; l <= P.degree; ++l) {
+            rot.step(l);
+            float El[NA][NM];
+            rot.rows(l, El);
+            const int top_p = l < A ? l : A;
+            const int keep = l < M ? l : M;
+            const int centre = starts + keep;
+            starts += 2 * (l < P.src_az ? l : P.src_az) + 1;
+#pragma unroll
+            for (int lam = 0; lam < kMaxBands; ++lam) {
+                if (lam >= P.bands) break;
+                const float2* chbase = P.channels + ((size_t)lam * P.npoints + p) * P.nchan * P.chstride + w;
+                float2 chv[NM];
+#pragma unroll
+                for (int b = 0; b < NM; ++b) {
+                    const int m = b - M;
+                    chv[b] = (m >= -keep && m <= keep) ? chbase[(size_t)(centre + m) * P.chstride]
+                                                       : make_float2(0.f, 0.f);
+                }
+                float2 cs[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) cs[a] = make_float2(0.f, 0.f);
+                const float2* cdeg = P.coef + ((size_t)lam * P.intervals + g.j) * (P.degree + 1) * cl
+                                     + (size_t)l * cl + w;
+                int k = 0;
+#pragma unroll
+                for (int dl = 0; dl <= A; ++dl)
+#pragma unroll
+                    for (int mu = 0; mu <= dl; ++mu, ++k) {
+                        if (mu > top_p) continue;
+                        const float2* c = cdeg + (size_t)k * 4 * P.cstride;
+                        const float2 c0 = c[0], c1 = c[P.cstride], c2 = c[2 * P.cstride],
+                                     c3 = c[3 * P.cstride];
+                        float2 radial;
+                        radial.x = ((c0.x * t + c1.x) * t + c2.x) * t + c3.x;
+                        radial.y = ((c0.y * t + c1.y) * t + c2.y) * t + c3.y;
+                        cs[A + mu].x += g.pos[k] * radial.x;
+                        cs[A + mu].y += g.pos[k] * radial.y;
+                        if (mu) {
+                            cs[A - mu].x += g.neg[k] * radial.x;
+                            cs[A - mu].y += g.neg[k] * radial.y;
+                        }
+                    }
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                    const int nu = a - A;
+                    if (nu < -top_p || nu > top_p) continue;
+                    float2 r = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int b = 0; b < NM; ++b) {
+                        r.x += El[a][b] * chv[b].x;
+                        r.y += El[a][b] * chv[b].y;
+                    }
+                    const float2 v = cmul(r, cs[a]);
+                    resp[lam].x += v.x;
+                    resp[lam].y += v.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int lam = 0; lam < kMaxBands; ++lam) {
+            if (lam >= P.bands) break;
+            const double scale = P.out_scale[(size_t)lam * P.nrec + d] * exp(-P.absorption[lam] * g.radius)
+                                 / (4.0 * 3.14159265358979323846 * g.radius * g.radius);
+            float2 sp = make_float2((float)scale, 0.f);
+            if (P.flight) {
+                const float2 ph = phase_of(P.omega[w] * g.radius / P.speed[lam]);
+                sp = make_float2((float)scale * ph.x, (float)scale * ph.y);
+            }
+            const float2 v = cmul(resp[lam], sp);
+            acc[lam].x += v.x;
+            acc[lam].y += v.y;
+        }
+    }
+#pragma unroll
+    for (int lam = 0; lam < kMaxBands; ++lam) red[lam][tid] = acc[lam];
+    __syncthreads();
+    for (int half = kThreads / 2; half > 0; half >>= 1) {
+        if (tid < half) {
+#pragma unroll
+            for (int lam = 0; lam < kMaxBands; ++lam) {
+                red[lam][tid].x += red[lam][tid + half].x;
+                red[lam][tid].y += red[lam][tid + half].y;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        float2 sum = make_float2(0.f, 0.f);
+        for (int lam = 0; lam < P.bands; ++lam) {
+            const float2 v = cmul(red[lam][0], phase_of(P.omega[w] * P.band_ref[lam]));
+            sum.x += v.x;
+            sum.y += v.y;
+        }
+        P.out[(size_t)w * P.nrec + d] = sum;
+    }
+}
+
+constexpr int kLdsBudget = 64 * 1024;
+
+template <int A, int M>
+int launch(Ge2Params P, hipStream_t stream) {
+    if (P.nfreq == 1) {
+        hipLaunchKernelGGL((ge2_narrow<A, M>), dim3(P.nrec), dim3(kThreads), 0, stream, P, 0);
+        return lh::check_launch("lh_ge2_apply (narrow)");
+    }
+    P.record_words = Record<A, M>::words(P.degree, P.bands);
+    const int reduce_words = (2 * P.bands * P.nfreq + 3) & ~3;
+    LH_REQUIRE(reduce_words * 4 <= kLdsBudget / 2, "lh_ge2_apply: bands x frequencies too large");
+    P.chunk = std::min(64, (kLdsBudget - reduce_words * 4) / (4 * P.record_words));
+    LH_REQUIRE(P.chunk >= 1, "lh_ge2_apply: degree too large for the LDS record");
+    const size_t lds = (size_t)(reduce_words + P.chunk * P.record_words) * 4;
+    hipLaunchKernelGGL((ge2_wide<A, M>), dim3(P.nrec), dim3(kThreads), lds, stream, P);
+    return lh::check_launch("lh_ge2_apply (wide)");
+}
+
+template <int A>
+int dispatch_m(int m, const Ge2Params& P, hipStream_t s) {
+    switch (m) {
+        case 0: return launch<A, 0>(P, s);
+        case 1: return launch<A, 1>(P, s);
+        case 2: return launch<A, 2>(P, s);
+        case 3: return launch<A, 3>(P, s);
+        case 4: return launch<A, 4>(P, s);
+        default: break;
+    }
+    lh::set_error("lh_ge2_apply: azimuthal degree %d not built (0..4)", m);
+    return LH_ERR_ARGUMENT;
+}
+
+}  // namespace
+
+extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source,
+                            const lh_ge2_receivers* receivers, const double* alpha, int nfreq,
+                            double radial_low, double radial_high, void* out, void* stream) {
+    LH_REQUIRE(pack && source && receivers, "lh_ge2_apply: null descriptor");
+    LH_REQUIRE(receivers->count >= 0, "lh_ge2_apply: negative receiver count");
+    if (receivers->count == 0) return LH_OK;
+    LH_REQUIRE(pack->bands >= 1 && pack->bands <= kMaxBands, "lh_ge2_apply: 1..%d bands supported",
+               kMaxBands);
+    LH_REQUIRE(pack->intervals >= 1 && pack->degree >= 0, "lh_ge2_apply: empty pack");
+    LH_REQUIRE(nfreq >= 1 && nfreq <= pack->freq_stride && nfreq <= source->freq_stride,
+               "lh_ge2_apply: %d frequencies requested, pack holds %d and source %d", nfreq,
+               pack->freq_stride, source->freq_stride);
+    LH_REQUIRE(source->degree >= pack->degree, "source degree is insufficient");
+    int expected = 0;
+    for (int l = 0; l <= source->degree; ++l)
+        expected += 2 * std::min(l, source->azimuthal_degree) + 1;
+    LH_REQUIRE(source->channels == expected,
+               "lh_ge2_apply: %d source channels, degree %d / azimuthal %d needs %d",
+               source->channels, source->degree, source->azimuthal_degree, expected);
+    LH_REQUIRE(source->points >= 0, "lh_ge2_apply: negative point count");
+    LH_REQUIRE(radial_low < radial_high, "radial window must lie inside the kernel cache");
+    LH_REQUIRE(pack->coefficients && pack->log_r && pack->absorption && pack->speed && pack->omega
+                   && alpha && receivers->positions && receivers->looks && receivers->out_scale
+                   && out && (source->points == 0 || (source->points_m && source->channels_data
+                                                      && source->reference_ns)),
+               "lh_ge2_apply: null device pointer");
+    Ge2Params P{};
+    P.coef = static_cast<const float2*>(pack->coefficients);
+    P.log_r = pack->log_r;
+    P.absorption = pack->absorption;
+    P.speed = pack->speed;
+    P.omega = pack->omega;
+    P.alpha = alpha;
+    P.points = source->points_m;
+    P.channels = static_cast<const float2*>(source->channels_data);
+    P.band_ref = source->reference_ns;
+    P.receivers = receivers->positions;
+    P.looks = receivers->looks;
+    P.out_scale = receivers->out_scale;
+    P.out = static_cast<float2*>(out);
+    P.bands = pack->bands;
+    P.intervals = pack->intervals;
+    P.degree = pack->degree;
+    P.cstride = pack->freq_stride;
+    P.flight = pack->flight;
+    P.npoints = source->points;
+    P.nchan = source->channels;
+    P.src_az = source->azimuthal_degree;
+    P.chstride = source->freq_stride;
+    P.nrec = receivers->count;
+    P.nfreq = nfreq;
+    P.r_low = radial_low;
+    P.r_high = radial_high;
+    const int m = std::min(source->azimuthal_degree, pack->degree);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (pack->acceptance_degree) {
+        case 2: return dispatch_m<2>(m, P, s);
+        case 3: return dispatch_m<3>(m, P, s);
+        default: break;
+    }
+    lh::set_error("lh_ge2_apply: acceptance degree %d not built (2, 3)", pack->acceptance_degree);
+    return LH_ERR_ARGUMENT;
+}
