@@ -1,0 +1,205 @@
+"""Charged-step sources: the two-field Cherenkov elements and synthetic showers.
+
+Mirrors ``SpectralLightElements``, ``SourcePose`` and ``SyntheticShower``
+(``sources.py:12-58, 60-78, 110-262, 396-441``): the same fields,
+validation, placement (photon-weighted centroid and principal axis moved onto
+a pose) and the same seeded generator, so a given seed gives bit-identical
+elements.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+#: Fine-structure constant (the reference's ``g4_source.FINE_STRUCTURE``).
+FINE_STRUCTURE = 7.2973525693e-3
+C_M_PER_NS = 0.299792458
+
+
+def _unit_vector(value, name):
+    vector = np.asarray(value, float)
+    if vector.shape != (3,) or not np.isfinite(vector).all():
+        raise ValueError(f"{name} must be a finite 3-vector")
+    norm = np.linalg.norm(vector)
+    if norm <= 0:
+        raise ValueError(f"{name} must be nonzero")
+    return vector / norm
+
+
+def _principal_axis(elements):
+    points = elements.midpoints_m
+    weight = np.asarray(elements.coefficient0, float)
+    share = weight / weight.sum()
+    centre = np.sum(share[:, None] * points, axis=0)
+    offset = points - centre
+    covariance = np.einsum("i,ij,ik->jk", share, offset, offset)
+    values, vectors = np.linalg.eigh(covariance)
+    if float(np.max(np.abs(values))) <= 1e-24:
+        axis = np.sum(share[:, None] * elements.direction, axis=0)
+    else:
+        axis = vectors[:, int(np.argmax(values))]
+    if float(np.sum(share * (elements.direction @ axis))) < 0:
+        axis = -axis
+    return centre, axis / np.linalg.norm(axis)
+
+
+def _rotation_between(source, target):
+    source = _unit_vector(source, "source direction")
+    target = _unit_vector(target, "target direction")
+    cosine = float(np.clip(source @ target, -1.0, 1.0))
+    if np.isclose(cosine, 1.0, atol=1e-14):
+        return np.eye(3)
+    if np.isclose(cosine, -1.0, atol=1e-14):
+        helper = np.array([1.0, 0.0, 0.0])
+        if abs(float(source @ helper)) > 0.9:
+            helper = np.array([0.0, 1.0, 0.0])
+        axis = np.cross(source, helper)
+        axis /= np.linalg.norm(axis)
+        return 2 * np.outer(axis, axis) - np.eye(3)
+    cross = np.cross(source, target)
+    skew = np.array([[0.0, -cross[2], cross[1]],
+                     [cross[2], 0.0, -cross[0]],
+                     [-cross[1], cross[0], 0.0]])
+    sine2 = float(cross @ cross)
+    return np.eye(3) + skew + skew @ skew * ((1.0 - cosine) / sine2)
+
+
+@dataclass(frozen=True)
+class SourcePose:
+    """Target centroid, axis and earliest emission time of a source."""
+    position_m: np.ndarray
+    direction: np.ndarray
+    time_ns: float = 0.0
+
+    def __post_init__(self):
+        position = np.asarray(self.position_m, float).copy()
+        if position.shape != (3,) or not np.isfinite(position).all():
+            raise ValueError("position_m must be a finite 3-vector")
+        if not np.isfinite(self.time_ns):
+            raise ValueError("time_ns must be finite")
+        direction = _unit_vector(self.direction, "direction").copy()
+        object.__setattr__(self, "position_m", position)
+        object.__setattr__(self, "direction", direction)
+
+
+@dataclass(frozen=True)
+class SpectralLightElements:
+    """Straight emitting chords with ``dN/dlambda = lambda^-2 (C0 + C2 / n^2)``."""
+    start_m: np.ndarray
+    direction: np.ndarray
+    length_m: np.ndarray
+    coefficient0: np.ndarray
+    coefficient2: np.ndarray
+    beta: np.ndarray
+    start_ns: np.ndarray
+    end_ns: np.ndarray
+    row_index: np.ndarray
+    uid: np.ndarray
+    reference_phase_index: float = 1.35
+    provenance: dict | None = None
+    cone_model: str = "frozen"
+
+    def __post_init__(self):
+        start = np.asarray(self.start_m, float)
+        direction = np.asarray(self.direction, float)
+        count = len(start)
+        if start.shape != (count, 3) or direction.shape != start.shape or not count:
+            raise ValueError("start_m and direction must be nonempty (N,3) arrays")
+        arrays = {}
+        for name in ("length_m", "coefficient0", "coefficient2", "beta",
+                     "start_ns", "end_ns", "row_index", "uid"):
+            value = np.asarray(getattr(self, name))
+            if value.shape != (count,) or not np.isfinite(value.astype(float)).all():
+                raise ValueError(f"{name} must contain one finite value per element")
+            arrays[name] = value
+        norm = np.linalg.norm(direction, axis=1)
+        if (not np.isfinite(start).all() or np.any(norm <= 0)
+                or np.any(arrays["length_m"] <= 0) or np.any(arrays["coefficient0"] < 0)
+                or np.any(arrays["coefficient2"] > 0) or np.any(arrays["beta"] <= 0)):
+            raise ValueError("invalid spectral element geometry or coefficients")
+        if self.cone_model not in ("frozen", "spectral"):
+            raise ValueError("cone_model must be frozen or spectral")
+        object.__setattr__(self, "start_m", start)
+        object.__setattr__(self, "direction", direction / norm[:, None])
+        for name, value in arrays.items():
+            object.__setattr__(self, name, value)
+
+    def __len__(self):
+        return len(self.length_m)
+
+    @property
+    def midpoints_m(self):
+        return self.start_m + 0.5 * self.length_m[:, None] * self.direction
+
+    @property
+    def centroid_m(self):
+        weight = self.coefficient0 / self.coefficient0.sum()
+        return np.sum(weight[:, None] * self.midpoints_m, axis=0)
+
+    def subset(self, mask):
+        mask = np.asarray(mask, bool)
+        if mask.shape != (len(self),) or not np.any(mask):
+            raise ValueError("subset mask must retain at least one element")
+        return replace(self, start_m=self.start_m[mask], direction=self.direction[mask],
+                       length_m=self.length_m[mask], coefficient0=self.coefficient0[mask],
+                       coefficient2=self.coefficient2[mask], beta=self.beta[mask],
+                       start_ns=self.start_ns[mask], end_ns=self.end_ns[mask],
+                       row_index=self.row_index[mask], uid=self.uid[mask],
+                       provenance={**(self.provenance or {}), "subset_elements": int(mask.sum())})
+
+    def placed(self, pose: SourcePose):
+        """Move the photon-weighted centroid and principal axis onto ``pose``."""
+        if not isinstance(pose, SourcePose):
+            raise TypeError("pose must be a SourcePose")
+        source_centroid, source_axis = _principal_axis(self)
+        rotation = _rotation_between(source_axis, pose.direction)
+        translation = pose.position_m - rotation @ source_centroid
+        delay = float(pose.time_ns - np.min(self.start_ns))
+        return replace(self, start_m=self.start_m @ rotation.T + translation,
+                       direction=self.direction @ rotation.T,
+                       start_ns=self.start_ns + delay, end_ns=self.end_ns + delay,
+                       provenance={**(self.provenance or {}), "pose": "placed"})
+
+
+@dataclass(frozen=True)
+class SyntheticShower:
+    """Seeded synthetic axial cascade (``sources.py:396-441``)."""
+    elements: SpectralLightElements
+    seed: int = 0
+
+    @classmethod
+    def gaussian(cls, pose: SourcePose, *, charged_track_length_m=100.0,
+                 longitudinal_sigma_m=1.2, transverse_sigma_m=0.08,
+                 elements=64, beta=0.999, reference_phase_index=1.35, seed=0):
+        if elements < 3:
+            raise ValueError("synthetic shower needs at least three elements")
+        if not np.isfinite(beta) or not 0 < beta <= 1:
+            raise ValueError("beta must be finite and in (0, 1]")
+        if not np.isfinite(reference_phase_index) or reference_phase_index <= 0:
+            raise ValueError("reference_phase_index must be finite and positive")
+        for name, value in (("charged_track_length_m", charged_track_length_m),
+                            ("longitudinal_sigma_m", longitudinal_sigma_m),
+                            ("transverse_sigma_m", transverse_sigma_m)):
+            if not np.isfinite(value) or value <= 0:
+                raise ValueError(f"{name} must be finite and positive")
+        rng = np.random.default_rng(seed)
+        z = np.linspace(-3 * longitudinal_sigma_m, 3 * longitudinal_sigma_m, int(elements))
+        share = np.exp(-0.5 * (z / longitudinal_sigma_m) ** 2)
+        share /= share.sum()
+        transverse = rng.normal(scale=transverse_sigma_m, size=(len(z), 2))
+        angular = rng.normal(scale=0.025, size=(len(z), 2))
+        start = np.column_stack((transverse, z))
+        direction = np.column_stack((angular, np.ones(len(z))))
+        direction /= np.linalg.norm(direction, axis=1)[:, None]
+        segment_length = max(0.01, 6 * longitudinal_sigma_m / len(z))
+        length = np.full(len(z), segment_length)
+        common = 2 * np.pi * FINE_STRUCTURE * 1e9 * charged_track_length_m * share
+        time = (z - z.min()) / (beta * C_M_PER_NS)
+        raw = SpectralLightElements(
+            start, direction, length, common, -common / beta ** 2,
+            np.full(len(z), beta), time, time + length / (beta * C_M_PER_NS),
+            np.arange(len(z)), np.zeros(len(z), np.int64), reference_phase_index,
+            {"source": "SyntheticShower", "seed": int(seed),
+             "charged_track_length_m": float(charged_track_length_m)})
+        return cls(raw.placed(pose), int(seed))

@@ -1,0 +1,141 @@
+"""Synthetic media, arrays and the benchmark workloads.
+
+``synthetic_medium_model`` and ``baikal_like_detector`` mirror the reference's
+public demonstration inputs (``examples/synthetic.py:12-54``). The
+detector-scale array and the workload builders freeze the choices the
+workload descriptions leave open (cluster layout, vertex rule):
+
+* ``detector_scale_array(clusters)``: clusters on a line along x every 300 m;
+  each cluster has 7 strings on a 60 m-radius heptagon plus a centre string,
+  36 OMs per string every 15 m, looking down like the demonstration array.
+* ``plain_small()``: 48 OMs, 2000-element shower of seed 7 at the array
+  centroid + (-12, 8, 0) m along (0.35, -0.15, 0.925), 300 m charged track;
+  5 ns bins over -60..745 ns; 3 wavelength nodes.
+* ``plain_detector_scale()``: 2304 OMs, 50,000 elements, 1500 m charged track,
+  vertex uniform in the OM bounding box and isotropic direction drawn from
+  ``default_rng(11)`` (vertex first, then direction); 1 ns bins; 8 nodes.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .model import DetectorArray, SpectralMedium, SphereSurface, WavelengthQuadrature
+from .sources import SourcePose, SyntheticShower
+
+
+def synthetic_medium_model():
+    return SpectralMedium(
+        [400., 450., 500.], [.030, .020, .040], [.030, .022, .017],
+        [1.344, 1.339, 1.336], [1.386, 1.374, 1.367], g=.9,
+        provenance="synthetic Baikal-like demo; not a calibration")
+
+
+def angular_acceptance(cosine):
+    x = np.asarray(cosine, float)
+    return .45 + .45 * x + .10 * x * x
+
+
+def spectral_efficiency(wavelength_nm):
+    wavelength = np.asarray(wavelength_nm, float)
+    return .20 * np.exp(-0.5 * ((wavelength - 450.) / 90.) ** 2)
+
+
+def _array(positions, identifiers, provenance):
+    count = len(positions)
+    return DetectorArray(positions, np.tile([0., 0., -1.], (count, 1)), np.pi * .20 ** 2,
+                         angular_acceptance, spectral_efficiency,
+                         receiver_surface=SphereSurface(.20), identifiers=identifiers,
+                         provenance=provenance)
+
+
+def baikal_like_detector(*, clusters=1, strings=8, modules=6, cluster_pitch_m=180.,
+                         string_radius_m=25., vertical_spacing_m=15.):
+    """The demonstration array (``examples/synthetic.py:19-54``)."""
+    cluster_centres = np.zeros((clusters, 3))
+    cluster_centres[:, 0] = np.arange(clusters) * cluster_pitch_m
+    azimuth = np.arange(strings) * 2 * np.pi / strings
+    string_offsets = np.column_stack((string_radius_m * np.cos(azimuth),
+                                      string_radius_m * np.sin(azimuth), np.zeros(strings)))
+    heights = (np.arange(modules) - (modules - 1) / 2) * vertical_spacing_m
+    positions = (cluster_centres[:, None, None, :] + string_offsets[None, :, None, :]
+                 + np.column_stack((np.zeros(modules), np.zeros(modules), heights))[None, None, :, :]
+                 ).reshape(-1, 3)
+    identifiers = {
+        "cluster_id": np.repeat(np.arange(clusters), strings * modules),
+        "string_id": np.tile(np.repeat(np.arange(strings), modules), clusters),
+        "module_id": np.tile(np.arange(modules), clusters * strings),
+    }
+    return _array(positions, identifiers, "synthetic Baikal-like geometry and OM; not a calibration")
+
+
+def detector_scale_array(clusters=8, *, modules=36, cluster_pitch_m=300., radius_m=60.,
+                         vertical_spacing_m=15.):
+    """Clusters of a heptagon of strings plus a centre string (8 strings x ``modules``)."""
+    azimuth = np.arange(7) * 2 * np.pi / 7
+    offsets = np.vstack([np.column_stack((radius_m * np.cos(azimuth), radius_m * np.sin(azimuth),
+                                          np.zeros(7))), np.zeros((1, 3))])
+    heights = (np.arange(modules) - (modules - 1) / 2) * vertical_spacing_m
+    centres = np.zeros((clusters, 3))
+    centres[:, 0] = (np.arange(clusters) - (clusters - 1) / 2) * cluster_pitch_m
+    positions = (centres[:, None, None, :] + offsets[None, :, None, :]
+                 + np.column_stack((np.zeros(modules), np.zeros(modules), heights))[None, None, :, :]
+                 ).reshape(-1, 3)
+    identifiers = {
+        "cluster_id": np.repeat(np.arange(clusters), 8 * modules),
+        "string_id": np.tile(np.repeat(np.arange(8), modules), clusters),
+        "module_id": np.tile(np.arange(modules), clusters * 8),
+    }
+    return _array(positions, identifiers, "synthetic detector-scale array; not a calibration")
+
+
+@dataclass(frozen=True)
+class Workload:
+    """Inputs of one benchmark event."""
+    name: str
+    medium: SpectralMedium
+    detector: DetectorArray
+    wavelength: WavelengthQuadrature
+    edges_ns: np.ndarray
+    omega_per_ns: np.ndarray
+    readout_omega_per_ns: np.ndarray
+    elements: object
+    source_degree: int = 8
+    azimuthal_degree: int = 2
+    radial_range_m: tuple = (2.0, 150.0)
+    radial_nodes: int = 48
+    threshold_pe: float = 0.01
+
+
+def _shower(pose, track_m, elements, seed):
+    from dataclasses import replace
+    shower = SyntheticShower.gaussian(pose, charged_track_length_m=track_m, elements=elements,
+                                      seed=seed)
+    return replace(shower.elements, cone_model="spectral")
+
+
+def plain_small():
+    medium = synthetic_medium_model()
+    detector = baikal_like_detector(clusters=1)
+    centre = detector.positions_m.mean(axis=0)
+    pose = SourcePose(centre + np.array([-12.0, 8.0, 0.0]), [0.35, -0.15, 0.925])
+    return Workload("plain-small: 48 OMs, 2000-element shower, 5 ns bins", medium, detector,
+                    WavelengthQuadrature.gauss_legendre(400., 500., 3),
+                    np.arange(-60.0, 745.0, 5.0), np.linspace(0., 1.2, 65),
+                    np.linspace(0., 1.2, 321), _shower(pose, 300.0, 2000, 7))
+
+
+def plain_detector_scale(elements=50_000):
+    medium = synthetic_medium_model()
+    detector = detector_scale_array(8)
+    rng = np.random.default_rng(11)
+    low, high = detector.positions_m.min(axis=0), detector.positions_m.max(axis=0)
+    vertex = low + rng.uniform(size=3) * (high - low)
+    direction = rng.normal(size=3)
+    direction /= np.linalg.norm(direction)
+    pose = SourcePose(vertex, direction)
+    return Workload(f"plain-detector-scale: 2304 OMs, {elements}-element shower, 1 ns bins",
+                    medium, detector, WavelengthQuadrature.gauss_legendre(400., 500., 8),
+                    np.arange(-60.0, 745.0, 1.0), np.linspace(0., 1.2, 65),
+                    np.linspace(0., 1.2, 321), _shower(pose, 1500.0, elements, 11))
