@@ -17,6 +17,8 @@
 #ifndef LIGHTHIT_AMD_H
 #define LIGHTHIT_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -109,6 +111,64 @@ typedef struct lh_ge2_receivers {
 int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source,
                  const lh_ge2_receivers* receivers, const double* alpha, int freq,
                  double radial_low, double radial_high, void* out, void* stream);
+
+/*
+ * Stage (a): order-0 (direct) light to the first glass entry of every OM.
+ *
+ * Replaces PreparedBallistic(kernel, source, config).charge()
+ * (_surface_ballistic.py:20-98, reached from transport_prompt's order-0
+ * block, prompt.py:637-655): the candidate (OM, element) pairs of
+ * interval_candidates (_ballistic_interval_numba.py:450-519) and the
+ * first-entry integral of each, summed per OM. Spherical receivers; regular
+ * pairs only (element line farther than the glass radius from the OM centre,
+ * no constant-phase panel); other pairs set flag bit 1 and are skipped.
+ */
+typedef struct lh_order0_optics {
+    const double* panel_edges;       /* (panels+1) nm, from prepare_optics */
+    const double* phase_nodes;       /* (panels+1) */
+    const double* group_nodes;       /* (panels+1) */
+    const double* extinction_nodes;  /* (panels+1) 1/m */
+    const double* efficiency;        /* (panels, efficiency_columns) cubic in q */
+    const int* major;                /* (major_count) panel-edge indices of the medium
+                                        nodes inside the band, both band ends included */
+    const double* alpha;             /* (acceptance_count) acceptance Legendre coefficients */
+    int panels, efficiency_columns, major_count, acceptance_count;
+    double phase_min, phase_max;     /* extrema of phase_nodes */
+} lh_order0_optics;
+
+typedef struct lh_order0_oms {
+    const double* positions;   /* (count, 3) m */
+    const double* looks;       /* (count, 3) -orientation */
+    const double* areas;       /* (count) effective area m^2 */
+    const double* radii;       /* (count) glass radius m */
+    int count;
+} lh_order0_oms;
+
+typedef struct lh_order0_elements {
+    const double* start_m;        /* (count, 3) */
+    const double* direction;      /* (count, 3) unit */
+    const double* length_m;       /* (count) */
+    const double* coefficient0;   /* (count) C0 */
+    const double* coefficient2;   /* (count) C2 = -C0 / beta^2 */
+    const double* beta;           /* (count) */
+    const double* start_ns;       /* (count) */
+    const double* end_ns;         /* (count) */
+    int count;
+} lh_order0_elements;
+
+/* Bytes of device workspace lh_order0_charge needs (flags + per-chunk partial sums). */
+size_t lh_order0_workspace_bytes(int n_oms, int n_elements);
+
+/*
+ * charge (oms->count) double, written. workspace: device memory of at least
+ * lh_order0_workspace_bytes; its first int holds flags after the call:
+ * bit 0 an emitting segment intersects the glass (the reference raises
+ * ValueError, _surface_ballistic.py:52-53), bit 1 a pair outside the
+ * supported geometry was skipped.
+ */
+int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                     const lh_order0_elements* elements, double* charge, void* workspace,
+                     size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

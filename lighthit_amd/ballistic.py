@@ -1,0 +1,182 @@
+"""Stage (a): order-0 direct light to the first glass entry, on the GPU.
+
+Host mirror of the first-entry direct light (``first_entry_ballistic``,
+``ballistic_interval.py:268-287``; ``PreparedBallistic.charge``,
+``_surface_ballistic.py:91-98``). The optical panels are prepared once per
+(medium, detector) (:func:`lighthit_amd.optics.prepare_optics`) and kept on
+the device with the acceptance coefficients (:class:`DeviceOptics`); one event
+is one launch of ``lh_order0_charge`` over all (OM, element) pairs.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .optics import prepare_optics
+
+
+class _Optics(ctypes.Structure):
+    _fields_ = [("panel_edges", ctypes.c_void_p), ("phase_nodes", ctypes.c_void_p),
+                ("group_nodes", ctypes.c_void_p), ("extinction_nodes", ctypes.c_void_p),
+                ("efficiency", ctypes.c_void_p), ("major", ctypes.c_void_p),
+                ("alpha", ctypes.c_void_p), ("panels", ctypes.c_int),
+                ("efficiency_columns", ctypes.c_int), ("major_count", ctypes.c_int),
+                ("acceptance_count", ctypes.c_int), ("phase_min", ctypes.c_double),
+                ("phase_max", ctypes.c_double)]
+
+
+class _Oms(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("looks", ctypes.c_void_p),
+                ("areas", ctypes.c_void_p), ("radii", ctypes.c_void_p), ("count", ctypes.c_int)]
+
+
+class _Elements(ctypes.Structure):
+    _fields_ = [("start_m", ctypes.c_void_p), ("direction", ctypes.c_void_p),
+                ("length_m", ctypes.c_void_p), ("coefficient0", ctypes.c_void_p),
+                ("coefficient2", ctypes.c_void_p), ("beta", ctypes.c_void_p),
+                ("start_ns", ctypes.c_void_p), ("end_ns", ctypes.c_void_p),
+                ("count", ctypes.c_int)]
+
+
+_lib.SIGNATURES["lh_order0_workspace_bytes"] = (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int])
+_lib.SIGNATURES["lh_order0_charge"] = (ctypes.c_int, [ctypes.POINTER(_Optics), ctypes.POINTER(_Oms),
+                                                      ctypes.POINTER(_Elements), ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_size_t,
+                                                      ctypes.c_void_p])
+
+FLAG_GLASS = 1
+FLAG_UNSUPPORTED = 2
+
+
+def _dev(array, dtype, device):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(array), dtype=dtype, device=device)
+
+
+class DeviceOptics:
+    """Optical panels, medium-node lines and acceptance of a detector, on the device."""
+
+    def __init__(self, optics, alpha, device="cuda"):
+        import torch
+        edges = np.asarray(optics["panel_edges"], float)
+        phase = np.asarray(optics["phase_nodes"], float)
+        if np.any(np.diff(phase) == 0.0):
+            raise ValueError("constant-phase optical panels need the direct-ray evaluator, "
+                             "which the GPU path does not provide")
+        eff = np.asarray(optics["efficiency_coefficients"], float)
+        if eff.shape[1] != 4:
+            raise ValueError("the GPU path takes cubic efficiency panels")
+        # Lines where the medium table (phase, group, extinction) has a kink:
+        # the band ends and the medium nodes inside it.
+        medium_nodes = optics.get("medium_nodes_nm")
+        if medium_nodes is None:
+            raise ValueError("optics must carry the medium nodes (prepare with lighthit_amd.optics)")
+        inside = [x for x in medium_nodes if edges[0] < x < edges[-1]]
+        major = np.unique(np.asarray([0, len(edges) - 1] + [int(np.searchsorted(edges, x))
+                                                              for x in inside], np.int64))
+        if not np.allclose(edges[major[1:-1]], inside):
+            raise ValueError("medium nodes are not panel edges")
+        self.optics = optics
+        self.alpha = np.asarray(alpha, float)
+        self.major = major.astype(np.int32)
+        self._keep = [_dev(edges, torch.float64, device), _dev(phase, torch.float64, device),
+                      _dev(optics["group_nodes"], torch.float64, device),
+                      _dev(optics["extinction_nodes"], torch.float64, device),
+                      _dev(eff, torch.float64, device), _dev(self.major, torch.int32, device),
+                      _dev(self.alpha, torch.float64, device)]
+        ptrs = [t.data_ptr() for t in self._keep]
+        self.struct = _Optics(*ptrs, len(edges) - 1, 4, len(self.major), len(self.alpha),
+                              float(phase.min()), float(phase.max()))
+
+
+class DeviceOMs:
+    """Positions, looks, areas and glass radii of a detector (or subset), on the device."""
+
+    def __init__(self, detector, device="cuda"):
+        import torch
+        if detector.receiver_surface is None:
+            raise ValueError("the first-entry GPU path needs detector.receiver_surface geometry")
+        self.count = len(detector)
+        self._keep = [_dev(detector.positions_m, torch.float64, device),
+                      _dev(detector.looks, torch.float64, device),
+                      _dev(detector.effective_area_m2, torch.float64, device),
+                      _dev(detector.receiver_surface.radii(self.count), torch.float64, device)]
+        self.struct = _Oms(*[t.data_ptr() for t in self._keep], self.count)
+
+
+class DeviceElements:
+    """Spectral light elements of one source, on the device."""
+
+    def __init__(self, elements, device="cuda"):
+        import torch
+        if elements.cone_model != "spectral":
+            raise ValueError('continuous ballistic spectrum requires spectral LightElements')
+        if not np.allclose(elements.coefficient2, -elements.coefficient0 / elements.beta ** 2,
+                           rtol=1e-10, atol=1e-20):
+            raise ValueError('source must have the physical Cherenkov C0/C2/beta relation')
+        self.count = len(elements)
+        names = ("start_m", "direction", "length_m", "coefficient0", "coefficient2", "beta",
+                 "start_ns", "end_ns")
+        self._keep = [_dev(np.asarray(getattr(elements, n), float), torch.float64, device)
+                      for n in names]
+        self.struct = _Elements(*[t.data_ptr() for t in self._keep], self.count)
+
+
+class Order0:
+    """Resident order-0 evaluator of one (medium, detector): workspace reused across events."""
+
+    def __init__(self, medium, detector, alpha, *, wavelength_range_nm=None, device="cuda"):
+        optics = prepare_optics(medium, detector, wavelength_range_nm)
+        self.optics = DeviceOptics(optics, alpha, device)
+        self.oms = DeviceOMs(detector, device)
+        self.device = device
+        self._workspace = None
+
+    def workspace(self, n_elements):
+        import torch
+        need = _workspace_bytes(self.oms.count, n_elements)
+        if self._workspace is None or self._workspace.numel() < need:
+            self._workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        return self._workspace
+
+    def charge_device(self, elements, out=None, stream=None):
+        """Enqueue the charge of every OM; returns (charge tensor, workspace)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.oms.count, dtype=torch.float64, device=self.device)
+        ws = self.workspace(elements.count)
+        _lib.call("lh_order0_charge", ctypes.byref(self.optics.struct), ctypes.byref(self.oms.struct),
+                  ctypes.byref(elements.struct), ctypes.c_void_p(out.data_ptr()),
+                  ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()),
+                  _lib.stream_handle(stream))
+        return out, ws
+
+    def charge(self, elements):
+        """Charge per OM (numpy), raising the reference's errors."""
+        import torch
+        device_elements = elements if isinstance(elements, DeviceElements) else DeviceElements(
+            elements, self.device)
+        out, ws = self.charge_device(device_elements)
+        torch.cuda.synchronize()
+        check_flags(ws)
+        return out.cpu().numpy()
+
+
+def _workspace_bytes(n_oms, n_elements):
+    lib = _lib.load()
+    fn = lib.lh_order0_workspace_bytes
+    fn.restype, fn.argtypes = _lib.SIGNATURES["lh_order0_workspace_bytes"]
+    return int(fn(int(n_oms), int(n_elements)))
+
+
+def check_flags(workspace):
+    """Raise the reference's error for flags an order-0 launch left in its workspace."""
+    import torch
+    flags = int(workspace[:4].view(dtype=torch.int32)[0].item())
+    if flags & FLAG_GLASS:
+        raise ValueError('emitting segment intersects the glass boundary')
+    if flags & FLAG_UNSUPPORTED:
+        raise ValueError("an element line passes within the glass radius of an OM centre; "
+                         "the direct-ray evaluator for such pairs is not on the GPU path")

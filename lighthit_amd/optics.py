@@ -54,4 +54,5 @@ def prepare_optics(medium, detector, wavelength_range_nm=None, *, spectral_knots
                 group_nodes=np.ascontiguousarray(sample['group_index']),
                 extinction_nodes=np.ascontiguousarray(sample['absorption_per_m']
                                                       + sample['scattering_per_m']),
-                efficiency_coefficients=np.ascontiguousarray(coefficients))
+                efficiency_coefficients=np.ascontiguousarray(coefficients),
+                medium_nodes_nm=np.asarray(medium.wavelength_nm, float).copy())

@@ -1,0 +1,79 @@
+"""GPU parity of stage (a), lh_order0_charge, against the FP64 oracle."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from lighthit_amd.model import DetectorArray, SpectralMedium, SphereSurface, acceptance_alpha
+from lighthit_amd.optics import prepare_optics
+from oracle import order0 as oracle
+
+pytestmark = pytest.mark.gpu
+
+# FP32 node arithmetic, FP64 set-up and sums: relative charge error per OM.
+TOLERANCE = 1e-5
+
+
+def _compare(detector, medium, elements, alpha, floor=1e-6):
+    from lighthit_amd.ballistic import Order0
+    optics = prepare_optics(medium, detector)
+    radii = detector.receiver_surface.radii(len(detector))
+    expected, _ = oracle.first_entry(detector.positions_m, detector.looks,
+                                     detector.effective_area_m2, radii, elements, optics, alpha,
+                                     rel_tol=1e-10)
+    got = Order0(medium, detector, alpha).charge(elements)
+    lit = expected > floor
+    assert lit.any()
+    err = np.abs(got[lit] / expected[lit] - 1)
+    assert err.max() < TOLERANCE, (err.max(), np.argmax(err))
+    # OMs the oracle leaves dark stay (nearly) dark
+    assert np.all(np.abs(got[~lit]) <= 2 * floor)
+    return got, expected
+
+
+def test_demo_event_all_oms(device):
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    got, expected = _compare(w.detector, w.medium, w.elements, acceptance_alpha(w.detector))
+    print("order-0 total", got.sum(), expected.sum())
+
+
+def test_detector_scale_event_subset(device):
+    from lighthit_amd.synthetic import plain_detector_scale
+    w = plain_detector_scale(elements=2000)
+    optics = prepare_optics(w.medium, w.detector)
+    om, _ = oracle.candidates(w.detector.positions_m, w.detector.receiver_surface.radii(2304),
+                              w.elements, optics)
+    # every lit OM and a stride of dark ones
+    ids = np.union1d(np.unique(om), np.arange(0, 2304, 97))
+    _compare(w.detector.subset(ids), w.medium, w.elements, acceptance_alpha(w.detector))
+
+
+@pytest.mark.parametrize('offset_m', [0., -.12])
+def test_dispersive_tilted_receiver(device, offset_m):
+    from lighthit_amd.sources import SpectralLightElements
+    beta, length, radius = .9, .05, .126
+    medium = SpectralMedium([350., 610.], [.07, .04], [0., 0.], [1.38, 1.32], [1.43, 1.36], g=.9)
+    look = np.array([.6, .3, np.sqrt(.55)])
+    centre = np.array([2., 0., 2 / np.sqrt((beta * 1.35) ** 2 - 1) + length / 2 + offset_m])
+    detector = DetectorArray([centre], [-look], .05, lambda x: 1 + .3 * x + .1 * x * x + .02 * x ** 3,
+                             lambda w: np.full_like(w, .3), receiver_surface=SphereSurface(radius))
+    elements = SpectralLightElements(
+        np.zeros((1, 3)), np.array([[0., 0., 1.]]), np.array([length]), np.array([1e8 * length]),
+        np.array([-1e8 * length / beta ** 2]), np.array([beta]), np.zeros(1),
+        np.array([length / (beta * .299792458)]), np.array([0]), np.array([0]),
+        cone_model='spectral')
+    _compare(detector, medium, elements, acceptance_alpha(detector, acceptance_degree=3))
+
+
+def test_glass_intersection_raises(device):
+    from lighthit_amd.ballistic import Order0
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    e = w.elements
+    # move one element's start onto the surface of OM 0
+    start = e.start_m.copy()
+    start[5] = w.detector.positions_m[0] + np.array([0.1, 0.0, 0.0]) - 0.005 * e.direction[5]
+    moved = replace(e, start_m=start)
+    with pytest.raises(ValueError, match='intersects the glass'):
+        Order0(w.medium, w.detector, acceptance_alpha(w.detector)).charge(moved)
