@@ -1,0 +1,239 @@
+"""Benchmark of the ShowerTransport GPU hot path on one MI355X.
+
+One step = one shower event through the GPU stages of the plain (FP32) path,
+inputs resident in HBM:
+  (a) order-0 first-entry charge of every OM from every element;
+  (b) order >=2 apply of the band-folded pack: zero-frequency charge at every
+      OM inside the radial window of the source points, then the full
+      65-frequency spectrum at those OMs;
+  (c) readout of those spectra into the workload's time bins.
+Workload (default): the detector-scale event — 2304 OMs, 50,000-element
+Gaussian shower of seed 11, 1 ns bins (lighthit_amd.synthetic). The order >=2
+pack and node-band channels have the route's shapes (3 node bands, 47 radial
+intervals, degree 8, 6 detector pairs, 65 frequencies; 165 points on the
+shower axis) with seeded random values: their cost does not depend on values.
+
+Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
+baseline of the oracle on a bounded sample). ``--gpus`` above 1 exits: this
+build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def axial_points(frame, rng, *, half_length=3.6, cell=0.25, stencil=0.2):
+    """Points of an axial-slice source: slices every ``cell`` m, 5-point stencils."""
+    z = np.arange(-half_length, half_length + 1e-9, cell)
+    offsets = np.array([[0, 0], [stencil, 0], [-stencil, 0], [0, stencil], [0, -stencil]])
+    zz = np.repeat(z, len(offsets))
+    tt = np.tile(offsets, (len(z), 1)).astype(float)
+    extra = 165 - len(zz)
+    if extra > 0:
+        zz = np.r_[zz, rng.uniform(-half_length, half_length, extra)]
+        tt = np.vstack([tt, rng.normal(scale=0.05, size=(extra, 2))])
+    return zz[:165], tt[:165]
+
+
+def build_ge2_inputs(workload, rng):
+    from lighthit_amd.axial import AxialSource, AxisFrame
+    from lighthit_amd.ge2 import canonical_pairs
+
+    class _Photons:
+        def __init__(self, e):
+            self.midpoints_m = e.midpoints_m
+            self.photons = e.coefficient0
+            self.direction = e.direction
+    frame = AxisFrame.of(_Photons(workload.elements))
+    degree, az, bands, nint, nfreq = workload.source_degree, workload.azimuthal_degree, 3, 47, 65
+    pl, pm = canonical_pairs(2)
+    log_r = np.linspace(np.log(2.0), np.log(150.0), nint + 1)
+    coef = (rng.normal(size=(bands, nint, degree + 1, len(pl), 4, nfreq))
+            + 1j * rng.normal(size=(bands, nint, degree + 1, len(pl), 4, nfreq))).astype(np.complex64)
+    bundle = dict(degree=degree, acceptance_degree=2, pair_lambda=pl, pair_mu=pm,
+                  omega=workload.omega_per_ns, phase="flight", coefficients=coef, log_r=log_r,
+                  absorption=np.array([0.02, 0.02, 0.02]), speed=np.array([0.218, 0.218, 0.218]))
+    z, tr = axial_points(frame, rng)
+    nchan = sum(2 * min(l, az) + 1 for l in range(degree + 1))
+    sources = [AxialSource(frame, z, tr, (rng.normal(size=(len(z), nchan, nfreq))
+                                          + 1j * rng.normal(size=(len(z), nchan, nfreq))),
+                           degree, az, 0.0) for _ in range(bands)]
+    return bundle, sources
+
+
+def reachable(sources, positions, window):
+    points = sources[0].points_m()
+    dist = np.linalg.norm(positions[:, None, :] - points[None, :, :], axis=2)
+    return np.flatnonzero(np.any((dist >= window[0]) & (dist <= window[1]), axis=1))
+
+
+def cpu_baseline(workload, bundle, sources, window, ids, alpha, fraction):
+    """The oracle on the host: order 0 on a fraction of the elements, (b) and (c) in full."""
+    from lighthit_amd.optics import prepare_optics
+    from oracle import ge2 as oracle_ge2
+    from oracle import order0 as oracle_order0
+    from oracle import readout as oracle_readout
+    d = workload.detector
+    optics = prepare_optics(workload.medium, d)
+    keep = np.arange(len(workload.elements)) % int(round(1 / fraction)) == 0
+    sample = workload.elements.subset(keep)
+    radii = d.receiver_surface.radii(len(d))
+    began = time.perf_counter()
+    # the reference's own tolerance: 2e-6 relative per OM
+    oracle_order0.first_entry(d.positions_m, d.looks, d.effective_area_m2, radii, sample, optics,
+                              alpha, rel_tol=2e-6)
+    t_order0 = (time.perf_counter() - began) / keep.mean()
+    began = time.perf_counter()
+    pos, looks, area = d.positions_m[ids], d.looks[ids], d.effective_area_m2[ids]
+    zero = dict(bundle, coefficients=np.ascontiguousarray(bundle["coefficients"][..., :1]),
+                omega=bundle["omega"][:1])
+    from dataclasses import replace
+    zero_sources = [replace(s, channels=np.ascontiguousarray(s.channels[:, :, :1])) for s in sources]
+    oracle_ge2.apply_folded(zero, zero_sources, pos, looks, alpha, area, window)
+    spectrum = oracle_ge2.apply_folded(bundle, sources, pos, looks, alpha, area, window)
+    t_ge2 = time.perf_counter() - began
+    began = time.perf_counter()
+    oracle_readout.route_readout(workload.omega_per_ns, spectrum, np.full(len(ids), 500.0),
+                                 workload.readout_omega_per_ns, workload.edges_ns)
+    t_readout = time.perf_counter() - began
+    total = t_order0 + t_ge2 + t_readout
+    return {"value": 1.0 / total, "unit": "events/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "sample": (f"FP64 oracle, one event: order 0 on every {int(round(1 / fraction))}th "
+                       f"element ({int(keep.sum())} of {len(keep)}) scaled to all, at the "
+                       f"reference tolerance 2e-6; order >=2 zero-frequency and full apply and "
+                       f"readout at all {len(ids)} windowed OMs"),
+            "stage_seconds": {"order0": t_order0, "ge2": t_ge2, "readout": t_readout}}
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=20)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--workload", choices=["detector", "small"], default="detector")
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--cpu-fraction", type=float, default=0.1)
+    parser.add_argument("--copy-kernel-mib", type=int, default=0,
+                        help="only run a plain device copy of this size (HBM counter calibration)")
+    args = parser.parse_args()
+    if args.copy_kernel_mib:
+        import torch
+        a = torch.ones(args.copy_kernel_mib * 2 ** 18, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_kernel_mib, "launches": 3}), flush=True)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(multi-GPU sharding by OM is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from lighthit_amd import ge2 as dge2
+    from lighthit_amd.ballistic import DeviceElements, Order0
+    from lighthit_amd.model import acceptance_alpha
+    from lighthit_amd.readout import ReadoutBasis
+    from lighthit_amd.synthetic import plain_detector_scale, plain_small
+
+    workload = plain_detector_scale() if args.workload == "detector" else plain_small()
+    rng = np.random.default_rng(1234)
+    d = workload.detector
+    alpha = acceptance_alpha(d)
+    window = workload.radial_range_m
+    bundle, sources = build_ge2_inputs(workload, rng)
+    ids = reachable(sources, d.positions_m, window)
+    dev = "cuda"
+
+    order0 = Order0(workload.medium, d, alpha)
+    elements = DeviceElements(workload.elements)
+    pack = dge2.DevicePack(bundle)
+    source = dge2.DeviceSource(sources)
+    receivers = dge2.DeviceReceivers(source.frame, d.positions_m[ids], d.looks[ids],
+                                     np.repeat(d.effective_area_m2[ids][None], 3, axis=0))
+    basis = ReadoutBasis(workload.omega_per_ns, workload.edges_ns, workload.readout_omega_per_ns)
+    origins = torch.full((len(ids),), 500.0, dtype=torch.float64, device=dev)
+    charge = torch.empty(len(d), dtype=torch.float64, device=dev)
+    zero = torch.empty((1, len(ids)), dtype=torch.complex64, device=dev)
+    spectrum = torch.empty((pack.nfreq, len(ids)), dtype=torch.complex64, device=dev)
+    bins = torch.empty((len(ids), basis.bins), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    events = {k: [] for k in ("order0", "ge2_zero", "ge2_full", "readout")}
+
+    def step(timed):
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(5)] if timed else None
+        if timed: marks[0].record(stream)
+        order0.charge_device(elements, out=charge, stream=stream)
+        if timed: marks[1].record(stream)
+        if len(ids):
+            dge2.apply_device(pack, source, receivers, alpha, window, frequencies=1, out=zero,
+                              stream=stream)
+        if timed: marks[2].record(stream)
+        if len(ids):
+            dge2.apply_device(pack, source, receivers, alpha, window, out=spectrum, stream=stream)
+        if timed: marks[3].record(stream)
+        if len(ids):
+            basis.apply(spectrum, origins, out=bins, stream=stream)
+        if timed: marks[4].record(stream)
+        return marks
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    all_marks = []
+    began = time.perf_counter()
+    for _ in range(args.steps):
+        all_marks.append(step(True))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - began
+    for marks in all_marks:
+        for key, (a, b) in zip(events, zip(marks[:-1], marks[1:])):
+            events[key].append(a.elapsed_time(b))
+    stage_ms = {k: float(np.mean(v)) for k, v in events.items()}
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    # roofline of the dominant kernel (order 0): algorithmic bytes per launch
+    # = 48 B per element (12 FP32 values) + 36 B per OM (position, look, area,
+    # radius, charge)
+    n_el, n_om = len(workload.elements), len(d)
+    algorithmic_bytes = 48 * n_el + 36 * n_om
+    achieved = algorithmic_bytes / (stage_ms["order0"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "order0_charge (+ order0_reduce)",
+                "note": "order 0 is VALU-bound (transcendental-heavy FP32 quadrature); "
+                        "the HBM fraction is reported as required and is small by design"}
+    baseline = None
+    if not args.no_cpu_baseline:
+        baseline = cpu_baseline(workload, bundle, sources, window, ids, alpha, args.cpu_fraction)
+    line = {
+        "metric": "events/sec + p50/p99 latency at <=1e-3 charge error vs FP64; served vs plain GPU",
+        "value": 1e3 / ms_per_step, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": workload.name, "form": "plain FP32 GPU path, stages (a)(b)(c)",
+                   "oms": n_om, "elements": n_el, "windowed_oms": int(len(ids)),
+                   "bins": int(basis.bins), "stage_ms": stage_ms,
+                   "order0_total_pe": float(charge.sum().item())},
+        "roofline": roofline, "cpu_baseline": baseline,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
