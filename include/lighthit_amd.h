@@ -170,6 +170,23 @@ int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
                      const lh_order0_elements* elements, double* charge, void* workspace,
                      size_t workspace_bytes, void* stream);
 
+/*
+ * Order-0 arrival bins of the OMs in `oms` (normally the active subset).
+ *
+ * Replaces PreparedBallistic.bins(ids, origins, edges) (_surface_ballistic.py:
+ * 100-112, called from prompt.py:863-868): arrival time
+ * t = start_ns + tau s + d_entry n_g / c - origin, bins half-open [e_k, e_k+1),
+ * light outside [e_0, e_n) dropped. Each azimuth chord is split exactly at the
+ * bin edges it crosses.
+ * origins (oms->count) double; bin_edges (n_bins+1) double, increasing;
+ * bins (oms->count, n_bins) double, written; n_bins <= 2048.
+ */
+size_t lh_order0_bins_workspace_bytes(int n_oms, int n_elements, int n_bins);
+int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                   const lh_order0_elements* elements, const double* origins,
+                   const double* bin_edges, int n_bins, double* bins, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,14 @@ _lib.SIGNATURES["lh_order0_charge"] = (ctypes.c_int, [ctypes.POINTER(_Optics), c
                                                       ctypes.c_void_p, ctypes.c_size_t,
                                                       ctypes.c_void_p])
 
+_lib.SIGNATURES["lh_order0_bins_workspace_bytes"] = (ctypes.c_size_t,
+                                                     [ctypes.c_int, ctypes.c_int, ctypes.c_int])
+_lib.SIGNATURES["lh_order0_bins"] = (ctypes.c_int, [ctypes.POINTER(_Optics), ctypes.POINTER(_Oms),
+                                                    ctypes.POINTER(_Elements), ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_size_t,
+                                                    ctypes.c_void_p])
+
 FLAG_GLASS = 1
 FLAG_UNSUPPORTED = 2
 
@@ -131,8 +139,10 @@ class Order0:
         optics = prepare_optics(medium, detector, wavelength_range_nm)
         self.optics = DeviceOptics(optics, alpha, device)
         self.oms = DeviceOMs(detector, device)
+        self.detector = detector
         self.device = device
         self._workspace = None
+        self._bins_workspace = None
 
     def workspace(self, n_elements):
         import torch
@@ -153,6 +163,43 @@ class Order0:
                   _lib.stream_handle(stream))
         return out, ws
 
+    def bins_device(self, elements, oms, origins, edges, out=None, stream=None):
+        """Enqueue the arrival bins of ``oms`` (a :class:`DeviceOMs`, normally the active
+        subset); ``origins`` (OM,) float64 and ``edges`` (bins+1,) float64 device tensors."""
+        import torch
+        nbins = int(edges.numel()) - 1
+        if out is None:
+            out = torch.empty((oms.count, nbins), dtype=torch.float64, device=self.device)
+        need = _workspace_bytes(oms.count, elements.count, nbins)
+        if self._bins_workspace is None or self._bins_workspace.numel() < need:
+            self._bins_workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        ws = self._bins_workspace
+        _lib.call("lh_order0_bins", ctypes.byref(self.optics.struct), ctypes.byref(oms.struct),
+                  ctypes.byref(elements.struct), ctypes.c_void_p(origins.data_ptr()),
+                  ctypes.c_void_p(edges.data_ptr()), nbins, ctypes.c_void_p(out.data_ptr()),
+                  ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()),
+                  _lib.stream_handle(stream))
+        return out, ws
+
+    def bins(self, elements, ids, origins, edges):
+        """Bins (len(ids), bins) of OMs ``ids`` (``PreparedBallistic.bins``, numpy)."""
+        import torch
+        ids = np.asarray(ids, dtype=np.int64)
+        origins, edges = np.asarray(origins, float), np.asarray(edges, float)
+        if (origins.shape != (len(ids),) or not np.isfinite(origins).all()
+                or edges.ndim != 1 or len(edges) < 2 or not np.isfinite(edges).all()
+                or np.any(np.diff(edges) <= 0)):
+            raise ValueError('invalid time origins or edges')
+        device_elements = elements if isinstance(elements, DeviceElements) else DeviceElements(
+            elements, self.device)
+        oms = DeviceOMs(self.detector.subset(ids), self.device)
+        out, ws = self.bins_device(device_elements, oms,
+                                   _dev(origins, torch.float64, self.device),
+                                   _dev(edges, torch.float64, self.device))
+        torch.cuda.synchronize()
+        check_flags(ws)
+        return out.cpu().numpy()
+
     def charge(self, elements):
         """Charge per OM (numpy), raising the reference's errors."""
         import torch
@@ -164,11 +211,13 @@ class Order0:
         return out.cpu().numpy()
 
 
-def _workspace_bytes(n_oms, n_elements):
+def _workspace_bytes(n_oms, n_elements, n_bins=None):
     lib = _lib.load()
-    fn = lib.lh_order0_workspace_bytes
-    fn.restype, fn.argtypes = _lib.SIGNATURES["lh_order0_workspace_bytes"]
-    return int(fn(int(n_oms), int(n_elements)))
+    name = "lh_order0_workspace_bytes" if n_bins is None else "lh_order0_bins_workspace_bytes"
+    fn = getattr(lib, name)
+    fn.restype, fn.argtypes = _lib.SIGNATURES[name]
+    args = (int(n_oms), int(n_elements)) + (() if n_bins is None else (int(n_bins),))
+    return int(fn(*args))
 
 
 def check_flags(workspace):

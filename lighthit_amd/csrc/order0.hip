@@ -44,7 +44,30 @@ constexpr int kMaxMajor = 12;      // medium nodes inside the band (+ ends)
 constexpr int kMaxPanels = 256;    // efficiency panels
 constexpr int kMaxAlpha = 8;
 constexpr double kC = 0.299792458;
+constexpr int kChunk = 512;        // elements per workgroup
+constexpr int kBoundWords = 12;
 constexpr double kPi = 3.14159265358979323846;
+
+// 8-point Gauss-Legendre rule on [-1, 1]
+__constant__ float kGauss8x[8] = {-0.960289856497536f, -0.796666477413627f, -0.525532409916329f,
+                                  -0.183434642495650f, 0.183434642495650f, 0.525532409916329f,
+                                  0.796666477413627f, 0.960289856497536f};
+__constant__ float kGauss8w[8] = {0.101228536290376f, 0.222381034453374f, 0.313706645877887f,
+                                  0.362683783378362f, 0.362683783378362f, 0.313706645877887f,
+                                  0.222381034453374f, 0.101228536290376f};
+
+// 6-point Gauss-Legendre rule on [-1, 1]
+__constant__ double kGauss6x[6] = {-0.932469514203152, -0.661209386466265, -0.238619186083197,
+                                   0.238619186083197, 0.661209386466265, 0.932469514203152};
+__constant__ double kGauss6w[6] = {0.171324492379170, 0.360761573048139, 0.467913934572691,
+                                   0.467913934572691, 0.360761573048139, 0.171324492379170};
+
+// Azimuth rule: eta = pi/4 (x + 1) on [0, pi/2] with the 8-point Gauss nodes;
+// sin, cos, 1 - sin (cancellation-free) and weight pi/4 w of every node.
+__constant__ float kEtaSin[8] = {3.1183217838e-02f, 1.5901983477e-01f, 3.6408107699e-01f, 5.9826084449e-01f, 8.0130141766e-01f, 9.3136725806e-01f, 9.8727538820e-01f, 9.9951368521e-01f};
+__constant__ float kEtaCos[8] = {9.9951368521e-01f, 9.8727538820e-01f, 9.3136725806e-01f, 8.0130141766e-01f, 5.9826084449e-01f, 3.6408107699e-01f, 1.5901983477e-01f, 3.1183217838e-02f};
+__constant__ float kEtaOneMinus[8] = {9.6881678216e-01f, 8.4098016523e-01f, 6.3591892301e-01f, 4.0173915551e-01f, 1.9869858234e-01f, 6.8632741945e-02f, 1.2724611798e-02f, 4.8631478841e-04f};
+__constant__ float kEtaWeight[8] = {7.9504706486e-02f, 1.7465765603e-01f, 2.4638462352e-01f, 2.8485117736e-01f, 2.8485117736e-01f, 2.4638462352e-01f, 1.7465765603e-01f, 7.9504706486e-02f};
 
 struct O0Params {
     // OMs
@@ -75,6 +98,13 @@ struct O0Params {
     double phase_min, phase_max;
     int chunk, nchunk;
     double* partial;          // (nom, nchunk)
+    double* bounds;           // (nchunk, kBoundWords) bounding sphere / direction cone
+    // bins only
+    const double* start_ns;   // (nel)
+    const double* end_ns;     // (nel)
+    const double* origins;    // (nom) time origins
+    const double* bin_edges;  // (nbins + 1)
+    int nbins;
     int* flags;               // [0]: glass intersected, [1]: unsupported geometry
 };
 
@@ -163,6 +193,8 @@ struct Slot {
     float psi_theta_a;  // psi - theta_a
     float theta_a, cos_theta_a;
     float ta, tb;
+    float base;       // emission time minus the OM time origin [ns] (bins only)
+    float DmR;        // D - R
     int piece;        // medium panel index, -1: empty
 };
 
@@ -170,7 +202,7 @@ struct Slot {
 // medium panel slot % pieces.
 __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double* scut, int nscut,
                           double z, double rho, double L, double R, double beta,
-                          const double* theta_major, double norm) {
+                          const double* theta_major, double norm, double t_start, double tau) {
     Slot out;
     out.piece = -1;
     const int snode = slot / pieces;
@@ -181,11 +213,7 @@ __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double*
     const double sa = scut[spiece], sb = scut[spiece + 1];
     if (!(sb > sa)) return out;
     // 6-point Gauss-Legendre on [sa, sb]
-    const double gx[kNs] = {-0.932469514203152, -0.661209386466265, -0.238619186083197,
-                            0.238619186083197, 0.661209386466265, 0.932469514203152};
-    const double gw[kNs] = {0.171324492379170, 0.360761573048139, 0.467913934572691,
-                            0.467913934572691, 0.360761573048139, 0.171324492379170};
-    const double s = 0.5 * (sa + sb) + 0.5 * (sb - sa) * gx[j];
+    const double s = 0.5 * (sa + sb) + 0.5 * (sb - sa) * kGauss6x[j];
     const double along = z - s;
     const double D = sqrt(rho * rho + along * along);
     const double psi = atan2(rho, along);
@@ -198,7 +226,7 @@ __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double*
     const double tb = xb >= 1. ? 0.5 * kPi : asin(xb);
     if (!(tb > ta)) return out;
     const double hs = sin(0.5 * eps);
-    out.weight = (float)(0.5 * (sb - sa) * gw[j] * norm);
+    out.weight = (float)(0.5 * (sb - sa) * kGauss6w[j] * norm);
     out.D = (float)D;
     out.D2R2 = (float)((D - R) * (D + R));
     out.eps = (float)eps;
@@ -209,6 +237,8 @@ __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double*
     out.cos_theta_a = (float)cos(th0);
     out.ta = (float)ta;
     out.tb = (float)tb;
+    out.base = (float)(t_start + tau * s);
+    out.DmR = (float)(D - R);
     out.piece = piece;
     return out;
 }
@@ -226,6 +256,8 @@ __device__ __forceinline__ Slot shuffle(const Slot& s, int lane) {
     o.cos_theta_a = __shfl(s.cos_theta_a, lane);
     o.ta = __shfl(s.ta, lane);
     o.tb = __shfl(s.tb, lane);
+    o.base = __shfl(s.base, lane);
+    o.DmR = __shfl(s.DmR, lane);
     o.piece = __shfl(s.piece, lane);
     return o;
 }
@@ -234,6 +266,7 @@ struct PanelTables {
     // medium panels between consecutive major lines
     float lam_a[kMaxMajor], dlam_dn[kMaxMajor], n_a[kMaxMajor];
     float mu_a[kMaxMajor], dmu[kMaxMajor], span[kMaxMajor];
+    float g_a[kMaxMajor], dg[kMaxMajor];
     // efficiency panels
     float edges[kMaxPanels + 1];
     float eff[kMaxPanels][4];
@@ -249,11 +282,22 @@ __device__ __forceinline__ int panel_of(const PanelTables& T, int npanel, float 
     return lo;
 }
 
-// Value of the lane's (theta, phi) node of one slot (without the slot weight).
-__device__ float node_value(const PanelTables& T, const Slot& S, int npanel, int nalpha,
-                            float beta, float c0, float c2, float inv_length, float axial,
-                            float across, float side, float xt, float wt, float se, float ce,
-                            float we) {
+// sin(x) for the small angles of the cap (|x| < 0.3: odd Taylor series to
+// x^11, relative error < 1e-9), the library sinf otherwise.
+__device__ __forceinline__ float sin_small(float x) {
+    if (fabsf(x) < 0.3f) {
+        const float x2 = x * x;
+        return x * (1.f + x2 * (-1.f / 6.f + x2 * (1.f / 120.f + x2 * (-1.f / 5040.f
+                    + x2 * (1.f / 362880.f + x2 * (-1.f / 39916800.f))))));
+    }
+    return sinf(x);
+}
+
+// Value of one slot at the lane's theta node, summed over the 8 azimuth
+// nodes (without the slot weight).
+__device__ float theta_node_value(const PanelTables& T, const Slot& S, int npanel, int nalpha,
+                                  float beta, float c0, float c2, float inv_length, float axial,
+                                  float across, float side, float xt, float wt) {
     const int k = S.piece;
     const float hw = 0.5f * (S.tb - S.ta);
     const float t = 0.5f * (S.ta + S.tb) + hw * xt;
@@ -265,7 +309,8 @@ __device__ float node_value(const PanelTables& T, const Slot& S, int npanel, int
     float sth, cth;
     sincosf(theta, &sth, &cth);
     // n - n_a = 2 sin((theta + theta_a)/2) sin((theta - theta_a)/2) / (beta cos(theta) cos(theta_a))
-    const float dn = 2.f * sinf(S.theta_a + 0.5f * dth) * sinf(0.5f * dth) / (beta * cth * S.cos_theta_a);
+    const float dn = 2.f * sinf(S.theta_a + 0.5f * dth) * sin_small(0.5f * dth)
+                     / (beta * cth * S.cos_theta_a);
     const float n = T.n_a[k] + dn;
     const float lam = T.lam_a[k] + dn * T.dlam_dn[k];
     const float qm = (lam - T.lam_a[k]) / T.span[k];
@@ -278,41 +323,384 @@ __device__ float node_value(const PanelTables& T, const Slot& S, int npanel, int
     // azimuth half-width inside the cap: hav(half) = (hav eps - hav delta) / (sin theta sin psi)
     float om, op;
     one_minus_plus(st, ct, om, op);
-    const float H = sinf(0.5f * S.eps * om) * sinf(0.5f * S.eps * op);
+    const float H = sin_small(0.5f * S.eps * om) * sin_small(0.5f * S.eps * op);
     const float sts = sth * S.sinpsi;
     const float hh = fminf(H / sts, 1.f);
     if (!(hh > 0.f)) return 0.f;
     const float half = 2.f * asinf(sqrtf(hh));
-    const float phi = half * se;
-    const float dphi = half * ce * we;
-    const float hphi = sinf(0.5f * phi);
-    const float hdel = sinf(0.5f * delta);
-    const float hav_g = hdel * hdel + sts * hphi * hphi;
-    // hav(eps) - hav(gamma) = sin(theta) sin(psi) sin((half - phi)/2) sin((half + phi)/2)
-    float em, ep;
-    one_minus_plus(se, ce, em, ep);
-    const float gap = sts * sinf(0.5f * half * em) * sinf(0.5f * half * ep);
-    const float inside = fmaxf(gap * (1.f - S.hav_eps - hav_g), 0.f);
-    const float d = S.D2R2 / (S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
-    float sp, cp;
-    sincosf(phi, &sp, &cp);
-    const float acc = acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across + sp * side))
-                    + acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across - sp * side));
-    return hw * wt * S.eps * ct * spectral * jac * dphi * acc * expf(-mu * d);
+    const float hdel = sin_small(0.5f * delta);
+    const float hav_d = hdel * hdel;
+    const float rest = 1.f - S.hav_eps;
+    float sum = 0.f;
+#pragma unroll
+    for (int ie = 0; ie < kNe; ++ie) {
+        const float se = kEtaSin[ie], ce = kEtaCos[ie], em = kEtaOneMinus[ie];
+        const float ep = 1.f + se;
+        const float phi = half * se;
+        const float hphi = sin_small(0.5f * phi);
+        const float hav_g = hav_d + sts * hphi * hphi;
+        // hav(eps) - hav(gamma) = sin(theta) sin(psi) sin((half - phi)/2) sin((half + phi)/2)
+        const float gap = sts * sin_small(0.5f * half * em) * sin_small(0.5f * half * ep);
+        const float inside = fmaxf(gap * (rest - hav_g), 0.f);
+        const float d = S.D2R2 / (S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
+        // cos(phi) = 1 - 2 hav(phi), sin(phi) = 2 sin(phi/2) cos(phi/2)
+        const float cp = 1.f - 2.f * hphi * hphi;
+        const float sp = 2.f * hphi * sqrtf(fmaxf(0.f, 1.f - hphi * hphi));
+        const float acc = acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across + sp * side))
+                        + acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across - sp * side));
+        sum += half * ce * kEtaWeight[ie] * acc * expf(-mu * d);
+    }
+    return hw * wt * S.eps * ct * spectral * jac * sum;
 }
 
-__global__ __launch_bounds__(kThreads) void order0_charge(O0Params P) {
+// Quantities of a slot at one theta (t) that the bins need.
+struct ThetaPoint {
+    float st, ct, sth, cth, n, lam, qm, mu, ng_c, sts, half, hav_d;
+    float d0;      // entry distance on the centre meridian (phi = 0)
+    bool valid;
+};
+
+__device__ __forceinline__ ThetaPoint theta_point(const PanelTables& T, const Slot& S, float beta,
+                                                  float t) {
+    ThetaPoint p;
+    const int k = S.piece;
+    sincosf(t, &p.st, &p.ct);
+    const float delta = S.eps * p.st;
+    const float dth = S.psi_theta_a + delta;
+    const float theta = S.theta_a + dth;
+    sincosf(theta, &p.sth, &p.cth);
+    const float dn = 2.f * sinf(S.theta_a + 0.5f * dth) * sin_small(0.5f * dth)
+                     / (beta * p.cth * S.cos_theta_a);
+    p.n = T.n_a[k] + dn;
+    p.lam = T.lam_a[k] + dn * T.dlam_dn[k];
+    p.qm = (p.lam - T.lam_a[k]) / T.span[k];
+    p.mu = T.mu_a[k] + p.qm * T.dmu[k];
+    p.ng_c = (T.g_a[k] + p.qm * T.dg[k]) * (float)(1.0 / kC);
+    float om, op;
+    one_minus_plus(p.st, p.ct, om, op);
+    const float H = sin_small(0.5f * S.eps * om) * sin_small(0.5f * S.eps * op);
+    p.sts = p.sth * S.sinpsi;
+    const float hh = fminf(H / p.sts, 1.f);
+    p.valid = hh > 0.f;
+    p.half = p.valid ? 2.f * asinf(sqrtf(hh)) : 0.f;
+    const float hdel = sin_small(0.5f * delta);
+    p.hav_d = hdel * hdel;
+    const float rest = 1.f - S.hav_eps;
+    p.d0 = S.D2R2 / (S.D * (1.f - 2.f * p.hav_d)
+                     + 2.f * S.D * sqrtf(fmaxf((S.hav_eps - p.hav_d) * (rest - p.hav_d), 0.f)));
+    return p;
+}
+
+// Arrival time on the centre meridian of a slot at t.
+__device__ __forceinline__ float meridian_time(const PanelTables& T, const Slot& S, float beta,
+                                               float t) {
+    const ThetaPoint p = theta_point(T, S, beta, t);
+    return S.base + p.d0 * p.ng_c;
+}
+
+// One theta node at t with weight wt_t (dtheta/dt and the rule weight
+// included): the azimuth chord [0, half] is split at every bin edge its
+// arrival time crosses (the time grows with the azimuth); each piece is
+// integrated with the 8-point rule and added to the bin it lies in.
+__device__ void node_bins(const PanelTables& T, const Slot& S, int npanel, int nalpha, float beta,
+                          float c0, float c2, float inv_length, float axial, float across,
+                          float side, float t, float wt_t, float R, const float* edges, int nbins,
+                          double* wave_bins) {
+    const ThetaPoint p = theta_point(T, S, beta, t);
+    if (!p.valid) return;
+    const int e = panel_of(T, npanel, p.lam);
+    const float qe = (p.lam - T.edges[e]) / (T.edges[e + 1] - T.edges[e]);
+    const float eta = ((T.eff[e][3] * qe + T.eff[e][2]) * qe + T.eff[e][1]) * qe + T.eff[e][0];
+    const float spectral = (c0 + c2 / (p.n * p.n)) * inv_length / (p.lam * p.lam) * eta;
+    const float jac = beta * p.n * p.n * p.sth * fabsf(T.dlam_dn[S.piece]);
+    const float prefactor = S.weight * wt_t * S.eps * p.ct * spectral * jac;
+    const float rest = 1.f - S.hav_eps;
+    const float drim = S.D * (1.f - 2.f * S.hav_eps);
+    const float t0 = S.base + p.d0 * p.ng_c, t1 = S.base + drim * p.ng_c;
+    int lo = 0, hi = nbins + 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (edges[mid] <= t0) lo = mid + 1; else hi = mid;
+    }
+    int edge = lo;
+    float eta_a = 0.f;
+    for (;;) {
+        float eta_b = 0.5f * 3.14159265358979f;
+        const bool split = edge <= nbins && edges[edge] < t1;
+        if (split) {
+            const float dT = (edges[edge] - S.base) / p.ng_c;
+            const float hg = (dT - S.DmR) * (S.DmR + 2.f * R - dT) / (4.f * S.D * dT);
+            const float hp = (hg - p.hav_d) / p.sts;
+            const float x = hp <= 0.f ? 0.f
+                                      : fminf(2.f * asinf(fminf(sqrtf(hp), 1.f)) / p.half, 1.f);
+            eta_b = fmaxf(asinf(x), eta_a);
+        }
+        const int bin = edge - 1;
+        if (eta_b > eta_a && bin >= 0 && bin < nbins) {
+            const float hwe = 0.5f * (eta_b - eta_a), mide = 0.5f * (eta_a + eta_b);
+            float sum = 0.f;
+#pragma unroll
+            for (int ie = 0; ie < kNe; ++ie) {
+                float se, ce;
+                sincosf(mide + hwe * kGauss8x[ie], &se, &ce);
+                const float em = ce * ce / (1.f + se), ep = 1.f + se;
+                const float phi = p.half * se;
+                const float hphi = sin_small(0.5f * phi);
+                const float hav_g = p.hav_d + p.sts * hphi * hphi;
+                const float gap = p.sts * sin_small(0.5f * p.half * em) * sin_small(0.5f * p.half * ep);
+                const float inside = fmaxf(gap * (rest - hav_g), 0.f);
+                const float d = S.D2R2 / (S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
+                const float cp = 1.f - 2.f * hphi * hphi;
+                const float sp = 2.f * hphi * sqrtf(fmaxf(0.f, 1.f - hphi * hphi));
+                const float acc = acceptance(T.alpha, nalpha, p.cth * axial + p.sth * (cp * across + sp * side))
+                                + acceptance(T.alpha, nalpha, p.cth * axial + p.sth * (cp * across - sp * side));
+                sum += p.half * ce * hwe * kGauss8w[ie] * acc * expf(-p.mu * d);
+            }
+            atomicAdd(&wave_bins[bin], (double)(prefactor * sum));
+        }
+        if (!split) break;
+        eta_a = eta_b;
+        ++edge;
+    }
+}
+
+constexpr int kMaxCuts = 12;
+
+// Bins of one slot at the lane's theta node. The slot's theta range is cut
+// where a bin edge crosses the centre meridian (sampled on 9 points, refined
+// by bisection) or the rim (there d_entry = D cos(eps) for every theta, so the
+// crossing is where the group index takes one value: closed form); every
+// piece gets the lane's node on a sine map, which absorbs the square-root
+// behaviour of the bin weights at the cuts.
+__device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int nalpha, float beta,
+                          float c0, float c2, float inv_length, float axial, float across,
+                          float side, float xt, float wt, float R, const float* edges, int nbins,
+                          double* wave_bins) {
+    const int k = S.piece;
+    float cuts[kMaxCuts + 2];
+    int ncut = 0;
+    cuts[ncut++] = S.ta;
+    float tv[9], tt[9];
+    float lo_t = 3.4e38f, hi_t = -3.4e38f;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        tt[j] = S.ta + (S.tb - S.ta) * (j * 0.125f);
+        tv[j] = meridian_time(T, S, beta, tt[j]);
+        lo_t = fminf(lo_t, tv[j]);
+        hi_t = fmaxf(hi_t, tv[j]);
+    }
+    // rim times at the panel's two group indices
+    const float drim = S.D * (1.f - 2.f * S.hav_eps);
+    const float ga = T.g_a[k] * (float)(1.0 / kC), gb = (T.g_a[k] + T.dg[k]) * (float)(1.0 / kC);
+    lo_t = fminf(lo_t, S.base + drim * fminf(ga, gb));
+    hi_t = fmaxf(hi_t, S.base + drim * fmaxf(ga, gb));
+    int e = 0;
+    {
+        int a = 0, b = nbins + 1;
+        while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (edges[mid] <= lo_t) a = mid + 1; else b = mid;
+        }
+        e = a;
+    }
+    for (; e <= nbins && edges[e] < hi_t && ncut < kMaxCuts; ++e) {
+        const float Tedge = edges[e];
+        for (int j = 0; j < 8 && ncut < kMaxCuts; ++j) {
+            if ((tv[j] < Tedge) == (tv[j + 1] < Tedge)) continue;
+            float a = tt[j], b = tt[j + 1];
+            const bool rising = tv[j] < Tedge;
+            for (int it = 0; it < 24; ++it) {
+                const float mid = 0.5f * (a + b);
+                if ((meridian_time(T, S, beta, mid) < Tedge) == rising) a = mid; else b = mid;
+            }
+            cuts[ncut++] = 0.5f * (a + b);
+        }
+        // rim crossing: n_g = (T - base) c / (D cos eps)
+        if (T.dg[k] != 0.f && ncut < kMaxCuts) {
+            const float q = ((Tedge - S.base) / drim * (float)kC - T.g_a[k]) / T.dg[k];
+            if (q > 0.f && q < 1.f) {
+                // theta of that wavelength, relative to theta_a: fixed point on
+                // sin((th - th_a)/2) = (n - n_a) beta cos(th) cos(th_a) / (2 sin((th + th_a)/2))
+                const float dn = q * T.span[k] / T.dlam_dn[k];
+                float dth = 0.f;
+                for (int it = 0; it < 3; ++it) {
+                    const float th = S.theta_a + dth;
+                    dth = 2.f * asinf(dn * beta * cosf(th) * S.cos_theta_a
+                                      / (2.f * sinf(S.theta_a + 0.5f * dth)));
+                }
+                const float x = (dth - S.psi_theta_a) / S.eps;
+                if (x > -1.f && x < 1.f) {
+                    const float tc = asinf(x);
+                    if (tc > S.ta && tc < S.tb) cuts[ncut++] = tc;
+                }
+            }
+        }
+    }
+    cuts[ncut++] = S.tb;
+    for (int a = 1; a < ncut; ++a)  // insertion sort (few entries)
+        for (int b = a; b > 0 && cuts[b - 1] > cuts[b]; --b) {
+            const float tmp = cuts[b];
+            cuts[b] = cuts[b - 1];
+            cuts[b - 1] = tmp;
+        }
+    const float xm = 1.57079632679f * xt;
+    float sxm, cxm;
+    sincosf(xm, &sxm, &cxm);
+    for (int piece = 0; piece + 1 < ncut; ++piece) {
+        const float pa = cuts[piece], pb = cuts[piece + 1];
+        if (!(pb > pa)) continue;
+        const float hw = 0.5f * (pb - pa);
+        node_bins(T, S, npanel, nalpha, beta, c0, c2, inv_length, axial, across, side,
+                  0.5f * (pa + pb) + hw * sxm, hw * cxm * 1.57079632679f * wt, R, edges, nbins,
+                  wave_bins);
+    }
+}
+
+// Bounding sphere of the chunk's segments (centre, radius), mean direction
+// and the cosine of the largest deviation from it, and the beta range.
+__global__ __launch_bounds__(kThreads) void order0_chunk_bounds(O0Params P) {
+    __shared__ double red[kThreads][4];
+    const int tid = threadIdx.x;
+    const int first = blockIdx.x * P.chunk;
+    const int last = min(P.nel, first + P.chunk);
+    double acc[4] = {0., 0., 0., 0.};
+    double dir[3] = {0., 0., 0.};
+    for (int i = first + tid; i < last; i += kThreads) {
+        const double h = 0.5 * P.length[i];
+        acc[0] += P.start[3 * i] + h * P.dir[3 * i];
+        acc[1] += P.start[3 * i + 1] + h * P.dir[3 * i + 1];
+        acc[2] += P.start[3 * i + 2] + h * P.dir[3 * i + 2];
+        dir[0] += P.dir[3 * i];
+        dir[1] += P.dir[3 * i + 1];
+        dir[2] += P.dir[3 * i + 2];
+    }
+    // centre
+    for (int k = 0; k < 3; ++k) red[tid][k] = acc[k];
+    __syncthreads();
+    for (int half = kThreads / 2; half > 0; half >>= 1) {
+        if (tid < half)
+            for (int k = 0; k < 3; ++k) red[tid][k] += red[tid + half][k];
+        __syncthreads();
+    }
+    const double n = (double)(last - first);
+    const double cx = red[0][0] / n, cy = red[0][1] / n, cz = red[0][2] / n;
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) red[tid][k] = dir[k];
+    __syncthreads();
+    for (int half = kThreads / 2; half > 0; half >>= 1) {
+        if (tid < half)
+            for (int k = 0; k < 3; ++k) red[tid][k] += red[tid + half][k];
+        __syncthreads();
+    }
+    double ux = red[0][0], uy = red[0][1], uz = red[0][2];
+    const double un = sqrt(ux * ux + uy * uy + uz * uz);
+    __syncthreads();
+    // radius, smallest direction cosine, beta range
+    double radius = 0., cosmin = 1., bmin = 1e300, bmax = 0.;
+    if (un > 0.) {
+        ux /= un;
+        uy /= un;
+        uz /= un;
+    } else {
+        cosmin = -1.;
+    }
+    for (int i = first + tid; i < last; i += kThreads) {
+        const double L = P.length[i];
+        for (int e = 0; e < 2; ++e) {
+            const double x = P.start[3 * i] + e * L * P.dir[3 * i] - cx;
+            const double y = P.start[3 * i + 1] + e * L * P.dir[3 * i + 1] - cy;
+            const double z = P.start[3 * i + 2] + e * L * P.dir[3 * i + 2] - cz;
+            radius = fmax(radius, sqrt(x * x + y * y + z * z));
+        }
+        cosmin = fmin(cosmin, P.dir[3 * i] * ux + P.dir[3 * i + 1] * uy + P.dir[3 * i + 2] * uz);
+        bmin = fmin(bmin, P.beta[i]);
+        bmax = fmax(bmax, P.beta[i]);
+    }
+    red[tid][0] = radius;
+    red[tid][1] = cosmin;
+    red[tid][2] = bmin;
+    red[tid][3] = bmax;
+    __syncthreads();
+    for (int half = kThreads / 2; half > 0; half >>= 1) {
+        if (tid < half) {
+            red[tid][0] = fmax(red[tid][0], red[tid + half][0]);
+            red[tid][1] = fmin(red[tid][1], red[tid + half][1]);
+            red[tid][2] = fmin(red[tid][2], red[tid + half][2]);
+            red[tid][3] = fmax(red[tid][3], red[tid + half][3]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double* b = P.bounds + (size_t)blockIdx.x * kBoundWords;
+        b[0] = cx; b[1] = cy; b[2] = cz; b[3] = red[0][0] * (1.0 + 1e-12) + 1e-9;
+        b[4] = ux; b[5] = uy; b[6] = uz; b[7] = red[0][1];
+        b[8] = red[0][2]; b[9] = red[0][3];
+    }
+}
+
+// Conservative test that no element of a chunk can light the OM: every
+// photon reaching the sphere leaves at an angle theta to its element with
+// |theta - alpha| <= asin(r/D) + delta_u + asin(R/(D - r)), alpha the angle of
+// the OM seen from the chunk centre against the mean direction.
+__device__ bool chunk_culled(const O0Params& P, int chunk, int om) {
+    const double* b = P.bounds + (size_t)chunk * kBoundWords;
+    const double vx = P.positions[3 * om] - b[0], vy = P.positions[3 * om + 1] - b[1],
+                 vz = P.positions[3 * om + 2] - b[2];
+    const double D = sqrt(vx * vx + vy * vy + vz * vz);
+    const double r = b[3], R = P.radii[om];
+    if (!(D > r + R + 1e-6)) return false;
+    const double dot = vx * b[4] + vy * b[5] + vz * b[6];
+    const double cx = vy * b[6] - vz * b[5], cy = vz * b[4] - vx * b[6], cz = vx * b[5] - vy * b[4];
+    const double alpha = atan2(sqrt(cx * cx + cy * cy + cz * cz), dot);
+    const double delta = acos(fmin(1.0, fmax(-1.0, b[7])));
+    const double slack = asin(fmin(1.0, r / D)) + delta + asin(fmin(1.0, R / (D - r))) + 1e-6;
+    const double lo_bn = b[8] * P.phase_min, hi_bn = b[9] * P.phase_max;
+    if (hi_bn <= 1.0) return true;
+    const double theta_min = lo_bn > 1.0 ? acos(1.0 / lo_bn) : 0.0;
+    const double theta_max = acos(1.0 / hi_bn);
+    return alpha - slack > theta_max || alpha + slack < theta_min;
+}
+
+template <bool kBins>
+__global__ __launch_bounds__(kThreads) void order0_pairs(O0Params P) {
+    extern __shared__ double dynamic_lds[];
     __shared__ PanelTables T;
-    __shared__ int list[2048];
+    __shared__ int list[kChunk];
     __shared__ int wave_count[kWaves];
     __shared__ int total_count;
     __shared__ double wave_sum[kWaves];
+    __shared__ double wave_theta[kWaves][kMaxMajor];
+    __shared__ double wave_roots[kWaves][2 * kMaxMajor];
+    __shared__ double wave_scut[kWaves][2 * kMaxMajor + 2];
+    __shared__ int wave_nscut[kWaves];
     __shared__ int glass;
+    __shared__ int culled;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int chunk = blockIdx.x;
     const int om = blockIdx.y;
+
+    if (tid == 0) culled = chunk_culled(P, chunk, om) ? 1 : 0;
+    __syncthreads();
+    if (culled) {
+        if (kBins) {
+            double* out = P.partial + ((size_t)om * P.nchunk + chunk) * P.nbins;
+            for (int b = tid; b < P.nbins; b += kThreads) out[b] = 0.0;
+        } else if (tid == 0) {
+            P.partial[(size_t)om * P.nchunk + chunk] = 0.0;
+        }
+        return;
+    }
+    // bins: one histogram per wave, then the bin edges (FP32)
+    double* wave_bins = dynamic_lds + (size_t)wave * P.nbins;
+    float* edges = reinterpret_cast<float*>(dynamic_lds + (size_t)kWaves * P.nbins);
+    if (kBins) {
+        for (int b = tid; b < kWaves * P.nbins; b += kThreads) dynamic_lds[b] = 0.0;
+        const double origin = P.origins[om];
+        for (int b = tid; b <= P.nbins; b += kThreads) edges[b] = (float)P.bin_edges[b];
+        (void)origin;
+    }
 
     // tables of this launch
     const int pieces = P.nmajor - 1;
@@ -324,6 +712,8 @@ __global__ __launch_bounds__(kThreads) void order0_charge(O0Params P) {
         T.dlam_dn[tid] = (float)((P.edges[b] - P.edges[a]) / (P.phase[b] - P.phase[a]));
         T.mu_a[tid] = (float)P.ext[a];
         T.dmu[tid] = (float)(P.ext[b] - P.ext[a]);
+        T.g_a[tid] = (float)P.group[a];
+        T.dg[tid] = (float)(P.group[b] - P.group[a]);
     }
     for (int k = tid; k <= P.npanel; k += kThreads) T.edges[k] = (float)P.edges[k];
     for (int k = tid; k < P.npanel * 4; k += kThreads)
@@ -362,21 +752,10 @@ __global__ __launch_bounds__(kThreads) void order0_charge(O0Params P) {
     if (tid == 0 && glass) atomicOr(&P.flags[0], 1);
     const int count = total_count;
 
-    // integrate: one wave per pair
-    const int it = lane >> 3;            // theta node
-    const int ie = lane & 7;             // azimuth node
-    const float gxt[kNt] = {-0.960289856497536f, -0.796666477413627f, -0.525532409916329f,
-                            -0.183434642495650f, 0.183434642495650f, 0.525532409916329f,
-                            0.796666477413627f, 0.960289856497536f};
-    const float gwt[kNt] = {0.101228536290376f, 0.222381034453374f, 0.313706645877887f,
-                            0.362683783378362f, 0.362683783378362f, 0.313706645877887f,
-                            0.222381034453374f, 0.101228536290376f};
-    const float xt = gxt[it], wt = gwt[it];
-    // azimuth node on [0, pi/2]: eta = pi/4 (x + 1)
-    const float eta_node = 0.78539816339744831f * (gxt[ie] + 1.f);
-    const float we = 0.78539816339744831f * gwt[ie];
-    float se, ce;
-    sincosf(eta_node, &se, &ce);
+    // integrate: one wave per pair; lane = (slot of a group of 8, theta node)
+    const int it = lane & 7;             // theta node
+    const int sub = lane >> 3;           // slot within the group of 8
+    const float xt = kGauss8x[it], wt = kGauss8w[it];
 
     const double* pos = P.positions + 3 * om;
     const double* look = P.looks + 3 * om;
@@ -401,46 +780,87 @@ __global__ __launch_bounds__(kThreads) void order0_charge(O0Params P) {
         const float across = (float)(e1x * look[0] + e1y * look[1] + e1z * look[2]);
         const float side = (float)(e2x * look[0] + e2y * look[1] + e2z * look[2]);
         const double L = P.length[i], beta = P.beta[i];
-        // cone angles of the medium panel edges and the s pieces
-        double theta_major[kMaxMajor];
-        double scut[2 * kMaxMajor + 2];
-        int nscut = 0;
-        scut[nscut++] = 0.0;
-        for (int m = 0; m < P.nmajor; ++m) {
-            const double c = 1.0 / (beta * P.phase[P.major[m]]);
+        const double t_start = kBins ? P.start_ns[i] - P.origins[om] : 0.0;
+        const double tau = kBins ? (P.end_ns[i] - P.start_ns[i]) / L : 0.0;
+        // cone angles of the medium panel edges (one lane each) and the s
+        // pieces: [0, L] cut where a panel edge is tangent to the cap
+        if (lane < P.nmajor) {
+            const double c = 1.0 / (beta * P.phase[P.major[lane]]);
             const double sn = sqrt(fmax(0.0, (1.0 - c) * (1.0 + c)));
-            theta_major[m] = atan2(sn, c);
+            wave_theta[wave][lane] = atan2(sn, c);
+            double r1 = -1.0, r2 = -1.0;
             if (sn > 0.0) {
                 const double centre = z - rho * c / sn, spread = R / sn;
-                const double r1 = centre - spread, r2 = centre + spread;
-                if (r1 > 0.0 && r1 < L) scut[nscut++] = r1;
-                if (r2 > 0.0 && r2 < L) scut[nscut++] = r2;
+                r1 = centre - spread;
+                r2 = centre + spread;
             }
+            wave_roots[wave][2 * lane] = r1;
+            wave_roots[wave][2 * lane + 1] = r2;
         }
-        scut[nscut++] = L;
-        for (int a = 1; a < nscut; ++a)  // insertion sort (few entries)
-            for (int b = a; b > 0 && scut[b - 1] > scut[b]; --b) {
-                const double tmp = scut[b];
-                scut[b] = scut[b - 1];
-                scut[b - 1] = tmp;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+            double* cut = wave_scut[wave];
+            int n = 0;
+            cut[n++] = 0.0;
+            for (int m = 0; m < 2 * P.nmajor; ++m) {
+                const double r = wave_roots[wave][m];
+                if (r > 0.0 && r < L) cut[n++] = r;
             }
+            cut[n++] = L;
+            for (int a = 1; a < n; ++a)  // insertion sort (few entries)
+                for (int b = a; b > 0 && cut[b - 1] > cut[b]; --b) {
+                    const double tmp = cut[b];
+                    cut[b] = cut[b - 1];
+                    cut[b - 1] = tmp;
+                }
+            wave_nscut[wave] = n;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nscut = wave_nscut[wave];
+        const double* scut = wave_scut[wave];
+        const double* theta_major = wave_theta[wave];
         const int slots = (nscut - 1) * kNs * pieces;
         const float c0 = (float)P.c0[i], c2 = (float)P.c2[i];
         const float inv_length = (float)(1.0 / L);
         float pair_sum = 0.f;
         for (int group = 0; group < slots; group += 64) {
             const Slot mine = (group + lane < slots)
-                ? make_slot(P, group + lane, pieces, scut, nscut, z, rho, L, R, beta, theta_major, norm)
-                : Slot{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
+                ? make_slot(P, group + lane, pieces, scut, nscut, z, rho, L, R, beta, theta_major, norm,
+                            t_start, tau)
+                : Slot{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
             const int active = min(64, slots - group);
-            for (int k = 0; k < active; ++k) {
-                const Slot S = shuffle(mine, k);
-                if (S.piece < 0) continue;
-                pair_sum += S.weight * node_value(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
-                                                  inv_length, axial, across, side, xt, wt, se, ce, we);
+            for (int k0 = 0; k0 < active; k0 += 8) {
+                const Slot S = shuffle(mine, min(k0 + sub, 63));
+                if (k0 + sub < active && S.piece >= 0) {
+                    if (kBins)
+                        slot_bins(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length, axial,
+                                  across, side, xt, wt, (float)R, edges, P.nbins, wave_bins);
+                    else
+                        pair_sum += S.weight * theta_node_value(T, S, P.npanel, P.nalpha, (float)beta,
+                                                                c0, c2, inv_length, axial, across,
+                                                                side, xt, wt);
+                }
             }
         }
         lane_sum += (double)pair_sum;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (kBins) {
+        // waves in a fixed order
+        __syncthreads();
+        double* out = P.partial + ((size_t)om * P.nchunk + chunk) * P.nbins;
+        for (int b = tid; b < P.nbins; b += kThreads) {
+            double sum = 0.0;
+            for (int w = 0; w < kWaves; ++w) sum += dynamic_lds[(size_t)w * P.nbins + b];
+            out[b] = sum;
+        }
+        return;
     }
     // fixed-order reductions: lanes, then waves
     for (int offset = 32; offset > 0; offset >>= 1) lane_sum += __shfl_down(lane_sum, offset);
@@ -453,48 +873,42 @@ __global__ __launch_bounds__(kThreads) void order0_charge(O0Params P) {
     }
 }
 
-__global__ void order0_reduce(const double* partial, int nom, int nchunk, double* charge) {
-    const int om = blockIdx.x * blockDim.x + threadIdx.x;
-    if (om >= nom) return;
+// out[om][b] = sum over chunks in order of partial[om][chunk][b] (width = bins or 1)
+__global__ void order0_reduce(const double* partial, int nom, int nchunk, int width, double* out) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nom * width) return;
+    const size_t om = idx / width, b = idx - om * width;
     double sum = 0.0;
-    for (int c = 0; c < nchunk; ++c) sum += partial[(size_t)om * nchunk + c];
-    charge[om] = sum;
+    for (int c = 0; c < nchunk; ++c) sum += partial[(om * nchunk + c) * width + b];
+    out[idx] = sum;
 }
-
-constexpr int kChunk = 2048;
 
 }  // namespace
 
-extern "C" size_t lh_order0_workspace_bytes(int n_oms, int n_elements) {
-    const size_t nchunk = n_elements > 0 ? (size_t)(n_elements + kChunk - 1) / kChunk : 1;
-    return 256 + sizeof(double) * (size_t)(n_oms > 0 ? n_oms : 0) * nchunk;
+namespace {
+
+size_t chunks_of(int n_elements) {
+    return n_elements > 0 ? (size_t)(n_elements + kChunk - 1) / kChunk : 1;
 }
 
-extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
-                                const lh_order0_elements* elements, double* charge,
-                                void* workspace, size_t workspace_bytes, void* stream) {
-    LH_REQUIRE(optics && oms && elements, "lh_order0_charge: null descriptor");
-    LH_REQUIRE(oms->count >= 0 && elements->count >= 0, "lh_order0_charge: negative count");
-    LH_REQUIRE(workspace_bytes >= lh_order0_workspace_bytes(oms->count, elements->count),
-               "lh_order0_charge: workspace of %zu bytes, %zu needed", workspace_bytes,
-               lh_order0_workspace_bytes(oms->count, elements->count));
+int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                      const lh_order0_elements* elements, const char* name) {
+    LH_REQUIRE(optics && oms && elements, "%s: null descriptor", name);
+    LH_REQUIRE(oms->count >= 0 && elements->count >= 0, "%s: negative count", name);
+    LH_REQUIRE(oms->count <= 65535, "%s: at most 65535 OMs per launch", name);
     LH_REQUIRE(optics->panels >= 1 && optics->panels <= kMaxPanels,
-               "lh_order0_charge: 1..%d optical panels supported", kMaxPanels);
+               "%s: 1..%d optical panels supported", name, kMaxPanels);
     LH_REQUIRE(optics->major_count >= 2 && optics->major_count <= kMaxMajor,
-               "lh_order0_charge: 2..%d medium nodes in the band supported", kMaxMajor);
+               "%s: 2..%d medium nodes in the band supported", name, kMaxMajor);
     LH_REQUIRE(optics->acceptance_count >= 1 && optics->acceptance_count <= kMaxAlpha,
-               "lh_order0_charge: 1..%d acceptance coefficients supported", kMaxAlpha);
-    LH_REQUIRE(optics->efficiency_columns == 4,
-               "lh_order0_charge: cubic efficiency panels (4 columns) required");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    int* flags = static_cast<int*>(workspace);
-    double* partial = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
-    LH_HIP_CALL(hipMemsetAsync(flags, 0, 256, s));
-    if (oms->count == 0) return LH_OK;
-    if (elements->count == 0) {
-        LH_HIP_CALL(hipMemsetAsync(charge, 0, sizeof(double) * oms->count, s));
-        return LH_OK;
-    }
+               "%s: 1..%d acceptance coefficients supported", name, kMaxAlpha);
+    LH_REQUIRE(optics->efficiency_columns == 4, "%s: cubic efficiency panels (4 columns) required",
+               name);
+    return LH_OK;
+}
+
+O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                     const lh_order0_elements* elements, void* workspace, int width) {
     O0Params P{};
     P.positions = oms->positions;
     P.looks = oms->looks;
@@ -507,6 +921,8 @@ extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_
     P.c0 = elements->coefficient0;
     P.c2 = elements->coefficient2;
     P.beta = elements->beta;
+    P.start_ns = elements->start_ns;
+    P.end_ns = elements->end_ns;
     P.nel = elements->count;
     P.edges = optics->panel_edges;
     P.phase = optics->phase_nodes;
@@ -522,13 +938,84 @@ extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_
     P.phase_min = optics->phase_min;
     P.phase_max = optics->phase_max;
     P.chunk = kChunk;
-    P.nchunk = (elements->count + kChunk - 1) / kChunk;
-    P.partial = partial;
-    P.flags = flags;
-    hipLaunchKernelGGL(order0_charge, dim3(P.nchunk, P.nom), dim3(kThreads), 0, s, P);
-    int status = lh::check_launch("lh_order0_charge");
+    P.nchunk = (int)chunks_of(elements->count);
+    P.flags = static_cast<int*>(workspace);
+    P.bounds = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
+    P.partial = P.bounds + (size_t)P.nchunk * kBoundWords;
+    (void)width;
+    return P;
+}
+
+template <bool kBins>
+int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const char* name) {
+    hipLaunchKernelGGL(order0_chunk_bounds, dim3(P.nchunk), dim3(kThreads), 0, s, P);
+    int status = lh::check_launch(name);
     if (status != LH_OK) return status;
-    hipLaunchKernelGGL(order0_reduce, dim3((P.nom + 255) / 256), dim3(256), 0, s, partial, P.nom,
-                       P.nchunk, charge);
-    return lh::check_launch("lh_order0_charge (reduce)");
+    hipLaunchKernelGGL((order0_pairs<kBins>), dim3(P.nchunk, P.nom), dim3(kThreads), lds, s, P);
+    status = lh::check_launch(name);
+    if (status != LH_OK) return status;
+    const size_t total = (size_t)P.nom * width;
+    hipLaunchKernelGGL(order0_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       P.partial, P.nom, P.nchunk, width, out);
+    return lh::check_launch(name);
+}
+
+}  // namespace
+
+extern "C" size_t lh_order0_workspace_bytes(int n_oms, int n_elements) {
+    const size_t nchunk = chunks_of(n_elements);
+    return 256 + sizeof(double) * (nchunk * kBoundWords + (size_t)(n_oms > 0 ? n_oms : 0) * nchunk);
+}
+
+extern "C" size_t lh_order0_bins_workspace_bytes(int n_oms, int n_elements, int n_bins) {
+    const size_t nchunk = chunks_of(n_elements);
+    return 256 + sizeof(double) * (nchunk * kBoundWords
+                                   + (size_t)(n_oms > 0 ? n_oms : 0) * nchunk
+                                         * (size_t)(n_bins > 0 ? n_bins : 0));
+}
+
+extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                                const lh_order0_elements* elements, double* charge,
+                                void* workspace, size_t workspace_bytes, void* stream) {
+    int status = check_descriptors(optics, oms, elements, "lh_order0_charge");
+    if (status != LH_OK) return status;
+    LH_REQUIRE(workspace_bytes >= lh_order0_workspace_bytes(oms->count, elements->count),
+               "lh_order0_charge: workspace of %zu bytes, %zu needed", workspace_bytes,
+               lh_order0_workspace_bytes(oms->count, elements->count));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    LH_HIP_CALL(hipMemsetAsync(workspace, 0, 256, s));
+    if (oms->count == 0) return LH_OK;
+    if (elements->count == 0) {
+        LH_HIP_CALL(hipMemsetAsync(charge, 0, sizeof(double) * oms->count, s));
+        return LH_OK;
+    }
+    O0Params P = make_params(optics, oms, elements, workspace, 1);
+    return run<false>(P, 1, charge, s, 0, "lh_order0_charge");
+}
+
+extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                              const lh_order0_elements* elements, const double* origins,
+                              const double* bin_edges, int n_bins, double* bins, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+    int status = check_descriptors(optics, oms, elements, "lh_order0_bins");
+    if (status != LH_OK) return status;
+    LH_REQUIRE(n_bins >= 1 && n_bins <= 2048, "lh_order0_bins: 1..2048 bins supported (got %d)",
+               n_bins);
+    LH_REQUIRE(workspace_bytes >= lh_order0_bins_workspace_bytes(oms->count, elements->count, n_bins),
+               "lh_order0_bins: workspace of %zu bytes, %zu needed", workspace_bytes,
+               lh_order0_bins_workspace_bytes(oms->count, elements->count, n_bins));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    LH_HIP_CALL(hipMemsetAsync(workspace, 0, 256, s));
+    if (oms->count == 0) return LH_OK;
+    LH_REQUIRE(origins && bin_edges && bins, "lh_order0_bins: null device pointer");
+    if (elements->count == 0) {
+        LH_HIP_CALL(hipMemsetAsync(bins, 0, sizeof(double) * oms->count * (size_t)n_bins, s));
+        return LH_OK;
+    }
+    O0Params P = make_params(optics, oms, elements, workspace, n_bins);
+    P.origins = origins;
+    P.bin_edges = bin_edges;
+    P.nbins = n_bins;
+    const size_t lds = sizeof(double) * kWaves * (size_t)n_bins + sizeof(float) * (n_bins + 1);
+    return run<true>(P, n_bins, bins, s, lds, "lh_order0_bins");
 }

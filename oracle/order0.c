@@ -710,13 +710,14 @@ int oracle_order0(const int64_t* pair_om, const int64_t* pair_el, int64_t npairs
                   const double* phase, const double* group, const double* ext, const double* eff,
                   int effcols, const double* alpha, int nalpha, const double* origins,
                   const double* edges, int nbins, double rel_tol, double bin_tol,
-                  int force_rays, double* charge, double* bins) {
+                  int force_rays, double* charge, double* bins, double* bins_change) {
     Optics o = {npanel, panel_edges, phase, group, ext, eff, effcols};
     int regular_phase = 1;
     for (int k = 0; k < npanel; ++k)
         if (phase[k + 1] == phase[k]) regular_phase = 0;
     memset(charge, 0, sizeof(double) * nom);
     if (bins) memset(bins, 0, sizeof(double) * (size_t)nom * nbins);
+    if (bins_change) memset(bins_change, 0, sizeof(double) * nom);
     /* row boundaries */
     int64_t* first = (int64_t*)calloc((size_t)nom + 1, sizeof(int64_t));
     for (int64_t k = 0; k < npairs; ++k) first[pair_om[k] + 1]++;
@@ -739,8 +740,8 @@ int oracle_order0(const int64_t* pair_om, const int64_t* pair_el, int64_t npairs
                           start + 3 * i, direction + 3 * i, length[i], c0[i], c2[i], beta[i],
                           t0[i], t1[i]);
             int regular = regular_phase && g.rho > g.R && !force_rays;
-            double previous = NAN, value = 0.;
-            int converged = 0;
+            double previous = NAN, value = 0., last_l1 = 0.;
+            int converged = 0, charge_converged = 0;
             for (int m = 0; m < (regular ? 5 : 5); ++m) {
                 if (pb) memset(pb, 0, sizeof(double) * nbins);
                 if (regular) {
@@ -762,14 +763,18 @@ int oracle_order0(const int64_t* pair_om, const int64_t* pair_el, int64_t npairs
                     for (int b = 0; b < nbins; ++b) l1 += fabs(pb[b] - pb2[b]);
                     memcpy(pb2, pb, sizeof(double) * nbins);
                 }
-                if (m > 0 && change <= rel_tol * fabs(value) + 1e-300
-                    && l1 <= bin_tol * fabs(value) + 1e-300) {
+                last_l1 = l1;
+                charge_converged = m > 0 && change <= rel_tol * fabs(value) + 1e-300;
+                if (charge_converged && l1 <= bin_tol * fabs(value) + 1e-300) {
                     converged = 1;
                     break;
                 }
                 previous = value;
             }
-            if (!converged) failed = 1;
+            /* charge must converge; bins that reach the last level are kept
+             * and their last change is reported as their uncertainty */
+            if (!converged && !charge_converged) failed = 1;
+            if (bins_change) bins_change[om] += last_l1;
             charge[om] += value;
             if (rowb)
                 for (int b = 0; b < nbins; ++b) rowb[b] += pb[b];

@@ -66,8 +66,14 @@ def candidates(positions, radii, elements, optics):
 
 
 def first_entry(positions, looks, areas, radii, elements, optics, alpha, *, origins=None,
-                edges=None, rel_tol=1e-11, bin_tol=5e-5, force_rays=False, pairs=None):
-    """Charge (OM,) and bins (OM, bin) or None; looks are ``-orientations``."""
+                edges=None, rel_tol=1e-11, bin_tol=5e-5, force_rays=False, pairs=None,
+                return_uncertainty=False):
+    """Charge (OM,) and bins (OM, bin) or None; looks are ``-orientations``.
+
+    Every pair's charge converges to ``rel_tol``; its bins to ``bin_tol`` of
+    its charge or, at the highest order, keep their last L1 change, summed per
+    OM into the bin uncertainty returned with ``return_uncertainty``.
+    """
     positions = np.ascontiguousarray(positions, float)
     nom = len(positions)
     radii = np.ascontiguousarray(np.broadcast_to(radii, (nom,)), float)
@@ -83,6 +89,7 @@ def first_entry(positions, looks, areas, radii, elements, optics, alpha, *, orig
     nbins = len(edges_a) - 1
     charge = np.zeros(nom)
     bins = np.zeros((nom, nbins)) if want_bins else None
+    change = np.zeros(nom)
     eff = np.ascontiguousarray(optics["efficiency_coefficients"], float)
     al = np.ascontiguousarray(alpha, float)
     keep = [positions, looks, areas, radii, po, pe, edges_a, orig, eff, al]
@@ -100,8 +107,10 @@ def first_entry(positions, looks, areas, radii, elements, optics, alpha, *, orig
                ptr(orig) if want_bins else None, ptr(edges_a) if want_bins else None,
                ctypes.c_int(nbins), ctypes.c_double(rel_tol), ctypes.c_double(bin_tol),
                ctypes.c_int(int(force_rays)),
-               ptr(charge), ptr(bins) if want_bins else None)
+               ptr(charge), ptr(bins) if want_bins else None, ptr(change))
     del keep
     if failed:
         raise ValueError("order-0 oracle quadrature did not converge for some pair")
+    if return_uncertainty:
+        return charge, bins, change
     return charge, bins

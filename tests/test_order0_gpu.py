@@ -77,3 +77,44 @@ def test_glass_intersection_raises(device):
     moved = replace(e, start_m=start)
     with pytest.raises(ValueError, match='intersects the glass'):
         Order0(w.medium, w.detector, acceptance_alpha(w.detector)).charge(moved)
+
+
+def _compare_bins(detector, medium, elements, alpha, ids, origins, edges, tolerance):
+    from lighthit_amd.ballistic import Order0
+    optics = prepare_optics(medium, detector)
+    sub = detector.subset(ids)
+    charge, expected, uncertainty = oracle.first_entry(
+        sub.positions_m, sub.looks, sub.effective_area_m2, sub.receiver_surface.radii(len(sub)),
+        elements, optics, alpha, origins=origins, edges=edges, rel_tol=1e-9, bin_tol=1e-6,
+        return_uncertainty=True)
+    assert np.all(uncertainty < 0.1 * tolerance * charge)
+    got = Order0(medium, detector, alpha).bins(elements, ids, origins, edges)
+    # bin L1 relative to each OM's charge (inside the window the bins hold it all)
+    l1 = np.abs(got - expected).sum(axis=1) / np.maximum(charge, 1e-300)
+    assert l1.max() < tolerance, l1
+    np.testing.assert_allclose(got.sum(axis=1), expected.sum(axis=1), rtol=1e-5, atol=1e-12)
+    return got, expected
+
+
+def test_demo_event_bins(device):
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    ids = np.array([4, 10, 22, 29, 35, 40])
+    e, d = w.elements, w.detector
+    first = int(np.argmin(e.start_ns))
+    # the route's clock: first start + distance at the fastest group speed
+    origins = e.start_ns[first] + np.linalg.norm(d.positions_m[ids] - e.start_m[first], axis=1) \
+        / (0.299792458 / 1.367)
+    got, expected = _compare_bins(w.detector, w.medium, w.elements, acceptance_alpha(w.detector),
+                                  ids, origins, w.edges_ns, 1e-4)
+    assert np.count_nonzero(expected.sum(axis=0)) >= 2
+
+
+def test_fine_bins_split_chords(device):
+    # 0.25 ns bins: arrival chords cross bin edges
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    ids = np.array([10, 29])
+    edges = np.arange(0., 300., 0.25)
+    _compare_bins(w.detector, w.medium, w.elements.subset(np.arange(2000) % 5 == 0),
+                  acceptance_alpha(w.detector), ids, np.zeros(2), edges, 1e-3)
