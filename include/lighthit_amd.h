@@ -101,16 +101,21 @@ typedef struct lh_ge2_receivers {
     int count;
 } lh_ge2_receivers;
 
+/* Device workspace bytes lh_ge2_apply needs (0 when none). */
+size_t lh_ge2_workspace_bytes(int bands, int freq, int receivers, int points);
+
 /*
  * Descriptors are host structs holding device pointers. alpha (A+1) double
  * are the acceptance Legendre coefficients. The first `freq` frequencies are
  * evaluated (freq = 1: the zero-frequency charge). Points at a distance
  * outside [radial_low, radial_high] are skipped (radial_window_m).
- * out (freq, count) complex64.
+ * out (freq, count) complex64. workspace: device memory of at least
+ * lh_ge2_workspace_bytes (per-split partial sums; may be NULL when 0).
  */
 int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source,
                  const lh_ge2_receivers* receivers, const double* alpha, int freq,
-                 double radial_low, double radial_high, void* out, void* stream);
+                 double radial_low, double radial_high, void* out, void* workspace,
+                 size_t workspace_bytes, void* stream);
 
 /*
  * Stage (a): order-0 (direct) light to the first glass entry of every OM.

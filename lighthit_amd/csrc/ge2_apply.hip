@@ -52,6 +52,8 @@ struct Ge2Params {
     int nrec, nfreq;
     double r_low, r_high;
     int chunk, record_words;
+    int nsplit, span;          // wide: points split over nsplit workgroups of `span` points
+    float2* partial;           // wide with nsplit > 1: (nrec, nsplit, bands * nfreq)
 };
 
 __device__ __forceinline__ double parity(int n) { return (n & 1) ? -1.0 : 1.0; }
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
     constexpr int NA = R::NA, NM = R::NM, NPAIR = R::NPAIR;
     extern __shared__ float lds[];
     const int d = blockIdx.x;
+    const int split = blockIdx.y;
     const int tid = threadIdx.x;
     const int lanes = P.bands * P.nfreq;
     // Per-(band, frequency) accumulators, each owned by one thread, then the
@@ -321,9 +324,11 @@ __global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
     float2* acc = reinterpret_cast<float2*>(lds);
     float* smem = lds + ((2 * lanes + 3) & ~3);
     for (int lw = tid; lw < lanes; lw += kThreads) acc[lw] = make_float2(0.f, 0.f);
+    const int first = split * P.span;
+    const int last = min(P.npoints, first + P.span);
 
-    for (int base = 0; base < P.npoints; base += P.chunk) {
-        const int count = min(P.chunk, P.npoints - base);
+    for (int base = first; base < last; base += P.chunk) {
+        const int count = min(P.chunk, last - base);
         __syncthreads();
         for (int i = tid; i < count; i += kThreads) {
             float* rec = smem + (size_t)i * P.record_words;
@@ -449,6 +454,13 @@ __global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
         }
     }
 
+    if (P.nsplit > 1) {
+        // partial sums of this split; ge2_wide_reduce combines them in order
+        __syncthreads();
+        float2* out = P.partial + ((size_t)d * P.nsplit + split) * lanes;
+        for (int lw = tid; lw < lanes; lw += kThreads) out[lw] = acc[lw];
+        return;
+    }
     // Combine the bands of every frequency in a fixed order.
     __syncthreads();
     float2* red = acc;
@@ -466,6 +478,35 @@ __global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
         }
         P.out[(size_t)w * P.nrec + d] = sum;
     }
+}
+
+// out[w, d] = sum_band e^{i w ref_band} sum_split partial[d, split, band, w]
+__global__ void ge2_wide_reduce(Ge2Params P) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.nrec * P.nfreq) return;
+    const int d = idx / P.nfreq, w = idx - d * P.nfreq;
+    const int lanes = P.bands * P.nfreq;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int lam = 0; lam < P.bands; ++lam) {
+        float2 band = make_float2(0.f, 0.f);
+        for (int c = 0; c < P.nsplit; ++c) {
+            const float2 v = P.partial[((size_t)d * P.nsplit + c) * lanes + lam * P.nfreq + w];
+            band.x += v.x;
+            band.y += v.y;
+        }
+        const float2 v = cmul(band, phase_of(P.omega[w] * P.band_ref[lam]));
+        sum.x += v.x;
+        sum.y += v.y;
+    }
+    P.out[(size_t)w * P.nrec + d] = sum;
+}
+
+// Workgroups of the wide kernel per OM: enough to fill the chip, at least 16
+// points each.
+int wide_splits(int nrec, int npoints) {
+    const int want = (1024 + nrec - 1) / (nrec > 0 ? nrec : 1);
+    const int most = (npoints + 15) / 16;
+    return std::max(1, std::min(want, most));
 }
 
 // One frequency: one thread per (OM, point); fixed-order tree reduction.
@@ -602,8 +643,14 @@ int launch(Ge2Params P, hipStream_t stream) {
     P.chunk = std::min(64, (kLdsBudget - reduce_words * 4) / (4 * P.record_words));
     LH_REQUIRE(P.chunk >= 1, "lh_ge2_apply: degree too large for the LDS record");
     const size_t lds = (size_t)(reduce_words + P.chunk * P.record_words) * 4;
-    hipLaunchKernelGGL((ge2_wide<A, M>), dim3(P.nrec), dim3(kThreads), lds, stream, P);
-    return lh::check_launch("lh_ge2_apply (wide)");
+    P.nsplit = wide_splits(P.nrec, P.npoints);
+    P.span = (P.npoints + P.nsplit - 1) / P.nsplit;
+    hipLaunchKernelGGL((ge2_wide<A, M>), dim3(P.nrec, P.nsplit), dim3(kThreads), lds, stream, P);
+    int status = lh::check_launch("lh_ge2_apply (wide)");
+    if (status != LH_OK || P.nsplit == 1) return status;
+    const int total = P.nrec * P.nfreq;
+    hipLaunchKernelGGL(ge2_wide_reduce, dim3((total + 255) / 256), dim3(256), 0, stream, P);
+    return lh::check_launch("lh_ge2_apply (wide reduce)");
 }
 
 template <int A>
@@ -622,9 +669,16 @@ int dispatch_m(int m, const Ge2Params& P, hipStream_t s) {
 
 }  // namespace
 
+extern "C" size_t lh_ge2_workspace_bytes(int bands, int freq, int receivers, int points) {
+    if (freq <= 1 || receivers <= 0) return 0;
+    const int nsplit = wide_splits(receivers, points);
+    return nsplit > 1 ? sizeof(float2) * (size_t)receivers * nsplit * bands * freq : 0;
+}
+
 extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source,
                             const lh_ge2_receivers* receivers, const double* alpha, int nfreq,
-                            double radial_low, double radial_high, void* out, void* stream) {
+                            double radial_low, double radial_high, void* out, void* workspace,
+                            size_t workspace_bytes, void* stream) {
     LH_REQUIRE(pack && source && receivers, "lh_ge2_apply: null descriptor");
     LH_REQUIRE(receivers->count >= 0, "lh_ge2_apply: negative receiver count");
     if (receivers->count == 0) return LH_OK;
@@ -675,6 +729,10 @@ extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source
     P.nfreq = nfreq;
     P.r_low = radial_low;
     P.r_high = radial_high;
+    const size_t need = lh_ge2_workspace_bytes(P.bands, nfreq, P.nrec, P.npoints);
+    LH_REQUIRE(workspace_bytes >= need && (need == 0 || workspace != nullptr),
+               "lh_ge2_apply: workspace of %zu bytes, %zu needed", workspace_bytes, need);
+    P.partial = static_cast<float2*>(workspace);
     const int m = std::min(source->azimuthal_degree, pack->degree);
     hipStream_t s = static_cast<hipStream_t>(stream);
     switch (pack->acceptance_degree) {

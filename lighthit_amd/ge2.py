@@ -41,7 +41,41 @@ class _Receivers(ctypes.Structure):
 _lib.SIGNATURES["lh_ge2_apply"] = (ctypes.c_int, [ctypes.POINTER(_Pack), ctypes.POINTER(_Source),
                                                   ctypes.POINTER(_Receivers), ctypes.c_void_p,
                                                   ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                                  ctypes.c_void_p, ctypes.c_void_p])
+                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                  ctypes.c_void_p])
+_lib.SIGNATURES["lh_ge2_workspace_bytes"] = (ctypes.c_size_t, [ctypes.c_int] * 4)
+
+_workspaces = {}
+
+
+def _workspace(device, nbytes):
+    """A reusable device workspace of at least ``nbytes`` (one per device)."""
+    import torch
+    buf = _workspaces.get(str(device))
+    if nbytes and (buf is None or buf.numel() < nbytes):
+        buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        _workspaces[str(device)] = buf
+    return buf
+
+
+_alphas = {}
+
+
+def _device_alpha(alpha, degree, device):
+    """Acceptance coefficients on the device, uploaded once per value."""
+    import torch
+    alpha = np.ascontiguousarray(np.asarray(alpha, float)[:int(degree) + 1])
+    key = (str(device), alpha.tobytes())
+    if key not in _alphas:
+        _alphas[key] = torch.as_tensor(alpha, device=device)
+    return _alphas[key]
+
+
+def workspace_bytes(bands, freq, receivers, points):
+    lib = _lib.load()
+    fn = lib.lh_ge2_workspace_bytes
+    fn.restype, fn.argtypes = _lib.SIGNATURES["lh_ge2_workspace_bytes"]
+    return int(fn(int(bands), int(freq), int(receivers), int(points)))
 
 
 def canonical_pairs(acceptance_degree):
@@ -177,15 +211,17 @@ def apply_device(pack, source, receivers, alpha, window, *, frequencies=None, ou
     if (not np.isfinite(low) or not np.isfinite(high) or low < pack.radial_low - 1e-9
             or high > pack.radial_high + 1e-9 or low >= high):
         raise ValueError("radial window must lie inside the kernel cache")
-    alpha = np.asarray(alpha, float)[:pack.acceptance_degree + 1]
-    d_alpha = torch.as_tensor(np.ascontiguousarray(alpha), device=source.d_points.device)
+    d_alpha = _device_alpha(alpha, pack.acceptance_degree, source.d_points.device)
     if out is None:
         out = torch.empty((nfreq, receivers.count), dtype=torch.complex64,
                           device=source.d_points.device)
     descriptor = pack.struct if band is None else pack.band(int(band))
+    need = workspace_bytes(descriptor.bands, nfreq, receivers.count, source.points)
+    ws = _workspace(source.d_points.device, need) if need else None
     _lib.call("lh_ge2_apply", ctypes.byref(descriptor), ctypes.byref(source.struct),
               ctypes.byref(receivers.struct), _address(d_alpha), nfreq, low, high,
-              _address(out), _lib.stream_handle(stream))
+              _address(out), _address(ws), ctypes.c_size_t(ws.numel() if ws is not None else 0),
+              _lib.stream_handle(stream))
     return out
 
 
