@@ -31,6 +31,7 @@
 // order, so the result does not depend on scheduling.
 #include "lh_common.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -46,6 +47,7 @@ constexpr int kMaxAlpha = 8;
 constexpr double kC = 0.299792458;
 constexpr int kChunk = 512;        // elements per workgroup
 constexpr int kBoundWords = 12;
+constexpr size_t kPersistentGroups = 2048;  // workgroups of the pair kernel
 constexpr double kPi = 3.14159265358979323846;
 
 // 8-point Gauss-Legendre rule on [-1, 1]
@@ -99,6 +101,8 @@ struct O0Params {
     int chunk, nchunk;
     double* partial;          // (nom, nchunk)
     double* bounds;           // (nchunk, kBoundWords) bounding sphere / direction cone
+    int* live;                // (nom * nchunk, 2) live (OM, chunk) items
+    int* live_count;
     // bins only
     const double* start_ns;   // (nel)
     const double* end_ns;     // (nel)
@@ -661,8 +665,35 @@ __device__ bool chunk_culled(const O0Params& P, int chunk, int om) {
     return alpha - slack > theta_max || alpha + slack < theta_min;
 }
 
+// Clears the flag words (status flags, live-item count) of a call.
+__global__ void order0_reset(int* flags) {
+    if (threadIdx.x < 64) flags[threadIdx.x] = 0;
+}
+
+// Zeroes `count` doubles.
+__global__ void order0_zero(double* out, size_t count) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) out[idx] = 0.0;
+}
+
+// One thread per (OM, chunk): items that may hold a contributing pair are
+// appended to the live list; the partial sums of the others are zeroed here.
+__global__ void order0_cull(O0Params P, int width) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)P.nom * P.nchunk) return;
+    const int om = (int)(idx / P.nchunk), chunk = (int)(idx - (size_t)om * P.nchunk);
+    if (chunk_culled(P, chunk, om)) {
+        double* out = P.partial + idx * width;
+        for (int b = 0; b < width; ++b) out[b] = 0.0;
+        return;
+    }
+    const int slot = atomicAdd(P.live_count, 1);
+    P.live[2 * slot] = om;
+    P.live[2 * slot + 1] = chunk;
+}
+
 template <bool kBins>
-__global__ __launch_bounds__(kThreads) void order0_pairs(O0Params P) {
+__global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     extern __shared__ double dynamic_lds[];
     __shared__ PanelTables T;
     __shared__ int list[kChunk];
@@ -674,32 +705,14 @@ __global__ __launch_bounds__(kThreads) void order0_pairs(O0Params P) {
     __shared__ double wave_scut[kWaves][2 * kMaxMajor + 2];
     __shared__ int wave_nscut[kWaves];
     __shared__ int glass;
-    __shared__ int culled;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int chunk = blockIdx.x;
-    const int om = blockIdx.y;
-
-    if (tid == 0) culled = chunk_culled(P, chunk, om) ? 1 : 0;
-    __syncthreads();
-    if (culled) {
-        if (kBins) {
-            double* out = P.partial + ((size_t)om * P.nchunk + chunk) * P.nbins;
-            for (int b = tid; b < P.nbins; b += kThreads) out[b] = 0.0;
-        } else if (tid == 0) {
-            P.partial[(size_t)om * P.nchunk + chunk] = 0.0;
-        }
-        return;
-    }
     // bins: one histogram per wave, then the bin edges (FP32)
     double* wave_bins = dynamic_lds + (size_t)wave * P.nbins;
     float* edges = reinterpret_cast<float*>(dynamic_lds + (size_t)kWaves * P.nbins);
     if (kBins) {
-        for (int b = tid; b < kWaves * P.nbins; b += kThreads) dynamic_lds[b] = 0.0;
-        const double origin = P.origins[om];
         for (int b = tid; b <= P.nbins; b += kThreads) edges[b] = (float)P.bin_edges[b];
-        (void)origin;
     }
 
     // tables of this launch
@@ -719,12 +732,20 @@ __global__ __launch_bounds__(kThreads) void order0_pairs(O0Params P) {
     for (int k = tid; k < P.npanel * 4; k += kThreads)
         T.eff[k / 4][k % 4] = (float)P.eff[(size_t)(k / 4) * P.effcols + (k % 4)];
     if (tid < P.nalpha) T.alpha[tid] = (float)(P.alpha[tid] / (4.0 * kPi));
+    __syncthreads();
+
+    // live (OM, chunk) items, strided over the persistent grid
+    const int live = *P.live_count;
+    for (int item = blockIdx.x; item < live; item += gridDim.x) {
+    const int om = P.live[2 * item];
+    const int chunk = P.live[2 * item + 1];
     if (tid == 0) {
         total_count = 0;
         glass = 0;
     }
+    if (kBins)
+        for (int b = tid; b < kWaves * P.nbins; b += kThreads) dynamic_lds[b] = 0.0;
     __syncthreads();
-
     // candidate pairs of this chunk, compacted in element order
     const int first = chunk * P.chunk;
     const int last = min(P.nel, first + P.chunk);
@@ -860,7 +881,8 @@ __global__ __launch_bounds__(kThreads) void order0_pairs(O0Params P) {
             for (int w = 0; w < kWaves; ++w) sum += dynamic_lds[(size_t)w * P.nbins + b];
             out[b] = sum;
         }
-        return;
+        __syncthreads();
+        continue;
     }
     // fixed-order reductions: lanes, then waves
     for (int offset = 32; offset > 0; offset >>= 1) lane_sum += __shfl_down(lane_sum, offset);
@@ -870,6 +892,8 @@ __global__ __launch_bounds__(kThreads) void order0_pairs(O0Params P) {
         double sum = 0.0;
         for (int w = 0; w < kWaves; ++w) sum += wave_sum[w];
         P.partial[(size_t)om * P.nchunk + chunk] = sum;
+    }
+    __syncthreads();
     }
 }
 
@@ -940,9 +964,10 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     P.chunk = kChunk;
     P.nchunk = (int)chunks_of(elements->count);
     P.flags = static_cast<int*>(workspace);
+    P.live_count = P.flags + 1;
     P.bounds = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
     P.partial = P.bounds + (size_t)P.nchunk * kBoundWords;
-    (void)width;
+    P.live = reinterpret_cast<int*>(P.partial + (size_t)P.nom * P.nchunk * width);
     return P;
 }
 
@@ -951,7 +976,13 @@ int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const cha
     hipLaunchKernelGGL(order0_chunk_bounds, dim3(P.nchunk), dim3(kThreads), 0, s, P);
     int status = lh::check_launch(name);
     if (status != LH_OK) return status;
-    hipLaunchKernelGGL((order0_pairs<kBins>), dim3(P.nchunk, P.nom), dim3(kThreads), lds, s, P);
+    const size_t items = (size_t)P.nom * P.nchunk;
+    hipLaunchKernelGGL(order0_cull, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, P,
+                       width);
+    status = lh::check_launch(name);
+    if (status != LH_OK) return status;
+    const unsigned grid = (unsigned)std::min<size_t>(items, kPersistentGroups);
+    hipLaunchKernelGGL((order0_pairs<kBins>), dim3(grid), dim3(kThreads), lds, s, P);
     status = lh::check_launch(name);
     if (status != LH_OK) return status;
     const size_t total = (size_t)P.nom * width;
@@ -962,16 +993,15 @@ int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const cha
 
 }  // namespace
 
-extern "C" size_t lh_order0_workspace_bytes(int n_oms, int n_elements) {
-    const size_t nchunk = chunks_of(n_elements);
-    return 256 + sizeof(double) * (nchunk * kBoundWords + (size_t)(n_oms > 0 ? n_oms : 0) * nchunk);
-}
-
 extern "C" size_t lh_order0_bins_workspace_bytes(int n_oms, int n_elements, int n_bins) {
     const size_t nchunk = chunks_of(n_elements);
-    return 256 + sizeof(double) * (nchunk * kBoundWords
-                                   + (size_t)(n_oms > 0 ? n_oms : 0) * nchunk
-                                         * (size_t)(n_bins > 0 ? n_bins : 0));
+    const size_t items = (size_t)(n_oms > 0 ? n_oms : 0) * nchunk;
+    return 256 + sizeof(double) * (nchunk * kBoundWords + items * (size_t)(n_bins > 0 ? n_bins : 0))
+           + 2 * sizeof(int) * items;
+}
+
+extern "C" size_t lh_order0_workspace_bytes(int n_oms, int n_elements) {
+    return lh_order0_bins_workspace_bytes(n_oms, n_elements, 1);
 }
 
 extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
@@ -983,11 +1013,13 @@ extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_
                "lh_order0_charge: workspace of %zu bytes, %zu needed", workspace_bytes,
                lh_order0_workspace_bytes(oms->count, elements->count));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    LH_HIP_CALL(hipMemsetAsync(workspace, 0, 256, s));
-    if (oms->count == 0) return LH_OK;
+    hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(workspace));
+    status = lh::check_launch("lh_order0_charge (reset)");
+    if (status != LH_OK || oms->count == 0) return status;
     if (elements->count == 0) {
-        LH_HIP_CALL(hipMemsetAsync(charge, 0, sizeof(double) * oms->count, s));
-        return LH_OK;
+        hipLaunchKernelGGL(order0_zero, dim3((oms->count + 255) / 256), dim3(256), 0, s, charge,
+                           (size_t)oms->count);
+        return lh::check_launch("lh_order0_charge (zero)");
     }
     O0Params P = make_params(optics, oms, elements, workspace, 1);
     return run<false>(P, 1, charge, s, 0, "lh_order0_charge");
@@ -1005,12 +1037,15 @@ extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_om
                "lh_order0_bins: workspace of %zu bytes, %zu needed", workspace_bytes,
                lh_order0_bins_workspace_bytes(oms->count, elements->count, n_bins));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    LH_HIP_CALL(hipMemsetAsync(workspace, 0, 256, s));
-    if (oms->count == 0) return LH_OK;
+    hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(workspace));
+    status = lh::check_launch("lh_order0_bins (reset)");
+    if (status != LH_OK || oms->count == 0) return status;
     LH_REQUIRE(origins && bin_edges && bins, "lh_order0_bins: null device pointer");
     if (elements->count == 0) {
-        LH_HIP_CALL(hipMemsetAsync(bins, 0, sizeof(double) * oms->count * (size_t)n_bins, s));
-        return LH_OK;
+        const size_t count = (size_t)oms->count * n_bins;
+        hipLaunchKernelGGL(order0_zero, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                           bins, count);
+        return lh::check_launch("lh_order0_bins (zero)");
     }
     O0Params P = make_params(optics, oms, elements, workspace, n_bins);
     P.origins = origins;
