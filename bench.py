@@ -119,6 +119,61 @@ def cpu_baseline(workload, bundle, sources, window, ids, alpha, fraction):
             "stage_seconds": {"order0": t_order0, "ge2": t_ge2, "readout": t_readout}}
 
 
+DOMINANT_KERNEL = "order0_pairs"
+
+
+def _counter_kib(db_dir, counter, name_part):
+    """Average KiB per dispatch of ``counter`` for kernels whose name holds ``name_part``."""
+    import glob
+    import sqlite3
+    values = []
+    for db in glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True):
+        cur = sqlite3.connect(db).cursor()
+        for kernel, value in cur.execute(
+                "select kernel_name, value from counters_collection where counter_name=?",
+                (counter,)):
+            if name_part in kernel:
+                values.append(float(value))
+    return sum(values) / len(values) if values else None
+
+
+def measure_traffic(workload_flag):
+    """HBM bytes per launch of the dominant kernel from two counter runs, each
+    scaled by what the same counter reads for a plain copy of known size."""
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    base = tempfile.mkdtemp(prefix="lh_traffic_")
+    me = os.path.abspath(__file__)
+    runs = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for kind, args in (("bench", ["--steps", "2", "--warmup", "1", "--no-cpu-baseline",
+                                      "--no-traffic", "--workload", workload_flag]),
+                           ("copy", ["--copy-kernel-mib", "256"])):
+            out = os.path.join(base, f"{kind}_{counter}")
+            cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--",
+                   sys.executable, me, *args]
+            done = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd="/tmp",
+                                  env=dict(os.environ, TMPDIR="/tmp"))
+            if done.returncode != 0:
+                return None, f"rocprofv3 {counter} {kind} failed ({done.returncode})"
+            runs[(kind, counter)] = out
+    result = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        copy = _counter_kib(runs[("copy", counter)], counter, "copyBuffer")
+        kernel = _counter_kib(runs[("bench", counter)], counter, DOMINANT_KERNEL)
+        if not copy or kernel is None:
+            return None, f"no {counter} samples"
+        scale = 256 * 1024 / copy    # the copy moves 256 MiB each way
+        result[counter] = (kernel * scale * 1024, scale)
+    traffic = result["FETCH_SIZE"][0] + result["WRITE_SIZE"][0]
+    note = (f"{DOMINANT_KERNEL}: FETCH_SIZE x{result['FETCH_SIZE'][1]:.3f} + "
+            f"WRITE_SIZE x{result['WRITE_SIZE'][1]:.3f} (copy-calibrated), bytes per launch")
+    return traffic, note
+
+
 def main():
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -127,6 +182,8 @@ def main():
     parser.add_argument("--workload", choices=["detector", "small"], default="detector")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-fraction", type=float, default=0.1)
+    parser.add_argument("--no-traffic", action="store_true",
+                        help="skip the rocprofv3 counter runs that measure HBM traffic")
     parser.add_argument("--copy-kernel-mib", type=int, default=0,
                         help="only run a plain device copy of this size (HBM counter calibration)")
     args = parser.parse_args()
@@ -213,9 +270,13 @@ def main():
     n_el, n_om = len(workload.elements), len(d)
     algorithmic_bytes = 48 * n_el + 36 * n_om
     achieved = algorithmic_bytes / (stage_ms["order0"] * 1e-3) / 1e9
+    traffic, traffic_note = (None, "not measured (--no-traffic)")
+    if not args.no_traffic:
+        traffic, traffic_note = measure_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "order0_charge (+ order0_reduce)",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "order-0 charge (reset, chunk bounds, cull, order0_pairs, reduce)",
+                "traffic_note": traffic_note,
                 "note": "order 0 is VALU-bound (transcendental-heavy FP32 quadrature); "
                         "the HBM fraction is reported as required and is small by design"}
     baseline = None

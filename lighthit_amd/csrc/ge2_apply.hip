@@ -31,6 +31,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxBands = 9;
+constexpr int kGroup = 8;     // points per degree-outer group in the wide kernel
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 struct Ge2Params {
@@ -311,7 +312,7 @@ struct Record {
 };
 
 template <int A, int M>
-__global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
+__global__ __launch_bounds__(kThreads, 2) void ge2_wide(Ge2Params P) {
     using R = Record<A, M>;
     constexpr int NA = R::NA, NM = R::NM, NPAIR = R::NPAIR;
     extern __shared__ float lds[];
@@ -373,82 +374,107 @@ __global__ __launch_bounds__(kThreads) void ge2_wide(Ge2Params P) {
             const double inv_speed_w = P.flight ? P.omega[w] / P.speed[lam] : 0.0;
             const size_t cl = (size_t)NPAIR * 4 * P.cstride;  // per degree
             float2 total = acc[lw];
-            for (int i = 0; i < count; ++i) {
-                const float* rec = smem + (size_t)i * P.record_words;
-                const int* irec = reinterpret_cast<const int*>(rec);
-                if (irec[0] == 0) continue;
-                const int j = irec[1];
-                const float t = rec[2];
-                const double radius = reinterpret_cast<const double*>(rec)[2];
-                const float* G = rec + 6;
-                const float scale = rec[6 + 2 * NPAIR + lam];
-                const float* E = rec + 6 + 2 * NPAIR + P.bands;
-                const float2* cbase = P.coef + ((size_t)lam * P.intervals + j) * (P.degree + 1) * cl + w;
-                const float2* chbase = P.channels + ((size_t)lam * P.npoints + base + i) * P.nchan * P.chstride + w;
-                float2 resp = make_float2(0.f, 0.f);
+            // Points in groups of kGroup with degree outermost: the pack
+            // coefficients of one degree are loaded once per run of points
+            // in the same radial interval and reused across the group.
+            for (int g0 = 0; g0 < count; g0 += kGroup) {
+                float2 accp[kGroup];
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q) accp[q] = make_float2(0.f, 0.f);
                 int starts = 0;
                 for (int l = 0; l <= P.degree; ++l) {
                     const int top_p = l < A ? l : A;
                     const int keep = l < M ? l : M;
                     const int centre = starts + keep;
                     starts += 2 * (l < P.src_az ? l : P.src_az) + 1;
-                    float2 chv[NM];
+                    int jc = -1;
+                    float2 cf[NPAIR][4];
 #pragma unroll
-                    for (int b = 0; b < NM; ++b) {
-                        const int m = b - M;
-                        chv[b] = (m >= -keep && m <= keep)
-                                     ? chbase[(size_t)(centre + m) * P.chstride]
-                                     : make_float2(0.f, 0.f);
-                    }
-                    float2 cs[NA];
+                    for (int q = 0; q < kGroup; ++q) {
+                        const int i = g0 + q;
+                        if (i >= count) break;
+                        const float* rec = smem + (size_t)i * P.record_words;
+                        const int* irec = reinterpret_cast<const int*>(rec);
+                        if (irec[0] == 0) continue;
+                        const int j = irec[1];
+                        if (j != jc) {
+                            const float2* cdeg = P.coef
+                                + (((size_t)lam * P.intervals + j) * (P.degree + 1) + l) * cl + w;
 #pragma unroll
-                    for (int a = 0; a < NA; ++a) cs[a] = make_float2(0.f, 0.f);
-                    const float2* cdeg = cbase + (size_t)l * cl;
-                    int k = 0;
+                            for (int k = 0; k < NPAIR; ++k)
 #pragma unroll
-                    for (int dl = 0; dl <= A; ++dl)
+                                for (int pw = 0; pw < 4; ++pw)
+                                    cf[k][pw] = cdeg[(size_t)(k * 4 + pw) * P.cstride];
+                            jc = j;
+                        }
+                        const float t = rec[2];
+                        const float* G = rec + 6;
+                        const float* El = rec + 6 + 2 * NPAIR + P.bands + l * NA * NM;
+                        const float2* chbase = P.channels
+                            + ((size_t)lam * P.npoints + base + i) * P.nchan * P.chstride + w;
+                        float2 chv[NM];
 #pragma unroll
-                        for (int mu = 0; mu <= dl; ++mu, ++k) {
-                            if (mu > top_p) continue;
-                            const float2* c = cdeg + (size_t)k * 4 * P.cstride;
-                            const float2 c0 = c[0], c1 = c[P.cstride], c2 = c[2 * P.cstride],
-                                         c3 = c[3 * P.cstride];
-                            float2 radial;
-                            radial.x = ((c0.x * t + c1.x) * t + c2.x) * t + c3.x;
-                            radial.y = ((c0.y * t + c1.y) * t + c2.y) * t + c3.y;
-                            const float pos = G[2 * k], neg = G[2 * k + 1];
-                            cs[A + mu].x += pos * radial.x;
-                            cs[A + mu].y += pos * radial.y;
-                            if (mu) {
-                                cs[A - mu].x += neg * radial.x;
-                                cs[A - mu].y += neg * radial.y;
+                        for (int bb = 0; bb < NM; ++bb) {
+                            const int m = bb - M;
+                            chv[bb] = (m >= -keep && m <= keep)
+                                          ? chbase[(size_t)(centre + m) * P.chstride]
+                                          : make_float2(0.f, 0.f);
+                        }
+                        float2 cs[NA];
+#pragma unroll
+                        for (int aa = 0; aa < NA; ++aa) cs[aa] = make_float2(0.f, 0.f);
+                        int k = 0;
+#pragma unroll
+                        for (int dl = 0; dl <= A; ++dl)
+#pragma unroll
+                            for (int mu = 0; mu <= dl; ++mu, ++k) {
+                                if (mu > top_p) continue;
+                                float2 radial;
+                                radial.x = ((cf[k][0].x * t + cf[k][1].x) * t + cf[k][2].x) * t + cf[k][3].x;
+                                radial.y = ((cf[k][0].y * t + cf[k][1].y) * t + cf[k][2].y) * t + cf[k][3].y;
+                                const float pos = G[2 * k], neg = G[2 * k + 1];
+                                cs[A + mu].x += pos * radial.x;
+                                cs[A + mu].y += pos * radial.y;
+                                if (mu) {
+                                    cs[A - mu].x += neg * radial.x;
+                                    cs[A - mu].y += neg * radial.y;
+                                }
                             }
-                        }
-                    const float* El = E + l * NA * NM;
 #pragma unroll
-                    for (int a = 0; a < NA; ++a) {
-                        const int nu = a - A;
-                        if (nu < -top_p || nu > top_p) continue;
-                        float2 r = make_float2(0.f, 0.f);
+                        for (int aa = 0; aa < NA; ++aa) {
+                            const int nu = aa - A;
+                            if (nu < -top_p || nu > top_p) continue;
+                            float2 r = make_float2(0.f, 0.f);
 #pragma unroll
-                        for (int b = 0; b < NM; ++b) {
-                            const float e = El[a * NM + b];
-                            r.x += e * chv[b].x;
-                            r.y += e * chv[b].y;
+                            for (int bb = 0; bb < NM; ++bb) {
+                                const float e = El[aa * NM + bb];
+                                r.x += e * chv[bb].x;
+                                r.y += e * chv[bb].y;
+                            }
+                            const float2 v = cmul(r, cs[aa]);
+                            accp[q].x += v.x;
+                            accp[q].y += v.y;
                         }
-                        const float2 v = cmul(r, cs[a]);
-                        resp.x += v.x;
-                        resp.y += v.y;
                     }
                 }
-                float2 sp = make_float2(scale, 0.f);
-                if (P.flight) {
-                    const float2 ph = phase_of(inv_speed_w * radius);
-                    sp = make_float2(scale * ph.x, scale * ph.y);
+                // attenuation and flight phase of every point of the group
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q) {
+                    const int i = g0 + q;
+                    if (i >= count) break;
+                    const float* rec = smem + (size_t)i * P.record_words;
+                    if (reinterpret_cast<const int*>(rec)[0] == 0) continue;
+                    const double radius = reinterpret_cast<const double*>(rec)[2];
+                    const float scale = rec[6 + 2 * NPAIR + lam];
+                    float2 sp = make_float2(scale, 0.f);
+                    if (P.flight) {
+                        const float2 ph = phase_of(inv_speed_w * radius);
+                        sp = make_float2(scale * ph.x, scale * ph.y);
+                    }
+                    const float2 v = cmul(accp[q], sp);
+                    total.x += v.x;
+                    total.y += v.y;
                 }
-                const float2 v = cmul(resp, sp);
-                total.x += v.x;
-                total.y += v.y;
             }
             acc[lw] = total;
         }
