@@ -192,6 +192,62 @@ int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
                    const double* bin_edges, int n_bins, double* bins, void* workspace,
                    size_t workspace_bytes, void* stream);
 
+/*
+ * Order >= 2 source: the axial-slice compile of one light field.
+ *
+ * Replaces compile_axial_slices (_shower_basis.py:188-645) as the shower route
+ * calls it per node band (shower/ge2.py:49-55): chord quadrature nodes, cubic
+ * (or linear) Lagrange weights on axis slices, a linear (or nearest) time
+ * lattice, a lattice row per occupied (slice, tick), thin/wide slice decision,
+ * deposit of weight * moment * P_l(cone) Y_lm(u) per row, the phase sum
+ * e^{i omega t} per slice and the five-point transverse stencil. Two calls:
+ * lh_compile_layout builds the lattice and the point layout and writes
+ * info = {slices, wide slices, points, rows, error flags} (device ints);
+ * after reading info the caller sizes the deposit buffer (rows * moments *
+ * channels doubles) and the channels, and calls lh_compile_channels with the
+ * same workspace. Element arrays are shared by every band; a band is
+ * selected by its photon scales and phase index (photons = scale0 * C0 -
+ * scale2 * (-C2), cone = 1 / (beta n), elements with beta n <= 1 or mask 0
+ * skipped), or photons/cone_cosine are given directly.
+ */
+typedef struct lh_compile_field {
+    const double* start_m;       /* (count, 3) */
+    const double* direction;     /* (count, 3) unit */
+    const double* length_m;      /* (count) */
+    const double* start_ns;      /* (count) */
+    const double* end_ns;        /* (count) */
+    const double* coefficient0;  /* (count) or NULL when photons is given */
+    const double* coefficient2;  /* (count) or NULL when photons is given */
+    const double* beta;          /* (count) or NULL when photons is given */
+    const double* photons;       /* (count) direct photon weights, or NULL */
+    const double* cone_cosine;   /* (count) direct cone cosines, or NULL */
+    const unsigned char* mask;   /* (count) elements taken, or NULL for all */
+    double scale0, scale2, phase_index;
+    int count;
+} lh_compile_field;
+
+typedef struct lh_compile_lattice {
+    double centre[3], axis[3], first[3], second[3];   /* shower frame */
+    double z_origin_m, reference_ns, cell_m, time_step_ns, stencil_m, stencil_threshold_m;
+    int degree, azimuthal_degree;
+    int axial_order;      /* 1 or 3 */
+    int moments;          /* 1 (collapse) or 5 (stencil) */
+    int element_order;    /* chord Gauss nodes, 1 or 2 */
+    int time_deposit;     /* 0 nearest, 1 linear, 2 charge only (one tick) */
+    int kmax;             /* ticks per slice in the row key */
+    int slice0;           /* lowest slice index of the lattice */
+    int positions;        /* slice positions of the lattice */
+} lh_compile_lattice;
+
+size_t lh_compile_workspace_bytes(const lh_compile_lattice* lattice, int count);
+int lh_compile_layout(const lh_compile_field* field, const lh_compile_lattice* lattice,
+                      int* info, double* points, void* workspace, size_t workspace_bytes,
+                      void* stream);
+int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice* lattice,
+                        const double* omega, int n_freq, double scale, int rows,
+                        double* deposit, void* channels, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,19 @@ class AxisFrame:
         first /= np.linalg.norm(first)
         return cls(centre, axis, first, np.cross(axis, first))
 
+    def coordinates(self, elements, fraction=0.5):
+        """``z`` along the axis, transverse offset, direction cosine and time of
+        the point at ``fraction`` of every chord (``axial_source.py:126-141``)."""
+        points = (elements.start_m
+                  + (fraction * np.asarray(elements.length_m, float))[:, None] * elements.direction)
+        offset = points - self.centre_m
+        z = offset @ self.axis
+        transverse = offset - z[:, None] * self.axis
+        cosine = elements.direction @ self.axis
+        start = np.asarray(elements.start_ns, float)
+        time = start + fraction * (np.asarray(elements.end_ns, float) - start)
+        return z, transverse, cosine, time
+
     def matrix(self):
         """Rows (first, second, axis): global -> frame coordinates."""
         return np.stack((self.first, self.second, self.axis))

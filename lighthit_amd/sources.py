@@ -84,6 +84,27 @@ class SourcePose:
 
 
 @dataclass(frozen=True)
+class LightField:
+    """Chords with one photon weight and one cone cosine each (the fields of
+    ``LightElements`` that the axial-slice compile reads, ``sources.py:193-205``)."""
+    start_m: np.ndarray
+    direction: np.ndarray
+    length_m: np.ndarray
+    photons: np.ndarray
+    cone_cosine: np.ndarray
+    start_ns: np.ndarray
+    end_ns: np.ndarray
+    row_index: np.ndarray
+
+    def __len__(self):
+        return len(self.length_m)
+
+    @property
+    def midpoints_m(self):
+        return self.start_m + 0.5 * self.length_m[:, None] * self.direction
+
+
+@dataclass(frozen=True)
 class SpectralLightElements:
     """Straight emitting chords with ``dN/dlambda = lambda^-2 (C0 + C2 / n^2)``."""
     start_m: np.ndarray
@@ -136,6 +157,18 @@ class SpectralLightElements:
     def centroid_m(self):
         weight = self.coefficient0 / self.coefficient0.sum()
         return np.sum(weight[:, None] * self.midpoints_m, axis=0)
+
+    def field(self, index, *, phase_index=None):
+        """One Cherenkov field of the spectrum as a :class:`LightField`
+        (``sources.py:193-205``): index 0 carries ``C0``, index 2 ``-C2``;
+        the cone cosine is ``1 / (beta n)`` at ``phase_index``."""
+        if index not in (0, 2):
+            raise ValueError("field index must be 0 or 2")
+        phase = self.reference_phase_index if phase_index is None else float(phase_index)
+        photons = self.coefficient0 if index == 0 else -self.coefficient2
+        return LightField(self.start_m, self.direction, self.length_m,
+                          np.asarray(photons, float), 1.0 / (self.beta * phase),
+                          self.start_ns, self.end_ns, self.row_index)
 
     def subset(self, mask):
         mask = np.asarray(mask, bool)
