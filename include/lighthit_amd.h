@@ -248,6 +248,110 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
                         double* deposit, void* channels, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/*
+ * Order 1 (single scattering): grouped pencils.
+ *
+ * Replaces the grouped branch of _order1 (prompt.py:1020-1258):
+ * compress (prompt.py:1317-1402 with compress_subsegments and finish_pencils,
+ * _prompt_numba.py:1094-1230, 2247-2354), compress_screen + pack_pencils
+ * (_prompt_screen.py:44-93), grouped_pencils_order1
+ * (_prompt_numba.py:1750-2065) and screen_order1 (_prompt_screen.py:170-477).
+ * Compression is two calls: lh_order1_compress_keys writes the number of
+ * pencil keys to info[0]; lh_order1_compress_pencils writes that many pencils
+ * (screen grids, parent_face > 0: parents and children in preorder, at most
+ * twice as many, count in info[1]). The grouped pass is two calls:
+ * lh_order1_group_keys writes the number of (tau, r, theta) groups to info[0];
+ * lh_order1_group_charge takes a values buffer of groups * (2 n_mono + 6)
+ * doubles and writes charge (and bins) of the listed OMs. Each pair shares its
+ * workspace. lh_order1_set_tables uploads the 24-term Taylor table of
+ * I_k(z) (3 x 24) and the 48-node inverse-angle rule (tangent, weight sec^2).
+ */
+typedef struct lh_order1_chords {
+    const double* start_m;       /* (elements, 3) */
+    const double* direction;     /* (elements, 3) unit */
+    const double* length_m;
+    const double* coefficient0;
+    const double* coefficient2;
+    const double* cone_cosine;   /* (elements) cone of the unit */
+    const double* start_ns;
+    const double* end_ns;
+    const int* index;            /* (count) element of every chord of the unit, or NULL */
+    const long long* sub_offset; /* (count + 1) first sub-segment of every chord */
+    int count;
+} lh_order1_chords;
+
+typedef struct lh_order1_grid {
+    double origin[3];
+    long long dims[3];
+    double cell_m;
+    int pixel_face, ring_nodes, parent_face;   /* parent_face 0: no screen hierarchy */
+    long long subsegments;
+} lh_order1_grid;
+
+typedef struct lh_order1_pencils {
+    double* position;    /* (count, 3) */
+    double* direction;   /* (count, 3) */
+    double* w0;
+    double* w2;
+    double* time;
+    double* major;       /* (count, 3) */
+    double* var_major;
+    double* var_minor;
+    double* pos_cov;     /* (count, 6) xx yy zz xy xz yz */
+    long long* parent;   /* screen: -2 for parents, else the parent's index; or NULL */
+    long long* next;     /* screen: next pencil after a parent's subtree; or NULL */
+    int count;
+} lh_order1_pencils;
+
+typedef struct lh_order1_oms {
+    const double* positions;   /* (N, 3) */
+    const double* looks;       /* (N, 3) */
+    const double* areas;       /* (N) */
+    const double* origins;     /* (N) time origins */
+    const int* work;           /* (work_count) OMs evaluated */
+    int work_count;
+} lh_order1_oms;
+
+typedef struct lh_order1_optics {
+    const double* mus;
+    const double* mut;
+    const double* speed;
+    const double* s0w;
+    const double* s2w;           /* (bands) */
+    const double* edges;         /* (bins + 1) relative time edges */
+    const double* eedge;         /* (bands, bins + 1) e^{kappa edge} */
+    const double* itab;          /* (bands, bins, 3) I_k(kappa width) */
+    const double* coef;          /* (n_mono) acceptance monomials */
+    const int* ipow;
+    const int* jpow;
+    int bands, bins, n_mono, max_power, direct_bins, max_path_nodes;
+    double g, y_g, path_step, path_step_tail, tail_exponent, smoothing, mut_min, speed_max;
+} lh_order1_optics;
+
+typedef struct lh_order1_merge {
+    double theta_lo, theta_ratio, tau_step, log_r_step, speed_ref;
+} lh_order1_merge;
+
+int lh_order1_set_tables(const double* taylor, const double* tangent,
+                         const double* weighted_sec2);
+size_t lh_order1_compress_workspace_bytes(const lh_order1_chords* chords,
+                                          const lh_order1_grid* grid);
+int lh_order1_compress_keys(const lh_order1_chords* chords, const lh_order1_grid* grid, int* info,
+                            void* workspace, size_t workspace_bytes, void* stream);
+int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_grid* grid,
+                               int keys, lh_order1_pencils* out, int* info, void* workspace,
+                               size_t workspace_bytes, void* stream);
+size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
+                                         const lh_order1_oms* oms,
+                                         const lh_order1_optics* optics);
+int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
+                         const lh_order1_optics* optics, const lh_order1_merge* merge, int* info,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
+                           const lh_order1_optics* optics, const lh_order1_merge* merge,
+                           int groups, double* values, int compute_bins, double* charge,
+                           double* bins, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1590 @@
+// Order 1 (single scattering): grouped pencils on gfx950.
+//
+// Restates the reference's grouped order-1 solver (prompt.py:1020-1258,
+// _prompt_numba.py:257-650, 1014-2065, 2247-2354; coarse screen
+// _prompt_screen.py:96-477). GPU formulation:
+//   compress   one thread per chord sub-segment generates its ring nodes with
+//              the reference's azimuth recurrence; nodes are radix-sorted by
+//              (cell, pixel) key (stable, so each pencil's nodes stay in
+//              enumeration order) and one thread per pencil sums its nodes in
+//              that order and finishes the pencil. The screen's parent pencils
+//              sum their children's rows in (parent key, child key) order and
+//              are laid out in preorder with their children.
+//   grouped    one thread per (OM, pencil) computes the pencil's (tau, r,
+//              theta) group key (two for the screen's two-node rule); a
+//              segmented radix sort per OM orders them by (key, pencil), which
+//              is the order in which the reference's hash rows receive their
+//              additions; one thread per group sums its members in that order;
+//              one workgroup per OM then walks its groups in key order, builds
+//              each (tau, r) batch's path grid and profile in parallel over
+//              path nodes, and deposits every wavelength's panels with one
+//              thread per wavelength (each accumulator has a single writer and
+//              a fixed order: deterministic, no atomics).
+// All arithmetic is FP64.
+#include "lh_common.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kNmom = 23;
+constexpr int kMaxMono = 15;        // acceptance degree <= 4
+constexpr int kMaxBands = 16;
+constexpr int kGauss = 48;
+constexpr int kMaxPathNodes = 2048;
+constexpr long long kSentinel = (1LL << 62) - 1;
+
+__constant__ double kTaylor[3][24];
+__constant__ double kTangent[kGauss];
+__constant__ double kWeightedSec2[kGauss];
+
+// ------------------------------------------------------------------ compress
+
+__device__ inline long long pixel_of(double sx, double sy, double sz, int n_face) {
+    double ax = fabs(sx), ay = fabs(sy), az = fabs(sz), u, v;
+    int face;
+    if (ax >= ay && ax >= az) { face = sx > 0 ? 0 : 1; u = sy / ax; v = sz / ax; }
+    else if (ay >= az) { face = sy > 0 ? 2 : 3; u = sx / ay; v = sz / ay; }
+    else { face = sz > 0 ? 4 : 5; u = sx / az; v = sy / az; }
+    double quarter = 0.25 * M_PI;
+    int i = (int)((atan(u) + quarter) / (2.0 * quarter) * n_face);
+    int j = (int)((atan(v) + quarter) / (2.0 * quarter) * n_face);
+    i = min(max(i, 0), n_face - 1);
+    j = min(max(j, 0), n_face - 1);
+    return ((long long)face * n_face + i) * n_face + j;
+}
+
+__device__ inline void ring_basis(const double* u, double* e1, double* e2) {
+    double h0 = 0, h1 = 0, h2 = 1;
+    if (!(fabs(u[2]) < 0.9)) { h0 = 1; h2 = 0; }
+    e1[0] = u[1] * h2 - u[2] * h1;
+    e1[1] = u[2] * h0 - u[0] * h2;
+    e1[2] = u[0] * h1 - u[1] * h0;
+    double norm = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    e1[0] /= norm; e1[1] /= norm; e1[2] /= norm;
+    e2[0] = u[1] * e1[2] - u[2] * e1[1];
+    e2[1] = u[2] * e1[0] - u[0] * e1[2];
+    e2[2] = u[0] * e1[1] - u[1] * e1[0];
+}
+
+struct Compress {
+    lh_order1_chords ch;
+    lh_order1_grid grid;
+    long long nodes;        // subsegments * ring_nodes
+    long long* keys;        // (nodes)
+    int* ids;               // (nodes)
+    long long* sorted_keys;
+    int* sorted_ids;
+    double* cp;             // (nodes) ring azimuth cosine
+    double* sp;             // (nodes) ring azimuth sine
+    int* sub_element;       // (subsegments) unit element index of a sub-segment
+    int* sub_index;         // (subsegments) index of the sub-segment in its element
+    int* run_flag;          // (nodes + 1)
+    int* run_scan;          // (nodes + 1)
+    int* run_start;         // (runs + 1)
+    long long* run_key;     // (runs)
+    double* rows;           // (runs, kNmom)
+    int* info;
+};
+
+__device__ inline int n_sub(double length, double a_step) {
+    int n = (int)ceil(length / a_step);
+    return n > 1 ? n : 1;
+}
+
+__device__ inline void sub_point(const lh_order1_chords& c, int e, int ia, int n_a, double* p,
+                                 double* t) {
+    double frac = (ia + 0.5) / n_a;
+    for (int k = 0; k < 3; ++k) p[k] = c.start_m[3 * e + k] + frac * c.length_m[e] * c.direction[3 * e + k];
+    *t = c.start_ns[e] + frac * (c.end_ns[e] - c.start_ns[e]);
+}
+
+__global__ void __launch_bounds__(kThreads) o1_subsegments(Compress k) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k.ch.count) return;
+    int e = k.ch.index ? k.ch.index[i] : i;
+    double a_step = 0.5 * k.grid.cell_m;
+    int n_a = n_sub(k.ch.length_m[e], a_step);
+    long long first = k.ch.sub_offset[i];
+    for (int ia = 0; ia < n_a; ++ia) {
+        k.sub_element[first + ia] = i;
+        k.sub_index[first + ia] = ia;
+    }
+}
+
+// Ring nodes of one sub-segment (compress_subsegments, _prompt_numba.py:1122-1190).
+__global__ void __launch_bounds__(kThreads) o1_nodes(Compress k) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= k.grid.subsegments) return;
+    int i = k.sub_element[s], ia = k.sub_index[s];
+    int e = k.ch.index ? k.ch.index[i] : i;
+    const lh_order1_chords& c = k.ch;
+    int R = k.grid.ring_nodes, face = k.grid.pixel_face;
+    double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
+    double e1[3], e2[3];
+    ring_basis(u, e1, e2);
+    double cc = c.cone_cosine[e];
+    double cs = sqrt(fmax(1.0 - cc * cc, 0.0));
+    double a_step = 0.5 * k.grid.cell_m;
+    int n_a = n_sub(c.length_m[e], a_step);
+    const double golden = 0.6180339887498949;
+    double offset = fmod(i * golden, 1.0) * (2.0 * M_PI) / R;
+    double p[3], t;
+    sub_point(c, e, ia, n_a, p, &t);
+    long long cell[3];
+    for (int d = 0; d < 3; ++d) {
+        long long v = (long long)floor((p[d] - k.grid.origin[d]) / k.grid.cell_m);
+        cell[d] = min(max(v, 0LL), k.grid.dims[d] - 1);
+    }
+    long long ck = (cell[0] * k.grid.dims[1] + cell[1]) * k.grid.dims[2] + cell[2];
+    long long n_pix = 6LL * face * face;
+    double step_c = cos(2.0 * M_PI / R), step_s = sin(2.0 * M_PI / R);
+    double cp = cos(offset), sp = sin(offset);
+    long long base = s * R;
+    for (int q = 0; q < R; ++q) {
+        if (q > 0) {
+            double c2 = cp * step_c - sp * step_s;
+            double s2 = sp * step_c + cp * step_s;
+            cp = c2;
+            sp = s2;
+        }
+        double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
+        double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
+        double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
+        k.keys[base + q] = ck * n_pix + pixel_of(sx, sy, sz, face);
+        k.ids[base + q] = (int)(base + q);
+        k.cp[base + q] = cp;
+        k.sp[base + q] = sp;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) o1_run_flags(Compress k) {
+    long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > k.nodes) return;
+    if (j == k.nodes) { k.run_flag[j] = 0; return; }
+    k.run_flag[j] = (j == 0 || k.sorted_keys[j] != k.sorted_keys[j - 1]) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_run_starts(Compress k) {
+    long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > k.nodes) return;
+    if (j == k.nodes) {
+        int runs = k.run_scan[j];
+        k.run_start[runs] = (int)k.nodes;
+        k.info[0] = runs;
+        return;
+    }
+    if (k.run_flag[j]) {
+        int r = k.run_scan[j];
+        k.run_start[r] = (int)j;
+        k.run_key[r] = k.sorted_keys[j];
+    }
+}
+
+// Moment row of one (cell, pixel) key: its nodes in enumeration order (:1192-1223).
+__global__ void __launch_bounds__(kThreads) o1_rows(Compress k, int runs) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    const lh_order1_chords& c = k.ch;
+    int R = k.grid.ring_nodes;
+    double a_step = 0.5 * k.grid.cell_m;
+    double row[kNmom];
+    for (int m = 0; m < kNmom; ++m) row[m] = 0.0;
+    for (int j = k.run_start[r]; j < k.run_start[r + 1]; ++j) {
+        int node = k.sorted_ids[j];
+        long long s = node / R;
+        int i = k.sub_element[s], ia = k.sub_index[s];
+        int e = c.index ? c.index[i] : i;
+        double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
+        double e1[3], e2[3];
+        ring_basis(u, e1, e2);
+        double cc = c.cone_cosine[e];
+        double cs = sqrt(fmax(1.0 - cc * cc, 0.0));
+        int n_a = n_sub(c.length_m[e], a_step);
+        double share = 1.0 / ((double)n_a * R);
+        double arc = 2.0 * M_PI * cs / R;
+        double arc2 = arc * arc / 12.0;
+        double sl = c.length_m[e] / n_a;
+        double sub2 = sl * sl / 12.0;
+        double w0 = c.coefficient0[e] * share, w2 = c.coefficient2[e] * share;
+        double p[3], t;
+        sub_point(c, e, ia, n_a, p, &t);
+        double cp = k.cp[node], sp = k.sp[node];
+        double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
+        double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
+        double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
+        double px = p[0], py = p[1], pz = p[2];
+        row[0] += w0;
+        row[1] += w2;
+        row[2] += w0 * px;
+        row[3] += w0 * py;
+        row[4] += w0 * pz;
+        row[5] += w0 * t;
+        row[6] += w0 * sx;
+        row[7] += w0 * sy;
+        row[8] += w0 * sz;
+        row[9] += w0 * (px * px + py * py + pz * pz + sub2);
+        row[10] += w0 * t * t;
+        double tx = -sp * e1[0] + cp * e2[0];
+        double ty = -sp * e1[1] + cp * e2[1];
+        double tz = -sp * e1[2] + cp * e2[2];
+        row[11] += w0 * (sx * sx + arc2 * tx * tx);
+        row[12] += w0 * (sy * sy + arc2 * ty * ty);
+        row[13] += w0 * (sz * sz + arc2 * tz * tz);
+        row[14] += w0 * (sx * sy + arc2 * tx * ty);
+        row[15] += w0 * (sx * sz + arc2 * tx * tz);
+        row[16] += w0 * (sy * sz + arc2 * ty * tz);
+        row[17] += w0 * (px * px + sub2 * u[0] * u[0]);
+        row[18] += w0 * (py * py + sub2 * u[1] * u[1]);
+        row[19] += w0 * (pz * pz + sub2 * u[2] * u[2]);
+        row[20] += w0 * (px * py + sub2 * u[0] * u[1]);
+        row[21] += w0 * (px * pz + sub2 * u[0] * u[2]);
+        row[22] += w0 * (py * pz + sub2 * u[1] * u[2]);
+    }
+    for (int m = 0; m < kNmom; ++m) k.rows[(size_t)r * kNmom + m] = row[m];
+}
+
+// Pencil from a moment row (finish_pencils, _prompt_numba.py:2282-2353).
+__device__ void finish_row(const double* a, const lh_order1_pencils& out, int q) {
+    double w = a[0];
+    double mx = a[2] / w, my = a[3] / w, mz = a[4] / w;
+    out.position[3 * q] = mx; out.position[3 * q + 1] = my; out.position[3 * q + 2] = mz;
+    out.time[q] = a[5] / w;
+    out.w0[q] = w;
+    out.w2[q] = a[1];
+    double rx = a[6] / w, ry = a[7] / w, rz = a[8] / w;
+    double norm = sqrt(rx * rx + ry * ry + rz * rz);
+    double d[3] = {rx / norm, ry / norm, rz / norm};
+    out.direction[3 * q] = d[0]; out.direction[3 * q + 1] = d[1]; out.direction[3 * q + 2] = d[2];
+    double cxx = a[11] / w - rx * rx, cyy = a[12] / w - ry * ry, czz = a[13] / w - rz * rz;
+    double cxy = a[14] / w - rx * ry, cxz = a[15] / w - rx * rz, cyz = a[16] / w - ry * rz;
+    double e1[3], e2[3];
+    ring_basis(d, e1, e2);
+    double c11 = (e1[0] * (cxx * e1[0] + cxy * e1[1] + cxz * e1[2])
+                  + e1[1] * (cxy * e1[0] + cyy * e1[1] + cyz * e1[2])
+                  + e1[2] * (cxz * e1[0] + cyz * e1[1] + czz * e1[2]));
+    double c22 = (e2[0] * (cxx * e2[0] + cxy * e2[1] + cxz * e2[2])
+                  + e2[1] * (cxy * e2[0] + cyy * e2[1] + cyz * e2[2])
+                  + e2[2] * (cxz * e2[0] + cyz * e2[1] + czz * e2[2]));
+    double c12 = (e1[0] * (cxx * e2[0] + cxy * e2[1] + cxz * e2[2])
+                  + e1[1] * (cxy * e2[0] + cyy * e2[1] + cyz * e2[2])
+                  + e1[2] * (cxz * e2[0] + cyz * e2[1] + czz * e2[2]));
+    double half = 0.5 * (c11 + c22);
+    double disc = sqrt(fmax(0.25 * (c11 - c22) * (c11 - c22) + c12 * c12, 0.0));
+    out.var_major[q] = fmax(half + disc, 0.0);
+    out.var_minor[q] = fmax(half - disc, 0.0);
+    double angle = 0.5 * atan2(2.0 * c12, c11 - c22);
+    double ca = cos(angle), sa = sin(angle);
+    for (int k = 0; k < 3; ++k) out.major[3 * q + k] = ca * e1[k] + sa * e2[k];
+    out.pos_cov[6 * q] = fmax(a[17] / w - mx * mx, 0.0);
+    out.pos_cov[6 * q + 1] = fmax(a[18] / w - my * my, 0.0);
+    out.pos_cov[6 * q + 2] = fmax(a[19] / w - mz * mz, 0.0);
+    out.pos_cov[6 * q + 3] = a[20] / w - mx * my;
+    out.pos_cov[6 * q + 4] = a[21] / w - mx * mz;
+    out.pos_cov[6 * q + 5] = a[22] / w - my * mz;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_finish(const double* rows, int runs,
+                                                      lh_order1_pencils out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    finish_row(rows + (size_t)r * kNmom, out, r);
+}
+
+// Screen hierarchy: parent key of a fine (cell, pixel) key (_prompt_screen.py:44-53).
+__device__ inline long long parent_key(long long key, int face, int parent_face) {
+    int factor = face / parent_face;
+    long long per_cell = 6LL * face * face;
+    long long cell = key / per_cell, pixel = key % per_cell;
+    long long cube = pixel / ((long long)face * face), ij = pixel % ((long long)face * face);
+    long long i = ij / face, j = ij % face;
+    return cell * (6LL * parent_face * parent_face)
+           + (cube * parent_face + i / factor) * parent_face + j / factor;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_parent_keys(Compress k, int runs, long long* pkeys,
+                                                           int* pids) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    pkeys[r] = parent_key(k.run_key[r], k.grid.pixel_face, k.grid.parent_face);
+    pids[r] = r;
+}
+
+// After sorting fine pencils by (parent key, fine key): parent runs.
+__global__ void __launch_bounds__(kThreads) o1_parent_flags(const long long* spkeys, int runs,
+                                                            int* flag) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > runs) return;
+    flag[j] = j == runs ? 0 : ((j == 0 || spkeys[j] != spkeys[j - 1]) ? 1 : 0);
+}
+
+// Parent rows (children's rows summed in fine-key order), preorder layout:
+// parent c at position start_c + c, its children right after it.
+__global__ void __launch_bounds__(kThreads) o1_parents(const double* rows, const int* spids,
+                                                       const int* flag, const int* scan,
+                                                       int runs, lh_order1_pencils out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= runs) return;
+    int c = scan[j] + (flag[j] ? 0 : -1);     // parent index of sorted child j
+    int position = j + c + 1;                  // preorder slot of the child
+    finish_row(rows + (size_t)spids[j] * kNmom, out, position);
+    if (!flag[j]) return;
+    // first child of parent c: sum the parent's rows and write the parent
+    int end = j + 1;
+    while (end < runs && !flag[end]) ++end;
+    double acc[kNmom];
+    for (int m = 0; m < kNmom; ++m) acc[m] = 0.0;
+    for (int q = j; q < end; ++q)
+        for (int m = 0; m < kNmom; ++m) acc[m] += rows[(size_t)spids[q] * kNmom + m];
+    int slot = j + c;
+    finish_row(acc, out, slot);
+    out.parent[slot] = -2;
+    out.next[slot] = end + c + 1;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_parent_links(const int* flag, const int* scan,
+                                                            int runs, lh_order1_pencils out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= runs) return;
+    int c = scan[j] + (flag[j] ? 0 : -1);
+    int position = j + c + 1;
+    // the parent's slot is the first child's position minus one
+    int first = j;
+    while (!flag[first]) --first;
+    out.parent[position] = first + c;
+    out.next[position] = position + 1;
+}
+
+// ------------------------------------------------------------------- grouped
+
+struct Group {
+    lh_order1_pencils pen;
+    lh_order1_oms om;
+    lh_order1_optics opt;
+    lh_order1_merge merge;
+    int compute_bins;
+    int screen;
+    int n_pen;
+    long long theta_bins;
+    double log_ratio;
+    double mut_min, speed_max;
+    long long* keys;          // (work, 2 * n_pen)
+    int* vals;
+    long long* sorted_keys;
+    int* sorted_vals;
+    int* seg_offsets;         // (work + 1)
+    int* group_flag;          // (work * 2 n_pen + 1)
+    int* group_scan;
+    int* group_start;         // (groups + 1)
+    int* om_groups;           // (work + 1) first group of every OM
+    double* values;           // (groups, nv) caller buffer
+    double* prepared;         // (n_pen, 7)
+    double* acc;              // (work, bands, bins, 4)
+    double* pre;              // (work, bands, 3)
+    double* lambda_bins;      // (work, bands, bins)
+    double* charge;           // (n_recv) output
+    double* bins;             // (n_recv, bins) output
+    int* info;
+};
+
+__global__ void __launch_bounds__(kThreads) o1_prepare_shape(Group G) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.n_pen) return;
+    const double* a = G.pen.major + 3 * p;
+    const double* s = G.pen.direction + 3 * p;
+    const double* cov = G.pen.pos_cov + 6 * p;
+    double bx = s[1] * a[2] - s[2] * a[1];
+    double by = s[2] * a[0] - s[0] * a[2];
+    double bz = s[0] * a[1] - s[1] * a[0];
+    double* q = G.prepared + 7 * (size_t)p;
+    q[0] = bx; q[1] = by; q[2] = bz;
+    q[3] = (a[0] * (cov[0] * a[0] + cov[3] * a[1] + cov[4] * a[2])
+            + a[1] * (cov[3] * a[0] + cov[1] * a[1] + cov[5] * a[2])
+            + a[2] * (cov[4] * a[0] + cov[5] * a[1] + cov[2] * a[2]));
+    q[4] = (bx * (cov[0] * bx + cov[3] * by + cov[4] * bz)
+            + by * (cov[3] * bx + cov[1] * by + cov[5] * bz)
+            + bz * (cov[4] * bx + cov[5] * by + cov[2] * bz));
+    q[5] = (a[0] * (cov[0] * bx + cov[3] * by + cov[4] * bz)
+            + a[1] * (cov[3] * bx + cov[1] * by + cov[5] * bz)
+            + a[2] * (cov[4] * bx + cov[5] * by + cov[2] * bz));
+    q[6] = cov[0] + cov[1] + cov[2];
+}
+
+__device__ double gauss_inverse_mean(double u, double v, double var_u, double var_v) {
+    double scale = 2.0 * (var_u + var_v) + u * u + v * v;
+    if (scale <= 0.0) return 1e300;
+    double s0 = 1.0 / sqrt(scale);
+    double total = 0.0;
+    for (int k = 0; k < kGauss; ++k) {
+        double s = s0 * kTangent[k];
+        double s2 = s * s;
+        double a = 1.0 + 2.0 * s2 * var_u;
+        double b = 1.0 + 2.0 * s2 * var_v;
+        total += kWeightedSec2[k] * exp(-s2 * (u * u / a + v * v / b)) / sqrt(a * b);
+    }
+    return 2.0 / sqrt(M_PI) * s0 * total;
+}
+
+// _pencil_shape_factor_from_ray (_prompt_numba.py:1389-1436).
+__device__ bool shape_factor(const Group& G, int p, double r, double theta, double sine,
+                             double hx, double hy, double hz, double* factor_out,
+                             double* theta_out, int* averaged_out) {
+    const double* a = G.pen.major + 3 * p;
+    const double* q = G.prepared + 7 * (size_t)p;
+    double smoothing = G.opt.smoothing;
+    double inv_r2 = 1.0 / (r * r);
+    double m11 = G.pen.var_major[p] + q[3] * inv_r2;
+    double m22 = G.pen.var_minor[p] + q[4] * inv_r2;
+    double m12 = q[5] * inv_r2;
+    double upper = fmax(m11, m22) + fabs(m12);
+    double biggest = fmax(fmax(fabs(m11), fabs(m22)), fabs(m12));
+    bool far = biggest < 1e150
+               && theta * theta > smoothing * smoothing * fmax(upper, 0.0) * (1.0 + 1e-12);
+    double mu1 = 0, mu2 = 0, extent = 0;
+    if (!far) {
+        double half_tr = 0.5 * (m11 + m22);
+        double disc = sqrt(fmax(0.25 * (m11 - m22) * (m11 - m22) + m12 * m12, 0.0));
+        mu1 = half_tr + disc;
+        mu2 = fmax(half_tr - disc, 1e-16 * mu1 + 1e-300);
+        extent = sqrt(fmax(mu1, 0.0));
+    }
+    double u = hx * a[0] + hy * a[1] + hz * a[2];
+    double v = hx * q[0] + hy * q[1] + hz * q[2];
+    if (!far && extent > 0.0 && theta < smoothing * extent) {
+        double angle = 0.5 * atan2(2.0 * m12, m11 - m22);
+        double ca = cos(angle), sa = sin(angle);
+        double mean = gauss_inverse_mean(ca * u + sa * v, -sa * u + ca * v, mu1, mu2);
+        double factor = mean * (sine > 1e-300 ? theta / sine : 1.0);
+        *theta_out = factor > 1.0 ? asin(fmin(1.0, 1.0 / factor)) : theta;
+        *factor_out = factor;
+        *averaged_out = theta < extent;
+        return true;
+    }
+    if (sine <= 1e-300) { *factor_out = 0.0; *theta_out = theta; *averaged_out = 0; return false; }
+    double d2 = u * u + v * v;
+    double factor = 1.0 / sine;
+    if (d2 > 0.0) {
+        double quad_form = m11 * u * u + 2.0 * m12 * u * v + m22 * v * v;
+        factor *= 1.0 + 0.5 * (3.0 * quad_form / d2 - (m11 + m22)) / d2;
+    }
+    *factor_out = factor;
+    *theta_out = theta;
+    *averaged_out = 0;
+    return true;
+}
+
+// inverse_radius_rule + pencil_radial_rule (_prompt_screen.py:96-167).
+__device__ bool radial_rule(const Group& G, int p, double r, double theta, double sine, double hx,
+                            double hy, double hz, double* f1, double* t1, double* f2, double* t2,
+                            int* averaged) {
+    const double* q = G.prepared + 7 * (size_t)p;
+    const double* a = G.pen.major + 3 * p;
+    double smoothing = G.opt.smoothing;
+    double m11 = G.pen.var_major[p] + q[3] / (r * r);
+    double m22 = G.pen.var_minor[p] + q[4] / (r * r);
+    double m12 = q[5] / (r * r);
+    double half = 0.5 * (m11 + m22);
+    double disc = sqrt(fmax(0.25 * (m11 - m22) * (m11 - m22) + m12 * m12, 0.0));
+    double mu1 = half + disc;
+    double mu2 = fmax(half - disc, 1e-16 * mu1 + 1e-300);
+    double extent = sqrt(fmax(mu1, 0.0));
+    if (extent > 0.0 && theta < smoothing * extent) {
+        double u = hx * a[0] + hy * a[1] + hz * a[2];
+        double v = hx * q[0] + hy * q[1] + hz * q[2];
+        if (sine > 0.0) { u *= theta / sine; v *= theta / sine; }
+        double angle = 0.5 * atan2(2 * m12, m11 - m22);
+        double ca = cos(angle), sa = sin(angle);
+        double uu = ca * u + sa * v, vv = -sa * u + ca * v;
+        double scale = 2.0 * (mu1 + mu2) + uu * uu + vv * vv;
+        if (scale > 0.0) {
+            double s0 = 1.0 / sqrt(scale);
+            double inverse = 0.0, radial = 0.0;
+            for (int k = 0; k < kGauss; ++k) {
+                double t = s0 * kTangent[k];
+                double aa = 1.0 + 2.0 * t * t * mu1;
+                double bb = 1.0 + 2.0 * t * t * mu2;
+                double mass = kWeightedSec2[k] * exp(-t * t * (uu * uu / aa + vv * vv / bb))
+                              / sqrt(aa * bb);
+                inverse += mass;
+                radial += mass * (mu1 / aa + mu2 / bb + uu * uu / (aa * aa) + vv * vv / (bb * bb));
+            }
+            double multiplier = 2.0 / sqrt(M_PI) * s0;
+            inverse *= multiplier;
+            radial *= multiplier;
+            double w0 = 0, a0 = 0, w1 = 0, a1 = 0;
+            if (inverse > 0.0) {
+                double mean = 1.0 / inverse;
+                double second = radial / inverse;
+                double variance = second - mean * mean;
+                if (variance <= 0.0) {
+                    w0 = inverse; a0 = mean; w1 = 0.0; a1 = mean;
+                } else {
+                    double third = (uu * uu + vv * vv + mu1 + mu2) / inverse;
+                    double other = (third - 2.0 * mean * second + mean * mean * mean) / variance;
+                    double dd = sqrt((mean - other) * (mean - other) + 4.0 * variance);
+                    double low = 0.5 * (mean + other - dd);
+                    double high = 0.5 * (mean + other + dd);
+                    double low_weight = inverse * (high - mean) / (high - low);
+                    w0 = low_weight; a0 = low; w1 = inverse - low_weight; a1 = high;
+                }
+            }
+            if (a0 > 0.0 && a1 < 0.5 * M_PI && w0 > 0.0 && w1 >= 0.0) {
+                *f1 = w0 * a0 / sin(a0); *t1 = a0;
+                *f2 = w1 * a1 / sin(a1); *t2 = a1;
+                *averaged = theta < extent;
+                return true;
+            }
+        }
+    }
+    double factor, angle;
+    bool ok = shape_factor(G, p, r, theta, sine, hx, hy, hz, &factor, &angle, averaged);
+    *f1 = factor; *t1 = angle; *f2 = 0.0; *t2 = angle;
+    return ok;
+}
+
+struct Entry {
+    long long key;
+    double w0, w2, c1, c3, theta, r, t_emit, tau;
+    int averaged;
+};
+
+__device__ inline int theta_index(double theta, double theta_lo, double log_ratio) {
+    if (theta < theta_lo) return 0;
+    return 1 + (int)(log(theta / theta_lo) / log_ratio);
+}
+
+__device__ inline void frame_of(const Group& G, int p, int d, double* r, double* h, double* theta,
+                                double* sine, double* c1, double* c3) {
+    const double* pp = G.pen.position + 3 * p;
+    double dx = G.om.positions[3 * d] - pp[0];
+    double dy = G.om.positions[3 * d + 1] - pp[1];
+    double dz = G.om.positions[3 * d + 2] - pp[2];
+    double rr = sqrt(dx * dx + dy * dy + dz * dz);
+    h[0] = dx / rr; h[1] = dy / rr; h[2] = dz / rr;
+    const double* s = G.pen.direction + 3 * p;
+    double nx = G.om.looks[3 * d], ny = G.om.looks[3 * d + 1], nz = G.om.looks[3 * d + 2];
+    *c1 = s[0] * nx + s[1] * ny + s[2] * nz;
+    double cosine = h[0] * s[0] + h[1] * s[1] + h[2] * s[2];
+    double px = h[0] - cosine * s[0], py = h[1] - cosine * s[1], pz = h[2] - cosine * s[2];
+    double sn = sqrt(px * px + py * py + pz * pz);
+    *c3 = sn > 1e-300 ? (px * nx + py * ny + pz * nz) / sn : 0.0;
+    *theta = atan2(sn, cosine);
+    *sine = sn;
+    *r = rr;
+}
+
+// The (up to two) group entries of pencil p at OM d (_grouped_pencils_order1_block
+// :1815-1922; screen :222-326).
+__device__ int pencil_entries(const Group& G, int p, int d, Entry out[2]) {
+    double r, h[3], theta, sine, c1, c3;
+    frame_of(G, p, d, &r, h, &theta, &sine, &c1, &c3);
+    if (G.screen) {
+        long long parent = G.pen.parent[p];
+        if (parent == -2) {
+            double extent2 = G.pen.var_major[p] + G.prepared[7 * (size_t)p + 6] / (r * r);
+            if (theta < 0.5 * M_PI && theta * theta < G.opt.smoothing * G.opt.smoothing * extent2)
+                return 0;
+        } else {
+            int q = (int)parent;
+            double pr, ph[3], pt, ps, pc1, pc3;
+            frame_of(G, q, d, &pr, ph, &pt, &ps, &pc1, &pc3);
+            double extent2 = G.pen.var_major[q] + G.prepared[7 * (size_t)q + 6] / (pr * pr);
+            bool near = pt < 0.5 * M_PI && pt * pt < G.opt.smoothing * G.opt.smoothing * extent2;
+            if (!near) return 0;   // the parent stands for its children
+        }
+    }
+    double t_emit = G.screen ? 0.0 : G.pen.time[p] - G.om.origins[d];
+    double tau = G.screen ? 0.0 : t_emit + r / G.merge.speed_ref;
+    long long base;
+    if (G.screen) {
+        base = ((long long)floor(log(r) / G.merge.log_r_step) + (1LL << 20)) * 2 * G.theta_bins;
+    } else {
+        long long k0 = (long long)floor(log(r) / G.merge.log_r_step) + (1LL << 20);
+        long long k1 = (long long)floor(tau / G.merge.tau_step) + (1LL << 20);
+        k1 = min(max(k1, 0LL), (1LL << 21) - 1);
+        base = (k1 * (1LL << 21) + k0) * 2 * G.theta_bins;
+    }
+    int backward = theta > 0.5 * M_PI;
+    double bound = 0.0;
+    if (!backward)
+        bound = G.pen.var_major[p] + G.pen.var_minor[p] + G.prepared[7 * (size_t)p + 6] / (r * r);
+    double weight, weight2 = 0.0, theta2 = theta;
+    int averaged;
+    if (backward) {
+        weight = 1.0 / r;
+        averaged = 0;
+    } else if (sine > 1e-300 && bound < 1e-4 * sine * sine) {
+        weight = (1.0 / sine) / r;
+        averaged = 0;
+    } else if (G.screen) {
+        double f1, t1, f2, t2;
+        if (!radial_rule(G, p, r, theta, sine, h[0], h[1], h[2], &f1, &t1, &f2, &t2, &averaged))
+            return 0;
+        weight = f1 / r;
+        weight2 = f2 / r;
+        theta = t1;
+        theta2 = t2;
+    } else {
+        double factor, theta_eff;
+        if (!shape_factor(G, p, r, theta, sine, h[0], h[1], h[2], &factor, &theta_eff, &averaged))
+            return 0;
+        weight = factor / r;
+        theta = theta_eff;
+    }
+    int nodes = weight2 > 0.0 ? 2 : 1;
+    for (int node = 0; node < nodes; ++node) {
+        double w = node == 0 ? weight : weight2;
+        double th = node == 0 ? theta : theta2;
+        Entry& e = out[node];
+        e.key = base + backward * G.theta_bins + theta_index(th, G.merge.theta_lo, G.log_ratio);
+        e.w0 = w * G.pen.w0[p];
+        e.w2 = w * G.pen.w2[p];
+        e.c1 = c1;
+        e.c3 = c3;
+        e.averaged = averaged;
+        e.theta = th;
+        e.r = r;
+        e.t_emit = t_emit;
+        e.tau = tau;
+    }
+    return nodes;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_group_keys(Group G) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int w = blockIdx.y;
+    if (p >= G.n_pen) return;
+    int d = G.om.work[w];
+    Entry e[2];
+    int n = pencil_entries(G, p, d, e);
+    size_t at = ((size_t)w * G.n_pen + p) * 2;
+    for (int node = 0; node < 2; ++node) {
+        G.keys[at + node] = node < n ? e[node].key : kSentinel;
+        G.vals[at + node] = 2 * p + node;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) o1_group_flags(Group G, long long total) {
+    long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > total) return;
+    if (j == total) { G.group_flag[j] = 0; return; }
+    long long seg = 2LL * G.n_pen;
+    long long k = G.sorted_keys[j];
+    bool start = k != kSentinel && (j % seg == 0 || G.sorted_keys[j - 1] != k);
+    G.group_flag[j] = start ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_group_starts(Group G, long long total, int work) {
+    long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > total) return;
+    long long seg = 2LL * G.n_pen;
+    if (j == total) {
+        int groups = G.group_scan[j];
+        G.info[0] = groups;
+        G.om_groups[work] = groups;
+        return;
+    }
+    if (j % seg == 0) G.om_groups[j / seg] = G.group_scan[j];
+    if (G.group_flag[j]) G.group_start[G.group_scan[j]] = (int)j;
+}
+
+// End of group g's member run: the next key change, sentinel or OM boundary.
+__device__ inline long long group_end(const Group& G, int g) {
+    long long j = G.group_start[g];
+    long long seg = 2LL * G.n_pen;
+    long long stop = (j / seg + 1) * seg;
+    long long k = G.sorted_keys[j];
+    long long e = j + 1;
+    while (e < stop && G.sorted_keys[e] == k) ++e;
+    return e;
+}
+
+__device__ inline void fill_moments(const Group& G, double weight0, double weight2, double c1,
+                                    double c3, int averaged, double* out0, double* out2) {
+    double cp[8], sp[8];
+    cp[0] = 1.0; sp[0] = 1.0;
+    double perp2 = fmax(1.0 - c1 * c1, 0.0);
+    for (int p = 1; p <= G.opt.max_power; ++p) {
+        cp[p] = cp[p - 1] * c1;
+        if (averaged) sp[p] = (p % 2 == 1) ? 0.0 : sp[p - 2] * perp2 * (p - 1.0) / p;
+        else sp[p] = sp[p - 1] * c3;
+    }
+    for (int m = 0; m < G.opt.n_mono; ++m) {
+        out0[m] = weight0 * cp[G.opt.ipow[m]] * sp[G.opt.jpow[m]];
+        out2[m] = weight2 * cp[G.opt.ipow[m]] * sp[G.opt.jpow[m]];
+    }
+}
+
+// Group sums: members in (key, pencil) order, as the reference's hash rows.
+__global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    int n_mono = G.opt.n_mono;
+    int nv = 2 * n_mono + 6;
+    long long seg = 2LL * G.n_pen;
+    long long j0 = G.group_start[g];
+    int w = (int)(j0 / seg);
+    int d = G.om.work[w];
+    long long j1 = group_end(G, g);
+    double row[2 * kMaxMono + 6];
+    for (int m = 0; m < nv; ++m) row[m] = 0.0;
+    double mom0[kMaxMono], mom2[kMaxMono];
+    for (long long j = j0; j < j1; ++j) {
+        int v = G.sorted_vals[j];
+        int p = v >> 1, node = v & 1;
+        Entry e[2];
+        pencil_entries(G, p, d, e);
+        const Entry& x = e[node];
+        fill_moments(G, x.w0, x.w2, x.c1, x.c3, x.averaged, mom0, mom2);
+        for (int m = 0; m < n_mono; ++m) {
+            row[m] += mom0[m];
+            row[n_mono + m] += mom2[m];
+        }
+        double omega = fabs(mom0[0]);
+        row[2 * n_mono] += omega;
+        row[2 * n_mono + 1] += omega * x.theta;
+        row[2 * n_mono + 2] += omega * x.r;
+        row[2 * n_mono + 3] += omega * x.t_emit;
+        row[2 * n_mono + 4] += omega * x.tau;
+        row[2 * n_mono + 5] += omega * x.tau * x.tau;
+    }
+    for (int m = 0; m < nv; ++m) G.values[(size_t)g * nv + m] = row[m];
+}
+
+// ---- batch core (one workgroup per OM)
+
+__device__ void ifuncs(double z, double ez, double* i0, double* i1, double* i2) {
+    if (fabs(z) < 1.0) {
+        double a = kTaylor[0][23], b = kTaylor[1][23], c = kTaylor[2][23];
+        for (int n = 22; n >= 0; --n) {
+            a = a * z + kTaylor[0][n];
+            b = b * z + kTaylor[1][n];
+            c = c * z + kTaylor[2][n];
+        }
+        *i0 = a; *i1 = b; *i2 = c;
+        return;
+    }
+    *i0 = (ez - 1.0) / z;
+    *i1 = (ez * (z - 1.0) + 1.0) / (z * z);
+    *i2 = (ez * (z * z - 2.0 * z + 2.0) - 2.0) / (z * z * z);
+}
+
+__device__ double quad_integral(double e_start, double alpha, double beta, double gamma,
+                                double width, double kappa, double e_end) {
+    double z = kappa * width;
+    double ez = e_start > 0.0 ? e_end / e_start : exp(z);
+    double i0, i1, i2;
+    ifuncs(z, ez, &i0, &i1, &i2);
+    return e_start * width * (alpha * i0 + width * (beta * i1 + width * gamma * i2));
+}
+
+__device__ int find_bin(const double* edges, int nb, double t) {
+    if (t < edges[0]) return -1;
+    if (t >= edges[nb]) return nb;
+    int low = 0, high = nb;
+    while (high - low > 1) {
+        int middle = (low + high) / 2;
+        if (t < edges[middle]) high = middle; else low = middle;
+    }
+    return low;
+}
+
+struct LambdaBins {
+    const double* edges;
+    const double* eedge;   // (nb + 1) of this wavelength
+    double* acc;           // (nb, 4)
+    double* pre;           // (3)
+    int nb, direct_bins;
+};
+
+// _deposit_panel (_prompt_numba.py:186-253).
+__device__ void deposit_panel(const LambdaBins& B, double t_a, double t_b, double e_a, double e_b,
+                              double alpha, double beta, double gamma, double kappa) {
+    int nb = B.nb;
+    const double* edges = B.edges;
+    double lo = t_a > edges[0] ? t_a : edges[0];
+    double hi = t_b < edges[nb] ? t_b : edges[nb];
+    if (hi <= lo) return;
+    int k_lo = find_bin(edges, nb, lo);
+    int k_hi = find_bin(edges, nb, hi);
+    if (k_hi >= nb || (k_hi > k_lo && hi <= edges[k_hi])) k_hi -= 1;
+    if (k_hi - k_lo + 1 <= B.direct_bins) {
+        for (int k = k_lo; k <= k_hi; ++k) {
+            double s_lo = lo > edges[k] ? lo : edges[k];
+            double s_hi = hi < edges[k + 1] ? hi : edges[k + 1];
+            if (s_hi <= s_lo) continue;
+            double e_lo = s_lo == t_a ? e_a : B.eedge[k];
+            if (s_lo != t_a && s_lo != edges[k]) e_lo = exp(kappa * s_lo);
+            double e_hi = s_hi == t_b ? e_b : B.eedge[k + 1];
+            if (s_hi != t_b && s_hi != edges[k + 1]) e_hi = exp(kappa * s_hi);
+            double shift = s_lo - t_a;
+            double a_loc = alpha + shift * (beta + shift * gamma);
+            double b_loc = beta + 2.0 * gamma * shift;
+            B.acc[4 * k + 3] += quad_integral(e_lo, a_loc, b_loc, gamma, s_hi - s_lo, kappa, e_hi);
+        }
+        return;
+    }
+    double big_a = alpha - t_a * (beta - gamma * t_a);
+    double big_b = beta - 2.0 * gamma * t_a;
+    double big_c = gamma;
+    if (t_a < edges[0]) {
+        B.pre[0] += big_a; B.pre[1] += big_b; B.pre[2] += big_c;
+    } else {
+        int k = k_lo;
+        B.acc[4 * k] += big_a; B.acc[4 * k + 1] += big_b; B.acc[4 * k + 2] += big_c;
+        B.acc[4 * k + 3] += quad_integral(e_a, alpha, beta, gamma, edges[k + 1] - t_a, kappa,
+                                          B.eedge[k + 1]);
+    }
+    if (t_b < edges[nb]) {
+        int k = find_bin(edges, nb, t_b);
+        B.acc[4 * k] -= big_a; B.acc[4 * k + 1] -= big_b; B.acc[4 * k + 2] -= big_c;
+        double shift = t_b - t_a;
+        double a_loc = alpha + shift * (beta + shift * gamma);
+        double b_loc = beta + 2.0 * gamma * shift;
+        B.acc[4 * k + 3] -= quad_integral(e_b, a_loc, b_loc, gamma, edges[k + 1] - t_b, kappa,
+                                          B.eedge[k + 1]);
+    }
+}
+
+__device__ void cubic_coefficients(const Group& G, const double* mom, double* a, double* b) {
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0;
+    for (int m = 0; m < G.opt.n_mono; ++m) {
+        double value = G.opt.coef[m] * mom[m];
+        int i = G.opt.ipow[m], j = G.opt.jpow[m];
+        if (j == 0) {
+            if (i == 0) a0 += value; else if (i == 1) a1 += value;
+            else if (i == 2) a2 += value; else a3 += value;
+        } else if (j == 1) {
+            if (i == 0) b0 += value; else if (i == 1) b1 += value; else b2 += value;
+        } else if (j == 2) {
+            if (i == 0) { a0 += value; a2 -= value; } else { a1 += value; a3 -= value; }
+        } else {
+            b0 += value; b2 -= value;
+        }
+    }
+    a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3; b[0] = b0; b[1] = b1; b[2] = b2;
+}
+
+constexpr int kBatchGroups = 64;   // groups staged in LDS per profile pass
+
+__global__ void __launch_bounds__(kThreads) o1_batches(Group G) {
+    __shared__ double s_buf[kMaxPathNodes];
+    __shared__ double g0[kMaxPathNodes];
+    __shared__ double g2[kMaxPathNodes];
+    __shared__ double coeff[kBatchGroups][16];   // A0[4] B0[3] A2[4] B2[3] sine cosine
+    __shared__ double batch[8];
+    __shared__ int batch_n;
+    __shared__ int batch_stop;
+    int w = blockIdx.x;
+    int d = G.om.work[w];
+    int nl = G.opt.bands, nb = G.opt.bins;
+    int n_mono = G.opt.n_mono;
+    int nv = 2 * n_mono + 6;
+    int g_begin = G.om_groups[w], g_end = G.om_groups[w + 1];
+    double* acc = G.acc + (size_t)w * nl * nb * 4;
+    double* pre = G.pre + (size_t)w * nl * 3;
+    double charge_l[1];   // per-thread charge of its wavelength (threads < nl)
+    charge_l[0] = 0.0;
+    if (G.compute_bins)
+        for (size_t i = threadIdx.x; i < (size_t)nl * nb * 4; i += kThreads) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < nl * 3; i += kThreads) pre[i] = 0.0;
+    __syncthreads();
+    int cubic = G.opt.max_power <= 3;
+    int start = g_begin;
+    while (start < g_end) {
+        if (threadIdx.x == 0) {
+            long long key = 0;
+            long long batch_key = 0;
+            int stop = start;
+            double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0;
+            double theta_small = M_PI;
+            // first group's key
+            long long j0 = G.group_start[start];
+            key = G.sorted_keys[j0];
+            batch_key = key / G.theta_bins;
+            while (stop < g_end) {
+                long long k = G.sorted_keys[G.group_start[stop]];
+                if (k / G.theta_bins != batch_key) break;
+                const double* row = G.values + (size_t)stop * nv;
+                double omega = row[2 * n_mono];
+                if (omega > 0.0) {
+                    omega_sum += omega;
+                    r_sum += row[2 * n_mono + 2];
+                    t_sum += row[2 * n_mono + 3];
+                    tau_sum += row[2 * n_mono + 4];
+                    tau2_sum += row[2 * n_mono + 5];
+                    theta_small = fmin(theta_small, row[2 * n_mono + 1] / omega);
+                }
+                ++stop;
+            }
+            batch_stop = stop;
+            batch_n = 0;
+            if (omega_sum > 0.0) {
+                int backward = batch_key % 2 == 1;
+                double r_b = r_sum / omega_sum;
+                double t_b = t_sum / omega_sum;
+                double tau_mean = tau_sum / omega_sum;
+                double tau_var = fmax(tau2_sum / omega_sum - tau_mean * tau_mean, 0.0);
+                theta_small = fmax(theta_small, 1e-9);
+                double sigma;
+                if (backward)
+                    sigma = fmax(0.5 * r_b * sin(theta_small) * G.opt.y_g,
+                                 0.25 * r_b * (1.0 - cos(theta_small)));
+                else
+                    sigma = fmax(0.5 * r_b * sin(theta_small)
+                                     * fmax(G.opt.y_g, 0.5 * tan(0.5 * theta_small)),
+                                 1e-9 * r_b);
+                // _batch_grid (:258-286)
+                double h_in = G.opt.path_step;
+                double h_out = G.screen ? G.opt.path_step : G.opt.path_step_tail;
+                double ext_end = G.opt.tail_exponent / G.mut_min;
+                double ext_win = G.screen ? ext_end
+                                          : G.speed_max * (G.opt.edges[nb] - t_b) - r_b;
+                if (ext_win < 0.0) ext_win = 0.0;
+                if (ext_win > ext_end) ext_win = ext_end;
+                double u_end = asinh(ext_end / sigma);
+                double u_win = asinh(ext_win / sigma);
+                int n_in = u_win > 0.0 ? (int)ceil(u_win / h_in) : 0;
+                int n_out = u_end > u_win ? (int)ceil((u_end - u_win) / h_out) : 0;
+                int capacity = min(G.opt.max_path_nodes, kMaxPathNodes);
+                if (n_in + n_out + 1 > capacity) {
+                    double scale = (capacity - 1.0) / (n_in + n_out);
+                    n_in = (int)(n_in * scale);
+                    n_out = capacity - 1 - n_in;
+                }
+                batch_n = n_in + n_out + 1;
+                batch[0] = r_b; batch[1] = sigma; batch[2] = u_win; batch[3] = u_end;
+                batch[4] = n_in; batch[5] = n_out; batch[6] = t_b;
+                batch[7] = sqrt(12.0 * tau_var);
+            }
+        }
+        __syncthreads();
+        int n = batch_n, stop = batch_stop;
+        if (n > 0) {
+            double r_b = batch[0], sigma = batch[1], u_win = batch[2], u_end = batch[3];
+            int n_in = (int)batch[4], n_out = (int)batch[5];
+            for (int k = threadIdx.x; k < n; k += kThreads) {
+                double s;
+                if (k == 0) s = r_b;
+                else if (k <= n_in) s = r_b + sigma * sinh(u_win * k / n_in);
+                else s = r_b + sigma * sinh(u_win + (u_end - u_win) * (k - n_in) / n_out);
+                s_buf[k] = s;
+                g0[k] = 0.0;
+                g2[k] = 0.0;
+            }
+            int backward = (G.sorted_keys[G.group_start[start]] / G.theta_bins) % 2 == 1;
+            // profile: every path node sums its batch's groups in key order
+            for (int g0i = start; g0i < stop; g0i += kBatchGroups) {
+                int chunk = min(kBatchGroups, stop - g0i);
+                __syncthreads();
+                for (int q = threadIdx.x; q < chunk; q += kThreads) {
+                    const double* row = G.values + (size_t)(g0i + q) * nv;
+                    double omega = row[2 * n_mono];
+                    double* cf = coeff[q];
+                    if (omega <= 0.0) { cf[14] = -1.0; continue; }
+                    double theta = fmax(row[2 * n_mono + 1] / omega, 1e-9);
+                    cf[14] = sin(theta);
+                    cf[15] = cos(theta);
+                    if (cubic) {
+                        cubic_coefficients(G, row, cf, cf + 4);
+                        cubic_coefficients(G, row + n_mono, cf + 7, cf + 11);
+                    }
+                }
+                __syncthreads();
+                double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
+                double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
+                for (int k = threadIdx.x; k < n; k += kThreads) {
+                    double acc0 = g0[k], acc2 = g2[k];
+                    double s_k = s_buf[k];
+                    for (int q = 0; q < chunk; ++q) {
+                        const double* cf = coeff[q];
+                        double sin_t = cf[14];
+                        if (sin_t < 0.0) continue;
+                        double cos_t = cf[15];
+                        double rs = r_b * sin_t;
+                        double c, s, path_jacobian;
+                        if (backward) {
+                            double shift = s_k - r_b * cos_t;
+                            double denominator = rs * rs + shift * shift;
+                            c = (rs * rs - shift * shift) / denominator;
+                            s = 2.0 * rs * shift / denominator;
+                            path_jacobian = 2.0 * r_b / denominator;
+                        } else {
+                            double y_theta = sin_t / (1.0 + cos_t);
+                            double jac = 2.0 / (r_b * sin_t);
+                            double y = (s_k - r_b) / rs + y_theta;
+                            double inv = 1.0 / (1.0 + y * y);
+                            c = (1.0 - y * y) * inv;
+                            s = 2.0 * y * inv;
+                            path_jacobian = jac * inv;
+                        }
+                        double a0, a2;
+                        if (cubic) {
+                            a0 = cf[0] + c * (cf[1] + c * (cf[2] + c * cf[3]));
+                            a0 += s * (cf[4] + c * (cf[5] + c * cf[6]));
+                            a2 = cf[7] + c * (cf[8] + c * (cf[9] + c * cf[10]));
+                            a2 += s * (cf[11] + c * (cf[12] + c * cf[13]));
+                        } else {
+                            const double* row = G.values + (size_t)(g0i + q) * nv;
+                            double cpow[8], spow[8];
+                            cpow[0] = 1.0; spow[0] = 1.0;
+                            for (int p = 1; p <= G.opt.max_power; ++p) {
+                                cpow[p] = cpow[p - 1] * c;
+                                spow[p] = spow[p - 1] * s;
+                            }
+                            a0 = 0.0; a2 = 0.0;
+                            for (int m = 0; m < n_mono; ++m) {
+                                double basis = G.opt.coef[m] * cpow[G.opt.ipow[m]] * spow[G.opt.jpow[m]];
+                                a0 += basis * row[m];
+                                a2 += basis * row[n_mono + m];
+                            }
+                        }
+                        double den = g1 - gg * c;
+                        double value = path_jacobian * norm / (den * sqrt(den));
+                        acc0 += a0 * value;
+                        acc2 += a2 * value;
+                    }
+                    g0[k] = acc0;
+                    g2[k] = acc2;
+                }
+            }
+            __syncthreads();
+            // _batch_deposit (:451-551): one thread per wavelength
+            int lam = threadIdx.x;
+            if (lam < nl && G.opt.mus[lam] != 0.0) {
+                double v = G.opt.speed[lam];
+                double mut = G.opt.mut[lam];
+                double kappa = -mut * v;
+                int last = n - 1;
+                double limit = r_b + G.opt.tail_exponent / mut;
+                while (last > 2 && s_buf[last - 1] > limit) --last;
+                if (last >= 2) {
+                    double t_emit_rel = batch[6];
+                    double spread = fmin(batch[7], (s_buf[last] - r_b) / v);
+                    double front_time = t_emit_rel + r_b / v;
+                    double end_time = t_emit_rel + s_buf[last] / v;
+                    int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
+                                  || end_time + 0.5 * spread <= G.opt.edges[0];
+                    double t_base = outside ? 0.0 : t_emit_rel;
+                    double e_emit = exp(kappa * t_base);
+                    double h_scale = G.opt.mus[lam] * v / e_emit;
+                    double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
+                    LambdaBins B;
+                    B.edges = G.opt.edges;
+                    B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
+                    B.acc = acc + (size_t)lam * nb * 4;
+                    B.pre = pre + 3 * lam;
+                    B.nb = nb;
+                    B.direct_bins = G.opt.direct_bins;
+                    bool bins = G.compute_bins && !outside;
+                    double h_front = w0 * g0[0] + w2 * g2[0];
+                    double total = 0.0;
+                    double e_first = e_emit * exp(-mut * s_buf[0]);
+                    double e_a = e_first, e_last = e_first;
+                    int j = 0;
+                    while (j < last) {
+                        double t_a, t_b, alpha, beta, gamma, e_b;
+                        int step;
+                        if (j + 2 <= last) {
+                            t_a = t_base + s_buf[j] / v;
+                            double t_m = t_base + s_buf[j + 1] / v;
+                            t_b = t_base + s_buf[j + 2] / v;
+                            double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                            double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                            double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
+                            double u_m = t_m - t_a, u_b = t_b - t_a;
+                            double d1 = (h1 - h0) / u_m;
+                            double d2 = ((h2 - h1) / (t_b - t_m) - d1) / u_b;
+                            alpha = h0;
+                            beta = d1 - d2 * u_m;
+                            gamma = d2;
+                            e_b = e_emit * exp(-mut * s_buf[j + 2]);
+                            step = 2;
+                        } else {
+                            double t_p = t_base + s_buf[j - 1] / v;
+                            t_a = t_base + s_buf[j] / v;
+                            t_b = t_base + s_buf[j + 1] / v;
+                            double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
+                            double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                            double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                            double u_m = t_a - t_p, u_b = t_b - t_p;
+                            double d1 = (h0 - hp) / u_m;
+                            double d2 = ((h1 - h0) / (t_b - t_a) - d1) / u_b;
+                            alpha = h0;
+                            beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
+                            gamma = d2;
+                            e_b = e_emit * exp(-mut * s_buf[j + 1]);
+                            step = 1;
+                        }
+                        total += quad_integral(e_a, alpha, beta, gamma, t_b - t_a, kappa, e_b);
+                        if (bins) deposit_panel(B, t_a, t_b, e_a, e_b, alpha, beta, gamma, kappa);
+                        e_a = e_b;
+                        e_last = e_b;
+                        j += step;
+                    }
+                    double t_f = t_base + s_buf[0] / v, t_end = t_base + s_buf[last] / v;
+                    double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa,
+                                                 e_last);
+                    total += exact;
+                    if (bins && spread > 0.0) {
+                        double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
+                        double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
+                        double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
+                        double flat = t_end > t_2
+                                          ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa, e_last)
+                                          : 0.0;
+                        double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
+                        deposit_panel(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa);
+                        if (t_end > t_2) deposit_panel(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa);
+                    } else if (bins && t_end > t_f) {
+                        deposit_panel(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa);
+                    }
+                    charge_l[0] += total;
+                }
+            }
+        }
+        __syncthreads();
+        start = stop;
+    }
+    // charge of the OM: sum over wavelengths in order (_finalize / charge-only)
+    __shared__ double lambda_charge[kMaxBands];
+    if (threadIdx.x < nl) lambda_charge[threadIdx.x] = charge_l[0];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        for (int l = 0; l < nl; ++l) total += lambda_charge[l];
+        G.charge[d] = G.om.areas[d] * total;
+    }
+    if (!G.compute_bins) return;
+    // _finalize (:610-632): per-wavelength prefix pass, then bins summed over wavelengths
+    double* lb = G.lambda_bins + (size_t)w * nl * nb;
+    if (threadIdx.x < nl) {
+        int lam = threadIdx.x;
+        const double* eedge = G.opt.eedge + (size_t)lam * (nb + 1);
+        const double* it = G.opt.itab + (size_t)lam * nb * 3;
+        const double* a4 = acc + (size_t)lam * nb * 4;
+        double a = pre[3 * lam], b = pre[3 * lam + 1], c = pre[3 * lam + 2];
+        for (int k = 0; k < nb; ++k) {
+            double e_k = G.opt.edges[k];
+            double width = G.opt.edges[k + 1] - e_k;
+            double a_loc = a + e_k * (b + c * e_k);
+            double b_loc = b + 2.0 * c * e_k;
+            double value = eedge[k] * width
+                           * (a_loc * it[3 * k] + width * (b_loc * it[3 * k + 1] + width * c * it[3 * k + 2]));
+            lb[(size_t)lam * nb + k] = value + a4[4 * k + 3];
+            a += a4[4 * k];
+            b += a4[4 * k + 1];
+            c += a4[4 * k + 2];
+        }
+    }
+    __syncthreads();
+    double area = G.om.areas[d];
+    for (int k = threadIdx.x; k < nb; k += kThreads) {
+        double total = 0.0;
+        for (int l = 0; l < nl; ++l) total += area * lb[(size_t)l * nb + k];
+        G.bins[(size_t)d * nb + k] = total;
+    }
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int bits_for(unsigned long long value) {
+    int bits = 1;
+    while (bits < 63 && (1ULL << bits) <= value) ++bits;
+    return bits;
+}
+
+// ---- compress layout
+
+struct CompressLayout {
+    size_t keys, ids, sorted_keys, sorted_ids, cp, sp, sub_element, sub_index, run_flag, run_scan;
+    size_t run_start, run_key, rows, info, sort_temp, scan_temp, pkeys, pids, spkeys, spids,
+        pflag, pscan, psort_temp, total;
+};
+
+int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
+                    CompressLayout& L) {
+    long long nodes = grid->subsegments * grid->ring_nodes;
+    size_t sort_temp = 0, scan_temp = 0, psort_temp = 0;
+    unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
+                                 * 6ULL * grid->pixel_face * grid->pixel_face;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_temp, (long long*)nullptr,
+                                           (long long*)nullptr, (int*)nullptr, (int*)nullptr,
+                                           (int)nodes, 0, bits_for(max_key), (hipStream_t)0)
+        != hipSuccess) {
+        lh::set_error("radix sort workspace query failed");
+        return LH_ERR_HIP;
+    }
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
+                                         (int)(nodes + 1), (hipStream_t)0) != hipSuccess) {
+        lh::set_error("scan workspace query failed");
+        return LH_ERR_HIP;
+    }
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, psort_temp, (long long*)nullptr,
+                                           (long long*)nullptr, (int*)nullptr, (int*)nullptr,
+                                           (int)runs_cap, 0, bits_for(max_key),
+                                           (hipStream_t)0) != hipSuccess) {
+        lh::set_error("radix sort workspace query failed");
+        return LH_ERR_HIP;
+    }
+    size_t at = 0;
+    auto take = [&](size_t bytes) { size_t here = at; at += align_up(bytes); return here; };
+    L.info = take(8 * sizeof(int));
+    L.keys = take(nodes * sizeof(long long));
+    L.ids = take(nodes * sizeof(int));
+    L.sorted_keys = take(nodes * sizeof(long long));
+    L.sorted_ids = take(nodes * sizeof(int));
+    L.cp = take(nodes * sizeof(double));
+    L.sp = take(nodes * sizeof(double));
+    L.sub_element = take(grid->subsegments * sizeof(int));
+    L.sub_index = take(grid->subsegments * sizeof(int));
+    L.run_flag = take((nodes + 1) * sizeof(int));
+    L.run_scan = take((nodes + 1) * sizeof(int));
+    L.run_start = take((nodes + 1) * sizeof(int));
+    L.run_key = take(nodes * sizeof(long long));
+    L.rows = take((size_t)runs_cap * kNmom * sizeof(double));
+    L.pkeys = take(runs_cap * sizeof(long long));
+    L.pids = take(runs_cap * sizeof(int));
+    L.spkeys = take(runs_cap * sizeof(long long));
+    L.spids = take(runs_cap * sizeof(int));
+    L.pflag = take((runs_cap + 1) * sizeof(int));
+    L.pscan = take((runs_cap + 1) * sizeof(int));
+    L.sort_temp = take(sort_temp);
+    L.scan_temp = take(scan_temp);
+    L.psort_temp = take(psort_temp);
+    L.total = at;
+    return LH_OK;
+}
+
+int check_grid(const lh_order1_chords* ch, const lh_order1_grid* grid) {
+    LH_REQUIRE(ch != nullptr && grid != nullptr, "descriptor is NULL");
+    LH_REQUIRE(ch->count >= 1, "no chords");
+    LH_REQUIRE(ch->start_m && ch->direction && ch->length_m && ch->coefficient0
+                   && ch->coefficient2 && ch->cone_cosine && ch->start_ns && ch->end_ns
+                   && ch->sub_offset, "chord pointers must be set");
+    LH_REQUIRE(grid->cell_m > 0 && grid->pixel_face >= 1 && grid->ring_nodes >= 1,
+               "cell_m, pixel_face and ring_nodes must be positive");
+    LH_REQUIRE(grid->subsegments >= ch->count, "subsegment count is inconsistent");
+    long long nodes = grid->subsegments * grid->ring_nodes;
+    LH_REQUIRE(nodes < (1LL << 31) - 1, "too many ring nodes (%lld)", nodes);
+    double cells = (double)grid->dims[0] * grid->dims[1] * grid->dims[2];
+    LH_REQUIRE(grid->dims[0] >= 1 && grid->dims[1] >= 1 && grid->dims[2] >= 1
+                   && cells * 6.0 * grid->pixel_face * grid->pixel_face < 4.0e18,
+               "cell grid too large for 64-bit pencil keys; increase cell_m");
+    LH_REQUIRE(grid->parent_face == 0
+                   || (grid->parent_face > 0 && grid->pixel_face % grid->parent_face == 0),
+               "angular refinement must be a multiple of its parent");
+    return LH_OK;
+}
+
+Compress make_compress(const lh_order1_chords* ch, const lh_order1_grid* grid, char* base,
+                       const CompressLayout& L) {
+    Compress k;
+    k.ch = *ch;
+    k.grid = *grid;
+    k.nodes = grid->subsegments * grid->ring_nodes;
+    k.keys = reinterpret_cast<long long*>(base + L.keys);
+    k.ids = reinterpret_cast<int*>(base + L.ids);
+    k.sorted_keys = reinterpret_cast<long long*>(base + L.sorted_keys);
+    k.sorted_ids = reinterpret_cast<int*>(base + L.sorted_ids);
+    k.cp = reinterpret_cast<double*>(base + L.cp);
+    k.sp = reinterpret_cast<double*>(base + L.sp);
+    k.sub_element = reinterpret_cast<int*>(base + L.sub_element);
+    k.sub_index = reinterpret_cast<int*>(base + L.sub_index);
+    k.run_flag = reinterpret_cast<int*>(base + L.run_flag);
+    k.run_scan = reinterpret_cast<int*>(base + L.run_scan);
+    k.run_start = reinterpret_cast<int*>(base + L.run_start);
+    k.run_key = reinterpret_cast<long long*>(base + L.run_key);
+    k.rows = reinterpret_cast<double*>(base + L.rows);
+    k.info = reinterpret_cast<int*>(base + L.info);
+    return k;
+}
+
+unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+// ---- grouped layout
+
+struct GroupLayout {
+    size_t keys, vals, sorted_keys, sorted_vals, seg_offsets, group_flag, group_scan, group_start,
+        om_groups, prepared, acc, pre, lambda_bins, info, sort_temp, scan_temp, total;
+};
+
+int grouped_layout(int work, int n_pen, int bands, int bins, GroupLayout& L) {
+    long long entries = 2LL * work * n_pen;
+    size_t sort_temp = 0, scan_temp = 0;
+    if (hipcub::DeviceSegmentedRadixSort::SortPairs(
+            nullptr, sort_temp, (long long*)nullptr, (long long*)nullptr, (int*)nullptr,
+            (int*)nullptr, (int)entries, work, (int*)nullptr, (int*)nullptr, 0, 63,
+            (hipStream_t)0) != hipSuccess) {
+        lh::set_error("segmented sort workspace query failed");
+        return LH_ERR_HIP;
+    }
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
+                                         (int)(entries + 1), (hipStream_t)0) != hipSuccess) {
+        lh::set_error("scan workspace query failed");
+        return LH_ERR_HIP;
+    }
+    size_t at = 0;
+    auto take = [&](size_t bytes) { size_t here = at; at += align_up(bytes); return here; };
+    L.info = take(8 * sizeof(int));
+    L.keys = take(entries * sizeof(long long));
+    L.vals = take(entries * sizeof(int));
+    L.sorted_keys = take(entries * sizeof(long long));
+    L.sorted_vals = take(entries * sizeof(int));
+    L.seg_offsets = take((work + 1) * sizeof(int));
+    L.group_flag = take((entries + 1) * sizeof(int));
+    L.group_scan = take((entries + 1) * sizeof(int));
+    L.group_start = take((entries + 1) * sizeof(int));
+    L.om_groups = take((work + 1) * sizeof(int));
+    L.prepared = take((size_t)n_pen * 7 * sizeof(double));
+    L.acc = take((size_t)work * bands * bins * 4 * sizeof(double));
+    L.pre = take((size_t)work * bands * 3 * sizeof(double));
+    L.lambda_bins = take((size_t)work * bands * bins * sizeof(double));
+    L.sort_temp = take(sort_temp);
+    L.scan_temp = take(scan_temp);
+    L.total = at;
+    return LH_OK;
+}
+
+__global__ void o1_offsets(int* offsets, int work, int seg) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w <= work) offsets[w] = w * seg;
+}
+
+int check_grouped(const lh_order1_pencils* pen, const lh_order1_oms* om,
+                  const lh_order1_optics* opt, const lh_order1_merge* merge) {
+    LH_REQUIRE(pen && om && opt && merge, "descriptor is NULL");
+    LH_REQUIRE(pen->count >= 1 && om->work_count >= 1, "no pencils or no OMs");
+    LH_REQUIRE(opt->bands >= 1 && opt->bands <= kMaxBands, "1..%d wavelength bands", kMaxBands);
+    LH_REQUIRE(opt->n_mono >= 1 && opt->n_mono <= kMaxMono && opt->max_power <= 4,
+               "acceptance degree must be at most 4");
+    LH_REQUIRE(opt->bins >= 1 && opt->edges && opt->eedge && opt->itab, "bins and tables needed");
+    LH_REQUIRE(opt->max_path_nodes >= 3 && opt->max_path_nodes <= kMaxPathNodes,
+               "max_path_nodes must lie in 3..%d", kMaxPathNodes);
+    LH_REQUIRE(2LL * om->work_count * pen->count < (1LL << 31) - 1,
+               "too many (OM, pencil) entries; split the OM list");
+    LH_REQUIRE(merge->log_r_step >= 710.0 / ((1 << 20) - 2),
+               "grouped radius/angular resolution exceeds integer key range");
+    return LH_OK;
+}
+
+Group make_group(const lh_order1_pencils* pen, const lh_order1_oms* om,
+                 const lh_order1_optics* opt, const lh_order1_merge* merge, char* base,
+                 const GroupLayout& L) {
+    Group G;
+    G.pen = *pen;
+    G.om = *om;
+    G.opt = *opt;
+    G.merge = *merge;
+    G.screen = pen->parent != nullptr;
+    G.n_pen = pen->count;
+    G.log_ratio = log(1.0 + merge->theta_ratio);
+    G.theta_bins = 2 + (long long)(log(M_PI / merge->theta_lo) / G.log_ratio);
+    G.keys = reinterpret_cast<long long*>(base + L.keys);
+    G.vals = reinterpret_cast<int*>(base + L.vals);
+    G.sorted_keys = reinterpret_cast<long long*>(base + L.sorted_keys);
+    G.sorted_vals = reinterpret_cast<int*>(base + L.sorted_vals);
+    G.seg_offsets = reinterpret_cast<int*>(base + L.seg_offsets);
+    G.group_flag = reinterpret_cast<int*>(base + L.group_flag);
+    G.group_scan = reinterpret_cast<int*>(base + L.group_scan);
+    G.group_start = reinterpret_cast<int*>(base + L.group_start);
+    G.om_groups = reinterpret_cast<int*>(base + L.om_groups);
+    G.prepared = reinterpret_cast<double*>(base + L.prepared);
+    G.acc = reinterpret_cast<double*>(base + L.acc);
+    G.pre = reinterpret_cast<double*>(base + L.pre);
+    G.lambda_bins = reinterpret_cast<double*>(base + L.lambda_bins);
+    G.values = nullptr;
+    G.info = reinterpret_cast<int*>(base + L.info);
+    return G;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lh_order1_set_tables(const double* taylor, const double* tangent,
+                         const double* weighted_sec2) {
+    LH_REQUIRE(taylor && tangent && weighted_sec2, "tables must be set");
+    LH_HIP_CALL(hipMemcpyToSymbol(HIP_SYMBOL(kTaylor), taylor, sizeof(double) * 72));
+    LH_HIP_CALL(hipMemcpyToSymbol(HIP_SYMBOL(kTangent), tangent, sizeof(double) * kGauss));
+    LH_HIP_CALL(hipMemcpyToSymbol(HIP_SYMBOL(kWeightedSec2), weighted_sec2,
+                                  sizeof(double) * kGauss));
+    return LH_OK;
+}
+
+size_t lh_order1_compress_workspace_bytes(const lh_order1_chords* chords,
+                                          const lh_order1_grid* grid) {
+    if (check_grid(chords, grid) != LH_OK) return 0;
+    CompressLayout L;
+    long long nodes = grid->subsegments * grid->ring_nodes;
+    if (compress_layout(grid, chords->count, nodes, L) != LH_OK) return 0;
+    return L.total;
+}
+
+int lh_order1_compress_keys(const lh_order1_chords* chords, const lh_order1_grid* grid, int* info,
+                            void* workspace, size_t workspace_bytes, void* stream) {
+    int status = check_grid(chords, grid);
+    if (status != LH_OK) return status;
+    CompressLayout L;
+    long long nodes = grid->subsegments * grid->ring_nodes;
+    if ((status = compress_layout(grid, chords->count, nodes, L)) != LH_OK) return status;
+    LH_REQUIRE(workspace && workspace_bytes >= L.total && info,
+               "workspace too small: %zu bytes given, %zu needed", workspace_bytes, L.total);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* base = static_cast<char*>(workspace);
+    Compress k = make_compress(chords, grid, base, L);
+    o1_subsegments<<<blocks(chords->count), kThreads, 0, s>>>(k);
+    if ((status = lh::check_launch("o1_subsegments")) != LH_OK) return status;
+    o1_nodes<<<blocks(grid->subsegments), kThreads, 0, s>>>(k);
+    if ((status = lh::check_launch("o1_nodes")) != LH_OK) return status;
+    unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
+                                 * 6ULL * grid->pixel_face * grid->pixel_face;
+    size_t sort_bytes = L.scan_temp - L.sort_temp;
+    LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(base + L.sort_temp, sort_bytes, k.keys,
+                                                   k.sorted_keys, k.ids, k.sorted_ids, (int)nodes,
+                                                   0, bits_for(max_key), s));
+    o1_run_flags<<<blocks(nodes + 1), kThreads, 0, s>>>(k);
+    if ((status = lh::check_launch("o1_run_flags")) != LH_OK) return status;
+    size_t scan_bytes = L.psort_temp - L.scan_temp;
+    LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(base + L.scan_temp, scan_bytes, k.run_flag,
+                                                 k.run_scan, (int)(nodes + 1), s));
+    o1_run_starts<<<blocks(nodes + 1), kThreads, 0, s>>>(k);
+    if ((status = lh::check_launch("o1_run_starts")) != LH_OK) return status;
+    LH_HIP_CALL(hipMemcpyAsync(info, k.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    return LH_OK;
+}
+
+int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_grid* grid,
+                               int runs, lh_order1_pencils* out, int* info, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+    int status = check_grid(chords, grid);
+    if (status != LH_OK) return status;
+    CompressLayout L;
+    long long nodes = grid->subsegments * grid->ring_nodes;
+    if ((status = compress_layout(grid, chords->count, nodes, L)) != LH_OK) return status;
+    LH_REQUIRE(workspace && workspace_bytes >= L.total, "workspace too small");
+    LH_REQUIRE(out && runs >= 1 && runs <= nodes, "invalid pencil count");
+    LH_REQUIRE(out->position && out->direction && out->w0 && out->w2 && out->time && out->major
+                   && out->var_major && out->var_minor && out->pos_cov, "pencil arrays must be set");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* base = static_cast<char*>(workspace);
+    Compress k = make_compress(chords, grid, base, L);
+    o1_rows<<<blocks(runs), kThreads, 0, s>>>(k, runs);
+    if ((status = lh::check_launch("o1_rows")) != LH_OK) return status;
+    if (grid->parent_face == 0) {
+        LH_REQUIRE(out->count >= runs, "pencil capacity %d below %d", out->count, runs);
+        o1_finish<<<blocks(runs), kThreads, 0, s>>>(k.rows, runs, *out);
+        return lh::check_launch("o1_finish");
+    }
+    LH_REQUIRE(out->parent && out->next && info, "screen pencils need parent, next and info");
+    LH_REQUIRE(out->count >= 2 * runs, "screen pencil capacity %d below %d", out->count, 2 * runs);
+    long long* pkeys = reinterpret_cast<long long*>(base + L.pkeys);
+    int* pids = reinterpret_cast<int*>(base + L.pids);
+    long long* spkeys = reinterpret_cast<long long*>(base + L.spkeys);
+    int* spids = reinterpret_cast<int*>(base + L.spids);
+    int* pflag = reinterpret_cast<int*>(base + L.pflag);
+    int* pscan = reinterpret_cast<int*>(base + L.pscan);
+    o1_parent_keys<<<blocks(runs), kThreads, 0, s>>>(k, runs, pkeys, pids);
+    if ((status = lh::check_launch("o1_parent_keys")) != LH_OK) return status;
+    unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
+                                 * 6ULL * grid->pixel_face * grid->pixel_face;
+    size_t psort_bytes = L.total - L.psort_temp;
+    // pencils are in fine-key order: a stable sort by parent key gives (parent, fine) order
+    LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(base + L.psort_temp, psort_bytes, pkeys,
+                                                   spkeys, pids, spids, runs, 0,
+                                                   bits_for(max_key), s));
+    o1_parent_flags<<<blocks(runs + 1), kThreads, 0, s>>>(spkeys, runs, pflag);
+    if ((status = lh::check_launch("o1_parent_flags")) != LH_OK) return status;
+    size_t scan_bytes = L.psort_temp - L.scan_temp;
+    LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(base + L.scan_temp, scan_bytes, pflag, pscan,
+                                                 runs + 1, s));
+    o1_parents<<<blocks(runs), kThreads, 0, s>>>(k.rows, spids, pflag, pscan, runs, *out);
+    if ((status = lh::check_launch("o1_parents")) != LH_OK) return status;
+    o1_parent_links<<<blocks(runs), kThreads, 0, s>>>(pflag, pscan, runs, *out);
+    if ((status = lh::check_launch("o1_parent_links")) != LH_OK) return status;
+    LH_HIP_CALL(hipMemcpyAsync(info + 1, pscan + runs, sizeof(int), hipMemcpyDeviceToDevice, s));
+    return LH_OK;
+}
+
+size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
+                                         const lh_order1_oms* oms,
+                                         const lh_order1_optics* optics) {
+    if (!pencils || !oms || !optics) return 0;
+    GroupLayout L;
+    if (grouped_layout(oms->work_count, pencils->count, optics->bands, optics->bins, L) != LH_OK)
+        return 0;
+    return L.total;
+}
+
+int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
+                         const lh_order1_optics* optics, const lh_order1_merge* merge, int* info,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+    int status = check_grouped(pencils, oms, optics, merge);
+    if (status != LH_OK) return status;
+    GroupLayout L;
+    if ((status = grouped_layout(oms->work_count, pencils->count, optics->bands, optics->bins,
+                                 L)) != LH_OK) return status;
+    LH_REQUIRE(workspace && workspace_bytes >= L.total && info, "workspace too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* base = static_cast<char*>(workspace);
+    Group G = make_group(pencils, oms, optics, merge, base, L);
+    int work = oms->work_count;
+    long long total = 2LL * work * pencils->count;
+    o1_prepare_shape<<<blocks(pencils->count), kThreads, 0, s>>>(G);
+    if ((status = lh::check_launch("o1_prepare_shape")) != LH_OK) return status;
+    dim3 grid(blocks(pencils->count), work);
+    o1_group_keys<<<grid, kThreads, 0, s>>>(G);
+    if ((status = lh::check_launch("o1_group_keys")) != LH_OK) return status;
+    o1_offsets<<<blocks(work + 1), kThreads, 0, s>>>(G.seg_offsets, work, 2 * pencils->count);
+    if ((status = lh::check_launch("o1_offsets")) != LH_OK) return status;
+    size_t sort_bytes = L.scan_temp - L.sort_temp;
+    LH_HIP_CALL(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        base + L.sort_temp, sort_bytes, G.keys, G.sorted_keys, G.vals, G.sorted_vals, (int)total,
+        work, G.seg_offsets, G.seg_offsets + 1, 0, 63, s));
+    o1_group_flags<<<blocks(total + 1), kThreads, 0, s>>>(G, total);
+    if ((status = lh::check_launch("o1_group_flags")) != LH_OK) return status;
+    size_t scan_bytes = L.total - L.scan_temp;
+    LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(base + L.scan_temp, scan_bytes, G.group_flag,
+                                                 G.group_scan, (int)(total + 1), s));
+    o1_group_starts<<<blocks(total + 1), kThreads, 0, s>>>(G, total, work);
+    if ((status = lh::check_launch("o1_group_starts")) != LH_OK) return status;
+    LH_HIP_CALL(hipMemcpyAsync(info, G.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    return LH_OK;
+}
+
+int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
+                           const lh_order1_optics* optics, const lh_order1_merge* merge,
+                           int groups, double* values, int compute_bins, double* charge,
+                           double* bins, void* workspace, size_t workspace_bytes, void* stream) {
+    int status = check_grouped(pencils, oms, optics, merge);
+    if (status != LH_OK) return status;
+    GroupLayout L;
+    if ((status = grouped_layout(oms->work_count, pencils->count, optics->bands, optics->bins,
+                                 L)) != LH_OK) return status;
+    LH_REQUIRE(workspace && workspace_bytes >= L.total, "workspace too small: %zu < %zu",
+               workspace_bytes, L.total);
+    LH_REQUIRE(charge != nullptr && (!compute_bins || bins != nullptr),
+               "charge (and bins) must be set");
+    LH_REQUIRE(groups >= 0 && (groups == 0 || values != nullptr), "values buffer needed");
+    LH_REQUIRE(optics->mut_min > 0 && optics->speed_max > 0, "mut_min and speed_max must be set");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* base = static_cast<char*>(workspace);
+    Group G = make_group(pencils, oms, optics, merge, base, L);
+    G.compute_bins = compute_bins;
+    G.charge = charge;
+    G.bins = bins;
+    G.values = values;
+    G.mut_min = optics->mut_min;
+    G.speed_max = optics->speed_max;
+    if (groups > 0) {
+        o1_group_sums<<<blocks(groups), kThreads, 0, s>>>(G, groups);
+        if ((status = lh::check_launch("o1_group_sums")) != LH_OK) return status;
+    }
+    o1_batches<<<oms->work_count, kThreads, 0, s>>>(G);
+    return lh::check_launch("o1_batches");
+}
+
+}  // extern "C"

@@ -21,7 +21,9 @@ Parity pins (tests/test_order1_oracle.py):
 The grouping discretisation itself (cells, pixels, merge widths) has no
 independent reference value: the grouped charge is pinned against the direct
 integral only to the accuracy of that discretisation (0.2 % off the cone), so
-the exact grouped numbers are "parity unpinned" beyond that level.
+the exact grouped numbers are pinned, beyond that level, by the reference's
+published demonstration total (order 1 of the README event, 50.68 pe), which
+the restatement reproduces to the quoted digits.
 """
 from __future__ import annotations
 
@@ -52,8 +54,8 @@ class PromptSettings:
     direct_bins: int = 4
     max_path_nodes: int = 2048
     cell_m: float = 0.5
-    pixel_face: int = 96
-    ring_nodes: int = 192
+    pixel_face: int = 32
+    ring_nodes: int = 96
     smoothing_factor: float = 6.0
     merge_theta_min: float = 2e-3
     merge_theta_ratio: float = 0.08

@@ -33,6 +33,7 @@ def test_python_binding_covers_every_symbol():
     import lighthit_amd.ge2  # noqa: F401  registers its entry points
     import lighthit_amd.ballistic  # noqa: F401
     import lighthit_amd.compile  # noqa: F401
+    import lighthit_amd.order1  # noqa: F401
     assert set(declared()) <= set(_lib.SIGNATURES)
 
 
