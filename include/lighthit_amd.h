@@ -352,6 +352,31 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
                            int groups, double* values, int compute_bins, double* charge,
                            double* bins, void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Band fold of the order >= 2 pack (one-time table preparation).
+ *
+ * Replaces fold_bundle (band_fold.py:155-208): coefficients (bands,
+ * intervals, degree+1, pairs, 4, freq) complex64 of the per-wavelength pack
+ * -> out (rank, intervals*subdivide, degree+1, pairs, 4, freq) complex64 with
+ * common attenuation absorption_ref and flight speed speed_ref. mixing is
+ * BandFold.interpolation (degree+1, bands, rank); inverse holds the 4x4
+ * inverse Vandermonde matrix of every new interval (node values -> t^3..t^0).
+ */
+typedef struct lh_fold_input {
+    const void* coefficients;
+    const double* log_r;        /* (intervals + 1) */
+    const double* new_log_r;    /* (intervals * subdivide + 1) */
+    const double* omega;        /* (freq) */
+    const double* absorption;   /* (bands) */
+    const double* speed;        /* (bands) */
+    const double* mixing;       /* (degree + 1, bands, rank) */
+    const double* inverse;      /* (intervals * subdivide, 4, 4) */
+    double absorption_ref, speed_ref;
+    int bands, intervals, degree, pairs, freq, rank, subdivide;
+} lh_fold_input;
+
+int lh_fold_bundle(const lh_fold_input* input, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,7 +97,10 @@ class DevicePack:
 
     def __init__(self, bundle, device="cuda"):
         import torch
-        coefficients = np.asarray(bundle["coefficients"])
+        coefficients = bundle["coefficients"]
+        resident = isinstance(coefficients, torch.Tensor)
+        if not resident:
+            coefficients = np.asarray(coefficients)
         if coefficients.ndim != 6 or coefficients.shape[4] != 4:
             raise ValueError("coefficients must be (bands, intervals, degree+1, pairs, 4, frequency)")
         bands, intervals, ndeg, npair, _, nfreq = coefficients.shape
@@ -122,7 +125,10 @@ class DevicePack:
         self.radial_low = float(np.exp(self.log_r[0]))
         self.radial_high = float(np.exp(self.log_r[-1]))
         as_dev = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
-        self.d_coef = as_dev(coefficients.astype(np.complex64, copy=False), torch.complex64)
+        if resident:
+            self.d_coef = coefficients.to(device=device, dtype=torch.complex64).contiguous()
+        else:
+            self.d_coef = as_dev(coefficients.astype(np.complex64, copy=False), torch.complex64)
         self.d_log_r = as_dev(self.log_r, torch.float64)
         self.d_absorption = as_dev(self.absorption, torch.float64)
         self.d_speed = as_dev(self.speed, torch.float64)
@@ -131,7 +137,7 @@ class DevicePack:
                             self.d_absorption.data_ptr(), self.d_speed.data_ptr(),
                             self.d_omega.data_ptr(), bands, intervals, self.degree,
                             self.acceptance_degree, nfreq, int(self.flight))
-        self.nbytes = coefficients.size * 8
+        self.nbytes = int(np.prod(coefficients.shape)) * 8
         self._band_stride = intervals * ndeg * npair * 4 * nfreq * 8
 
     def band(self, j):
