@@ -377,6 +377,18 @@ typedef struct lh_fold_input {
 
 int lh_fold_bundle(const lh_fold_input* input, void* out, void* stream);
 
+/*
+ * Per-OM bounds of a source's segments: distance (n_receivers) to the nearest
+ * point of any segment (min_distance_to_segments, _prompt_numba.py:2221) and,
+ * when earliest is set, the earliest arrival start_ns + flight * distance
+ * (earliest_element_arrival, _shower_basis.py:51-78; flight = n_g,min / c).
+ * Either output may be NULL.
+ */
+int lh_segment_bounds(const double* receivers, int n_receivers, const double* starts,
+                      const double* directions, const double* lengths, const double* start_ns,
+                      int n_segments, double flight, double* distance, double* earliest,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,14 @@ class CompiledBand:
     reference_ns: float
     summary: dict = field(default_factory=dict)
 
+    def to_points_m(self):
+        """Point positions in detector coordinates (host array)."""
+        p = self.points.double().cpu().numpy()
+        f = self.frame
+        return (np.asarray(f.centre_m)[None, :] + p[:, 2:3] * np.asarray(f.axis)[None, :]
+                + p[:, 0:1] * np.asarray(f.first)[None, :]
+                + p[:, 1:2] * np.asarray(f.second)[None, :])
+
     def to_source(self):
         """The reference's ``AxialSource`` with host arrays (complex128 channels)."""
         points = self.points.double().cpu().numpy()

@@ -182,6 +182,32 @@ class DeviceSource:
                               self.azimuthal_degree, nfreq)
 
 
+class DeviceBandSource:
+    """Node-band sources compiled on the device (:class:`lighthit_amd.compile.CompiledBand`)
+    sharing one point set, in the layout of :class:`DeviceSource`."""
+
+    def __init__(self, bands, frequencies=None):
+        import torch
+        bands = list(bands)
+        first = bands[0]
+        self.frame = first.frame
+        self.degree = int(first.degree)
+        self.azimuthal_degree = int(first.azimuthal_degree)
+        npoints, nchan, nfreq = first.channels.shape
+        if frequencies is not None:
+            nfreq = int(frequencies)
+        channels = [b.channels if frequencies is None else b.channels[..., :nfreq] for b in bands]
+        self.d_points = first.points.contiguous()
+        self.d_channels = torch.stack(channels).contiguous()
+        self.reference_ns = np.asarray([float(b.reference_ns) for b in bands])
+        self.d_reference = torch.as_tensor(self.reference_ns, device=self.d_points.device)
+        self.bands, self.points, self.nfreq = len(bands), npoints, nfreq
+        self.local_points = None
+        self.struct = _Source(self.d_points.data_ptr(), self.d_channels.data_ptr(),
+                              self.d_reference.data_ptr(), npoints, nchan, self.degree,
+                              self.azimuthal_degree, nfreq)
+
+
 class DeviceReceivers:
     """OM positions, looks and detector scales in a source's frame, on the device."""
 

@@ -1,0 +1,983 @@
+"""The shower route: ``ShowerTransport(kernel, data)(shower)`` on the GPU.
+
+Host mirror of ``shower/api.py`` (``ShowerConfig``, ``ShowerTransport``,
+``transport_shower``, ``shower/api.py:49-539``), ``shower/ge2.py``
+(``node_sources``, ``collapse``, ``delay_groups``), the prompt orders of
+``transport_prompt`` that the route uses (``prompt.py:500-947``: first-entry
+order 0, grouped order 1 with fused bands and the global screen), the radial
+window screens (``shower_basis.py:112-135, 388-439``) and the response type
+(``transport.py:197-385``). Every per-element and per-(OM, point) computation
+runs in the library: order 0 (``lh_order0_*``), order 1 (``lh_order1_*``),
+the source compile (``lh_compile_*``), the order >=2 apply (``lh_ge2_apply``),
+the readout (``lh_readout_bins``), the fold (``lh_fold_bundle``) and the
+segment bounds (``lh_segment_bounds``). The host keeps the route's control
+flow: masks, thresholds, the predictor fallback and the response.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import warnings
+from dataclasses import asdict, dataclass, field, replace
+from pathlib import Path
+from time import perf_counter, process_time
+
+import numpy as np
+
+from . import _lib
+from .axial import AxisFrame
+from .model import WavelengthQuadrature, acceptance_alpha
+from .order1 import PromptSettings
+
+C_M_PER_NS = 0.299792458
+#: RTE frequency grid of the prepared tables and the readout grid (``shower/api.py:31-32``).
+FREQUENCIES_PER_NS = np.linspace(0.0, 1.2, 65)
+READOUT_PER_NS = np.linspace(0.0, 1.2, 321)
+
+_lib.SIGNATURES["lh_segment_bounds"] = (ctypes.c_int, [
+    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p])
+
+
+# --------------------------------------------------------------- configuration
+
+@dataclass(frozen=True)
+class RouteConfig:
+    """The ``KernelConfig`` fields the shower route reads (``transport.py:40-140``)."""
+    omega_per_ns: np.ndarray = field(default_factory=lambda: FREQUENCIES_PER_NS.copy())
+    readout_omega_per_ns: np.ndarray | None = field(
+        default_factory=lambda: READOUT_PER_NS.copy())
+    relative_time_edges_ns: np.ndarray = field(
+        default_factory=lambda: np.arange(-60.0, 745.0, 5.0))
+    wavelength_nodes: int = 3
+    wavelength_range_nm: tuple | None = None
+    scattering_degree: int = 8
+    source_degree: int = 8
+    azimuthal_degree: int = 2
+    radial_range_m: tuple = (2.0, 150.0)
+    radial_nodes: int = 48
+    threshold_pe: float = 0.01
+    detail_threshold_pe: float | None = None
+
+    def __post_init__(self):
+        edges = np.asarray(self.relative_time_edges_ns, float)
+        if edges.ndim != 1 or len(edges) < 2 or np.any(np.diff(edges) <= 0):
+            raise ValueError("relative_time_edges_ns must increase")
+        if not self.threshold_pe >= 0:
+            raise ValueError("threshold_pe must be nonnegative")
+        object.__setattr__(self, "relative_time_edges_ns", edges)
+        object.__setattr__(self, "omega_per_ns", np.asarray(self.omega_per_ns, float))
+
+
+class RouteKernel:
+    """Medium, detector and configuration of a route (the parts of
+    ``TransportKernel`` the shower route uses)."""
+
+    def __init__(self, medium, detector, config=None):
+        self.medium = medium
+        self.detector = detector
+        self.config = config or RouteConfig()
+        low, high = self.config.wavelength_range_nm or medium.wavelength_range_nm
+        self.wavelength = WavelengthQuadrature.gauss_legendre(low, high,
+                                                              int(self.config.wavelength_nodes))
+        self._acceptance = None
+
+    def acceptance(self):
+        """Acceptance Legendre coefficients (``transport.py:496-530``)."""
+        if self._acceptance is None:
+            alpha = acceptance_alpha(self.detector,
+                                     scattering_degree=int(self.config.scattering_degree))
+            self._acceptance = {"alpha": alpha, "degree": len(alpha) - 1}
+        return self._acceptance
+
+    def subset(self, receivers, threshold_pe=None):
+        """The kernel on OMs ``receivers`` (``muon/universal_primary.py`` ``subset_kernel``)."""
+        other = RouteKernel(self.medium, self.detector.subset(np.asarray(receivers)),
+                            self.config if threshold_pe is None
+                            else replace(self.config, threshold_pe=float(threshold_pe)))
+        other.wavelength = self.wavelength
+        other._acceptance = self._acceptance
+        return other
+
+
+@dataclass(frozen=True)
+class ShowerBasis:
+    """The order >=2 axial basis (``shower_basis.py:30-99``)."""
+    cell_m: float = 0.25
+    axial_order: int = 3
+    transverse: str = "stencil"
+    stencil_m: float = 0.2
+    stencil_threshold_m: float = 0.03
+    time_step_ns: float = 0.02
+    radial_high_m: float | None = None
+    radial_tail_tolerance_pe: float = 1e-4
+    radial_tail_policy: str = "warn"
+
+    def __post_init__(self):
+        if self.radial_tail_policy not in ("warn", "raise"):
+            raise ValueError("radial_tail_policy must be 'warn' or 'raise'")
+
+    def options(self):
+        from .compile import BasisOptions
+        return BasisOptions(self.cell_m, self.axial_order, self.transverse, self.stencil_m,
+                            self.stencil_threshold_m, self.time_step_ns, "linear")
+
+
+@dataclass(frozen=True)
+class ShowerPrompt(PromptSettings):
+    """Orders 0 and 1 of the route (``shower/api.py:36-45``: first-entry direct light,
+    fused bands at 96 pixels per face, 192 ring nodes)."""
+    max_distance_m: float | None = None
+    min_distance_m: float = 0.1
+
+
+@dataclass(frozen=True)
+class ShowerConfig:
+    """Numerical choices of the shower route (``shower/api.py:49-94``)."""
+    basis: ShowerBasis = field(default_factory=ShowerBasis)
+    prompt: ShowerPrompt = field(default_factory=ShowerPrompt)
+    frequency_nodes: int = 65
+    band_rank: int = 3
+    fold_subdivide: int = 1
+    predictor_cell_m: float = 2.0
+    predictor_safety: float = 2.0
+    prompt_delay_ns: float = 20.0
+    delayed_cell_ns: float = 20.0
+
+    def __post_init__(self):
+        if not 1 <= int(self.band_rank) <= 9 or int(self.fold_subdivide) < 1:
+            raise ValueError("band_rank must lie in 1..9 and fold_subdivide be positive")
+        if (not self.predictor_safety >= 1 or not self.predictor_cell_m > 0
+                or not self.prompt_delay_ns >= 0 or not self.delayed_cell_ns > 0):
+            raise ValueError("predictor_safety >= 1 and positive predictor/delay cells required")
+
+
+# ------------------------------------------------------------------- response
+
+@dataclass
+class TransportComponent:
+    """One selected signal (``transport.py:197-207``)."""
+    charge_pe: np.ndarray
+    bins_pe: np.ndarray
+    time_origin_ns: np.ndarray
+    relative_time_edges_ns: np.ndarray
+    active: np.ndarray
+
+    @property
+    def rate_pe_per_ns(self):
+        return self.bins_pe / np.diff(self.relative_time_edges_ns)[None, :]
+
+
+@dataclass
+class HybridTransportResponse:
+    """Physical-time orders 0+1 and frequency-domain order >=2 (``transport.py:332-385``)."""
+    detector: object
+    omega_per_ns: np.ndarray
+    relative_time_edges_ns: np.ndarray
+    spectrum_ge2_pe: np.ndarray
+    components_pe: np.ndarray
+    charge_components_pe: np.ndarray
+    time_origin_ns: np.ndarray
+    active: np.ndarray
+    method: str
+    metadata: dict
+    computed_orders: tuple = (0, 1, 2)
+
+    @property
+    def charge_pe(self):
+        return self.charge_components_pe.sum(axis=1)
+
+    @property
+    def bins_pe(self):
+        return self.components_pe.sum(axis=2)
+
+    def select(self, component="all"):
+        choices = {0: (0,), 1: (1,), 2: (2,), "0": (0,), "1": (1,), ">=2": (2,),
+                   "0+1": (0, 1), "all": (0, 1, 2)}
+        if component not in choices:
+            raise ValueError("component must be 0, 1, >=2, 0+1 or all")
+        selected = choices[component]
+        return TransportComponent(self.charge_components_pe[:, selected].sum(axis=1),
+                                  self.components_pe[:, :, selected].sum(axis=2),
+                                  self.time_origin_ns, self.relative_time_edges_ns, self.active)
+
+    def save(self, path):
+        """``.npz`` with schema ``lighthit/hybrid-response/1``."""
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez_compressed(
+            path, omega_per_ns=self.omega_per_ns,
+            relative_time_edges_ns=self.relative_time_edges_ns,
+            spectrum_ge2_pe=self.spectrum_ge2_pe, components_pe=self.components_pe,
+            charge_components_pe=self.charge_components_pe, time_origin_ns=self.time_origin_ns,
+            active=self.active,
+            metadata=json.dumps({"schema": "lighthit/hybrid-response/1", "method": self.method,
+                                 "computed_orders": list(self.computed_orders), **self.metadata},
+                                default=str))
+        return path
+
+
+# -------------------------------------------------------------- small helpers
+
+def segment_bounds(positions, elements, flight=None, device="cuda"):
+    """Distance of every OM to the nearest segment and, with ``flight`` (n_g/c), the
+    earliest arrival (``lh_segment_bounds``)."""
+    import torch
+
+    def up(a):
+        return torch.as_tensor(np.ascontiguousarray(a, float), device=device)
+    rec = up(positions)
+    starts, dirs = up(elements.start_m), up(elements.direction)
+    lengths, t0 = up(elements.length_m), up(elements.start_ns)
+    n = len(positions)
+    distance = torch.empty(n, dtype=torch.float64, device=device)
+    earliest = torch.empty(n, dtype=torch.float64, device=device) if flight is not None else None
+    _lib.call("lh_segment_bounds", _lib.ptr(rec), n, _lib.ptr(starts), _lib.ptr(dirs),
+              _lib.ptr(lengths), _lib.ptr(t0), len(elements),
+              float(flight) if flight is not None else 0.0, _lib.ptr(distance),
+              _lib.ptr(earliest), _lib.stream_handle())
+    return distance.cpu().numpy(), (earliest.cpu().numpy() if earliest is not None else None)
+
+
+def delay_groups(elements, *, prompt_delay_ns, cell_ns, window_end_ns, group_index_max):
+    """Prompt / delayed-in-window / beyond-window steps (``shower/ge2.py:122-147``)."""
+    c = C_M_PER_NS
+    first = int(np.argmin(elements.start_ns))
+    distance = np.linalg.norm(elements.start_m - elements.start_m[first], axis=1)
+    delay = elements.start_ns - elements.start_ns[first] - distance / c
+    prompt = delay <= prompt_delay_ns
+    beyond = ~prompt & (delay - (group_index_max - 1.0) * distance / c > window_end_ns)
+    within = ~prompt & ~beyond
+    groups = []
+    if np.any(within):
+        cells = np.floor(delay / cell_ns).astype(np.int64)
+        for cell in np.unique(cells[within]):
+            mask = within & (cells == cell)
+            groups.append((mask, float(delay[mask].min())))
+    return prompt, groups, beyond
+
+
+def spectral_attenuation(kernel):
+    """Absorption factor of the detected spectrum vs distance (``shower_basis.py:388-405``)."""
+    wavelengths = np.asarray(kernel.wavelength.wavelength_nm, float)
+    weights = (np.asarray(kernel.wavelength.weight_nm, float)
+               * np.asarray(kernel.detector.spectral_weight(wavelengths), float)
+               / wavelengths ** 2)
+    absorption = np.asarray(kernel.medium.sample(wavelengths)["absorption_per_m"], float)
+    lowest = float(absorption.min())
+    radius = np.geomspace(1e-3, 1e5, 1025)
+    log_factor = (np.log(np.exp(-np.outer(radius, absorption - lowest)) @ weights
+                         / weights.sum()) - lowest * radius)
+    return lambda distance: np.exp(np.interp(distance, radius, log_factor))
+
+
+def window_report(kernel, points, photons, positions, low, high, attenuation, efficiency):
+    """Usable basis points per OM and the far-tail estimate (``shower_basis.py:408-439``)."""
+    peak_acceptance = float(np.max(np.abs(np.asarray(
+        kernel.detector.angular_acceptance(np.linspace(-1, 1, 257)), float))))
+    area = np.asarray(kernel.detector.effective_area_m2, float)
+    n = len(positions)
+    used = np.zeros(n, np.int64)
+    far = np.zeros(n, np.int64)
+    near = np.zeros(n, np.int64)
+    tail = np.zeros(n)
+    block = max(1, min(n, 1_000_000 // max(len(points), 1)))
+    for start in range(0, n, block):
+        stop = min(start + block, n)
+        radius = np.linalg.norm(points[None] - positions[start:stop, None], axis=2)
+        near[start:stop] = np.count_nonzero(radius < low, axis=1)
+        far_mask = radius > high
+        far[start:stop] = np.count_nonzero(far_mask, axis=1)
+        used[start:stop] = len(points) - near[start:stop] - far[start:stop]
+        distance = np.where(far_mask, radius, 1.)
+        tail[start:stop] = (10 * area[start:stop] * peak_acceptance * efficiency
+                            * np.sum(np.where(far_mask, photons[None] * attenuation(distance)
+                                              / (4 * np.pi * distance ** 2), 0.), axis=1))
+    return used, far, near, tail
+
+
+def screen_radial_tail(tail_pe, basis, high_m):
+    """Warn (or raise) for light estimated beyond the cache (``shower_basis.py:112-135``)."""
+    tail_pe = np.asarray(tail_pe, float)
+    over = np.flatnonzero(tail_pe > basis.radial_tail_tolerance_pe)
+    entry = {"policy": basis.radial_tail_policy, "radial_high_m": float(high_m),
+             "tolerance_pe": float(basis.radial_tail_tolerance_pe),
+             "max_estimate_pe": float(tail_pe.max(initial=0.0)),
+             "oms_above_tolerance": over.tolist(), "estimate_pe": tail_pe[over].tolist()}
+    if len(over):
+        message = (f"{len(over)} OMs have estimated light beyond {high_m:g} m above "
+                   f"{basis.radial_tail_tolerance_pe:g} pe; largest estimate "
+                   f"{entry['max_estimate_pe']:g} pe")
+        if basis.radial_tail_policy == "raise":
+            raise ValueError(message)
+        warnings.warn(message + " (not computed; listed in metadata['radial_tail'])",
+                      RuntimeWarning, stacklevel=3)
+    return entry
+
+
+@dataclass
+class CollapsedSource:
+    """Zero-frequency source reduced to one point per predictor cell."""
+    frame: AxisFrame
+    z_m: np.ndarray
+    transverse_m: np.ndarray
+    channels: np.ndarray
+    degree: int
+    azimuthal_degree: int
+    reference_ns: float
+
+    def points_m(self):
+        return (self.frame.centre_m[None, :] + self.z_m[:, None] * self.frame.axis[None, :]
+                + self.transverse_m[:, 0:1] * self.frame.first[None, :]
+                + self.transverse_m[:, 1:2] * self.frame.second[None, :])
+
+
+def collapse(bands, cell_m):
+    """One point per ``cell_m`` cube of the frame (``shower/ge2.py:70-96``)."""
+    reduced = []
+    for band in bands:
+        points = band.points.cpu().numpy()
+        z, transverse = points[:, 2], points[:, :2]
+        channels = band.channels[:, :, :1].cpu().numpy().astype(np.complex128)
+        weight = np.abs(channels[:, 0, 0])
+        keys = np.floor(np.column_stack((z, transverse)) / float(cell_m)).astype(np.int64)
+        _, inverse = np.unique(keys, axis=0, return_inverse=True)
+        inverse = np.asarray(inverse).ravel()
+        chunks = [np.flatnonzero(inverse == g) for g in range(int(inverse.max()) + 1)]
+        zz, tt, cc = [], [], []
+        for chunk in chunks:
+            total = weight[chunk].sum()
+            share = (weight[chunk] / total if total > 0 else np.full(len(chunk), 1.0 / len(chunk)))
+            zz.append(np.sum(share * z[chunk]))
+            tt.append(share @ transverse[chunk])
+            cc.append(channels[chunk].sum(axis=0))
+        reduced.append(CollapsedSource(band.frame, np.asarray(zz), np.asarray(tt),
+                                       np.ascontiguousarray(np.asarray(cc)), band.degree,
+                                       band.azimuthal_degree, band.reference_ns))
+    return reduced
+
+
+# ---------------------------------------------------------------- order >= 2
+
+def _same_points(bands):
+    import torch
+    first = bands[0].points
+    return all(b.points.shape == first.shape and torch.equal(b.points, first) for b in bands[1:])
+
+
+def apply_bands(pack, bands, positions, looks, alpha, area, window, *, frequencies=None):
+    """Folded order >=2 spectrum (frequency, OM) complex128 of node-band sources
+    (device :class:`CompiledBand` or host sources), ``apply_folded`` semantics."""
+    from .ge2 import DeviceBandSource, DeviceReceivers, DeviceSource, apply_device
+    nfreq = pack.nfreq if frequencies is None else int(frequencies)
+    if not len(positions):
+        return np.zeros((nfreq, 0), complex)
+    area = np.asarray(area, float)
+    device_bands = hasattr(bands[0], "points") and not callable(getattr(bands[0], "points"))
+    if device_bands and _same_points(bands):
+        source = DeviceBandSource(bands, frequencies)
+        receivers = DeviceReceivers(source.frame, positions, looks,
+                                    np.repeat(area[None, :], len(bands), 0))
+        out = apply_device(pack, source, receivers, alpha, window, frequencies=frequencies)
+        return out.cpu().numpy().astype(np.complex128)
+    if not device_bands and all(np.array_equal(b.points_m(), bands[0].points_m())
+                                for b in bands[1:]):
+        source = DeviceSource(bands)
+        receivers = DeviceReceivers(source.frame, positions, looks,
+                                    np.repeat(area[None, :], len(bands), 0))
+        out = apply_device(pack, source, receivers, alpha, window, frequencies=frequencies)
+        return out.cpu().numpy().astype(np.complex128)
+    total = 0
+    for j, band in enumerate(bands):
+        source = DeviceBandSource([band], frequencies) if device_bands else DeviceSource([band])
+        receivers = DeviceReceivers(source.frame, positions, looks, area[None, :])
+        out = apply_device(pack, source, receivers, alpha, window, frequencies=frequencies, band=j)
+        total = total + out.cpu().numpy().astype(np.complex128)
+    return total
+
+
+def node_sources(kernel, chords, basis, fold, omega, frame, mask=None, omega_device=None):
+    """Node-band sources compiled on the device, scaled by ``fold.scale``
+    (``shower/ge2.py:35-67``)."""
+    import torch
+    from .compile import CompiledBand, compile_band
+    wavelength = np.asarray(kernel.wavelength.wavelength_nm, float)
+    compiled = {}
+    for band in fold.bands:
+        item = compile_band(chords, frame, int(kernel.config.source_degree), omega,
+                            azimuthal_degree=int(kernel.config.azimuthal_degree),
+                            options=basis.options(), wavelength_nm=float(wavelength[band]),
+                            phase_index=float(fold.phase_index[band]), mask=mask,
+                            scale=float(fold.scale[band]), omega_device=omega_device)
+        if item is not None:
+            compiled[band] = item
+    if not compiled:
+        raise ValueError("no element emits in any node band")
+    template = next(iter(compiled.values()))
+    return [compiled.get(band) or CompiledBand(template.frame, template.points,
+                                               torch.zeros_like(template.channels),
+                                               template.degree, template.azimuthal_degree,
+                                               template.reference_ns, dict(template.summary))
+            for band in fold.bands]
+
+
+# ------------------------------------------------------------------- prompt
+
+@dataclass
+class PromptResult:
+    """Orders 0 and 1 of the route (the fields of ``PromptTransportResponse`` it reads)."""
+    charge_orders_pe: np.ndarray      # (N, 2)
+    bins_orders_pe: np.ndarray        # (N, nb, 2)
+    active: np.ndarray
+    computed: np.ndarray
+    time_origin_ns: np.ndarray
+    level: np.ndarray
+    metadata: dict
+
+    @property
+    def charge_pe(self):
+        return self.charge_orders_pe.sum(axis=1)
+
+
+def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra_charge=None,
+                     time_origin_ns=None, compute_bins_for_all=False, device="cuda"):
+    """Orders 0 and 1 of a spectral-cone element source (``prompt.py:500-947`` with
+    the shower route's settings: first-entry order 0, grouped fused order 1)."""
+    from .ballistic import DeviceElements, Order0
+    from .compile import DeviceChords
+    from .order1 import DeviceOMs, spectral_order1, spectral_tables
+    began = perf_counter()
+    timings = {}
+    if compute_bins_for_all and settings.screen:
+        settings = replace(settings, screen=False)
+    tables = spectral_tables(kernel.medium, kernel.detector, kernel.wavelength)
+    alpha = np.ascontiguousarray(kernel.acceptance()["alpha"], float)
+    detector = kernel.detector
+    positions = np.ascontiguousarray(detector.positions_m, float)
+    n = len(positions)
+    extra = np.zeros(n) if selection_extra_charge is None else np.asarray(selection_extra_charge,
+                                                                          float)
+    if extra.shape != (n,) or not np.isfinite(extra).all():
+        raise ValueError("selection_extra_charge must be finite per OM")
+    looks = detector.looks
+    areas = np.asarray(detector.effective_area_m2, float)
+    edges = np.asarray(kernel.config.relative_time_edges_ns, float)
+    threshold = float(kernel.config.threshold_pe)
+    g = float(kernel.medium.g)
+    # threshold of the continuous spectrum (prompt.py:543-552)
+    low, high = kernel.config.wavelength_range_nm or kernel.medium.wavelength_range_nm
+    native = kernel.medium.wavelength_nm
+    critical = np.r_[low, native[(native > low) & (native < high)], high]
+    threshold_phase = float(np.max(kernel.medium.sample(critical)["phase_index"]))
+    valid = elements.beta * threshold_phase > 1
+    if not np.any(valid):
+        raise ValueError("source is below Cherenkov threshold over the wavelength range")
+    if not np.all(valid):
+        elements = elements.subset(valid)
+    fastest = float(np.min(tables["group_index"]))
+    if time_origin_ns is not None:
+        origins = np.asarray(time_origin_ns, float)
+        if origins.shape != (n,) or not np.isfinite(origins).all():
+            raise ValueError("time_origin_ns must be finite with one value per OM")
+    else:
+        first = int(np.argmin(elements.start_ns))
+        origins = (float(elements.start_ns[first])
+                   + np.linalg.norm(positions - elements.start_m[first], axis=1)
+                   / (C_M_PER_NS / fastest))
+    # module selection (prompt.py:596-626)
+    stage = perf_counter()
+    max_distance = (float(kernel.config.radial_range_m[1]) if settings.max_distance_m is None
+                    else float(settings.max_distance_m))
+    distance, _ = segment_bounds(positions, elements, None, device)
+    inside = distance <= max_distance
+    density = (elements.coefficient0[:, None] / tables["wavelength_nm"][None, :] ** 2
+               + elements.coefficient2[:, None]
+               / (tables["wavelength_nm"][None, :] ** 2 * tables["phase_index"][None, :] ** 2))
+    emitted = float(np.sum(tables["weight_nm"][None, :] * np.maximum(density, 0.0)))
+    peak_acceptance = float(np.max(np.abs(np.asarray(
+        detector.angular_acceptance(np.linspace(-1, 1, 257)), float))))
+    far = np.maximum(distance, 1e-9)
+    upper = (10 * max(emitted, 0.0) * areas * peak_acceptance
+             * float(np.max(tables["efficiency"]))
+             * np.exp(-float(np.min(tables["mua"])) * far) / (4 * np.pi * far ** 2))
+    unsafe = (~inside) & (upper >= threshold)
+    if np.any(unsafe):
+        raise ValueError(f"{int(unsafe.sum())} OMs beyond max_distance_m={max_distance:g} have a "
+                         "conservative estimate above threshold; increase max_distance_m")
+    too_close = inside & (distance < settings.min_distance_m)
+    if np.any(too_close):
+        raise ValueError(f"{int(too_close.sum())} OMs are closer than min_distance_m="
+                         f"{settings.min_distance_m:g} m to the source")
+    work = np.flatnonzero(inside)
+    timings["selection_seconds"] = perf_counter() - stage
+    # order 0 (first entry)
+    stage = perf_counter()
+    if order0 is None:
+        order0 = Order0(kernel.medium, detector, alpha, device=device)
+    device_elements = DeviceElements(elements, device)
+    charge = np.zeros((n, 2))
+    if len(work):
+        q0 = order0.charge(device_elements)
+        charge[work, 0] = q0[work]
+    timings["order0_charge_seconds"] = perf_counter() - stage
+    # order 1
+    stage = perf_counter()
+    chords = DeviceChords(elements, device)
+    oms = DeviceOMs(positions, looks, areas, origins, device)
+    q1, bins1, level = spectral_order1(chords, oms, work, tables, alpha, g, edges, threshold,
+                                       charge[:, 0], selection_extra_charge, settings)
+    charge[:, 1] = q1
+    timings["order1_seconds"] = perf_counter() - stage
+    computed = inside.copy()
+    active = computed & (charge.sum(axis=1) + extra >= threshold)
+    if threshold == 0:
+        active = computed.copy()
+    if selection_extra_charge is not None and np.any(active & (level < 2)):
+        raise ValueError("prompt screen omitted an OM selected by the combined charge")
+    stage = perf_counter()
+    bins = np.zeros((n, len(edges) - 1, 2))
+    index = work if compute_bins_for_all else np.flatnonzero(active)
+    if len(index):
+        bins[index, :, 0] = order0.bins(device_elements, index, origins[index], edges)
+        bins[index, :, 1] = bins1[index]
+    timings["order0_bins_seconds"] = perf_counter() - stage
+    timings["elapsed_seconds"] = perf_counter() - began
+    metadata = {"computed_orders": [0, 1], "prompt_algorithm": "grouped",
+                "max_distance_m": max_distance, "computed_modules": int(computed.sum()),
+                "active_modules": int(active.sum()), "timings": timings,
+                "order1_level": level.tolist(),
+                "ballistic": "first-entry", "source_elements": len(elements)}
+    return PromptResult(charge, bins, active, computed, origins, level, metadata)
+
+
+# -------------------------------------------------------------------- route
+
+@dataclass(frozen=True)
+class ShowerData:
+    """Tables of the route (``shower/data.py:26-82``): the order >=2 pack directory (or
+    an in-memory :class:`KernelPack`) and an optional prepared fold."""
+    ge2_pack: object
+    ge2_band_fold: object = None
+    manifest: Path | None = None
+
+    @classmethod
+    def of(cls, ge2_pack, ge2_band_fold=None):
+        return cls(ge2_pack, ge2_band_fold)
+
+    @classmethod
+    def load(cls, path):
+        """Read ``shower-data.json`` (``ge2_pack``, optional ``ge2_band_fold``)."""
+        path = Path(path).expanduser()
+        if path.is_dir():
+            path = path / "shower-data.json"
+        entries = json.loads(path.read_text())
+        if "ge2_pack" not in entries:
+            raise ValueError(f"{path}: missing entries ['ge2_pack']")
+        base = path.parent
+        fold = entries.get("ge2_band_fold")
+        return cls((base / entries["ge2_pack"]).resolve(),
+                   None if fold is None else (base / fold).resolve(), path.resolve())
+
+
+class ShowerTransport:
+    """Transport for shower events on one detector (``shower/api.py:115-220``).
+
+    ``data`` is a :class:`ShowerData`, a manifest path, or a
+    :class:`lighthit_amd.fold.KernelPack`. The pack is folded onto the node
+    bands once (on the GPU) and stays resident with the order-0 optics and
+    the readout basis.
+    """
+
+    def __init__(self, kernel, data=None, *, config=None, threads=None, device="cuda"):
+        from .fold import KernelPack
+        if data is None:
+            raise ValueError("the shower route needs its order >=2 pack (ShowerData)")
+        if isinstance(data, KernelPack):
+            data = ShowerData(data)
+        elif not isinstance(data, ShowerData):
+            data = ShowerData.load(data)
+        self.data = data
+        self.kernel = kernel
+        self.config = config or ShowerConfig()
+        self.threads = threads
+        self.device = device
+        self._tables = None
+        self._compiled = {}
+        self._order0 = None
+        self._readout = {}
+
+    def tables(self):
+        """The pack folded onto the node bands, resident on the device (once)."""
+        if self._tables is None:
+            from .fold import BandFold, KernelPack, fold_bundle, frequency_indices, load_folded
+            from .ge2 import DevicePack
+            pack = self.data.ge2_pack
+            if not isinstance(pack, KernelPack):
+                pack = KernelPack.load(pack)
+            kernel = self.kernel
+            rank, subdivide = int(self.config.band_rank), int(self.config.fold_subdivide)
+            indices = frequency_indices(kernel.config.omega_per_ns, self.config.frequency_nodes)
+            omega = kernel.config.omega_per_ns[indices]
+            folded = None
+            prepared = self.data.ge2_band_fold
+            if prepared is not None:
+                prepared = load_folded(prepared) if not isinstance(prepared, dict) else prepared
+                if (prepared["band_fold"]["rank"] == rank
+                        and int(prepared.get("subdivide", 1)) == subdivide
+                        and np.asarray(prepared["omega"]).shape == omega.shape
+                        and np.allclose(prepared["omega"], omega, rtol=0, atol=1e-12)):
+                    folded = prepared
+            if folded is None:
+                fold = BandFold.of(kernel, rank=rank)
+            else:
+                fold = BandFold.of(kernel, rank=rank, bands=folded["band_fold"]["bands"])
+                stored = folded["band_fold"]["worst_residual_design"]
+                if not np.isclose(fold.metadata()["worst_residual_design"], stored, rtol=1e-9,
+                                  atol=0):
+                    raise ValueError("the prepared fold was made for another medium or detector")
+            began = perf_counter()
+            if folded is None:
+                bundle = pack.bundle(kernel.wavelength.wavelength_nm,
+                                     int(kernel.config.source_degree), indices)
+                folded = fold_bundle(bundle, fold, subdivide=subdivide, device=self.device)
+            device_pack = DevicePack(folded, self.device)
+            self._tables = dict(pack=pack, fold=fold, folded=folded, device_pack=device_pack,
+                                fold_seconds=perf_counter() - began)
+        return self._tables
+
+    def order0(self):
+        if self._order0 is None:
+            from .ballistic import Order0
+            self._order0 = Order0(self.kernel.medium, self.kernel.detector,
+                                  self.kernel.acceptance()["alpha"], device=self.device)
+        return self._order0
+
+    def readout(self, omega, edges):
+        from .readout import ReadoutBasis
+        key = (tuple(np.round(edges, 12)), len(omega))
+        if key not in self._readout:
+            self._readout[key] = ReadoutBasis(omega, edges, self.kernel.config.readout_omega_per_ns,
+                                              device=self.device)
+        return self._readout[key]
+
+    def __call__(self, source, *, progress=None, time_origin_ns=None, selection_extra_charge=None,
+                 receivers=None):
+        """Response for one event (``HybridTransportResponse``)."""
+        kernel = self.kernel
+        detail = kernel.config.detail_threshold_pe
+        lowered = (detail is not None and receivers is None
+                   and detail < kernel.config.threshold_pe)
+        if lowered:
+            kernel = RouteKernel(kernel.medium, kernel.detector,
+                                 replace(kernel.config, threshold_pe=float(detail),
+                                         detail_threshold_pe=None))
+            kernel.wavelength = self.kernel.wavelength
+            kernel._acceptance = self.kernel._acceptance
+        order0 = self.order0()
+        if receivers is not None:
+            kernel = kernel.subset(receivers, threshold_pe=0.0)
+            order0 = None
+        response = transport_shower(kernel, source, config=self.config, tables=self.tables(),
+                                    time_origin_ns=time_origin_ns,
+                                    selection_extra_charge=selection_extra_charge,
+                                    compiled=self._compiled, route=self if receivers is None
+                                    else None, order0=order0, device=self.device)
+        response.metadata["route"] = "lighthit.shower"
+        response.metadata["fold_prepare_seconds"] = self.tables()["fold_seconds"]
+        if lowered:
+            threshold = float(self.kernel.config.threshold_pe)
+            charge = response.charge_pe + (0 if selection_extra_charge is None
+                                           else np.asarray(selection_extra_charge, float))
+            response.active = response.active & (charge >= threshold)
+            response.metadata["threshold_pe"] = threshold
+            response.metadata["detail_threshold_pe"] = float(detail)
+            response.metadata["active_modules"] = int(np.count_nonzero(response.active))
+        return response
+
+
+def _elements_of(source):
+    elements = getattr(source, "elements", source)
+    if not hasattr(elements, "coefficient0"):
+        raise TypeError("the shower route takes spectral light elements or a synthetic shower")
+    return elements
+
+
+def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None,
+                     selection_extra_charge=None, compiled=None, route=None, order0=None,
+                     device="cuda"):
+    """All orders for one shower source (``shower/api.py:223-539``)."""
+    import torch
+    from .compile import DeviceChords
+    from .fold import frequency_indices
+    config = config or ShowerConfig()
+    basis = config.basis
+    began, cpu = perf_counter(), process_time()
+    timings = {}
+    stage = perf_counter()
+    indices = frequency_indices(kernel.config.omega_per_ns, config.frequency_nodes)
+    omega = kernel.config.omega_per_ns[indices]
+    if omega[0] != 0.0:
+        raise ValueError("the shower route needs omega = 0 in its frequency set")
+    output_omega = (kernel.config.readout_omega_per_ns
+                    if kernel.config.readout_omega_per_ns is not None else omega)
+    elements = _elements_of(source)
+    if elements.cone_model != "spectral":
+        raise ValueError("the shower route requires spectral cones (cone_model='spectral')")
+    wavelengths = np.asarray(kernel.wavelength.wavelength_nm, float)
+    sample = kernel.medium.sample(wavelengths)
+    max_phase = float(np.max(sample["phase_index"]))
+    valid = elements.beta * max_phase > 1
+    if not np.any(valid):
+        raise ValueError("source is below Cherenkov threshold over the wavelength range")
+    original_coefficient = float(elements.coefficient0.sum())
+    if not np.all(valid):
+        elements = elements.subset(valid)
+    efficiency = np.asarray(kernel.detector.spectral_weight(wavelengths), float)
+    positions = np.asarray(kernel.detector.positions_m, float)
+    looks = kernel.detector.looks
+    area = np.asarray(kernel.detector.effective_area_m2, float)
+    alpha = np.ascontiguousarray(kernel.acceptance()["alpha"], float)
+    low, cache_high = kernel.config.radial_range_m
+    high = cache_high if basis.radial_high_m is None else basis.radial_high_m
+    threshold = float(kernel.config.threshold_pe)
+    n = len(positions)
+    extra = (np.zeros(n) if selection_extra_charge is None
+             else np.maximum(np.asarray(selection_extra_charge, float), 0.0))
+    if extra.shape != (n,) or not np.isfinite(extra).all():
+        raise ValueError("selection_extra_charge must hold one finite value per OM")
+    fold, folded, pack = tables["fold"], tables["folded"], tables["device_pack"]
+    if not np.allclose(folded["omega"], omega, rtol=0, atol=1e-12):
+        raise ValueError("folded >=2 kernels and route frequencies differ")
+    timings["setup_seconds"] = perf_counter() - stage
+
+    # order >=2 source on the node bands
+    stage = perf_counter()
+    edges = np.asarray(kernel.config.relative_time_edges_ns, float)
+    prompt_mask, late_groups, beyond = delay_groups(
+        elements, prompt_delay_ns=float(config.prompt_delay_ns),
+        cell_ns=float(config.delayed_cell_ns), window_end_ns=float(edges[-1]),
+        group_index_max=float(np.max(sample["group_index"])))
+    key = (fold.bands, basis, tuple(np.round(omega, 12)), int(kernel.config.source_degree),
+           int(kernel.config.azimuthal_degree), float(config.prompt_delay_ns),
+           float(config.delayed_cell_ns), float(edges[-1]))
+    if compiled is not None and compiled.get("source") is source and compiled.get("key") == key:
+        sources, late_zero, late_sources = compiled["sources"]
+    else:
+        chords = DeviceChords(elements, device)
+        omega_device = torch.as_tensor(omega, device=device)
+
+        def compiled_part(mask, frequencies):
+            part = elements if np.all(mask) else elements.subset(mask)
+            frame = AxisFrame.of(part.field(0, phase_index=max_phase))
+            return node_sources(kernel, chords, basis, fold, frequencies, frame,
+                                mask=None if np.all(mask) else mask,
+                                omega_device=omega_device if len(frequencies) == len(omega)
+                                else None)
+        sources = compiled_part(prompt_mask, omega)
+        late_zero = compiled_part(~prompt_mask, omega[:1]) if np.any(~prompt_mask) else None
+        late_sources = [(shift, compiled_part(mask, omega)) for mask, shift in late_groups]
+        if compiled is not None:
+            compiled.clear()
+            compiled.update(source=source, key=key, sources=(sources, late_zero, late_sources))
+    timings["compile_seconds"] = perf_counter() - stage
+
+    # radial windows of every node-band point set
+    stage = perf_counter()
+    used = np.zeros(n, np.int64)
+    far = np.zeros(n, np.int64)
+    tail = np.zeros(n)
+    near = np.zeros(n, bool)
+    attenuation = spectral_attenuation(kernel)
+    weight_ratio = float(np.sum(fold.scale) / np.sum(fold.scale[list(fold.bands)]))
+    for group in [sources] + ([late_zero] if late_zero is not None else []):
+        for band, item in zip(fold.bands, group):
+            photons = (np.sqrt(4 * np.pi) * np.abs(item.channels[:, 0, 0].cpu().numpy())
+                       / efficiency[band])
+            points = item.to_points_m()
+            part_used, part_far, part_near, part_tail = window_report(
+                kernel, points, photons, positions, low, high, attenuation,
+                float(efficiency[band]))
+            near |= np.asarray(part_near) > 0
+            used = np.maximum(used, part_used)
+            far = np.maximum(far, part_far)
+            tail += weight_ratio * part_tail
+    radial_tail = screen_radial_tail(tail, basis, high)
+    reach = np.flatnonzero(used > 0)
+    if np.any(near):
+        raise NotImplementedError(
+            f"{int(near.sum())} OMs have source points closer than the {low:g} m radial minimum "
+            "of the order >=2 tables; the near-field clamp (near_field.py) is not on the GPU path")
+
+    # predicted zero-frequency >=2 charge of every reachable OM
+    predicted = np.zeros(n)
+    collapsed = collapse(sources, float(config.predictor_cell_m))
+    late_charge = np.zeros(n)
+    if len(reach):
+        predicted[reach] = apply_bands(pack, collapsed, positions[reach], looks[reach], alpha,
+                                       area[reach], (low, high), frequencies=1)[0].real
+        if late_zero is not None:
+            late_charge[reach] = apply_bands(pack, late_zero, positions[reach], looks[reach],
+                                             alpha, area[reach], (low, high),
+                                             frequencies=1)[0].real
+    timings["predict_seconds"] = perf_counter() - stage
+
+    # time origins (shower/api.py:356-375)
+    fastest = float(np.min(sample["group_index"]))
+    time_width = float(np.ptp(edges))
+    source_time_span = float(np.max(elements.end_ns) - np.min(elements.start_ns))
+    source_extent = float(np.linalg.norm(np.ptp(elements.midpoints_m, axis=0)))
+    local_origin = source_extent > high or source_time_span > 0.5 * time_width
+    if time_origin_ns is not None:
+        origins = np.asarray(time_origin_ns, float)
+        if origins.shape != (n,) or not np.isfinite(origins).all():
+            raise ValueError("time_origin_ns must hold one finite origin per OM")
+        local_origin = True
+    elif local_origin:
+        _, origins = segment_bounds(positions, elements, fastest / C_M_PER_NS, device)
+    else:
+        first = int(np.argmin(elements.start_ns))
+        origins = (float(elements.start_ns[first])
+                   + np.linalg.norm(positions - elements.start_m[first], axis=1)
+                   / (C_M_PER_NS / fastest))
+
+    # prompt orders 0 and 1 with the predicted >=2 in the selection
+    safety = float(config.predictor_safety)
+    stage = perf_counter()
+    prompt = transport_prompt(kernel, elements, config.prompt, order0=order0,
+                              selection_extra_charge=safety * np.abs(predicted)
+                              + np.abs(late_charge) + extra,
+                              time_origin_ns=origins if local_origin else None, device=device)
+    timings["prompt_seconds"] = perf_counter() - stage
+    if not np.allclose(prompt.time_origin_ns, origins, rtol=0, atol=1e-10):
+        raise ValueError("prompt and RTE time origins differ")
+
+    # exact zero-frequency >=2 where the threshold decision needs it
+    stage = perf_counter()
+    charge = np.zeros((n, 3))
+    charge[:, :2] = prompt.charge_orders_pe
+    prompt_charge = charge[:, :2].sum(axis=1)
+    computable = (used > 0) | (prompt.charge_pe > 0)
+    candidate = (used > 0) & (prompt_charge + safety * np.abs(predicted) + late_charge + extra
+                              >= threshold)
+    if threshold == 0:
+        candidate = used > 0
+
+    def exact(mask):
+        ids = np.flatnonzero(mask)
+        values = np.zeros(n)
+        if len(ids):
+            values[ids] = apply_bands(pack, sources, positions[ids], looks[ids], alpha, area[ids],
+                                      (low, high), frequencies=1)[0].real
+        return values
+
+    level = np.where(used > 0, 1, 0).astype(np.int8)
+    exact_charge = exact(candidate)
+    ratio = np.abs(exact_charge[candidate]) / np.maximum(np.abs(predicted[candidate]), 1e-300)
+    fallback = bool(np.any(np.abs(exact_charge[candidate])
+                           > safety * np.abs(predicted[candidate])))
+    if fallback:
+        candidate = used > 0
+        exact_charge = exact(candidate)
+    charge[:, 2] = predicted + late_charge
+    charge[candidate, 2] = exact_charge[candidate] + late_charge[candidate]
+    level[candidate] = 2
+    active = computable & (charge.sum(axis=1) + extra >= threshold)
+    if threshold == 0:
+        active = computable
+    if np.any(active & (level == 1)):
+        raise RuntimeError("an OM was activated by a predicted >=2 charge")
+    if not np.all(prompt.active[active]) and not fallback:
+        raise RuntimeError("prompt bins are missing for an active OM")
+    timings["zero_exact_seconds"] = perf_counter() - stage
+
+    # full frequencies for the active OMs, then the readout
+    stage = perf_counter()
+    spectrum = np.zeros((len(omega), n), complex)
+    work = np.flatnonzero(active & (used > 0))
+    if len(work):
+        spectrum[:, work] = apply_bands(pack, sources, positions[work], looks[work], alpha,
+                                        area[work], (low, high))
+    late_spectra = []
+    for shift, group in late_sources:
+        values = np.zeros((len(omega), n), complex)
+        if len(work):
+            values[:, work] = apply_bands(pack, group, positions[work], looks[work], alpha,
+                                          area[work], (low, high))
+        late_spectra.append((shift, values))
+    timings["full_apply_seconds"] = perf_counter() - stage
+    stage = perf_counter()
+    spectrum[0] = charge[:, 2] - late_charge
+    components = np.zeros((n, len(edges) - 1, 3))
+    if np.any(active):
+        if not np.all(prompt.active[active]):
+            redo = transport_prompt(kernel, elements, replace(config.prompt, screen=False),
+                                    order0=order0, compute_bins_for_all=True,
+                                    time_origin_ns=origins if local_origin else None,
+                                    device=device)
+            components[active, :, :2] = redo.bins_orders_pe[active]
+        else:
+            components[active, :, :2] = prompt.bins_orders_pe[active]
+        ids = np.flatnonzero(active)
+        readout = (route.readout(omega, edges) if route is not None else
+                   _readout_basis(kernel, omega, edges, device))
+        device_spectrum = torch.as_tensor(np.ascontiguousarray(spectrum[:, ids], np.complex64),
+                                          device=device)
+        device_origins = torch.as_tensor(origins[ids], device=device)
+        components[ids, :, 2] = readout.apply(device_spectrum, device_origins).double().cpu().numpy()
+        for shift, values in late_spectra:
+            basis_shift = _readout_basis(kernel, omega, edges - shift, device)
+            late = torch.as_tensor(np.ascontiguousarray(values[:, ids], np.complex64),
+                                   device=device)
+            components[ids, :, 2] += basis_shift.apply(
+                late, torch.as_tensor(origins[ids] + shift, device=device)).double().cpu().numpy()
+    timings["readout_seconds"] = perf_counter() - stage
+    elapsed = perf_counter() - began
+    timings.update(elapsed_seconds=elapsed, cpu_seconds=process_time() - cpu)
+    metadata = {
+        "route": "lighthit.shower", "timings": timings, "prompt_timings": prompt.metadata["timings"],
+        "elapsed_seconds": elapsed, "prompt_elapsed_seconds": timings["prompt_seconds"],
+        "ge2_basis": asdict(basis), "band_fold": fold.metadata(), "radial_tail": radial_tail,
+        "ge2_points": [int(item.points.shape[0]) for item in sources],
+        "delayed_light": {"prompt_delay_ns": float(config.prompt_delay_ns),
+                          "groups_in_window": len(late_sources),
+                          "steps_in_window_groups": int(sum(int(m.sum()) for m, _ in late_groups)),
+                          "steps_beyond_window": int(beyond.sum()),
+                          "delayed_charge_pe": float(late_charge.sum())},
+        "ge2_charge_level": level.tolist(),
+        "ge2_predictor": {"points": int(len(collapsed[0].z_m)),
+                          "cell_m": float(config.predictor_cell_m), "safety": safety,
+                          "candidates": int(candidate.sum()), "reachable": int(len(reach)),
+                          "fallback_to_exact": fallback,
+                          "exact_over_prediction_max": float(ratio.max(initial=0.0)),
+                          "exact_over_prediction_min": (float(ratio.min()) if len(ratio)
+                                                        else None)},
+        "radial_window_m": [float(low), float(high)],
+        "radial_windowed_modules": int(np.count_nonzero(far)),
+        "time_origin": ("supplied event clock" if time_origin_ns is not None else
+                        "earliest local segment and fastest group flight" if local_origin
+                        else "earliest element start and fastest group flight"),
+        "selection_extra_charge_pe": float(extra.sum()),
+        "source_elements": len(elements),
+        "source_coefficient_fraction_dropped":
+            1 - float(elements.coefficient0.sum()) / original_coefficient,
+        "active_modules": int(active.sum()), "threshold_pe": threshold,
+        "cone_model": "spectral", "spectrum_computed_orders": [2],
+        "frequency_nodes_computed": len(omega), "frequency_nodes_readout": len(output_omega),
+        "prompt_order1_level": prompt.metadata["order1_level"],
+        "backend": "lighthit_amd (gfx950)",
+    }
+    return HybridTransportResponse(kernel.detector, omega.copy(), edges.copy(), spectrum,
+                                   components, charge, origins, active, "shower", metadata)
+
+
+_readout_cache = {}
+
+
+def _readout_basis(kernel, omega, edges, device):
+    from .readout import ReadoutBasis
+    key = (tuple(np.round(edges, 12)), len(omega), str(device))
+    if key not in _readout_cache:
+        _readout_cache[key] = ReadoutBasis(omega, edges, kernel.config.readout_omega_per_ns,
+                                           device=device)
+    return _readout_cache[key]

@@ -139,3 +139,50 @@ def plain_detector_scale(elements=50_000):
                     medium, detector, WavelengthQuadrature.gauss_legendre(400., 500., 8),
                     np.arange(-60.0, 745.0, 1.0), np.linspace(0., 1.2, 65),
                     np.linspace(0., 1.2, 321), _shower(pose, 1500.0, elements, 11))
+
+
+def synthetic_pack(medium, quadrature, *, degree=8, acceptance_degree=2,
+                   omega=np.linspace(0.0, 1.2, 65), radial_range_m=(2.0, 150.0),
+                   radial_nodes=48):
+    """An order >=2 kernel pack of the reference's layout with a smooth, positive,
+    made-up radial dependence (the RTE table build is not part of this package).
+
+    ``K_(l, pair)(r, w) = 0.3^l / (1 + pair) * 0.5 (mu_s r)^2 / (1 + mu_s r)
+    * e^{-i w 0.05 r} / (1 + (0.02 r w)^2)`` per wavelength band, written as cubics
+    in ``log r`` on ``radial_nodes - 1`` intervals with flight phase.
+    Returns ``(wavelengths, indices, bundle)`` for :meth:`KernelPack.add`.
+    """
+    wavelengths = np.asarray(quadrature.wavelength_nm, float)
+    sample = medium.sample(wavelengths)
+    mus = np.asarray(sample["scattering_per_m"], float)
+    pl, pm = [], []
+    for a in range(acceptance_degree + 1):
+        for m in range(a + 1):
+            pl.append(a)
+            pm.append(m)
+    log_r = np.linspace(np.log(radial_range_m[0]), np.log(radial_range_m[1]), radial_nodes)
+    omega = np.asarray(omega, float)
+    nodes = np.array([0.0, 1 / 3, 2 / 3, 1.0])
+    nint = radial_nodes - 1
+    coefficients = np.zeros((len(wavelengths), nint, degree + 1, len(pl), 4, len(omega)),
+                            np.complex64)
+    amplitude = np.array([[0.3 ** ell / (1.0 + p) for p in range(len(pl))]
+                          for ell in range(degree + 1)])
+    for j in range(nint):
+        width = log_r[j + 1] - log_r[j]
+        t = nodes * width
+        radius = np.exp(log_r[j] + t)
+        inverse = np.linalg.inv(np.vander(t, 4))
+        for band in range(len(wavelengths)):
+            g = 0.5 * (mus[band] * radius) ** 2 / (1.0 + mus[band] * radius)
+            shape = (g[:, None] * np.exp(-1j * omega[None, :] * 0.05 * radius[:, None])
+                     / (1.0 + (0.02 * radius[:, None] * omega[None, :]) ** 2))   # (node, w)
+            cubic = inverse @ shape                                              # (4, w)
+            coefficients[band, j] = (amplitude[:, :, None, None] * cubic[None, None]).astype(
+                np.complex64)
+    bundle = dict(degree=int(degree), acceptance_degree=int(acceptance_degree),
+                  pair_lambda=np.asarray(pl, np.int64), pair_mu=np.asarray(pm, np.int64),
+                  omega=omega, phase="flight", coefficients=coefficients, log_r=log_r,
+                  absorption=np.asarray(sample["absorption_per_m"], float),
+                  speed=0.299792458 / np.asarray(sample["group_index"], float))
+    return wavelengths, np.arange(len(omega)), bundle
