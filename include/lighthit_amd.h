@@ -261,8 +261,9 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
  * (screen grids, parent_face > 0: parents and children in preorder, at most
  * twice as many, count in info[1]). The grouped pass is two calls:
  * lh_order1_group_keys writes the number of (tau, r, theta) groups to info[0];
- * lh_order1_group_charge takes a values buffer of groups * (2 n_mono + 6)
- * doubles and writes charge (and bins) of the listed OMs. Each pair shares its
+ * lh_order1_group_charge takes a scratch buffer of
+ * lh_order1_group_scratch_bytes(oms, optics, groups) and writes charge (and
+ * bins) of the listed OMs. Each pair shares its
  * workspace. lh_order1_set_tables uploads the 24-term Taylor table of
  * I_k(z) (3 x 24) and the 48-node inverse-angle rule (tangent, weight sec^2).
  */
@@ -347,10 +348,13 @@ size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
 int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                          const lh_order1_optics* optics, const lh_order1_merge* merge, int* info,
                          void* workspace, size_t workspace_bytes, void* stream);
+size_t lh_order1_group_scratch_bytes(const lh_order1_oms* oms, const lh_order1_optics* optics,
+                                     int groups);
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
-                           int groups, double* values, int compute_bins, double* charge,
-                           double* bins, void* workspace, size_t workspace_bytes, void* stream);
+                           int groups, void* scratch, size_t scratch_bytes, int compute_bins,
+                           double* charge, double* bins, void* workspace, size_t workspace_bytes,
+                           void* stream);
 
 /*
  * Band fold of the order >= 2 pack (one-time table preparation).

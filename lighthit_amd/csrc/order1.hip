@@ -25,7 +25,9 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <cmath>
+#include <cstring>
 
 namespace {
 
@@ -380,11 +382,19 @@ struct Group {
     int* group_scan;
     int* group_start;         // (groups + 1)
     int* om_groups;           // (work + 1) first group of every OM
-    double* values;           // (groups, nv) caller buffer
+    double* values;           // (groups, nv)            caller scratch
+    int* batch_flag;          // (groups + 1)            caller scratch
+    int* batch_scan;          // (groups + 1)
+    int* batch_start;         // (batches + 1) first group of every batch
+    int* om_batches;          // (work + 1) first batch of every OM
+    double* part_acc;         // (work, segments, bands, bins, 4) partial deposits
+    double* part_pre;         // (work, segments, bands, 3)
+    double* part_charge;      // (work, segments, bands)
+    int segments;             // batch segments per OM (one wave each)
     double* prepared;         // (n_pen, 7)
-    double* acc;              // (work, bands, bins, 4)
-    double* pre;              // (work, bands, 3)
     double* lambda_bins;      // (work, bands, bins)
+    int* k_range;             // (work, 4) k1 min, k1 max, k0 min, k0 max
+    unsigned long long* max_key;
     double* charge;           // (n_recv) output
     double* bins;             // (n_recv, bins) output
     int* info;
@@ -578,7 +588,27 @@ __device__ inline void frame_of(const Group& G, int p, int d, double* r, double*
 
 // The (up to two) group entries of pencil p at OM d (_grouped_pencils_order1_block
 // :1815-1922; screen :222-326).
-__device__ int pencil_entries(const Group& G, int p, int d, Entry out[2]) {
+__device__ inline void radius_keys(const Group& G, int p, int d, long long* k0, long long* k1) {
+    const double* pp = G.pen.position + 3 * p;
+    double dx = G.om.positions[3 * d] - pp[0];
+    double dy = G.om.positions[3 * d + 1] - pp[1];
+    double dz = G.om.positions[3 * d + 2] - pp[2];
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    *k0 = (long long)floor(log(r) / G.merge.log_r_step) + (1LL << 20);
+    if (G.screen) {
+        *k1 = 0;
+        return;
+    }
+    double t_emit = G.pen.time[p] - G.om.origins[d];
+    double tau = t_emit + r / G.merge.speed_ref;
+    long long k = (long long)floor(tau / G.merge.tau_step) + (1LL << 20);
+    *k1 = min(max(k, 0LL), (1LL << 21) - 1);
+}
+
+// Keys are compact per OM: ((k1 - k1min) * n_k0 + (k0 - k0min)) * 2 theta_bins
+// + backward * theta_bins + theta index. They order and group exactly as the
+// reference's (k1 * 2^21 + k0) * 2 theta_bins + ... keys.
+__device__ int pencil_entries(const Group& G, int p, int d, int w, Entry out[2]) {
     double r, h[3], theta, sine, c1, c3;
     frame_of(G, p, d, &r, h, &theta, &sine, &c1, &c3);
     if (G.screen) {
@@ -599,13 +629,19 @@ __device__ int pencil_entries(const Group& G, int p, int d, Entry out[2]) {
     double t_emit = G.screen ? 0.0 : G.pen.time[p] - G.om.origins[d];
     double tau = G.screen ? 0.0 : t_emit + r / G.merge.speed_ref;
     long long base;
-    if (G.screen) {
-        base = ((long long)floor(log(r) / G.merge.log_r_step) + (1LL << 20)) * 2 * G.theta_bins;
-    } else {
+    {
         long long k0 = (long long)floor(log(r) / G.merge.log_r_step) + (1LL << 20);
-        long long k1 = (long long)floor(tau / G.merge.tau_step) + (1LL << 20);
-        k1 = min(max(k1, 0LL), (1LL << 21) - 1);
-        base = (k1 * (1LL << 21) + k0) * 2 * G.theta_bins;
+        long long k1 = 0;
+        if (!G.screen) {
+            k1 = (long long)floor(tau / G.merge.tau_step) + (1LL << 20);
+            k1 = min(max(k1, 0LL), (1LL << 21) - 1);
+        }
+        const int* range = G.k_range + 4 * w;
+        long long n_k0 = (long long)range[3] - range[2] + 1;
+        long long n_k1 = (long long)range[1] - range[0] + 1;
+        long long d1 = min(max(k1 - range[0], 0LL), n_k1 - 1);
+        long long d0 = min(max(k0 - range[2], 0LL), n_k0 - 1);
+        base = (d1 * n_k0 + d0) * 2 * G.theta_bins;
     }
     int backward = theta > 0.5 * M_PI;
     double bound = 0.0;
@@ -653,18 +689,46 @@ __device__ int pencil_entries(const Group& G, int p, int d, Entry out[2]) {
     return nodes;
 }
 
+__global__ void __launch_bounds__(kThreads) o1_range_init(Group G, int work) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w == 0) G.max_key[0] = 0ULL;
+    if (w >= work) return;
+    G.k_range[4 * w] = INT_MAX;
+    G.k_range[4 * w + 1] = INT_MIN;
+    G.k_range[4 * w + 2] = INT_MAX;
+    G.k_range[4 * w + 3] = INT_MIN;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_group_ranges(Group G) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int w = blockIdx.y;
+    if (p >= G.n_pen) return;
+    long long k0, k1;
+    radius_keys(G, p, G.om.work[w], &k0, &k1);
+    // one key of margin on each side: the key pass evaluates the same expressions
+    // in another function, where contraction may differ in the last bit
+    atomicMin(G.k_range + 4 * w, (int)k1 - 1);
+    atomicMax(G.k_range + 4 * w + 1, (int)k1 + 1);
+    atomicMin(G.k_range + 4 * w + 2, (int)k0 - 1);
+    atomicMax(G.k_range + 4 * w + 3, (int)k0 + 1);
+}
+
 __global__ void __launch_bounds__(kThreads) o1_group_keys(Group G) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int w = blockIdx.y;
     if (p >= G.n_pen) return;
     int d = G.om.work[w];
     Entry e[2];
-    int n = pencil_entries(G, p, d, e);
+    int n = pencil_entries(G, p, d, w, e);
     size_t at = ((size_t)w * G.n_pen + p) * 2;
+    unsigned long long top = 0;
     for (int node = 0; node < 2; ++node) {
-        G.keys[at + node] = node < n ? e[node].key : kSentinel;
+        long long key = node < n ? e[node].key : kSentinel;
+        G.keys[at + node] = key;
         G.vals[at + node] = 2 * p + node;
+        if (node < n) top = max(top, (unsigned long long)key);
     }
+    if (n) atomicMax(G.max_key, top);
 }
 
 __global__ void __launch_bounds__(kThreads) o1_group_flags(Group G, long long total) {
@@ -718,9 +782,21 @@ __device__ inline void fill_moments(const Group& G, double weight0, double weigh
     }
 }
 
-// Group sums: members in (key, pencil) order, as the reference's hash rows.
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Group sums: one wave per group, lanes over its members, a fixed butterfly
+// reduction (deterministic).
 __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int g = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
     if (g >= groups) return;
     int n_mono = G.opt.n_mono;
     int nv = 2 * n_mono + 6;
@@ -732,11 +808,11 @@ __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
     double row[2 * kMaxMono + 6];
     for (int m = 0; m < nv; ++m) row[m] = 0.0;
     double mom0[kMaxMono], mom2[kMaxMono];
-    for (long long j = j0; j < j1; ++j) {
+    for (long long j = j0 + lane; j < j1; j += 64) {
         int v = G.sorted_vals[j];
         int p = v >> 1, node = v & 1;
         Entry e[2];
-        pencil_entries(G, p, d, e);
+        pencil_entries(G, p, d, w, e);
         const Entry& x = e[node];
         fill_moments(G, x.w0, x.w2, x.c1, x.c3, x.averaged, mom0, mom2);
         for (int m = 0; m < n_mono; ++m) {
@@ -751,10 +827,31 @@ __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
         row[2 * n_mono + 4] += omega * x.tau;
         row[2 * n_mono + 5] += omega * x.tau * x.tau;
     }
-    for (int m = 0; m < nv; ++m) G.values[(size_t)g * nv + m] = row[m];
+    for (int m = 0; m < nv; ++m) {
+        double total = wave_sum(row[m]);
+        if (lane == 0) G.values[(size_t)g * nv + m] = total;
+    }
 }
 
-// ---- batch core (one workgroup per OM)
+// Batches: consecutive groups of one OM with one key / theta_bins.
+__global__ void __launch_bounds__(kThreads) o1_batch_flags(Group G, int groups, int work) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > groups) return;
+    if (g == groups) { G.batch_flag[g] = 0; return; }
+    long long seg = 2LL * G.n_pen;
+    int w = (int)(G.group_start[g] / seg);
+    long long key = G.sorted_keys[G.group_start[g]] / G.theta_bins;
+    bool first = g == G.om_groups[w];
+    G.batch_flag[g] = (first || G.sorted_keys[G.group_start[g - 1]] / G.theta_bins != key) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) o1_batch_starts(Group G, int groups, int work) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g <= groups && (g == groups || G.batch_flag[g])) G.batch_start[G.batch_scan[g]] = g;
+    if (g <= work) G.om_batches[g] = G.batch_scan[G.om_groups[g]];
+}
+
+// ---- batch core
 
 __device__ void ifuncs(double z, double ez, double* i0, double* i1, double* i2) {
     if (fabs(z) < 1.0) {
@@ -870,303 +967,298 @@ __device__ void cubic_coefficients(const Group& G, const double* mom, double* a,
 
 constexpr int kBatchGroups = 64;   // groups staged in LDS per profile pass
 
-__global__ void __launch_bounds__(kThreads) o1_batches(Group G) {
+// One wave per (OM, segment): the segment's batches b = first + s, first + s + S, ...
+// (_grouped_pencils_order1_block :1924-1995 and _batch_deposit :451-551) into the
+// segment's own partial accumulators.
+__global__ void __launch_bounds__(64) o1_batches(Group G) {
     __shared__ double s_buf[kMaxPathNodes];
     __shared__ double g0[kMaxPathNodes];
     __shared__ double g2[kMaxPathNodes];
     __shared__ double coeff[kBatchGroups][16];   // A0[4] B0[3] A2[4] B2[3] sine cosine
-    __shared__ double batch[8];
-    __shared__ int batch_n;
-    __shared__ int batch_stop;
-    int w = blockIdx.x;
-    int d = G.om.work[w];
+    int w = blockIdx.x, seg = blockIdx.y;
+    int lane = threadIdx.x;
     int nl = G.opt.bands, nb = G.opt.bins;
     int n_mono = G.opt.n_mono;
     int nv = 2 * n_mono + 6;
-    int g_begin = G.om_groups[w], g_end = G.om_groups[w + 1];
-    double* acc = G.acc + (size_t)w * nl * nb * 4;
-    double* pre = G.pre + (size_t)w * nl * 3;
-    double charge_l[1];   // per-thread charge of its wavelength (threads < nl)
-    charge_l[0] = 0.0;
+    size_t slot = (size_t)w * G.segments + seg;
+    double* acc = G.part_acc + slot * nl * nb * 4;
+    double* pre = G.part_pre + slot * nl * 3;
     if (G.compute_bins)
-        for (size_t i = threadIdx.x; i < (size_t)nl * nb * 4; i += kThreads) acc[i] = 0.0;
-    for (int i = threadIdx.x; i < nl * 3; i += kThreads) pre[i] = 0.0;
-    __syncthreads();
+        for (size_t i = lane; i < (size_t)nl * nb * 4; i += 64) acc[i] = 0.0;
+    for (int i = lane; i < nl * 3; i += 64) pre[i] = 0.0;
+    double charge_lane = 0.0;
     int cubic = G.opt.max_power <= 3;
-    int start = g_begin;
-    while (start < g_end) {
-        if (threadIdx.x == 0) {
-            long long key = 0;
-            long long batch_key = 0;
-            int stop = start;
-            double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0;
-            double theta_small = M_PI;
-            // first group's key
-            long long j0 = G.group_start[start];
-            key = G.sorted_keys[j0];
-            batch_key = key / G.theta_bins;
-            while (stop < g_end) {
-                long long k = G.sorted_keys[G.group_start[stop]];
-                if (k / G.theta_bins != batch_key) break;
-                const double* row = G.values + (size_t)stop * nv;
-                double omega = row[2 * n_mono];
-                if (omega > 0.0) {
-                    omega_sum += omega;
-                    r_sum += row[2 * n_mono + 2];
-                    t_sum += row[2 * n_mono + 3];
-                    tau_sum += row[2 * n_mono + 4];
-                    tau2_sum += row[2 * n_mono + 5];
-                    theta_small = fmin(theta_small, row[2 * n_mono + 1] / omega);
-                }
-                ++stop;
-            }
-            batch_stop = stop;
-            batch_n = 0;
-            if (omega_sum > 0.0) {
-                int backward = batch_key % 2 == 1;
-                double r_b = r_sum / omega_sum;
-                double t_b = t_sum / omega_sum;
-                double tau_mean = tau_sum / omega_sum;
-                double tau_var = fmax(tau2_sum / omega_sum - tau_mean * tau_mean, 0.0);
-                theta_small = fmax(theta_small, 1e-9);
-                double sigma;
-                if (backward)
-                    sigma = fmax(0.5 * r_b * sin(theta_small) * G.opt.y_g,
-                                 0.25 * r_b * (1.0 - cos(theta_small)));
-                else
-                    sigma = fmax(0.5 * r_b * sin(theta_small)
-                                     * fmax(G.opt.y_g, 0.5 * tan(0.5 * theta_small)),
-                                 1e-9 * r_b);
-                // _batch_grid (:258-286)
-                double h_in = G.opt.path_step;
-                double h_out = G.screen ? G.opt.path_step : G.opt.path_step_tail;
-                double ext_end = G.opt.tail_exponent / G.mut_min;
-                double ext_win = G.screen ? ext_end
-                                          : G.speed_max * (G.opt.edges[nb] - t_b) - r_b;
-                if (ext_win < 0.0) ext_win = 0.0;
-                if (ext_win > ext_end) ext_win = ext_end;
-                double u_end = asinh(ext_end / sigma);
-                double u_win = asinh(ext_win / sigma);
-                int n_in = u_win > 0.0 ? (int)ceil(u_win / h_in) : 0;
-                int n_out = u_end > u_win ? (int)ceil((u_end - u_win) / h_out) : 0;
-                int capacity = min(G.opt.max_path_nodes, kMaxPathNodes);
-                if (n_in + n_out + 1 > capacity) {
-                    double scale = (capacity - 1.0) / (n_in + n_out);
-                    n_in = (int)(n_in * scale);
-                    n_out = capacity - 1 - n_in;
-                }
-                batch_n = n_in + n_out + 1;
-                batch[0] = r_b; batch[1] = sigma; batch[2] = u_win; batch[3] = u_end;
-                batch[4] = n_in; batch[5] = n_out; batch[6] = t_b;
-                batch[7] = sqrt(12.0 * tau_var);
+    double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
+    double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
+    for (int b = G.om_batches[w] + seg; b < G.om_batches[w + 1]; b += G.segments) {
+        int start = G.batch_start[b], stop = G.batch_start[b + 1];
+        int backward = (G.sorted_keys[G.group_start[start]] / G.theta_bins) % 2 == 1;
+        double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0, theta_small = M_PI;
+        for (int q = start + lane; q < stop; q += 64) {
+            const double* row = G.values + (size_t)q * nv;
+            double omega = row[2 * n_mono];
+            if (omega > 0.0) {
+                omega_sum += omega;
+                r_sum += row[2 * n_mono + 2];
+                t_sum += row[2 * n_mono + 3];
+                tau_sum += row[2 * n_mono + 4];
+                tau2_sum += row[2 * n_mono + 5];
+                theta_small = fmin(theta_small, row[2 * n_mono + 1] / omega);
             }
         }
-        __syncthreads();
-        int n = batch_n, stop = batch_stop;
-        if (n > 0) {
-            double r_b = batch[0], sigma = batch[1], u_win = batch[2], u_end = batch[3];
-            int n_in = (int)batch[4], n_out = (int)batch[5];
-            for (int k = threadIdx.x; k < n; k += kThreads) {
-                double s;
-                if (k == 0) s = r_b;
-                else if (k <= n_in) s = r_b + sigma * sinh(u_win * k / n_in);
-                else s = r_b + sigma * sinh(u_win + (u_end - u_win) * (k - n_in) / n_out);
-                s_buf[k] = s;
-                g0[k] = 0.0;
-                g2[k] = 0.0;
-            }
-            int backward = (G.sorted_keys[G.group_start[start]] / G.theta_bins) % 2 == 1;
-            // profile: every path node sums its batch's groups in key order
-            for (int g0i = start; g0i < stop; g0i += kBatchGroups) {
-                int chunk = min(kBatchGroups, stop - g0i);
-                __syncthreads();
-                for (int q = threadIdx.x; q < chunk; q += kThreads) {
-                    const double* row = G.values + (size_t)(g0i + q) * nv;
-                    double omega = row[2 * n_mono];
-                    double* cf = coeff[q];
-                    if (omega <= 0.0) { cf[14] = -1.0; continue; }
-                    double theta = fmax(row[2 * n_mono + 1] / omega, 1e-9);
-                    cf[14] = sin(theta);
-                    cf[15] = cos(theta);
-                    if (cubic) {
-                        cubic_coefficients(G, row, cf, cf + 4);
-                        cubic_coefficients(G, row + n_mono, cf + 7, cf + 11);
-                    }
-                }
-                __syncthreads();
-                double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
-                double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
-                for (int k = threadIdx.x; k < n; k += kThreads) {
-                    double acc0 = g0[k], acc2 = g2[k];
-                    double s_k = s_buf[k];
-                    for (int q = 0; q < chunk; ++q) {
-                        const double* cf = coeff[q];
-                        double sin_t = cf[14];
-                        if (sin_t < 0.0) continue;
-                        double cos_t = cf[15];
-                        double rs = r_b * sin_t;
-                        double c, s, path_jacobian;
-                        if (backward) {
-                            double shift = s_k - r_b * cos_t;
-                            double denominator = rs * rs + shift * shift;
-                            c = (rs * rs - shift * shift) / denominator;
-                            s = 2.0 * rs * shift / denominator;
-                            path_jacobian = 2.0 * r_b / denominator;
-                        } else {
-                            double y_theta = sin_t / (1.0 + cos_t);
-                            double jac = 2.0 / (r_b * sin_t);
-                            double y = (s_k - r_b) / rs + y_theta;
-                            double inv = 1.0 / (1.0 + y * y);
-                            c = (1.0 - y * y) * inv;
-                            s = 2.0 * y * inv;
-                            path_jacobian = jac * inv;
-                        }
-                        double a0, a2;
-                        if (cubic) {
-                            a0 = cf[0] + c * (cf[1] + c * (cf[2] + c * cf[3]));
-                            a0 += s * (cf[4] + c * (cf[5] + c * cf[6]));
-                            a2 = cf[7] + c * (cf[8] + c * (cf[9] + c * cf[10]));
-                            a2 += s * (cf[11] + c * (cf[12] + c * cf[13]));
-                        } else {
-                            const double* row = G.values + (size_t)(g0i + q) * nv;
-                            double cpow[8], spow[8];
-                            cpow[0] = 1.0; spow[0] = 1.0;
-                            for (int p = 1; p <= G.opt.max_power; ++p) {
-                                cpow[p] = cpow[p - 1] * c;
-                                spow[p] = spow[p - 1] * s;
-                            }
-                            a0 = 0.0; a2 = 0.0;
-                            for (int m = 0; m < n_mono; ++m) {
-                                double basis = G.opt.coef[m] * cpow[G.opt.ipow[m]] * spow[G.opt.jpow[m]];
-                                a0 += basis * row[m];
-                                a2 += basis * row[n_mono + m];
-                            }
-                        }
-                        double den = g1 - gg * c;
-                        double value = path_jacobian * norm / (den * sqrt(den));
-                        acc0 += a0 * value;
-                        acc2 += a2 * value;
-                    }
-                    g0[k] = acc0;
-                    g2[k] = acc2;
+        omega_sum = wave_sum(omega_sum);
+        r_sum = wave_sum(r_sum);
+        t_sum = wave_sum(t_sum);
+        tau_sum = wave_sum(tau_sum);
+        tau2_sum = wave_sum(tau2_sum);
+        theta_small = wave_min(theta_small);
+        if (!(omega_sum > 0.0)) continue;
+        double r_b = r_sum / omega_sum;
+        double t_b = t_sum / omega_sum;
+        double tau_mean = tau_sum / omega_sum;
+        double tau_var = fmax(tau2_sum / omega_sum - tau_mean * tau_mean, 0.0);
+        theta_small = fmax(theta_small, 1e-9);
+        double sigma;
+        if (backward)
+            sigma = fmax(0.5 * r_b * sin(theta_small) * G.opt.y_g,
+                         0.25 * r_b * (1.0 - cos(theta_small)));
+        else
+            sigma = fmax(0.5 * r_b * sin(theta_small)
+                             * fmax(G.opt.y_g, 0.5 * tan(0.5 * theta_small)),
+                         1e-9 * r_b);
+        // _batch_grid (:258-286)
+        double h_in = G.opt.path_step;
+        double h_out = G.screen ? G.opt.path_step : G.opt.path_step_tail;
+        double ext_end = G.opt.tail_exponent / G.mut_min;
+        double ext_win = G.screen ? ext_end : G.speed_max * (G.opt.edges[nb] - t_b) - r_b;
+        if (ext_win < 0.0) ext_win = 0.0;
+        if (ext_win > ext_end) ext_win = ext_end;
+        double u_end = asinh(ext_end / sigma);
+        double u_win = asinh(ext_win / sigma);
+        int n_in = u_win > 0.0 ? (int)ceil(u_win / h_in) : 0;
+        int n_out = u_end > u_win ? (int)ceil((u_end - u_win) / h_out) : 0;
+        int capacity = min(G.opt.max_path_nodes, kMaxPathNodes);
+        if (n_in + n_out + 1 > capacity) {
+            double scale = (capacity - 1.0) / (n_in + n_out);
+            n_in = (int)(n_in * scale);
+            n_out = capacity - 1 - n_in;
+        }
+        int n = n_in + n_out + 1;
+        for (int k = lane; k < n; k += 64) {
+            double sv;
+            if (k == 0) sv = r_b;
+            else if (k <= n_in) sv = r_b + sigma * sinh(u_win * k / n_in);
+            else sv = r_b + sigma * sinh(u_win + (u_end - u_win) * (k - n_in) / n_out);
+            s_buf[k] = sv;
+            g0[k] = 0.0;
+            g2[k] = 0.0;
+        }
+        // profile: every path node sums its batch's groups in key order
+        for (int c0 = start; c0 < stop; c0 += kBatchGroups) {
+            int chunk = min(kBatchGroups, stop - c0);
+            __syncthreads();
+            for (int q = lane; q < chunk; q += 64) {
+                const double* row = G.values + (size_t)(c0 + q) * nv;
+                double omega = row[2 * n_mono];
+                double* cf = coeff[q];
+                if (omega <= 0.0) { cf[14] = -1.0; continue; }
+                double theta = fmax(row[2 * n_mono + 1] / omega, 1e-9);
+                cf[14] = sin(theta);
+                cf[15] = cos(theta);
+                if (cubic) {
+                    cubic_coefficients(G, row, cf, cf + 4);
+                    cubic_coefficients(G, row + n_mono, cf + 7, cf + 11);
                 }
             }
             __syncthreads();
-            // _batch_deposit (:451-551): one thread per wavelength
-            int lam = threadIdx.x;
-            if (lam < nl && G.opt.mus[lam] != 0.0) {
-                double v = G.opt.speed[lam];
-                double mut = G.opt.mut[lam];
-                double kappa = -mut * v;
-                int last = n - 1;
-                double limit = r_b + G.opt.tail_exponent / mut;
-                while (last > 2 && s_buf[last - 1] > limit) --last;
-                if (last >= 2) {
-                    double t_emit_rel = batch[6];
-                    double spread = fmin(batch[7], (s_buf[last] - r_b) / v);
-                    double front_time = t_emit_rel + r_b / v;
-                    double end_time = t_emit_rel + s_buf[last] / v;
-                    int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
-                                  || end_time + 0.5 * spread <= G.opt.edges[0];
-                    double t_base = outside ? 0.0 : t_emit_rel;
-                    double e_emit = exp(kappa * t_base);
-                    double h_scale = G.opt.mus[lam] * v / e_emit;
-                    double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
-                    LambdaBins B;
-                    B.edges = G.opt.edges;
-                    B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
-                    B.acc = acc + (size_t)lam * nb * 4;
-                    B.pre = pre + 3 * lam;
-                    B.nb = nb;
-                    B.direct_bins = G.opt.direct_bins;
-                    bool bins = G.compute_bins && !outside;
-                    double h_front = w0 * g0[0] + w2 * g2[0];
-                    double total = 0.0;
-                    double e_first = e_emit * exp(-mut * s_buf[0]);
-                    double e_a = e_first, e_last = e_first;
-                    int j = 0;
-                    while (j < last) {
-                        double t_a, t_b, alpha, beta, gamma, e_b;
-                        int step;
-                        if (j + 2 <= last) {
-                            t_a = t_base + s_buf[j] / v;
-                            double t_m = t_base + s_buf[j + 1] / v;
-                            t_b = t_base + s_buf[j + 2] / v;
-                            double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
-                            double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
-                            double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
-                            double u_m = t_m - t_a, u_b = t_b - t_a;
-                            double d1 = (h1 - h0) / u_m;
-                            double d2 = ((h2 - h1) / (t_b - t_m) - d1) / u_b;
-                            alpha = h0;
-                            beta = d1 - d2 * u_m;
-                            gamma = d2;
-                            e_b = e_emit * exp(-mut * s_buf[j + 2]);
-                            step = 2;
-                        } else {
-                            double t_p = t_base + s_buf[j - 1] / v;
-                            t_a = t_base + s_buf[j] / v;
-                            t_b = t_base + s_buf[j + 1] / v;
-                            double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
-                            double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
-                            double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
-                            double u_m = t_a - t_p, u_b = t_b - t_p;
-                            double d1 = (h0 - hp) / u_m;
-                            double d2 = ((h1 - h0) / (t_b - t_a) - d1) / u_b;
-                            alpha = h0;
-                            beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
-                            gamma = d2;
-                            e_b = e_emit * exp(-mut * s_buf[j + 1]);
-                            step = 1;
+            for (int k = lane; k < n; k += 64) {
+                double acc0 = g0[k], acc2 = g2[k];
+                double s_k = s_buf[k];
+                for (int q = 0; q < chunk; ++q) {
+                    const double* cf = coeff[q];
+                    double sin_t = cf[14];
+                    if (sin_t < 0.0) continue;
+                    double cos_t = cf[15];
+                    double rs = r_b * sin_t;
+                    double c, sn, path_jacobian;
+                    if (backward) {
+                        double shift = s_k - r_b * cos_t;
+                        double denominator = rs * rs + shift * shift;
+                        c = (rs * rs - shift * shift) / denominator;
+                        sn = 2.0 * rs * shift / denominator;
+                        path_jacobian = 2.0 * r_b / denominator;
+                    } else {
+                        double y_theta = sin_t / (1.0 + cos_t);
+                        double jac = 2.0 / (r_b * sin_t);
+                        double y = (s_k - r_b) / rs + y_theta;
+                        double inv = 1.0 / (1.0 + y * y);
+                        c = (1.0 - y * y) * inv;
+                        sn = 2.0 * y * inv;
+                        path_jacobian = jac * inv;
+                    }
+                    double a0, a2;
+                    if (cubic) {
+                        a0 = cf[0] + c * (cf[1] + c * (cf[2] + c * cf[3]));
+                        a0 += sn * (cf[4] + c * (cf[5] + c * cf[6]));
+                        a2 = cf[7] + c * (cf[8] + c * (cf[9] + c * cf[10]));
+                        a2 += sn * (cf[11] + c * (cf[12] + c * cf[13]));
+                    } else {
+                        const double* row = G.values + (size_t)(c0 + q) * nv;
+                        double cpow[8], spow[8];
+                        cpow[0] = 1.0; spow[0] = 1.0;
+                        for (int pw = 1; pw <= G.opt.max_power; ++pw) {
+                            cpow[pw] = cpow[pw - 1] * c;
+                            spow[pw] = spow[pw - 1] * sn;
                         }
-                        total += quad_integral(e_a, alpha, beta, gamma, t_b - t_a, kappa, e_b);
-                        if (bins) deposit_panel(B, t_a, t_b, e_a, e_b, alpha, beta, gamma, kappa);
-                        e_a = e_b;
-                        e_last = e_b;
-                        j += step;
+                        a0 = 0.0; a2 = 0.0;
+                        for (int m = 0; m < n_mono; ++m) {
+                            double basis = G.opt.coef[m] * cpow[G.opt.ipow[m]] * spow[G.opt.jpow[m]];
+                            a0 += basis * row[m];
+                            a2 += basis * row[n_mono + m];
+                        }
                     }
-                    double t_f = t_base + s_buf[0] / v, t_end = t_base + s_buf[last] / v;
-                    double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa,
-                                                 e_last);
-                    total += exact;
-                    if (bins && spread > 0.0) {
-                        double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
-                        double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
-                        double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
-                        double flat = t_end > t_2
-                                          ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa, e_last)
-                                          : 0.0;
-                        double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
-                        deposit_panel(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa);
-                        if (t_end > t_2) deposit_panel(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa);
-                    } else if (bins && t_end > t_f) {
-                        deposit_panel(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa);
-                    }
-                    charge_l[0] += total;
+                    double den = g1 - gg * c;
+                    double value = path_jacobian * norm / (den * sqrt(den));
+                    acc0 += a0 * value;
+                    acc2 += a2 * value;
                 }
+                g0[k] = acc0;
+                g2[k] = acc2;
             }
         }
         __syncthreads();
-        start = stop;
+        // _batch_deposit (:451-551): one lane per wavelength
+        int lam = lane;
+        if (lam < nl && G.opt.mus[lam] != 0.0) {
+            double v = G.opt.speed[lam];
+            double mut = G.opt.mut[lam];
+            double kappa = -mut * v;
+            int last = n - 1;
+            double limit = r_b + G.opt.tail_exponent / mut;
+            while (last > 2 && s_buf[last - 1] > limit) --last;
+            if (last >= 2) {
+                double spread = fmin(sqrt(12.0 * tau_var), (s_buf[last] - r_b) / v);
+                double front_time = t_b + r_b / v;
+                double end_time = t_b + s_buf[last] / v;
+                int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
+                              || end_time + 0.5 * spread <= G.opt.edges[0];
+                double t_base = outside ? 0.0 : t_b;
+                double e_emit = exp(kappa * t_base);
+                double h_scale = G.opt.mus[lam] * v / e_emit;
+                double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
+                LambdaBins B;
+                B.edges = G.opt.edges;
+                B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
+                B.acc = acc + (size_t)lam * nb * 4;
+                B.pre = pre + 3 * lam;
+                B.nb = nb;
+                B.direct_bins = G.opt.direct_bins;
+                bool bins = G.compute_bins && !outside;
+                double h_front = w0 * g0[0] + w2 * g2[0];
+                double total = 0.0;
+                double e_first = e_emit * exp(-mut * s_buf[0]);
+                double e_a = e_first, e_last = e_first;
+                int j = 0;
+                while (j < last) {
+                    double t_a, t_e, alpha, beta, gamma, e_b;
+                    int step;
+                    if (j + 2 <= last) {
+                        t_a = t_base + s_buf[j] / v;
+                        double t_m = t_base + s_buf[j + 1] / v;
+                        t_e = t_base + s_buf[j + 2] / v;
+                        double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                        double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                        double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
+                        double u_m = t_m - t_a, u_b = t_e - t_a;
+                        double d1 = (h1 - h0) / u_m;
+                        double d2 = ((h2 - h1) / (t_e - t_m) - d1) / u_b;
+                        alpha = h0;
+                        beta = d1 - d2 * u_m;
+                        gamma = d2;
+                        e_b = e_emit * exp(-mut * s_buf[j + 2]);
+                        step = 2;
+                    } else {
+                        double t_p = t_base + s_buf[j - 1] / v;
+                        t_a = t_base + s_buf[j] / v;
+                        t_e = t_base + s_buf[j + 1] / v;
+                        double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
+                        double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                        double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                        double u_m = t_a - t_p, u_b = t_e - t_p;
+                        double d1 = (h0 - hp) / u_m;
+                        double d2 = ((h1 - h0) / (t_e - t_a) - d1) / u_b;
+                        alpha = h0;
+                        beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
+                        gamma = d2;
+                        e_b = e_emit * exp(-mut * s_buf[j + 1]);
+                        step = 1;
+                    }
+                    total += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
+                    if (bins) deposit_panel(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa);
+                    e_a = e_b;
+                    e_last = e_b;
+                    j += step;
+                }
+                double t_f = t_base + s_buf[0] / v, t_end = t_base + s_buf[last] / v;
+                double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
+                total += exact;
+                if (bins && spread > 0.0) {
+                    double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
+                    double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
+                    double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
+                    double flat = t_end > t_2
+                                      ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa, e_last)
+                                      : 0.0;
+                    double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
+                    deposit_panel(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa);
+                    if (t_end > t_2) deposit_panel(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa);
+                } else if (bins && t_end > t_f) {
+                    deposit_panel(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa);
+                }
+                charge_lane += total;
+            }
+        }
+        __syncthreads();
     }
-    // charge of the OM: sum over wavelengths in order (_finalize / charge-only)
-    __shared__ double lambda_charge[kMaxBands];
-    if (threadIdx.x < nl) lambda_charge[threadIdx.x] = charge_l[0];
-    __syncthreads();
+    if (lane < nl) G.part_charge[slot * nl + lane] = charge_lane;
+}
+
+// Segments summed in order, then the per-wavelength prefix pass (_finalize :610-632).
+__global__ void __launch_bounds__(kThreads) o1_finalize(Group G) {
+    int w = blockIdx.x;
+    int d = G.om.work[w];
+    int nl = G.opt.bands, nb = G.opt.bins, S = G.segments;
+    size_t base = (size_t)w * S;
+    double* acc0 = G.part_acc + base * nl * nb * 4;
+    double* pre0 = G.part_pre + base * nl * 3;
     if (threadIdx.x == 0) {
         double total = 0.0;
-        for (int l = 0; l < nl; ++l) total += lambda_charge[l];
+        for (int l = 0; l < nl; ++l) {
+            double c = 0.0;
+            for (int s = 0; s < S; ++s) c += G.part_charge[(base + s) * nl + l];
+            total += c;
+        }
         G.charge[d] = G.om.areas[d] * total;
     }
     if (!G.compute_bins) return;
-    // _finalize (:610-632): per-wavelength prefix pass, then bins summed over wavelengths
+    for (size_t i = threadIdx.x; i < (size_t)nl * nb * 4; i += kThreads) {
+        double v = acc0[i];
+        for (int s = 1; s < S; ++s) v += G.part_acc[(base + s) * nl * nb * 4 + i];
+        acc0[i] = v;
+    }
+    for (int i = threadIdx.x; i < nl * 3; i += kThreads) {
+        double v = pre0[i];
+        for (int s = 1; s < S; ++s) v += G.part_pre[(base + s) * nl * 3 + i];
+        pre0[i] = v;
+    }
+    __syncthreads();
     double* lb = G.lambda_bins + (size_t)w * nl * nb;
     if (threadIdx.x < nl) {
         int lam = threadIdx.x;
         const double* eedge = G.opt.eedge + (size_t)lam * (nb + 1);
         const double* it = G.opt.itab + (size_t)lam * nb * 3;
-        const double* a4 = acc + (size_t)lam * nb * 4;
-        double a = pre[3 * lam], b = pre[3 * lam + 1], c = pre[3 * lam + 2];
+        const double* a4 = acc0 + (size_t)lam * nb * 4;
+        double a = pre0[3 * lam], b = pre0[3 * lam + 1], c = pre0[3 * lam + 2];
         for (int k = 0; k < nb; ++k) {
             double e_k = G.opt.edges[k];
             double width = G.opt.edges[k + 1] - e_k;
@@ -1309,7 +1401,7 @@ unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads);
 
 struct GroupLayout {
     size_t keys, vals, sorted_keys, sorted_vals, seg_offsets, group_flag, group_scan, group_start,
-        om_groups, prepared, acc, pre, lambda_bins, info, sort_temp, scan_temp, total;
+        om_groups, prepared, lambda_bins, k_range, max_key, info, sort_temp, scan_temp, total;
 };
 
 int grouped_layout(int work, int n_pen, int bands, int bins, GroupLayout& L) {
@@ -1330,6 +1422,8 @@ int grouped_layout(int work, int n_pen, int bands, int bins, GroupLayout& L) {
     size_t at = 0;
     auto take = [&](size_t bytes) { size_t here = at; at += align_up(bytes); return here; };
     L.info = take(8 * sizeof(int));
+    L.max_key = take(sizeof(unsigned long long));
+    L.k_range = take((size_t)work * 4 * sizeof(int));
     L.keys = take(entries * sizeof(long long));
     L.vals = take(entries * sizeof(int));
     L.sorted_keys = take(entries * sizeof(long long));
@@ -1340,10 +1434,45 @@ int grouped_layout(int work, int n_pen, int bands, int bins, GroupLayout& L) {
     L.group_start = take((entries + 1) * sizeof(int));
     L.om_groups = take((work + 1) * sizeof(int));
     L.prepared = take((size_t)n_pen * 7 * sizeof(double));
-    L.acc = take((size_t)work * bands * bins * 4 * sizeof(double));
-    L.pre = take((size_t)work * bands * 3 * sizeof(double));
     L.lambda_bins = take((size_t)work * bands * bins * sizeof(double));
     L.sort_temp = take(sort_temp);
+    L.scan_temp = take(scan_temp);
+    L.total = at;
+    return LH_OK;
+}
+
+// Batch segments per OM: enough waves to fill the GPU, partials within 512 MiB.
+int segments_for(int work, int bands, int bins) {
+    int S = std::max(1, std::min(32, (2048 + work - 1) / work));
+    while (S > 1 && (double)work * S * bands * bins * 32.0 > 512.0 * 1024 * 1024) S /= 2;
+    return S;
+}
+
+struct ScratchLayout {
+    size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
+        part_charge, scan_temp, total;
+    int segments;
+};
+
+int scratch_layout(int work, int bands, int bins, int n_mono, int groups, ScratchLayout& L) {
+    size_t scan_temp = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
+                                         groups + 1, (hipStream_t)0) != hipSuccess) {
+        lh::set_error("scan workspace query failed");
+        return LH_ERR_HIP;
+    }
+    L.segments = segments_for(work, bands, bins);
+    size_t parts = (size_t)work * L.segments;
+    size_t at = 0;
+    auto take = [&](size_t bytes) { size_t here = at; at += align_up(bytes); return here; };
+    L.values = take((size_t)std::max(groups, 1) * (2 * n_mono + 6) * sizeof(double));
+    L.batch_flag = take((groups + 1) * sizeof(int));
+    L.batch_scan = take((groups + 1) * sizeof(int));
+    L.batch_start = take((groups + 1) * sizeof(int));
+    L.om_batches = take((work + 1) * sizeof(int));
+    L.part_acc = take(parts * bands * bins * 4 * sizeof(double));
+    L.part_pre = take(parts * bands * 3 * sizeof(double));
+    L.part_charge = take(parts * bands * sizeof(double));
     L.scan_temp = take(scan_temp);
     L.total = at;
     return LH_OK;
@@ -1358,6 +1487,7 @@ int check_grouped(const lh_order1_pencils* pen, const lh_order1_oms* om,
                   const lh_order1_optics* opt, const lh_order1_merge* merge) {
     LH_REQUIRE(pen && om && opt && merge, "descriptor is NULL");
     LH_REQUIRE(pen->count >= 1 && om->work_count >= 1, "no pencils or no OMs");
+    LH_REQUIRE(om->work_count <= 65535, "at most 65535 OMs per grouped call");
     LH_REQUIRE(opt->bands >= 1 && opt->bands <= kMaxBands, "1..%d wavelength bands", kMaxBands);
     LH_REQUIRE(opt->n_mono >= 1 && opt->n_mono <= kMaxMono && opt->max_power <= 4,
                "acceptance degree must be at most 4");
@@ -1375,6 +1505,7 @@ Group make_group(const lh_order1_pencils* pen, const lh_order1_oms* om,
                  const lh_order1_optics* opt, const lh_order1_merge* merge, char* base,
                  const GroupLayout& L) {
     Group G;
+    memset(&G, 0, sizeof(G));
     G.pen = *pen;
     G.om = *om;
     G.opt = *opt;
@@ -1393,11 +1524,12 @@ Group make_group(const lh_order1_pencils* pen, const lh_order1_oms* om,
     G.group_start = reinterpret_cast<int*>(base + L.group_start);
     G.om_groups = reinterpret_cast<int*>(base + L.om_groups);
     G.prepared = reinterpret_cast<double*>(base + L.prepared);
-    G.acc = reinterpret_cast<double*>(base + L.acc);
-    G.pre = reinterpret_cast<double*>(base + L.pre);
     G.lambda_bins = reinterpret_cast<double*>(base + L.lambda_bins);
-    G.values = nullptr;
+    G.k_range = reinterpret_cast<int*>(base + L.k_range);
+    G.max_key = reinterpret_cast<unsigned long long*>(base + L.max_key);
     G.info = reinterpret_cast<int*>(base + L.info);
+    G.mut_min = opt->mut_min;
+    G.speed_max = opt->speed_max;
     return G;
 }
 
@@ -1519,6 +1651,15 @@ size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
     return L.total;
 }
 
+size_t lh_order1_group_scratch_bytes(const lh_order1_oms* oms, const lh_order1_optics* optics,
+                                     int groups) {
+    if (!oms || !optics || groups < 0) return 0;
+    ScratchLayout L;
+    if (scratch_layout(oms->work_count, optics->bands, optics->bins, optics->n_mono, groups, L)
+        != LH_OK) return 0;
+    return L.total;
+}
+
 int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                          const lh_order1_optics* optics, const lh_order1_merge* merge, int* info,
                          void* workspace, size_t workspace_bytes, void* stream) {
@@ -1535,15 +1676,25 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     long long total = 2LL * work * pencils->count;
     o1_prepare_shape<<<blocks(pencils->count), kThreads, 0, s>>>(G);
     if ((status = lh::check_launch("o1_prepare_shape")) != LH_OK) return status;
+    o1_range_init<<<blocks(work), kThreads, 0, s>>>(G, work);
+    if ((status = lh::check_launch("o1_range_init")) != LH_OK) return status;
     dim3 grid(blocks(pencils->count), work);
+    o1_group_ranges<<<grid, kThreads, 0, s>>>(G);
+    if ((status = lh::check_launch("o1_group_ranges")) != LH_OK) return status;
     o1_group_keys<<<grid, kThreads, 0, s>>>(G);
     if ((status = lh::check_launch("o1_group_keys")) != LH_OK) return status;
     o1_offsets<<<blocks(work + 1), kThreads, 0, s>>>(G.seg_offsets, work, 2 * pencils->count);
     if ((status = lh::check_launch("o1_offsets")) != LH_OK) return status;
+    // the sort covers the bits of the largest compact key; the sentinel's low bits
+    // are all ones, so it stays after every key
+    unsigned long long top = 0;
+    LH_HIP_CALL(hipMemcpyAsync(&top, G.max_key, sizeof(top), hipMemcpyDeviceToHost, s));
+    LH_HIP_CALL(hipStreamSynchronize(s));
+    int end_bit = bits_for(top + 1);
     size_t sort_bytes = L.scan_temp - L.sort_temp;
     LH_HIP_CALL(hipcub::DeviceSegmentedRadixSort::SortPairs(
         base + L.sort_temp, sort_bytes, G.keys, G.sorted_keys, G.vals, G.sorted_vals, (int)total,
-        work, G.seg_offsets, G.seg_offsets + 1, 0, 63, s));
+        work, G.seg_offsets, G.seg_offsets + 1, 0, end_bit, s));
     o1_group_flags<<<blocks(total + 1), kThreads, 0, s>>>(G, total);
     if ((status = lh::check_launch("o1_group_flags")) != LH_OK) return status;
     size_t scan_bytes = L.total - L.scan_temp;
@@ -1557,8 +1708,9 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
 
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
-                           int groups, double* values, int compute_bins, double* charge,
-                           double* bins, void* workspace, size_t workspace_bytes, void* stream) {
+                           int groups, void* scratch, size_t scratch_bytes, int compute_bins,
+                           double* charge, double* bins, void* workspace, size_t workspace_bytes,
+                           void* stream) {
     int status = check_grouped(pencils, oms, optics, merge);
     if (status != LH_OK) return status;
     GroupLayout L;
@@ -1568,23 +1720,45 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
                workspace_bytes, L.total);
     LH_REQUIRE(charge != nullptr && (!compute_bins || bins != nullptr),
                "charge (and bins) must be set");
-    LH_REQUIRE(groups >= 0 && (groups == 0 || values != nullptr), "values buffer needed");
     LH_REQUIRE(optics->mut_min > 0 && optics->speed_max > 0, "mut_min and speed_max must be set");
+    int work = oms->work_count;
+    ScratchLayout SL;
+    if ((status = scratch_layout(work, optics->bands, optics->bins, optics->n_mono,
+                                 std::max(groups, 0), SL)) != LH_OK) return status;
+    LH_REQUIRE(groups >= 0 && scratch && scratch_bytes >= SL.total,
+               "scratch too small: %zu < %zu", scratch_bytes, SL.total);
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* base = static_cast<char*>(workspace);
+    char* sc = static_cast<char*>(scratch);
     Group G = make_group(pencils, oms, optics, merge, base, L);
     G.compute_bins = compute_bins;
     G.charge = charge;
     G.bins = bins;
-    G.values = values;
-    G.mut_min = optics->mut_min;
-    G.speed_max = optics->speed_max;
+    G.values = reinterpret_cast<double*>(sc + SL.values);
+    G.batch_flag = reinterpret_cast<int*>(sc + SL.batch_flag);
+    G.batch_scan = reinterpret_cast<int*>(sc + SL.batch_scan);
+    G.batch_start = reinterpret_cast<int*>(sc + SL.batch_start);
+    G.om_batches = reinterpret_cast<int*>(sc + SL.om_batches);
+    G.part_acc = reinterpret_cast<double*>(sc + SL.part_acc);
+    G.part_pre = reinterpret_cast<double*>(sc + SL.part_pre);
+    G.part_charge = reinterpret_cast<double*>(sc + SL.part_charge);
+    G.segments = SL.segments;
     if (groups > 0) {
-        o1_group_sums<<<blocks(groups), kThreads, 0, s>>>(G, groups);
+        o1_group_sums<<<(groups + 3) / 4, kThreads, 0, s>>>(G, groups);
         if ((status = lh::check_launch("o1_group_sums")) != LH_OK) return status;
     }
-    o1_batches<<<oms->work_count, kThreads, 0, s>>>(G);
-    return lh::check_launch("o1_batches");
+    o1_batch_flags<<<blocks(groups + 1), kThreads, 0, s>>>(G, groups, work);
+    if ((status = lh::check_launch("o1_batch_flags")) != LH_OK) return status;
+    size_t scan_bytes = SL.total - SL.scan_temp;
+    LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, G.batch_flag,
+                                                 G.batch_scan, groups + 1, s));
+    o1_batch_starts<<<blocks(std::max(groups, work) + 1), kThreads, 0, s>>>(G, groups, work);
+    if ((status = lh::check_launch("o1_batch_starts")) != LH_OK) return status;
+    dim3 grid(work, SL.segments);
+    o1_batches<<<grid, 64, 0, s>>>(G);
+    if ((status = lh::check_launch("o1_batches")) != LH_OK) return status;
+    o1_finalize<<<work, kThreads, 0, s>>>(G);
+    return lh::check_launch("o1_finalize");
 }
 
 }  // extern "C"

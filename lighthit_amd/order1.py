@@ -73,9 +73,10 @@ _lib.SIGNATURES.update({
     "lh_order1_group_keys": (_I, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
                                   ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _P, _P, _S,
                                   _P]),
+    "lh_order1_group_scratch_bytes": (_S, [ctypes.POINTER(_Oms), ctypes.POINTER(_Optics), _I]),
     "lh_order1_group_charge": (_I, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
-                                    ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _I, _P, _I,
-                                    _P, _P, _P, _S, _P]),
+                                    ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _I, _P, _S,
+                                    _I, _P, _P, _P, _S, _P]),
 })
 
 _GL48 = np.polynomial.legendre.leggauss(48)
@@ -374,12 +375,14 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
               ctypes.byref(optics.struct), ctypes.byref(m), _addr(info), _addr(ws),
               ctypes.c_size_t(ws.numel()), handle)
     groups = int(info[0].item())
-    values = torch.empty(max(groups, 1) * (2 * optics.n_mono + 6), dtype=torch.float64,
-                         device=device)
+    scratch_bytes = _size("lh_order1_group_scratch_bytes", ctypes.byref(om),
+                          ctypes.byref(optics.struct), groups)
+    scratch = _workspace("group_scratch", device, scratch_bytes)
     _lib.call("lh_order1_group_charge", ctypes.byref(pen), ctypes.byref(om),
-              ctypes.byref(optics.struct), ctypes.byref(m), groups, _addr(values),
-              1 if compute_bins else 0, _addr(charge), _addr(bins) if compute_bins else None,
-              _addr(ws), ctypes.c_size_t(ws.numel()), handle)
+              ctypes.byref(optics.struct), ctypes.byref(m), groups, _addr(scratch),
+              ctypes.c_size_t(scratch.numel()), 1 if compute_bins else 0, _addr(charge),
+              _addr(bins) if compute_bins else None, _addr(ws), ctypes.c_size_t(ws.numel()),
+              handle)
     del work_dev
     return charge, bins
 
