@@ -1,0 +1,91 @@
+"""CPU tests of the route's host logic and of the oracle route composition."""
+import json
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from lighthit_amd.route import (HybridTransportResponse, ShowerConfig, ShowerData, delay_groups,
+                                screen_radial_tail, ShowerBasis)
+from lighthit_amd.sources import SourcePose, SyntheticShower
+
+
+def test_delay_groups_split_prompt_delayed_and_beyond():
+    elements = SyntheticShower.gaussian(SourcePose([0., 0., 0.], [0., 0., 1.]),
+                                        charged_track_length_m=20.0, elements=60, seed=1).elements
+    start = elements.start_ns.copy()
+    start[10:20] += 55.0          # delayed inside the window
+    start[20:25] += 5000.0        # beyond the window
+    moved = replace(elements, start_ns=start, end_ns=elements.end_ns + (start - elements.start_ns))
+    prompt, groups, beyond = delay_groups(moved, prompt_delay_ns=20.0, cell_ns=20.0,
+                                          window_end_ns=745.0, group_index_max=1.39)
+    assert np.all(prompt[:10]) and np.all(prompt[25:])
+    assert np.all(beyond[20:25]) and not np.any(beyond[:20])
+    in_groups = np.zeros(len(moved), bool)
+    for mask, shift in groups:
+        assert not np.any(in_groups & mask)
+        in_groups |= mask
+        assert shift > 20.0
+    assert np.array_equal(in_groups, ~prompt & ~beyond)
+
+
+def test_config_validation():
+    with pytest.raises(ValueError):
+        ShowerConfig(band_rank=0)
+    with pytest.raises(ValueError):
+        ShowerConfig(predictor_safety=0.5)
+    with pytest.raises(ValueError):
+        ShowerBasis(radial_tail_policy="ignore")
+
+
+def test_radial_tail_screen_warns_and_raises():
+    basis = ShowerBasis()
+    with pytest.warns(RuntimeWarning):
+        entry = screen_radial_tail([0.0, 1e-3], basis, 150.0)
+    assert entry["oms_above_tolerance"] == [1]
+    with pytest.raises(ValueError):
+        screen_radial_tail([1e-3], replace(basis, radial_tail_policy="raise"), 150.0)
+
+
+def test_response_select_and_save(tmp_path):
+    rng = np.random.default_rng(0)
+    components = rng.uniform(size=(4, 6, 3))
+    response = HybridTransportResponse(None, np.linspace(0, 1.2, 5), np.arange(7.0),
+                                       np.zeros((5, 4), complex), components,
+                                       components.sum(axis=1), np.zeros(4),
+                                       np.array([True, False, True, True]), "shower", {"a": 1})
+    np.testing.assert_allclose(response.select(">=2").charge_pe, components[:, :, 2].sum(axis=1))
+    with pytest.raises(ValueError):
+        response.select("3")
+    path = response.save(tmp_path / "r.npz")
+    with np.load(path, allow_pickle=False) as saved:
+        meta = json.loads(str(saved["metadata"]))
+    assert meta["schema"] == "lighthit/hybrid-response/1" and meta["a"] == 1
+
+
+def test_shower_data_manifest(tmp_path):
+    (tmp_path / "pack").mkdir()
+    (tmp_path / "shower-data.json").write_text(json.dumps({"ge2_pack": "pack"}))
+    data = ShowerData.load(tmp_path)
+    assert data.ge2_pack == (tmp_path / "pack").resolve() and data.ge2_band_fold is None
+
+
+def test_oracle_route_reproduces_the_published_demonstration_orders():
+    """The oracle route on the README event: orders 0 and 1 (63.74 / 50.68 pe quoted;
+    order 0 of the current first-entry method is 64.16 pe, see the order-0 oracle)."""
+    from route_cases import demonstration_route
+    from oracle import route as oracle_route
+    from lighthit_amd.fold import BandFold
+    from lighthit_amd.route import RouteConfig, RouteKernel
+    from lighthit_amd.fold import KernelPack
+    from lighthit_amd.synthetic import plain_small, synthetic_pack
+    w = plain_small()
+    kernel = RouteKernel(w.medium, w.detector, RouteConfig(relative_time_edges_ns=w.edges_ns))
+    waves, indices, bundle = synthetic_pack(w.medium, kernel.wavelength)
+    fold = BandFold.of(kernel, rank=3)
+    out = oracle_route.transport_shower(kernel, w.elements, bundle, fold, config=ShowerConfig())
+    totals = out["charge"].sum(axis=0)
+    assert abs(totals[0] - 64.164) < 0.01
+    assert abs(totals[1] - 50.68) < 0.01
+    assert not out["fallback"]
+    assert out["active"].sum() >= 34
