@@ -1,25 +1,30 @@
-"""Benchmark of the ShowerTransport GPU hot path on one MI355X.
+"""Benchmark of the ShowerTransport call on one MI355X.
 
-One step = one shower event through the GPU stages of the plain (FP32) path,
-inputs resident in HBM:
-  (a) order-0 first-entry charge of every OM from every element;
-  (b) order >=2 apply of the band-folded pack: zero-frequency charge at every
-      OM inside the radial window of the source points, then the full
-      65-frequency spectrum at those OMs;
-  (c) readout of those spectra into the workload's time bins.
-Workload (default): the detector-scale event — 2304 OMs, 50,000-element
-Gaussian shower of seed 11, 1 ns bins (lighthit_amd.synthetic). The order >=2
-pack and node-band channels have the route's shapes (3 node bands, 47 radial
-intervals, degree 8, 6 detector pairs, 65 frequencies; 165 points on the
-shower axis) with seeded random values: their cost does not depend on values.
+One step = one shower event through ``ShowerTransport(kernel, data)(shower)``:
+the whole route of the plain FP32 GPU form, as a user calls it — node-band
+order >=2 compile, radial windows, collapsed predictor, orders 0 and 1, exact
+zero-frequency order >=2 where the threshold needs it, the full 65-frequency
+apply at the active OMs and the readout into time bins. The detector
+description, the RTE tables and the folded order >=2 pack are resident before
+the timed region (built once, as a served route keeps them); every step hands
+the route a fresh source object, so the per-event compile is never a cache hit.
 
-Prints one JSON line (metric, value, roofline of the dominant kernel, CPU
-baseline of the oracle on a bounded sample). ``--gpus`` above 1 exits: this
-build runs on one GPU.
+Workload (default): the detector-scale event — 2304 OMs (8 clusters x 8
+strings x 36 OMs), a 50,000-element Gaussian shower of seed 11 with a 1500 m
+charged track, 8 wavelength nodes, 1 ns bins over -60..745 ns, 65 RTE and 321
+readout frequencies, degree 8, azimuthal degree 2, radial window 2..150 m,
+order-1 selection radius 300 m. The order >=2 pack is the seeded synthetic
+pack of lighthit_amd.synthetic (the reference's pack needs its RTE solver).
+
+Prints one JSON line: events/s, p50/p99 latency, GPU time per stage and per
+dominant kernel (HIP events on the launch stream), the roofline of the
+dominant kernel and the FP64 oracle timed on the host over a bounded sample.
+``--gpus`` above 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
 import sys
@@ -34,93 +39,169 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
-
-def axial_points(frame, rng, *, half_length=3.6, cell=0.25, stencil=0.2):
-    """Points of an axial-slice source: slices every ``cell`` m, 5-point stencils."""
-    z = np.arange(-half_length, half_length + 1e-9, cell)
-    offsets = np.array([[0, 0], [stencil, 0], [-stencil, 0], [0, stencil], [0, -stencil]])
-    zz = np.repeat(z, len(offsets))
-    tt = np.tile(offsets, (len(z), 1)).astype(float)
-    extra = 165 - len(zz)
-    if extra > 0:
-        zz = np.r_[zz, rng.uniform(-half_length, half_length, extra)]
-        tt = np.vstack([tt, rng.normal(scale=0.05, size=(extra, 2))])
-    return zz[:165], tt[:165]
+#: Kernels the library times with HIP events on their stream.
+TIMED_KERNELS = ("order0_pairs<false>", "order0_pairs<true>", "o1_batches", "o1_group_sums",
+                 "ge2_wide", "ge2_narrow", "readout_kernel", "compile_deposit", "compile_phase")
 
 
-def build_ge2_inputs(workload, rng):
-    from lighthit_amd.axial import AxialSource, AxisFrame
-    from lighthit_amd.ge2 import canonical_pairs
-
-    class _Photons:
-        def __init__(self, e):
-            self.midpoints_m = e.midpoints_m
-            self.photons = e.coefficient0
-            self.direction = e.direction
-    frame = AxisFrame.of(_Photons(workload.elements))
-    degree, az, bands, nint, nfreq = workload.source_degree, workload.azimuthal_degree, 3, 47, 65
-    pl, pm = canonical_pairs(2)
-    log_r = np.linspace(np.log(2.0), np.log(150.0), nint + 1)
-    coef = (rng.normal(size=(bands, nint, degree + 1, len(pl), 4, nfreq))
-            + 1j * rng.normal(size=(bands, nint, degree + 1, len(pl), 4, nfreq))).astype(np.complex64)
-    bundle = dict(degree=degree, acceptance_degree=2, pair_lambda=pl, pair_mu=pm,
-                  omega=workload.omega_per_ns, phase="flight", coefficients=coef, log_r=log_r,
-                  absorption=np.array([0.02, 0.02, 0.02]), speed=np.array([0.218, 0.218, 0.218]))
-    z, tr = axial_points(frame, rng)
-    nchan = sum(2 * min(l, az) + 1 for l in range(degree + 1))
-    sources = [AxialSource(frame, z, tr, (rng.normal(size=(len(z), nchan, nfreq))
-                                          + 1j * rng.normal(size=(len(z), nchan, nfreq))),
-                           degree, az, 0.0) for _ in range(bands)]
-    return bundle, sources
+def build_route(name):
+    """Workload, route (tables resident) and the event's source."""
+    from lighthit_amd.fold import KernelPack
+    from lighthit_amd.route import (RouteConfig, RouteKernel, ShowerConfig, ShowerPrompt,
+                                    ShowerTransport)
+    from lighthit_amd.synthetic import plain_detector_scale, plain_small, synthetic_pack
+    workload = plain_detector_scale() if name == "detector" else plain_small()
+    kernel = RouteKernel(workload.medium, workload.detector,
+                         RouteConfig(relative_time_edges_ns=workload.edges_ns,
+                                     wavelength_nodes=len(workload.wavelength.wavelength_nm),
+                                     omega_per_ns=workload.omega_per_ns,
+                                     readout_omega_per_ns=workload.readout_omega_per_ns))
+    pack = KernelPack()
+    pack.add(*synthetic_pack(workload.medium, kernel.wavelength,
+                             omega=workload.omega_per_ns))
+    config = ShowerConfig(prompt=ShowerPrompt(max_distance_m=300.0))
+    route = ShowerTransport(kernel, pack, config=config)
+    return workload, kernel, route
 
 
-def reachable(sources, positions, window):
-    points = sources[0].points_m()
-    dist = np.linalg.norm(positions[:, None, :] - points[None, :, :], axis=2)
-    return np.flatnonzero(np.any((dist >= window[0]) & (dist <= window[1]), axis=1))
+# ----------------------------------------------------------- roofline models
+
+def order0_bytes(n_elements, n_oms, bins):
+    """Algorithmic HBM bytes of one order-0 launch: every element read once
+    (12 FP32 words: start, direction, length, times, two coefficients, beta,
+    weight), every OM once (position, look, area, radius: 9 words) and the
+    FP64 result written once (a charge, or ``bins`` per OM)."""
+    return 48 * n_elements + 36 * n_oms + 8 * n_oms * max(1, bins)
 
 
-def cpu_baseline(workload, bundle, sources, window, ids, alpha, fraction):
-    """The oracle on the host: order 0 on a fraction of the elements, (b) and (c) in full."""
+def ge2_flops_per_term(degree, azimuthal, acceptance, pairs):
+    """FP32 flops of one (OM, source point, band, frequency) term of the full
+    order >=2 apply: the cubic radial interpolation of every (multipole,
+    pair) coefficient (4 complex-by-real multiply-adds, 4 flops each), the
+    pair sum into the detector-side vector (4 flops per pair), the Wigner
+    rotation of the source channels (complex-by-real, 4 flops per matrix
+    element), their complex dot product (8 flops per term) and the phase and
+    scale (16 flops)."""
+    cubic = (degree + 1) * pairs * 4 * 4
+    pair_sum = (degree + 1) * pairs * 4
+    rotation = sum((2 * min(l, acceptance) + 1) * (2 * min(l, azimuthal) + 1)
+                   for l in range(degree + 1)) * 4
+    dot = sum(2 * min(l, acceptance) + 1 for l in range(degree + 1)) * 8
+    return cubic + pair_sum + rotation + dot + 16
+
+
+def ge2_window_terms(route, positions, window):
+    """(OM, source point, band) triples inside the radial window for the OMs
+    ``positions`` and the node-band points of the route's last compile."""
+    sources = route._compiled["sources"][0]
+    low, high = window
+    total = 0
+    for item in sources:
+        points = item.to_points_m()
+        distance = np.linalg.norm(positions[:, None, :] - points[None, :, :], axis=2)
+        total += int(np.count_nonzero((distance >= low) & (distance <= high)))
+    return total
+
+
+def cpu_baseline(workload, kernel, route, response, fraction):
+    """The FP64 oracle on the host cores, one thread, over a bounded sample of
+    the same event: order 0 (charge at the selected OMs, bins at the active
+    OMs) on every ``1/fraction``-th element scaled to all elements; order 1,
+    the node-band compile, the exact zero-frequency and full order >=2 apply
+    and the readout in full on the OM sets the GPU route chose."""
+    from threadpoolctl import threadpool_limits
+
+    from lighthit_amd.axial import AxisFrame
     from lighthit_amd.optics import prepare_optics
     from oracle import ge2 as oracle_ge2
     from oracle import order0 as oracle_order0
+    from oracle import order1 as oracle_order1
     from oracle import readout as oracle_readout
+    from oracle import route as oracle_route
+
     d = workload.detector
-    optics = prepare_optics(workload.medium, d)
-    keep = np.arange(len(workload.elements)) % int(round(1 / fraction)) == 0
-    sample = workload.elements.subset(keep)
+    elements = workload.elements
+    tables = route.tables()
+    fold = tables["fold"]
+    folded = dict(tables["folded"])
+    folded["coefficients"] = np.asarray(folded["coefficients"].cpu().numpy(), np.complex64)
+    zero = dict(folded, coefficients=np.ascontiguousarray(folded["coefficients"][..., :1]),
+                omega=np.asarray(folded["omega"][:1]))
+    alpha = np.asarray(kernel.acceptance()["alpha"], float)
+    optics = prepare_optics(kernel.medium, d)
     radii = d.receiver_surface.radii(len(d))
-    began = time.perf_counter()
-    # the reference's own tolerance: 2e-6 relative per OM
-    oracle_order0.first_entry(d.positions_m, d.looks, d.effective_area_m2, radii, sample, optics,
-                              alpha, rel_tol=2e-6)
-    t_order0 = (time.perf_counter() - began) / keep.mean()
-    began = time.perf_counter()
-    pos, looks, area = d.positions_m[ids], d.looks[ids], d.effective_area_m2[ids]
-    zero = dict(bundle, coefficients=np.ascontiguousarray(bundle["coefficients"][..., :1]),
-                omega=bundle["omega"][:1])
-    from dataclasses import replace
-    zero_sources = [replace(s, channels=np.ascontiguousarray(s.channels[:, :, :1])) for s in sources]
-    oracle_ge2.apply_folded(zero, zero_sources, pos, looks, alpha, area, window)
-    spectrum = oracle_ge2.apply_folded(bundle, sources, pos, looks, alpha, area, window)
-    t_ge2 = time.perf_counter() - began
-    began = time.perf_counter()
-    oracle_readout.route_readout(workload.omega_per_ns, spectrum, np.full(len(ids), 500.0),
-                                 workload.readout_omega_per_ns, workload.edges_ns)
-    t_readout = time.perf_counter() - began
-    total = t_order0 + t_ge2 + t_readout
-    return {"value": 1.0 / total, "unit": "events/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
-            "sample": (f"FP64 oracle, one event: order 0 on every {int(round(1 / fraction))}th "
-                       f"element ({int(keep.sum())} of {len(keep)}) scaled to all, at the "
-                       f"reference tolerance 2e-6; order >=2 zero-frequency and full apply and "
-                       f"readout at all {len(ids)} windowed OMs"),
-            "stage_seconds": {"order0": t_order0, "ge2": t_ge2, "readout": t_readout}}
+    level1 = np.asarray(response.metadata["prompt_order1_level"])
+    level2 = np.asarray(response.metadata["ge2_charge_level"])
+    work1 = np.flatnonzero(level1 > 0)
+    candidates = np.flatnonzero(level2 == 2)
+    active = np.flatnonzero(response.active)
+    full = np.flatnonzero(response.active & (level2 > 0))
+    origins = np.asarray(response.time_origin_ns, float)
+    edges = np.asarray(kernel.config.relative_time_edges_ns, float)
+    omega = np.asarray(kernel.config.omega_per_ns, float)
+    stride = max(1, int(round(1 / fraction)))
+    keep = np.arange(len(elements)) % stride == 0
+    sample = elements.subset(keep)
+    seconds = {}
+    with threadpool_limits(limits=1):
+        began = time.perf_counter()
+        oracle_order0.first_entry(d.positions_m[work1], d.looks[work1],
+                                  d.effective_area_m2[work1], radii[work1], sample, optics,
+                                  alpha, rel_tol=2e-6)
+        if len(active):
+            oracle_order0.first_entry(d.positions_m[active], d.looks[active],
+                                      d.effective_area_m2[active], radii[active], sample,
+                                      optics, alpha, origins=origins[active], edges=edges,
+                                      rel_tol=2e-6)
+        seconds["order0"] = (time.perf_counter() - began) * len(elements) / len(sample)
+        began = time.perf_counter()
+        spectral = oracle_order1.spectral_tables(kernel.medium, d, kernel.wavelength)
+        charge = response.charge_components_pe
+        oracle_order1.spectral_order1(elements, d.positions_m, d.looks, d.effective_area_m2,
+                                      origins, work1, spectral, alpha, float(kernel.medium.g),
+                                      edges, float(kernel.config.threshold_pe), charge[:, 0],
+                                      2.0 * np.abs(charge[:, 2]), oracle_order1.PromptSettings())
+        seconds["order1"] = time.perf_counter() - began
+        began = time.perf_counter()
+        wavelengths = np.asarray(kernel.wavelength.wavelength_nm, float)
+        max_phase = float(np.max(kernel.medium.sample(wavelengths)["phase_index"]))
+        frame = AxisFrame.of(elements.field(0, phase_index=max_phase))
+        sources = oracle_route._node_sources(
+            elements, wavelengths, fold, omega, frame, int(kernel.config.source_degree),
+            int(kernel.config.azimuthal_degree), route.config.basis)
+        seconds["compile"] = time.perf_counter() - began
+        began = time.perf_counter()
+        window = tuple(kernel.config.radial_range_m)
+        pos, looks, area = d.positions_m, d.looks, d.effective_area_m2
+        zero_sources = [copy.copy(s) for s in sources]
+        for s in zero_sources:
+            object.__setattr__(s, "channels", np.ascontiguousarray(s.channels[:, :, :1]))
+        if len(candidates):
+            oracle_ge2.apply_folded(zero, zero_sources, pos[candidates], looks[candidates],
+                                    alpha, area[candidates], window)
+        spectrum = np.zeros((len(omega), len(full)), complex)
+        if len(full):
+            spectrum = oracle_ge2.apply_folded(folded, sources, pos[full], looks[full], alpha,
+                                               area[full], window)
+        seconds["order2"] = time.perf_counter() - began
+        began = time.perf_counter()
+        if len(full):
+            oracle_readout.route_readout(omega, spectrum, origins[full],
+                                         kernel.config.readout_omega_per_ns, edges)
+        seconds["readout"] = time.perf_counter() - began
+    total = sum(seconds.values())
+    return {"value": 1.0 / total, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": (f"FP64 oracle (C and numpy, one thread), one event: order 0 on every "
+                       f"{stride}th element ({len(sample)} of {len(elements)}) at the reference "
+                       f"tolerance 2e-6, scaled to all elements, at {len(work1)} selected and "
+                       f"{len(active)} active OMs; order 1 on {len(work1)} OMs, node-band "
+                       f"compile, order >=2 zero frequency at {len(candidates)} and full "
+                       f"apply and readout at {len(full)} OMs in full; windows and predictor "
+                       f"not timed"),
+            "stage_seconds": seconds}
 
 
-DOMINANT_KERNEL = "order0_pairs"
-
+# --------------------------------------------------------- HBM traffic (pmc)
 
 def _counter_kib(db_dir, counter, name_part):
     """Average KiB per dispatch of ``counter`` for kernels whose name holds ``name_part``."""
@@ -129,16 +210,17 @@ def _counter_kib(db_dir, counter, name_part):
     values = []
     for db in glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True):
         cur = sqlite3.connect(db).cursor()
-        for kernel, value in cur.execute(
-                "select kernel_name, value from counters_collection where counter_name=?",
-                (counter,)):
-            if name_part in kernel:
-                values.append(float(value))
+        try:
+            rows = cur.execute("select kernel_name, value from counters_collection "
+                               "where counter_name=?", (counter,)).fetchall()
+        except sqlite3.Error:
+            continue
+        values += [float(v) for k, v in rows if name_part in k]
     return sum(values) / len(values) if values else None
 
 
-def measure_traffic(workload_flag):
-    """HBM bytes per launch of the dominant kernel from two counter runs, each
+def measure_traffic(workload_flag, kernel_name):
+    """HBM bytes per launch of ``kernel_name`` from two counter runs, each
     scaled by what the same counter reads for a plain copy of known size."""
     import shutil
     import subprocess
@@ -149,29 +231,62 @@ def measure_traffic(workload_flag):
     me = os.path.abspath(__file__)
     runs = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        for kind, args in (("bench", ["--steps", "2", "--warmup", "1", "--no-cpu-baseline",
+        for kind, args in (("bench", ["--steps", "1", "--warmup", "1", "--no-cpu-baseline",
                                       "--no-traffic", "--workload", workload_flag]),
                            ("copy", ["--copy-kernel-mib", "256"])):
             out = os.path.join(base, f"{kind}_{counter}")
             cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--",
                    sys.executable, me, *args]
-            done = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd="/tmp",
+            print(f"[bench] counter pass {counter} {kind}", file=sys.stderr, flush=True)
+            done = subprocess.run(cmd, capture_output=True, text=True, timeout=600, cwd="/tmp",
                                   env=dict(os.environ, TMPDIR="/tmp"))
             if done.returncode != 0:
                 return None, f"rocprofv3 {counter} {kind} failed ({done.returncode})"
             runs[(kind, counter)] = out
     result = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        copy = _counter_kib(runs[("copy", counter)], counter, "copyBuffer")
-        kernel = _counter_kib(runs[("bench", counter)], counter, DOMINANT_KERNEL)
-        if not copy or kernel is None:
+        copy_kib = _counter_kib(runs[("copy", counter)], counter, "copyBuffer")
+        kernel_kib = _counter_kib(runs[("bench", counter)], counter, kernel_name)
+        if not copy_kib or kernel_kib is None:
             return None, f"no {counter} samples"
-        scale = 256 * 1024 / copy    # the copy moves 256 MiB each way
-        result[counter] = (kernel * scale * 1024, scale)
+        scale = 256 * 1024 / copy_kib    # the copy moves 256 MiB each way
+        result[counter] = (kernel_kib * scale * 1024, scale)
     traffic = result["FETCH_SIZE"][0] + result["WRITE_SIZE"][0]
-    note = (f"{DOMINANT_KERNEL}: FETCH_SIZE x{result['FETCH_SIZE'][1]:.3f} + "
+    note = (f"{kernel_name}: FETCH_SIZE x{result['FETCH_SIZE'][1]:.3f} + "
             f"WRITE_SIZE x{result['WRITE_SIZE'][1]:.3f} (copy-calibrated), bytes per launch")
     return traffic, note
+
+
+# ------------------------------------------------------------------- driver
+
+def roofline_of(name, kernel_ms, workload, response, route):
+    """Roofline entry of kernel ``name`` at its average launch duration."""
+    d = workload.detector
+    n_el = len(workload.elements)
+    if name.startswith("order0_pairs"):
+        bins = 0 if name.endswith("<false>)") or name.endswith("<false>") else (
+            len(workload.edges_ns) - 1)
+        oms = len(d) if bins == 0 else int(np.count_nonzero(response.active))
+        achieved = order0_bytes(n_el, oms, bins) / (kernel_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "algorithmic": f"48 B/element x {n_el} + 36 B/OM x {oms} + 8 B x {max(1, bins)} "
+                               f"outputs/OM"}
+    if name == "ge2_wide":
+        from lighthit_amd.ge2 import canonical_pairs
+        level2 = np.asarray(response.metadata["ge2_charge_level"])
+        full = np.flatnonzero(response.active & (level2 > 0))
+        terms = ge2_window_terms(route, d.positions_m[full], workload.radial_range_m)
+        pack = route.tables()["device_pack"]
+        per = ge2_flops_per_term(workload.source_degree, workload.azimuthal_degree,
+                                 pack.acceptance_degree, len(canonical_pairs(pack.acceptance_degree)[0]))
+        flops = float(terms) * len(workload.omega_per_ns) * per
+        achieved = flops / (kernel_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                "algorithmic": f"{terms} (OM, point, band) terms in the window x "
+                               f"{len(workload.omega_per_ns)} frequencies x {per} flops"}
+    return None
 
 
 def main():
@@ -181,7 +296,7 @@ def main():
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--workload", choices=["detector", "small"], default="detector")
     parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--cpu-fraction", type=float, default=0.1)
+    parser.add_argument("--cpu-fraction", type=float, default=0.05)
     parser.add_argument("--no-traffic", action="store_true",
                         help="skip the rocprofv3 counter runs that measure HBM traffic")
     parser.add_argument("--copy-kernel-mib", type=int, default=0,
@@ -198,99 +313,95 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(multi-GPU sharding by OM is a design note only)", file=sys.stderr)
+              "(sharding the route by OM is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    import warnings
+
     import torch
-    from lighthit_amd import ge2 as dge2
-    from lighthit_amd.ballistic import DeviceElements, Order0
-    from lighthit_amd.model import acceptance_alpha
-    from lighthit_amd.readout import ReadoutBasis
-    from lighthit_amd.synthetic import plain_detector_scale, plain_small
+    from lighthit_amd._lib import KernelTimer, StageClock
 
-    workload = plain_detector_scale() if args.workload == "detector" else plain_small()
-    rng = np.random.default_rng(1234)
-    d = workload.detector
-    alpha = acceptance_alpha(d)
-    window = workload.radial_range_m
-    bundle, sources = build_ge2_inputs(workload, rng)
-    ids = reachable(sources, d.positions_m, window)
-    dev = "cuda"
-
-    order0 = Order0(workload.medium, d, alpha)
-    elements = DeviceElements(workload.elements)
-    pack = dge2.DevicePack(bundle)
-    source = dge2.DeviceSource(sources)
-    receivers = dge2.DeviceReceivers(source.frame, d.positions_m[ids], d.looks[ids],
-                                     np.repeat(d.effective_area_m2[ids][None], 3, axis=0))
-    basis = ReadoutBasis(workload.omega_per_ns, workload.edges_ns, workload.readout_omega_per_ns)
-    origins = torch.full((len(ids),), 500.0, dtype=torch.float64, device=dev)
-    charge = torch.empty(len(d), dtype=torch.float64, device=dev)
-    zero = torch.empty((1, len(ids)), dtype=torch.complex64, device=dev)
-    spectrum = torch.empty((pack.nfreq, len(ids)), dtype=torch.complex64, device=dev)
-    bins = torch.empty((len(ids), basis.bins), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
-    events = {k: [] for k in ("order0", "ge2_zero", "ge2_full", "readout")}
-
-    def step(timed):
-        marks = [torch.cuda.Event(enable_timing=True) for _ in range(5)] if timed else None
-        if timed: marks[0].record(stream)
-        order0.charge_device(elements, out=charge, stream=stream)
-        if timed: marks[1].record(stream)
-        if len(ids):
-            dge2.apply_device(pack, source, receivers, alpha, window, frequencies=1, out=zero,
-                              stream=stream)
-        if timed: marks[2].record(stream)
-        if len(ids):
-            dge2.apply_device(pack, source, receivers, alpha, window, out=spectrum, stream=stream)
-        if timed: marks[3].record(stream)
-        if len(ids):
-            basis.apply(spectrum, origins, out=bins, stream=stream)
-        if timed: marks[4].record(stream)
-        return marks
-
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-    all_marks = []
-    began = time.perf_counter()
-    for _ in range(args.steps):
-        all_marks.append(step(True))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - began
-    for marks in all_marks:
-        for key, (a, b) in zip(events, zip(marks[:-1], marks[1:])):
-            events[key].append(a.elapsed_time(b))
-    stage_ms = {k: float(np.mean(v)) for k, v in events.items()}
+    workload, kernel, route = build_route(args.workload)
+    elements = workload.elements
+    route.tables()
+    print(f"[bench] {workload.name}: warmup {args.warmup}", file=sys.stderr, flush=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        first = None
+        for _ in range(args.warmup):
+            first = route(copy.copy(elements))
+        torch.cuda.synchronize()
+        latencies = []
+        same = True
+        print(f"[bench] timing {args.steps} events", file=sys.stderr, flush=True)
+        with StageClock() as clock, KernelTimer() as timer:
+            torch.cuda.synchronize()
+            began = time.perf_counter()
+            for _ in range(args.steps):
+                start = time.perf_counter()
+                response = route(copy.copy(elements))
+                torch.cuda.synchronize()
+                latencies.append(time.perf_counter() - start)
+                if first is not None:
+                    same &= bool(np.array_equal(response.charge_components_pe,
+                                                first.charge_components_pe))
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - began
+            stage_ms = {k: v / args.steps for k, v in clock.milliseconds().items()}
+            kernels = {}
+            for name in TIMED_KERNELS:
+                total, launches = timer.read(name)
+                if launches:
+                    kernels[name] = {"ms_per_launch": total / launches,
+                                     "launches_per_event": launches / args.steps,
+                                     "ms_per_event": total / args.steps}
+            timer.reset()
     ms_per_step = 1e3 * elapsed / args.steps
-
-    # roofline of the dominant kernel (order 0): algorithmic bytes per launch
-    # = 48 B per element (12 FP32 values) + 36 B per OM (position, look, area,
-    # radius, charge)
-    n_el, n_om = len(workload.elements), len(d)
-    algorithmic_bytes = 48 * n_el + 36 * n_om
-    achieved = algorithmic_bytes / (stage_ms["order0"] * 1e-3) / 1e9
-    traffic, traffic_note = (None, "not measured (--no-traffic)")
-    if not args.no_traffic:
-        traffic, traffic_note = measure_traffic(args.workload)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": "order-0 charge (reset, chunk bounds, cull, order0_pairs, reduce)",
-                "traffic_note": traffic_note,
-                "note": "order 0 is VALU-bound (transcendental-heavy FP32 quadrature); "
-                        "the HBM fraction is reported as required and is small by design"}
+    lat = np.asarray(latencies) * 1e3
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_event"]) if kernels else None
+    roofline = None
+    if dominant is not None:
+        roofline = roofline_of(dominant, kernels[dominant]["ms_per_launch"], workload, response,
+                               route)
+        if roofline is None:
+            roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": None, "algorithmic": "no cost model for this kernel"}
+        roofline["kernel"] = dominant
+        roofline["ms_per_launch"] = kernels[dominant]["ms_per_launch"]
+        traffic, note = (None, "not measured (--no-traffic)")
+        if not args.no_traffic:
+            traffic, note = measure_traffic(args.workload, dominant.split("<")[0])
+        roofline["traffic"] = traffic
+        roofline["traffic_note"] = note
+    print(json.dumps({"gpu_part": {"ms_per_step": ms_per_step, "kernels": kernels,
+                                   "stages": stage_ms, "roofline": roofline}}),
+          file=sys.stderr, flush=True)
     baseline = None
     if not args.no_cpu_baseline:
-        baseline = cpu_baseline(workload, bundle, sources, window, ids, alpha, args.cpu_fraction)
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        baseline = cpu_baseline(workload, kernel, route, response, args.cpu_fraction)
     line = {
         "metric": "events/sec + p50/p99 latency at <=1e-3 charge error vs FP64; served vs plain GPU",
         "value": 1e3 / ms_per_step, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": workload.name, "form": "plain FP32 GPU path, stages (a)(b)(c)",
-                   "oms": n_om, "elements": n_el, "windowed_oms": int(len(ids)),
-                   "bins": int(basis.bins), "stage_ms": stage_ms,
-                   "order0_total_pe": float(charge.sum().item())},
+        "config": {"workload": workload.name,
+                   "form": "plain FP32 GPU route: ShowerTransport(kernel, data)(shower)",
+                   "oms": len(workload.detector), "elements": len(elements),
+                   "wavelength_nodes": len(workload.wavelength.wavelength_nm),
+                   "bins": len(workload.edges_ns) - 1,
+                   "active_oms": int(np.count_nonzero(response.active)),
+                   "order1_oms": int(np.count_nonzero(
+                       np.asarray(response.metadata["prompt_order1_level"]) > 0)),
+                   "charge_pe_by_order": [float(x) for x in
+                                          response.charge_components_pe.sum(axis=0)],
+                   "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                  "p99": float(np.percentile(lat, 99)),
+                                  "max": float(lat.max())},
+                   "repeatable": same,
+                   "stage_gpu_ms": stage_ms, "kernel_gpu_ms": kernels,
+                   "host_stage_ms": {k: 1e3 * v for k, v in
+                                     response.metadata["timings"].items()}},
         "roofline": roofline, "cpu_baseline": baseline,
     }
     print(json.dumps(line), flush=True)

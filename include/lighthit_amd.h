@@ -37,6 +37,19 @@ void lh_clear_error(void);
 int lh_abi_version(void);
 
 /*
+ * Kernel timer (measurement only; no counterpart in the reference, which
+ * times stages with perf_counter on the host). While enabled, the dominant
+ * kernels of every entry point record a HIP event before and after their
+ * launch on the stream they are launched on. lh_timing_read synchronises
+ * those events and returns the summed milliseconds and the launch count of
+ * one kernel name; lh_timing_reset releases every recorded event. Timing
+ * adds two event records per launch; leave it off outside measurements.
+ */
+int lh_timing_enable(int on);
+int lh_timing_read(const char* kernel, double* total_ms, long long* launches);
+int lh_timing_reset(void);
+
+/*
  * Stage (c): order >=2 time bins of OMs from their spectra.
  *
  * Replaces, for the order >=2 column of the shower readout,
@@ -260,10 +273,11 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
  * pencil keys to info[0]; lh_order1_compress_pencils writes that many pencils
  * (screen grids, parent_face > 0: parents and children in preorder, at most
  * twice as many, count in info[1]). The grouped pass is two calls:
- * lh_order1_group_keys writes the number of (tau, r, theta) groups to info[0];
- * lh_order1_group_charge takes a scratch buffer of
- * lh_order1_group_scratch_bytes(oms, optics, groups) and writes charge (and
- * bins) of the listed OMs. Each pair shares its
+ * lh_order1_group_keys writes the number of (tau, r, theta) groups to info[0]
+ * and the bits of its compact per-OM keys to info[1] (it returns after the
+ * stream has drained); lh_order1_group_charge takes both, a scratch buffer of
+ * lh_order1_group_scratch_bytes(pencils, oms, optics, groups) and writes
+ * charge (and bins) of the listed OMs. Each pair shares its
  * workspace. lh_order1_set_tables uploads the 24-term Taylor table of
  * I_k(z) (3 x 24) and the 48-node inverse-angle rule (tangent, weight sec^2).
  */
@@ -348,13 +362,13 @@ size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
 int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                          const lh_order1_optics* optics, const lh_order1_merge* merge, int* info,
                          void* workspace, size_t workspace_bytes, void* stream);
-size_t lh_order1_group_scratch_bytes(const lh_order1_oms* oms, const lh_order1_optics* optics,
-                                     int groups);
+size_t lh_order1_group_scratch_bytes(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
+                                     const lh_order1_optics* optics, int groups);
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
-                           int groups, void* scratch, size_t scratch_bytes, int compute_bins,
-                           double* charge, double* bins, void* workspace, size_t workspace_bytes,
-                           void* stream);
+                           int groups, int key_bits, void* scratch, size_t scratch_bytes,
+                           int compute_bins, double* charge, double* bins, void* workspace,
+                           size_t workspace_bytes, void* stream);
 
 /*
  * Band fold of the order >= 2 pack (one-time table preparation).
@@ -392,6 +406,36 @@ int lh_segment_bounds(const double* receivers, int n_receivers, const double* st
                       const double* directions, const double* lengths, const double* start_ns,
                       int n_segments, double flight, double* distance, double* earliest,
                       void* stream);
+
+/*
+ * Window report of one compiled band (shower_basis.py:408-439
+ * _radial_window_report): for every OM, the basis points closer than low
+ * (near), farther than high (far) and inside [low, high] (used), and the
+ * estimate of the light beyond high,
+ *   tail = scale * area * sum_far weight |ch_p| att(r) / (4 pi r^2),
+ * with att(r) = exp(interp(r, radius, log_factor)) (np.interp, clamped).
+ * points are frame coordinates (first, second, axis); frame holds the
+ * centre, axis, first and second vectors (12 doubles); channels is the
+ * band's complex64 channel array, read at point p's first element
+ * (channel_stride complex values apart). The caller passes
+ * weight = sqrt(4 pi) / efficiency and scale = 10 peak_acceptance efficiency.
+ */
+typedef struct lh_window_input {
+    const double* points;
+    const void* channels;
+    int count, channel_stride;
+    const double* frame;
+    const double* positions;
+    const double* areas;
+    int n_oms;
+    double low, high, weight, scale;
+    const double* radius;
+    const double* log_factor;
+    int table;
+} lh_window_input;
+
+int lh_window_report(const lh_window_input* input, int* used, int* far, int* near, double* tail,
+                     void* stream);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,9 @@ SIGNATURES = {
     "lh_clear_error": (None, []),
     "lh_abi_version": (_i, []),
     "lh_readout_bins": (_i, [_p, _p, _p, _p, _i, _i, _i, _p, _p]),
+    "lh_timing_enable": (_i, [_i]),
+    "lh_timing_read": (_i, [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong)]),
+    "lh_timing_reset": (_i, []),
 }
 
 _lib = None
@@ -83,3 +86,91 @@ def stream_handle(stream=None):
     import torch
     stream = torch.cuda.current_stream() if stream is None else stream
     return ctypes.c_void_p(stream.cuda_stream)
+
+
+class StageClock:
+    """HIP events around the library's stages on the launch stream.
+
+    While a clock is active (``with StageClock() as clock``), every wrapped
+    stage records a start and an end event on the current stream;
+    :meth:`milliseconds` sums the intervals per stage after synchronising.
+    """
+
+    active = None
+
+    def __init__(self):
+        self.marks = []
+
+    def __enter__(self):
+        StageClock.active = self
+        return self
+
+    def __exit__(self, *exc):
+        StageClock.active = None
+        return False
+
+    def milliseconds(self):
+        import torch
+        torch.cuda.synchronize()
+        out = {}
+        for name, start, end in self.marks:
+            out[name] = out.get(name, 0.0) + start.elapsed_time(end)
+        return out
+
+    def counts(self):
+        out = {}
+        for name, _, _ in self.marks:
+            out[name] = out.get(name, 0) + 1
+        return out
+
+
+class stage:
+    """Context manager recording one stage interval on the active clock (if any)."""
+
+    def __init__(self, name):
+        self.name = name
+        self.start = None
+
+    def __enter__(self):
+        if StageClock.active is not None:
+            import torch
+            self.start = torch.cuda.Event(enable_timing=True)
+            self.start.record()
+        return self
+
+    def __exit__(self, *exc):
+        clock = StageClock.active
+        if clock is not None and self.start is not None:
+            import torch
+            end = torch.cuda.Event(enable_timing=True)
+            end.record()
+            clock.marks.append((self.name, self.start, end))
+        return False
+
+
+class KernelTimer:
+    """HIP events that the library records around its dominant kernels.
+
+    ``with KernelTimer() as timer:`` turns the library's kernel timer on (and
+    clears earlier marks); ``timer.read(name)`` returns ``(total_ms, launches)``
+    of every launch of kernel ``name`` inside the block.
+    """
+
+    def __enter__(self):
+        call("lh_timing_reset")
+        call("lh_timing_enable", 1)
+        return self
+
+    def __exit__(self, *exc):
+        call("lh_timing_enable", 0)
+        return False
+
+    @staticmethod
+    def read(name):
+        total, launches = ctypes.c_double(), ctypes.c_longlong()
+        call("lh_timing_read", name.encode(), ctypes.byref(total), ctypes.byref(launches))
+        return total.value, launches.value
+
+    @staticmethod
+    def reset():
+        call("lh_timing_reset")

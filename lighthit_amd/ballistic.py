@@ -157,10 +157,11 @@ class Order0:
         if out is None:
             out = torch.empty(self.oms.count, dtype=torch.float64, device=self.device)
         ws = self.workspace(elements.count)
-        _lib.call("lh_order0_charge", ctypes.byref(self.optics.struct), ctypes.byref(self.oms.struct),
-                  ctypes.byref(elements.struct), ctypes.c_void_p(out.data_ptr()),
-                  ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()),
-                  _lib.stream_handle(stream))
+        with _lib.stage("order0_charge"):
+            _lib.call("lh_order0_charge", ctypes.byref(self.optics.struct),
+                      ctypes.byref(self.oms.struct), ctypes.byref(elements.struct),
+                      ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                      ctypes.c_size_t(ws.numel()), _lib.stream_handle(stream))
         return out, ws
 
     def bins_device(self, elements, oms, origins, edges, out=None, stream=None):
@@ -174,11 +175,12 @@ class Order0:
         if self._bins_workspace is None or self._bins_workspace.numel() < need:
             self._bins_workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
         ws = self._bins_workspace
-        _lib.call("lh_order0_bins", ctypes.byref(self.optics.struct), ctypes.byref(oms.struct),
-                  ctypes.byref(elements.struct), ctypes.c_void_p(origins.data_ptr()),
-                  ctypes.c_void_p(edges.data_ptr()), nbins, ctypes.c_void_p(out.data_ptr()),
-                  ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()),
-                  _lib.stream_handle(stream))
+        with _lib.stage("order0_bins"):
+            _lib.call("lh_order0_bins", ctypes.byref(self.optics.struct),
+                      ctypes.byref(oms.struct), ctypes.byref(elements.struct),
+                      ctypes.c_void_p(origins.data_ptr()), ctypes.c_void_p(edges.data_ptr()),
+                      nbins, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                      ctypes.c_size_t(ws.numel()), _lib.stream_handle(stream))
         return out, ws
 
     def bins(self, elements, ids, origins, edges):

@@ -250,9 +250,10 @@ def compile_band(chords, frame, degree, omega, *, azimuthal_degree=4, options=No
     info = torch.zeros(8, dtype=torch.int32, device=device)
     points = torch.empty((5 * lattice.positions, 3), dtype=torch.float64, device=device)
     handle = _lib.stream_handle(stream)
-    _lib.call("lh_compile_layout", ctypes.byref(descriptor), ctypes.byref(lattice),
-              _address(info), _address(points), _address(ws), ctypes.c_size_t(ws.numel()),
-              handle)
+    with _lib.stage("compile_layout"):
+        _lib.call("lh_compile_layout", ctypes.byref(descriptor), ctypes.byref(lattice),
+                  _address(info), _address(points), _address(ws), ctypes.c_size_t(ws.numel()),
+                  handle)
     counts = info.cpu().numpy()
     if counts[INFO_FLAGS] & FLAG_RANGE:
         raise RuntimeError("a chord node fell outside the slice lattice")
@@ -262,9 +263,10 @@ def compile_band(chords, frame, degree, omega, *, azimuthal_degree=4, options=No
     channels = torch.empty((npoints, nch, len(omega)), dtype=torch.complex64, device=device)
     if omega_device is None:
         omega_device = torch.as_tensor(omega, device=device)
-    _lib.call("lh_compile_channels", ctypes.byref(descriptor), ctypes.byref(lattice),
-              _address(omega_device), len(omega), float(scale), rows, _address(deposit),
-              _address(channels), _address(ws), ctypes.c_size_t(ws.numel()), handle)
+    with _lib.stage("compile_channels"):
+        _lib.call("lh_compile_channels", ctypes.byref(descriptor), ctypes.byref(lattice),
+                  _address(omega_device), len(omega), float(scale), rows, _address(deposit),
+                  _address(channels), _address(ws), ctypes.c_size_t(ws.numel()), handle)
     summary = dict(elements=int(taken.sum()), slices=int(counts[INFO_SLICES]),
                    stencil_slices=int(counts[INFO_WIDE]), time_rows=rows,
                    element_order=lattice.element_order, z_origin_m=lattice.z_origin_m,

@@ -704,10 +704,14 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
     LH_REQUIRE(rows >= 1 && deposit != nullptr && channels != nullptr,
                "rows, deposit and channels must be set");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    compile_deposit<<<rows, kThreads, 0, s>>>(p, rows, deposit);
+    {
+        lh::KernelMark mark("compile_deposit", s);
+        compile_deposit<<<rows, kThreads, 0, s>>>(p, rows, deposit);
+    }
     if ((status = lh::check_launch("compile_deposit")) != LH_OK) return status;
     // slice and wide counts: the workspace info words of lh_compile_layout
     dim3 grid(lattice->positions, (p.channels + kChannelGroup - 1) / kChannelGroup);
+    lh::KernelMark mark("compile_phase", s);
     compile_phase<<<grid, kThreads, 0, s>>>(p, omega, n_freq, scale, deposit,
                                            static_cast<float2*>(channels));
     return lh::check_launch("compile_phase");

@@ -660,7 +660,10 @@ constexpr int kLdsBudget = 64 * 1024;
 template <int A, int M>
 int launch(Ge2Params P, hipStream_t stream) {
     if (P.nfreq == 1) {
-        hipLaunchKernelGGL((ge2_narrow<A, M>), dim3(P.nrec), dim3(kThreads), 0, stream, P, 0);
+        {
+            lh::KernelMark mark("ge2_narrow", stream);
+            hipLaunchKernelGGL((ge2_narrow<A, M>), dim3(P.nrec), dim3(kThreads), 0, stream, P, 0);
+        }
         return lh::check_launch("lh_ge2_apply (narrow)");
     }
     P.record_words = Record<A, M>::words(P.degree, P.bands);
@@ -671,7 +674,11 @@ int launch(Ge2Params P, hipStream_t stream) {
     const size_t lds = (size_t)(reduce_words + P.chunk * P.record_words) * 4;
     P.nsplit = wide_splits(P.nrec, P.npoints);
     P.span = (P.npoints + P.nsplit - 1) / P.nsplit;
-    hipLaunchKernelGGL((ge2_wide<A, M>), dim3(P.nrec, P.nsplit), dim3(kThreads), lds, stream, P);
+    {
+        lh::KernelMark mark("ge2_wide", stream);
+        hipLaunchKernelGGL((ge2_wide<A, M>), dim3(P.nrec, P.nsplit), dim3(kThreads), lds, stream,
+                           P);
+    }
     int status = lh::check_launch("lh_ge2_apply (wide)");
     if (status != LH_OK || P.nsplit == 1) return status;
     const int total = P.nrec * P.nfreq;

@@ -1,7 +1,11 @@
 // Error reporting and version of the lighthit_amd C-ABI.
 #include "lh_common.h"
 
+#include <atomic>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
 
 namespace lh {
 
@@ -14,9 +18,82 @@ void set_error(const char* fmt, ...) {
     va_end(args);
 }
 
+namespace {
+
+struct Mark {
+    std::string name;
+    hipEvent_t start;
+    hipEvent_t end;
+};
+
+std::atomic<bool> g_timing{false};
+std::mutex g_marks_lock;
+std::vector<Mark> g_marks;
+
+}  // namespace
+
+bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+
+hipEvent_t timing_start(hipStream_t stream) {
+    hipEvent_t start = nullptr;
+    if (hipEventCreate(&start) != hipSuccess) return nullptr;
+    if (hipEventRecord(start, stream) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        return nullptr;
+    }
+    return start;
+}
+
+void timing_record(const char* name, hipStream_t stream, hipEvent_t start) {
+    hipEvent_t end = nullptr;
+    if (hipEventCreate(&end) != hipSuccess || hipEventRecord(end, stream) != hipSuccess) {
+        if (end != nullptr) (void)hipEventDestroy(end);
+        (void)hipEventDestroy(start);
+        return;
+    }
+    std::lock_guard<std::mutex> guard(g_marks_lock);
+    g_marks.push_back(Mark{name, start, end});
+}
+
 }  // namespace lh
 
 extern "C" {
+
+int lh_timing_enable(int on) {
+    lh::g_timing.store(on != 0);
+    return LH_OK;
+}
+
+int lh_timing_read(const char* kernel, double* total_ms, long long* launches) {
+    LH_REQUIRE(kernel != nullptr && total_ms != nullptr && launches != nullptr,
+               "lh_timing_read: null argument");
+    std::lock_guard<std::mutex> guard(lh::g_marks_lock);
+    double total = 0.0;
+    long long count = 0;
+    for (const lh::Mark& m : lh::g_marks) {
+        if (m.name != kernel) continue;
+        LH_HIP_CALL(hipEventSynchronize(m.end));
+        float ms = 0.0f;
+        LH_HIP_CALL(hipEventElapsedTime(&ms, m.start, m.end));
+        total += ms;
+        ++count;
+    }
+    *total_ms = total;
+    *launches = count;
+    return LH_OK;
+}
+
+int lh_timing_reset(void) {
+    std::lock_guard<std::mutex> guard(lh::g_marks_lock);
+    for (lh::Mark& m : lh::g_marks) {
+        (void)hipEventSynchronize(m.end);
+        (void)hipEventDestroy(m.start);
+        (void)hipEventDestroy(m.end);
+    }
+    lh::g_marks.clear();
+    return LH_OK;
+}
+
 
 const char* lh_last_error(void) { return lh::g_error; }
 

@@ -26,6 +26,29 @@ inline int check_launch(const char* what) {
     return LH_OK;
 }
 
+// Kernel timer: while enabled (lh_timing_enable), a KernelMark records a HIP
+// event on its stream when built and another when destroyed, under one
+// kernel name. Scope it around exactly one launch.
+bool timing_enabled();
+void timing_record(const char* name, hipStream_t stream, hipEvent_t start);
+hipEvent_t timing_start(hipStream_t stream);
+
+class KernelMark {
+  public:
+    KernelMark(const char* name, hipStream_t stream)
+        : name_(name), stream_(stream), start_(timing_enabled() ? timing_start(stream) : nullptr) {}
+    ~KernelMark() {
+        if (start_ != nullptr) timing_record(name_, stream_, start_);
+    }
+    KernelMark(const KernelMark&) = delete;
+    KernelMark& operator=(const KernelMark&) = delete;
+
+  private:
+    const char* name_;
+    hipStream_t stream_;
+    hipEvent_t start_;
+};
+
 }  // namespace lh
 
 #define LH_REQUIRE(cond, ...)                                   \

@@ -982,7 +982,10 @@ int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const cha
     status = lh::check_launch(name);
     if (status != LH_OK) return status;
     const unsigned grid = (unsigned)std::min<size_t>(items, kPersistentGroups);
-    hipLaunchKernelGGL((order0_pairs<kBins>), dim3(grid), dim3(kThreads), lds, s, P);
+    {
+        lh::KernelMark mark(kBins ? "order0_pairs<true>" : "order0_pairs<false>", s);
+        hipLaunchKernelGGL((order0_pairs<kBins>), dim3(grid), dim3(kThreads), lds, s, P);
+    }
     status = lh::check_launch(name);
     if (status != LH_OK) return status;
     const size_t total = (size_t)P.nom * width;

@@ -37,7 +37,6 @@ constexpr int kMaxMono = 15;        // acceptance degree <= 4
 constexpr int kMaxBands = 16;
 constexpr int kGauss = 48;
 constexpr int kMaxPathNodes = 2048;
-constexpr long long kSentinel = (1LL << 62) - 1;
 
 __constant__ double kTaylor[3][24];
 __constant__ double kTangent[kGauss];
@@ -187,66 +186,127 @@ __global__ void __launch_bounds__(kThreads) o1_run_starts(Compress k) {
 }
 
 // Moment row of one (cell, pixel) key: its nodes in enumeration order (:1192-1223).
-__global__ void __launch_bounds__(kThreads) o1_rows(Compress k, int runs) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= runs) return;
+// Moment contributions of one ring node (the terms summed per pencil by
+// compress_ring_pencils, _prompt_numba.py:2246-2279).
+__device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
     const lh_order1_chords& c = k.ch;
     int R = k.grid.ring_nodes;
     double a_step = 0.5 * k.grid.cell_m;
-    double row[kNmom];
-    for (int m = 0; m < kNmom; ++m) row[m] = 0.0;
-    for (int j = k.run_start[r]; j < k.run_start[r + 1]; ++j) {
-        int node = k.sorted_ids[j];
-        long long s = node / R;
-        int i = k.sub_element[s], ia = k.sub_index[s];
-        int e = c.index ? c.index[i] : i;
-        double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
-        double e1[3], e2[3];
-        ring_basis(u, e1, e2);
-        double cc = c.cone_cosine[e];
-        double cs = sqrt(fmax(1.0 - cc * cc, 0.0));
-        int n_a = n_sub(c.length_m[e], a_step);
-        double share = 1.0 / ((double)n_a * R);
-        double arc = 2.0 * M_PI * cs / R;
-        double arc2 = arc * arc / 12.0;
-        double sl = c.length_m[e] / n_a;
-        double sub2 = sl * sl / 12.0;
-        double w0 = c.coefficient0[e] * share, w2 = c.coefficient2[e] * share;
-        double p[3], t;
-        sub_point(c, e, ia, n_a, p, &t);
-        double cp = k.cp[node], sp = k.sp[node];
-        double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
-        double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
-        double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
-        double px = p[0], py = p[1], pz = p[2];
-        row[0] += w0;
-        row[1] += w2;
-        row[2] += w0 * px;
-        row[3] += w0 * py;
-        row[4] += w0 * pz;
-        row[5] += w0 * t;
-        row[6] += w0 * sx;
-        row[7] += w0 * sy;
-        row[8] += w0 * sz;
-        row[9] += w0 * (px * px + py * py + pz * pz + sub2);
-        row[10] += w0 * t * t;
-        double tx = -sp * e1[0] + cp * e2[0];
-        double ty = -sp * e1[1] + cp * e2[1];
-        double tz = -sp * e1[2] + cp * e2[2];
-        row[11] += w0 * (sx * sx + arc2 * tx * tx);
-        row[12] += w0 * (sy * sy + arc2 * ty * ty);
-        row[13] += w0 * (sz * sz + arc2 * tz * tz);
-        row[14] += w0 * (sx * sy + arc2 * tx * ty);
-        row[15] += w0 * (sx * sz + arc2 * tx * tz);
-        row[16] += w0 * (sy * sz + arc2 * ty * tz);
-        row[17] += w0 * (px * px + sub2 * u[0] * u[0]);
-        row[18] += w0 * (py * py + sub2 * u[1] * u[1]);
-        row[19] += w0 * (pz * pz + sub2 * u[2] * u[2]);
-        row[20] += w0 * (px * py + sub2 * u[0] * u[1]);
-        row[21] += w0 * (px * pz + sub2 * u[0] * u[2]);
-        row[22] += w0 * (py * pz + sub2 * u[1] * u[2]);
+    long long s = node / R;
+    int i = k.sub_element[s], ia = k.sub_index[s];
+    int e = c.index ? c.index[i] : i;
+    double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
+    double e1[3], e2[3];
+    ring_basis(u, e1, e2);
+    double cc = c.cone_cosine[e];
+    double cs = sqrt(fmax(1.0 - cc * cc, 0.0));
+    int n_a = n_sub(c.length_m[e], a_step);
+    double share = 1.0 / ((double)n_a * R);
+    double arc = 2.0 * M_PI * cs / R;
+    double arc2 = arc * arc / 12.0;
+    double sl = c.length_m[e] / n_a;
+    double sub2 = sl * sl / 12.0;
+    double w0 = c.coefficient0[e] * share, w2 = c.coefficient2[e] * share;
+    double p[3], t;
+    sub_point(c, e, ia, n_a, p, &t);
+    double cp = k.cp[node], sp = k.sp[node];
+    double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
+    double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
+    double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
+    double px = p[0], py = p[1], pz = p[2];
+    v[0] = w0;
+    v[1] = w2;
+    v[2] = w0 * px;
+    v[3] = w0 * py;
+    v[4] = w0 * pz;
+    v[5] = w0 * t;
+    v[6] = w0 * sx;
+    v[7] = w0 * sy;
+    v[8] = w0 * sz;
+    v[9] = w0 * (px * px + py * py + pz * pz + sub2);
+    v[10] = w0 * t * t;
+    double tx = -sp * e1[0] + cp * e2[0];
+    double ty = -sp * e1[1] + cp * e2[1];
+    double tz = -sp * e1[2] + cp * e2[2];
+    v[11] = w0 * (sx * sx + arc2 * tx * tx);
+    v[12] = w0 * (sy * sy + arc2 * ty * ty);
+    v[13] = w0 * (sz * sz + arc2 * tz * tz);
+    v[14] = w0 * (sx * sy + arc2 * tx * ty);
+    v[15] = w0 * (sx * sz + arc2 * tx * tz);
+    v[16] = w0 * (sy * sz + arc2 * ty * tz);
+    v[17] = w0 * (px * px + sub2 * u[0] * u[0]);
+    v[18] = w0 * (py * py + sub2 * u[1] * u[1]);
+    v[19] = w0 * (pz * pz + sub2 * u[2] * u[2]);
+    v[20] = w0 * (px * py + sub2 * u[0] * u[1]);
+    v[21] = w0 * (px * pz + sub2 * u[0] * u[2]);
+    v[22] = w0 * (py * pz + sub2 * u[1] * u[2]);
+}
+
+__device__ inline double wave_total(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+constexpr int kRowPiece = 64;   // ring nodes per wave of the moment rows
+
+// Pieces of every run: ceil(nodes / kRowPiece); runs of more than one piece
+// also count their pieces in `big` (their partial rows are kept).
+__global__ void __launch_bounds__(kThreads) o1_row_counts(Compress k, int runs, int* counts,
+                                                          int* big) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > runs) return;
+    int c = r == runs ? 0 : (k.run_start[r + 1] - k.run_start[r] + kRowPiece - 1) / kRowPiece;
+    counts[r] = c;
+    big[r] = c > 1 ? c : 0;
+}
+
+// Moment rows: one wave per piece of at most kRowPiece nodes of one run, a
+// fixed butterfly sum; a run of one piece writes its row, longer runs write
+// partial rows that o1_row_combine adds in piece order (deterministic).
+__global__ void __launch_bounds__(kThreads) o1_row_pieces(Compress k, int runs,
+                                                          const int* piece_start,
+                                                          const int* big_start,
+                                                          double* piece_rows) {
+    int q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (q >= piece_start[runs]) return;
+    int low = 0, high = runs;
+    while (high - low > 1) {
+        int middle = (low + high) >> 1;
+        if (piece_start[middle] <= q) low = middle; else high = middle;
     }
-    for (int m = 0; m < kNmom; ++m) k.rows[(size_t)r * kNmom + m] = row[m];
+    int r = low, piece = q - piece_start[r];
+    int j0 = k.run_start[r] + piece * kRowPiece;
+    int j1 = min(k.run_start[r + 1], j0 + kRowPiece);
+    double v[kNmom];
+    if (j0 + lane < j1) {
+        node_moments(k, k.sorted_ids[j0 + lane], v);
+    } else {
+        for (int m = 0; m < kNmom; ++m) v[m] = 0.0;
+    }
+    bool single = piece_start[r + 1] - piece_start[r] == 1;
+    double* out = single ? k.rows + (size_t)r * kNmom
+                         : piece_rows + ((size_t)big_start[r] + piece) * kNmom;
+    for (int m = 0; m < kNmom; ++m) {
+        double total = wave_total(v[m]);
+        if (lane == 0) out[m] = total;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) o1_row_combine(Compress k, int runs,
+                                                           const int* piece_start,
+                                                           const int* big_start,
+                                                           const double* piece_rows) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    int n = piece_start[r + 1] - piece_start[r];
+    if (n < 2) return;
+    const double* in = piece_rows + (size_t)big_start[r] * kNmom;
+    for (int m = 0; m < kNmom; ++m) {
+        double total = in[m];
+        for (int i = 1; i < n; ++i) total += in[(size_t)i * kNmom + m];
+        k.rows[(size_t)r * kNmom + m] = total;
+    }
 }
 
 // Pencil from a moment row (finish_pencils, _prompt_numba.py:2282-2353).
@@ -370,17 +430,22 @@ struct Group {
     int compute_bins;
     int screen;
     int n_pen;
+    int slots;                // entries per (pencil, OM): 2 on the screen, else 1
+    int key_bits;             // bits of the compact per-OM key; the OM sits above them
+    long long key_mask;       // (1 << key_bits) - 1, also the compact sentinel
     long long theta_bins;
     double log_ratio;
     double mut_min, speed_max;
-    long long* keys;          // (work, 2 * n_pen)
+    long long* keys;          // (work, slots * n_pen) (OM << key_bits) | compact key
     int* vals;
     long long* sorted_keys;
     int* sorted_vals;
-    int* seg_offsets;         // (work + 1)
-    int* group_flag;          // (work * 2 n_pen + 1)
+    int* group_flag;          // (work * slots * n_pen + 1)
     int* group_scan;
     int* group_start;         // (groups + 1)
+    int* group_end;           // (groups + 1) one past the last member of every group
+    int* piece_start;         // (groups + 1) first piece of every group   caller scratch
+    double* piece_values;     // (pieces, nv) partial group sums          caller scratch
     int* om_groups;           // (work + 1) first group of every OM
     double* values;           // (groups, nv)            caller scratch
     int* batch_flag;          // (groups + 1)            caller scratch
@@ -699,18 +764,53 @@ __global__ void __launch_bounds__(kThreads) o1_range_init(Group G, int work) {
     G.k_range[4 * w + 3] = INT_MIN;
 }
 
+__device__ inline int wave_min_int(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ inline int wave_max_int(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Radius and delay key ranges of every OM: min/max within each wave, then one
+// atomic per wave and bound.
 __global__ void __launch_bounds__(kThreads) o1_group_ranges(Group G) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int w = blockIdx.y;
-    if (p >= G.n_pen) return;
-    long long k0, k1;
-    radius_keys(G, p, G.om.work[w], &k0, &k1);
-    // one key of margin on each side: the key pass evaluates the same expressions
-    // in another function, where contraction may differ in the last bit
-    atomicMin(G.k_range + 4 * w, (int)k1 - 1);
-    atomicMax(G.k_range + 4 * w + 1, (int)k1 + 1);
-    atomicMin(G.k_range + 4 * w + 2, (int)k0 - 1);
-    atomicMax(G.k_range + 4 * w + 3, (int)k0 + 1);
+    int lo1 = INT_MAX, hi1 = INT_MIN, lo0 = INT_MAX, hi0 = INT_MIN;
+    if (p < G.n_pen) {
+        long long k0, k1;
+        radius_keys(G, p, G.om.work[w], &k0, &k1);
+        // one key of margin on each side: the key pass evaluates the same
+        // expressions in another function, where contraction may differ in the
+        // last bit
+        lo1 = (int)k1 - 1;
+        hi1 = (int)k1 + 1;
+        lo0 = (int)k0 - 1;
+        hi0 = (int)k0 + 1;
+    }
+    lo1 = wave_min_int(lo1);
+    hi1 = wave_max_int(hi1);
+    lo0 = wave_min_int(lo0);
+    hi0 = wave_max_int(hi0);
+    if ((threadIdx.x & 63) == 0 && lo1 != INT_MAX) {
+        atomicMin(G.k_range + 4 * w, lo1);
+        atomicMax(G.k_range + 4 * w + 1, hi1);
+        atomicMin(G.k_range + 4 * w + 2, lo0);
+        atomicMax(G.k_range + 4 * w + 3, hi0);
+    }
+}
+
+// Size of the largest compact key space over the OMs.
+__global__ void __launch_bounds__(kThreads) o1_key_space(Group G, int work) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= work) return;
+    const int* range = G.k_range + 4 * w;
+    unsigned long long n_k1 = (unsigned long long)((long long)range[1] - range[0] + 1);
+    unsigned long long n_k0 = (unsigned long long)((long long)range[3] - range[2] + 1);
+    atomicMax(G.max_key, n_k1 * n_k0 * 2ULL * (unsigned long long)G.theta_bins);
 }
 
 __global__ void __launch_bounds__(kThreads) o1_group_keys(Group G) {
@@ -720,31 +820,32 @@ __global__ void __launch_bounds__(kThreads) o1_group_keys(Group G) {
     int d = G.om.work[w];
     Entry e[2];
     int n = pencil_entries(G, p, d, w, e);
-    size_t at = ((size_t)w * G.n_pen + p) * 2;
-    unsigned long long top = 0;
-    for (int node = 0; node < 2; ++node) {
-        long long key = node < n ? e[node].key : kSentinel;
-        G.keys[at + node] = key;
+    size_t at = ((size_t)w * G.n_pen + p) * G.slots;
+    long long high = (long long)w << G.key_bits;
+    for (int node = 0; node < G.slots; ++node) {
+        G.keys[at + node] = high | (node < n ? e[node].key : G.key_mask);
         G.vals[at + node] = 2 * p + node;
-        if (node < n) top = max(top, (unsigned long long)key);
     }
-    if (n) atomicMax(G.max_key, top);
+}
+
+__device__ inline bool is_sentinel(const Group& G, long long key) {
+    return (key & G.key_mask) == G.key_mask;
 }
 
 __global__ void __launch_bounds__(kThreads) o1_group_flags(Group G, long long total) {
     long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j > total) return;
     if (j == total) { G.group_flag[j] = 0; return; }
-    long long seg = 2LL * G.n_pen;
+    long long seg = (long long)G.slots * G.n_pen;
     long long k = G.sorted_keys[j];
-    bool start = k != kSentinel && (j % seg == 0 || G.sorted_keys[j - 1] != k);
+    bool start = !is_sentinel(G, k) && (j % seg == 0 || G.sorted_keys[j - 1] != k);
     G.group_flag[j] = start ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kThreads) o1_group_starts(Group G, long long total, int work) {
     long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j > total) return;
-    long long seg = 2LL * G.n_pen;
+    long long seg = (long long)G.slots * G.n_pen;
     if (j == total) {
         int groups = G.group_scan[j];
         G.info[0] = groups;
@@ -753,17 +854,9 @@ __global__ void __launch_bounds__(kThreads) o1_group_starts(Group G, long long t
     }
     if (j % seg == 0) G.om_groups[j / seg] = G.group_scan[j];
     if (G.group_flag[j]) G.group_start[G.group_scan[j]] = (int)j;
-}
-
-// End of group g's member run: the next key change, sentinel or OM boundary.
-__device__ inline long long group_end(const Group& G, int g) {
-    long long j = G.group_start[g];
-    long long seg = 2LL * G.n_pen;
-    long long stop = (j / seg + 1) * seg;
     long long k = G.sorted_keys[j];
-    long long e = j + 1;
-    while (e < stop && G.sorted_keys[e] == k) ++e;
-    return e;
+    if (!is_sentinel(G, k) && (j + 1 == total || (j + 1) % seg == 0 || G.sorted_keys[j + 1] != k))
+        G.group_end[G.group_scan[j] + G.group_flag[j] - 1] = (int)(j + 1);
 }
 
 __device__ inline void fill_moments(const Group& G, double weight0, double weight2, double c1,
@@ -792,19 +885,37 @@ __device__ inline double wave_min(double v) {
     return v;
 }
 
-// Group sums: one wave per group, lanes over its members, a fixed butterfly
-// reduction (deterministic).
+constexpr int kPieceEntries = 256;   // group members per wave of the group sums
+
+// Pieces of every group: ceil(members / kPieceEntries).
+__global__ void __launch_bounds__(kThreads) o1_piece_counts(Group G, int groups, int* counts) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > groups) return;
+    counts[g] = g == groups ? 0
+                            : (G.group_end[g] - G.group_start[g] + kPieceEntries - 1)
+                                  / kPieceEntries;
+}
+
+// Group sums: one wave per piece of at most kPieceEntries consecutive members
+// of one group, lanes over its members, a fixed butterfly reduction; the
+// pieces of a group are then added in piece order (deterministic).
 __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
-    int g = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    int q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
-    if (g >= groups) return;
+    if (q >= G.piece_start[groups]) return;
+    int low = 0, high = groups;     // piece_start[low] <= q < piece_start[high]
+    while (high - low > 1) {
+        int middle = (low + high) >> 1;
+        if (G.piece_start[middle] <= q) low = middle; else high = middle;
+    }
+    int g = low;
     int n_mono = G.opt.n_mono;
     int nv = 2 * n_mono + 6;
-    long long seg = 2LL * G.n_pen;
-    long long j0 = G.group_start[g];
+    long long seg = (long long)G.slots * G.n_pen;
+    long long j0 = G.group_start[g] + (long long)(q - G.piece_start[g]) * kPieceEntries;
+    long long j1 = min((long long)G.group_end[g], j0 + kPieceEntries);
     int w = (int)(j0 / seg);
     int d = G.om.work[w];
-    long long j1 = group_end(G, g);
     double row[2 * kMaxMono + 6];
     for (int m = 0; m < nv; ++m) row[m] = 0.0;
     double mom0[kMaxMono], mom2[kMaxMono];
@@ -829,7 +940,19 @@ __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
     }
     for (int m = 0; m < nv; ++m) {
         double total = wave_sum(row[m]);
-        if (lane == 0) G.values[(size_t)g * nv + m] = total;
+        if (lane == 0) G.piece_values[(size_t)q * nv + m] = total;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) o1_group_combine(Group G, int groups) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    int nv = 2 * G.opt.n_mono + 6;
+    int first = G.piece_start[g], last = G.piece_start[g + 1];
+    for (int m = 0; m < nv; ++m) {
+        double total = G.piece_values[(size_t)first * nv + m];
+        for (int q = first + 1; q < last; ++q) total += G.piece_values[(size_t)q * nv + m];
+        G.values[(size_t)g * nv + m] = total;
     }
 }
 
@@ -838,11 +961,12 @@ __global__ void __launch_bounds__(kThreads) o1_batch_flags(Group G, int groups, 
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g > groups) return;
     if (g == groups) { G.batch_flag[g] = 0; return; }
-    long long seg = 2LL * G.n_pen;
+    long long seg = (long long)G.slots * G.n_pen;
     int w = (int)(G.group_start[g] / seg);
-    long long key = G.sorted_keys[G.group_start[g]] / G.theta_bins;
+    long long key = (G.sorted_keys[G.group_start[g]] & G.key_mask) / G.theta_bins;
     bool first = g == G.om_groups[w];
-    G.batch_flag[g] = (first || G.sorted_keys[G.group_start[g - 1]] / G.theta_bins != key) ? 1 : 0;
+    G.batch_flag[g] = (first || (G.sorted_keys[G.group_start[g - 1]] & G.key_mask)
+                                        / G.theta_bins != key) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kThreads) o1_batch_starts(Group G, int groups, int work) {
@@ -992,7 +1116,8 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
     double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
     for (int b = G.om_batches[w] + seg; b < G.om_batches[w + 1]; b += G.segments) {
         int start = G.batch_start[b], stop = G.batch_start[b + 1];
-        int backward = (G.sorted_keys[G.group_start[start]] / G.theta_bins) % 2 == 1;
+        int backward = ((G.sorted_keys[G.group_start[start]] & G.key_mask) / G.theta_bins) % 2
+                       == 1;
         double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0, theta_small = M_PI;
         for (int q = start + lane; q < stop; q += 64) {
             const double* row = G.values + (size_t)q * nv;
@@ -1294,7 +1419,8 @@ int bits_for(unsigned long long value) {
 struct CompressLayout {
     size_t keys, ids, sorted_keys, sorted_ids, cp, sp, sub_element, sub_index, run_flag, run_scan;
     size_t run_start, run_key, rows, info, sort_temp, scan_temp, pkeys, pids, spkeys, spids,
-        pflag, pscan, psort_temp, total;
+        pflag, pscan, psort_temp, piece_count, piece_start, big_count, big_start, piece_rows,
+        total;
 };
 
 int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
@@ -1344,6 +1470,12 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
     L.spids = take(runs_cap * sizeof(int));
     L.pflag = take((runs_cap + 1) * sizeof(int));
     L.pscan = take((runs_cap + 1) * sizeof(int));
+    L.piece_count = take((runs_cap + 1) * sizeof(int));
+    L.piece_start = take((runs_cap + 1) * sizeof(int));
+    L.big_count = take((runs_cap + 1) * sizeof(int));
+    L.big_start = take((runs_cap + 1) * sizeof(int));
+    // pieces of runs longer than one piece: at most 2 nodes / kRowPiece
+    L.piece_rows = take((size_t)(2 * nodes / kRowPiece + 1) * kNmom * sizeof(double));
     L.sort_temp = take(sort_temp);
     L.scan_temp = take(scan_temp);
     L.psort_temp = take(psort_temp);
@@ -1400,18 +1532,20 @@ unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads);
 // ---- grouped layout
 
 struct GroupLayout {
-    size_t keys, vals, sorted_keys, sorted_vals, seg_offsets, group_flag, group_scan, group_start,
+    size_t keys, vals, sorted_keys, sorted_vals, group_flag, group_scan, group_start, group_end,
         om_groups, prepared, lambda_bins, k_range, max_key, info, sort_temp, scan_temp, total;
 };
 
-int grouped_layout(int work, int n_pen, int bands, int bins, GroupLayout& L) {
-    long long entries = 2LL * work * n_pen;
+// Entries per (pencil, OM): the screen's radial rule may give two nodes.
+inline int slots_of(const lh_order1_pencils* pen) { return pen->parent != nullptr ? 2 : 1; }
+
+int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLayout& L) {
+    long long entries = (long long)slots * work * n_pen;
     size_t sort_temp = 0, scan_temp = 0;
-    if (hipcub::DeviceSegmentedRadixSort::SortPairs(
-            nullptr, sort_temp, (long long*)nullptr, (long long*)nullptr, (int*)nullptr,
-            (int*)nullptr, (int)entries, work, (int*)nullptr, (int*)nullptr, 0, 63,
-            (hipStream_t)0) != hipSuccess) {
-        lh::set_error("segmented sort workspace query failed");
+    if (hipcub::DeviceRadixSort::SortPairs(
+            nullptr, sort_temp, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+            (int*)nullptr, (int*)nullptr, (int)entries, 0, 64, (hipStream_t)0) != hipSuccess) {
+        lh::set_error("sort workspace query failed");
         return LH_ERR_HIP;
     }
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
@@ -1428,10 +1562,10 @@ int grouped_layout(int work, int n_pen, int bands, int bins, GroupLayout& L) {
     L.vals = take(entries * sizeof(int));
     L.sorted_keys = take(entries * sizeof(long long));
     L.sorted_vals = take(entries * sizeof(int));
-    L.seg_offsets = take((work + 1) * sizeof(int));
     L.group_flag = take((entries + 1) * sizeof(int));
     L.group_scan = take((entries + 1) * sizeof(int));
     L.group_start = take((entries + 1) * sizeof(int));
+    L.group_end = take((entries + 1) * sizeof(int));
     L.om_groups = take((work + 1) * sizeof(int));
     L.prepared = take((size_t)n_pen * 7 * sizeof(double));
     L.lambda_bins = take((size_t)work * bands * bins * sizeof(double));
@@ -1450,11 +1584,13 @@ int segments_for(int work, int bands, int bins) {
 
 struct ScratchLayout {
     size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
-        part_charge, scan_temp, total;
+        part_charge, piece_count, piece_start, piece_values, scan_temp, total;
     int segments;
+    long long pieces;         // bound on the group-sum pieces
 };
 
-int scratch_layout(int work, int bands, int bins, int n_mono, int groups, ScratchLayout& L) {
+int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mono, int groups,
+                   ScratchLayout& L) {
     size_t scan_temp = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
                                          groups + 1, (hipStream_t)0) != hipSuccess) {
@@ -1473,14 +1609,14 @@ int scratch_layout(int work, int bands, int bins, int n_mono, int groups, Scratc
     L.part_acc = take(parts * bands * bins * 4 * sizeof(double));
     L.part_pre = take(parts * bands * 3 * sizeof(double));
     L.part_charge = take(parts * bands * sizeof(double));
+    long long entries = (long long)slots * work * n_pen;
+    L.pieces = groups + (entries + kPieceEntries - 1) / kPieceEntries;
+    L.piece_count = take((groups + 1) * sizeof(int));
+    L.piece_start = take((groups + 1) * sizeof(int));
+    L.piece_values = take((size_t)std::max(L.pieces, 1LL) * (2 * n_mono + 6) * sizeof(double));
     L.scan_temp = take(scan_temp);
     L.total = at;
     return LH_OK;
-}
-
-__global__ void o1_offsets(int* offsets, int work, int seg) {
-    int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w <= work) offsets[w] = w * seg;
 }
 
 int check_grouped(const lh_order1_pencils* pen, const lh_order1_oms* om,
@@ -1511,6 +1647,7 @@ Group make_group(const lh_order1_pencils* pen, const lh_order1_oms* om,
     G.opt = *opt;
     G.merge = *merge;
     G.screen = pen->parent != nullptr;
+    G.slots = slots_of(pen);
     G.n_pen = pen->count;
     G.log_ratio = log(1.0 + merge->theta_ratio);
     G.theta_bins = 2 + (long long)(log(M_PI / merge->theta_lo) / G.log_ratio);
@@ -1518,10 +1655,10 @@ Group make_group(const lh_order1_pencils* pen, const lh_order1_oms* om,
     G.vals = reinterpret_cast<int*>(base + L.vals);
     G.sorted_keys = reinterpret_cast<long long*>(base + L.sorted_keys);
     G.sorted_vals = reinterpret_cast<int*>(base + L.sorted_vals);
-    G.seg_offsets = reinterpret_cast<int*>(base + L.seg_offsets);
     G.group_flag = reinterpret_cast<int*>(base + L.group_flag);
     G.group_scan = reinterpret_cast<int*>(base + L.group_scan);
     G.group_start = reinterpret_cast<int*>(base + L.group_start);
+    G.group_end = reinterpret_cast<int*>(base + L.group_end);
     G.om_groups = reinterpret_cast<int*>(base + L.om_groups);
     G.prepared = reinterpret_cast<double*>(base + L.prepared);
     G.lambda_bins = reinterpret_cast<double*>(base + L.lambda_bins);
@@ -1604,8 +1741,31 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* base = static_cast<char*>(workspace);
     Compress k = make_compress(chords, grid, base, L);
-    o1_rows<<<blocks(runs), kThreads, 0, s>>>(k, runs);
-    if ((status = lh::check_launch("o1_rows")) != LH_OK) return status;
+    {
+        int* counts = reinterpret_cast<int*>(base + L.piece_count);
+        int* piece_start = reinterpret_cast<int*>(base + L.piece_start);
+        int* big = reinterpret_cast<int*>(base + L.big_count);
+        int* big_start = reinterpret_cast<int*>(base + L.big_start);
+        double* piece_rows = reinterpret_cast<double*>(base + L.piece_rows);
+        o1_row_counts<<<blocks(runs + 1), kThreads, 0, s>>>(k, runs, counts, big);
+        if ((status = lh::check_launch("o1_row_counts")) != LH_OK) return status;
+        size_t scan_bytes = L.psort_temp - L.scan_temp;
+        LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(base + L.scan_temp, scan_bytes, counts,
+                                                     piece_start, runs + 1, s));
+        LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(base + L.scan_temp, scan_bytes, big,
+                                                     big_start, runs + 1, s));
+        // pieces <= runs + nodes / kRowPiece
+        long long bound = runs + nodes / kRowPiece + 1;
+        {
+            lh::KernelMark mark("o1_row_pieces", s);
+            o1_row_pieces<<<(unsigned)((bound + 3) / 4), kThreads, 0, s>>>(k, runs, piece_start,
+                                                                          big_start, piece_rows);
+        }
+        if ((status = lh::check_launch("o1_row_pieces")) != LH_OK) return status;
+        o1_row_combine<<<blocks(runs), kThreads, 0, s>>>(k, runs, piece_start, big_start,
+                                                         piece_rows);
+        if ((status = lh::check_launch("o1_row_combine")) != LH_OK) return status;
+    }
     if (grid->parent_face == 0) {
         LH_REQUIRE(out->count >= runs, "pencil capacity %d below %d", out->count, runs);
         o1_finish<<<blocks(runs), kThreads, 0, s>>>(k.rows, runs, *out);
@@ -1646,17 +1806,18 @@ size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
                                          const lh_order1_optics* optics) {
     if (!pencils || !oms || !optics) return 0;
     GroupLayout L;
-    if (grouped_layout(oms->work_count, pencils->count, optics->bands, optics->bins, L) != LH_OK)
+    if (grouped_layout(oms->work_count, pencils->count, slots_of(pencils), optics->bands,
+                       optics->bins, L) != LH_OK)
         return 0;
     return L.total;
 }
 
-size_t lh_order1_group_scratch_bytes(const lh_order1_oms* oms, const lh_order1_optics* optics,
-                                     int groups) {
-    if (!oms || !optics || groups < 0) return 0;
+size_t lh_order1_group_scratch_bytes(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
+                                     const lh_order1_optics* optics, int groups) {
+    if (!pencils || !oms || !optics || groups < 0) return 0;
     ScratchLayout L;
-    if (scratch_layout(oms->work_count, optics->bands, optics->bins, optics->n_mono, groups, L)
-        != LH_OK) return 0;
+    if (scratch_layout(oms->work_count, pencils->count, slots_of(pencils), optics->bands,
+                       optics->bins, optics->n_mono, groups, L) != LH_OK) return 0;
     return L.total;
 }
 
@@ -1666,14 +1827,14 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     int status = check_grouped(pencils, oms, optics, merge);
     if (status != LH_OK) return status;
     GroupLayout L;
-    if ((status = grouped_layout(oms->work_count, pencils->count, optics->bands, optics->bins,
-                                 L)) != LH_OK) return status;
+    if ((status = grouped_layout(oms->work_count, pencils->count, slots_of(pencils),
+                                 optics->bands, optics->bins, L)) != LH_OK) return status;
     LH_REQUIRE(workspace && workspace_bytes >= L.total && info, "workspace too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* base = static_cast<char*>(workspace);
     Group G = make_group(pencils, oms, optics, merge, base, L);
     int work = oms->work_count;
-    long long total = 2LL * work * pencils->count;
+    long long total = (long long)G.slots * work * pencils->count;
     o1_prepare_shape<<<blocks(pencils->count), kThreads, 0, s>>>(G);
     if ((status = lh::check_launch("o1_prepare_shape")) != LH_OK) return status;
     o1_range_init<<<blocks(work), kThreads, 0, s>>>(G, work);
@@ -1681,20 +1842,31 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     dim3 grid(blocks(pencils->count), work);
     o1_group_ranges<<<grid, kThreads, 0, s>>>(G);
     if ((status = lh::check_launch("o1_group_ranges")) != LH_OK) return status;
-    o1_group_keys<<<grid, kThreads, 0, s>>>(G);
-    if ((status = lh::check_launch("o1_group_keys")) != LH_OK) return status;
-    o1_offsets<<<blocks(work + 1), kThreads, 0, s>>>(G.seg_offsets, work, 2 * pencils->count);
-    if ((status = lh::check_launch("o1_offsets")) != LH_OK) return status;
-    // the sort covers the bits of the largest compact key; the sentinel's low bits
-    // are all ones, so it stays after every key
-    unsigned long long top = 0;
-    LH_HIP_CALL(hipMemcpyAsync(&top, G.max_key, sizeof(top), hipMemcpyDeviceToHost, s));
+    o1_key_space<<<blocks(work), kThreads, 0, s>>>(G, work);
+    if ((status = lh::check_launch("o1_key_space")) != LH_OK) return status;
+    // keys hold (OM << key_bits) | compact key; the compact sentinel (all ones)
+    // lies above every compact key of its OM, so one sort over all OMs orders
+    // every OM's entries by key with the skipped ones last
+    unsigned long long space = 0;
+    LH_HIP_CALL(hipMemcpyAsync(&space, G.max_key, sizeof(space), hipMemcpyDeviceToHost, s));
     LH_HIP_CALL(hipStreamSynchronize(s));
-    int end_bit = bits_for(top + 1);
+    G.key_bits = bits_for(space);
+    G.key_mask = (1LL << G.key_bits) - 1;
+    int end_bit = G.key_bits + bits_for((unsigned long long)std::max(work - 1, 0));
+    LH_REQUIRE(end_bit <= 63, "grouped keys need %d bits; split the OM list", end_bit);
+    {
+        lh::KernelMark mark("o1_group_keys", s);
+        o1_group_keys<<<grid, kThreads, 0, s>>>(G);
+    }
+    if ((status = lh::check_launch("o1_group_keys")) != LH_OK) return status;
     size_t sort_bytes = L.scan_temp - L.sort_temp;
-    LH_HIP_CALL(hipcub::DeviceSegmentedRadixSort::SortPairs(
-        base + L.sort_temp, sort_bytes, G.keys, G.sorted_keys, G.vals, G.sorted_vals, (int)total,
-        work, G.seg_offsets, G.seg_offsets + 1, 0, end_bit, s));
+    {
+        lh::KernelMark mark("o1_sort", s);
+        LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+            base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned long long*>(G.keys),
+            reinterpret_cast<unsigned long long*>(G.sorted_keys), G.vals, G.sorted_vals,
+            (int)total, 0, end_bit, s));
+    }
     o1_group_flags<<<blocks(total + 1), kThreads, 0, s>>>(G, total);
     if ((status = lh::check_launch("o1_group_flags")) != LH_OK) return status;
     size_t scan_bytes = L.total - L.scan_temp;
@@ -1703,19 +1875,23 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     o1_group_starts<<<blocks(total + 1), kThreads, 0, s>>>(G, total, work);
     if ((status = lh::check_launch("o1_group_starts")) != LH_OK) return status;
     LH_HIP_CALL(hipMemcpyAsync(info, G.info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    int host_bits = G.key_bits;
+    LH_HIP_CALL(hipMemcpyAsync(info + 1, &host_bits, sizeof(int), hipMemcpyHostToDevice, s));
+    LH_HIP_CALL(hipStreamSynchronize(s));
     return LH_OK;
 }
 
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
-                           int groups, void* scratch, size_t scratch_bytes, int compute_bins,
+                           int groups, int key_bits, void* scratch, size_t scratch_bytes,
+                           int compute_bins,
                            double* charge, double* bins, void* workspace, size_t workspace_bytes,
                            void* stream) {
     int status = check_grouped(pencils, oms, optics, merge);
     if (status != LH_OK) return status;
     GroupLayout L;
-    if ((status = grouped_layout(oms->work_count, pencils->count, optics->bands, optics->bins,
-                                 L)) != LH_OK) return status;
+    if ((status = grouped_layout(oms->work_count, pencils->count, slots_of(pencils),
+                                 optics->bands, optics->bins, L)) != LH_OK) return status;
     LH_REQUIRE(workspace && workspace_bytes >= L.total, "workspace too small: %zu < %zu",
                workspace_bytes, L.total);
     LH_REQUIRE(charge != nullptr && (!compute_bins || bins != nullptr),
@@ -1723,14 +1899,18 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     LH_REQUIRE(optics->mut_min > 0 && optics->speed_max > 0, "mut_min and speed_max must be set");
     int work = oms->work_count;
     ScratchLayout SL;
-    if ((status = scratch_layout(work, optics->bands, optics->bins, optics->n_mono,
-                                 std::max(groups, 0), SL)) != LH_OK) return status;
+    LH_REQUIRE(key_bits >= 1 && key_bits <= 62, "key_bits must come from lh_order1_group_keys");
+    if ((status = scratch_layout(work, pencils->count, slots_of(pencils), optics->bands,
+                                 optics->bins, optics->n_mono, std::max(groups, 0), SL)) != LH_OK)
+        return status;
     LH_REQUIRE(groups >= 0 && scratch && scratch_bytes >= SL.total,
                "scratch too small: %zu < %zu", scratch_bytes, SL.total);
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* base = static_cast<char*>(workspace);
     char* sc = static_cast<char*>(scratch);
     Group G = make_group(pencils, oms, optics, merge, base, L);
+    G.key_bits = key_bits;
+    G.key_mask = (1LL << key_bits) - 1;
     G.compute_bins = compute_bins;
     G.charge = charge;
     G.bins = bins;
@@ -1743,19 +1923,34 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.part_pre = reinterpret_cast<double*>(sc + SL.part_pre);
     G.part_charge = reinterpret_cast<double*>(sc + SL.part_charge);
     G.segments = SL.segments;
+    G.piece_start = reinterpret_cast<int*>(sc + SL.piece_start);
+    G.piece_values = reinterpret_cast<double*>(sc + SL.piece_values);
+    size_t scan_bytes = SL.total - SL.scan_temp;
     if (groups > 0) {
-        o1_group_sums<<<(groups + 3) / 4, kThreads, 0, s>>>(G, groups);
+        int* counts = reinterpret_cast<int*>(sc + SL.piece_count);
+        o1_piece_counts<<<blocks(groups + 1), kThreads, 0, s>>>(G, groups, counts);
+        if ((status = lh::check_launch("o1_piece_counts")) != LH_OK) return status;
+        LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, counts,
+                                                     G.piece_start, groups + 1, s));
+        {
+            lh::KernelMark mark("o1_group_sums", s);
+            o1_group_sums<<<(unsigned)((SL.pieces + 3) / 4), kThreads, 0, s>>>(G, groups);
+        }
         if ((status = lh::check_launch("o1_group_sums")) != LH_OK) return status;
+        o1_group_combine<<<blocks(groups), kThreads, 0, s>>>(G, groups);
+        if ((status = lh::check_launch("o1_group_combine")) != LH_OK) return status;
     }
     o1_batch_flags<<<blocks(groups + 1), kThreads, 0, s>>>(G, groups, work);
     if ((status = lh::check_launch("o1_batch_flags")) != LH_OK) return status;
-    size_t scan_bytes = SL.total - SL.scan_temp;
     LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, G.batch_flag,
                                                  G.batch_scan, groups + 1, s));
     o1_batch_starts<<<blocks(std::max(groups, work) + 1), kThreads, 0, s>>>(G, groups, work);
     if ((status = lh::check_launch("o1_batch_starts")) != LH_OK) return status;
     dim3 grid(work, SL.segments);
-    o1_batches<<<grid, 64, 0, s>>>(G);
+    {
+        lh::KernelMark mark("o1_batches", s);
+        o1_batches<<<grid, 64, 0, s>>>(G);
+    }
     if ((status = lh::check_launch("o1_batches")) != LH_OK) return status;
     o1_finalize<<<work, kThreads, 0, s>>>(G);
     return lh::check_launch("o1_finalize");

@@ -122,6 +122,7 @@ extern "C" int lh_readout_bins(const float* basis, const double* omega, const vo
     if (n_obs == 0) return LH_OK;
     LH_REQUIRE(basis && omega && spectrum && bins, "lh_readout_bins: null device pointer");
     dim3 grid((n_bins + kTile - 1) / kTile, (n_obs + kTile - 1) / kTile);
+    lh::KernelMark mark("readout_kernel", static_cast<hipStream_t>(stream));
     hipLaunchKernelGGL(readout_kernel, grid, dim3(64), 0, static_cast<hipStream_t>(stream),
                        basis, omega, static_cast<const float2*>(spectrum), origins, n_bins,
                        n_freq, n_obs, bins);

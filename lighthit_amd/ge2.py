@@ -250,10 +250,12 @@ def apply_device(pack, source, receivers, alpha, window, *, frequencies=None, ou
     descriptor = pack.struct if band is None else pack.band(int(band))
     need = workspace_bytes(descriptor.bands, nfreq, receivers.count, source.points)
     ws = _workspace(source.d_points.device, need) if need else None
-    _lib.call("lh_ge2_apply", ctypes.byref(descriptor), ctypes.byref(source.struct),
-              ctypes.byref(receivers.struct), _address(d_alpha), nfreq, low, high,
-              _address(out), _address(ws), ctypes.c_size_t(ws.numel() if ws is not None else 0),
-              _lib.stream_handle(stream))
+    with _lib.stage("ge2_apply" if nfreq > 1 else "ge2_zero"):
+        _lib.call("lh_ge2_apply", ctypes.byref(descriptor), ctypes.byref(source.struct),
+                  ctypes.byref(receivers.struct), _address(d_alpha), nfreq, low, high,
+                  _address(out), _address(ws),
+                  ctypes.c_size_t(ws.numel() if ws is not None else 0),
+                  _lib.stream_handle(stream))
     return out
 
 

@@ -73,10 +73,11 @@ _lib.SIGNATURES.update({
     "lh_order1_group_keys": (_I, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
                                   ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _P, _P, _S,
                                   _P]),
-    "lh_order1_group_scratch_bytes": (_S, [ctypes.POINTER(_Oms), ctypes.POINTER(_Optics), _I]),
+    "lh_order1_group_scratch_bytes": (_S, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
+                                           ctypes.POINTER(_Optics), _I]),
     "lh_order1_group_charge": (_I, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
-                                    ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _I, _P, _S,
-                                    _I, _P, _P, _P, _S, _P]),
+                                    ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _I, _I, _P,
+                                    _S, _I, _P, _P, _P, _S, _P]),
 })
 
 _GL48 = np.polynomial.legendre.leggauss(48)
@@ -186,17 +187,45 @@ def spectral_tables(medium, detector, quadrature):
 
 
 def _ifuncs(z):
-    if abs(z) < 1.0:
-        out = []
-        for j in range(3):
-            acc = _TAYLOR[j, 23]
-            for n in range(22, -1, -1):
-                acc = acc * z + _TAYLOR[j, n]
-            out.append(acc)
-        return out
-    ez = math.exp(z)
-    return [(ez - 1.0) / z, (ez * (z - 1.0) + 1.0) / (z * z),
-            (ez * (z * z - 2.0 * z + 2.0) - 2.0) / (z * z * z)]
+    """``(i0, i1, i2)`` of every entry of ``z``: the integrals of ``t^k e^{z t}``
+    over [0, 1], by their Taylor series inside |z| < 1 and in closed form
+    outside (``_prompt_numba.py:160-183``)."""
+    z = np.asarray(z, float)
+    out = np.empty(z.shape + (3,))
+    small = np.abs(z) < 1.0
+    zs = z[small]
+    for j in range(3):
+        acc = np.full(zs.shape, _TAYLOR[j, 23])
+        for n in range(22, -1, -1):
+            acc = acc * zs + _TAYLOR[j, n]
+        out[small, j] = acc
+    zl = z[~small]
+    ez = np.exp(zl)
+    out[~small, 0] = (ez - 1.0) / zl
+    out[~small, 1] = (ez * (zl - 1.0) + 1.0) / (zl * zl)
+    out[~small, 2] = (ez * (zl * zl - 2.0 * zl + 2.0) - 2.0) / (zl * zl * zl)
+    return out
+
+
+_optics_cache = {}
+
+
+def device_optics(tables, alpha, g, edges, settings, device="cuda"):
+    """The :class:`DeviceOptics` of these inputs, built once per process and
+    kept resident (its tables depend on the medium, detector and bins only)."""
+    arrays = [np.asarray(tables[k], float).tobytes() for k in ("mus", "mut", "speed", "s0w",
+                                                               "s2w")]
+    key = (*arrays, np.asarray(alpha, float).tobytes(), float(g),
+           np.asarray(edges, float).tobytes(), int(settings.direct_bins),
+           int(settings.max_path_nodes), float(settings.path_step),
+           float(settings.path_step_tail), float(settings.tail_exponent),
+           float(settings.smoothing_factor), str(device))
+    optics = _optics_cache.get(key)
+    if optics is None:
+        if len(_optics_cache) >= 32:
+            _optics_cache.clear()
+        optics = _optics_cache[key] = DeviceOptics(tables, alpha, g, edges, settings, device)
+    return optics
 
 
 class DeviceOptics:
@@ -209,7 +238,7 @@ class DeviceOptics:
         kappa = -tables["mut"] * tables["speed"]
         eedge = np.exp(np.outer(kappa, edges))
         widths = np.diff(edges)
-        itab = np.array([[_ifuncs(k * w) for w in widths] for k in kappa])
+        itab = _ifuncs(np.outer(kappa, widths))
         coef, ipow, jpow, degree = acceptance_monomials(alpha)
 
         def up(a, dtype=torch.float64):
@@ -339,8 +368,9 @@ def compress(unit, cell_m, pixel_face, ring_nodes, *, parent_face=0, stream=None
     ws = _workspace("compress", device, need)
     info = torch.zeros(8, dtype=torch.int32, device=device)
     handle = _lib.stream_handle(stream)
-    _lib.call("lh_order1_compress_keys", ctypes.byref(chords), ctypes.byref(grid), _addr(info),
-              _addr(ws), ctypes.c_size_t(ws.numel()), handle)
+    with _lib.stage("order1_compress"):
+        _lib.call("lh_order1_compress_keys", ctypes.byref(chords), ctypes.byref(grid),
+                  _addr(info), _addr(ws), ctypes.c_size_t(ws.numel()), handle)
     runs = int(info[0].item())
     pencils = DevicePencils(2 * runs if parent_face else runs, device, bool(parent_face))
     out = pencils.struct()
@@ -371,18 +401,21 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
     ws = _workspace("grouped", device, need)
     info = torch.zeros(8, dtype=torch.int32, device=device)
     handle = _lib.stream_handle(stream)
-    _lib.call("lh_order1_group_keys", ctypes.byref(pen), ctypes.byref(om),
-              ctypes.byref(optics.struct), ctypes.byref(m), _addr(info), _addr(ws),
-              ctypes.c_size_t(ws.numel()), handle)
-    groups = int(info[0].item())
-    scratch_bytes = _size("lh_order1_group_scratch_bytes", ctypes.byref(om),
+    name = "order1_screen" if pencils.parent is not None else "order1_grouped"
+    with _lib.stage(name + "_keys"):
+        _lib.call("lh_order1_group_keys", ctypes.byref(pen), ctypes.byref(om),
+                  ctypes.byref(optics.struct), ctypes.byref(m), _addr(info), _addr(ws),
+                  ctypes.c_size_t(ws.numel()), handle)
+    groups, key_bits = (int(v) for v in info[:2].tolist())
+    scratch_bytes = _size("lh_order1_group_scratch_bytes", ctypes.byref(pen), ctypes.byref(om),
                           ctypes.byref(optics.struct), groups)
     scratch = _workspace("group_scratch", device, scratch_bytes)
-    _lib.call("lh_order1_group_charge", ctypes.byref(pen), ctypes.byref(om),
-              ctypes.byref(optics.struct), ctypes.byref(m), groups, _addr(scratch),
-              ctypes.c_size_t(scratch.numel()), 1 if compute_bins else 0, _addr(charge),
-              _addr(bins) if compute_bins else None, _addr(ws), ctypes.c_size_t(ws.numel()),
-              handle)
+    with _lib.stage(name + "_charge"):
+        _lib.call("lh_order1_group_charge", ctypes.byref(pen), ctypes.byref(om),
+                  ctypes.byref(optics.struct), ctypes.byref(m), groups, key_bits, _addr(scratch),
+                  ctypes.c_size_t(scratch.numel()), 1 if compute_bins else 0, _addr(charge),
+                  _addr(bins) if compute_bins else None, _addr(ws), ctypes.c_size_t(ws.numel()),
+                  handle)
     del work_dev
     return charge, bins
 
@@ -423,7 +456,7 @@ def spectral_order1(chords, oms, work, tables, alpha, g, edges, threshold, order
     for unit in evaluation_units(tables, band_valid, True):
         if not np.any(unit["valid"]):
             continue
-        optics = DeviceOptics(unit["tables"], alpha, g, edges, settings, device)
+        optics = device_optics(unit["tables"], alpha, g, edges, settings, device)
         face = settings.fused_pixel_face if unit["fused"] else settings.pixel_face
         ring = settings.fused_ring_nodes if unit["fused"] else settings.ring_nodes
         tasks.append((DeviceUnitChords(chords, unit["valid"], unit["cone"]), optics, face, ring))

@@ -144,9 +144,10 @@ class ReadoutBasis:
             raise ValueError("origins must be a float64 tensor with one value per observation")
         if out is None:
             out = torch.empty((nobs, self.bins), dtype=torch.float32, device=spectrum.device)
-        _lib.call("lh_readout_bins", _lib.ptr(self.device), _lib.ptr(self.omega_device),
-                  _lib.ptr(spectrum), _lib.ptr(origins), self.bins, nfreq, nobs,
-                  _lib.ptr(out), _lib.stream_handle(stream))
+        with _lib.stage("readout"):
+            _lib.call("lh_readout_bins", _lib.ptr(self.device), _lib.ptr(self.omega_device),
+                      _lib.ptr(spectrum), _lib.ptr(origins), self.bins, nfreq, nobs,
+                      _lib.ptr(out), _lib.stream_handle(stream))
         return out
 
 

@@ -259,7 +259,9 @@ def delay_groups(elements, *, prompt_delay_ns, cell_ns, window_end_ns, group_ind
 
 
 def spectral_attenuation(kernel):
-    """Absorption factor of the detected spectrum vs distance (``shower_basis.py:388-405``)."""
+    """Absorption factor of the detected spectrum vs distance as the table
+    ``(radius, log_factor)``: factor(d) = exp(interp(d, radius, log_factor))
+    (``shower_basis.py:388-405``)."""
     wavelengths = np.asarray(kernel.wavelength.wavelength_nm, float)
     weights = (np.asarray(kernel.wavelength.weight_nm, float)
                * np.asarray(kernel.detector.spectral_weight(wavelengths), float)
@@ -269,31 +271,73 @@ def spectral_attenuation(kernel):
     radius = np.geomspace(1e-3, 1e5, 1025)
     log_factor = (np.log(np.exp(-np.outer(radius, absorption - lowest)) @ weights
                          / weights.sum()) - lowest * radius)
-    return lambda distance: np.exp(np.interp(distance, radius, log_factor))
+    return radius, log_factor
 
 
-def window_report(kernel, points, photons, positions, low, high, attenuation, efficiency):
-    """Usable basis points per OM and the far-tail estimate (``shower_basis.py:408-439``)."""
-    peak_acceptance = float(np.max(np.abs(np.asarray(
-        kernel.detector.angular_acceptance(np.linspace(-1, 1, 257)), float))))
-    area = np.asarray(kernel.detector.effective_area_m2, float)
-    n = len(positions)
-    used = np.zeros(n, np.int64)
-    far = np.zeros(n, np.int64)
-    near = np.zeros(n, np.int64)
-    tail = np.zeros(n)
-    block = max(1, min(n, 1_000_000 // max(len(points), 1)))
-    for start in range(0, n, block):
-        stop = min(start + block, n)
-        radius = np.linalg.norm(points[None] - positions[start:stop, None], axis=2)
-        near[start:stop] = np.count_nonzero(radius < low, axis=1)
-        far_mask = radius > high
-        far[start:stop] = np.count_nonzero(far_mask, axis=1)
-        used[start:stop] = len(points) - near[start:stop] - far[start:stop]
-        distance = np.where(far_mask, radius, 1.)
-        tail[start:stop] = (10 * area[start:stop] * peak_acceptance * efficiency
-                            * np.sum(np.where(far_mask, photons[None] * attenuation(distance)
-                                              / (4 * np.pi * distance ** 2), 0.), axis=1))
+class _WindowInput(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("channels", ctypes.c_void_p),
+                ("count", ctypes.c_int), ("channel_stride", ctypes.c_int),
+                ("frame", ctypes.c_void_p), ("positions", ctypes.c_void_p),
+                ("areas", ctypes.c_void_p), ("n_oms", ctypes.c_int),
+                ("low", ctypes.c_double), ("high", ctypes.c_double),
+                ("weight", ctypes.c_double), ("scale", ctypes.c_double),
+                ("radius", ctypes.c_void_p), ("log_factor", ctypes.c_void_p),
+                ("table", ctypes.c_int)]
+
+
+_lib.SIGNATURES["lh_window_report"] = (ctypes.c_int, [
+    ctypes.POINTER(_WindowInput), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_void_p])
+
+
+class WindowGeometry:
+    """OM positions and areas, the attenuation table and the acceptance peak of
+    one kernel on the device, for :func:`window_report`."""
+
+    def __init__(self, kernel, device="cuda"):
+        import torch
+        detector = kernel.detector
+        radius, log_factor = spectral_attenuation(kernel)
+        self.peak_acceptance = float(np.max(np.abs(np.asarray(
+            detector.angular_acceptance(np.linspace(-1, 1, 257)), float))))
+        self.positions = torch.as_tensor(np.ascontiguousarray(detector.positions_m, float),
+                                         device=device)
+        self.areas = torch.as_tensor(np.ascontiguousarray(detector.effective_area_m2, float),
+                                     device=device)
+        self.radius = torch.as_tensor(radius, device=device)
+        self.log_factor = torch.as_tensor(log_factor, device=device)
+        self.count = len(detector.positions_m)
+        self.device = device
+
+
+def window_report(geometry, band, low, high, efficiency, stream=None):
+    """Usable, beyond and below-window basis points per OM and the far-tail
+    estimate of one compiled band (``shower_basis.py:408-439``), on the device:
+    ``(used, far, near, tail)`` tensors over the OMs of ``geometry``."""
+    import torch
+    n = geometry.count
+    device = geometry.device
+    used = torch.empty(n, dtype=torch.int32, device=device)
+    far = torch.empty(n, dtype=torch.int32, device=device)
+    near = torch.empty(n, dtype=torch.int32, device=device)
+    tail = torch.empty(n, dtype=torch.float64, device=device)
+    points = band.points if band.points.dtype == torch.float64 else band.points.double()
+    points = points.contiguous()
+    channels = band.channels.contiguous()
+    f = band.frame
+    frame = torch.as_tensor(np.concatenate([np.asarray(f.centre_m, float), np.asarray(f.axis),
+                                            np.asarray(f.first), np.asarray(f.second)]),
+                            dtype=torch.float64, device=device)
+    stride = int(np.prod(channels.shape[1:])) if channels.dim() > 1 else 1
+    w = _WindowInput(_lib.ptr(points), _lib.ptr(channels), int(points.shape[0]), stride,
+                     _lib.ptr(frame), _lib.ptr(geometry.positions), _lib.ptr(geometry.areas), n,
+                     float(low), float(high), float(np.sqrt(4 * np.pi) / efficiency),
+                     float(10 * geometry.peak_acceptance * efficiency),
+                     _lib.ptr(geometry.radius), _lib.ptr(geometry.log_factor),
+                     int(geometry.radius.numel()))
+    with _lib.stage("window_report"):
+        _lib.call("lh_window_report", ctypes.byref(w), _lib.ptr(used), _lib.ptr(far),
+                  _lib.ptr(near), _lib.ptr(tail), _lib.stream_handle(stream))
     return used, far, near, tail
 
 
@@ -606,6 +650,7 @@ class ShowerTransport:
         self._compiled = {}
         self._order0 = None
         self._readout = {}
+        self._window = None
 
     def tables(self):
         """The pack folded onto the node bands, resident on the device (once)."""
@@ -652,6 +697,12 @@ class ShowerTransport:
             self._order0 = Order0(self.kernel.medium, self.kernel.detector,
                                   self.kernel.acceptance()["alpha"], device=self.device)
         return self._order0
+
+    def window_geometry(self, kernel):
+        """Device geometry of the radial-window report (once per detector)."""
+        if self._window is None or self._window[0] is not kernel.detector:
+            self._window = (kernel.detector, WindowGeometry(kernel, self.device))
+        return self._window[1]
 
     def readout(self, omega, edges):
         from .readout import ReadoutBasis
@@ -784,24 +835,25 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
 
     # radial windows of every node-band point set
     stage = perf_counter()
-    used = np.zeros(n, np.int64)
-    far = np.zeros(n, np.int64)
-    tail = np.zeros(n)
-    near = np.zeros(n, bool)
-    attenuation = spectral_attenuation(kernel)
+    geometry = (route.window_geometry(kernel) if route is not None
+                else WindowGeometry(kernel, device))
+    used_dev = torch.zeros(n, dtype=torch.int32, device=device)
+    far_dev = torch.zeros(n, dtype=torch.int32, device=device)
+    near_dev = torch.zeros(n, dtype=torch.int32, device=device)
+    tail_dev = torch.zeros(n, dtype=torch.float64, device=device)
     weight_ratio = float(np.sum(fold.scale) / np.sum(fold.scale[list(fold.bands)]))
     for group in [sources] + ([late_zero] if late_zero is not None else []):
         for band, item in zip(fold.bands, group):
-            photons = (np.sqrt(4 * np.pi) * np.abs(item.channels[:, 0, 0].cpu().numpy())
-                       / efficiency[band])
-            points = item.to_points_m()
             part_used, part_far, part_near, part_tail = window_report(
-                kernel, points, photons, positions, low, high, attenuation,
-                float(efficiency[band]))
-            near |= np.asarray(part_near) > 0
-            used = np.maximum(used, part_used)
-            far = np.maximum(far, part_far)
-            tail += weight_ratio * part_tail
+                geometry, item, low, high, float(efficiency[band]))
+            near_dev = torch.maximum(near_dev, part_near)
+            used_dev = torch.maximum(used_dev, part_used)
+            far_dev = torch.maximum(far_dev, part_far)
+            tail_dev += weight_ratio * part_tail
+    used = used_dev.cpu().numpy().astype(np.int64)
+    far = far_dev.cpu().numpy().astype(np.int64)
+    near = near_dev.cpu().numpy() > 0
+    tail = tail_dev.cpu().numpy()
     radial_tail = screen_radial_tail(tail, basis, high)
     reach = np.flatnonzero(used > 0)
     if np.any(near):

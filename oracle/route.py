@@ -86,6 +86,26 @@ def _attenuation(kernel):
     return lambda d: np.exp(np.interp(d, radius, log_factor))
 
 
+def window_report(points_m, photons, positions, area, low, high, radius, log_factor,
+                  peak_acceptance, efficiency):
+    """Basis points below, inside and beyond the radial window per OM and the
+    far-tail estimate (``shower_basis.py:408-439``), one OM at a time:
+    ``(used, far, near, tail)``."""
+    n = len(positions)
+    used, far, near = (np.zeros(n, np.int64) for _ in range(3))
+    tail = np.zeros(n)
+    for d in range(n):
+        r = np.sqrt(np.sum((points_m - positions[d]) ** 2, axis=1))
+        is_near, is_far = r < low, r > high
+        near[d], far[d] = int(is_near.sum()), int(is_far.sum())
+        used[d] = len(points_m) - near[d] - far[d]
+        rf = r[is_far]
+        tail[d] = (10 * area[d] * peak_acceptance * efficiency
+                   * np.sum(photons[is_far] * np.exp(np.interp(rf, radius, log_factor))
+                            / (4 * np.pi * rf ** 2)))
+    return used, far, near, tail
+
+
 def transport_shower(kernel, elements, bundle, fold, *, config, subdivide=1):
     """Route response arrays of one event: dict with ``charge`` (N, 3), ``bins``
     (N, nb, 3), ``spectrum`` (W, N), ``origins``, ``active``, ``level``."""

@@ -82,3 +82,69 @@ def test_receiver_subset_computes_every_listed_om(demonstration):
     assert np.all(part.active)
     np.testing.assert_allclose(part.charge_components_pe[:, :2],
                                response.charge_components_pe[ids, :2], rtol=1e-6)
+
+
+def _oracle_window(geometry_kernel, item, low, high, efficiency):
+    from lighthit_amd.route import spectral_attenuation
+    from oracle import route as oracle_route
+    d = geometry_kernel.detector
+    radius, log_factor = spectral_attenuation(geometry_kernel)
+    peak = float(np.max(np.abs(np.asarray(d.angular_acceptance(np.linspace(-1, 1, 257)),
+                                          float))))
+    shape = tuple(item.channels.shape)
+    ch = item.channels.cpu().numpy().reshape(shape[0], int(np.prod(shape[1:])))[:, 0]
+    photons = np.sqrt(4 * np.pi) * np.abs(ch).astype(float) / efficiency
+    return oracle_route.window_report(item.to_points_m(), photons, d.positions_m,
+                                      d.effective_area_m2, low, high, radius, log_factor,
+                                      peak, efficiency)
+
+
+def test_window_report_matches_the_oracle(demonstration):
+    from lighthit_amd.route import window_report
+    w, route, response, want = demonstration
+    kernel = route.kernel
+    geometry = route.window_geometry(kernel)
+    low, high = kernel.config.radial_range_m
+    efficiency = np.asarray(kernel.detector.spectral_weight(kernel.wavelength.wavelength_nm))
+    sources = route._compiled["sources"][0]
+    bands = route.tables()["fold"].bands
+    for band, item in zip(bands, sources):
+        got = [t.cpu().numpy() for t in window_report(geometry, item, low, high,
+                                                      float(efficiency[band]))]
+        ref = _oracle_window(kernel, item, low, high, float(efficiency[band]))
+        for a, b in zip(got[:3], ref[:3]):
+            assert np.array_equal(a.astype(np.int64), b)
+        # the channel magnitude is a float32 hypot on both sides (one ulp apart)
+        np.testing.assert_allclose(got[3], ref[3], rtol=1e-6, atol=1e-300)
+        assert np.all(got[0] + got[1] + got[2] == item.points.shape[0])
+
+
+def test_window_report_every_branch(device):
+    """Points below, inside and beyond the window of every OM, with an empty band."""
+    import torch
+    from lighthit_amd.axial import AxisFrame
+    from lighthit_amd.compile import CompiledBand
+    from lighthit_amd.route import RouteConfig, RouteKernel, window_report, WindowGeometry
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    kernel = RouteKernel(w.medium, w.detector, RouteConfig())
+    geometry = WindowGeometry(kernel, device)
+    rng = np.random.default_rng(5)
+    centre = w.detector.positions_m.mean(axis=0)
+    axis = np.array([0.2, -0.3, 0.93]) / np.linalg.norm([0.2, -0.3, 0.93])
+    first = np.cross(axis, [1.0, 0, 0])
+    first /= np.linalg.norm(first)
+    frame = AxisFrame(centre, axis, first, np.cross(axis, first))
+    for count in (0, 1, 700):
+        pts = np.column_stack([rng.normal(scale=40, size=count), rng.normal(scale=40, size=count),
+                               rng.uniform(-400, 400, size=count)])
+        ch = (rng.normal(size=(count, 9, 3)) + 1j * rng.normal(size=(count, 9, 3)))
+        item = CompiledBand(frame, torch.as_tensor(pts, device=device),
+                            torch.as_tensor(ch.astype(np.complex64), device=device), 2, 2, 0.0)
+        got = [t.cpu().numpy() for t in window_report(geometry, item, 2.0, 150.0, 0.7)]
+        ref = _oracle_window(kernel, item, 2.0, 150.0, 0.7)
+        for a, b in zip(got[:3], ref[:3]):
+            assert np.array_equal(a.astype(np.int64), b)
+        np.testing.assert_allclose(got[3], ref[3], rtol=1e-6, atol=1e-300)
+        if count == 700:
+            assert ref[1].min() > 0 and ref[0].min() > 0
