@@ -456,6 +456,7 @@ struct Group {
     double* part_pre;         // (work, segments, bands, 3)
     double* part_charge;      // (work, segments, bands)
     int segments;             // batch segments per OM (one wave each)
+    double* path;             // (work, segments, 3, kMaxPathNodes) path nodes, g0, g2
     double* prepared;         // (n_pen, 7)
     double* lambda_bins;      // (work, bands, bins)
     int* k_range;             // (work, 4) k1 min, k1 max, k0 min, k0 max
@@ -493,6 +494,7 @@ __device__ double gauss_inverse_mean(double u, double v, double var_u, double va
     if (scale <= 0.0) return 1e300;
     double s0 = 1.0 / sqrt(scale);
     double total = 0.0;
+#pragma unroll 2
     for (int k = 0; k < kGauss; ++k) {
         double s = s0 * kTangent[k];
         double s2 = s * s;
@@ -673,7 +675,10 @@ __device__ inline void radius_keys(const Group& G, int p, int d, long long* k0, 
 // Keys are compact per OM: ((k1 - k1min) * n_k0 + (k0 - k0min)) * 2 theta_bins
 // + backward * theta_bins + theta index. They order and group exactly as the
 // reference's (k1 * 2^21 + k0) * 2 theta_bins + ... keys.
-__device__ int pencil_entries(const Group& G, int p, int d, int w, Entry out[2]) {
+// Fills `out` with entry `want` (0 or 1) and, when key_other is set and there
+// are two entries, the key of entry 1; returns the number of entries.
+__device__ int pencil_entries(const Group& G, int p, int d, int w, int want, Entry& out,
+                              long long* key_other) {
     double r, h[3], theta, sine, c1, c3;
     frame_of(G, p, d, &r, h, &theta, &sine, &c1, &c3);
     if (G.screen) {
@@ -736,21 +741,21 @@ __device__ int pencil_entries(const Group& G, int p, int d, int w, Entry out[2])
         theta = theta_eff;
     }
     int nodes = weight2 > 0.0 ? 2 : 1;
-    for (int node = 0; node < nodes; ++node) {
-        double w = node == 0 ? weight : weight2;
-        double th = node == 0 ? theta : theta2;
-        Entry& e = out[node];
-        e.key = base + backward * G.theta_bins + theta_index(th, G.merge.theta_lo, G.log_ratio);
-        e.w0 = w * G.pen.w0[p];
-        e.w2 = w * G.pen.w2[p];
-        e.c1 = c1;
-        e.c3 = c3;
-        e.averaged = averaged;
-        e.theta = th;
-        e.r = r;
-        e.t_emit = t_emit;
-        e.tau = tau;
-    }
+    double wsel = want == 0 ? weight : weight2;
+    double th = want == 0 ? theta : theta2;
+    out.key = base + backward * G.theta_bins + theta_index(th, G.merge.theta_lo, G.log_ratio);
+    out.w0 = wsel * G.pen.w0[p];
+    out.w2 = wsel * G.pen.w2[p];
+    out.c1 = c1;
+    out.c3 = c3;
+    out.averaged = averaged;
+    out.theta = th;
+    out.r = r;
+    out.t_emit = t_emit;
+    out.tau = tau;
+    if (key_other != nullptr && nodes == 2)
+        *key_other = base + backward * G.theta_bins
+                     + theta_index(theta2, G.merge.theta_lo, G.log_ratio);
     return nodes;
 }
 
@@ -818,13 +823,16 @@ __global__ void __launch_bounds__(kThreads) o1_group_keys(Group G) {
     int w = blockIdx.y;
     if (p >= G.n_pen) return;
     int d = G.om.work[w];
-    Entry e[2];
-    int n = pencil_entries(G, p, d, w, e);
+    Entry e0;
+    long long key1 = 0;
+    int n = pencil_entries(G, p, d, w, 0, e0, &key1);
     size_t at = ((size_t)w * G.n_pen + p) * G.slots;
     long long high = (long long)w << G.key_bits;
-    for (int node = 0; node < G.slots; ++node) {
-        G.keys[at + node] = high | (node < n ? e[node].key : G.key_mask);
-        G.vals[at + node] = 2 * p + node;
+    G.keys[at] = high | (n > 0 ? e0.key : G.key_mask);
+    G.vals[at] = 2 * p;
+    if (G.slots == 2) {
+        G.keys[at + 1] = high | (n > 1 ? key1 : G.key_mask);
+        G.vals[at + 1] = 2 * p + 1;
     }
 }
 
@@ -859,19 +867,33 @@ __global__ void __launch_bounds__(kThreads) o1_group_starts(Group G, long long t
         G.group_end[G.group_scan[j] + G.group_flag[j] - 1] = (int)(j + 1);
 }
 
-__device__ inline void fill_moments(const Group& G, double weight0, double weight2, double c1,
-                                    double c3, int averaged, double* out0, double* out2) {
-    double cp[8], sp[8];
+// Acceptance monomials c1^i c3^j in the order k = 0..P, i = 0..k (j = k - i);
+// the host builds coef, ipow and jpow in this order.
+template <int P>
+struct Monomials {
+    static constexpr int N = (P + 1) * (P + 2) / 2;
+};
+
+template <int P>
+__device__ inline void fill_moments(double weight0, double weight2, double c1, double c3,
+                                    int averaged, double* out0, double* out2) {
+    double cp[P + 1], sp[P + 1];
     cp[0] = 1.0; sp[0] = 1.0;
     double perp2 = fmax(1.0 - c1 * c1, 0.0);
-    for (int p = 1; p <= G.opt.max_power; ++p) {
+#pragma unroll
+    for (int p = 1; p <= P; ++p) {
         cp[p] = cp[p - 1] * c1;
-        if (averaged) sp[p] = (p % 2 == 1) ? 0.0 : sp[p - 2] * perp2 * (p - 1.0) / p;
+        if (averaged) sp[p] = (p % 2 == 1) ? 0.0 : sp[p - 2 >= 0 ? p - 2 : 0] * perp2 * (p - 1.0) / p;
         else sp[p] = sp[p - 1] * c3;
     }
-    for (int m = 0; m < G.opt.n_mono; ++m) {
-        out0[m] = weight0 * cp[G.opt.ipow[m]] * sp[G.opt.jpow[m]];
-        out2[m] = weight2 * cp[G.opt.ipow[m]] * sp[G.opt.jpow[m]];
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k <= P; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i, ++m) {
+            out0[m] = weight0 * cp[i] * sp[k - i];
+            out2[m] = weight2 * cp[i] * sp[k - i];
+        }
     }
 }
 
@@ -885,7 +907,7 @@ __device__ inline double wave_min(double v) {
     return v;
 }
 
-constexpr int kPieceEntries = 256;   // group members per wave of the group sums
+constexpr int kPieceEntries = 64;    // group members per wave of the group sums
 
 // Pieces of every group: ceil(members / kPieceEntries).
 __global__ void __launch_bounds__(kThreads) o1_piece_counts(Group G, int groups, int* counts) {
@@ -897,8 +919,9 @@ __global__ void __launch_bounds__(kThreads) o1_piece_counts(Group G, int groups,
 }
 
 // Group sums: one wave per piece of at most kPieceEntries consecutive members
-// of one group, lanes over its members, a fixed butterfly reduction; the
+// of one group, a lane per member, a fixed butterfly reduction; the
 // pieces of a group are then added in piece order (deterministic).
+template <int P>
 __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
     int q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
@@ -909,23 +932,27 @@ __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
         if (G.piece_start[middle] <= q) low = middle; else high = middle;
     }
     int g = low;
-    int n_mono = G.opt.n_mono;
-    int nv = 2 * n_mono + 6;
+    constexpr int n_mono = Monomials<P>::N;
+    constexpr int nv = 2 * n_mono + 6;
     long long seg = (long long)G.slots * G.n_pen;
     long long j0 = G.group_start[g] + (long long)(q - G.piece_start[g]) * kPieceEntries;
     long long j1 = min((long long)G.group_end[g], j0 + kPieceEntries);
     int w = (int)(j0 / seg);
     int d = G.om.work[w];
-    double row[2 * kMaxMono + 6];
+    double row[nv];
+#pragma unroll
     for (int m = 0; m < nv; ++m) row[m] = 0.0;
-    double mom0[kMaxMono], mom2[kMaxMono];
-    for (long long j = j0 + lane; j < j1; j += 64) {
+    double mom0[n_mono], mom2[n_mono];
+    // one member per lane (a loop here makes the compiler keep far more
+    // registers live and drops the occupancy to one wave per SIMD)
+    long long j = j0 + lane;
+    if (j < j1) {
         int v = G.sorted_vals[j];
         int p = v >> 1, node = v & 1;
-        Entry e[2];
-        pencil_entries(G, p, d, w, e);
-        const Entry& x = e[node];
-        fill_moments(G, x.w0, x.w2, x.c1, x.c3, x.averaged, mom0, mom2);
+        Entry x;
+        pencil_entries(G, p, d, w, node, x, nullptr);
+        fill_moments<P>(x.w0, x.w2, x.c1, x.c3, x.averaged, mom0, mom2);
+#pragma unroll
         for (int m = 0; m < n_mono; ++m) {
             row[m] += mom0[m];
             row[n_mono + m] += mom2[m];
@@ -938,6 +965,7 @@ __global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
         row[2 * n_mono + 4] += omega * x.tau;
         row[2 * n_mono + 5] += omega * x.tau * x.tau;
     }
+#pragma unroll
     for (int m = 0; m < nv; ++m) {
         double total = wave_sum(row[m]);
         if (lane == 0) G.piece_values[(size_t)q * nv + m] = total;
@@ -1090,15 +1118,91 @@ __device__ void cubic_coefficients(const Group& G, const double* mom, double* a,
 }
 
 constexpr int kBatchGroups = 64;   // groups staged in LDS per profile pass
+constexpr int kMaxDirect = 8;      // direct_bins limit of the panel events
+
+// Deposits of one panel (_deposit_panel, _prompt_numba.py:186-253) as events:
+// kind 0 none; kind 1 direct, v[e] added to bin k_lo + e for the set bits of
+// mask; kind 2 prefix, (A, B, C, I_start) added at k_lo (or A, B, C to the
+// pre-window sums when pre is set) and (A, B, C, I_end) subtracted at k_end
+// when end is set.
+struct PanelEvents {
+    int kind, k_lo, k_end, n, mask, pre, end;
+    double v[kMaxDirect];
+};
+
+__device__ void panel_events(const LambdaBins& B, double t_a, double t_b, double e_a, double e_b,
+                             double alpha, double beta, double gamma, double kappa,
+                             PanelEvents& out) {
+    out.kind = 0;
+    int nb = B.nb;
+    const double* edges = B.edges;
+    double lo = t_a > edges[0] ? t_a : edges[0];
+    double hi = t_b < edges[nb] ? t_b : edges[nb];
+    if (hi <= lo) return;
+    int k_lo = find_bin(edges, nb, lo);
+    int k_hi = find_bin(edges, nb, hi);
+    if (k_hi >= nb || (k_hi > k_lo && hi <= edges[k_hi])) k_hi -= 1;
+    out.k_lo = k_lo;
+    if (k_hi - k_lo + 1 <= B.direct_bins) {
+        out.kind = 1;
+        out.n = k_hi - k_lo + 1;
+        out.mask = 0;
+#pragma unroll
+        for (int e = 0; e < kMaxDirect; ++e) {
+            if (e >= out.n) break;
+            int k = k_lo + e;
+            double s_lo = lo > edges[k] ? lo : edges[k];
+            double s_hi = hi < edges[k + 1] ? hi : edges[k + 1];
+            if (s_hi <= s_lo) continue;
+            double e_lo = s_lo == t_a ? e_a : B.eedge[k];
+            if (s_lo != t_a && s_lo != edges[k]) e_lo = exp(kappa * s_lo);
+            double e_hi = s_hi == t_b ? e_b : B.eedge[k + 1];
+            if (s_hi != t_b && s_hi != edges[k + 1]) e_hi = exp(kappa * s_hi);
+            double shift = s_lo - t_a;
+            double a_loc = alpha + shift * (beta + shift * gamma);
+            double b_loc = beta + 2.0 * gamma * shift;
+            out.v[e] = quad_integral(e_lo, a_loc, b_loc, gamma, s_hi - s_lo, kappa, e_hi);
+            out.mask |= 1 << e;
+        }
+        return;
+    }
+    out.kind = 2;
+    double big_a = alpha - t_a * (beta - gamma * t_a);
+    double big_b = beta - 2.0 * gamma * t_a;
+    out.v[0] = big_a;
+    out.v[1] = big_b;
+    out.v[2] = gamma;
+    out.pre = t_a < edges[0];
+    if (!out.pre)
+        out.v[3] = quad_integral(e_a, alpha, beta, gamma, edges[k_lo + 1] - t_a, kappa,
+                                 B.eedge[k_lo + 1]);
+    out.end = t_b < edges[nb];
+    out.k_end = k_hi;
+    if (out.end) {
+        int k = find_bin(edges, nb, t_b);
+        out.k_end = k;
+        double shift = t_b - t_a;
+        double a_loc = alpha + shift * (beta + shift * gamma);
+        double b_loc = beta + 2.0 * gamma * shift;
+        out.v[4] = quad_integral(e_b, a_loc, b_loc, gamma, edges[k + 1] - t_b, kappa,
+                                 B.eedge[k + 1]);
+    }
+}
 
 // One wave per (OM, segment): the segment's batches b = first + s, first + s + S, ...
 // (_grouped_pencils_order1_block :1924-1995 and _batch_deposit :451-551) into the
-// segment's own partial accumulators.
+// segment's own partial accumulators. The path nodes and profiles live in the
+// slot's global scratch; the panels of one wavelength are spread over the lanes
+// 64 at a time, and every bin then adds its panels' events in panel order (the
+// order of the reference's serial loop, so the bins match it term by term).
+template <int P>
 __global__ void __launch_bounds__(64) o1_batches(Group G) {
-    __shared__ double s_buf[kMaxPathNodes];
-    __shared__ double g0[kMaxPathNodes];
-    __shared__ double g2[kMaxPathNodes];
     __shared__ double coeff[kBatchGroups][16];   // A0[4] B0[3] A2[4] B2[3] sine cosine
+    __shared__ int ev_bin[64 * kMaxDirect];
+    __shared__ double ev_val[64 * kMaxDirect][4];
+    __shared__ int head_at[64 * kMaxDirect];
+    __shared__ int pre_flag[64];
+    __shared__ double pre_val[64][3];
     int w = blockIdx.x, seg = blockIdx.y;
     int lane = threadIdx.x;
     int nl = G.opt.bands, nb = G.opt.bins;
@@ -1107,11 +1211,14 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
     size_t slot = (size_t)w * G.segments + seg;
     double* acc = G.part_acc + slot * nl * nb * 4;
     double* pre = G.part_pre + slot * nl * 3;
+    double* path = G.path + slot * 3 * kMaxPathNodes;
+    double* g0 = path + kMaxPathNodes;
+    double* g2 = path + 2 * kMaxPathNodes;
     if (G.compute_bins)
         for (size_t i = lane; i < (size_t)nl * nb * 4; i += 64) acc[i] = 0.0;
     for (int i = lane; i < nl * 3; i += 64) pre[i] = 0.0;
     double charge_lane = 0.0;
-    int cubic = G.opt.max_power <= 3;
+    constexpr bool cubic = P <= 3;
     double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
     double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
     for (int b = G.om_batches[w] + seg; b < G.om_batches[w + 1]; b += G.segments) {
@@ -1174,7 +1281,7 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             if (k == 0) sv = r_b;
             else if (k <= n_in) sv = r_b + sigma * sinh(u_win * k / n_in);
             else sv = r_b + sigma * sinh(u_win + (u_end - u_win) * (k - n_in) / n_out);
-            s_buf[k] = sv;
+            path[k] = sv;
             g0[k] = 0.0;
             g2[k] = 0.0;
         }
@@ -1198,7 +1305,7 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             __syncthreads();
             for (int k = lane; k < n; k += 64) {
                 double acc0 = g0[k], acc2 = g2[k];
-                double s_k = s_buf[k];
+                double s_k = path[k];
                 for (int q = 0; q < chunk; ++q) {
                     const double* cf = coeff[q];
                     double sin_t = cf[14];
@@ -1229,17 +1336,23 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
                         a2 += sn * (cf[11] + c * (cf[12] + c * cf[13]));
                     } else {
                         const double* row = G.values + (size_t)(c0 + q) * nv;
-                        double cpow[8], spow[8];
+                        double cpow[P + 1], spow[P + 1];
                         cpow[0] = 1.0; spow[0] = 1.0;
-                        for (int pw = 1; pw <= G.opt.max_power; ++pw) {
+#pragma unroll
+                        for (int pw = 1; pw <= P; ++pw) {
                             cpow[pw] = cpow[pw - 1] * c;
                             spow[pw] = spow[pw - 1] * sn;
                         }
                         a0 = 0.0; a2 = 0.0;
-                        for (int m = 0; m < n_mono; ++m) {
-                            double basis = G.opt.coef[m] * cpow[G.opt.ipow[m]] * spow[G.opt.jpow[m]];
-                            a0 += basis * row[m];
-                            a2 += basis * row[n_mono + m];
+                        int m = 0;
+#pragma unroll
+                        for (int kk = 0; kk <= P; ++kk) {
+#pragma unroll
+                            for (int i = 0; i <= kk; ++i, ++m) {
+                                double basis = G.opt.coef[m] * cpow[i] * spow[kk - i];
+                                a0 += basis * row[m];
+                                a2 += basis * row[Monomials<P>::N + m];
+                            }
                         }
                     }
                     double den = g1 - gg * c;
@@ -1252,45 +1365,52 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             }
         }
         __syncthreads();
-        // _batch_deposit (:451-551): one lane per wavelength
-        int lam = lane;
-        if (lam < nl && G.opt.mus[lam] != 0.0) {
+        // _batch_deposit (:451-551), one wavelength at a time
+        for (int lam = 0; lam < nl; ++lam) {
+            if (G.opt.mus[lam] == 0.0) continue;
             double v = G.opt.speed[lam];
             double mut = G.opt.mut[lam];
             double kappa = -mut * v;
             int last = n - 1;
             double limit = r_b + G.opt.tail_exponent / mut;
-            while (last > 2 && s_buf[last - 1] > limit) --last;
-            if (last >= 2) {
-                double spread = fmin(sqrt(12.0 * tau_var), (s_buf[last] - r_b) / v);
-                double front_time = t_b + r_b / v;
-                double end_time = t_b + s_buf[last] / v;
-                int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
-                              || end_time + 0.5 * spread <= G.opt.edges[0];
-                double t_base = outside ? 0.0 : t_b;
-                double e_emit = exp(kappa * t_base);
-                double h_scale = G.opt.mus[lam] * v / e_emit;
-                double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
-                LambdaBins B;
-                B.edges = G.opt.edges;
-                B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
-                B.acc = acc + (size_t)lam * nb * 4;
-                B.pre = pre + 3 * lam;
-                B.nb = nb;
-                B.direct_bins = G.opt.direct_bins;
-                bool bins = G.compute_bins && !outside;
-                double h_front = w0 * g0[0] + w2 * g2[0];
-                double total = 0.0;
-                double e_first = e_emit * exp(-mut * s_buf[0]);
-                double e_a = e_first, e_last = e_first;
-                int j = 0;
-                while (j < last) {
-                    double t_a, t_e, alpha, beta, gamma, e_b;
-                    int step;
-                    if (j + 2 <= last) {
-                        t_a = t_base + s_buf[j] / v;
-                        double t_m = t_base + s_buf[j + 1] / v;
-                        t_e = t_base + s_buf[j + 2] / v;
+            while (last > 2 && path[last - 1] > limit) --last;
+            if (last < 2) continue;
+            double spread = fmin(sqrt(12.0 * tau_var), (path[last] - r_b) / v);
+            double front_time = t_b + r_b / v;
+            double end_time = t_b + path[last] / v;
+            int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
+                          || end_time + 0.5 * spread <= G.opt.edges[0];
+            double t_base = outside ? 0.0 : t_b;
+            double e_emit = exp(kappa * t_base);
+            double h_scale = G.opt.mus[lam] * v / e_emit;
+            double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
+            LambdaBins B;
+            B.edges = G.opt.edges;
+            B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
+            B.acc = acc + (size_t)lam * nb * 4;
+            B.pre = pre + 3 * lam;
+            B.nb = nb;
+            B.direct_bins = G.opt.direct_bins;
+            bool bins = G.compute_bins && !outside;
+            double h_front = w0 * g0[0] + w2 * g2[0];
+            double e_first = e_emit * exp(-mut * path[0]);
+            double e_last = e_emit * exp(-mut * path[last]);
+            // panels: (2i, 2i+1, 2i+2) for i < last / 2; an odd last closes with
+            // the one-step panel at j = last - 1
+            int panels = last / 2 + (last & 1);
+            double total_lane = 0.0;
+            for (int base = 0; base < panels; base += 64) {
+                int i = base + lane;
+                PanelEvents ev;
+                ev.kind = 0;
+                if (i < panels) {
+                    double t_a, t_e, alpha, beta, gamma, e_a, e_b;
+                    int j;
+                    if (i < last / 2) {
+                        j = 2 * i;
+                        t_a = t_base + path[j] / v;
+                        double t_m = t_base + path[j + 1] / v;
+                        t_e = t_base + path[j + 2] / v;
                         double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
                         double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
                         double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
@@ -1300,12 +1420,12 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
                         alpha = h0;
                         beta = d1 - d2 * u_m;
                         gamma = d2;
-                        e_b = e_emit * exp(-mut * s_buf[j + 2]);
-                        step = 2;
+                        e_b = e_emit * exp(-mut * path[j + 2]);
                     } else {
-                        double t_p = t_base + s_buf[j - 1] / v;
-                        t_a = t_base + s_buf[j] / v;
-                        t_e = t_base + s_buf[j + 1] / v;
+                        j = last - 1;
+                        double t_p = t_base + path[j - 1] / v;
+                        t_a = t_base + path[j] / v;
+                        t_e = t_base + path[j + 1] / v;
                         double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
                         double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
                         double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
@@ -1315,35 +1435,109 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
                         alpha = h0;
                         beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
                         gamma = d2;
-                        e_b = e_emit * exp(-mut * s_buf[j + 1]);
-                        step = 1;
+                        e_b = e_emit * exp(-mut * path[j + 1]);
                     }
-                    total += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
-                    if (bins) deposit_panel(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa);
-                    e_a = e_b;
-                    e_last = e_b;
-                    j += step;
+                    // the serial loop carries e_a from the previous panel's e_b
+                    // (the same expression at node j)
+                    e_a = j == 0 ? e_first : e_emit * exp(-mut * path[j]);
+                    total_lane += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
+                    if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
                 }
-                double t_f = t_base + s_buf[0] / v, t_end = t_base + s_buf[last] / v;
-                double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
-                total += exact;
+                if (!bins) continue;
+                // flatten the panels' bin events in (panel, bin) order: bins never
+                // decrease along it, so every bin's events are contiguous and in
+                // panel order
+                int mine = 0;
+                if (ev.kind == 1) mine = __popc(ev.mask);
+                else if (ev.kind == 2) mine = (ev.pre ? 0 : 1) + (ev.end ? 1 : 0);
+                int offset = mine;
+                for (int off = 1; off < 64; off <<= 1) {
+                    int other = __shfl_up(offset, off, 64);
+                    if (lane >= off) offset += other;
+                }
+                int n_events = __shfl(offset, 63, 64);
+                offset -= mine;
+                if (ev.kind == 1) {
+#pragma unroll
+                    for (int e = 0; e < kMaxDirect; ++e) {
+                        if (!((ev.mask >> e) & 1)) continue;
+                        ev_bin[offset] = ev.k_lo + e;
+                        ev_val[offset][0] = 0.0; ev_val[offset][1] = 0.0;
+                        ev_val[offset][2] = 0.0; ev_val[offset][3] = ev.v[e];
+                        ++offset;
+                    }
+                } else if (ev.kind == 2) {
+                    if (!ev.pre) {
+                        ev_bin[offset] = ev.k_lo;
+                        ev_val[offset][0] = ev.v[0]; ev_val[offset][1] = ev.v[1];
+                        ev_val[offset][2] = ev.v[2]; ev_val[offset][3] = ev.v[3];
+                        ++offset;
+                    }
+                    if (ev.end) {
+                        ev_bin[offset] = ev.k_end;
+                        ev_val[offset][0] = -ev.v[0]; ev_val[offset][1] = -ev.v[1];
+                        ev_val[offset][2] = -ev.v[2]; ev_val[offset][3] = -ev.v[4];
+                    }
+                }
+                pre_flag[lane] = ev.kind == 2 && ev.pre;
+                pre_val[lane][0] = ev.v[0]; pre_val[lane][1] = ev.v[1];
+                pre_val[lane][2] = ev.v[2];
+                __syncthreads();
+                // the first event of every bin, compacted in order
+                int heads = 0;
+                for (int c = 0; c < n_events; c += 64) {
+                    int i = c + lane;
+                    bool head = i < n_events && (i == 0 || ev_bin[i] != ev_bin[i - 1]);
+                    unsigned long long ballot = __ballot(head);
+                    if (head) head_at[heads + __popcll(ballot & ((1ULL << lane) - 1))] = i;
+                    heads += __popcll(ballot);
+                }
+                __syncthreads();
+                // one lane per bin adds its events in panel order
+                for (int h = lane; h < heads; h += 64) {
+                    int i0 = head_at[h], i1 = h + 1 < heads ? head_at[h + 1] : n_events;
+                    double* a = B.acc + 4 * (size_t)ev_bin[i0];
+                    double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+                    for (int i = i0; i < i1; ++i) {
+                        a0 += ev_val[i][0]; a1 += ev_val[i][1];
+                        a2 += ev_val[i][2]; a3 += ev_val[i][3];
+                    }
+                    a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+                }
+                if (lane == 0) {
+                    int count = min(64, panels - base);
+                    for (int q = 0; q < count; ++q) {
+                        if (!pre_flag[q]) continue;
+                        B.pre[0] += pre_val[q][0]; B.pre[1] += pre_val[q][1];
+                        B.pre[2] += pre_val[q][2];
+                    }
+                }
+                __syncthreads();
+            }
+            double total = wave_sum(total_lane);
+            double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
+            double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
+            total += exact;
+            if (lane == 0) {
                 if (bins && spread > 0.0) {
                     double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
                     double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
                     double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
                     double flat = t_end > t_2
-                                      ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa, e_last)
+                                      ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa,
+                                                      e_last)
                                       : 0.0;
                     double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
                     deposit_panel(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa);
-                    if (t_end > t_2) deposit_panel(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa);
+                    if (t_end > t_2)
+                        deposit_panel(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa);
                 } else if (bins && t_end > t_f) {
                     deposit_panel(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa);
                 }
-                charge_lane += total;
             }
+            if (lane == lam) charge_lane += total;
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (lane < nl) G.part_charge[slot * nl + lane] = charge_lane;
 }
@@ -1576,15 +1770,19 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
 }
 
 // Batch segments per OM: enough waves to fill the GPU, partials within 512 MiB.
+// Batch segments per OM: about 4096 waves in all, within 2 GiB of per-slot
+// partial deposits and path arrays.
 int segments_for(int work, int bands, int bins) {
-    int S = std::max(1, std::min(32, (2048 + work - 1) / work));
-    while (S > 1 && (double)work * S * bands * bins * 32.0 > 512.0 * 1024 * 1024) S /= 2;
+    int S = std::max(1, std::min(64, (4096 + work - 1) / work));
+    while (S > 1 && (double)work * S * (bands * bins * 32.0 + 24.0 * kMaxPathNodes)
+                        > 2048.0 * 1024 * 1024)
+        S /= 2;
     return S;
 }
 
 struct ScratchLayout {
     size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
-        part_charge, piece_count, piece_start, piece_values, scan_temp, total;
+        part_charge, path, piece_count, piece_start, piece_values, scan_temp, total;
     int segments;
     long long pieces;         // bound on the group-sum pieces
 };
@@ -1609,6 +1807,7 @@ int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mo
     L.part_acc = take(parts * bands * bins * 4 * sizeof(double));
     L.part_pre = take(parts * bands * 3 * sizeof(double));
     L.part_charge = take(parts * bands * sizeof(double));
+    L.path = take(parts * 3 * kMaxPathNodes * sizeof(double));
     long long entries = (long long)slots * work * n_pen;
     L.pieces = groups + (entries + kPieceEntries - 1) / kPieceEntries;
     L.piece_count = take((groups + 1) * sizeof(int));
@@ -1625,9 +1824,12 @@ int check_grouped(const lh_order1_pencils* pen, const lh_order1_oms* om,
     LH_REQUIRE(pen->count >= 1 && om->work_count >= 1, "no pencils or no OMs");
     LH_REQUIRE(om->work_count <= 65535, "at most 65535 OMs per grouped call");
     LH_REQUIRE(opt->bands >= 1 && opt->bands <= kMaxBands, "1..%d wavelength bands", kMaxBands);
-    LH_REQUIRE(opt->n_mono >= 1 && opt->n_mono <= kMaxMono && opt->max_power <= 4,
-               "acceptance degree must be at most 4");
+    LH_REQUIRE(opt->max_power >= 0 && opt->max_power <= 4
+                   && opt->n_mono == (opt->max_power + 1) * (opt->max_power + 2) / 2,
+               "acceptance degree must lie in 0..4 with (P+1)(P+2)/2 monomials");
     LH_REQUIRE(opt->bins >= 1 && opt->edges && opt->eedge && opt->itab, "bins and tables needed");
+    LH_REQUIRE(opt->direct_bins >= 0 && opt->direct_bins <= kMaxDirect,
+               "direct_bins must lie in 0..%d", kMaxDirect);
     LH_REQUIRE(opt->max_path_nodes >= 3 && opt->max_path_nodes <= kMaxPathNodes,
                "max_path_nodes must lie in 3..%d", kMaxPathNodes);
     LH_REQUIRE(2LL * om->work_count * pen->count < (1LL << 31) - 1,
@@ -1922,6 +2124,7 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.part_acc = reinterpret_cast<double*>(sc + SL.part_acc);
     G.part_pre = reinterpret_cast<double*>(sc + SL.part_pre);
     G.part_charge = reinterpret_cast<double*>(sc + SL.part_charge);
+    G.path = reinterpret_cast<double*>(sc + SL.path);
     G.segments = SL.segments;
     G.piece_start = reinterpret_cast<int*>(sc + SL.piece_start);
     G.piece_values = reinterpret_cast<double*>(sc + SL.piece_values);
@@ -1934,7 +2137,14 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
                                                      G.piece_start, groups + 1, s));
         {
             lh::KernelMark mark("o1_group_sums", s);
-            o1_group_sums<<<(unsigned)((SL.pieces + 3) / 4), kThreads, 0, s>>>(G, groups);
+            const unsigned grid_sums = (unsigned)((SL.pieces + 3) / 4);
+            switch (optics->max_power) {
+                case 0: o1_group_sums<0><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
+                case 1: o1_group_sums<1><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
+                case 2: o1_group_sums<2><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
+                case 3: o1_group_sums<3><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
+                default: o1_group_sums<4><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
+            }
         }
         if ((status = lh::check_launch("o1_group_sums")) != LH_OK) return status;
         o1_group_combine<<<blocks(groups), kThreads, 0, s>>>(G, groups);
@@ -1949,7 +2159,13 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     dim3 grid(work, SL.segments);
     {
         lh::KernelMark mark("o1_batches", s);
-        o1_batches<<<grid, 64, 0, s>>>(G);
+        switch (optics->max_power) {
+            case 0: o1_batches<0><<<grid, 64, 0, s>>>(G); break;
+            case 1: o1_batches<1><<<grid, 64, 0, s>>>(G); break;
+            case 2: o1_batches<2><<<grid, 64, 0, s>>>(G); break;
+            case 3: o1_batches<3><<<grid, 64, 0, s>>>(G); break;
+            default: o1_batches<4><<<grid, 64, 0, s>>>(G); break;
+        }
     }
     if ((status = lh::check_launch("o1_batches")) != LH_OK) return status;
     o1_finalize<<<work, kThreads, 0, s>>>(G);
