@@ -32,7 +32,7 @@ class AxisFrame:
         share = weights / weights.sum()
         centre = (share[:, None] * points).sum(axis=0)
         offset = points - centre
-        covariance = np.einsum("i,ij,ik->jk", share, offset, offset)
+        covariance = (offset * share[:, None]).T @ offset
         values, vectors = np.linalg.eigh(covariance)
         if np.max(np.abs(values)) <= 1e-24:
             axis = np.sum(share[:, None] * elements.direction, axis=0)

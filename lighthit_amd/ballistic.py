@@ -117,13 +117,30 @@ class DeviceOMs:
 class DeviceElements:
     """Spectral light elements of one source, on the device."""
 
-    def __init__(self, elements, device="cuda"):
-        import torch
+    @classmethod
+    def from_chords(cls, chords):
+        """The elements of a :class:`lighthit_amd.compile.DeviceChords` (shares its
+        device arrays)."""
+        elements = chords.host
+        cls._check(elements)
+        self = cls.__new__(cls)
+        self.count = len(elements)
+        self._keep = [chords.start_m, chords.direction, chords.length_m, chords.coefficient0,
+                      chords.coefficient2, chords.beta, chords.start_ns, chords.end_ns]
+        self.struct = _Elements(*[t.data_ptr() for t in self._keep], self.count)
+        return self
+
+    @staticmethod
+    def _check(elements):
         if elements.cone_model != "spectral":
             raise ValueError('continuous ballistic spectrum requires spectral LightElements')
         if not np.allclose(elements.coefficient2, -elements.coefficient0 / elements.beta ** 2,
                            rtol=1e-10, atol=1e-20):
             raise ValueError('source must have the physical Cherenkov C0/C2/beta relation')
+
+    def __init__(self, elements, device="cuda"):
+        import torch
+        self._check(elements)
         self.count = len(elements)
         names = ("start_m", "direction", "length_m", "coefficient0", "coefficient2", "beta",
                  "start_ns", "end_ns")

@@ -223,8 +223,16 @@ def compile_band(chords, frame, degree, omega, *, azimuthal_degree=4, options=No
         taken &= np.asarray(host.beta, float) * float(phase_index) > 1
     if not np.any(taken):
         return None
-    lattice = _lattice(host, taken, frame, degree, azimuthal_degree, omega, options,
-                       element_order)
+    # the lattice depends on the taken elements, the frame and the options only:
+    # bands with the same elements share it
+    key = (taken.tobytes(), id(frame), int(degree), int(azimuthal_degree),
+           bool(len(omega) == 1 and omega[0] == 0), options, str(element_order))
+    cache = chords.__dict__.setdefault("_lattices", {})
+    entry = cache.get(key)
+    if entry is None or entry[0] is not frame:
+        entry = cache[key] = (frame, _lattice(host, taken, frame, degree, azimuthal_degree,
+                                              omega, options, element_order))
+    lattice = entry[1]
     device = chords.device
     mask_dev = (None if mask is None else
                 torch.as_tensor(np.asarray(mask, np.uint8), device=device))

@@ -460,6 +460,7 @@ struct Group {
     double* prepared;         // (n_pen, 7)
     double* lambda_bins;      // (work, bands, bins)
     int* k_range;             // (work, 4) k1 min, k1 max, k0 min, k0 max
+    unsigned long long* x_range;  // (work, 4) ordered r min, r max, tau min, tau max
     unsigned long long* max_key;
     double* charge;           // (n_recv) output
     double* bins;             // (n_recv, bins) output
@@ -763,58 +764,95 @@ __global__ void __launch_bounds__(kThreads) o1_range_init(Group G, int work) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w == 0) G.max_key[0] = 0ULL;
     if (w >= work) return;
-    G.k_range[4 * w] = INT_MAX;
-    G.k_range[4 * w + 1] = INT_MIN;
-    G.k_range[4 * w + 2] = INT_MAX;
-    G.k_range[4 * w + 3] = INT_MIN;
+    G.x_range[4 * w] = ~0ULL;
+    G.x_range[4 * w + 1] = 0ULL;
+    G.x_range[4 * w + 2] = ~0ULL;
+    G.x_range[4 * w + 3] = 0ULL;
 }
 
-__device__ inline int wave_min_int(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+
+// Orderable 64-bit encoding of a double (unsigned order = numeric order).
+__device__ inline unsigned long long order_bits(double x) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+
+__device__ inline double from_order_bits(unsigned long long b) {
+    return __longlong_as_double((long long)((b >> 63) ? (b & 0x7fffffffffffffffULL) : ~b));
+}
+
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
 }
 
-__device__ inline int wave_max_int(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+__device__ inline double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
     return v;
 }
 
-// Radius and delay key ranges of every OM: min/max within each wave, then one
-// atomic per wave and bound.
+// Radius and delay extremes of every OM over its pencils: min/max within each
+// wave, then one atomic per wave and bound on the ordered encodings. The keys
+// floor(log r / step) and floor(tau / step) do not decrease with r and tau, so
+// their ranges follow from the extremes.
 __global__ void __launch_bounds__(kThreads) o1_group_ranges(Group G) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int w = blockIdx.y;
-    int lo1 = INT_MAX, hi1 = INT_MIN, lo0 = INT_MAX, hi0 = INT_MIN;
+    const double inf = __builtin_huge_val();
+    double r_lo = inf, r_hi = -inf, t_lo = inf, t_hi = -inf;
     if (p < G.n_pen) {
-        long long k0, k1;
-        radius_keys(G, p, G.om.work[w], &k0, &k1);
-        // one key of margin on each side: the key pass evaluates the same
-        // expressions in another function, where contraction may differ in the
-        // last bit
-        lo1 = (int)k1 - 1;
-        hi1 = (int)k1 + 1;
-        lo0 = (int)k0 - 1;
-        hi0 = (int)k0 + 1;
+        int d = G.om.work[w];
+        const double* pp = G.pen.position + 3 * p;
+        double dx = G.om.positions[3 * d] - pp[0];
+        double dy = G.om.positions[3 * d + 1] - pp[1];
+        double dz = G.om.positions[3 * d + 2] - pp[2];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        r_lo = r_hi = r;
+        if (!G.screen) {
+            double t_emit = G.pen.time[p] - G.om.origins[d];
+            t_lo = t_hi = t_emit + r / G.merge.speed_ref;
+        }
     }
-    lo1 = wave_min_int(lo1);
-    hi1 = wave_max_int(hi1);
-    lo0 = wave_min_int(lo0);
-    hi0 = wave_max_int(hi0);
-    if ((threadIdx.x & 63) == 0 && lo1 != INT_MAX) {
-        atomicMin(G.k_range + 4 * w, lo1);
-        atomicMax(G.k_range + 4 * w + 1, hi1);
-        atomicMin(G.k_range + 4 * w + 2, lo0);
-        atomicMax(G.k_range + 4 * w + 3, hi0);
+    r_lo = wave_min(r_lo);
+    r_hi = wave_max(r_hi);
+    t_lo = wave_min(t_lo);
+    t_hi = wave_max(t_hi);
+    if ((threadIdx.x & 63) == 0 && r_lo <= r_hi) {
+        unsigned long long* x = G.x_range + 4 * w;
+        atomicMin(x, order_bits(r_lo));
+        atomicMax(x + 1, order_bits(r_hi));
+        if (!G.screen) {
+            atomicMin(x + 2, order_bits(t_lo));
+            atomicMax(x + 3, order_bits(t_hi));
+        }
     }
 }
 
-// Size of the largest compact key space over the OMs.
+// Key ranges of every OM (one key of margin on each side: the key pass
+// evaluates the same expressions elsewhere, where contraction may differ in the
+// last bit) and the largest compact key space.
 __global__ void __launch_bounds__(kThreads) o1_key_space(Group G, int work) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= work) return;
-    const int* range = G.k_range + 4 * w;
-    unsigned long long n_k1 = (unsigned long long)((long long)range[1] - range[0] + 1);
-    unsigned long long n_k0 = (unsigned long long)((long long)range[3] - range[2] + 1);
+    const unsigned long long* x = G.x_range + 4 * w;
+    long long k0_lo = (long long)floor(log(from_order_bits(x[0])) / G.merge.log_r_step)
+                      + (1LL << 20);
+    long long k0_hi = (long long)floor(log(from_order_bits(x[1])) / G.merge.log_r_step)
+                      + (1LL << 20);
+    long long k1_lo = 0, k1_hi = 0;
+    if (!G.screen) {
+        k1_lo = (long long)floor(from_order_bits(x[2]) / G.merge.tau_step) + (1LL << 20);
+        k1_hi = (long long)floor(from_order_bits(x[3]) / G.merge.tau_step) + (1LL << 20);
+        k1_lo = min(max(k1_lo, 0LL), (1LL << 21) - 1);
+        k1_hi = min(max(k1_hi, 0LL), (1LL << 21) - 1);
+    }
+    int* range = G.k_range + 4 * w;
+    range[0] = (int)k1_lo - 1;
+    range[1] = (int)k1_hi + 1;
+    range[2] = (int)k0_lo - 1;
+    range[3] = (int)k0_hi + 1;
+    unsigned long long n_k1 = (unsigned long long)(k1_hi - k1_lo + 3);
+    unsigned long long n_k0 = (unsigned long long)(k0_hi - k0_lo + 3);
     atomicMax(G.max_key, n_k1 * n_k0 * 2ULL * (unsigned long long)G.theta_bins);
 }
 
@@ -902,10 +940,6 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-__device__ inline double wave_min(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-    return v;
-}
 
 constexpr int kPieceEntries = 64;    // group members per wave of the group sums
 
@@ -1542,7 +1576,28 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
     if (lane < nl) G.part_charge[slot * nl + lane] = charge_lane;
 }
 
-// Segments summed in order, then the per-wavelength prefix pass (_finalize :610-632).
+// Partial deposits of the segments summed in segment order into segment 0.
+__global__ void __launch_bounds__(kThreads) o1_sum_segments(Group G) {
+    int w = blockIdx.y;
+    int nl = G.opt.bands, nb = G.opt.bins, S = G.segments;
+    size_t per = (size_t)nl * nb * 4;
+    size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    double* acc0 = G.part_acc + (size_t)w * S * per;
+    if (i < per) {
+        double v = acc0[i];
+        for (int s = 1; s < S; ++s) v += acc0[(size_t)s * per + i];
+        acc0[i] = v;
+    }
+    if (i < (size_t)nl * 3) {
+        double* pre0 = G.part_pre + (size_t)w * S * nl * 3;
+        double v = pre0[i];
+        for (int s = 1; s < S; ++s) v += pre0[(size_t)s * nl * 3 + i];
+        pre0[i] = v;
+    }
+}
+
+// Charge, then the per-wavelength prefix pass over the summed deposits
+// (_finalize :610-632).
 __global__ void __launch_bounds__(kThreads) o1_finalize(Group G) {
     int w = blockIdx.x;
     int d = G.om.work[w];
@@ -1560,17 +1615,6 @@ __global__ void __launch_bounds__(kThreads) o1_finalize(Group G) {
         G.charge[d] = G.om.areas[d] * total;
     }
     if (!G.compute_bins) return;
-    for (size_t i = threadIdx.x; i < (size_t)nl * nb * 4; i += kThreads) {
-        double v = acc0[i];
-        for (int s = 1; s < S; ++s) v += G.part_acc[(base + s) * nl * nb * 4 + i];
-        acc0[i] = v;
-    }
-    for (int i = threadIdx.x; i < nl * 3; i += kThreads) {
-        double v = pre0[i];
-        for (int s = 1; s < S; ++s) v += G.part_pre[(base + s) * nl * 3 + i];
-        pre0[i] = v;
-    }
-    __syncthreads();
     double* lb = G.lambda_bins + (size_t)w * nl * nb;
     if (threadIdx.x < nl) {
         int lam = threadIdx.x;
@@ -1727,7 +1771,8 @@ unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads);
 
 struct GroupLayout {
     size_t keys, vals, sorted_keys, sorted_vals, group_flag, group_scan, group_start, group_end,
-        om_groups, prepared, lambda_bins, k_range, max_key, info, sort_temp, scan_temp, total;
+        om_groups, prepared, lambda_bins, k_range, x_range, max_key, info, sort_temp, scan_temp,
+        total;
 };
 
 // Entries per (pencil, OM): the screen's radial rule may give two nodes.
@@ -1752,6 +1797,7 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
     L.info = take(8 * sizeof(int));
     L.max_key = take(sizeof(unsigned long long));
     L.k_range = take((size_t)work * 4 * sizeof(int));
+    L.x_range = take((size_t)work * 4 * sizeof(unsigned long long));
     L.keys = take(entries * sizeof(long long));
     L.vals = take(entries * sizeof(int));
     L.sorted_keys = take(entries * sizeof(long long));
@@ -1865,6 +1911,7 @@ Group make_group(const lh_order1_pencils* pen, const lh_order1_oms* om,
     G.prepared = reinterpret_cast<double*>(base + L.prepared);
     G.lambda_bins = reinterpret_cast<double*>(base + L.lambda_bins);
     G.k_range = reinterpret_cast<int*>(base + L.k_range);
+    G.x_range = reinterpret_cast<unsigned long long*>(base + L.x_range);
     G.max_key = reinterpret_cast<unsigned long long*>(base + L.max_key);
     G.info = reinterpret_cast<int*>(base + L.info);
     G.mut_min = opt->mut_min;
@@ -2168,6 +2215,12 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
         }
     }
     if ((status = lh::check_launch("o1_batches")) != LH_OK) return status;
+    if (compute_bins) {
+        dim3 sum_grid((unsigned)((optics->bands * optics->bins * 4 + kThreads - 1) / kThreads),
+                      work);
+        o1_sum_segments<<<sum_grid, kThreads, 0, s>>>(G);
+        if ((status = lh::check_launch("o1_sum_segments")) != LH_OK) return status;
+    }
     o1_finalize<<<work, kThreads, 0, s>>>(G);
     return lh::check_launch("o1_finalize");
 }

@@ -343,9 +343,12 @@ class DeviceUnitChords:
         return s, int(offsets[-1])
 
     def grid(self, cell_m, pixel_face, ring_nodes, parent_face=0):
-        ends = self.start + self.length[:, None] * self.direction
-        low = np.minimum(self.start.min(axis=0), ends.min(axis=0)) - cell_m
-        high = np.maximum(self.start.max(axis=0), ends.max(axis=0)) + cell_m
+        if getattr(self, "_box", None) is None:
+            ends = self.start + self.length[:, None] * self.direction
+            self._box = (np.minimum(self.start.min(axis=0), ends.min(axis=0)),
+                         np.maximum(self.start.max(axis=0), ends.max(axis=0)))
+        low = self._box[0] - cell_m
+        high = self._box[1] + cell_m
         dims = np.ceil((high - low) / cell_m).astype(np.int64) + 1
         if float(np.prod(dims.astype(float))) * 6 * int(pixel_face) ** 2 >= 2.0 ** 62:
             raise ValueError("cell grid too large for 64-bit pencil keys; increase cell_m")

@@ -220,16 +220,21 @@ class HybridTransportResponse:
 
 # -------------------------------------------------------------- small helpers
 
-def segment_bounds(positions, elements, flight=None, device="cuda"):
+def segment_bounds(positions, elements, flight=None, device="cuda", chords=None):
     """Distance of every OM to the nearest segment and, with ``flight`` (n_g/c), the
-    earliest arrival (``lh_segment_bounds``)."""
+    earliest arrival (``lh_segment_bounds``). ``chords`` (a DeviceChords of these
+    elements) saves the upload."""
     import torch
 
     def up(a):
         return torch.as_tensor(np.ascontiguousarray(a, float), device=device)
     rec = up(positions)
-    starts, dirs = up(elements.start_m), up(elements.direction)
-    lengths, t0 = up(elements.length_m), up(elements.start_ns)
+    if chords is not None and chords.host is elements:
+        starts, dirs, lengths, t0 = (chords.start_m, chords.direction, chords.length_m,
+                                     chords.start_ns)
+    else:
+        starts, dirs = up(elements.start_m), up(elements.direction)
+        lengths, t0 = up(elements.length_m), up(elements.start_ns)
     n = len(positions)
     distance = torch.empty(n, dtype=torch.float64, device=device)
     earliest = torch.empty(n, dtype=torch.float64, device=device) if flight is not None else None
@@ -485,7 +490,8 @@ class PromptResult:
 
 
 def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra_charge=None,
-                     time_origin_ns=None, compute_bins_for_all=False, device="cuda"):
+                     time_origin_ns=None, compute_bins_for_all=False, device="cuda",
+                     chords=None):
     """Orders 0 and 1 of a spectral-cone element source (``prompt.py:500-947`` with
     the shower route's settings: first-entry order 0, grouped fused order 1)."""
     from .ballistic import DeviceElements, Order0
@@ -533,7 +539,9 @@ def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra
     stage = perf_counter()
     max_distance = (float(kernel.config.radial_range_m[1]) if settings.max_distance_m is None
                     else float(settings.max_distance_m))
-    distance, _ = segment_bounds(positions, elements, None, device)
+    if chords is None or chords.host is not elements:
+        chords = DeviceChords(elements, device)
+    distance, _ = segment_bounds(positions, elements, None, device, chords)
     inside = distance <= max_distance
     density = (elements.coefficient0[:, None] / tables["wavelength_nm"][None, :] ** 2
                + elements.coefficient2[:, None]
@@ -559,7 +567,7 @@ def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra
     stage = perf_counter()
     if order0 is None:
         order0 = Order0(kernel.medium, detector, alpha, device=device)
-    device_elements = DeviceElements(elements, device)
+    device_elements = DeviceElements.from_chords(chords)
     charge = np.zeros((n, 2))
     if len(work):
         q0 = order0.charge(device_elements)
@@ -567,7 +575,6 @@ def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra
     timings["order0_charge_seconds"] = perf_counter() - stage
     # order 1
     stage = perf_counter()
-    chords = DeviceChords(elements, device)
     oms = DeviceOMs(positions, looks, areas, origins, device)
     q1, bins1, level = spectral_order1(chords, oms, work, tables, alpha, g, edges, threshold,
                                        charge[:, 0], selection_extra_charge, settings)
@@ -814,6 +821,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
            float(config.delayed_cell_ns), float(edges[-1]))
     if compiled is not None and compiled.get("source") is source and compiled.get("key") == key:
         sources, late_zero, late_sources = compiled["sources"]
+        chords = compiled["chords"]
     else:
         chords = DeviceChords(elements, device)
         omega_device = torch.as_tensor(omega, device=device)
@@ -830,7 +838,8 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         late_sources = [(shift, compiled_part(mask, omega)) for mask, shift in late_groups]
         if compiled is not None:
             compiled.clear()
-            compiled.update(source=source, key=key, sources=(sources, late_zero, late_sources))
+            compiled.update(source=source, key=key, sources=(sources, late_zero, late_sources),
+                            chords=chords)
     timings["compile_seconds"] = perf_counter() - stage
 
     # radial windows of every node-band point set
@@ -886,7 +895,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
             raise ValueError("time_origin_ns must hold one finite origin per OM")
         local_origin = True
     elif local_origin:
-        _, origins = segment_bounds(positions, elements, fastest / C_M_PER_NS, device)
+        _, origins = segment_bounds(positions, elements, fastest / C_M_PER_NS, device, chords)
     else:
         first = int(np.argmin(elements.start_ns))
         origins = (float(elements.start_ns[first])
@@ -899,7 +908,8 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     prompt = transport_prompt(kernel, elements, config.prompt, order0=order0,
                               selection_extra_charge=safety * np.abs(predicted)
                               + np.abs(late_charge) + extra,
-                              time_origin_ns=origins if local_origin else None, device=device)
+                              time_origin_ns=origins if local_origin else None, device=device,
+                              chords=chords)
     timings["prompt_seconds"] = perf_counter() - stage
     if not np.allclose(prompt.time_origin_ns, origins, rtol=0, atol=1e-10):
         raise ValueError("prompt and RTE time origins differ")
@@ -966,7 +976,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
             redo = transport_prompt(kernel, elements, replace(config.prompt, screen=False),
                                     order0=order0, compute_bins_for_all=True,
                                     time_origin_ns=origins if local_origin else None,
-                                    device=device)
+                                    device=device, chords=chords)
             components[active, :, :2] = redo.bins_orders_pe[active]
         else:
             components[active, :, :2] = prompt.bins_orders_pe[active]
