@@ -401,7 +401,9 @@ def main():
                    "repeatable": same,
                    "stage_gpu_ms": stage_ms, "kernel_gpu_ms": kernels,
                    "host_stage_ms": {k: 1e3 * v for k, v in
-                                     response.metadata["timings"].items()}},
+                                     response.metadata["timings"].items()},
+                   "host_prompt_ms": {k: 1e3 * v for k, v in
+                                      response.metadata["prompt_timings"].items()}},
         "roofline": roofline, "cpu_baseline": baseline,
     }
     print(json.dumps(line), flush=True)

@@ -516,4 +516,9 @@ def spectral_order1(chords, oms, work, tables, alpha, g, edges, threshold, order
                     bins_dev[index] += part_bins[index]
                 level[extra_work] = 2
                 charge = charge_dev.cpu().numpy()
-    return charge, bins_dev.cpu().numpy(), level
+    # only the OMs of the fine pass carry bins
+    bins = np.zeros((n, nb))
+    rows = np.flatnonzero(level == 2)
+    if len(rows):
+        bins[rows] = bins_dev[torch.as_tensor(rows, device=device)].cpu().numpy()
+    return charge, bins, level
