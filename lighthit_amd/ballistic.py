@@ -158,22 +158,17 @@ class Order0:
         self.oms = DeviceOMs(detector, device)
         self.detector = detector
         self.device = device
-        self._workspace = None
-        self._bins_workspace = None
 
-    def workspace(self, n_elements):
-        import torch
+    def workspace(self, n_elements, stream=None):
         need = _workspace_bytes(self.oms.count, n_elements)
-        if self._workspace is None or self._workspace.numel() < need:
-            self._workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
-        return self._workspace
+        return _lib.workspace("order0_charge", self.device, need, stream)
 
     def charge_device(self, elements, out=None, stream=None):
         """Enqueue the charge of every OM; returns (charge tensor, workspace)."""
         import torch
         if out is None:
             out = torch.empty(self.oms.count, dtype=torch.float64, device=self.device)
-        ws = self.workspace(elements.count)
+        ws = self.workspace(elements.count, stream)
         with _lib.stage("order0_charge"):
             _lib.call("lh_order0_charge", ctypes.byref(self.optics.struct),
                       ctypes.byref(self.oms.struct), ctypes.byref(elements.struct),
@@ -189,9 +184,7 @@ class Order0:
         if out is None:
             out = torch.empty((oms.count, nbins), dtype=torch.float64, device=self.device)
         need = _workspace_bytes(oms.count, elements.count, nbins)
-        if self._bins_workspace is None or self._bins_workspace.numel() < need:
-            self._bins_workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
-        ws = self._bins_workspace
+        ws = _lib.workspace("order0_bins", self.device, need, stream)
         with _lib.stage("order0_bins"):
             _lib.call("lh_order0_bins", ctypes.byref(self.optics.struct),
                       ctypes.byref(oms.struct), ctypes.byref(elements.struct),

@@ -137,16 +137,8 @@ def _address(tensor):
     return None if tensor is None else ctypes.c_void_p(tensor.data_ptr())
 
 
-_workspaces = {}
-
-
-def _workspace(device, nbytes):
-    import torch
-    buf = _workspaces.get(str(device))
-    if buf is None or buf.numel() < nbytes:
-        buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
-        _workspaces[str(device)] = buf
-    return buf
+def _workspace(device, nbytes, stream=None):
+    return _lib.workspace("compile", device, nbytes, stream)
 
 
 def _lattice(chords_host, taken, frame, degree, azimuthal_degree, omega, options,
@@ -254,7 +246,7 @@ def compile_band(chords, frame, degree, omega, *, azimuthal_degree=4, options=No
     need = int(fn(ctypes.byref(lattice), chords.count))
     if need == 0:
         raise ValueError(lib.lh_last_error().decode(errors="replace"))
-    ws = _workspace(device, need)
+    ws = _workspace(device, need, stream)
     info = torch.zeros(8, dtype=torch.int32, device=device)
     points = torch.empty((5 * lattice.positions, 3), dtype=torch.float64, device=device)
     handle = _lib.stream_handle(stream)

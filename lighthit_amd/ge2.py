@@ -45,17 +45,10 @@ _lib.SIGNATURES["lh_ge2_apply"] = (ctypes.c_int, [ctypes.POINTER(_Pack), ctypes.
                                                   ctypes.c_void_p])
 _lib.SIGNATURES["lh_ge2_workspace_bytes"] = (ctypes.c_size_t, [ctypes.c_int] * 4)
 
-_workspaces = {}
-
-
-def _workspace(device, nbytes):
-    """A reusable device workspace of at least ``nbytes`` (one per device)."""
-    import torch
-    buf = _workspaces.get(str(device))
-    if nbytes and (buf is None or buf.numel() < nbytes):
-        buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
-        _workspaces[str(device)] = buf
-    return buf
+def _workspace(device, nbytes, stream=None):
+    """A reusable device workspace of at least ``nbytes`` (one per device and
+    stream), or None for none."""
+    return _lib.workspace("ge2", device, nbytes, stream) if nbytes else None
 
 
 _alphas = {}
@@ -249,7 +242,7 @@ def apply_device(pack, source, receivers, alpha, window, *, frequencies=None, ou
                           device=source.d_points.device)
     descriptor = pack.struct if band is None else pack.band(int(band))
     need = workspace_bytes(descriptor.bands, nfreq, receivers.count, source.points)
-    ws = _workspace(source.d_points.device, need) if need else None
+    ws = _workspace(source.d_points.device, need, stream) if need else None
     with _lib.stage("ge2_apply" if nfreq > 1 else "ge2_zero"):
         _lib.call("lh_ge2_apply", ctypes.byref(descriptor), ctypes.byref(source.struct),
                   ctypes.byref(receivers.struct), _address(d_alpha), nfreq, low, high,

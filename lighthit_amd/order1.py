@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -89,15 +90,22 @@ _TAYLOR = np.asarray([[1.0 / (math.factorial(n) * (n + j + 1.0)) for n in range(
 _tables_ready = set()
 
 
+_tables_lock = threading.Lock()
+
+
 def _ensure_tables(device):
     import torch
     key = str(torch.device(device))
     if key in _tables_ready:
         return
-    with torch.cuda.device(torch.device(device)):
-        _lib.call("lh_order1_set_tables", _TAYLOR.ctypes.data_as(_P),
-                  _TANGENT.ctypes.data_as(_P), _WEIGHTED.ctypes.data_as(_P))
-    _tables_ready.add(key)
+    with _tables_lock:
+        if key in _tables_ready:
+            return
+        with torch.cuda.device(torch.device(device)):
+            _lib.call("lh_order1_set_tables", _TAYLOR.ctypes.data_as(_P),
+                      _TANGENT.ctypes.data_as(_P), _WEIGHTED.ctypes.data_as(_P))
+            torch.cuda.synchronize(torch.device(device))
+        _tables_ready.add(key)
 
 
 def _size(name, *args):
@@ -114,17 +122,8 @@ def _addr(tensor):
     return None if tensor is None else ctypes.c_void_p(tensor.data_ptr())
 
 
-_workspaces = {}
-
-
-def _workspace(tag, device, nbytes):
-    import torch
-    key = (tag, str(device))
-    buf = _workspaces.get(key)
-    if buf is None or buf.numel() < nbytes:
-        buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
-        _workspaces[key] = buf
-    return buf
+def _workspace(tag, device, nbytes, stream=None):
+    return _lib.workspace("order1_" + tag, device, nbytes, stream)
 
 
 @dataclass(frozen=True)
@@ -368,7 +367,7 @@ def compress(unit, cell_m, pixel_face, ring_nodes, *, parent_face=0, stream=None
     grid = unit.grid(cell_m, pixel_face, ring_nodes, parent_face)
     grid.subsegments = subsegments
     need = _size("lh_order1_compress_workspace_bytes", ctypes.byref(chords), ctypes.byref(grid))
-    ws = _workspace("compress", device, need)
+    ws = _workspace("compress", device, need, stream)
     info = torch.zeros(8, dtype=torch.int32, device=device)
     handle = _lib.stream_handle(stream)
     with _lib.stage("order1_compress"):
@@ -401,7 +400,7 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
     m = _Merge(*[float(v) for v in merge], optics.speed_ref)
     need = _size("lh_order1_grouped_workspace_bytes", ctypes.byref(pen), ctypes.byref(om),
                  ctypes.byref(optics.struct))
-    ws = _workspace("grouped", device, need)
+    ws = _workspace("grouped", device, need, stream)
     info = torch.zeros(8, dtype=torch.int32, device=device)
     handle = _lib.stream_handle(stream)
     name = "order1_screen" if pencils.parent is not None else "order1_grouped"
@@ -412,7 +411,7 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
     groups, key_bits = (int(v) for v in info[:2].tolist())
     scratch_bytes = _size("lh_order1_group_scratch_bytes", ctypes.byref(pen), ctypes.byref(om),
                           ctypes.byref(optics.struct), groups)
-    scratch = _workspace("group_scratch", device, scratch_bytes)
+    scratch = _workspace("group_scratch", device, scratch_bytes, stream)
     with _lib.stage(name + "_charge"):
         _lib.call("lh_order1_group_charge", ctypes.byref(pen), ctypes.byref(om),
                   ctypes.byref(optics.struct), ctypes.byref(m), groups, key_bits, _addr(scratch),

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes
 import json
+import threading
 import warnings
 from dataclasses import asdict, dataclass, field, replace
 from pathlib import Path
@@ -654,13 +655,25 @@ class ShowerTransport:
         self.threads = threads
         self.device = device
         self._tables = None
-        self._compiled = {}
+        self._local = threading.local()   # per-thread cache of the last compile
+        self._lock = threading.RLock()     # guards the lazily built resident parts
         self._order0 = None
         self._readout = {}
         self._window = None
 
+    @property
+    def _compiled(self):
+        cache = getattr(self._local, "compiled", None)
+        if cache is None:
+            cache = self._local.compiled = {}
+        return cache
+
     def tables(self):
         """The pack folded onto the node bands, resident on the device (once)."""
+        with self._lock:
+            return self._tables_locked()
+
+    def _tables_locked(self):
         if self._tables is None:
             from .fold import BandFold, KernelPack, fold_bundle, frequency_indices, load_folded
             from .ge2 import DevicePack
@@ -699,25 +712,29 @@ class ShowerTransport:
         return self._tables
 
     def order0(self):
-        if self._order0 is None:
-            from .ballistic import Order0
-            self._order0 = Order0(self.kernel.medium, self.kernel.detector,
-                                  self.kernel.acceptance()["alpha"], device=self.device)
-        return self._order0
+        with self._lock:
+            if self._order0 is None:
+                from .ballistic import Order0
+                self._order0 = Order0(self.kernel.medium, self.kernel.detector,
+                                      self.kernel.acceptance()["alpha"], device=self.device)
+            return self._order0
 
     def window_geometry(self, kernel):
         """Device geometry of the radial-window report (once per detector)."""
-        if self._window is None or self._window[0] is not kernel.detector:
-            self._window = (kernel.detector, WindowGeometry(kernel, self.device))
-        return self._window[1]
+        with self._lock:
+            if self._window is None or self._window[0] is not kernel.detector:
+                self._window = (kernel.detector, WindowGeometry(kernel, self.device))
+            return self._window[1]
 
     def readout(self, omega, edges):
         from .readout import ReadoutBasis
         key = (tuple(np.round(edges, 12)), len(omega))
-        if key not in self._readout:
-            self._readout[key] = ReadoutBasis(omega, edges, self.kernel.config.readout_omega_per_ns,
-                                              device=self.device)
-        return self._readout[key]
+        with self._lock:
+            if key not in self._readout:
+                self._readout[key] = ReadoutBasis(omega, edges,
+                                                  self.kernel.config.readout_omega_per_ns,
+                                                  device=self.device)
+            return self._readout[key]
 
     def __call__(self, source, *, progress=None, time_origin_ns=None, selection_extra_charge=None,
                  receivers=None):
@@ -752,6 +769,21 @@ class ShowerTransport:
             response.metadata["detail_threshold_pe"] = float(detail)
             response.metadata["active_modules"] = int(np.count_nonzero(response.active))
         return response
+
+
+class ResidentShower:
+    """A shower source whose element arrays are already on the device.
+
+    ``ResidentShower(source)`` uploads the elements once; handing it to
+    :class:`ShowerTransport` skips that copy in the call (the route otherwise
+    uploads the host arrays of every source it is given). It carries the
+    host elements too, for the host-side steps of the route.
+    """
+
+    def __init__(self, source, device="cuda"):
+        from .compile import DeviceChords
+        self.elements = _elements_of(source)
+        self.chords = DeviceChords(self.elements, device)
 
 
 def _elements_of(source):
@@ -819,11 +851,13 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     key = (fold.bands, basis, tuple(np.round(omega, 12)), int(kernel.config.source_degree),
            int(kernel.config.azimuthal_degree), float(config.prompt_delay_ns),
            float(config.delayed_cell_ns), float(edges[-1]))
+    resident = getattr(source, "chords", None)
     if compiled is not None and compiled.get("source") is source and compiled.get("key") == key:
         sources, late_zero, late_sources = compiled["sources"]
         chords = compiled["chords"]
     else:
-        chords = DeviceChords(elements, device)
+        chords = (resident if resident is not None and resident.host is elements
+                  else DeviceChords(elements, device))
         omega_device = torch.as_tensor(omega, device=device)
 
         def compiled_part(mask, frequencies):

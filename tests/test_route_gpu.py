@@ -148,3 +148,26 @@ def test_window_report_every_branch(device):
         np.testing.assert_allclose(got[3], ref[3], rtol=1e-6, atol=1e-300)
         if count == 700:
             assert ref[1].min() > 0 and ref[0].min() > 0
+
+
+def test_served_results_match_alone(demonstration, device):
+    """Concurrent calls on several streams give every event the result it has alone."""
+    from lighthit_amd.route import ResidentShower
+    from dataclasses import replace
+
+    from lighthit_amd.serve import ShowerServer
+    w, route, response, want = demonstration
+    # three different events: the demonstration shower and two shifted copies
+    events = [w.elements]
+    for shift in ([4.0, -3.0, 2.0], [-6.0, 5.0, -1.0]):
+        events.append(replace(w.elements, start_m=w.elements.start_m + np.asarray(shift)))
+    alone = [route(ResidentShower(e, device)) for e in events]
+    with ShowerServer(route, workers=3, device=device) as server:
+        futures = [server.submit(ResidentShower(events[i % 3], device)) for i in range(9)]
+        mixed = [f.result() for f in futures]
+    for i, r in enumerate(mixed):
+        a = alone[i % 3]
+        assert np.array_equal(r.charge_components_pe, a.charge_components_pe)
+        assert np.array_equal(r.components_pe, a.components_pe)
+        assert np.array_equal(r.active, a.active)
+    assert all(f.latency_s > 0 for f in futures)
