@@ -38,6 +38,19 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+# FP64 operations per unit of the order-1 batch kernel (each add, multiply,
+# divide, square root or exponential counted once):
+#   profile term (one path node x one group): scattering geometry 16, the two
+#     cubic acceptance polynomials 28, Henyey-Greenstein weight 8, sums 4;
+#   time panel: quadratic coefficients 30, two attenuation factors 4, the
+#     exponential-quadratic integral 154 (three 24-term Taylor series and the
+#     combination);
+#   bin event: one more exponential-quadratic integral over a bin piece 160.
+O1_PROFILE_TERM_FLOPS = 56
+O1_PANEL_FLOPS = 188
+O1_EVENT_FLOPS = 160
 
 #: Kernels the library times with HIP events on their stream.
 TIMED_KERNELS = ("order0_pairs<false>", "order0_pairs<true>", "o1_batches", "o1_group_sums",
@@ -232,7 +245,8 @@ def measure_traffic(workload_flag, kernel_name):
     runs = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         for kind, args in (("bench", ["--steps", "1", "--warmup", "1", "--no-cpu-baseline",
-                                      "--no-traffic", "--workload", workload_flag]),
+                                      "--no-traffic", "--workers", "1", "--workload",
+                                      workload_flag]),
                            ("copy", ["--copy-kernel-mib", "256"])):
             out = os.path.join(base, f"{kind}_{counter}")
             cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--",
@@ -259,7 +273,7 @@ def measure_traffic(workload_flag, kernel_name):
 
 # ------------------------------------------------------------------- driver
 
-def roofline_of(name, kernel_ms, workload, response, route):
+def roofline_of(name, kernel_ms, workload, response, route, counts=None, launches=1):
     """Roofline entry of kernel ``name`` at its average launch duration."""
     d = workload.detector
     n_el = len(workload.elements)
@@ -272,6 +286,17 @@ def roofline_of(name, kernel_ms, workload, response, route):
                 "frac": achieved / HBM_PEAK_GBS,
                 "algorithmic": f"48 B/element x {n_el} + 36 B/OM x {oms} + 8 B x {max(1, bins)} "
                                f"outputs/OM"}
+    if name == "o1_batches" and counts is not None:
+        terms, panels, events = (c / launches for c in counts)
+        flops = (O1_PROFILE_TERM_FLOPS * terms + O1_PANEL_FLOPS * panels
+                 + O1_EVENT_FLOPS * events)
+        achieved = flops / (kernel_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "algorithmic": f"per launch {terms:.4g} profile terms x {O1_PROFILE_TERM_FLOPS} "
+                               f"+ {panels:.4g} panels x {O1_PANEL_FLOPS} + {events:.4g} bin "
+                               f"events x {O1_EVENT_FLOPS} FP64 flops (counted in the kernel)",
+                "note": "FP64 vector work (no matrix form); priced against the FP64 peak"}
     if name == "ge2_wide":
         from lighthit_amd.ge2 import canonical_pairs
         level2 = np.asarray(response.metadata["ge2_charge_level"])
@@ -344,6 +369,8 @@ def main():
         same = True
         print(f"[bench] timing {args.steps} events (plain, one at a time)", file=sys.stderr,
               flush=True)
+        from lighthit_amd import order1 as order1_module
+        order1_module.count_work("cuda")
         with StageClock() as clock, KernelTimer() as timer:
             torch.cuda.synchronize()
             began = time.perf_counter()
@@ -366,6 +393,8 @@ def main():
                                      "launches_per_event": launches / args.steps,
                                      "ms_per_event": total / args.steps}
             timer.reset()
+        o1_counts = order1_module.work_counts("cuda")
+        order1_module.count_work("cuda", on=False)
         del resident
         # the same calls with the host arrays handed over (PCIe upload inside)
         host_began = time.perf_counter()
@@ -400,7 +429,8 @@ def main():
     roofline = None
     if dominant is not None:
         roofline = roofline_of(dominant, kernels[dominant]["ms_per_launch"], workload, response,
-                               route)
+                               route, counts=o1_counts,
+                               launches=kernels[dominant]["launches_per_event"] * args.steps)
         if roofline is None:
             roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": None, "algorithmic": "no cost model for this kernel"}

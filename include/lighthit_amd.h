@@ -277,7 +277,9 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
  * and the bits of its compact per-OM keys to info[1] (it returns after the
  * stream has drained); lh_order1_group_charge takes both, a scratch buffer of
  * lh_order1_group_scratch_bytes(pencils, oms, optics, groups) and writes
- * charge (and bins) of the listed OMs. Each pair shares its
+ * charge (and bins) of the listed OMs. Its optional work_counts (3 device
+ * counters, added to) receive the number of profile terms (path nodes x
+ * groups), time panels and bin events it evaluated. Each pair shares its
  * workspace. lh_order1_set_tables uploads the 24-term Taylor table of
  * I_k(z) (3 x 24) and the 48-node inverse-angle rule (tangent, weight sec^2).
  */
@@ -341,6 +343,9 @@ typedef struct lh_order1_optics {
     const int* jpow;
     int bands, bins, n_mono, max_power, direct_bins, max_path_nodes;
     double g, y_g, path_step, path_step_tail, tail_exponent, smoothing, mut_min, speed_max;
+    /* equal bins: 1 / width (0 when the edges are not equally spaced); bin
+     * lookups then start at floor((t - edges[0]) / width) */
+    double bin_inverse_width;
 } lh_order1_optics;
 
 typedef struct lh_order1_merge {
@@ -367,7 +372,8 @@ size_t lh_order1_group_scratch_bytes(const lh_order1_pencils* pencils, const lh_
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
                            int groups, int key_bits, void* scratch, size_t scratch_bytes,
-                           int compute_bins, double* charge, double* bins, void* workspace,
+                           int compute_bins, double* charge, double* bins,
+                           unsigned long long* work_counts, void* workspace,
                            size_t workspace_bytes, void* stream);
 
 /*

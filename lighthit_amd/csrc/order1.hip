@@ -456,7 +456,7 @@ struct Group {
     double* part_pre;         // (work, segments, bands, 3)
     double* part_charge;      // (work, segments, bands)
     int segments;             // batch segments per OM (one wave each)
-    double* path;             // (work, segments, 3, kMaxPathNodes) path nodes, g0, g2
+    unsigned long long* work_counts;  // optional: profile terms, panels, bin events
     double* prepared;         // (n_pen, 7)
     double* lambda_bins;      // (work, bands, bins)
     int* k_range;             // (work, 4) k1 min, k1 max, k0 min, k0 max
@@ -1064,9 +1064,20 @@ __device__ double quad_integral(double e_start, double alpha, double beta, doubl
     return e_start * width * (alpha * i0 + width * (beta * i1 + width * gamma * i2));
 }
 
-__device__ int find_bin(const double* edges, int nb, double t) {
+__device__ int find_bin(const double* edges, int nb, double t, double inverse_width = 0.0) {
     if (t < edges[0]) return -1;
     if (t >= edges[nb]) return nb;
+    if (inverse_width > 0.0) {
+        // equal bins: start at the arithmetic guess and step to the edges' bin
+        // (the same bin as the search below: edges[k] <= t < edges[k + 1])
+        int k = (int)((t - edges[0]) * inverse_width);
+        k = min(max(k, 0), nb - 1);
+        for (int step = 0; step < 4; ++step) {
+            if (t < edges[k]) --k;
+            else if (t >= edges[k + 1]) ++k;
+            else return k;
+        }
+    }
     int low = 0, high = nb;
     while (high - low > 1) {
         int middle = (low + high) / 2;
@@ -1076,61 +1087,13 @@ __device__ int find_bin(const double* edges, int nb, double t) {
 }
 
 struct LambdaBins {
+    double inverse_width;  // equal bins: 1 / width, else 0
     const double* edges;
     const double* eedge;   // (nb + 1) of this wavelength
     double* acc;           // (nb, 4)
     double* pre;           // (3)
     int nb, direct_bins;
 };
-
-// _deposit_panel (_prompt_numba.py:186-253).
-__device__ void deposit_panel(const LambdaBins& B, double t_a, double t_b, double e_a, double e_b,
-                              double alpha, double beta, double gamma, double kappa) {
-    int nb = B.nb;
-    const double* edges = B.edges;
-    double lo = t_a > edges[0] ? t_a : edges[0];
-    double hi = t_b < edges[nb] ? t_b : edges[nb];
-    if (hi <= lo) return;
-    int k_lo = find_bin(edges, nb, lo);
-    int k_hi = find_bin(edges, nb, hi);
-    if (k_hi >= nb || (k_hi > k_lo && hi <= edges[k_hi])) k_hi -= 1;
-    if (k_hi - k_lo + 1 <= B.direct_bins) {
-        for (int k = k_lo; k <= k_hi; ++k) {
-            double s_lo = lo > edges[k] ? lo : edges[k];
-            double s_hi = hi < edges[k + 1] ? hi : edges[k + 1];
-            if (s_hi <= s_lo) continue;
-            double e_lo = s_lo == t_a ? e_a : B.eedge[k];
-            if (s_lo != t_a && s_lo != edges[k]) e_lo = exp(kappa * s_lo);
-            double e_hi = s_hi == t_b ? e_b : B.eedge[k + 1];
-            if (s_hi != t_b && s_hi != edges[k + 1]) e_hi = exp(kappa * s_hi);
-            double shift = s_lo - t_a;
-            double a_loc = alpha + shift * (beta + shift * gamma);
-            double b_loc = beta + 2.0 * gamma * shift;
-            B.acc[4 * k + 3] += quad_integral(e_lo, a_loc, b_loc, gamma, s_hi - s_lo, kappa, e_hi);
-        }
-        return;
-    }
-    double big_a = alpha - t_a * (beta - gamma * t_a);
-    double big_b = beta - 2.0 * gamma * t_a;
-    double big_c = gamma;
-    if (t_a < edges[0]) {
-        B.pre[0] += big_a; B.pre[1] += big_b; B.pre[2] += big_c;
-    } else {
-        int k = k_lo;
-        B.acc[4 * k] += big_a; B.acc[4 * k + 1] += big_b; B.acc[4 * k + 2] += big_c;
-        B.acc[4 * k + 3] += quad_integral(e_a, alpha, beta, gamma, edges[k + 1] - t_a, kappa,
-                                          B.eedge[k + 1]);
-    }
-    if (t_b < edges[nb]) {
-        int k = find_bin(edges, nb, t_b);
-        B.acc[4 * k] -= big_a; B.acc[4 * k + 1] -= big_b; B.acc[4 * k + 2] -= big_c;
-        double shift = t_b - t_a;
-        double a_loc = alpha + shift * (beta + shift * gamma);
-        double b_loc = beta + 2.0 * gamma * shift;
-        B.acc[4 * k + 3] -= quad_integral(e_b, a_loc, b_loc, gamma, edges[k + 1] - t_b, kappa,
-                                          B.eedge[k + 1]);
-    }
-}
 
 __device__ void cubic_coefficients(const Group& G, const double* mom, double* a, double* b) {
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0;
@@ -1152,16 +1115,20 @@ __device__ void cubic_coefficients(const Group& G, const double* mom, double* a,
 }
 
 constexpr int kBatchGroups = 64;   // groups staged in LDS per profile pass
-constexpr int kMaxDirect = 8;      // direct_bins limit of the panel events
+constexpr int kMaxDirect = 4;      // direct_bins limit of the panel events
+static_assert(kMaxDirect >= 4, "prefix panels keep A, B, C and I_start in v[0..3]");
+constexpr int kBatchWaves = 8;     // waves of a batch workgroup (one wavelength each)
+constexpr int kBatchThreads = 64 * kBatchWaves;
 
 // Deposits of one panel (_deposit_panel, _prompt_numba.py:186-253) as events:
 // kind 0 none; kind 1 direct, v[e] added to bin k_lo + e for the set bits of
-// mask; kind 2 prefix, (A, B, C, I_start) added at k_lo (or A, B, C to the
-// pre-window sums when pre is set) and (A, B, C, I_end) subtracted at k_end
-// when end is set.
+// mask; kind 2 prefix, (A, B, C, I_start) = v[0..3] added at k_lo (or A, B, C
+// to the pre-window sums when pre is set) and (A, B, C, I_end = v_end)
+// subtracted at k_end when end is set.
 struct PanelEvents {
     int kind, k_lo, k_end, n, mask, pre, end;
-    double v[kMaxDirect];
+    double v[kMaxDirect];   // direct: one integral per bin; prefix: A, B, C, I_start
+    double v_end;           // prefix: I_end
 };
 
 __device__ void panel_events(const LambdaBins& B, double t_a, double t_b, double e_a, double e_b,
@@ -1173,8 +1140,8 @@ __device__ void panel_events(const LambdaBins& B, double t_a, double t_b, double
     double lo = t_a > edges[0] ? t_a : edges[0];
     double hi = t_b < edges[nb] ? t_b : edges[nb];
     if (hi <= lo) return;
-    int k_lo = find_bin(edges, nb, lo);
-    int k_hi = find_bin(edges, nb, hi);
+    int k_lo = find_bin(edges, nb, lo, B.inverse_width);
+    int k_hi = find_bin(edges, nb, hi, B.inverse_width);
     if (k_hi >= nb || (k_hi > k_lo && hi <= edges[k_hi])) k_hi -= 1;
     out.k_lo = k_lo;
     if (k_hi - k_lo + 1 <= B.direct_bins) {
@@ -1213,45 +1180,129 @@ __device__ void panel_events(const LambdaBins& B, double t_a, double t_b, double
     out.end = t_b < edges[nb];
     out.k_end = k_hi;
     if (out.end) {
-        int k = find_bin(edges, nb, t_b);
+        int k = find_bin(edges, nb, t_b, B.inverse_width);
         out.k_end = k;
         double shift = t_b - t_a;
         double a_loc = alpha + shift * (beta + shift * gamma);
         double b_loc = beta + 2.0 * gamma * shift;
-        out.v[4] = quad_integral(e_b, a_loc, b_loc, gamma, edges[k + 1] - t_b, kappa,
-                                 B.eedge[k + 1]);
+        out.v_end = quad_integral(e_b, a_loc, b_loc, gamma, edges[k + 1] - t_b, kappa,
+                                  B.eedge[k + 1]);
     }
 }
 
-// One wave per (OM, segment): the segment's batches b = first + s, first + s + S, ...
-// (_grouped_pencils_order1_block :1924-1995 and _batch_deposit :451-551) into the
-// segment's own partial accumulators. The path nodes and profiles live in the
-// slot's global scratch; the panels of one wavelength are spread over the lanes
-// 64 at a time, and every bin then adds its panels' events in panel order (the
-// order of the reference's serial loop, so the bins match it term by term).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One round of deposits: every lane holds the events of one panel (panels in
+// time order across the lanes). The events are flattened in (panel, bin) order,
+// along which the bins never decrease, so every bin's events are contiguous and
+// in panel order; one lane per bin adds them, the order of the reference's
+// serial loop. Returns the number of bin events.
+__device__ int deposit_round(const PanelEvents& ev, int lane, const LambdaBins& B, int* ev_bin,
+                             double (*ev_val)[4], int* head_at) {
+    int mine = 0;
+    if (ev.kind == 1) mine = __popc(ev.mask);
+    else if (ev.kind == 2) mine = (ev.pre ? 0 : 1) + (ev.end ? 1 : 0);
+    int offset = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        int other = __shfl_up(offset, off, 64);
+        if (lane >= off) offset += other;
+    }
+    int n_events = __shfl(offset, 63, 64);
+    offset -= mine;
+    if (ev.kind == 1) {
+#pragma unroll
+        for (int e = 0; e < kMaxDirect; ++e) {
+            if (!((ev.mask >> e) & 1)) continue;
+            ev_bin[offset] = ev.k_lo + e;
+            ev_val[offset][0] = 0.0; ev_val[offset][1] = 0.0;
+            ev_val[offset][2] = 0.0; ev_val[offset][3] = ev.v[e];
+            ++offset;
+        }
+    } else if (ev.kind == 2) {
+        if (!ev.pre) {
+            ev_bin[offset] = ev.k_lo;
+            ev_val[offset][0] = ev.v[0]; ev_val[offset][1] = ev.v[1];
+            ev_val[offset][2] = ev.v[2]; ev_val[offset][3] = ev.v[3];
+            ++offset;
+        }
+        if (ev.end) {
+            ev_bin[offset] = ev.k_end;
+            ev_val[offset][0] = -ev.v[0]; ev_val[offset][1] = -ev.v[1];
+            ev_val[offset][2] = -ev.v[2]; ev_val[offset][3] = -ev.v_end;
+        }
+    }
+    wave_sync();
+    // the first event of every bin, compacted in order
+    int heads = 0;
+    for (int c = 0; c < n_events; c += 64) {
+        int i = c + lane;
+        bool head = i < n_events && (i == 0 || ev_bin[i] != ev_bin[i - 1]);
+        unsigned long long ballot = __ballot(head);
+        if (head) head_at[heads + __popcll(ballot & ((1ULL << lane) - 1))] = i;
+        heads += __popcll(ballot);
+    }
+    wave_sync();
+    for (int h = lane; h < heads; h += 64) {
+        int i0 = head_at[h], i1 = h + 1 < heads ? head_at[h + 1] : n_events;
+        double* a = B.acc + 4 * (size_t)ev_bin[i0];
+        double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+        for (int i = i0; i < i1; ++i) {
+            a0 += ev_val[i][0]; a1 += ev_val[i][1];
+            a2 += ev_val[i][2]; a3 += ev_val[i][3];
+        }
+        a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+    }
+    // window-front sums of the panels that start before the first edge, in
+    // panel order (rare: a ballot finds them)
+    unsigned long long pre = __ballot(ev.kind == 2 && ev.pre);
+    for (unsigned long long rest = pre; rest != 0; rest &= rest - 1) {
+        int q = __ffsll((long long)rest) - 1;
+        double a = __shfl(ev.v[0], q, 64), b = __shfl(ev.v[1], q, 64),
+               c = __shfl(ev.v[2], q, 64);
+        if (lane == 0) { B.pre[0] += a; B.pre[1] += b; B.pre[2] += c; }
+    }
+    wave_sync();
+    return n_events;
+}
+
+// One workgroup of 8 waves per (OM, segment): the segment's batches b = first + s,
+// first + s + S, ... (_grouped_pencils_order1_block :1924-1995 and _batch_deposit
+// :451-551) into the segment's own partial accumulators. The workgroup builds a
+// batch's path nodes and profiles in LDS; then every wave deposits one
+// wavelength: its panels are spread over the lanes 64 at a time, and every bin
+// adds its panels' events in panel order (the order of the reference's serial
+// loop, so the bins match it term by term).
 template <int P>
-__global__ void __launch_bounds__(64) o1_batches(Group G) {
+__global__ void __launch_bounds__(kBatchThreads) o1_batches(Group G) {
     __shared__ double coeff[kBatchGroups][16];   // A0[4] B0[3] A2[4] B2[3] sine cosine
-    __shared__ int ev_bin[64 * kMaxDirect];
-    __shared__ double ev_val[64 * kMaxDirect][4];
-    __shared__ int head_at[64 * kMaxDirect];
-    __shared__ int pre_flag[64];
-    __shared__ double pre_val[64][3];
+    __shared__ double path[kMaxPathNodes];
+    __shared__ double g0[kMaxPathNodes];
+    __shared__ double g2[kMaxPathNodes];
+    __shared__ int ev_bin_all[kBatchWaves][64 * kMaxDirect];
+    __shared__ double ev_val_all[kBatchWaves][64 * kMaxDirect][4];
+    __shared__ int head_at_all[kBatchWaves][64 * kMaxDirect];
+    __shared__ double charge_acc[kMaxBands];
     int w = blockIdx.x, seg = blockIdx.y;
-    int lane = threadIdx.x;
+    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int* ev_bin = ev_bin_all[wave];
+    double (*ev_val)[4] = ev_val_all[wave];
+    int* head_at = head_at_all[wave];
     int nl = G.opt.bands, nb = G.opt.bins;
     int n_mono = G.opt.n_mono;
     int nv = 2 * n_mono + 6;
     size_t slot = (size_t)w * G.segments + seg;
     double* acc = G.part_acc + slot * nl * nb * 4;
     double* pre = G.part_pre + slot * nl * 3;
-    double* path = G.path + slot * 3 * kMaxPathNodes;
-    double* g0 = path + kMaxPathNodes;
-    double* g2 = path + 2 * kMaxPathNodes;
     if (G.compute_bins)
-        for (size_t i = lane; i < (size_t)nl * nb * 4; i += 64) acc[i] = 0.0;
-    for (int i = lane; i < nl * 3; i += 64) pre[i] = 0.0;
-    double charge_lane = 0.0;
+        for (size_t i = tid; i < (size_t)nl * nb * 4; i += kBatchThreads) acc[i] = 0.0;
+    for (int i = tid; i < nl * 3; i += kBatchThreads) pre[i] = 0.0;
+    if (tid < nl) charge_acc[tid] = 0.0;
+    __syncthreads();
+    unsigned long long count_terms = 0, count_panels = 0, count_events = 0;
     constexpr bool cubic = P <= 3;
     double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
     double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
@@ -1310,7 +1361,9 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             n_out = capacity - 1 - n_in;
         }
         int n = n_in + n_out + 1;
-        for (int k = lane; k < n; k += 64) {
+        count_terms += (unsigned long long)n * (unsigned long long)(stop - start);
+        __syncthreads();   // the previous batch's deposits have read the path
+        for (int k = tid; k < n; k += kBatchThreads) {
             double sv;
             if (k == 0) sv = r_b;
             else if (k <= n_in) sv = r_b + sigma * sinh(u_win * k / n_in);
@@ -1323,7 +1376,7 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
         for (int c0 = start; c0 < stop; c0 += kBatchGroups) {
             int chunk = min(kBatchGroups, stop - c0);
             __syncthreads();
-            for (int q = lane; q < chunk; q += 64) {
+            for (int q = tid; q < chunk; q += kBatchThreads) {
                 const double* row = G.values + (size_t)(c0 + q) * nv;
                 double omega = row[2 * n_mono];
                 double* cf = coeff[q];
@@ -1337,7 +1390,7 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
                 }
             }
             __syncthreads();
-            for (int k = lane; k < n; k += 64) {
+            for (int k = tid; k < n; k += kBatchThreads) {
                 double acc0 = g0[k], acc2 = g2[k];
                 double s_k = path[k];
                 for (int q = 0; q < chunk; ++q) {
@@ -1399,15 +1452,22 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             }
         }
         __syncthreads();
-        // _batch_deposit (:451-551), one wavelength at a time
-        for (int lam = 0; lam < nl; ++lam) {
+        // _batch_deposit (:451-551): one wavelength per wave
+        for (int lam = wave; lam < nl; lam += kBatchWaves) {
             if (G.opt.mus[lam] == 0.0) continue;
             double v = G.opt.speed[lam];
             double mut = G.opt.mut[lam];
             double kappa = -mut * v;
-            int last = n - 1;
+            // last = the serial walk-back `while (last > 2 && path[last - 1] >
+            // limit) --last` from n - 1; path increases, so it is the count of
+            // path[0 .. n - 2] within the limit, clamped to [2, n - 1]
             double limit = r_b + G.opt.tail_exponent / mut;
-            while (last > 2 && path[last - 1] > limit) --last;
+            int within = 0;
+            for (int k0 = 0; k0 < n - 1; k0 += 64) {
+                int k = k0 + lane;
+                within += __popcll(__ballot(k < n - 1 && path[k] <= limit));
+            }
+            int last = min(max(within, 2), n - 1);
             if (last < 2) continue;
             double spread = fmin(sqrt(12.0 * tau_var), (path[last] - r_b) / v);
             double front_time = t_b + r_b / v;
@@ -1419,6 +1479,7 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             double h_scale = G.opt.mus[lam] * v / e_emit;
             double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
             LambdaBins B;
+            B.inverse_width = G.opt.bin_inverse_width;
             B.edges = G.opt.edges;
             B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
             B.acc = acc + (size_t)lam * nb * 4;
@@ -1433,6 +1494,7 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
             // the one-step panel at j = last - 1
             int panels = last / 2 + (last & 1);
             double total_lane = 0.0;
+            unsigned long long events_total = 0;
             for (int base = 0; base < panels; base += 64) {
                 int i = base + lane;
                 PanelEvents ev;
@@ -1478,82 +1540,18 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
                     if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
                 }
                 if (!bins) continue;
-                // flatten the panels' bin events in (panel, bin) order: bins never
-                // decrease along it, so every bin's events are contiguous and in
-                // panel order
-                int mine = 0;
-                if (ev.kind == 1) mine = __popc(ev.mask);
-                else if (ev.kind == 2) mine = (ev.pre ? 0 : 1) + (ev.end ? 1 : 0);
-                int offset = mine;
-                for (int off = 1; off < 64; off <<= 1) {
-                    int other = __shfl_up(offset, off, 64);
-                    if (lane >= off) offset += other;
-                }
-                int n_events = __shfl(offset, 63, 64);
-                offset -= mine;
-                if (ev.kind == 1) {
-#pragma unroll
-                    for (int e = 0; e < kMaxDirect; ++e) {
-                        if (!((ev.mask >> e) & 1)) continue;
-                        ev_bin[offset] = ev.k_lo + e;
-                        ev_val[offset][0] = 0.0; ev_val[offset][1] = 0.0;
-                        ev_val[offset][2] = 0.0; ev_val[offset][3] = ev.v[e];
-                        ++offset;
-                    }
-                } else if (ev.kind == 2) {
-                    if (!ev.pre) {
-                        ev_bin[offset] = ev.k_lo;
-                        ev_val[offset][0] = ev.v[0]; ev_val[offset][1] = ev.v[1];
-                        ev_val[offset][2] = ev.v[2]; ev_val[offset][3] = ev.v[3];
-                        ++offset;
-                    }
-                    if (ev.end) {
-                        ev_bin[offset] = ev.k_end;
-                        ev_val[offset][0] = -ev.v[0]; ev_val[offset][1] = -ev.v[1];
-                        ev_val[offset][2] = -ev.v[2]; ev_val[offset][3] = -ev.v[4];
-                    }
-                }
-                pre_flag[lane] = ev.kind == 2 && ev.pre;
-                pre_val[lane][0] = ev.v[0]; pre_val[lane][1] = ev.v[1];
-                pre_val[lane][2] = ev.v[2];
-                __syncthreads();
-                // the first event of every bin, compacted in order
-                int heads = 0;
-                for (int c = 0; c < n_events; c += 64) {
-                    int i = c + lane;
-                    bool head = i < n_events && (i == 0 || ev_bin[i] != ev_bin[i - 1]);
-                    unsigned long long ballot = __ballot(head);
-                    if (head) head_at[heads + __popcll(ballot & ((1ULL << lane) - 1))] = i;
-                    heads += __popcll(ballot);
-                }
-                __syncthreads();
-                // one lane per bin adds its events in panel order
-                for (int h = lane; h < heads; h += 64) {
-                    int i0 = head_at[h], i1 = h + 1 < heads ? head_at[h + 1] : n_events;
-                    double* a = B.acc + 4 * (size_t)ev_bin[i0];
-                    double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
-                    for (int i = i0; i < i1; ++i) {
-                        a0 += ev_val[i][0]; a1 += ev_val[i][1];
-                        a2 += ev_val[i][2]; a3 += ev_val[i][3];
-                    }
-                    a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
-                }
-                if (lane == 0) {
-                    int count = min(64, panels - base);
-                    for (int q = 0; q < count; ++q) {
-                        if (!pre_flag[q]) continue;
-                        B.pre[0] += pre_val[q][0]; B.pre[1] += pre_val[q][1];
-                        B.pre[2] += pre_val[q][2];
-                    }
-                }
-                __syncthreads();
+                events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
             }
             double total = wave_sum(total_lane);
             double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
             double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
             total += exact;
-            if (lane == 0) {
-                if (bins && spread > 0.0) {
+            // the front ramp and flat tail after every panel, as one more round
+            // (lanes 0 and 1; their times increase, so the round's bins do too)
+            if (bins) {
+                PanelEvents ev;
+                ev.kind = 0;
+                if (spread > 0.0) {
                     double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
                     double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
                     double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
@@ -1562,18 +1560,28 @@ __global__ void __launch_bounds__(64) o1_batches(Group G) {
                                                       e_last)
                                       : 0.0;
                     double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
-                    deposit_panel(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa);
-                    if (t_end > t_2)
-                        deposit_panel(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa);
-                } else if (bins && t_end > t_f) {
-                    deposit_panel(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa);
+                    if (lane == 0)
+                        panel_events(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa, ev);
+                    else if (lane == 1 && t_end > t_2)
+                        panel_events(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa, ev);
+                } else if (lane == 0 && t_end > t_f) {
+                    panel_events(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa, ev);
                 }
+                events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
             }
-            if (lane == lam) charge_lane += total;
-            __syncthreads();
+            count_panels += (unsigned long long)panels;
+            count_events += events_total;
+            if (lane == 0) charge_acc[lam] += total;
+            wave_sync();
         }
     }
-    if (lane < nl) G.part_charge[slot * nl + lane] = charge_lane;
+    __syncthreads();
+    if (tid < nl) G.part_charge[slot * nl + tid] = charge_acc[tid];
+    if (G.work_counts != nullptr && lane == 0) {
+        atomicAdd(G.work_counts, count_terms);
+        atomicAdd(G.work_counts + 1, count_panels);
+        atomicAdd(G.work_counts + 2, count_events);
+    }
 }
 
 // Partial deposits of the segments summed in segment order into segment 0.
@@ -1816,19 +1824,17 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
 }
 
 // Batch segments per OM: enough waves to fill the GPU, partials within 512 MiB.
-// Batch segments per OM: about 4096 waves in all, within 2 GiB of per-slot
-// partial deposits and path arrays.
+// Batch segments per OM: about 2048 workgroups in all, within 2 GiB of per-slot
+// partial deposits.
 int segments_for(int work, int bands, int bins) {
-    int S = std::max(1, std::min(64, (4096 + work - 1) / work));
-    while (S > 1 && (double)work * S * (bands * bins * 32.0 + 24.0 * kMaxPathNodes)
-                        > 2048.0 * 1024 * 1024)
-        S /= 2;
+    int S = std::max(1, std::min(64, (2048 + work - 1) / work));
+    while (S > 1 && (double)work * S * bands * bins * 32.0 > 2048.0 * 1024 * 1024) S /= 2;
     return S;
 }
 
 struct ScratchLayout {
     size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
-        part_charge, path, piece_count, piece_start, piece_values, scan_temp, total;
+        part_charge, piece_count, piece_start, piece_values, scan_temp, total;
     int segments;
     long long pieces;         // bound on the group-sum pieces
 };
@@ -1853,7 +1859,6 @@ int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mo
     L.part_acc = take(parts * bands * bins * 4 * sizeof(double));
     L.part_pre = take(parts * bands * 3 * sizeof(double));
     L.part_charge = take(parts * bands * sizeof(double));
-    L.path = take(parts * 3 * kMaxPathNodes * sizeof(double));
     long long entries = (long long)slots * work * n_pen;
     L.pieces = groups + (entries + kPieceEntries - 1) / kPieceEntries;
     L.piece_count = take((groups + 1) * sizeof(int));
@@ -2133,9 +2138,9 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
                            int groups, int key_bits, void* scratch, size_t scratch_bytes,
-                           int compute_bins,
-                           double* charge, double* bins, void* workspace, size_t workspace_bytes,
-                           void* stream) {
+                           int compute_bins, double* charge, double* bins,
+                           unsigned long long* work_counts, void* workspace,
+                           size_t workspace_bytes, void* stream) {
     int status = check_grouped(pencils, oms, optics, merge);
     if (status != LH_OK) return status;
     GroupLayout L;
@@ -2161,6 +2166,7 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.key_bits = key_bits;
     G.key_mask = (1LL << key_bits) - 1;
     G.compute_bins = compute_bins;
+    G.work_counts = work_counts;
     G.charge = charge;
     G.bins = bins;
     G.values = reinterpret_cast<double*>(sc + SL.values);
@@ -2171,7 +2177,6 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.part_acc = reinterpret_cast<double*>(sc + SL.part_acc);
     G.part_pre = reinterpret_cast<double*>(sc + SL.part_pre);
     G.part_charge = reinterpret_cast<double*>(sc + SL.part_charge);
-    G.path = reinterpret_cast<double*>(sc + SL.path);
     G.segments = SL.segments;
     G.piece_start = reinterpret_cast<int*>(sc + SL.piece_start);
     G.piece_values = reinterpret_cast<double*>(sc + SL.piece_values);
@@ -2207,11 +2212,11 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     {
         lh::KernelMark mark("o1_batches", s);
         switch (optics->max_power) {
-            case 0: o1_batches<0><<<grid, 64, 0, s>>>(G); break;
-            case 1: o1_batches<1><<<grid, 64, 0, s>>>(G); break;
-            case 2: o1_batches<2><<<grid, 64, 0, s>>>(G); break;
-            case 3: o1_batches<3><<<grid, 64, 0, s>>>(G); break;
-            default: o1_batches<4><<<grid, 64, 0, s>>>(G); break;
+            case 0: o1_batches<0><<<grid, kBatchThreads, 0, s>>>(G); break;
+            case 1: o1_batches<1><<<grid, kBatchThreads, 0, s>>>(G); break;
+            case 2: o1_batches<2><<<grid, kBatchThreads, 0, s>>>(G); break;
+            case 3: o1_batches<3><<<grid, kBatchThreads, 0, s>>>(G); break;
+            default: o1_batches<4><<<grid, kBatchThreads, 0, s>>>(G); break;
         }
     }
     if ((status = lh::check_launch("o1_batches")) != LH_OK) return status;

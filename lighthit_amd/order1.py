@@ -52,7 +52,8 @@ class _Optics(ctypes.Structure):
         (n, ctypes.c_int) for n in ("bands", "bins", "n_mono", "max_power", "direct_bins",
                                     "max_path_nodes")] + [
         (n, ctypes.c_double) for n in ("g", "y_g", "path_step", "path_step_tail",
-                                       "tail_exponent", "smoothing", "mut_min", "speed_max")]
+                                       "tail_exponent", "smoothing", "mut_min", "speed_max",
+                                       "bin_inverse_width")]
 
 
 class _Merge(ctypes.Structure):
@@ -78,7 +79,7 @@ _lib.SIGNATURES.update({
                                            ctypes.POINTER(_Optics), _I]),
     "lh_order1_group_charge": (_I, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
                                     ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _I, _I, _P,
-                                    _S, _I, _P, _P, _P, _S, _P]),
+                                    _S, _I, _P, _P, _P, _P, _S, _P]),
 })
 
 _GL48 = np.polynomial.legendre.leggauss(48)
@@ -256,6 +257,9 @@ class DeviceOptics:
         s.path_step, s.path_step_tail = float(settings.path_step), float(settings.path_step_tail)
         s.tail_exponent, s.smoothing = float(settings.tail_exponent), float(settings.smoothing_factor)
         s.mut_min, s.speed_max = float(np.min(tables["mut"])), float(np.max(tables["speed"]))
+        widths = np.diff(edges)
+        equal = bool(np.all(np.abs(widths - widths[0]) <= 1e-12 * abs(widths[0])))
+        s.bin_inverse_width = 1.0 / float(widths[0]) if equal and widths[0] > 0 else 0.0
         self.struct = s
         self.n_mono = len(coef)
         self.bins = len(widths)
@@ -382,6 +386,27 @@ def compress(unit, cell_m, pixel_face, ring_nodes, *, parent_face=0, stream=None
     return pencils
 
 
+_work_counts = {}
+
+
+def count_work(device="cuda", on=True):
+    """Start (or stop) counting the grouped passes' work on ``device``: profile
+    terms, time panels and bin events (for cost models; off by default)."""
+    import torch
+    key = str(torch.device(device))
+    if on:
+        _work_counts[key] = torch.zeros(3, dtype=torch.int64, device=device)
+    else:
+        _work_counts.pop(key, None)
+
+
+def work_counts(device="cuda"):
+    """``(profile_terms, panels, bin_events)`` counted since :func:`count_work`."""
+    import torch
+    counts = _work_counts.get(str(torch.device(device)))
+    return None if counts is None else tuple(int(v) for v in counts.cpu().tolist())
+
+
 def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None, bins=None,
             stream=None):
     """Grouped (or screen) order 1 of OMs ``work``: fills ``charge`` (and ``bins``)
@@ -416,8 +441,9 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
         _lib.call("lh_order1_group_charge", ctypes.byref(pen), ctypes.byref(om),
                   ctypes.byref(optics.struct), ctypes.byref(m), groups, key_bits, _addr(scratch),
                   ctypes.c_size_t(scratch.numel()), 1 if compute_bins else 0, _addr(charge),
-                  _addr(bins) if compute_bins else None, _addr(ws), ctypes.c_size_t(ws.numel()),
-                  handle)
+                  _addr(bins) if compute_bins else None,
+                  _addr(_work_counts.get(str(torch.device(device)))),
+                  _addr(ws), ctypes.c_size_t(ws.numel()), handle)
     del work_dev
     return charge, bins
 
