@@ -791,39 +791,57 @@ __device__ inline double wave_min(double v) {
     return v;
 }
 
-// Radius and delay extremes of every OM over its pencils: min/max within each
-// wave, then one atomic per wave and bound on the ordered encodings. The keys
-// floor(log r / step) and floor(tau / step) do not decrease with r and tau, so
-// their ranges follow from the extremes.
+// Radius and delay extremes of every OM over its pencils: each thread strides
+// over pencils, the workgroup reduces in LDS, and one thread adds the result
+// with atomics on the ordered encodings (a handful of workgroups per OM keeps
+// the atomics few). The keys floor(log r / step) and floor(tau / step) do not
+// decrease with r and tau, so their ranges follow from the extremes.
+constexpr int kRangeGroups = 16;   // workgroups per OM
+
 __global__ void __launch_bounds__(kThreads) o1_group_ranges(Group G) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double s_lo[kThreads / 64][2], s_hi[kThreads / 64][2];
     int w = blockIdx.y;
+    int d = G.om.work[w];
     const double inf = __builtin_huge_val();
     double r_lo = inf, r_hi = -inf, t_lo = inf, t_hi = -inf;
-    if (p < G.n_pen) {
-        int d = G.om.work[w];
+    const double ox = G.om.positions[3 * d], oy = G.om.positions[3 * d + 1],
+                 oz = G.om.positions[3 * d + 2], origin = G.om.origins[d];
+    for (int p = blockIdx.x * kThreads + threadIdx.x; p < G.n_pen;
+         p += gridDim.x * kThreads) {
         const double* pp = G.pen.position + 3 * p;
-        double dx = G.om.positions[3 * d] - pp[0];
-        double dy = G.om.positions[3 * d + 1] - pp[1];
-        double dz = G.om.positions[3 * d + 2] - pp[2];
+        double dx = ox - pp[0], dy = oy - pp[1], dz = oz - pp[2];
         double r = sqrt(dx * dx + dy * dy + dz * dz);
-        r_lo = r_hi = r;
+        r_lo = fmin(r_lo, r);
+        r_hi = fmax(r_hi, r);
         if (!G.screen) {
-            double t_emit = G.pen.time[p] - G.om.origins[d];
-            t_lo = t_hi = t_emit + r / G.merge.speed_ref;
+            double tau = (G.pen.time[p] - origin) + r / G.merge.speed_ref;
+            t_lo = fmin(t_lo, tau);
+            t_hi = fmax(t_hi, tau);
         }
     }
     r_lo = wave_min(r_lo);
     r_hi = wave_max(r_hi);
     t_lo = wave_min(t_lo);
     t_hi = wave_max(t_hi);
-    if ((threadIdx.x & 63) == 0 && r_lo <= r_hi) {
-        unsigned long long* x = G.x_range + 4 * w;
-        atomicMin(x, order_bits(r_lo));
-        atomicMax(x + 1, order_bits(r_hi));
-        if (!G.screen) {
-            atomicMin(x + 2, order_bits(t_lo));
-            atomicMax(x + 3, order_bits(t_hi));
+    int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[wave][0] = r_lo; s_hi[wave][0] = r_hi;
+        s_lo[wave][1] = t_lo; s_hi[wave][1] = t_hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kThreads / 64; ++k) {
+            r_lo = fmin(r_lo, s_lo[k][0]); r_hi = fmax(r_hi, s_hi[k][0]);
+            t_lo = fmin(t_lo, s_lo[k][1]); t_hi = fmax(t_hi, s_hi[k][1]);
+        }
+        if (r_lo <= r_hi) {
+            unsigned long long* x = G.x_range + 4 * w;
+            atomicMin(x, order_bits(r_lo));
+            atomicMax(x + 1, order_bits(r_hi));
+            if (!G.screen) {
+                atomicMin(x + 2, order_bits(t_lo));
+                atomicMax(x + 3, order_bits(t_hi));
+            }
         }
     }
 }
@@ -2093,8 +2111,9 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     if ((status = lh::check_launch("o1_prepare_shape")) != LH_OK) return status;
     o1_range_init<<<blocks(work), kThreads, 0, s>>>(G, work);
     if ((status = lh::check_launch("o1_range_init")) != LH_OK) return status;
+    dim3 range_grid(std::min(blocks(pencils->count), (unsigned)kRangeGroups), work);
+    o1_group_ranges<<<range_grid, kThreads, 0, s>>>(G);
     dim3 grid(blocks(pencils->count), work);
-    o1_group_ranges<<<grid, kThreads, 0, s>>>(G);
     if ((status = lh::check_launch("o1_group_ranges")) != LH_OK) return status;
     o1_key_space<<<blocks(work), kThreads, 0, s>>>(G, work);
     if ((status = lh::check_launch("o1_key_space")) != LH_OK) return status;
