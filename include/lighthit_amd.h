@@ -216,8 +216,9 @@ int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
  * e^{i omega t} per slice and the five-point transverse stencil. Two calls:
  * lh_compile_layout builds the lattice and the point layout and writes
  * info = {slices, wide slices, points, rows, error flags} (device ints);
- * after reading info the caller sizes the deposit buffer (rows * moments *
- * channels doubles) and the channels, and calls lh_compile_channels with the
+ * after reading info the caller sizes the deposit buffer (rows * (moments *
+ * channels + 2 * n_freq) doubles: the rows' deposits, then their phase table)
+ * and the channels, and calls lh_compile_channels with the
  * same workspace. Element arrays are shared by every band; a band is
  * selected by its photon scales and phase index (photons = scale0 * C0 -
  * scale2 * (-C2), cone = 1 / (beta n), elements with beta n <= 1 or mask 0

@@ -259,7 +259,8 @@ def compile_band(chords, frame, degree, omega, *, azimuthal_degree=4, options=No
         raise RuntimeError("a chord node fell outside the slice lattice")
     npoints, rows = int(counts[INFO_POINTS]), int(counts[INFO_ROWS])
     nch = int(channel_layout(int(degree), min(int(degree), int(azimuthal_degree)))[-1])
-    deposit = torch.empty(rows * lattice.moments * nch, dtype=torch.float64, device=device)
+    deposit = torch.empty(rows * (lattice.moments * nch + 2 * len(omega)), dtype=torch.float64,
+                          device=device)
     channels = torch.empty((npoints, nch, len(omega)), dtype=torch.complex64, device=device)
     if omega_device is None:
         omega_device = torch.as_tensor(omega, device=device)

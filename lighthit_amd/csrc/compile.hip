@@ -24,8 +24,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxDegree = 16;
 constexpr int kMaxFreq = 128;
-constexpr int kPhaseRows = 16;
-constexpr int kChannelGroup = 8;   // channels per phase workgroup (2 per wave)
 constexpr int kLayoutThreads = 1024;
 
 // info words written by lh_compile_layout
@@ -421,87 +419,84 @@ __global__ void __launch_bounds__(kThreads) compile_deposit(Params p, int rows, 
 
 // Per slice: values[k, c, w] = sum_rows D[row, k, c] e^{i omega_w tau dt}, then the
 // thin point or the five stencil points (_shower_basis.py:563-598).
-__global__ void __launch_bounds__(kThreads) compile_phase(Params p, const double* omega, int nfreq,
-                                                          double scale, const double* deposit,
-                                                          float2* channels) {
-    __shared__ double table_c[kPhaseRows][kMaxFreq];
-    __shared__ double table_s[kPhaseRows][kMaxFreq];
+// Phase table of every row: e^{i omega t} at the row's tick, t = tick * dt (FP64).
+__global__ void __launch_bounds__(kThreads) compile_phase_table(Params p, const double* omega,
+                                                                int nfreq, int rows,
+                                                                double* table_c,
+                                                                double* table_s) {
+    long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (e >= (long long)rows * nfreq) return;
+    int r = (int)(e / nfreq), w = (int)(e - (long long)r * nfreq);
+    const lh_compile_lattice& L = p.lat;
+    int key = p.row_key[r];
+    double t = (double)(key % L.kmax) * L.time_step_ns;
+    double sv, cv;
+    sincos(t * omega[w], &sv, &cv);
+    table_c[e] = cv;
+    table_s[e] = sv;
+}
+
+// Per slice and channel: the rows' deposits summed with the phase table in row
+// order (FP64), then the stencil combination. One wave per (slice, channel);
+// lane w holds frequencies w and w + 64; the deposits of a row are the same
+// for every lane.
+__global__ void __launch_bounds__(64) compile_phase(Params p, int nfreq, double scale,
+                                                   const double* deposit, const double* table_c,
+                                                   const double* table_s, float2* channels) {
     int s = blockIdx.x;
     if (!p.slice_used[s]) return;
     const lh_compile_lattice& L = p.lat;
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int width = L.moments * p.channels;
-    int row_lo = p.row_scan[(long long)s * L.kmax];
-    int row_hi = p.row_scan[(long long)(s + 1) * L.kmax];
-    double re[2][2][5], im[2][2][5];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int k = 0; k < 5; ++k) re[a][b][k] = im[a][b][k] = 0.0;
-    int c_base = blockIdx.y * kChannelGroup + 2 * wave;
-    for (int r0 = row_lo; r0 < row_hi; r0 += kPhaseRows) {
-        int nr = min(kPhaseRows, row_hi - r0);
-        for (int e = threadIdx.x; e < nr * nfreq; e += kThreads) {
-            int rr = e / nfreq, w = e - rr * nfreq;
-            int key = p.row_key[r0 + rr];
-            double t = (double)(key % L.kmax) * L.time_step_ns;
-            double ph = t * omega[w];
-            double sv, cv;
-            sincos(ph, &sv, &cv);
-            table_c[rr][w] = cv;
-            table_s[rr][w] = sv;
+    const int c = blockIdx.y;
+    const int lane = threadIdx.x;
+    const int width = L.moments * p.channels;
+    const int row_lo = p.row_scan[(long long)s * L.kmax];
+    const int row_hi = p.row_scan[(long long)(s + 1) * L.kmax];
+    double re[2][5], im[2][5];
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 5; ++k) re[b][k] = im[b][k] = 0.0;
+    const bool second = lane + 64 < nfreq;
+    const bool first = lane < nfreq;
+    for (int r = row_lo; r < row_hi; ++r) {
+        const double* d = deposit + (size_t)r * width + c;
+        const double* tc = table_c + (size_t)r * nfreq;
+        const double* ts = table_s + (size_t)r * nfreq;
+        double c0 = first ? tc[lane] : 0.0, s0 = first ? ts[lane] : 0.0;
+        double c1 = second ? tc[lane + 64] : 0.0, s1 = second ? ts[lane + 64] : 0.0;
+        for (int k = 0; k < L.moments; ++k) {
+            double v = d[k * p.channels];
+            re[0][k] += v * c0;
+            im[0][k] += v * s0;
+            re[1][k] += v * c1;
+            im[1][k] += v * s1;
         }
-        __syncthreads();
-        for (int rr = 0; rr < nr; ++rr) {
-            const double* d = deposit + (size_t)(r0 + rr) * width;
-            for (int a = 0; a < 2; ++a) {
-                int c = c_base + a;
-                if (c >= p.channels) continue;
-                for (int b = 0; b < 2; ++b) {
-                    int w = lane + 64 * b;
-                    if (w >= nfreq) continue;
-                    double cv = table_c[rr][w], sv = table_s[rr][w];
-                    for (int k = 0; k < L.moments; ++k) {
-                        double v = d[k * p.channels + c];
-                        re[a][b][k] += v * cv;
-                        im[a][b][k] += v * sv;
-                    }
-                }
-            }
-        }
-        __syncthreads();
     }
     int slices = p.info[kInfoSlices], wide_count = p.info[kInfoWide];
     double h = L.stencil_m;
-    for (int a = 0; a < 2; ++a) {
-        int c = c_base + a;
-        if (c >= p.channels) continue;
-        for (int b = 0; b < 2; ++b) {
-            int w = lane + 64 * b;
-            if (w >= nfreq) continue;
-            if (L.moments == 1 || !p.wide[s]) {
-                size_t at = ((size_t)p.point_of[s] * p.channels + c) * nfreq + w;
-                channels[at] = make_float2((float)(re[a][b][0] * scale),
-                                           (float)(im[a][b][0] * scale));
-                continue;
-            }
-            double h2 = h * h;
-            double vr[5], vi[5];
-            double m0r = re[a][b][0], mxr = re[a][b][1], myr = re[a][b][2];
-            double mxxr = re[a][b][3], myyr = re[a][b][4];
-            double m0i = im[a][b][0], mxi = im[a][b][1], myi = im[a][b][2];
-            double mxxi = im[a][b][3], myyi = im[a][b][4];
-            vr[0] = m0r - (mxxr + myyr) / h2;         vi[0] = m0i - (mxxi + myyi) / h2;
-            vr[1] = 0.5 * (mxxr / h2 + mxr / h);      vi[1] = 0.5 * (mxxi / h2 + mxi / h);
-            vr[2] = 0.5 * (mxxr / h2 - mxr / h);      vi[2] = 0.5 * (mxxi / h2 - mxi / h);
-            vr[3] = 0.5 * (myyr / h2 + myr / h);      vi[3] = 0.5 * (myyi / h2 + myi / h);
-            vr[4] = 0.5 * (myyr / h2 - myr / h);      vi[4] = 0.5 * (myyi / h2 - myi / h);
-            int ids[5] = {p.point_of[s], slices + p.wide_of[s], slices + wide_count + p.wide_of[s],
-                          slices + 2 * wide_count + p.wide_of[s],
-                          slices + 3 * wide_count + p.wide_of[s]};
-            for (int o = 0; o < 5; ++o) {
-                size_t at = ((size_t)ids[o] * p.channels + c) * nfreq + w;
-                channels[at] = make_float2((float)(vr[o] * scale), (float)(vi[o] * scale));
-            }
+    for (int b = 0; b < 2; ++b) {
+        int w = lane + 64 * b;
+        if (w >= nfreq) continue;
+        if (L.moments == 1 || !p.wide[s]) {
+            size_t at = ((size_t)p.point_of[s] * p.channels + c) * nfreq + w;
+            channels[at] = make_float2((float)(re[b][0] * scale), (float)(im[b][0] * scale));
+            continue;
+        }
+        double h2 = h * h;
+        double vr[5], vi[5];
+        double m0r = re[b][0], mxr = re[b][1], myr = re[b][2];
+        double mxxr = re[b][3], myyr = re[b][4];
+        double m0i = im[b][0], mxi = im[b][1], myi = im[b][2];
+        double mxxi = im[b][3], myyi = im[b][4];
+        vr[0] = m0r - (mxxr + myyr) / h2;         vi[0] = m0i - (mxxi + myyi) / h2;
+        vr[1] = 0.5 * (mxxr / h2 + mxr / h);      vi[1] = 0.5 * (mxxi / h2 + mxi / h);
+        vr[2] = 0.5 * (mxxr / h2 - mxr / h);      vi[2] = 0.5 * (mxxi / h2 - mxi / h);
+        vr[3] = 0.5 * (myyr / h2 + myr / h);      vi[3] = 0.5 * (myyi / h2 + myi / h);
+        vr[4] = 0.5 * (myyr / h2 - myr / h);      vi[4] = 0.5 * (myyi / h2 - myi / h);
+        int ids[5] = {p.point_of[s], slices + p.wide_of[s], slices + wide_count + p.wide_of[s],
+                      slices + 2 * wide_count + p.wide_of[s],
+                      slices + 3 * wide_count + p.wide_of[s]};
+        for (int o = 0; o < 5; ++o) {
+            size_t at = ((size_t)ids[o] * p.channels + c) * nfreq + w;
+            channels[at] = make_float2((float)(vr[o] * scale), (float)(vi[o] * scale));
         }
     }
 }
@@ -709,11 +704,17 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
         compile_deposit<<<rows, kThreads, 0, s>>>(p, rows, deposit);
     }
     if ((status = lh::check_launch("compile_deposit")) != LH_OK) return status;
+    // the phase table follows the rows' deposits in the deposit buffer
+    double* table_c = deposit + (size_t)rows * lattice->moments * p.channels;
+    double* table_s = table_c + (size_t)rows * n_freq;
+    compile_phase_table<<<(unsigned)(((long long)rows * n_freq + kThreads - 1) / kThreads),
+                          kThreads, 0, s>>>(p, omega, n_freq, rows, table_c, table_s);
+    if ((status = lh::check_launch("compile_phase_table")) != LH_OK) return status;
     // slice and wide counts: the workspace info words of lh_compile_layout
-    dim3 grid(lattice->positions, (p.channels + kChannelGroup - 1) / kChannelGroup);
+    dim3 grid(lattice->positions, p.channels);
     lh::KernelMark mark("compile_phase", s);
-    compile_phase<<<grid, kThreads, 0, s>>>(p, omega, n_freq, scale, deposit,
-                                           static_cast<float2*>(channels));
+    compile_phase<<<grid, 64, 0, s>>>(p, n_freq, scale, deposit, table_c, table_s,
+                                      static_cast<float2*>(channels));
     return lh::check_launch("compile_phase");
 }
 
