@@ -378,6 +378,39 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
                            size_t workspace_bytes, void* stream);
 
 /*
+ * Near-field clamp of the order >= 2 light (near_field.py:91-173 with
+ * _cone_many_looks, muon/track_ge2_primitive.py:612-660): for every element
+ * portion closer to an OM than the tables' radial minimum, the spectrum
+ * (relative to the OM's time origin) of its clamped order >= 2 light,
+ *   out[q, w] = density e^{i w phase_ns} sum_nodes w_i e^{-i w u_i clock}
+ *               sum_bands scale_b(R) e^{i w r / v_b} sum_l h_bl sum_pairs
+ *               rot_l(mu) alpha_lambda Y_lambda,mu(look') cubic_b,j,l,pair(t),
+ * r = hypot(impact, u_i), R = max(r, radial_low) for the amplitude
+ * (attenuation, 1/(4 pi R^2), spline), r for the flight phase; nodes with
+ * r < clamp_low or r > radial_high are left out. pack is the per-wavelength
+ * (unfolded) bundle; portion q's frame is (ex toward the OM, ey = axis x ex,
+ * axis = element direction); look is the OM look in that frame. out is
+ * (count, nfreq) complex128 (interleaved doubles).
+ */
+typedef struct lh_near_portions {
+    const double* impact;      /* (count) OM distance from the element line [m] */
+    const double* look;        /* (count, 3) OM look in the portion frame */
+    const double* weight;      /* (count, bands) wavelength weight x efficiency x area */
+    const double* harmonics;   /* (count, bands, degree + 1) cone harmonics per unit C0 */
+    const double* phase_ns;    /* (count) closest-point emission time minus the OM origin */
+    const double* clock;       /* (count) element time per metre [ns/m] */
+    const double* density;     /* (count) C0 per metre */
+    const int* node_start;     /* (count + 1) first quadrature node of each portion */
+    const double* u;           /* (nodes) OM minus source along the axis [m] */
+    const double* w;           /* (nodes) quadrature weights [m] */
+    int count;
+} lh_near_portions;
+
+int lh_near_field(const lh_ge2_pack* pack, const lh_near_portions* portions, const double* alpha,
+                  int nfreq, double clamp_low, double radial_low, double radial_high, double* out,
+                  void* stream);
+
+/*
  * Band fold of the order >= 2 pack (one-time table preparation).
  *
  * Replaces fold_bundle (band_fold.py:155-208): coefficients (bands,

@@ -776,3 +776,196 @@ extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source
     lh::set_error("lh_ge2_apply: acceptance degree %d not built (2, 3)", pack->acceptance_degree);
     return LH_ERR_ARGUMENT;
 }
+
+// ---------------------------------------------------------------- near field
+//
+// Clamped order >=2 of element portions closer to an OM than the tables'
+// radial minimum (near_field.py:91-173 with the fused-looks cone evaluator
+// _cone_many_looks, muon/track_ge2_primitive.py:612-660). A portion is an
+// axially symmetric point source moving along its element's axis; the OM sits
+// at (impact, 0, u) of the portion frame for every quadrature node u. Points
+// inside the radial minimum take the amplitude at the minimum along the same
+// direction and the flight phase of the true distance. One workgroup per
+// portion; the first thread sets up each node (Wigner column, detector
+// harmonics, spline offset, band scales) in LDS; every thread then owns one
+// frequency and sums nodes, bands, degrees and pairs in a fixed order (FP64).
+
+namespace {
+
+constexpr int kNearThreads = 128;   // frequencies per portion
+constexpr int kNearMaxDegree = 16;
+constexpr int kNearMaxBands = 32;
+
+struct NearParams {
+    lh_ge2_pack pack;
+    lh_near_portions por;
+    const double* alpha;
+    double clamp_low, radial_low, radial_high;
+    double* out;     // (count, nfreq, 2)
+    int nfreq;
+};
+
+template <int A>
+__global__ void __launch_bounds__(kNearThreads) near_field(NearParams P) {
+    constexpr int kPairs = (A + 1) * (A + 2) / 2;
+    __shared__ double s_k[kNearMaxDegree + 1][kPairs];   // rotation x alpha x detector harmonic
+    __shared__ double s_scale[kNearMaxBands];
+    __shared__ double s_radius, s_t, s_weight, s_u;
+    __shared__ int s_j, s_skip;
+    const int q = blockIdx.x;
+    const int f = threadIdx.x;
+    const lh_ge2_pack& pk = P.pack;
+    const lh_near_portions& por = P.por;
+    const int degree = pk.degree, bands = pk.bands;
+    const double omega = f < P.nfreq ? pk.omega[f] : 0.0;
+    const double clock = por.clock[q];
+    double acc_re = 0.0, acc_im = 0.0;
+    const double* harm = por.harmonics + (size_t)q * bands * (degree + 1);
+    for (int node = por.node_start[q]; node < por.node_start[q + 1]; ++node) {
+        if (threadIdx.x == 0) {
+            const double impact = por.impact[q];
+            const double u = por.u[node];
+            const double radius = hypot(impact, u);
+            int skip = radius < P.clamp_low || radius > P.radial_high || radius == 0.0;
+            if (!skip) {
+                const double cosine = u / radius, sine = impact / radius;
+                const double amplitude = fmax(radius, P.radial_low);
+                const double la = log(amplitude);
+                int lo = 0, hi = pk.intervals;   // searchsorted(log_r, la, 'right') - 1
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (pk.log_r[mid] <= la) lo = mid; else hi = mid;
+                }
+                if (la < pk.log_r[0]) lo = 0;
+                s_j = lo;
+                s_t = la - pk.log_r[lo];
+                s_radius = radius;
+                for (int b = 0; b < bands; ++b)
+                    s_scale[b] = por.weight[(size_t)q * bands + b]
+                                 * exp(-pk.absorption[b] * amplitude)
+                                 / (4.0 * M_PI * amplitude * amplitude);
+                const double* look = por.look + 3 * (size_t)q;
+                double y[(A + 1) * (A + 1)];
+                harmonics<A>(cosine * look[0] - sine * look[2], look[1],
+                             sine * look[0] + cosine * look[2], y);
+                Rotation<A, 0> rot;
+                rot.init(cosine, 0.0);
+                for (int l = 0; l <= degree; ++l) {
+                    rot.step(l);
+                    int pair = 0;
+#pragma unroll
+                    for (int lam = 0; lam <= A; ++lam) {
+#pragma unroll
+                        for (int mu = 0; mu <= lam; ++mu, ++pair) {
+                            double value = 0.0;
+                            if (mu <= l) {
+                                const double rotation =
+                                    mu == 0 ? rot.cur[0][0][0]
+                                            : 1.4142135623730950488 * parity(mu) * rot.cur[mu][0][0];
+                                value = rotation * P.alpha[lam] * y[lam * lam + lam + mu];
+                            }
+                            s_k[l][pair] = value;
+                        }
+                    }
+                }
+                s_weight = por.w[node];
+                s_u = u;
+            }
+            s_skip = skip;
+        }
+        __syncthreads();
+        if (!s_skip && f < P.nfreq) {
+            const double t = s_t;
+            const int j = s_j;
+            double node_re = 0.0, node_im = 0.0;
+            for (int b = 0; b < bands; ++b) {
+                double band_re = 0.0, band_im = 0.0;
+                const float2* c = static_cast<const float2*>(pk.coefficients)
+                                  + (((size_t)b * pk.intervals + j) * (degree + 1)) * kPairs * 4
+                                        * pk.freq_stride;
+                for (int l = 0; l <= degree; ++l) {
+                    const double h = harm[b * (degree + 1) + l];
+                    if (h == 0.0) continue;
+                    double l_re = 0.0, l_im = 0.0;
+#pragma unroll
+                    for (int pair = 0; pair < kPairs; ++pair) {
+                        const float2* cc = c + ((size_t)l * kPairs + pair) * 4 * pk.freq_stride + f;
+                        const double re = (((double)cc[0].x * t + cc[pk.freq_stride].x) * t
+                                           + cc[2 * pk.freq_stride].x) * t
+                                          + cc[3 * pk.freq_stride].x;
+                        const double im = (((double)cc[0].y * t + cc[pk.freq_stride].y) * t
+                                           + cc[2 * pk.freq_stride].y) * t
+                                          + cc[3 * pk.freq_stride].y;
+                        l_re += s_k[l][pair] * re;
+                        l_im += s_k[l][pair] * im;
+                    }
+                    band_re += h * l_re;
+                    band_im += h * l_im;
+                }
+                const double angle = pk.flight ? omega * s_radius / pk.speed[b] : 0.0;
+                double sa, ca;
+                sincos(angle, &sa, &ca);
+                const double pr = s_scale[b] * ca, pi = s_scale[b] * sa;
+                node_re += band_re * pr - band_im * pi;
+                node_im += band_re * pi + band_im * pr;
+            }
+            // quadrature weight and the emission-time shift e^{-i u clock omega}
+            double sd, cd;
+            sincos(-s_u * clock * omega, &sd, &cd);
+            const double w = s_weight;
+            acc_re += w * (node_re * cd - node_im * sd);
+            acc_im += w * (node_re * sd + node_im * cd);
+        }
+        __syncthreads();
+    }
+    if (f < P.nfreq) {
+        double sc, cc;
+        sincos(omega * por.phase_ns[q], &sc, &cc);
+        const double density = por.density[q];
+        double* o = P.out + ((size_t)q * P.nfreq + f) * 2;
+        o[0] = density * (acc_re * cc - acc_im * sc);
+        o[1] = density * (acc_re * sc + acc_im * cc);
+    }
+}
+
+}  // namespace
+
+extern "C" int lh_near_field(const lh_ge2_pack* pack, const lh_near_portions* portions,
+                             const double* alpha, int nfreq, double clamp_low, double radial_low,
+                             double radial_high, double* out, void* stream) {
+    LH_REQUIRE(pack && portions, "lh_near_field: null descriptor");
+    if (portions->count == 0) return LH_OK;
+    LH_REQUIRE(portions->count > 0, "lh_near_field: negative portion count");
+    LH_REQUIRE(nfreq >= 1 && nfreq <= kNearThreads && nfreq <= pack->freq_stride,
+               "lh_near_field: 1..%d frequencies", kNearThreads);
+    LH_REQUIRE(pack->degree >= 0 && pack->degree <= kNearMaxDegree && pack->bands >= 1
+                   && pack->bands <= kNearMaxBands && pack->intervals >= 1,
+               "lh_near_field: degree <= %d and 1..%d bands", kNearMaxDegree, kNearMaxBands);
+    LH_REQUIRE(pack->acceptance_degree >= 0 && pack->acceptance_degree <= 4,
+               "lh_near_field: acceptance degree must lie in 0..4");
+    LH_REQUIRE(radial_low < radial_high && clamp_low <= radial_low,
+               "lh_near_field: need clamp_low <= radial_low < radial_high");
+    const lh_near_portions& p = *portions;
+    LH_REQUIRE(p.impact && p.look && p.weight && p.harmonics && p.phase_ns && p.clock && p.density
+                   && p.node_start && p.u && p.w && alpha && out && pack->coefficients
+                   && pack->log_r && pack->absorption && pack->speed && pack->omega,
+               "lh_near_field: null device pointer");
+    NearParams P{};
+    P.pack = *pack;
+    P.por = p;
+    P.alpha = alpha;
+    P.clamp_low = clamp_low;
+    P.radial_low = radial_low;
+    P.radial_high = radial_high;
+    P.out = out;
+    P.nfreq = nfreq;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (pack->acceptance_degree) {
+        case 0: near_field<0><<<p.count, kNearThreads, 0, s>>>(P); break;
+        case 1: near_field<1><<<p.count, kNearThreads, 0, s>>>(P); break;
+        case 2: near_field<2><<<p.count, kNearThreads, 0, s>>>(P); break;
+        case 3: near_field<3><<<p.count, kNearThreads, 0, s>>>(P); break;
+        default: near_field<4><<<p.count, kNearThreads, 0, s>>>(P); break;
+    }
+    return lh::check_launch("lh_near_field");
+}

@@ -719,6 +719,14 @@ class ShowerTransport:
                                       self.kernel.acceptance()["alpha"], device=self.device)
             return self._order0
 
+    def near_pack(self, kernel):
+        """The per-wavelength pack of the near-field clamp (built on first use)."""
+        with self._lock:
+            tables = self.tables()
+            if tables.get("near_pack") is None:
+                tables["near_pack"] = _near_pack(tables, kernel, self.config, self.device)
+            return tables["near_pack"]
+
     def window_geometry(self, kernel):
         """Device geometry of the radial-window report (once per detector)."""
         with self._lock:
@@ -899,10 +907,6 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     tail = tail_dev.cpu().numpy()
     radial_tail = screen_radial_tail(tail, basis, high)
     reach = np.flatnonzero(used > 0)
-    if np.any(near):
-        raise NotImplementedError(
-            f"{int(near.sum())} OMs have source points closer than the {low:g} m radial minimum "
-            "of the order >=2 tables; the near-field clamp (near_field.py) is not on the GPU path")
 
     # predicted zero-frequency >=2 charge of every reachable OM
     predicted = np.zeros(n)
@@ -1063,8 +1067,35 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         "prompt_order1_level": prompt.metadata["order1_level"],
         "backend": "lighthit_amd (gfx950)",
     }
-    return HybridTransportResponse(kernel.detector, omega.copy(), edges.copy(), spectrum,
-                                   components, charge, origins, active, "shower", metadata)
+    response = HybridTransportResponse(kernel.detector, omega.copy(), edges.copy(), spectrum,
+                                       components, charge, origins, active, "shower", metadata)
+    if np.any(near):
+        # source points inside the tables' radial minimum: the near-field clamp
+        # (shower/api.py:536-538)
+        from .near_field import add_near_field
+        near_pack = (route.near_pack(kernel) if route is not None else
+                     _near_pack(tables, kernel, config, device))
+        distance, _ = segment_bounds(positions, elements, None, device, chords)
+        candidates = np.flatnonzero(distance < near_pack.radial_low * (1 + 1e-12))
+        readout = (route.readout(omega, edges) if route is not None else
+                   _readout_basis(kernel, omega, edges, device))
+        add_near_field(response, kernel, elements, near_pack, readout, label="shower",
+                       candidates=candidates, device=device)
+    return response
+
+
+def _near_pack(tables, kernel, config, device):
+    """The per-wavelength order >=2 pack on the route's frequencies, on the device
+    (the near-field clamp evaluates it wavelength by wavelength)."""
+    from .fold import KernelPack, frequency_indices
+    from .ge2 import DevicePack
+    pack = tables["pack"]
+    if not isinstance(pack, KernelPack):
+        pack = KernelPack.load(pack)
+    indices = frequency_indices(kernel.config.omega_per_ns, config.frequency_nodes)
+    bundle = pack.bundle(kernel.wavelength.wavelength_nm, int(kernel.config.source_degree),
+                         indices)
+    return DevicePack(bundle, device)
 
 
 _readout_cache = {}

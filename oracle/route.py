@@ -138,9 +138,11 @@ def transport_shower(kernel, elements, bundle, fold, *, config, subdivide=1):
     attenuation = _attenuation(kernel)
     peak = float(np.max(np.abs(detector.angular_acceptance(np.linspace(-1, 1, 257)))))
     used = np.zeros(n, np.int64)
+    near = np.zeros(n, bool)
     for band, item in zip(fold.bands, sources):
         radius = np.linalg.norm(item.points_m()[None] - positions[:, None], axis=2)
         used = np.maximum(used, np.count_nonzero((radius >= low) & (radius <= high), axis=1))
+        near |= np.any(radius < low, axis=1)
     reach = np.flatnonzero(used > 0)
     predicted = np.zeros(n)
     collapsed = _collapse(sources, float(config.predictor_cell_m))
@@ -207,6 +209,20 @@ def transport_shower(kernel, elements, bundle, fold, *, config, subdivide=1):
         components[ids, :, 1] = bins1[ids]
         components[ids, :, 2] = oracle_readout.route_readout(
             omega, spectrum[:, ids], origins[ids], kernel.config.readout_omega_per_ns, edges)
+    near_oms = np.zeros(0, int)
+    if np.any(near):
+        # the near-field clamp (shower/api.py:536-538, near_field.py:176-217)
+        from . import near_field as oracle_near
+        near_oms, near_charge, near_spectrum = oracle_near.near_field_ge2(
+            kernel, elements, bundle, origins)
+        charge[near_oms, 2] += near_charge[near_oms]
+        detailed = near_oms[active[near_oms]]
+        if len(detailed):
+            components[detailed, :, 2] += oracle_readout.route_readout(
+                omega, near_spectrum[:, detailed], np.zeros(len(detailed)),
+                kernel.config.readout_omega_per_ns, edges)
+        spectrum[:, near_oms] += near_spectrum[:, near_oms] * np.exp(
+            1j * omega[:, None] * origins[None, near_oms])
     return dict(charge=charge, bins=components, spectrum=spectrum, origins=origins,
                 active=active, level=level, prompt_active=prompt_active, fallback=fallback,
-                order1_level=level1)
+                order1_level=level1, near_oms=near_oms)

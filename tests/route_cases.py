@@ -12,3 +12,19 @@ def demonstration_route(device="cuda"):
     key = pack.add(*synthetic_pack(w.medium, kernel.wavelength))
     route = ShowerTransport(kernel, pack, device=device)
     return w, kernel, pack, route
+
+
+def near_event(om=20, elements=400, impact=(1.0, 0.3, 0.0), seed=5):
+    """A short shower passing about 1 m from OM ``om`` of the demonstration array
+    (inside the tables' 2 m radial minimum, outside the 0.2 m glass)."""
+    from dataclasses import replace
+
+    from lighthit_amd.sources import SourcePose, SyntheticShower
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    position = w.detector.positions_m[om]
+    direction = np.array([0.05, 0.0, 1.0])
+    pose = SourcePose(position + np.asarray(impact), direction / np.linalg.norm(direction))
+    shower = SyntheticShower.gaussian(pose, charged_track_length_m=30.0, elements=elements,
+                                      seed=seed)
+    return w, replace(shower.elements, cone_model="spectral")

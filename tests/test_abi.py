@@ -36,6 +36,7 @@ def test_python_binding_covers_every_symbol():
     import lighthit_amd.order1  # noqa: F401
     import lighthit_amd.fold  # noqa: F401
     import lighthit_amd.route  # noqa: F401
+    import lighthit_amd.near_field  # noqa: F401
     assert set(declared()) <= set(_lib.SIGNATURES)
 
 
