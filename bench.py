@@ -232,15 +232,20 @@ def _counter_kib(db_dir, counter, name_part):
     return sum(values) / len(values) if values else None
 
 
-def measure_traffic(workload_flag, kernel_name):
+def measure_traffic(workload_flag, kernel_name, keep_dir=None):
     """HBM bytes per launch of ``kernel_name`` from two counter runs, each
-    scaled by what the same counter reads for a plain copy of known size."""
+    scaled by what the same counter reads for a plain copy of known size.
+    ``keep_dir`` keeps the counter databases there instead of a temporary folder."""
     import shutil
     import subprocess
     import tempfile
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
-    base = tempfile.mkdtemp(prefix="lh_traffic_")
+    if keep_dir:
+        base = os.path.abspath(keep_dir)
+        os.makedirs(base, exist_ok=True)
+    else:
+        base = tempfile.mkdtemp(prefix="lh_traffic_")
     me = os.path.abspath(__file__)
     runs = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -264,10 +269,11 @@ def measure_traffic(workload_flag, kernel_name):
         if not copy_kib or kernel_kib is None:
             return None, f"no {counter} samples"
         scale = 256 * 1024 / copy_kib    # the copy moves 256 MiB each way
-        result[counter] = (kernel_kib * scale * 1024, scale)
+        result[counter] = (kernel_kib * scale * 1024, scale, kernel_kib)
     traffic = result["FETCH_SIZE"][0] + result["WRITE_SIZE"][0]
-    note = (f"{kernel_name}: FETCH_SIZE x{result['FETCH_SIZE'][1]:.3f} + "
-            f"WRITE_SIZE x{result['WRITE_SIZE'][1]:.3f} (copy-calibrated), bytes per launch")
+    note = (f"{kernel_name}: FETCH_SIZE {result['FETCH_SIZE'][2]:.0f} KiB x"
+            f"{result['FETCH_SIZE'][1]:.3f} + WRITE_SIZE {result['WRITE_SIZE'][2]:.0f} KiB x"
+            f"{result['WRITE_SIZE'][1]:.3f} per dispatch (copy-calibrated), bytes per launch")
     return traffic, note
 
 
@@ -328,6 +334,8 @@ def main():
     parser.add_argument("--served-events", type=int, default=60)
     parser.add_argument("--no-traffic", action="store_true",
                         help="skip the rocprofv3 counter runs that measure HBM traffic")
+    parser.add_argument("--traffic-dir", default=None,
+                        help="keep the counter runs' databases in this folder")
     parser.add_argument("--copy-kernel-mib", type=int, default=0,
                         help="only run a plain device copy of this size (HBM counter calibration)")
     args = parser.parse_args()
@@ -438,7 +446,8 @@ def main():
         roofline["ms_per_launch"] = kernels[dominant]["ms_per_launch"]
         traffic, note = (None, "not measured (--no-traffic)")
         if not args.no_traffic:
-            traffic, note = measure_traffic(args.workload, dominant.split("<")[0])
+            traffic, note = measure_traffic(args.workload, dominant.split("<")[0],
+                                             args.traffic_dir)
         roofline["traffic"] = traffic
         roofline["traffic_note"] = note
     print(json.dumps({"gpu_part": {"ms_per_step": ms_per_step, "kernels": kernels,

@@ -1593,12 +1593,23 @@ __global__ void __launch_bounds__(kBatchThreads) o1_batches(Group G) {
             wave_sync();
         }
     }
+    __shared__ unsigned long long s_counts[3];
+    if (tid < 3) s_counts[tid] = 0;
     __syncthreads();
     if (tid < nl) G.part_charge[slot * nl + tid] = charge_acc[tid];
-    if (G.work_counts != nullptr && lane == 0) {
-        atomicAdd(G.work_counts, count_terms);
-        atomicAdd(G.work_counts + 1, count_panels);
-        atomicAdd(G.work_counts + 2, count_events);
+    if (G.work_counts != nullptr) {
+        // profile terms are the same in every wave: count them once
+        if (tid == 0) s_counts[0] = count_terms;
+        if (lane == 0) {
+            atomicAdd(&s_counts[1], count_panels);
+            atomicAdd(&s_counts[2], count_events);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(G.work_counts, s_counts[0]);
+            atomicAdd(G.work_counts + 1, s_counts[1]);
+            atomicAdd(G.work_counts + 2, s_counts[2]);
+        }
     }
 }
 
@@ -1841,7 +1852,6 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
     return LH_OK;
 }
 
-// Batch segments per OM: enough waves to fill the GPU, partials within 512 MiB.
 // Batch segments per OM: about 2048 workgroups in all, within 2 GiB of per-slot
 // partial deposits.
 int segments_for(int work, int bands, int bins) {
