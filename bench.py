@@ -77,6 +77,11 @@ def build_route(name):
     return workload, kernel, route
 
 
+def route_factory(name):
+    """The route alone, built the same way in a serving worker process."""
+    return build_route(name)[2]
+
+
 # ----------------------------------------------------------- roofline models
 
 def order0_bytes(n_elements, n_oms, bins):
@@ -250,7 +255,8 @@ def measure_traffic(workload_flag, kernel_name, keep_dir=None):
     runs = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         for kind, args in (("bench", ["--steps", "1", "--warmup", "1", "--no-cpu-baseline",
-                                      "--no-traffic", "--workers", "1", "--workload",
+                                      "--no-traffic", "--workers", "1", "--processes", "0",
+                                      "--workload",
                                       workload_flag]),
                            ("copy", ["--copy-kernel-mib", "256"])):
             out = os.path.join(base, f"{kind}_{counter}")
@@ -331,7 +337,9 @@ def main():
     parser.add_argument("--cpu-fraction", type=float, default=0.05)
     parser.add_argument("--workers", type=int, default=3,
                         help="concurrent route calls of the served form (1: plain only)")
-    parser.add_argument("--served-events", type=int, default=60)
+    parser.add_argument("--served-events", type=int, default=120)
+    parser.add_argument("--processes", type=int, default=8,
+                        help="also serve from this many worker processes (0: threads only)")
     parser.add_argument("--no-traffic", action="store_true",
                         help="skip the rocprofv3 counter runs that measure HBM traffic")
     parser.add_argument("--traffic-dir", default=None,
@@ -431,6 +439,52 @@ def main():
                       "ms_per_event": 1e3 * served_elapsed / len(batch),
                       "latency_ms_all_submitted_at_once": latency_summary(futures),
                       "same_as_alone": bool(mixed_same)}
+    processes = None
+    if args.processes > 0:
+        import functools
+
+        from lighthit_amd.serve import ProcessShowerServer
+        print(f"[bench] serving {args.served_events} events from {args.processes} processes",
+              file=sys.stderr, flush=True)
+        with ProcessShowerServer(functools.partial(route_factory, args.workload),
+                                 processes=args.processes, detector=workload.detector,
+                                 device="cuda:0") as server:
+            server.map([elements for _ in range(2 * args.processes)])
+            # every event's source is staged in device memory in the workers
+            # before the timed region (a separate source per event)
+            staged = [server.stage(elements) for _ in range(args.served_events)]
+            began_p = time.perf_counter()
+            futures = [server.submit(handle) for handle in staged]
+            results = [f.result() for f in futures]
+            elapsed_p = time.perf_counter() - began_p
+            for handle in staged:
+                server.release(handle)
+            # latency under Poisson arrivals at 70% of the throughput just measured
+            rate = 0.7 * len(results) / elapsed_p
+            staged = [server.stage(elements) for _ in range(args.served_events)]
+            gaps = np.random.default_rng(23).exponential(1.0 / rate, len(staged))
+            arrivals = []
+            start_at = time.perf_counter()
+            for handle, at in zip(staged, np.cumsum(gaps)):
+                delay = start_at + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                arrivals.append(server.submit(handle))
+            for f in arrivals:
+                f.result()
+            for handle in staged:
+                server.release(handle)
+        same_p = all(np.array_equal(r.charge_components_pe, first.charge_components_pe)
+                     and np.array_equal(r.components_pe, first.components_pe)
+                     for r in results)
+        processes = {"events_per_s": len(results) / elapsed_p, "events": len(results),
+                     "processes": args.processes,
+                     "ms_per_event": 1e3 * elapsed_p / len(results),
+                     "latency_ms_all_submitted_at_once": latency_summary(futures),
+                     "latency_ms_poisson": dict(latency_summary(arrivals),
+                                                rate_events_per_s=rate),
+                     "same_as_alone": bool(same_p),
+                     "sources": "staged in device memory in every worker before timing"}
     ms_per_step = 1e3 * elapsed / args.steps
     lat = np.asarray(latencies) * 1e3
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_event"]) if kernels else None
@@ -461,11 +515,20 @@ def main():
              "latency_ms": {"p50": float(np.percentile(lat, 50)),
                             "p99": float(np.percentile(lat, 99)), "max": float(lat.max())},
              "events_per_s_host_arrays": args.steps / host_elapsed}
-    if served is not None:
+    if processes is not None and (served is None
+                                  or processes["events_per_s"] >= served["events_per_s"]):
+        value, step_ms, form = (processes["events_per_s"], processes["ms_per_event"],
+                                f"served: {args.processes} worker processes, one "
+                                "ShowerTransport call each at a time")
+        value_latency = dict(processes["latency_ms_poisson"], form="served, Poisson arrivals")
+    elif served is not None:
         value, step_ms, form = (served["events_per_s"], served["ms_per_event"],
                                 f"served: {args.workers} concurrent ShowerTransport calls")
+        value_latency = dict(served["latency_ms_all_submitted_at_once"],
+                             form="served, all submitted at once")
     else:
         value, step_ms, form = plain["events_per_s"], ms_per_step, "plain: one call at a time"
+        value_latency = dict(plain["latency_ms"], form="plain")
     line = {
         "metric": "events/sec + p50/p99 latency at <=1e-3 charge error vs FP64; served vs plain GPU",
         "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
@@ -482,10 +545,9 @@ def main():
                        np.asarray(response.metadata["prompt_order1_level"]) > 0)),
                    "charge_pe_by_order": [float(x) for x in
                                           response.charge_components_pe.sum(axis=0)],
-                   "plain": plain, "served": served,
-                   "served_over_plain": (served["events_per_s"] / plain["events_per_s"]
-                                         if served else None),
-                   "latency_ms": plain["latency_ms"],
+                   "plain": plain, "served": served, "served_processes": processes,
+                   "served_over_plain": value / plain["events_per_s"],
+                   "latency_ms": value_latency,
                    "repeatable": same,
                    "stage_gpu_ms": stage_ms, "kernel_gpu_ms": kernels,
                    "host_stage_ms": {k: 1e3 * v for k, v in

@@ -9,6 +9,11 @@ footprint fits the budget beside the events already running. Every event's
 result is the same as when it runs alone: the kernels of one call read and
 write only that call's buffers and sum in a fixed order.
 
+``ProcessShowerServer`` runs the same loop over worker processes, each with its
+own route on the same GPU: the host steps of different events then run in
+parallel rather than under one interpreter lock, and their kernels still share
+the device.
+
 The reference evaluates one event per call (``shower/api.py:176-190``); this
 loop is the serving form around that call.
 """
@@ -120,6 +125,304 @@ class ShowerServer:
             self._queue.put(None)
         for thread in self._threads:
             thread.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+
+# ------------------------------------------------------------ worker processes
+
+def _pack_response(response):
+    """A response as plain arrays, with only the bin rows that hold values (the
+    full (OMs, bins, 3) array is mostly zero rows)."""
+    components = response.components_pe
+    rows = np.flatnonzero(components.reshape(len(components), -1).any(axis=1))
+    return {"omega": response.omega_per_ns, "edges": response.relative_time_edges_ns,
+            "spectrum": response.spectrum_ge2_pe, "rows": rows,
+            "row_components": np.ascontiguousarray(components[rows]),
+            "components_shape": components.shape,
+            "charge": response.charge_components_pe, "origins": response.time_origin_ns,
+            "active": response.active, "method": response.method,
+            "metadata": response.metadata, "orders": tuple(response.computed_orders)}
+
+
+def _unpack_response(packed, detector):
+    from .route import HybridTransportResponse
+    components = np.zeros(packed["components_shape"])
+    components[packed["rows"]] = packed["row_components"]
+    return HybridTransportResponse(detector, packed["omega"], packed["edges"], packed["spectrum"],
+                                   components, packed["charge"], packed["origins"],
+                                   packed["active"], packed["method"], packed["metadata"],
+                                   packed["orders"])
+
+
+class StagedSource:
+    """Handle of a source staged in every worker process (its elements already
+    in device memory there); ``ProcessShowerServer.stage`` makes it."""
+
+    def __init__(self, key, count):
+        self.key = key
+        self.count = count
+
+    def __len__(self):
+        return self.count
+
+
+def _process_main(factory, device, threads, tasks, results, slot):
+    """One worker process: build the route, then answer tasks until ``None``.
+    ``device`` None leaves the GPU alone (host-only routes)."""
+    from threadpoolctl import threadpool_limits
+    limits = threadpool_limits(limits=threads)
+    drain = None
+    try:
+        if device is not None:
+            import torch
+            torch.cuda.set_device(torch.device(device))
+            drain = torch.cuda.synchronize
+        route = factory()
+        route.tables()
+        route.order0()
+        results.put(("ready", slot, None, None))
+    except BaseException as error:      # reported to the parent, which raises it
+        results.put(("failed", slot, None, f"{type(error).__name__}: {error}"))
+        return
+    staged = {}
+    while True:
+        item = tasks.get()
+        if item is None:
+            break
+        kind, key, payload, call_kwargs = item
+        if kind == "stage":
+            try:
+                from .route import ResidentShower
+                staged[key] = ResidentShower(payload, device) if device is not None else payload
+                results.put(("staged", slot, key, None))
+            except BaseException as error:
+                results.put(("stage_error", slot, key, f"{type(error).__name__}: {error}"))
+            continue
+        if kind == "release":
+            staged.pop(key, None)
+            continue
+        try:
+            source = staged[payload.key] if isinstance(payload, StagedSource) else payload
+            response = route(source, **call_kwargs)
+            if drain is not None:
+                drain()
+            results.put(("done", slot, key, _pack_response(response)))
+        except BaseException as error:
+            results.put(("error", slot, key, (type(error).__name__, str(error))))
+    limits.restore_original_limits()
+
+
+class ProcessShowerServer:
+    """``ShowerTransport`` calls on one GPU from ``processes`` worker processes.
+
+    ``factory`` is a picklable callable that returns the route; each process
+    calls it once, so every process holds the tables resident in HBM. An
+    event is sent to a free process when its estimated footprint fits the
+    memory budget beside the events already running; the response comes back
+    as arrays. ``submit`` returns a ``Future`` as ``ShowerServer.submit`` does.
+    Sources travel as host element arrays (a ``ResidentShower`` is sent as its
+    host elements), or are staged ahead in device memory with ``stage``.
+    """
+
+    def __init__(self, factory, *, processes=2, detector, memory_budget_bytes=None,
+                 device="cuda:0", threads_per_process=None, start_timeout_s=600.0):
+        import multiprocessing as mp
+        import os
+        if processes < 1:
+            raise ValueError("processes must be positive")
+        self.detector = detector
+        self.device = device
+        if memory_budget_bytes is None:
+            if device is None:
+                raise ValueError("a host-only server needs memory_budget_bytes")
+            import torch
+            free, _ = torch.cuda.mem_get_info(torch.device(device))
+            memory_budget_bytes = int(0.5 * free)
+        self.budget = int(memory_budget_bytes)
+        share = int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
+        threads = threads_per_process or max(1, share // (processes + 1))
+        context = mp.get_context("spawn")
+        self._results = context.Queue()
+        self._tasks = [context.Queue() for _ in range(processes)]
+        self._processes = [context.Process(target=_process_main,
+                                           args=(factory, device, threads, self._tasks[i],
+                                                 self._results, i),
+                                           name=f"shower-process-{i}", daemon=True)
+                           for i in range(processes)]
+        for process in self._processes:
+            process.start()
+        self._lock = threading.Condition()
+        self._idle = []
+        self._inflight = 0
+        self._pending = {}
+        self._staging = {}
+        self._waiting = queue.Queue()
+        self._closed = False
+        self._next_key = 0
+        started = time.perf_counter()
+        ready = 0
+        while ready < processes:
+            remaining = start_timeout_s - (time.perf_counter() - started)
+            try:
+                kind, slot, _, message = self._results.get(timeout=max(remaining, 0.1))
+            except queue.Empty:
+                self._terminate()
+                raise RuntimeError("worker processes did not start in time") from None
+            if kind == "failed":
+                self._terminate()
+                raise RuntimeError(f"worker process {slot} failed to start: {message}")
+            self._idle.append(slot)
+            ready += 1
+        self._collector = threading.Thread(target=self._collect, name="shower-collector",
+                                           daemon=True)
+        self._dispatcher = threading.Thread(target=self._dispatch, name="shower-dispatcher",
+                                            daemon=True)
+        self._collector.start()
+        self._dispatcher.start()
+
+    def submit(self, source, **call_kwargs):
+        from .route import _elements_of
+        if self._closed:
+            raise RuntimeError("the server is closed")
+        future = Future()
+        future.submitted = time.perf_counter()
+        elements = source if isinstance(source, StagedSource) else _elements_of(source)
+        need = footprint_bytes(elements, len(self.detector))
+        self._waiting.put((future, elements, call_kwargs, need))
+        return future
+
+    def stage(self, source):
+        """Upload ``source`` to the device in every worker process ahead of its
+        call; returns the handle to ``submit``. Each staged handle is a separate
+        source (the per-event compile runs for it)."""
+        from .route import _elements_of
+        if self._closed:
+            raise RuntimeError("the server is closed")
+        elements = _elements_of(source)
+        with self._lock:
+            key = self._next_key
+            self._next_key += 1
+            self._staging[key] = []
+        for tasks in self._tasks:
+            tasks.put(("stage", key, elements, None))
+        with self._lock:
+            while len(self._staging[key]) < len(self._processes):
+                if not all(process.is_alive() for process in self._processes):
+                    raise RuntimeError("a worker process ended while staging")
+                self._lock.wait(timeout=1.0)
+            errors = [m for m in self._staging.pop(key) if m is not None]
+        if errors:
+            raise RuntimeError(f"staging failed: {errors[0]}")
+        return StagedSource(key, len(elements))
+
+    def release(self, handle):
+        """Free a staged source in every worker process."""
+        for tasks in self._tasks:
+            tasks.put(("release", handle.key, None, None))
+
+    def map(self, sources, **call_kwargs):
+        futures = [self.submit(source, **call_kwargs) for source in sources]
+        return [f.result() for f in futures]
+
+    def _dispatch(self):
+        while True:
+            item = self._waiting.get()
+            if item is None:
+                return
+            future, elements, call_kwargs, need = item
+            with self._lock:
+                self._idle = [i for i in self._idle if self._processes[i].is_alive()]
+                while not self._idle or (self._inflight > 0
+                                         and self._inflight + need > self.budget):
+                    if not any(process.is_alive() for process in self._processes):
+                        break
+                    self._lock.wait(timeout=1.0)
+                    self._idle = [i for i in self._idle if self._processes[i].is_alive()]
+                if not self._idle:
+                    future.latency_s = time.perf_counter() - future.submitted
+                    future.set_exception(RuntimeError("no worker process is running"))
+                    continue
+                slot = self._idle.pop(0)
+                self._inflight += need
+                key = self._next_key
+                self._next_key += 1
+                self._pending[key] = (future, need, slot)
+            self._tasks[slot].put(("run", key, elements, call_kwargs))
+
+    def _collect(self):
+        errors = {"ValueError": ValueError, "RuntimeError": RuntimeError,
+                  "TypeError": TypeError, "MemoryError": MemoryError}
+        while True:
+            try:
+                kind, slot, key, payload = self._results.get(timeout=1.0)
+            except queue.Empty:
+                self._fail_dead()
+                continue
+            if kind == "stop":
+                return
+            if kind in ("staged", "stage_error"):
+                with self._lock:
+                    self._staging[key].append(payload)
+                    self._lock.notify_all()
+                continue
+            with self._lock:
+                future, need, _ = self._pending.pop(key)
+                self._inflight -= need
+                self._idle.append(slot)
+                self._lock.notify_all()
+            future.latency_s = time.perf_counter() - future.submitted
+            if kind == "done":
+                future.set_result(_unpack_response(payload, self.detector))
+            else:
+                name, message = payload
+                error = errors.get(name)
+                future.set_exception(error(message) if error is not None
+                                     else RuntimeError(f"{name}: {message}"))
+
+    def _fail_dead(self):
+        """Events held by a worker process that has ended fail with an error;
+        the process is not replaced."""
+        with self._lock:
+            dead = {i for i, process in enumerate(self._processes) if not process.is_alive()}
+            lost = [key for key, (_, _, slot) in self._pending.items() if slot in dead]
+            for key in lost:
+                future, need, slot = self._pending.pop(key)
+                self._inflight -= need
+                future.latency_s = time.perf_counter() - future.submitted
+                future.set_exception(RuntimeError(f"worker process {slot} ended "
+                                                  f"(exit code {self._processes[slot].exitcode})"))
+            if lost:
+                self._lock.notify_all()
+
+    def _terminate(self):
+        for process in self._processes:
+            if process.is_alive():
+                process.terminate()
+        for process in self._processes:
+            process.join(timeout=30)
+
+    def close(self):
+        if self._closed:
+            return
+        self._closed = True
+        self._waiting.put(None)
+        self._dispatcher.join()
+        with self._lock:
+            while self._pending:
+                self._lock.wait()
+        for tasks in self._tasks:
+            tasks.put(None)
+        for process in self._processes:
+            process.join(timeout=60)
+        self._results.put(("stop", None, None, None))
+        self._collector.join()
+        self._terminate()
 
     def __enter__(self):
         return self

@@ -28,3 +28,8 @@ def near_event(om=20, elements=400, impact=(1.0, 0.3, 0.0), seed=5):
     shower = SyntheticShower.gaussian(pose, charged_track_length_m=30.0, elements=elements,
                                       seed=seed)
     return w, replace(shower.elements, cone_model="spectral")
+
+
+def demonstration_route_only(device="cuda"):
+    """The demonstration route alone (a picklable factory for worker processes)."""
+    return demonstration_route(device)[3]

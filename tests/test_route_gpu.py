@@ -171,3 +171,36 @@ def test_served_results_match_alone(demonstration, device):
         assert np.array_equal(r.components_pe, a.components_pe)
         assert np.array_equal(r.active, a.active)
     assert all(f.latency_s > 0 for f in futures)
+
+
+def test_process_served_results_match_alone(demonstration, device):
+    """Events served by two worker processes, each with its own route on the
+    same GPU, carry the result the parent's route gives them alone."""
+    import functools
+    from dataclasses import replace
+
+    from route_cases import demonstration_route_only
+
+    from lighthit_amd.serve import ProcessShowerServer
+    w, route, response, want = demonstration
+    events = [w.elements]
+    for shift in ([4.0, -3.0, 2.0], [-6.0, 5.0, -1.0]):
+        events.append(replace(w.elements, start_m=w.elements.start_m + np.asarray(shift)))
+    alone = [route(e) for e in events]
+    with ProcessShowerServer(functools.partial(demonstration_route_only, "cuda:0"), processes=2,
+                             detector=route.kernel.detector, device="cuda:0",
+                             threads_per_process=2) as server:
+        # host arrays for the first three, sources staged in device memory after
+        staged = [server.stage(e) for e in events]
+        futures = [server.submit(events[i % 3] if i < 3 else staged[i % 3]) for i in range(6)]
+        mixed = [f.result(timeout=300) for f in futures]
+        for handle in staged:
+            server.release(handle)
+    for i, r in enumerate(mixed):
+        a = alone[i % 3]
+        assert np.array_equal(r.charge_components_pe, a.charge_components_pe)
+        assert np.array_equal(r.components_pe, a.components_pe)
+        assert np.array_equal(r.spectrum_ge2_pe, a.spectrum_ge2_pe)
+        assert np.array_equal(r.active, a.active)
+        assert np.array_equal(r.time_origin_ns, a.time_origin_ns)
+    assert all(f.latency_s > 0 for f in futures)
