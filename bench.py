@@ -53,8 +53,9 @@ O1_PANEL_FLOPS = 188
 O1_EVENT_FLOPS = 160
 
 #: Kernels the library times with HIP events on their stream.
-TIMED_KERNELS = ("order0_pairs<false>", "order0_pairs<true>", "o1_batches", "o1_group_sums",
-                 "ge2_wide", "ge2_narrow", "readout_kernel", "compile_deposit", "compile_phase")
+TIMED_KERNELS = ("order0_pairs<false>", "order0_pairs<true>", "o1_batch_profile",
+                 "o1_batch_deposit", "o1_batches", "o1_group_sums", "ge2_wide", "ge2_narrow",
+                 "readout_kernel", "compile_deposit", "compile_phase")
 
 
 def build_route(name):
@@ -298,6 +299,21 @@ def roofline_of(name, kernel_ms, workload, response, route, counts=None, launche
                 "frac": achieved / HBM_PEAK_GBS,
                 "algorithmic": f"48 B/element x {n_el} + 36 B/OM x {oms} + 8 B x {max(1, bins)} "
                                f"outputs/OM"}
+    if name in ("o1_batch_deposit", "o1_batch_profile") and counts is not None:
+        terms, panels, events = (c / launches for c in counts)
+        if name == "o1_batch_profile":
+            flops = O1_PROFILE_TERM_FLOPS * terms
+            algorithmic = (f"per launch {terms:.4g} profile terms x {O1_PROFILE_TERM_FLOPS} "
+                           f"FP64 flops (counted in the kernel)")
+        else:
+            flops = O1_PANEL_FLOPS * panels + O1_EVENT_FLOPS * events
+            algorithmic = (f"per launch {panels:.4g} panels x {O1_PANEL_FLOPS} + {events:.4g} "
+                           f"bin events x {O1_EVENT_FLOPS} FP64 flops (counted in the kernel)")
+        achieved = flops / (kernel_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "algorithmic": algorithmic,
+                "note": "FP64 vector work (no matrix form); priced against the FP64 peak"}
     if name == "o1_batches" and counts is not None:
         terms, panels, events = (c / launches for c in counts)
         flops = (O1_PROFILE_TERM_FLOPS * terms + O1_PANEL_FLOPS * panels

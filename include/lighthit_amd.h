@@ -277,8 +277,13 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
  * lh_order1_group_keys writes the number of (tau, r, theta) groups to info[0]
  * and the bits of its compact per-OM keys to info[1] (it returns after the
  * stream has drained); lh_order1_group_charge takes both, a scratch buffer of
- * lh_order1_group_scratch_bytes(pencils, oms, optics, groups) and writes
- * charge (and bins) of the listed OMs. Its optional work_counts (3 device
+ * lh_order1_group_scratch_bytes(pencils, oms, optics, groups, pool_nodes) and
+ * writes charge (and bins) of the listed OMs. The scratch beyond the fixed
+ * part is a pool of path nodes (24 bytes each; pool_nodes -1 asks for the
+ * default size, 0 for none): when every batch's path fits it, the batches run
+ * as wide kernels (one wave per batch, then one per (OM, segment, wavelength));
+ * otherwise one workgroup per (OM, segment) does it all. Both give the same
+ * sums in the same order. Its optional work_counts (3 device
  * counters, added to) receive the number of profile terms (path nodes x
  * groups), time panels and bin events it evaluated. Each pair shares its
  * workspace. lh_order1_set_tables uploads the 24-term Taylor table of
@@ -369,7 +374,8 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
                          const lh_order1_optics* optics, const lh_order1_merge* merge, int* info,
                          void* workspace, size_t workspace_bytes, void* stream);
 size_t lh_order1_group_scratch_bytes(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
-                                     const lh_order1_optics* optics, int groups);
+                                     const lh_order1_optics* optics, int groups,
+                                     long long pool_nodes);
 int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
                            const lh_order1_optics* optics, const lh_order1_merge* merge,
                            int groups, int key_bits, void* scratch, size_t scratch_bytes,

@@ -422,6 +422,8 @@ __global__ void __launch_bounds__(kThreads) o1_parent_links(const int* flag, con
 
 // ------------------------------------------------------------------- grouped
 
+struct BatchHead;
+
 struct Group {
     lh_order1_pencils pen;
     lh_order1_oms om;
@@ -455,7 +457,14 @@ struct Group {
     double* part_acc;         // (work, segments, bands, bins, 4) partial deposits
     double* part_pre;         // (work, segments, bands, 3)
     double* part_charge;      // (work, segments, bands)
-    int segments;             // batch segments per OM (one wave each)
+    int segments;             // batch segments per OM
+    BatchHead* heads;         // (groups + 1) batch headers           caller scratch
+    long long* node_n;        // (groups + 1) path nodes per batch
+    long long* node_off;      // (groups + 1) first node of every batch in the pool
+    double* pool_path;        // (pool_nodes) path nodes of every batch
+    double* pool_g0;          // (pool_nodes) profile sums
+    double* pool_g2;
+    long long pool_nodes;     // nodes the pool holds (0: the single-kernel form)
     unsigned long long* work_counts;  // optional: profile terms, panels, bin events
     double* prepared;         // (n_pen, 7)
     double* lambda_bins;      // (work, bands, bins)
@@ -1149,7 +1158,7 @@ struct PanelEvents {
     double v_end;           // prefix: I_end
 };
 
-__device__ void panel_events(const LambdaBins& B, double t_a, double t_b, double e_a, double e_b,
+__device__ __forceinline__ void panel_events(const LambdaBins& B, double t_a, double t_b, double e_a, double e_b,
                              double alpha, double beta, double gamma, double kappa,
                              PanelEvents& out) {
     out.kind = 0;
@@ -1219,7 +1228,7 @@ __device__ inline void wave_sync() {
 // along which the bins never decrease, so every bin's events are contiguous and
 // in panel order; one lane per bin adds them, the order of the reference's
 // serial loop. Returns the number of bin events.
-__device__ int deposit_round(const PanelEvents& ev, int lane, const LambdaBins& B, int* ev_bin,
+__device__ __forceinline__ int deposit_round(const PanelEvents& ev, int lane, const LambdaBins& B, int* ev_bin,
                              double (*ev_val)[4], int* head_at) {
     int mine = 0;
     if (ev.kind == 1) mine = __popc(ev.mask);
@@ -1287,16 +1296,451 @@ __device__ int deposit_round(const PanelEvents& ev, int lane, const LambdaBins& 
     return n_events;
 }
 
-// One workgroup of 8 waves per (OM, segment): the segment's batches b = first + s,
-// first + s + S, ... (_grouped_pencils_order1_block :1924-1995 and _batch_deposit
-// :451-551) into the segment's own partial accumulators. The workgroup builds a
-// batch's path nodes and profiles in LDS; then every wave deposits one
-// wavelength: its panels are spread over the lanes 64 at a time, and every bin
-// adds its panels' events in panel order (the order of the reference's serial
-// loop, so the bins match it term by term).
+// ---- batches: header, path grid and profile, per-wavelength deposit
+
+// A batch's moment means and path grid (_grouped_pencils_order1_block
+// :1924-1960, _batch_grid :258-286). n = 0 marks a batch that carries no light.
+struct BatchHead {
+    double r_b, t_b, tau_var, sigma, u_win, u_end;
+    int n, n_in, n_out, backward;
+    int start, stop, pad0, pad1;
+};
+
+__device__ __forceinline__ BatchHead batch_head(const Group& G, int b, int lane) {
+    BatchHead h;
+    int start = G.batch_start[b], stop = G.batch_start[b + 1];
+    int n_mono = G.opt.n_mono;
+    int nv = 2 * n_mono + 6;
+    int nb = G.opt.bins;
+    h.start = start;
+    h.stop = stop;
+    h.pad0 = h.pad1 = 0;
+    h.n = h.n_in = h.n_out = 0;
+    h.r_b = h.t_b = h.tau_var = h.sigma = h.u_win = h.u_end = 0.0;
+    h.backward = ((G.sorted_keys[G.group_start[start]] & G.key_mask) / G.theta_bins) % 2 == 1;
+    double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0, theta_small = M_PI;
+    for (int q = start + lane; q < stop; q += 64) {
+        const double* row = G.values + (size_t)q * nv;
+        double omega = row[2 * n_mono];
+        if (omega > 0.0) {
+            omega_sum += omega;
+            r_sum += row[2 * n_mono + 2];
+            t_sum += row[2 * n_mono + 3];
+            tau_sum += row[2 * n_mono + 4];
+            tau2_sum += row[2 * n_mono + 5];
+            theta_small = fmin(theta_small, row[2 * n_mono + 1] / omega);
+        }
+    }
+    omega_sum = wave_sum(omega_sum);
+    r_sum = wave_sum(r_sum);
+    t_sum = wave_sum(t_sum);
+    tau_sum = wave_sum(tau_sum);
+    tau2_sum = wave_sum(tau2_sum);
+    theta_small = wave_min(theta_small);
+    if (!(omega_sum > 0.0)) return h;
+    double r_b = r_sum / omega_sum;
+    double t_b = t_sum / omega_sum;
+    double tau_mean = tau_sum / omega_sum;
+    double tau_var = fmax(tau2_sum / omega_sum - tau_mean * tau_mean, 0.0);
+    theta_small = fmax(theta_small, 1e-9);
+    double sigma;
+    if (h.backward)
+        sigma = fmax(0.5 * r_b * sin(theta_small) * G.opt.y_g,
+                     0.25 * r_b * (1.0 - cos(theta_small)));
+    else
+        sigma = fmax(0.5 * r_b * sin(theta_small)
+                         * fmax(G.opt.y_g, 0.5 * tan(0.5 * theta_small)),
+                     1e-9 * r_b);
+    double h_in = G.opt.path_step;
+    double h_out = G.screen ? G.opt.path_step : G.opt.path_step_tail;
+    double ext_end = G.opt.tail_exponent / G.mut_min;
+    double ext_win = G.screen ? ext_end : G.speed_max * (G.opt.edges[nb] - t_b) - r_b;
+    if (ext_win < 0.0) ext_win = 0.0;
+    if (ext_win > ext_end) ext_win = ext_end;
+    double u_end = asinh(ext_end / sigma);
+    double u_win = asinh(ext_win / sigma);
+    int n_in = u_win > 0.0 ? (int)ceil(u_win / h_in) : 0;
+    int n_out = u_end > u_win ? (int)ceil((u_end - u_win) / h_out) : 0;
+    int capacity = min(G.opt.max_path_nodes, kMaxPathNodes);
+    if (n_in + n_out + 1 > capacity) {
+        double scale = (capacity - 1.0) / (n_in + n_out);
+        n_in = (int)(n_in * scale);
+        n_out = capacity - 1 - n_in;
+    }
+    h.r_b = r_b;
+    h.t_b = t_b;
+    h.tau_var = tau_var;
+    h.sigma = sigma;
+    h.u_win = u_win;
+    h.u_end = u_end;
+    h.n_in = n_in;
+    h.n_out = n_out;
+    h.n = n_in + n_out + 1;
+    return h;
+}
+
+// Path node k: asinh-spaced in the window, then with the tail step.
+__device__ inline double path_node(const BatchHead& h, int k) {
+    if (k == 0) return h.r_b;
+    if (k <= h.n_in) return h.r_b + h.sigma * sinh(h.u_win * k / h.n_in);
+    return h.r_b + h.sigma * sinh(h.u_win + (h.u_end - h.u_win) * (k - h.n_in) / h.n_out);
+}
+
+// A group's direction and (for degree <= 3) its cubic acceptance coefficients:
+// A0[4] B0[3] A2[4] B2[3] sine cosine; a negative sine marks an empty group.
 template <int P>
-__global__ void __launch_bounds__(kBatchThreads) o1_batches(Group G) {
-    __shared__ double coeff[kBatchGroups][16];   // A0[4] B0[3] A2[4] B2[3] sine cosine
+__device__ inline void stage_group(const Group& G, const double* row, double* cf) {
+    int n_mono = G.opt.n_mono;
+    double omega = row[2 * n_mono];
+    if (omega <= 0.0) { cf[14] = -1.0; return; }
+    double theta = fmax(row[2 * n_mono + 1] / omega, 1e-9);
+    cf[14] = sin(theta);
+    cf[15] = cos(theta);
+    if (P <= 3) {
+        cubic_coefficients(G, row, cf, cf + 4);
+        cubic_coefficients(G, row + n_mono, cf + 7, cf + 11);
+    }
+}
+
+// One group's HG single-scattering term at path node s_k, added to the two
+// profile sums (_batch_profile, :300-370).
+template <int P>
+__device__ inline void profile_add(const Group& G, const double* cf, const double* row,
+                                   double s_k, double r_b, int backward, double gg, double g1,
+                                   double norm, double& acc0, double& acc2) {
+    double sin_t = cf[14];
+    if (sin_t < 0.0) return;
+    double cos_t = cf[15];
+    double rs = r_b * sin_t;
+    double c, sn, path_jacobian;
+    if (backward) {
+        double shift = s_k - r_b * cos_t;
+        double denominator = rs * rs + shift * shift;
+        c = (rs * rs - shift * shift) / denominator;
+        sn = 2.0 * rs * shift / denominator;
+        path_jacobian = 2.0 * r_b / denominator;
+    } else {
+        double y_theta = sin_t / (1.0 + cos_t);
+        double jac = 2.0 / (r_b * sin_t);
+        double y = (s_k - r_b) / rs + y_theta;
+        double inv = 1.0 / (1.0 + y * y);
+        c = (1.0 - y * y) * inv;
+        sn = 2.0 * y * inv;
+        path_jacobian = jac * inv;
+    }
+    double a0, a2;
+    if (P <= 3) {
+        a0 = cf[0] + c * (cf[1] + c * (cf[2] + c * cf[3]));
+        a0 += sn * (cf[4] + c * (cf[5] + c * cf[6]));
+        a2 = cf[7] + c * (cf[8] + c * (cf[9] + c * cf[10]));
+        a2 += sn * (cf[11] + c * (cf[12] + c * cf[13]));
+    } else {
+        double cpow[P + 1], spow[P + 1];
+        cpow[0] = 1.0; spow[0] = 1.0;
+#pragma unroll
+        for (int pw = 1; pw <= P; ++pw) {
+            cpow[pw] = cpow[pw - 1] * c;
+            spow[pw] = spow[pw - 1] * sn;
+        }
+        a0 = 0.0; a2 = 0.0;
+        int m = 0;
+#pragma unroll
+        for (int kk = 0; kk <= P; ++kk) {
+#pragma unroll
+            for (int i = 0; i <= kk; ++i, ++m) {
+                double basis = G.opt.coef[m] * cpow[i] * spow[kk - i];
+                a0 += basis * row[m];
+                a2 += basis * row[Monomials<P>::N + m];
+            }
+        }
+    }
+    double den = g1 - gg * c;
+    double value = path_jacobian * norm / (den * sqrt(den));
+    acc0 += a0 * value;
+    acc2 += a2 * value;
+}
+
+// _batch_deposit (:451-551) of one batch at one wavelength: the panels are
+// spread over the lanes 64 at a time and every bin adds its panels' events in
+// panel order (the order of the reference's serial loop). acc and pre are the
+// wavelength's accumulators. Returns the wavelength's charge of the batch (the
+// same value in every lane).
+__device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const BatchHead& h, const double* path,
+                                 const double* g0, const double* g2, double* acc, double* pre,
+                                 int lane, int* ev_bin, double (*ev_val)[4], int* head_at,
+                                 unsigned long long& count_panels,
+                                 unsigned long long& count_events) {
+    int nb = G.opt.bins;
+    int n = h.n;
+    double r_b = h.r_b, t_b = h.t_b;
+    double v = G.opt.speed[lam];
+    double mut = G.opt.mut[lam];
+    double kappa = -mut * v;
+    // last = the serial walk-back `while (last > 2 && path[last - 1] >
+    // limit) --last` from n - 1; path increases, so it is the count of
+    // path[0 .. n - 2] within the limit, clamped to [2, n - 1]
+    double limit = r_b + G.opt.tail_exponent / mut;
+    int within = 0;
+    for (int k0 = 0; k0 < n - 1; k0 += 64) {
+        int k = k0 + lane;
+        within += __popcll(__ballot(k < n - 1 && path[k] <= limit));
+    }
+    int last = min(max(within, 2), n - 1);
+    if (last < 2) return 0.0;
+    double spread = fmin(sqrt(12.0 * h.tau_var), (path[last] - r_b) / v);
+    double front_time = t_b + r_b / v;
+    double end_time = t_b + path[last] / v;
+    int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
+                  || end_time + 0.5 * spread <= G.opt.edges[0];
+    double t_base = outside ? 0.0 : t_b;
+    double e_emit = exp(kappa * t_base);
+    double h_scale = G.opt.mus[lam] * v / e_emit;
+    double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
+    LambdaBins B;
+    B.inverse_width = G.opt.bin_inverse_width;
+    B.edges = G.opt.edges;
+    B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
+    B.acc = acc;
+    B.pre = pre;
+    B.nb = nb;
+    B.direct_bins = G.opt.direct_bins;
+    bool bins = G.compute_bins && !outside;
+    double h_front = w0 * g0[0] + w2 * g2[0];
+    double e_first = e_emit * exp(-mut * path[0]);
+    double e_last = e_emit * exp(-mut * path[last]);
+    // panels: (2i, 2i+1, 2i+2) for i < last / 2; an odd last closes with
+    // the one-step panel at j = last - 1
+    int panels = last / 2 + (last & 1);
+    double total_lane = 0.0;
+    unsigned long long events_total = 0;
+    for (int base = 0; base < panels; base += 64) {
+        int i = base + lane;
+        PanelEvents ev;
+        ev.kind = 0;
+        if (i < panels) {
+            double t_a, t_e, alpha, beta, gamma, e_a, e_b;
+            int j;
+            if (i < last / 2) {
+                j = 2 * i;
+                t_a = t_base + path[j] / v;
+                double t_m = t_base + path[j + 1] / v;
+                t_e = t_base + path[j + 2] / v;
+                double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
+                double u_m = t_m - t_a, u_b = t_e - t_a;
+                double d1 = (h1 - h0) / u_m;
+                double d2 = ((h2 - h1) / (t_e - t_m) - d1) / u_b;
+                alpha = h0;
+                beta = d1 - d2 * u_m;
+                gamma = d2;
+                e_b = e_emit * exp(-mut * path[j + 2]);
+            } else {
+                j = last - 1;
+                double t_p = t_base + path[j - 1] / v;
+                t_a = t_base + path[j] / v;
+                t_e = t_base + path[j + 1] / v;
+                double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
+                double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                double u_m = t_a - t_p, u_b = t_e - t_p;
+                double d1 = (h0 - hp) / u_m;
+                double d2 = ((h1 - h0) / (t_e - t_a) - d1) / u_b;
+                alpha = h0;
+                beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
+                gamma = d2;
+                e_b = e_emit * exp(-mut * path[j + 1]);
+            }
+            // the serial loop carries e_a from the previous panel's e_b
+            // (the same expression at node j)
+            e_a = j == 0 ? e_first : e_emit * exp(-mut * path[j]);
+            total_lane += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
+            if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
+        }
+        if (!bins) continue;
+        events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
+    }
+    double total = wave_sum(total_lane);
+    double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
+    double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
+    total += exact;
+    // the front ramp and flat tail after every panel, as one more round
+    // (lanes 0 and 1; their times increase, so the round's bins do too)
+    if (bins) {
+        PanelEvents ev;
+        ev.kind = 0;
+        if (spread > 0.0) {
+            double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
+            double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
+            double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
+            double flat = t_end > t_2
+                              ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa, e_last)
+                              : 0.0;
+            double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
+            if (lane == 0)
+                panel_events(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa, ev);
+            else if (lane == 1 && t_end > t_2)
+                panel_events(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa, ev);
+        } else if (lane == 0 && t_end > t_f) {
+            panel_events(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa, ev);
+        }
+        events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
+    }
+    count_panels += (unsigned long long)panels;
+    count_events += events_total;
+    return total;
+}
+
+// The batch kernels run in one of two forms. The wide form (when the caller's
+// scratch holds every batch's path nodes): o1_batch_heads (one wave per
+// batch: moment means and node count), a scan of the node counts into the
+// node pool, o1_batch_profile (one wave per batch: path nodes and profile
+// sums into the pool) and o1_batch_deposit (one wave per (OM, segment,
+// wavelength)). Otherwise o1_batches does all of it with one workgroup per
+// (OM, segment) and the path in LDS. Both sum every slot's batches in the same
+// order.
+constexpr int kWideWaves = 4;
+constexpr int kWideThreads = 64 * kWideWaves;
+constexpr int kWideMaxBlocks = 2048;   // grid-stride bound of the per-batch kernels
+
+__device__ inline bool wide_form(const Group& G, int groups) {
+    return G.node_off[groups] <= G.pool_nodes;
+}
+
+// Headers and node counts of every batch; node_n is 0 from the last batch to groups.
+__global__ void __launch_bounds__(kWideThreads) o1_batch_heads(Group G, int groups) {
+    int lane = threadIdx.x & 63;
+    int wave = blockIdx.x * kWideWaves + (threadIdx.x >> 6);
+    int waves = gridDim.x * kWideWaves;
+    int batches = G.om_batches[G.om.work_count];
+    for (int b = wave; b <= groups; b += waves) {
+        if (b < batches) {
+            BatchHead h = batch_head(G, b, lane);
+            if (lane == 0) {
+                G.heads[b] = h;
+                G.node_n[b] = h.n;
+            }
+        } else if (lane == 0) {
+            G.node_n[b] = 0;
+        }
+    }
+}
+
+// Path nodes and the profile of every batch, one wave per batch: each lane
+// owns nodes lane, lane + 64, ... and sums the batch's groups in key order.
+template <int P>
+__global__ void __launch_bounds__(kWideThreads) o1_batch_profile(Group G, int groups) {
+    __shared__ double coeff_all[kWideWaves][kBatchGroups][16];
+    __shared__ unsigned long long s_terms;
+    int tid = threadIdx.x, lane = tid & 63;
+    int wave = blockIdx.x * kWideWaves + (tid >> 6);
+    int waves = gridDim.x * kWideWaves;
+    double (*coeff)[16] = coeff_all[tid >> 6];
+    if (tid == 0) s_terms = 0;
+    __syncthreads();
+    bool wide = wide_form(G, groups);
+    int batches = G.om_batches[G.om.work_count];
+    int nv = 2 * G.opt.n_mono + 6;
+    double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
+    double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
+    unsigned long long terms = 0;
+    for (int b = wave; wide && b < batches; b += waves) {
+        BatchHead h = G.heads[b];
+        if (h.n == 0) continue;
+        int n = h.n;
+        size_t off = (size_t)G.node_off[b];
+        double* path = G.pool_path + off;
+        double* g0 = G.pool_g0 + off;
+        double* g2 = G.pool_g2 + off;
+        for (int k = lane; k < n; k += 64) path[k] = path_node(h, k);
+        terms += (unsigned long long)n * (unsigned long long)(h.stop - h.start);
+        for (int c0 = h.start; c0 < h.stop; c0 += kBatchGroups) {
+            int chunk = min(kBatchGroups, h.stop - c0);
+            wave_sync();   // the previous chunk's coefficients have been read
+            for (int q = lane; q < chunk; q += 64)
+                stage_group<P>(G, G.values + (size_t)(c0 + q) * nv, coeff[q]);
+            wave_sync();
+            for (int k = lane; k < n; k += 64) {
+                double acc0 = c0 == h.start ? 0.0 : g0[k];
+                double acc2 = c0 == h.start ? 0.0 : g2[k];
+                double s_k = path_node(h, k);
+                for (int q = 0; q < chunk; ++q)
+                    profile_add<P>(G, coeff[q], G.values + (size_t)(c0 + q) * nv, s_k, h.r_b,
+                                   h.backward, gg, g1, norm, acc0, acc2);
+                g0[k] = acc0;
+                g2[k] = acc2;
+            }
+        }
+    }
+    if (G.work_counts != nullptr) {
+        if (lane == 0 && terms) atomicAdd(&s_terms, terms);
+        __syncthreads();
+        if (tid == 0 && s_terms) atomicAdd(G.work_counts, s_terms);
+    }
+}
+
+// One wave per (OM, segment, wavelength): the segment's batches b = first + s,
+// first + s + S, ... deposited in order into the slot's accumulators of that
+// wavelength.
+__global__ void __launch_bounds__(kWideThreads) o1_batch_deposit(Group G, int groups) {
+    __shared__ int ev_bin_all[kWideWaves][64 * kMaxDirect];
+    __shared__ double ev_val_all[kWideWaves][64 * kMaxDirect][4];
+    __shared__ int head_at_all[kWideWaves][64 * kMaxDirect];
+    __shared__ unsigned long long s_counts[2];
+    int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid < 2) s_counts[tid] = 0;
+    __syncthreads();
+    int nl = G.opt.bands, nb = G.opt.bins, S = G.segments;
+    long long gw = (long long)blockIdx.x * kWideWaves + wv;
+    long long slots = (long long)G.om.work_count * S;
+    unsigned long long count_panels = 0, count_events = 0;
+    if (wide_form(G, groups) && gw < slots * nl) {
+        int lam = (int)(gw % nl);
+        long long slot = gw / nl;
+        int w = (int)(slot / S), seg = (int)(slot % S);
+        double* acc = G.part_acc + ((size_t)slot * nl + lam) * nb * 4;
+        double* pre = G.part_pre + (size_t)slot * nl * 3 + 3 * lam;
+        if (G.compute_bins)
+            for (int i = lane; i < nb * 4; i += 64) acc[i] = 0.0;
+        if (lane < 3) pre[lane] = 0.0;
+        // the zeros are stored before any lane of this wave adds to them
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        wave_sync();
+        double charge = 0.0;
+        if (G.opt.mus[lam] != 0.0) {
+            for (int b = G.om_batches[w] + seg; b < G.om_batches[w + 1]; b += S) {
+                BatchHead h = G.heads[b];
+                if (h.n == 0) continue;
+                size_t off = (size_t)G.node_off[b];
+                charge += deposit_lambda(G, lam, h, G.pool_path + off, G.pool_g0 + off,
+                                         G.pool_g2 + off, acc, pre, lane, ev_bin_all[wv],
+                                         ev_val_all[wv], head_at_all[wv], count_panels,
+                                         count_events);
+                wave_sync();
+            }
+        }
+        if (lane == 0) G.part_charge[slot * nl + lam] = charge;
+    }
+    if (G.work_counts != nullptr) {
+        if (lane == 0 && (count_panels || count_events)) {
+            atomicAdd(&s_counts[0], count_panels);
+            atomicAdd(&s_counts[1], count_events);
+        }
+        __syncthreads();
+        if (tid == 0 && (s_counts[0] || s_counts[1])) {
+            atomicAdd(G.work_counts + 1, s_counts[0]);
+            atomicAdd(G.work_counts + 2, s_counts[1]);
+        }
+    }
+}
+
+// The single-kernel form: one workgroup of 8 waves per (OM, segment) builds a
+// batch's path nodes and profile in LDS; then every wave deposits one
+// wavelength. It runs when the node pool cannot hold every batch's path.
+template <int P>
+__global__ void __launch_bounds__(kBatchThreads) o1_batches(Group G, int groups) {
+    if (wide_form(G, groups)) return;
+    __shared__ double coeff[kBatchGroups][16];
     __shared__ double path[kMaxPathNodes];
     __shared__ double g0[kMaxPathNodes];
     __shared__ double g2[kMaxPathNodes];
@@ -1304,14 +1748,14 @@ __global__ void __launch_bounds__(kBatchThreads) o1_batches(Group G) {
     __shared__ double ev_val_all[kBatchWaves][64 * kMaxDirect][4];
     __shared__ int head_at_all[kBatchWaves][64 * kMaxDirect];
     __shared__ double charge_acc[kMaxBands];
+    __shared__ unsigned long long s_counts[3];
     int w = blockIdx.x, seg = blockIdx.y;
     int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int* ev_bin = ev_bin_all[wave];
     double (*ev_val)[4] = ev_val_all[wave];
     int* head_at = head_at_all[wave];
     int nl = G.opt.bands, nb = G.opt.bins;
-    int n_mono = G.opt.n_mono;
-    int nv = 2 * n_mono + 6;
+    int nv = 2 * G.opt.n_mono + 6;
     size_t slot = (size_t)w * G.segments + seg;
     double* acc = G.part_acc + slot * nl * nb * 4;
     double* pre = G.part_pre + slot * nl * 3;
@@ -1319,282 +1763,49 @@ __global__ void __launch_bounds__(kBatchThreads) o1_batches(Group G) {
         for (size_t i = tid; i < (size_t)nl * nb * 4; i += kBatchThreads) acc[i] = 0.0;
     for (int i = tid; i < nl * 3; i += kBatchThreads) pre[i] = 0.0;
     if (tid < nl) charge_acc[tid] = 0.0;
+    if (tid < 3) s_counts[tid] = 0;
     __syncthreads();
     unsigned long long count_terms = 0, count_panels = 0, count_events = 0;
-    constexpr bool cubic = P <= 3;
     double gg = 2.0 * G.opt.g, g1 = 1.0 + G.opt.g * G.opt.g;
     double norm = (1.0 - G.opt.g * G.opt.g) / (4.0 * M_PI);
     for (int b = G.om_batches[w] + seg; b < G.om_batches[w + 1]; b += G.segments) {
-        int start = G.batch_start[b], stop = G.batch_start[b + 1];
-        int backward = ((G.sorted_keys[G.group_start[start]] & G.key_mask) / G.theta_bins) % 2
-                       == 1;
-        double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0, theta_small = M_PI;
-        for (int q = start + lane; q < stop; q += 64) {
-            const double* row = G.values + (size_t)q * nv;
-            double omega = row[2 * n_mono];
-            if (omega > 0.0) {
-                omega_sum += omega;
-                r_sum += row[2 * n_mono + 2];
-                t_sum += row[2 * n_mono + 3];
-                tau_sum += row[2 * n_mono + 4];
-                tau2_sum += row[2 * n_mono + 5];
-                theta_small = fmin(theta_small, row[2 * n_mono + 1] / omega);
-            }
-        }
-        omega_sum = wave_sum(omega_sum);
-        r_sum = wave_sum(r_sum);
-        t_sum = wave_sum(t_sum);
-        tau_sum = wave_sum(tau_sum);
-        tau2_sum = wave_sum(tau2_sum);
-        theta_small = wave_min(theta_small);
-        if (!(omega_sum > 0.0)) continue;
-        double r_b = r_sum / omega_sum;
-        double t_b = t_sum / omega_sum;
-        double tau_mean = tau_sum / omega_sum;
-        double tau_var = fmax(tau2_sum / omega_sum - tau_mean * tau_mean, 0.0);
-        theta_small = fmax(theta_small, 1e-9);
-        double sigma;
-        if (backward)
-            sigma = fmax(0.5 * r_b * sin(theta_small) * G.opt.y_g,
-                         0.25 * r_b * (1.0 - cos(theta_small)));
-        else
-            sigma = fmax(0.5 * r_b * sin(theta_small)
-                             * fmax(G.opt.y_g, 0.5 * tan(0.5 * theta_small)),
-                         1e-9 * r_b);
-        // _batch_grid (:258-286)
-        double h_in = G.opt.path_step;
-        double h_out = G.screen ? G.opt.path_step : G.opt.path_step_tail;
-        double ext_end = G.opt.tail_exponent / G.mut_min;
-        double ext_win = G.screen ? ext_end : G.speed_max * (G.opt.edges[nb] - t_b) - r_b;
-        if (ext_win < 0.0) ext_win = 0.0;
-        if (ext_win > ext_end) ext_win = ext_end;
-        double u_end = asinh(ext_end / sigma);
-        double u_win = asinh(ext_win / sigma);
-        int n_in = u_win > 0.0 ? (int)ceil(u_win / h_in) : 0;
-        int n_out = u_end > u_win ? (int)ceil((u_end - u_win) / h_out) : 0;
-        int capacity = min(G.opt.max_path_nodes, kMaxPathNodes);
-        if (n_in + n_out + 1 > capacity) {
-            double scale = (capacity - 1.0) / (n_in + n_out);
-            n_in = (int)(n_in * scale);
-            n_out = capacity - 1 - n_in;
-        }
-        int n = n_in + n_out + 1;
-        count_terms += (unsigned long long)n * (unsigned long long)(stop - start);
+        BatchHead h = batch_head(G, b, lane);
+        if (h.n == 0) continue;
+        int n = h.n;
+        count_terms += (unsigned long long)n * (unsigned long long)(h.stop - h.start);
         __syncthreads();   // the previous batch's deposits have read the path
         for (int k = tid; k < n; k += kBatchThreads) {
-            double sv;
-            if (k == 0) sv = r_b;
-            else if (k <= n_in) sv = r_b + sigma * sinh(u_win * k / n_in);
-            else sv = r_b + sigma * sinh(u_win + (u_end - u_win) * (k - n_in) / n_out);
-            path[k] = sv;
+            path[k] = path_node(h, k);
             g0[k] = 0.0;
             g2[k] = 0.0;
         }
         // profile: every path node sums its batch's groups in key order
-        for (int c0 = start; c0 < stop; c0 += kBatchGroups) {
-            int chunk = min(kBatchGroups, stop - c0);
+        for (int c0 = h.start; c0 < h.stop; c0 += kBatchGroups) {
+            int chunk = min(kBatchGroups, h.stop - c0);
             __syncthreads();
-            for (int q = tid; q < chunk; q += kBatchThreads) {
-                const double* row = G.values + (size_t)(c0 + q) * nv;
-                double omega = row[2 * n_mono];
-                double* cf = coeff[q];
-                if (omega <= 0.0) { cf[14] = -1.0; continue; }
-                double theta = fmax(row[2 * n_mono + 1] / omega, 1e-9);
-                cf[14] = sin(theta);
-                cf[15] = cos(theta);
-                if (cubic) {
-                    cubic_coefficients(G, row, cf, cf + 4);
-                    cubic_coefficients(G, row + n_mono, cf + 7, cf + 11);
-                }
-            }
+            for (int q = tid; q < chunk; q += kBatchThreads)
+                stage_group<P>(G, G.values + (size_t)(c0 + q) * nv, coeff[q]);
             __syncthreads();
             for (int k = tid; k < n; k += kBatchThreads) {
                 double acc0 = g0[k], acc2 = g2[k];
                 double s_k = path[k];
-                for (int q = 0; q < chunk; ++q) {
-                    const double* cf = coeff[q];
-                    double sin_t = cf[14];
-                    if (sin_t < 0.0) continue;
-                    double cos_t = cf[15];
-                    double rs = r_b * sin_t;
-                    double c, sn, path_jacobian;
-                    if (backward) {
-                        double shift = s_k - r_b * cos_t;
-                        double denominator = rs * rs + shift * shift;
-                        c = (rs * rs - shift * shift) / denominator;
-                        sn = 2.0 * rs * shift / denominator;
-                        path_jacobian = 2.0 * r_b / denominator;
-                    } else {
-                        double y_theta = sin_t / (1.0 + cos_t);
-                        double jac = 2.0 / (r_b * sin_t);
-                        double y = (s_k - r_b) / rs + y_theta;
-                        double inv = 1.0 / (1.0 + y * y);
-                        c = (1.0 - y * y) * inv;
-                        sn = 2.0 * y * inv;
-                        path_jacobian = jac * inv;
-                    }
-                    double a0, a2;
-                    if (cubic) {
-                        a0 = cf[0] + c * (cf[1] + c * (cf[2] + c * cf[3]));
-                        a0 += sn * (cf[4] + c * (cf[5] + c * cf[6]));
-                        a2 = cf[7] + c * (cf[8] + c * (cf[9] + c * cf[10]));
-                        a2 += sn * (cf[11] + c * (cf[12] + c * cf[13]));
-                    } else {
-                        const double* row = G.values + (size_t)(c0 + q) * nv;
-                        double cpow[P + 1], spow[P + 1];
-                        cpow[0] = 1.0; spow[0] = 1.0;
-#pragma unroll
-                        for (int pw = 1; pw <= P; ++pw) {
-                            cpow[pw] = cpow[pw - 1] * c;
-                            spow[pw] = spow[pw - 1] * sn;
-                        }
-                        a0 = 0.0; a2 = 0.0;
-                        int m = 0;
-#pragma unroll
-                        for (int kk = 0; kk <= P; ++kk) {
-#pragma unroll
-                            for (int i = 0; i <= kk; ++i, ++m) {
-                                double basis = G.opt.coef[m] * cpow[i] * spow[kk - i];
-                                a0 += basis * row[m];
-                                a2 += basis * row[Monomials<P>::N + m];
-                            }
-                        }
-                    }
-                    double den = g1 - gg * c;
-                    double value = path_jacobian * norm / (den * sqrt(den));
-                    acc0 += a0 * value;
-                    acc2 += a2 * value;
-                }
+                for (int q = 0; q < chunk; ++q)
+                    profile_add<P>(G, coeff[q], G.values + (size_t)(c0 + q) * nv, s_k, h.r_b,
+                                   h.backward, gg, g1, norm, acc0, acc2);
                 g0[k] = acc0;
                 g2[k] = acc2;
             }
         }
         __syncthreads();
-        // _batch_deposit (:451-551): one wavelength per wave
         for (int lam = wave; lam < nl; lam += kBatchWaves) {
             if (G.opt.mus[lam] == 0.0) continue;
-            double v = G.opt.speed[lam];
-            double mut = G.opt.mut[lam];
-            double kappa = -mut * v;
-            // last = the serial walk-back `while (last > 2 && path[last - 1] >
-            // limit) --last` from n - 1; path increases, so it is the count of
-            // path[0 .. n - 2] within the limit, clamped to [2, n - 1]
-            double limit = r_b + G.opt.tail_exponent / mut;
-            int within = 0;
-            for (int k0 = 0; k0 < n - 1; k0 += 64) {
-                int k = k0 + lane;
-                within += __popcll(__ballot(k < n - 1 && path[k] <= limit));
-            }
-            int last = min(max(within, 2), n - 1);
-            if (last < 2) continue;
-            double spread = fmin(sqrt(12.0 * tau_var), (path[last] - r_b) / v);
-            double front_time = t_b + r_b / v;
-            double end_time = t_b + path[last] / v;
-            int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
-                          || end_time + 0.5 * spread <= G.opt.edges[0];
-            double t_base = outside ? 0.0 : t_b;
-            double e_emit = exp(kappa * t_base);
-            double h_scale = G.opt.mus[lam] * v / e_emit;
-            double w0 = G.opt.s0w[lam] * h_scale, w2 = G.opt.s2w[lam] * h_scale;
-            LambdaBins B;
-            B.inverse_width = G.opt.bin_inverse_width;
-            B.edges = G.opt.edges;
-            B.eedge = G.opt.eedge + (size_t)lam * (nb + 1);
-            B.acc = acc + (size_t)lam * nb * 4;
-            B.pre = pre + 3 * lam;
-            B.nb = nb;
-            B.direct_bins = G.opt.direct_bins;
-            bool bins = G.compute_bins && !outside;
-            double h_front = w0 * g0[0] + w2 * g2[0];
-            double e_first = e_emit * exp(-mut * path[0]);
-            double e_last = e_emit * exp(-mut * path[last]);
-            // panels: (2i, 2i+1, 2i+2) for i < last / 2; an odd last closes with
-            // the one-step panel at j = last - 1
-            int panels = last / 2 + (last & 1);
-            double total_lane = 0.0;
-            unsigned long long events_total = 0;
-            for (int base = 0; base < panels; base += 64) {
-                int i = base + lane;
-                PanelEvents ev;
-                ev.kind = 0;
-                if (i < panels) {
-                    double t_a, t_e, alpha, beta, gamma, e_a, e_b;
-                    int j;
-                    if (i < last / 2) {
-                        j = 2 * i;
-                        t_a = t_base + path[j] / v;
-                        double t_m = t_base + path[j + 1] / v;
-                        t_e = t_base + path[j + 2] / v;
-                        double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
-                        double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
-                        double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
-                        double u_m = t_m - t_a, u_b = t_e - t_a;
-                        double d1 = (h1 - h0) / u_m;
-                        double d2 = ((h2 - h1) / (t_e - t_m) - d1) / u_b;
-                        alpha = h0;
-                        beta = d1 - d2 * u_m;
-                        gamma = d2;
-                        e_b = e_emit * exp(-mut * path[j + 2]);
-                    } else {
-                        j = last - 1;
-                        double t_p = t_base + path[j - 1] / v;
-                        t_a = t_base + path[j] / v;
-                        t_e = t_base + path[j + 1] / v;
-                        double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
-                        double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
-                        double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
-                        double u_m = t_a - t_p, u_b = t_e - t_p;
-                        double d1 = (h0 - hp) / u_m;
-                        double d2 = ((h1 - h0) / (t_e - t_a) - d1) / u_b;
-                        alpha = h0;
-                        beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
-                        gamma = d2;
-                        e_b = e_emit * exp(-mut * path[j + 1]);
-                    }
-                    // the serial loop carries e_a from the previous panel's e_b
-                    // (the same expression at node j)
-                    e_a = j == 0 ? e_first : e_emit * exp(-mut * path[j]);
-                    total_lane += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
-                    if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
-                }
-                if (!bins) continue;
-                events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
-            }
-            double total = wave_sum(total_lane);
-            double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
-            double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
-            total += exact;
-            // the front ramp and flat tail after every panel, as one more round
-            // (lanes 0 and 1; their times increase, so the round's bins do too)
-            if (bins) {
-                PanelEvents ev;
-                ev.kind = 0;
-                if (spread > 0.0) {
-                    double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
-                    double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
-                    double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
-                    double flat = t_end > t_2
-                                      ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa,
-                                                      e_last)
-                                      : 0.0;
-                    double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
-                    if (lane == 0)
-                        panel_events(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa, ev);
-                    else if (lane == 1 && t_end > t_2)
-                        panel_events(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa, ev);
-                } else if (lane == 0 && t_end > t_f) {
-                    panel_events(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa, ev);
-                }
-                events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
-            }
-            count_panels += (unsigned long long)panels;
-            count_events += events_total;
+            double total = deposit_lambda(G, lam, h, path, g0, g2, acc + (size_t)lam * nb * 4,
+                                          pre + 3 * lam, lane, ev_bin, ev_val, head_at,
+                                          count_panels, count_events);
             if (lane == 0) charge_acc[lam] += total;
             wave_sync();
         }
     }
-    __shared__ unsigned long long s_counts[3];
-    if (tid < 3) s_counts[tid] = 0;
     __syncthreads();
     if (tid < nl) G.part_charge[slot * nl + tid] = charge_acc[tid];
     if (G.work_counts != nullptr) {
@@ -1852,29 +2063,35 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
     return LH_OK;
 }
 
-// Batch segments per OM: about 2048 workgroups in all, within 2 GiB of per-slot
-// partial deposits.
+// Batch segments per OM: about 8192 deposit waves (OM x segment x wavelength)
+// in all, within 2 GiB of per-slot partial deposits.
 int segments_for(int work, int bands, int bins) {
-    int S = std::max(1, std::min(64, (2048 + work - 1) / work));
+    long long lanes = (long long)work * std::max(bands, 1);
+    int S = (int)std::max(1LL, std::min(64LL, (8192 + lanes - 1) / lanes));
     while (S > 1 && (double)work * S * bands * bins * 32.0 > 2048.0 * 1024 * 1024) S /= 2;
     return S;
 }
 
 struct ScratchLayout {
     size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
-        part_charge, piece_count, piece_start, piece_values, scan_temp, total;
+        part_charge, piece_count, piece_start, piece_values, heads, node_n, node_off,
+        scan_temp, total;   // total: the fixed part; the node pool follows it
     int segments;
     long long pieces;         // bound on the group-sum pieces
 };
 
 int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mono, int groups,
                    ScratchLayout& L) {
-    size_t scan_temp = 0;
+    size_t scan_temp = 0, scan_nodes = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
-                                         groups + 1, (hipStream_t)0) != hipSuccess) {
+                                         groups + 1, (hipStream_t)0) != hipSuccess ||
+        hipcub::DeviceScan::ExclusiveSum(nullptr, scan_nodes, (long long*)nullptr,
+                                         (long long*)nullptr, groups + 1,
+                                         (hipStream_t)0) != hipSuccess) {
         lh::set_error("scan workspace query failed");
         return LH_ERR_HIP;
     }
+    scan_temp = std::max(scan_temp, scan_nodes);
     L.segments = segments_for(work, bands, bins);
     size_t parts = (size_t)work * L.segments;
     size_t at = 0;
@@ -1892,9 +2109,25 @@ int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mo
     L.piece_count = take((groups + 1) * sizeof(int));
     L.piece_start = take((groups + 1) * sizeof(int));
     L.piece_values = take((size_t)std::max(L.pieces, 1LL) * (2 * n_mono + 6) * sizeof(double));
+    L.heads = take((size_t)(groups + 1) * sizeof(BatchHead));
+    L.node_n = take((size_t)(groups + 1) * sizeof(long long));
+    L.node_off = take((size_t)(groups + 1) * sizeof(long long));
     L.scan_temp = take(scan_temp);
     L.total = at;
     return LH_OK;
+}
+
+// Path nodes the pool of a scratch buffer of `bytes` holds beyond the fixed part
+// (three FP64 arrays, each 256-byte aligned).
+long long pool_nodes_of(const ScratchLayout& L, size_t bytes) {
+    if (bytes <= L.total + 3 * 256) return 0;
+    return (long long)((bytes - L.total - 3 * 256) / (3 * sizeof(double)));
+}
+
+// The default pool: 64 nodes per group and at least 2^20 (path grids have
+// tens of nodes; a batch holds one group or more).
+long long default_pool_nodes(int groups) {
+    return std::max(64LL * groups, 1LL << 20);
 }
 
 int check_grouped(const lh_order1_pencils* pen, const lh_order1_oms* om,
@@ -2095,12 +2328,14 @@ size_t lh_order1_grouped_workspace_bytes(const lh_order1_pencils* pencils,
 }
 
 size_t lh_order1_group_scratch_bytes(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
-                                     const lh_order1_optics* optics, int groups) {
+                                     const lh_order1_optics* optics, int groups,
+                                     long long pool_nodes) {
     if (!pencils || !oms || !optics || groups < 0) return 0;
     ScratchLayout L;
     if (scratch_layout(oms->work_count, pencils->count, slots_of(pencils), optics->bands,
                        optics->bins, optics->n_mono, groups, L) != LH_OK) return 0;
-    return L.total;
+    long long pool = pool_nodes < 0 ? default_pool_nodes(groups) : pool_nodes;
+    return pool == 0 ? L.total : L.total + 3 * 256 + (size_t)pool * 3 * sizeof(double);
 }
 
 int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* oms,
@@ -2209,6 +2444,16 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.segments = SL.segments;
     G.piece_start = reinterpret_cast<int*>(sc + SL.piece_start);
     G.piece_values = reinterpret_cast<double*>(sc + SL.piece_values);
+    G.heads = reinterpret_cast<BatchHead*>(sc + SL.heads);
+    G.node_n = reinterpret_cast<long long*>(sc + SL.node_n);
+    G.node_off = reinterpret_cast<long long*>(sc + SL.node_off);
+    G.pool_nodes = pool_nodes_of(SL, scratch_bytes);
+    {
+        size_t pool_bytes = align_up((size_t)G.pool_nodes * sizeof(double));
+        G.pool_path = reinterpret_cast<double*>(sc + SL.total);
+        G.pool_g0 = reinterpret_cast<double*>(sc + SL.total + pool_bytes);
+        G.pool_g2 = reinterpret_cast<double*>(sc + SL.total + 2 * pool_bytes);
+    }
     size_t scan_bytes = SL.total - SL.scan_temp;
     if (groups > 0) {
         int* counts = reinterpret_cast<int*>(sc + SL.piece_count);
@@ -2237,15 +2482,42 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
                                                  G.batch_scan, groups + 1, s));
     o1_batch_starts<<<blocks(std::max(groups, work) + 1), kThreads, 0, s>>>(G, groups, work);
     if ((status = lh::check_launch("o1_batch_starts")) != LH_OK) return status;
+    // batch headers and node counts, the node offsets, then either the wide
+    // kernels or (the pool too small) the single-kernel form; each kernel
+    // reads the total from the device and the other form's kernels return
+    const unsigned grid_batches = (unsigned)std::min<long long>(
+        ((long long)groups + 1 + kWideWaves - 1) / kWideWaves, kWideMaxBlocks);
+    o1_batch_heads<<<grid_batches, kWideThreads, 0, s>>>(G, groups);
+    if ((status = lh::check_launch("o1_batch_heads")) != LH_OK) return status;
+    LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, G.node_n,
+                                                 G.node_off, groups + 1, s));
+    {
+        lh::KernelMark mark("o1_batch_profile", s);
+        switch (optics->max_power) {
+            case 0: o1_batch_profile<0><<<grid_batches, kWideThreads, 0, s>>>(G, groups); break;
+            case 1: o1_batch_profile<1><<<grid_batches, kWideThreads, 0, s>>>(G, groups); break;
+            case 2: o1_batch_profile<2><<<grid_batches, kWideThreads, 0, s>>>(G, groups); break;
+            case 3: o1_batch_profile<3><<<grid_batches, kWideThreads, 0, s>>>(G, groups); break;
+            default: o1_batch_profile<4><<<grid_batches, kWideThreads, 0, s>>>(G, groups); break;
+        }
+    }
+    if ((status = lh::check_launch("o1_batch_profile")) != LH_OK) return status;
+    {
+        long long deposit_waves = (long long)work * SL.segments * optics->bands;
+        const unsigned grid_deposit = (unsigned)((deposit_waves + kWideWaves - 1) / kWideWaves);
+        lh::KernelMark mark("o1_batch_deposit", s);
+        o1_batch_deposit<<<grid_deposit, kWideThreads, 0, s>>>(G, groups);
+    }
+    if ((status = lh::check_launch("o1_batch_deposit")) != LH_OK) return status;
     dim3 grid(work, SL.segments);
     {
         lh::KernelMark mark("o1_batches", s);
         switch (optics->max_power) {
-            case 0: o1_batches<0><<<grid, kBatchThreads, 0, s>>>(G); break;
-            case 1: o1_batches<1><<<grid, kBatchThreads, 0, s>>>(G); break;
-            case 2: o1_batches<2><<<grid, kBatchThreads, 0, s>>>(G); break;
-            case 3: o1_batches<3><<<grid, kBatchThreads, 0, s>>>(G); break;
-            default: o1_batches<4><<<grid, kBatchThreads, 0, s>>>(G); break;
+            case 0: o1_batches<0><<<grid, kBatchThreads, 0, s>>>(G, groups); break;
+            case 1: o1_batches<1><<<grid, kBatchThreads, 0, s>>>(G, groups); break;
+            case 2: o1_batches<2><<<grid, kBatchThreads, 0, s>>>(G, groups); break;
+            case 3: o1_batches<3><<<grid, kBatchThreads, 0, s>>>(G, groups); break;
+            default: o1_batches<4><<<grid, kBatchThreads, 0, s>>>(G, groups); break;
         }
     }
     if ((status = lh::check_launch("o1_batches")) != LH_OK) return status;

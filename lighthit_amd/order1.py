@@ -76,7 +76,7 @@ _lib.SIGNATURES.update({
                                   ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _P, _P, _S,
                                   _P]),
     "lh_order1_group_scratch_bytes": (_S, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
-                                           ctypes.POINTER(_Optics), _I]),
+                                           ctypes.POINTER(_Optics), _I, ctypes.c_longlong]),
     "lh_order1_group_charge": (_I, [ctypes.POINTER(_Pencils), ctypes.POINTER(_Oms),
                                     ctypes.POINTER(_Optics), ctypes.POINTER(_Merge), _I, _I, _P,
                                     _S, _I, _P, _P, _P, _P, _S, _P]),
@@ -408,9 +408,10 @@ def work_counts(device="cuda"):
 
 
 def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None, bins=None,
-            stream=None):
+            stream=None, pool_nodes=-1):
     """Grouped (or screen) order 1 of OMs ``work``: fills ``charge`` (and ``bins``)
-    device tensors at those OMs."""
+    device tensors at those OMs. ``pool_nodes`` sizes the path-node pool of the
+    wide batch kernels (-1: the library's default; 0: the single-kernel form)."""
     import torch
     device = oms.device
     if charge is None:
@@ -435,12 +436,12 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
                   ctypes.c_size_t(ws.numel()), handle)
     groups, key_bits = (int(v) for v in info[:2].tolist())
     scratch_bytes = _size("lh_order1_group_scratch_bytes", ctypes.byref(pen), ctypes.byref(om),
-                          ctypes.byref(optics.struct), groups)
+                          ctypes.byref(optics.struct), groups, int(pool_nodes))
     scratch = _workspace("group_scratch", device, scratch_bytes, stream)
     with _lib.stage(name + "_charge"):
         _lib.call("lh_order1_group_charge", ctypes.byref(pen), ctypes.byref(om),
                   ctypes.byref(optics.struct), ctypes.byref(m), groups, key_bits, _addr(scratch),
-                  ctypes.c_size_t(scratch.numel()), 1 if compute_bins else 0, _addr(charge),
+                  ctypes.c_size_t(scratch_bytes), 1 if compute_bins else 0, _addr(charge),
                   _addr(bins) if compute_bins else None,
                   _addr(_work_counts.get(str(torch.device(device)))),
                   _addr(ws), ctypes.c_size_t(ws.numel()), handle)

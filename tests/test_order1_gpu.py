@@ -59,8 +59,9 @@ def test_screen_pencils_match_oracle(device):
         _close(got[key], want[key])
 
 
+@pytest.mark.parametrize("pool_nodes", [-1, 0], ids=["wide", "single-kernel"])
 @pytest.mark.parametrize("screen", [False, True])
-def test_grouped_charge_and_bins_match_oracle(device, screen):
+def test_grouped_charge_and_bins_match_oracle(device, screen, pool_nodes):
     from lighthit_amd.compile import DeviceChords
     from lighthit_amd.order1 import (DeviceOMs, DeviceOptics, DeviceUnitChords, PromptSettings,
                                      compress, grouped)
@@ -80,13 +81,33 @@ def test_grouped_charge_and_bins_match_oracle(device, screen):
         merge = (2e-3, 0.08, 0.5, 0.02)
         pencils = compress(unit, 0.5, 96, 192)
         want_pencils = oracle.compress(el, cone, 0.5, 96, 192)
-    charge, bins = grouped(pencils, oms, work, optics, merge, compute_bins=not screen)
+    charge, bins = grouped(pencils, oms, work, optics, merge, compute_bins=not screen,
+                           pool_nodes=pool_nodes)
     want_charge, want_bins, _, _ = oracle.grouped(
         d.positions_m, d.looks, d.effective_area_m2, origins, work, want_pencils, tables, alpha,
         w.medium.g, w.edges_ns, oracle.PromptSettings(), merge, compute_bins=not screen)
     _close(charge.cpu().numpy(), want_charge)
     if not screen:
         _close(bins.cpu().numpy(), want_bins)
+
+
+def test_wide_and_single_kernel_batches_agree(device):
+    """The two forms of the batch kernels sum the same terms in the same order."""
+    from lighthit_amd.compile import DeviceChords
+    from lighthit_amd.order1 import (DeviceOMs, DeviceOptics, DeviceUnitChords, PromptSettings,
+                                     compress, grouped)
+    w, d, tables, el, origins, phase = _setup()
+    alpha = acceptance_alpha(d)
+    unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
+    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, PromptSettings(), device)
+    pencils = compress(unit, 0.5, 96, 192)
+    merge = (2e-3, 0.08, 0.5, 0.02)
+    work = np.arange(len(d))
+    wide = [t.cpu().numpy() for t in grouped(pencils, oms, work, optics, merge)]
+    single = [t.cpu().numpy() for t in grouped(pencils, oms, work, optics, merge, pool_nodes=0)]
+    for a, b in zip(wide, single):
+        _close(a, b, 1e-13)
 
 
 def test_spectral_order1_with_global_screen_matches_oracle(device):
