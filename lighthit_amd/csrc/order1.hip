@@ -446,6 +446,8 @@ struct Group {
     int* group_scan;
     int* group_start;         // (groups + 1)
     int* group_end;           // (groups + 1) one past the last member of every group
+    int* member_off;          // (groups + 1) first member of every group, numbered
+                              // consecutively over the groups           caller scratch
     int* piece_start;         // (groups + 1) first piece of every group   caller scratch
     double* piece_values;     // (pieces, nv) partial group sums          caller scratch
     int* om_groups;           // (work + 1) first group of every OM
@@ -962,6 +964,12 @@ __device__ inline void fill_moments(double weight0, double weight2, double c1, d
     }
 }
 
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -971,65 +979,118 @@ __device__ inline double wave_sum(double v) {
 constexpr int kPieceEntries = 64;    // group members per wave of the group sums
 
 // Pieces of every group: ceil(members / kPieceEntries).
+// Members of every group (groups + 1 entries, the last 0), scanned into
+// member_off: the groups' members numbered consecutively.
+__global__ void __launch_bounds__(kThreads) o1_member_counts(Group G, int groups, int* counts) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > groups) return;
+    counts[g] = g == groups ? 0 : G.group_end[g] - G.group_start[g];
+}
+
+// Pieces of every group: the tiles of kPieceEntries consecutive members it touches.
 __global__ void __launch_bounds__(kThreads) o1_piece_counts(Group G, int groups, int* counts) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g > groups) return;
-    counts[g] = g == groups ? 0
-                            : (G.group_end[g] - G.group_start[g] + kPieceEntries - 1)
-                                  / kPieceEntries;
+    if (g == groups) { counts[g] = 0; return; }
+    int first = G.member_off[g], last = G.member_off[g + 1] - 1;
+    counts[g] = last / kPieceEntries - first / kPieceEntries + 1;
 }
 
-// Group sums: one wave per piece of at most kPieceEntries consecutive members
-// of one group, a lane per member, a fixed butterfly reduction; the
-// pieces of a group are then added in piece order (deterministic).
+// Group sums: one wave per tile of kPieceEntries consecutive members (of any
+// number of groups), a lane per member; a fixed-order segmented scan across
+// the lanes sums each group's members in the tile, and the group's last lane
+// writes that piece. The pieces of a group are then added in piece order
+// (deterministic). One tile per wave, no loop: a loop here makes the compiler
+// keep about twice the registers live.
 template <int P>
-__global__ void __launch_bounds__(kThreads) o1_group_sums(Group G, int groups) {
-    int q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups) {
+    __shared__ int head_all[kThreads / 64][kPieceEntries];
     int lane = threadIdx.x & 63;
-    if (q >= G.piece_start[groups]) return;
-    int low = 0, high = groups;     // piece_start[low] <= q < piece_start[high]
-    while (high - low > 1) {
-        int middle = (low + high) >> 1;
-        if (G.piece_start[middle] <= q) low = middle; else high = middle;
-    }
-    int g = low;
+    int* head = head_all[threadIdx.x >> 6];
     constexpr int n_mono = Monomials<P>::N;
     constexpr int nv = 2 * n_mono + 6;
     long long seg = (long long)G.slots * G.n_pen;
-    long long j0 = G.group_start[g] + (long long)(q - G.piece_start[g]) * kPieceEntries;
-    long long j1 = min((long long)G.group_end[g], j0 + kPieceEntries);
-    int w = (int)(j0 / seg);
-    int d = G.om.work[w];
-    double row[nv];
-#pragma unroll
-    for (int m = 0; m < nv; ++m) row[m] = 0.0;
-    double mom0[n_mono], mom2[n_mono];
-    // one member per lane (a loop here makes the compiler keep far more
-    // registers live and drops the occupancy to one wave per SIMD)
-    long long j = j0 + lane;
-    if (j < j1) {
-        int v = G.sorted_vals[j];
-        int p = v >> 1, node = v & 1;
-        Entry x;
-        pencil_entries(G, p, d, w, node, x, nullptr);
-        fill_moments<P>(x.w0, x.w2, x.c1, x.c3, x.averaged, mom0, mom2);
-#pragma unroll
-        for (int m = 0; m < n_mono; ++m) {
-            row[m] += mom0[m];
-            row[n_mono + m] += mom2[m];
+    long long members = G.member_off[groups];
+    long long tiles = (members + kPieceEntries - 1) / kPieceEntries;
+    {
+        long long t = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+        if (t >= tiles) return;
+        long long c0 = t * kPieceEntries;
+        int low = 0, high = groups;     // member_off[low] <= c0 < member_off[high]
+        while (high - low > 1) {
+            int middle = (low + high) >> 1;
+            if (G.member_off[middle] <= c0) low = middle; else high = middle;
         }
-        double omega = fabs(mom0[0]);
-        row[2 * n_mono] += omega;
-        row[2 * n_mono + 1] += omega * x.theta;
-        row[2 * n_mono + 2] += omega * x.r;
-        row[2 * n_mono + 3] += omega * x.t_emit;
-        row[2 * n_mono + 4] += omega * x.tau;
-        row[2 * n_mono + 5] += omega * x.tau * x.tau;
-    }
+        // the lane where each group of the tile starts holds the group's index;
+        // a running maximum carries it to the group's other lanes
+        wave_sync();
+        head[lane] = -1;
+        wave_sync();
+        int gk = low + lane;
+        if (gk < groups) {
+            long long at = (long long)G.member_off[gk] - c0;
+            if (lane == 0) at = 0;
+            if (at < kPieceEntries) head[at] = gk;
+        }
+        wave_sync();
+        int g = head[lane];
+        for (int off = 1; off < 64; off <<= 1) {
+            int other = __shfl_up(g, off, 64);
+            if (lane >= off) g = max(g, other);
+        }
+        long long c = c0 + lane;
+        bool valid = c < members;
+        if (!valid) g = -1 - lane;      // a segment of its own
+        double row[nv];
 #pragma unroll
-    for (int m = 0; m < nv; ++m) {
-        double total = wave_sum(row[m]);
-        if (lane == 0) G.piece_values[(size_t)q * nv + m] = total;
+        for (int m = 0; m < nv; ++m) row[m] = 0.0;
+        if (valid) {
+            long long j = G.group_start[g] + (c - G.member_off[g]);
+            int w = (int)(j / seg);
+            int d = G.om.work[w];
+            int v = G.sorted_vals[j];
+            int p = v >> 1, node = v & 1;
+            Entry x;
+            pencil_entries(G, p, d, w, node, x, nullptr);
+            double mom0[n_mono], mom2[n_mono];
+            fill_moments<P>(x.w0, x.w2, x.c1, x.c3, x.averaged, mom0, mom2);
+#pragma unroll
+            for (int m = 0; m < n_mono; ++m) {
+                row[m] = mom0[m];
+                row[n_mono + m] = mom2[m];
+            }
+            double omega = fabs(mom0[0]);
+            row[2 * n_mono] = omega;
+            row[2 * n_mono + 1] = omega * x.theta;
+            row[2 * n_mono + 2] = omega * x.r;
+            row[2 * n_mono + 3] = omega * x.t_emit;
+            row[2 * n_mono + 4] = omega * x.tau;
+            row[2 * n_mono + 5] = omega * x.tau * x.tau;
+        }
+        // inclusive segmented scan: lane i adds lane i - off when both lie in
+        // one group (groups are contiguous runs of lanes)
+        unsigned take = 0;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            int other = __shfl_up(g, 1 << s, 64);
+            if (lane >= (1 << s) && other == g) take |= 1u << s;
+        }
+#pragma unroll
+        for (int m = 0; m < nv; ++m) {
+            double value = row[m];
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                double other = __shfl_up(value, 1 << s, 64);
+                if ((take >> s) & 1u) value += other;
+            }
+            row[m] = value;
+        }
+        int next = __shfl_down(g, 1, 64);
+        if (valid && (lane == 63 || next != g)) {
+            long long q = G.piece_start[g] + (t - G.member_off[g] / kPieceEntries);
+#pragma unroll
+            for (int m = 0; m < nv; ++m) G.piece_values[(size_t)q * nv + m] = row[m];
+        }
     }
 }
 
@@ -1215,12 +1276,6 @@ __device__ __forceinline__ void panel_events(const LambdaBins& B, double t_a, do
         out.v_end = quad_integral(e_b, a_loc, b_loc, gamma, edges[k + 1] - t_b, kappa,
                                   B.eedge[k + 1]);
     }
-}
-
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // One round of deposits: every lane holds the events of one panel (panels in
@@ -2074,7 +2129,7 @@ int segments_for(int work, int bands, int bins) {
 
 struct ScratchLayout {
     size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
-        part_charge, piece_count, piece_start, piece_values, heads, node_n, node_off,
+        part_charge, piece_count, member_off, piece_start, piece_values, heads, node_n, node_off,
         scan_temp, total;   // total: the fixed part; the node pool follows it
     int segments;
     long long pieces;         // bound on the group-sum pieces
@@ -2107,6 +2162,7 @@ int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mo
     long long entries = (long long)slots * work * n_pen;
     L.pieces = groups + (entries + kPieceEntries - 1) / kPieceEntries;
     L.piece_count = take((groups + 1) * sizeof(int));
+    L.member_off = take((groups + 1) * sizeof(int));
     L.piece_start = take((groups + 1) * sizeof(int));
     L.piece_values = take((size_t)std::max(L.pieces, 1LL) * (2 * n_mono + 6) * sizeof(double));
     L.heads = take((size_t)(groups + 1) * sizeof(BatchHead));
@@ -2443,6 +2499,7 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.part_charge = reinterpret_cast<double*>(sc + SL.part_charge);
     G.segments = SL.segments;
     G.piece_start = reinterpret_cast<int*>(sc + SL.piece_start);
+    G.member_off = reinterpret_cast<int*>(sc + SL.member_off);
     G.piece_values = reinterpret_cast<double*>(sc + SL.piece_values);
     G.heads = reinterpret_cast<BatchHead*>(sc + SL.heads);
     G.node_n = reinterpret_cast<long long*>(sc + SL.node_n);
@@ -2457,13 +2514,20 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     size_t scan_bytes = SL.total - SL.scan_temp;
     if (groups > 0) {
         int* counts = reinterpret_cast<int*>(sc + SL.piece_count);
+        o1_member_counts<<<blocks(groups + 1), kThreads, 0, s>>>(G, groups, counts);
+        if ((status = lh::check_launch("o1_member_counts")) != LH_OK) return status;
+        LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, counts,
+                                                     G.member_off, groups + 1, s));
         o1_piece_counts<<<blocks(groups + 1), kThreads, 0, s>>>(G, groups, counts);
         if ((status = lh::check_launch("o1_piece_counts")) != LH_OK) return status;
         LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, counts,
                                                      G.piece_start, groups + 1, s));
         {
             lh::KernelMark mark("o1_group_sums", s);
-            const unsigned grid_sums = (unsigned)((SL.pieces + 3) / 4);
+            // one wave per tile of members; their count is on the device and
+            // at most the entries' (waves past it return at once)
+            long long entries = (long long)slots_of(pencils) * work * pencils->count;
+            const unsigned grid_sums = (unsigned)((entries / kPieceEntries + 1 + 3) / 4);
             switch (optics->max_power) {
                 case 0: o1_group_sums<0><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
                 case 1: o1_group_sums<1><<<grid_sums, kThreads, 0, s>>>(G, groups); break;
