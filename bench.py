@@ -284,6 +284,80 @@ def measure_traffic(workload_flag, kernel_name, keep_dir=None):
     return traffic, note
 
 
+# ------------------------------------------------------------ serving stream
+
+def _compact(response):
+    rows = np.flatnonzero(response.components_pe.reshape(len(response.components_pe), -1)
+                          .any(axis=1))
+    return response.charge_components_pe.copy(), rows, response.components_pe[rows].copy()
+
+
+def plain_stream(route, stream):
+    """The stream's events one plain call at a time (host arrays, upload inside):
+    per-event service times and the compact results the served events must equal."""
+    import torch
+    import warnings
+    results, times = [], []
+    began = time.perf_counter()
+    for elements in stream:
+        start = time.perf_counter()
+        try:
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", RuntimeWarning)
+                response = route(elements)
+            results.append(_compact(response))
+        except ValueError as error:
+            results.append(("error", str(error)))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - start)
+    elapsed = time.perf_counter() - began
+    lat = np.asarray(times) * 1e3
+    return {"results": results,
+            "report": {"events_per_s": len(stream) / elapsed,
+                       "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                      "p99": float(np.percentile(lat, 99))},
+                       "errors": sum(1 for r in results if isinstance(r[0], str))}}
+
+
+def served_stream(server, stream, plain, rates):
+    """The stream's events submitted at Poisson arrival times to ``server``, at
+    each rate: throughput, latency from arrival to result, and whether every
+    result equals the plain call's."""
+    from lighthit_amd.serve import latency_summary
+    from lighthit_amd.synthetic import poisson_arrivals
+    report = {"events": len(stream), "plain": plain["report"],
+              "element_counts": {"min": min(len(e) for e in stream),
+                                 "median": int(np.median([len(e) for e in stream])),
+                                 "max": max(len(e) for e in stream)},
+              "rates": []}
+    for rate in rates:
+        print(f"[bench] serving stream at {rate:g} events/s", file=sys.stderr, flush=True)
+        times = poisson_arrivals(rate, len(stream))
+        futures = []
+        start_at = time.perf_counter()
+        for elements, at in zip(stream, times):
+            delay = start_at + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futures.append(server.submit(elements))
+        same, errors = True, 0
+        for f, want in zip(futures, plain["results"]):
+            want_error = isinstance(want[0], str)
+            try:
+                got = _compact(f.result())
+            except ValueError:
+                errors += 1
+                same &= want_error
+                continue
+            same &= (not want_error) and all(np.array_equal(a, b) for a, b in zip(got, want))
+        finished = max(f.submitted + f.latency_s for f in futures)
+        report["rates"].append({"offered_events_per_s": rate,
+                                "events_per_s": len(futures) / (finished - start_at),
+                                "latency_ms": latency_summary(futures),
+                                "errors": errors, "same_as_plain": bool(same)})
+    return report
+
+
 # ------------------------------------------------------------------- driver
 
 def roofline_of(name, kernel_ms, workload, response, route, counts=None, launches=1):
@@ -354,6 +428,10 @@ def main():
     parser.add_argument("--workers", type=int, default=3,
                         help="concurrent route calls of the served form (1: plain only)")
     parser.add_argument("--served-events", type=int, default=120)
+    parser.add_argument("--stream-events", type=int, default=256,
+                        help="events of the serving stream (0: none)")
+    parser.add_argument("--stream-rates", default="50,200,800,2000",
+                        help="Poisson arrival rates (events/s) of the serving stream")
     parser.add_argument("--processes", type=int, default=8,
                         help="also serve from this many worker processes (0: threads only)")
     parser.add_argument("--no-traffic", action="store_true",
@@ -455,6 +533,17 @@ def main():
                       "ms_per_event": 1e3 * served_elapsed / len(batch),
                       "latency_ms_all_submitted_at_once": latency_summary(futures),
                       "same_as_alone": bool(mixed_same)}
+    stream_plain = None
+    if args.processes > 0 and args.stream_events > 0:
+        from lighthit_amd.synthetic import serving_stream
+        print(f"[bench] serving stream: {args.stream_events} events, plain calls",
+              file=sys.stderr, flush=True)
+        stream = list(serving_stream(workload.detector, args.stream_events))
+        stream_plain = plain_stream(route, stream)
+        # the worker processes get the device memory this process cached
+        from lighthit_amd import _lib as lib
+        lib.release_workspaces()
+        torch.cuda.empty_cache()
     processes = None
     if args.processes > 0:
         import functools
@@ -490,6 +579,10 @@ def main():
                 f.result()
             for handle in staged:
                 server.release(handle)
+            stream_report = None
+            if stream_plain is not None:
+                rates = [float(r) for r in args.stream_rates.split(",") if r]
+                stream_report = served_stream(server, stream, stream_plain, rates)
         same_p = all(np.array_equal(r.charge_components_pe, first.charge_components_pe)
                      and np.array_equal(r.components_pe, first.components_pe)
                      for r in results)
@@ -500,7 +593,8 @@ def main():
                      "latency_ms_poisson": dict(latency_summary(arrivals),
                                                 rate_events_per_s=rate),
                      "same_as_alone": bool(same_p),
-                     "sources": "staged in device memory in every worker before timing"}
+                     "sources": "staged in device memory in every worker before timing",
+                     "stream": stream_report}
     ms_per_step = 1e3 * elapsed / args.steps
     lat = np.asarray(latencies) * 1e3
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_event"]) if kernels else None

@@ -106,9 +106,23 @@ def workspace(tag, device, nbytes, stream=None):
     with _workspaces_lock:
         buf = _workspaces.get(key)
         if buf is None or buf.numel() < nbytes:
+            # the smaller buffer goes back to the allocator first (its reuse is
+            # ordered after the work already queued on this stream)
+            _workspaces.pop(key, None)
+            del buf
             buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
             _workspaces[key] = buf
     return buf
+
+
+def release_workspaces(device=None):
+    """Drop the cached workspaces (of ``device``, or all); their memory returns
+    to the allocator."""
+    import torch
+    with _workspaces_lock:
+        for key in [k for k in _workspaces
+                    if device is None or k[1] == str(torch.device(device))]:
+            del _workspaces[key]
 
 
 class StageClock:

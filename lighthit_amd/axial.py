@@ -12,6 +12,14 @@ from dataclasses import dataclass, field
 import numpy as np
 
 
+def dot3(vectors, direction):
+    """``vectors @ direction`` for 3-vectors, summed in one fixed order (a BLAS
+    product may split its work by thread count, and so its last bits)."""
+    vectors = np.asarray(vectors, float)
+    return (vectors[..., 0] * direction[0] + vectors[..., 1] * direction[1]
+            + vectors[..., 2] * direction[2])
+
+
 @dataclass(frozen=True)
 class AxisFrame:
     """Centre, axis and transverse pair of a shower (``axial_source.py:80-141``)."""
@@ -32,7 +40,7 @@ class AxisFrame:
         share = weights / weights.sum()
         centre = (share[:, None] * points).sum(axis=0)
         offset = points - centre
-        covariance = (offset * share[:, None]).T @ offset
+        covariance = np.einsum("ij,ik->jk", offset * share[:, None], offset)
         values, vectors = np.linalg.eigh(covariance)
         if np.max(np.abs(values)) <= 1e-24:
             axis = np.sum(share[:, None] * elements.direction, axis=0)
@@ -43,7 +51,7 @@ class AxisFrame:
         else:
             axis = vectors[:, int(np.argmax(values))]
         axis = axis / np.linalg.norm(axis)
-        if float((share * (elements.direction @ axis)).sum()) < 0:
+        if float((share * dot3(elements.direction, axis)).sum()) < 0:
             axis = -axis
         helper = np.array([0.0, 0.0, 1.0])
         if abs(float(axis @ helper)) > 0.9:
@@ -58,9 +66,9 @@ class AxisFrame:
         points = (elements.start_m
                   + (fraction * np.asarray(elements.length_m, float))[:, None] * elements.direction)
         offset = points - self.centre_m
-        z = offset @ self.axis
+        z = dot3(offset, self.axis)
         transverse = offset - z[:, None] * self.axis
-        cosine = elements.direction @ self.axis
+        cosine = dot3(elements.direction, self.axis)
         start = np.asarray(elements.start_ns, float)
         time = start + fraction * (np.asarray(elements.end_ns, float) - start)
         return z, transverse, cosine, time
@@ -72,8 +80,8 @@ class AxisFrame:
     def rotate(self, vectors):
         """``(first, second, axis)`` components, the axis as z."""
         vectors = np.asarray(vectors, float)
-        return np.stack((vectors @ self.first, vectors @ self.second,
-                         vectors @ self.axis), axis=-1)
+        return np.stack((dot3(vectors, self.first), dot3(vectors, self.second),
+                         dot3(vectors, self.axis)), axis=-1)
 
 
 def channel_layout(degree, azimuthal_degree):

@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .axial import AxialSource, AxisFrame, channel_layout
+from .axial import AxialSource, AxisFrame, channel_layout, dot3
 
 
 class _Field(ctypes.Structure):
@@ -163,7 +163,7 @@ def _lattice(chords_host, taken, frame, degree, azimuthal_degree, omega, options
 
     def node_z(fraction):
         points = start + (fraction * length)[:, None] * direction
-        return (points - centre) @ axis
+        return dot3(points - centre, axis)
     z_origin = float(node_z(0.5).min()) - margin * options.cell_m
     times = [start_ns + fraction * (end_ns - start_ns) for fraction in nodes]
     if min(float(t.min()) for t in times) < reference:

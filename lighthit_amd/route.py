@@ -400,7 +400,7 @@ def collapse(bands, cell_m):
             total = weight[chunk].sum()
             share = (weight[chunk] / total if total > 0 else np.full(len(chunk), 1.0 / len(chunk)))
             zz.append(np.sum(share * z[chunk]))
-            tt.append(share @ transverse[chunk])
+            tt.append(np.sum(share[:, None] * transverse[chunk], axis=0))
             cc.append(channels[chunk].sum(axis=0))
         reduced.append(CollapsedSource(band.frame, np.asarray(zz), np.asarray(tt),
                                        np.ascontiguousarray(np.asarray(cc)), band.degree,

@@ -172,17 +172,24 @@ class StagedSource:
         return self.count
 
 
-def _process_main(factory, device, threads, tasks, results, slot):
+def _process_main(factory, device, threads, tasks, results, slot, memory_fraction=None):
     """One worker process: build the route, then answer tasks until ``None``.
-    ``device`` None leaves the GPU alone (host-only routes)."""
+    ``device`` None leaves the GPU alone (host-only routes). The process uses at
+    most ``memory_fraction`` of the device memory; an event that runs out of it
+    is run once more after the cached workspaces are released."""
     from threadpoolctl import threadpool_limits
     limits = threadpool_limits(limits=threads)
     drain = None
+    out_of_memory = ()
     try:
         if device is not None:
             import torch
             torch.cuda.set_device(torch.device(device))
+            if memory_fraction is not None:
+                torch.cuda.set_per_process_memory_fraction(float(memory_fraction),
+                                                           torch.device(device))
             drain = torch.cuda.synchronize
+            out_of_memory = (torch.cuda.OutOfMemoryError,)
         route = factory()
         route.tables()
         route.order0()
@@ -209,7 +216,13 @@ def _process_main(factory, device, threads, tasks, results, slot):
             continue
         try:
             source = staged[payload.key] if isinstance(payload, StagedSource) else payload
-            response = route(source, **call_kwargs)
+            try:
+                response = route(source, **call_kwargs)
+            except out_of_memory:
+                from . import _lib
+                _lib.release_workspaces()
+                torch.cuda.empty_cache()
+                response = route(source, **call_kwargs)
             if drain is not None:
                 drain()
             results.put(("done", slot, key, _pack_response(response)))
@@ -222,7 +235,9 @@ class ProcessShowerServer:
     """``ShowerTransport`` calls on one GPU from ``processes`` worker processes.
 
     ``factory`` is a picklable callable that returns the route; each process
-    calls it once, so every process holds the tables resident in HBM. An
+    calls it once, so every process holds the tables resident in HBM and uses
+    at most ``memory_fraction`` of the device memory (default: an equal share
+    of 90% among the workers and the caller). An
     event is sent to a free process when its estimated footprint fits the
     memory budget beside the events already running; the response comes back
     as arrays. ``submit`` returns a ``Future`` as ``ShowerServer.submit`` does.
@@ -231,7 +246,8 @@ class ProcessShowerServer:
     """
 
     def __init__(self, factory, *, processes=2, detector, memory_budget_bytes=None,
-                 device="cuda:0", threads_per_process=None, start_timeout_s=600.0):
+                 device="cuda:0", threads_per_process=None, memory_fraction=None,
+                 start_timeout_s=600.0):
         import multiprocessing as mp
         import os
         if processes < 1:
@@ -250,9 +266,11 @@ class ProcessShowerServer:
         context = mp.get_context("spawn")
         self._results = context.Queue()
         self._tasks = [context.Queue() for _ in range(processes)]
+        # each worker's share of the device memory (the caller keeps one share)
+        fraction = memory_fraction if memory_fraction is not None else 0.9 / (processes + 1)
         self._processes = [context.Process(target=_process_main,
                                            args=(factory, device, threads, self._tasks[i],
-                                                 self._results, i),
+                                                 self._results, i, fraction),
                                            name=f"shower-process-{i}", daemon=True)
                            for i in range(processes)]
         for process in self._processes:

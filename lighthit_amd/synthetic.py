@@ -141,6 +141,31 @@ def plain_detector_scale(elements=50_000):
                     np.linspace(0., 1.2, 321), _shower(pose, 1500.0, elements, 11))
 
 
+def serving_stream(detector, count=4096, *, seed=23, low=500, high=200_000, inflate_m=100.0,
+                   track_m=300.0):
+    """Events of the serving stream: element counts log-uniform in [low, high],
+    vertices uniform in the array's bounding box inflated by ``inflate_m``,
+    isotropic directions, all from ``seed``. Every event has a ``track_m``
+    charged track (the element count sets the work, not the brightness).
+    Yields ``count`` host element sets in order."""
+    rng = np.random.default_rng(seed)
+    lo = detector.positions_m.min(axis=0) - inflate_m
+    hi = detector.positions_m.max(axis=0) + inflate_m
+    for _ in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(low), np.log(high)))))
+        vertex = lo + rng.uniform(size=3) * (hi - lo)
+        direction = rng.normal(size=3)
+        direction /= np.linalg.norm(direction)
+        yield _shower(SourcePose(vertex, direction), track_m, n, int(rng.integers(2 ** 31)))
+
+
+def poisson_arrivals(rate_per_s, count, seed=23):
+    """Arrival times (s from the first) of a Poisson stream of ``count`` events."""
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate_per_s, count)
+    gaps[0] = 0.0
+    return np.cumsum(gaps)
+
+
 def synthetic_pack(medium, quadrature, *, degree=8, acceptance_degree=2,
                    omega=np.linspace(0.0, 1.2, 65), radial_range_m=(2.0, 150.0),
                    radial_nodes=48):
