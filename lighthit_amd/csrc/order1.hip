@@ -1087,9 +1087,14 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
         }
         int next = __shfl_down(g, 1, 64);
         if (valid && (lane == 63 || next != g)) {
-            long long q = G.piece_start[g] + (t - G.member_off[g] / kPieceEntries);
+            // a group within one tile is complete: its row goes straight to values
+            int first = G.piece_start[g];
+            double* out = G.piece_start[g + 1] - first == 1
+                              ? G.values + (size_t)g * nv
+                              : G.piece_values
+                                    + (size_t)(first + (t - G.member_off[g] / kPieceEntries)) * nv;
 #pragma unroll
-            for (int m = 0; m < nv; ++m) G.piece_values[(size_t)q * nv + m] = row[m];
+            for (int m = 0; m < nv; ++m) out[m] = row[m];
         }
     }
 }
@@ -1099,6 +1104,7 @@ __global__ void __launch_bounds__(kThreads) o1_group_combine(Group G, int groups
     if (g >= groups) return;
     int nv = 2 * G.opt.n_mono + 6;
     int first = G.piece_start[g], last = G.piece_start[g + 1];
+    if (last - first == 1) return;   // written by o1_group_sums
     for (int m = 0; m < nv; ++m) {
         double total = G.piece_values[(size_t)first * nv + m];
         for (int q = first + 1; q < last; ++q) total += G.piece_values[(size_t)q * nv + m];
@@ -1899,15 +1905,61 @@ __global__ void __launch_bounds__(kThreads) o1_sum_segments(Group G) {
     }
 }
 
-// Charge, then the per-wavelength prefix pass over the summed deposits
-// (_finalize :610-632).
+// The per-wavelength prefix pass over the summed deposits (_finalize
+// :610-632), one wave per (OM, wavelength): the running sums (a, b, c) at every
+// bin are the window-front sums plus an exclusive scan of the bins' deposits,
+// 64 bins at a time with the carry between chunks.
+__global__ void __launch_bounds__(kThreads) o1_finalize_bins(Group G) {
+    int lane = threadIdx.x & 63;
+    long long gw = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    int nl = G.opt.bands, nb = G.opt.bins, S = G.segments;
+    if (gw >= (long long)G.om.work_count * nl) return;
+    int w = (int)(gw / nl), lam = (int)(gw % nl);
+    size_t base = (size_t)w * S;
+    const double* a4 = G.part_acc + base * nl * nb * 4 + (size_t)lam * nb * 4;
+    const double* pre0 = G.part_pre + base * nl * 3;
+    const double* eedge = G.opt.eedge + (size_t)lam * (nb + 1);
+    const double* it = G.opt.itab + (size_t)lam * nb * 3;
+    double* lb = G.lambda_bins + ((size_t)w * nl + lam) * nb;
+    double a = pre0[3 * lam], b = pre0[3 * lam + 1], c = pre0[3 * lam + 2];
+    for (int k0 = 0; k0 < nb; k0 += 64) {
+        int k = k0 + lane;
+        bool in = k < nb;
+        double da = in ? a4[4 * (size_t)k] : 0.0;
+        double db = in ? a4[4 * (size_t)k + 1] : 0.0;
+        double dc = in ? a4[4 * (size_t)k + 2] : 0.0;
+        double ia = da, ib = db, ic = dc;   // inclusive scans
+        for (int off = 1; off < 64; off <<= 1) {
+            double xa = __shfl_up(ia, off, 64), xb = __shfl_up(ib, off, 64),
+                   xc = __shfl_up(ic, off, 64);
+            if (lane >= off) { ia += xa; ib += xb; ic += xc; }
+        }
+        double ea = __shfl_up(ia, 1, 64), eb = __shfl_up(ib, 1, 64), ec = __shfl_up(ic, 1, 64);
+        if (lane == 0) { ea = 0.0; eb = 0.0; ec = 0.0; }
+        if (in) {
+            double ak = a + ea, bk = b + eb, ck = c + ec;
+            double e_k = G.opt.edges[k];
+            double width = G.opt.edges[k + 1] - e_k;
+            double a_loc = ak + e_k * (bk + ck * e_k);
+            double b_loc = bk + 2.0 * ck * e_k;
+            double value = eedge[k] * width
+                           * (a_loc * it[3 * k] + width * (b_loc * it[3 * k + 1]
+                                                           + width * ck * it[3 * k + 2]));
+            lb[k] = value + a4[4 * (size_t)k + 3];
+        }
+        a += __shfl(ia, 63, 64);
+        b += __shfl(ib, 63, 64);
+        c += __shfl(ic, 63, 64);
+    }
+}
+
+// Charge of every OM (segments, then wavelengths, in order) and its bins, the
+// wavelengths' bins summed in order.
 __global__ void __launch_bounds__(kThreads) o1_finalize(Group G) {
     int w = blockIdx.x;
     int d = G.om.work[w];
     int nl = G.opt.bands, nb = G.opt.bins, S = G.segments;
     size_t base = (size_t)w * S;
-    double* acc0 = G.part_acc + base * nl * nb * 4;
-    double* pre0 = G.part_pre + base * nl * 3;
     if (threadIdx.x == 0) {
         double total = 0.0;
         for (int l = 0; l < nl; ++l) {
@@ -1918,27 +1970,7 @@ __global__ void __launch_bounds__(kThreads) o1_finalize(Group G) {
         G.charge[d] = G.om.areas[d] * total;
     }
     if (!G.compute_bins) return;
-    double* lb = G.lambda_bins + (size_t)w * nl * nb;
-    if (threadIdx.x < nl) {
-        int lam = threadIdx.x;
-        const double* eedge = G.opt.eedge + (size_t)lam * (nb + 1);
-        const double* it = G.opt.itab + (size_t)lam * nb * 3;
-        const double* a4 = acc0 + (size_t)lam * nb * 4;
-        double a = pre0[3 * lam], b = pre0[3 * lam + 1], c = pre0[3 * lam + 2];
-        for (int k = 0; k < nb; ++k) {
-            double e_k = G.opt.edges[k];
-            double width = G.opt.edges[k + 1] - e_k;
-            double a_loc = a + e_k * (b + c * e_k);
-            double b_loc = b + 2.0 * c * e_k;
-            double value = eedge[k] * width
-                           * (a_loc * it[3 * k] + width * (b_loc * it[3 * k + 1] + width * c * it[3 * k + 2]));
-            lb[(size_t)lam * nb + k] = value + a4[4 * k + 3];
-            a += a4[4 * k];
-            b += a4[4 * k + 1];
-            c += a4[4 * k + 2];
-        }
-    }
-    __syncthreads();
+    const double* lb = G.lambda_bins + (size_t)w * nl * nb;
     double area = G.om.areas[d];
     for (int k = threadIdx.x; k < nb; k += kThreads) {
         double total = 0.0;
@@ -2590,6 +2622,11 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
                       work);
         o1_sum_segments<<<sum_grid, kThreads, 0, s>>>(G);
         if ((status = lh::check_launch("o1_sum_segments")) != LH_OK) return status;
+    }
+    if (compute_bins) {
+        long long waves = (long long)work * optics->bands;
+        o1_finalize_bins<<<(unsigned)((waves + 3) / 4), kThreads, 0, s>>>(G);
+        if ((status = lh::check_launch("o1_finalize_bins")) != LH_OK) return status;
     }
     o1_finalize<<<work, kThreads, 0, s>>>(G);
     return lh::check_launch("o1_finalize");
