@@ -63,17 +63,25 @@ def build_route(name):
     from lighthit_amd.fold import KernelPack
     from lighthit_amd.route import (RouteConfig, RouteKernel, ShowerConfig, ShowerPrompt,
                                     ShowerTransport)
-    from lighthit_amd.synthetic import plain_detector_scale, plain_small, synthetic_pack
-    workload = plain_detector_scale() if name == "detector" else plain_small()
+    from lighthit_amd.synthetic import (heavy_event, plain_detector_scale, plain_small,
+                                        synthetic_pack)
+    workload = {"detector": plain_detector_scale, "small": plain_small,
+                "heavy": heavy_event}[name]()
+    heavy = name == "heavy"
+    degree, azimuthal = (16, 4) if heavy else (8, 2)
     kernel = RouteKernel(workload.medium, workload.detector,
                          RouteConfig(relative_time_edges_ns=workload.edges_ns,
                                      wavelength_nodes=len(workload.wavelength.wavelength_nm),
                                      omega_per_ns=workload.omega_per_ns,
-                                     readout_omega_per_ns=workload.readout_omega_per_ns))
+                                     readout_omega_per_ns=workload.readout_omega_per_ns,
+                                     scattering_degree=degree, source_degree=degree,
+                                     azimuthal_degree=azimuthal))
     pack = KernelPack()
-    pack.add(*synthetic_pack(workload.medium, kernel.wavelength,
+    pack.add(*synthetic_pack(workload.medium, kernel.wavelength, degree=degree,
                              omega=workload.omega_per_ns))
-    config = ShowerConfig(prompt=ShowerPrompt(max_distance_m=300.0))
+    # the order-1 selection radius: the reference's default guard (150 m)
+    # raises for these events; the heavy event's 10 km track needs 1000 m
+    config = ShowerConfig(prompt=ShowerPrompt(max_distance_m=1000.0 if heavy else 300.0))
     route = ShowerTransport(kernel, pack, config=config)
     return workload, kernel, route
 
@@ -422,7 +430,7 @@ def main():
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", choices=["detector", "small"], default="detector")
+    parser.add_argument("--workload", choices=["detector", "small", "heavy"], default="detector")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-fraction", type=float, default=0.05)
     parser.add_argument("--workers", type=int, default=3,

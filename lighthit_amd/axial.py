@@ -12,6 +12,14 @@ from dataclasses import dataclass, field
 import numpy as np
 
 
+def column_bounds(points):
+    """Per-column minimum and maximum of an (N, 3) array (column by column: a
+    reduction over axis 0 of an (N, 3) array is about ten times slower)."""
+    points = np.asarray(points, float)
+    return (np.array([points[:, i].min() for i in range(points.shape[1])]),
+            np.array([points[:, i].max() for i in range(points.shape[1])]))
+
+
 def dot3(vectors, direction):
     """``vectors @ direction`` for 3-vectors, summed in one fixed order (a BLAS
     product may split its work by thread count, and so its last bits)."""

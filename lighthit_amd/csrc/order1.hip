@@ -2115,6 +2115,10 @@ inline int slots_of(const lh_order1_pencils* pen) { return pen->parent != nullpt
 
 int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLayout& L) {
     long long entries = (long long)slots * work * n_pen;
+    // entries, groups and their offsets are 32-bit on the device
+    LH_REQUIRE(entries < (1LL << 31) - 1,
+               "%lld (pencil, OM) entries in one grouped call; pass fewer OMs per call",
+               entries);
     size_t sort_temp = 0, scan_temp = 0;
     if (hipcub::DeviceRadixSort::SortPairs(
             nullptr, sort_temp, (unsigned long long*)nullptr, (unsigned long long*)nullptr,

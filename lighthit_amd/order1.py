@@ -347,9 +347,10 @@ class DeviceUnitChords:
 
     def grid(self, cell_m, pixel_face, ring_nodes, parent_face=0):
         if getattr(self, "_box", None) is None:
+            from .axial import column_bounds
             ends = self.start + self.length[:, None] * self.direction
-            self._box = (np.minimum(self.start.min(axis=0), ends.min(axis=0)),
-                         np.maximum(self.start.max(axis=0), ends.max(axis=0)))
+            (s_low, s_high), (e_low, e_high) = column_bounds(self.start), column_bounds(ends)
+            self._box = (np.minimum(s_low, e_low), np.maximum(s_high, e_high))
         low = self._box[0] - cell_m
         high = self._box[1] + cell_m
         dims = np.ceil((high - low) / cell_m).astype(np.int64) + 1
@@ -407,6 +408,12 @@ def work_counts(device="cuda"):
     return None if counts is None else tuple(int(v) for v in counts.cpu().tolist())
 
 
+#: (pencil, OM) entries of one grouped call: its sort takes ~40 bytes per entry
+MAX_GROUPED_ENTRIES = 1 << 28
+#: OMs of one grouped call (the library's limit)
+MAX_GROUPED_OMS = 65535
+
+
 def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None, bins=None,
             stream=None, pool_nodes=-1):
     """Grouped (or screen) order 1 of OMs ``work``: fills ``charge`` (and ``bins``)
@@ -419,6 +426,15 @@ def grouped(pencils, oms, work, optics, merge, *, compute_bins=True, charge=None
     if compute_bins and bins is None:
         bins = torch.zeros((oms.count, optics.bins), dtype=torch.float64, device=device)
     if not len(work):
+        return charge, bins
+    per_om = (2 if pencils.parent is not None else 1) * max(int(pencils.count), 1)
+    most = max(1, min(MAX_GROUPED_OMS, MAX_GROUPED_ENTRIES // per_om))
+    if len(work) > most:
+        # OMs are independent: a long list runs in parts of bounded size
+        for first in range(0, len(work), most):
+            grouped(pencils, oms, work[first:first + most], optics, merge,
+                    compute_bins=compute_bins, charge=charge, bins=bins, stream=stream,
+                    pool_nodes=pool_nodes)
         return charge, bins
     _ensure_tables(device)
     pen = pencils.struct()

@@ -26,7 +26,7 @@ from time import perf_counter, process_time
 import numpy as np
 
 from . import _lib
-from .axial import AxisFrame
+from .axial import AxisFrame, column_bounds
 from .model import WavelengthQuadrature, acceptance_alpha
 from .order1 import PromptSettings
 
@@ -925,7 +925,8 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     fastest = float(np.min(sample["group_index"]))
     time_width = float(np.ptp(edges))
     source_time_span = float(np.max(elements.end_ns) - np.min(elements.start_ns))
-    source_extent = float(np.linalg.norm(np.ptp(elements.midpoints_m, axis=0)))
+    low_m, high_m = column_bounds(elements.midpoints_m)
+    source_extent = float(np.linalg.norm(high_m - low_m))
     local_origin = source_extent > high or source_time_span > 0.5 * time_width
     if time_origin_ns is not None:
         origins = np.asarray(time_origin_ns, float)

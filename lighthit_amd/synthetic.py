@@ -141,6 +141,25 @@ def plain_detector_scale(elements=50_000):
                     np.linspace(0., 1.2, 321), _shower(pose, 1500.0, elements, 11))
 
 
+def heavy_event(elements=1_000_000):
+    """The heavy single event: 16 wavelength nodes, 1 ns bins over -60..1500 ns,
+    a 16-cluster array (4608 OMs), one Gaussian shower of ``elements`` elements
+    with a 10 km charged track, vertex and direction from seed 31 as for the
+    detector-scale event."""
+    medium = synthetic_medium_model()
+    detector = detector_scale_array(16)
+    rng = np.random.default_rng(31)
+    low, high = detector.positions_m.min(axis=0), detector.positions_m.max(axis=0)
+    vertex = low + rng.uniform(size=3) * (high - low)
+    direction = rng.normal(size=3)
+    direction /= np.linalg.norm(direction)
+    pose = SourcePose(vertex, direction)
+    return Workload(f"heavy: 4608 OMs, {elements}-element shower, 16 wavelengths, 1 ns bins",
+                    medium, detector, WavelengthQuadrature.gauss_legendre(400., 500., 16),
+                    np.arange(-60.0, 1501.0, 1.0), np.linspace(0., 1.2, 65),
+                    np.linspace(0., 1.2, 321), _shower(pose, 10_000.0, elements, 31))
+
+
 def serving_stream(detector, count=4096, *, seed=23, low=500, high=200_000, inflate_m=100.0,
                    track_m=300.0):
     """Events of the serving stream: element counts log-uniform in [low, high],

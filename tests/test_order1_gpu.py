@@ -110,6 +110,28 @@ def test_wide_and_single_kernel_batches_agree(device):
         _close(a, b, 1e-13)
 
 
+def test_grouped_in_parts_matches_one_call(device, monkeypatch):
+    """A long OM list runs in parts of bounded size; every OM gets the charge and
+    bins of the single call (segment sums may round differently)."""
+    from lighthit_amd import order1
+    from lighthit_amd.compile import DeviceChords
+    from lighthit_amd.order1 import (DeviceOMs, DeviceOptics, DeviceUnitChords, PromptSettings,
+                                     compress, grouped)
+    w, d, tables, el, origins, phase = _setup()
+    alpha = acceptance_alpha(d)
+    unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
+    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, PromptSettings(), device)
+    pencils = compress(unit, 0.5, 96, 192)
+    merge = (2e-3, 0.08, 0.5, 0.02)
+    work = np.arange(len(d))
+    whole = [t.cpu().numpy() for t in grouped(pencils, oms, work, optics, merge)]
+    monkeypatch.setattr(order1, "MAX_GROUPED_ENTRIES", 7 * int(pencils.count))
+    parts = [t.cpu().numpy() for t in grouped(pencils, oms, work, optics, merge)]
+    for a, b in zip(whole, parts):
+        _close(b, a, 1e-12)
+
+
 def test_spectral_order1_with_global_screen_matches_oracle(device):
     from lighthit_amd.compile import DeviceChords
     from lighthit_amd.order1 import DeviceOMs, spectral_order1
