@@ -72,6 +72,18 @@ int lh_readout_bins(const float* basis, const double* omega, const void* spectru
                     float* bins, void* stream);
 
 /*
+ * Stage (c) in split precision (the precision form of the same call sites):
+ * basis_hi, basis_lo (n_bins, 2*n_freq) FP16 with basis = (hi + lo) /
+ * row_scale[row] (row_scale (n_bins) float, powers of two); the spectrum is
+ * split into FP16 hi + lo per observation after a power-of-two scale, and the
+ * products hi*hi + hi*lo + lo*hi accumulate in FP32 on FP16 MFMA. The phase
+ * w * origin is formed and reduced in FP64. Other arguments as lh_readout_bins.
+ */
+int lh_readout_bins_split(const void* basis_hi, const void* basis_lo, const float* row_scale,
+                          const double* omega, const void* spectrum, const double* origins,
+                          int n_bins, int n_freq, int n_obs, float* bins, void* stream);
+
+/*
  * Stage (b): order >=2 response of OMs to a fused multi-band axial source,
  * applied with the band-folded kernel pack.
  *

@@ -27,6 +27,7 @@ SIGNATURES = {
     "lh_clear_error": (None, []),
     "lh_abi_version": (_i, []),
     "lh_readout_bins": (_i, [_p, _p, _p, _p, _i, _i, _i, _p, _p]),
+    "lh_readout_bins_split": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _p, _p]),
     "lh_timing_enable": (_i, [_i]),
     "lh_timing_read": (_i, [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong)]),
     "lh_timing_reset": (_i, []),

@@ -129,10 +129,29 @@ class ReadoutBasis:
     def period_ns(self):
         return 2 * np.pi / float(np.max(np.diff(self.grid))) if len(self.grid) > 1 else np.inf
 
-    def apply(self, spectrum, origins=None, out=None, stream=None):
+    def split(self):
+        """Device FP16 ``(hi, lo, row_scale)`` of ``[Re M | -Im M]`` for the split-precision
+        readout: every row scaled by a power of two so its largest value lies in
+        [2^13, 2^14), hi its FP16 rounding and lo the FP16 rounding of the rest."""
+        if getattr(self, "_split", None) is None:
+            import torch
+            packed = np.concatenate([self.matrix.real, -self.matrix.imag], axis=1)
+            peak = np.abs(packed).max(axis=1)
+            exponent = np.where(peak > 0, np.floor(np.log2(np.where(peak > 0, peak, 1.0))), 0.0)
+            row_scale = np.ldexp(1.0, (13 - exponent).astype(int))
+            scaled = packed * row_scale[:, None]
+            hi = scaled.astype(np.float16)
+            lo = (scaled - hi.astype(np.float64)).astype(np.float16)
+            device = self.device.device
+            self._split = (torch.as_tensor(hi, device=device), torch.as_tensor(lo, device=device),
+                           torch.as_tensor(row_scale.astype(np.float32), device=device))
+        return self._split
+
+    def apply(self, spectrum, origins=None, out=None, stream=None, precision="fp32"):
         """Bins (observation, bin) float32 of device spectra (frequency, observation) complex64.
 
         ``origins`` (observation,) float64 demodulates absolute-time spectra.
+        ``precision`` "fp32" (FP32 MFMA) or "split" (FP16 hi + lo MFMA).
         """
         import torch
         if spectrum.dtype != torch.complex64 or spectrum.dim() != 2:
@@ -144,10 +163,19 @@ class ReadoutBasis:
             raise ValueError("origins must be a float64 tensor with one value per observation")
         if out is None:
             out = torch.empty((nobs, self.bins), dtype=torch.float32, device=spectrum.device)
+        if precision not in ("fp32", "split"):
+            raise ValueError("precision must be 'fp32' or 'split'")
         with _lib.stage("readout"):
-            _lib.call("lh_readout_bins", _lib.ptr(self.device), _lib.ptr(self.omega_device),
-                      _lib.ptr(spectrum), _lib.ptr(origins), self.bins, nfreq, nobs,
-                      _lib.ptr(out), _lib.stream_handle(stream))
+            if precision == "split":
+                hi, lo, row_scale = self.split()
+                _lib.call("lh_readout_bins_split", _lib.ptr(hi), _lib.ptr(lo),
+                          _lib.ptr(row_scale), _lib.ptr(self.omega_device), _lib.ptr(spectrum),
+                          _lib.ptr(origins), self.bins, nfreq, nobs, _lib.ptr(out),
+                          _lib.stream_handle(stream))
+            else:
+                _lib.call("lh_readout_bins", _lib.ptr(self.device), _lib.ptr(self.omega_device),
+                          _lib.ptr(spectrum), _lib.ptr(origins), self.bins, nfreq, nobs,
+                          _lib.ptr(out), _lib.stream_handle(stream))
         return out
 
 
