@@ -198,7 +198,7 @@ def near_field_ge2(kernel, elements, device_pack, time_origin_ns, *, radius_m=No
 
 
 def add_near_field(response, kernel, elements, device_pack, readout, *, label=None,
-                   candidates=None, device="cuda"):
+                   candidates=None, device="cuda", precision="fp32"):
     """Add the clamped near-field order >=2 to a ``HybridTransportResponse``
     (``near_field.py:176-217``): charges of the near OMs, bins of the active ones
     (``readout``, a :class:`lighthit_amd.readout.ReadoutBasis` of the response's
@@ -221,7 +221,8 @@ def add_near_field(response, kernel, elements, device_pack, readout, *, label=No
     if len(detailed):
         spectrum = torch.as_tensor(np.ascontiguousarray(near.spectrum_pe[:, detailed],
                                                         np.complex64), device=device)
-        response.components_pe[detailed, :, 2] += readout.apply(spectrum).double().cpu().numpy()
+        response.components_pe[detailed, :, 2] += readout.apply(
+            spectrum, precision=precision).double().cpu().numpy()
     if response.spectrum_ge2_pe is not None:
         response.spectrum_ge2_pe[:, oms] += (
             near.spectrum_pe[:, oms]

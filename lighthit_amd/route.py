@@ -145,8 +145,13 @@ class ShowerConfig:
     predictor_safety: float = 2.0
     prompt_delay_ns: float = 20.0
     delayed_cell_ns: float = 20.0
+    #: "fp32": the plain form; "mixed": the precision form (the readout's
+    #: inverse Fourier product in FP16 hi + lo on MFMA)
+    precision: str = "fp32"
 
     def __post_init__(self):
+        if self.precision not in ("fp32", "mixed"):
+            raise ValueError("precision must be 'fp32' or 'mixed'")
         if not 1 <= int(self.band_rank) <= 9 or int(self.fold_subdivide) < 1:
             raise ValueError("band_rank must lie in 1..9 and fold_subdivide be positive")
         if (not self.predictor_safety >= 1 or not self.predictor_cell_m > 0
@@ -810,6 +815,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     from .fold import frequency_indices
     config = config or ShowerConfig()
     basis = config.basis
+    readout_precision = "split" if config.precision == "mixed" else "fp32"
     began, cpu = perf_counter(), process_time()
     timings = {}
     stage = perf_counter()
@@ -1025,13 +1031,15 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         device_spectrum = torch.as_tensor(np.ascontiguousarray(spectrum[:, ids], np.complex64),
                                           device=device)
         device_origins = torch.as_tensor(origins[ids], device=device)
-        components[ids, :, 2] = readout.apply(device_spectrum, device_origins).double().cpu().numpy()
+        components[ids, :, 2] = readout.apply(device_spectrum, device_origins,
+                                              precision=readout_precision).double().cpu().numpy()
         for shift, values in late_spectra:
             basis_shift = _readout_basis(kernel, omega, edges - shift, device)
             late = torch.as_tensor(np.ascontiguousarray(values[:, ids], np.complex64),
                                    device=device)
             components[ids, :, 2] += basis_shift.apply(
-                late, torch.as_tensor(origins[ids] + shift, device=device)).double().cpu().numpy()
+                late, torch.as_tensor(origins[ids] + shift, device=device),
+                precision=readout_precision).double().cpu().numpy()
     timings["readout_seconds"] = perf_counter() - stage
     elapsed = perf_counter() - began
     timings.update(elapsed_seconds=elapsed, cpu_seconds=process_time() - cpu)
@@ -1081,7 +1089,8 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         readout = (route.readout(omega, edges) if route is not None else
                    _readout_basis(kernel, omega, edges, device))
         add_near_field(response, kernel, elements, near_pack, readout, label="shower",
-                       candidates=candidates, device=device)
+                       candidates=candidates, device=device,
+                       precision="split" if config.precision == "mixed" else "fp32")
     return response
 
 
