@@ -110,6 +110,8 @@ typedef struct lh_ge2_pack {
     const double* omega;        /* (freq_stride) [rad/ns] */
     int bands, intervals, degree, acceptance_degree, freq_stride;
     int flight;                 /* 1: radial splines carry the flight phase */
+    int coefficient_format;     /* 0: complex64; 1: bfloat16 (re, im) pairs, 4 bytes each
+                                   (the precision form; FP32 accumulation either way) */
 } lh_ge2_pack;
 
 typedef struct lh_ge2_source {

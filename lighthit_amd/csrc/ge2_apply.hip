@@ -35,7 +35,8 @@ constexpr int kGroup = 8;     // points per degree-outer group in the wide kerne
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 struct Ge2Params {
-    const float2* coef;        // (bands, intervals, degree+1, pairs, 4, cstride)
+    const float2* coef;        // (bands, intervals, degree+1, pairs, 4, cstride) complex64
+    const unsigned* coef16;    // the same as bfloat16 (re, im) pairs: re low, im high half
     const double* log_r;       // (intervals+1)
     const double* absorption;  // (bands)
     const double* speed;       // (bands)
@@ -58,6 +59,17 @@ struct Ge2Params {
 };
 
 __device__ __forceinline__ double parity(int n) { return (n & 1) ? -1.0 : 1.0; }
+
+// Pack coefficient i: complex64 (F = 0) or a bfloat16 pair widened exactly (F = 1).
+template <int F>
+__device__ __forceinline__ float2 coef_at(const Ge2Params& P, size_t i) {
+    if constexpr (F == 0) {
+        return P.coef[i];
+    } else {
+        const unsigned v = P.coef16[i];
+        return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
+    }
+}
 
 __device__ __forceinline__ double ipow(double x, int k) {
     double r = 1.0;
@@ -311,7 +323,7 @@ struct Record {
     }
 };
 
-template <int A, int M>
+template <int A, int M, int F>
 __global__ __launch_bounds__(kThreads, 2) void ge2_wide(Ge2Params P) {
     using R = Record<A, M>;
     constexpr int NA = R::NA, NM = R::NM, NPAIR = R::NPAIR;
@@ -398,13 +410,14 @@ __global__ __launch_bounds__(kThreads, 2) void ge2_wide(Ge2Params P) {
                         if (irec[0] == 0) continue;
                         const int j = irec[1];
                         if (j != jc) {
-                            const float2* cdeg = P.coef
-                                + (((size_t)lam * P.intervals + j) * (P.degree + 1) + l) * cl + w;
+                            const size_t cdeg
+                                = (((size_t)lam * P.intervals + j) * (P.degree + 1) + l) * cl + w;
 #pragma unroll
                             for (int k = 0; k < NPAIR; ++k)
 #pragma unroll
                                 for (int pw = 0; pw < 4; ++pw)
-                                    cf[k][pw] = cdeg[(size_t)(k * 4 + pw) * P.cstride];
+                                    cf[k][pw] = coef_at<F>(P, cdeg + (size_t)(k * 4 + pw)
+                                                                     * P.cstride);
                             jc = j;
                         }
                         const float t = rec[2];
@@ -536,7 +549,7 @@ int wide_splits(int nrec, int npoints) {
 }
 
 // One frequency: one thread per (OM, point); fixed-order tree reduction.
-template <int A, int M>
+template <int A, int M, int F>
 __global__ __launch_bounds__(kThreads) void ge2_narrow(Ge2Params P, int w) {
     constexpr int NA = 2 * A + 1, NM = 2 * M + 1;
     __shared__ float2 red[kMaxBands][kThreads];
@@ -579,17 +592,18 @@ __global__ __launch_bounds__(kThreads) void ge2_narrow(Ge2Params P, int w) {
                 float2 cs[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) cs[a] = make_float2(0.f, 0.f);
-                const float2* cdeg = P.coef + ((size_t)lam * P.intervals + g.j) * (P.degree + 1) * cl
-                                     + (size_t)l * cl + w;
+                const size_t cdeg = ((size_t)lam * P.intervals + g.j) * (P.degree + 1) * cl
+                                    + (size_t)l * cl + w;
                 int k = 0;
 #pragma unroll
                 for (int dl = 0; dl <= A; ++dl)
 #pragma unroll
                     for (int mu = 0; mu <= dl; ++mu, ++k) {
                         if (mu > top_p) continue;
-                        const float2* c = cdeg + (size_t)k * 4 * P.cstride;
-                        const float2 c0 = c[0], c1 = c[P.cstride], c2 = c[2 * P.cstride],
-                                     c3 = c[3 * P.cstride];
+                        const size_t c = cdeg + (size_t)k * 4 * P.cstride;
+                        const float2 c0 = coef_at<F>(P, c), c1 = coef_at<F>(P, c + P.cstride),
+                                     c2 = coef_at<F>(P, c + 2 * P.cstride),
+                                     c3 = coef_at<F>(P, c + 3 * P.cstride);
                         float2 radial;
                         radial.x = ((c0.x * t + c1.x) * t + c2.x) * t + c3.x;
                         radial.y = ((c0.y * t + c1.y) * t + c2.y) * t + c3.y;
@@ -657,12 +671,13 @@ __global__ __launch_bounds__(kThreads) void ge2_narrow(Ge2Params P, int w) {
 
 constexpr int kLdsBudget = 64 * 1024;
 
-template <int A, int M>
+template <int A, int M, int F>
 int launch(Ge2Params P, hipStream_t stream) {
     if (P.nfreq == 1) {
         {
             lh::KernelMark mark("ge2_narrow", stream);
-            hipLaunchKernelGGL((ge2_narrow<A, M>), dim3(P.nrec), dim3(kThreads), 0, stream, P, 0);
+            hipLaunchKernelGGL((ge2_narrow<A, M, F>), dim3(P.nrec), dim3(kThreads), 0, stream, P,
+                               0);
         }
         return lh::check_launch("lh_ge2_apply (narrow)");
     }
@@ -676,7 +691,7 @@ int launch(Ge2Params P, hipStream_t stream) {
     P.span = (P.npoints + P.nsplit - 1) / P.nsplit;
     {
         lh::KernelMark mark("ge2_wide", stream);
-        hipLaunchKernelGGL((ge2_wide<A, M>), dim3(P.nrec, P.nsplit), dim3(kThreads), lds, stream,
+        hipLaunchKernelGGL((ge2_wide<A, M, F>), dim3(P.nrec, P.nsplit), dim3(kThreads), lds, stream,
                            P);
     }
     int status = lh::check_launch("lh_ge2_apply (wide)");
@@ -686,14 +701,14 @@ int launch(Ge2Params P, hipStream_t stream) {
     return lh::check_launch("lh_ge2_apply (wide reduce)");
 }
 
-template <int A>
+template <int A, int F>
 int dispatch_m(int m, const Ge2Params& P, hipStream_t s) {
     switch (m) {
-        case 0: return launch<A, 0>(P, s);
-        case 1: return launch<A, 1>(P, s);
-        case 2: return launch<A, 2>(P, s);
-        case 3: return launch<A, 3>(P, s);
-        case 4: return launch<A, 4>(P, s);
+        case 0: return launch<A, 0, F>(P, s);
+        case 1: return launch<A, 1, F>(P, s);
+        case 2: return launch<A, 2, F>(P, s);
+        case 3: return launch<A, 3, F>(P, s);
+        case 4: return launch<A, 4, F>(P, s);
         default: break;
     }
     lh::set_error("lh_ge2_apply: azimuthal degree %d not built (0..4)", m);
@@ -736,7 +751,11 @@ extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source
                                                       && source->reference_ns)),
                "lh_ge2_apply: null device pointer");
     Ge2Params P{};
-    P.coef = static_cast<const float2*>(pack->coefficients);
+    LH_REQUIRE(pack->coefficient_format == 0 || pack->coefficient_format == 1,
+               "lh_ge2_apply: coefficient_format must be 0 (complex64) or 1 (bfloat16 pairs)");
+    const bool bf16 = pack->coefficient_format == 1;
+    P.coef = bf16 ? nullptr : static_cast<const float2*>(pack->coefficients);
+    P.coef16 = bf16 ? static_cast<const unsigned*>(pack->coefficients) : nullptr;
     P.log_r = pack->log_r;
     P.absorption = pack->absorption;
     P.speed = pack->speed;
@@ -769,8 +788,8 @@ extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source
     const int m = std::min(source->azimuthal_degree, pack->degree);
     hipStream_t s = static_cast<hipStream_t>(stream);
     switch (pack->acceptance_degree) {
-        case 2: return dispatch_m<2>(m, P, s);
-        case 3: return dispatch_m<3>(m, P, s);
+        case 2: return bf16 ? dispatch_m<2, 1>(m, P, s) : dispatch_m<2, 0>(m, P, s);
+        case 3: return bf16 ? dispatch_m<3, 1>(m, P, s) : dispatch_m<3, 0>(m, P, s);
         default: break;
     }
     lh::set_error("lh_ge2_apply: acceptance degree %d not built (2, 3)", pack->acceptance_degree);
@@ -943,6 +962,7 @@ extern "C" int lh_near_field(const lh_ge2_pack* pack, const lh_near_portions* po
                "lh_near_field: degree <= %d and 1..%d bands", kNearMaxDegree, kNearMaxBands);
     LH_REQUIRE(pack->acceptance_degree >= 0 && pack->acceptance_degree <= 4,
                "lh_near_field: acceptance degree must lie in 0..4");
+    LH_REQUIRE(pack->coefficient_format == 0, "lh_near_field: complex64 coefficients only");
     LH_REQUIRE(radial_low < radial_high && clamp_low <= radial_low,
                "lh_near_field: need clamp_low <= radial_low < radial_high");
     const lh_near_portions& p = *portions;

@@ -23,7 +23,7 @@ class _Pack(ctypes.Structure):
                 ("omega", ctypes.c_void_p), ("bands", ctypes.c_int),
                 ("intervals", ctypes.c_int), ("degree", ctypes.c_int),
                 ("acceptance_degree", ctypes.c_int), ("freq_stride", ctypes.c_int),
-                ("flight", ctypes.c_int)]
+                ("flight", ctypes.c_int), ("coefficient_format", ctypes.c_int)]
 
 
 class _Source(ctypes.Structure):
@@ -88,8 +88,12 @@ def _address(tensor):
 class DevicePack:
     """A folded ``Ge2KernelStore``-format bundle resident in device memory."""
 
-    def __init__(self, bundle, device="cuda"):
+    def __init__(self, bundle, device="cuda", coefficient_format="complex64"):
+        """``coefficient_format`` "bfloat16" keeps the coefficients as bfloat16
+        (re, im) pairs, half the bytes (the precision form)."""
         import torch
+        if coefficient_format not in ("complex64", "bfloat16"):
+            raise ValueError("coefficient_format must be 'complex64' or 'bfloat16'")
         coefficients = bundle["coefficients"]
         resident = isinstance(coefficients, torch.Tensor)
         if not resident:
@@ -126,19 +130,26 @@ class DevicePack:
         self.d_absorption = as_dev(self.absorption, torch.float64)
         self.d_speed = as_dev(self.speed, torch.float64)
         self.d_omega = as_dev(self.omega, torch.float64)
+        self.coefficient_format = coefficient_format
+        width = 8
+        if coefficient_format == "bfloat16":
+            # (re, im) bfloat16 pairs: one 32-bit word per coefficient, re in the low half
+            self.d_coef = torch.view_as_real(self.d_coef).to(torch.bfloat16).contiguous()
+            width = 4
         self.struct = _Pack(self.d_coef.data_ptr(), self.d_log_r.data_ptr(),
                             self.d_absorption.data_ptr(), self.d_speed.data_ptr(),
                             self.d_omega.data_ptr(), bands, intervals, self.degree,
-                            self.acceptance_degree, nfreq, int(self.flight))
-        self.nbytes = int(np.prod(coefficients.shape)) * 8
-        self._band_stride = intervals * ndeg * npair * 4 * nfreq * 8
+                            self.acceptance_degree, nfreq, int(self.flight),
+                            int(coefficient_format == "bfloat16"))
+        self.nbytes = int(np.prod(coefficients.shape)) * width
+        self._band_stride = intervals * ndeg * npair * 4 * nfreq * width
 
     def band(self, j):
         """Descriptor of node band ``j`` alone (for sources on different point sets)."""
         s = self.struct
         view = _Pack(s.coefficients + j * self._band_stride, s.log_r, s.absorption + 8 * j,
                      s.speed + 8 * j, s.omega, 1, s.intervals, s.degree, s.acceptance_degree,
-                     s.freq_stride, s.flight)
+                     s.freq_stride, s.flight, s.coefficient_format)
         view.bands = 1
         return view
 

@@ -146,12 +146,14 @@ class ShowerConfig:
     prompt_delay_ns: float = 20.0
     delayed_cell_ns: float = 20.0
     #: "fp32": the plain form; "mixed": the precision form (the readout's
-    #: inverse Fourier product in FP16 hi + lo on MFMA)
+    #: inverse Fourier product in FP16 hi + lo on MFMA); "mixed_bf16": also
+    #: order >=2 from bfloat16 pack coefficients with FP32 accumulation (half
+    #: the pack's bytes; its order >=2 charges stray by a few parts in 10^3)
     precision: str = "fp32"
 
     def __post_init__(self):
-        if self.precision not in ("fp32", "mixed"):
-            raise ValueError("precision must be 'fp32' or 'mixed'")
+        if self.precision not in ("fp32", "mixed", "mixed_bf16"):
+            raise ValueError("precision must be 'fp32', 'mixed' or 'mixed_bf16'")
         if not 1 <= int(self.band_rank) <= 9 or int(self.fold_subdivide) < 1:
             raise ValueError("band_rank must lie in 1..9 and fold_subdivide be positive")
         if (not self.predictor_safety >= 1 or not self.predictor_cell_m > 0
@@ -714,6 +716,10 @@ class ShowerTransport:
             device_pack = DevicePack(folded, self.device)
             self._tables = dict(pack=pack, fold=fold, folded=folded, device_pack=device_pack,
                                 fold_seconds=perf_counter() - began)
+            if self.config.precision == "mixed_bf16":
+                # order >=2 from bfloat16 coefficients
+                self._tables["device_pack_bf16"] = DevicePack(folded, self.device,
+                                                              coefficient_format="bfloat16")
         return self._tables
 
     def order0(self):
@@ -815,7 +821,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     from .fold import frequency_indices
     config = config or ShowerConfig()
     basis = config.basis
-    readout_precision = "split" if config.precision == "mixed" else "fp32"
+    readout_precision = "fp32" if config.precision == "fp32" else "split"
     began, cpu = perf_counter(), process_time()
     timings = {}
     stage = perf_counter()
@@ -851,6 +857,10 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     if extra.shape != (n,) or not np.isfinite(extra).all():
         raise ValueError("selection_extra_charge must hold one finite value per OM")
     fold, folded, pack = tables["fold"], tables["folded"], tables["device_pack"]
+    if config.precision == "mixed_bf16":
+        from .ge2 import DevicePack
+        pack = tables.get("device_pack_bf16") or DevicePack(folded, device,
+                                                            coefficient_format="bfloat16")
     if not np.allclose(folded["omega"], omega, rtol=0, atol=1e-12):
         raise ValueError("folded >=2 kernels and route frequencies differ")
     timings["setup_seconds"] = perf_counter() - stage
@@ -1090,7 +1100,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
                    _readout_basis(kernel, omega, edges, device))
         add_near_field(response, kernel, elements, near_pack, readout, label="shower",
                        candidates=candidates, device=device,
-                       precision="split" if config.precision == "mixed" else "fp32")
+                       precision=readout_precision)
     return response
 
 

@@ -81,3 +81,25 @@ def test_invalid_window_raises(device):
     bundle, sources, positions, looks, alpha, area = make_case(8, points=4, receivers=2, freq=3)
     with pytest.raises(ValueError, match="radial window"):
         apply_folded(bundle, sources, positions, looks, alpha, area, (1.0, 150.0))
+
+
+@pytest.mark.parametrize("seed,kw", [(0, dict()), (2, dict(degree=16, azimuthal=4, points=30,
+                                                             receivers=5))])
+def test_bfloat16_coefficients_track_the_oracle(device, seed, kw):
+    """The precision form's bfloat16 pack (8-bit significands, FP32 accumulation)
+    against the FP64 oracle: errors of a few parts in 10^3 of the largest value,
+    the same order as the coefficients' own rounding (2^-9)."""
+    from lighthit_amd.ge2 import DevicePack, apply_folded
+    args = dict(points=165, receivers=34)
+    args.update(kw)
+    bundle, sources, positions, looks, alpha, area = make_case(seed, **args)
+    window = (2.0, 150.0)
+    expected = oracle.apply_folded(bundle, sources, positions, looks, alpha, area, window)
+    pack = DevicePack(bundle, coefficient_format="bfloat16")
+    assert pack.d_coef.element_size() * pack.d_coef.numel() * 2 == (
+        DevicePack(bundle).d_coef.element_size() * DevicePack(bundle).d_coef.numel())
+    got = apply_folded(bundle, sources, positions, looks, alpha, area, window, pack=pack)
+    _check(got, expected, 5e-3)
+    # and it is not the complex64 result: the storage really is bfloat16
+    plain = apply_folded(bundle, sources, positions, looks, alpha, area, window)
+    assert np.abs(got - plain).max() > 0

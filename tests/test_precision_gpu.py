@@ -1,5 +1,6 @@
-"""Precision form (FP16 hi + lo readout) beside the plain form on the
-demonstration event: charge error per order (0, 1, >=2) as max and median
+"""Precision forms beside the plain form on the demonstration event ("mixed":
+FP16 hi + lo readout; "mixed_bf16": also order >=2 from bfloat16 pack
+coefficients): charge error per order (0, 1, >=2) as max and median
 relative error, and the bins' L1 error per order, both against the FP64
 oracle route. The target is a charge within 1e-3 of FP64 on every OM that
 carries light of that order."""
@@ -21,14 +22,16 @@ def forms(device):
     from lighthit_amd.route import ShowerTransport
     w, kernel, pack, route = demonstration_route(device)
     plain = route(w.elements)
-    mixed_route = ShowerTransport(kernel, pack, device=device,
-                                  config=replace(route.config, precision="mixed"))
-    mixed = mixed_route(w.elements)
+    forms = {"fp32": plain}
+    for form in ("mixed", "mixed_bf16"):
+        other = ShowerTransport(kernel, pack, device=device,
+                                config=replace(route.config, precision=form))
+        forms[form] = other(w.elements)
     bundle = pack.bundle(kernel.wavelength.wavelength_nm, kernel.config.source_degree,
                          np.arange(65))
     want = oracle_route.transport_shower(kernel, w.elements, bundle, route.tables()["fold"],
                                          config=route.config)
-    return {"fp32": plain, "mixed": mixed}, want
+    return forms, want
 
 
 def error_table(response, want):
@@ -54,12 +57,19 @@ def test_precision_report_meets_the_charge_target(forms):
     responses, want = forms
     report = {form: error_table(r, want) for form, r in responses.items()}
     print("precision report:", json.dumps(report))
-    for form, table in report.items():
-        for order, row in table.items():
+    for form in ("fp32", "mixed"):
+        for order, row in report[form].items():
             assert row["charge_max_rel"] <= 1e-3, (form, order, row)
+    # bfloat16 coefficients (2^-9 rounding) keep orders 0 and 1 and leave
+    # order >=2 a few parts in 10^3 off: outside the 1e-3 target
+    bf16 = report["mixed_bf16"]
+    assert bf16["0"] == report["fp32"]["0"] and bf16["1"] == report["fp32"]["1"]
+    assert bf16[">=2"]["charge_max_rel"] <= 1e-2
 
 
-def test_precision_form_changes_only_the_readout(forms):
+def test_mixed_form_changes_only_the_readout(forms):
+    """The mixed form keeps every charge and the orders 0 and 1 bins of the
+    plain form; its order >=2 bins differ by the split product's rounding."""
     responses, _ = forms
     plain, mixed = responses["fp32"], responses["mixed"]
     assert np.array_equal(plain.charge_components_pe, mixed.charge_components_pe)
