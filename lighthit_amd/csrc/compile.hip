@@ -456,18 +456,37 @@ __global__ void __launch_bounds__(64) compile_phase(Params p, int nfreq, double 
         for (int k = 0; k < 5; ++k) re[b][k] = im[b][k] = 0.0;
     const bool second = lane + 64 < nfreq;
     const bool first = lane < nfreq;
-    for (int r = row_lo; r < row_hi; ++r) {
-        const double* d = deposit + (size_t)r * width + c;
-        const double* tc = table_c + (size_t)r * nfreq;
-        const double* ts = table_s + (size_t)r * nfreq;
-        double c0 = first ? tc[lane] : 0.0, s0 = first ? ts[lane] : 0.0;
-        double c1 = second ? tc[lane + 64] : 0.0, s1 = second ? ts[lane + 64] : 0.0;
-        for (int k = 0; k < L.moments; ++k) {
-            double v = d[k * p.channels];
-            re[0][k] += v * c0;
-            im[0][k] += v * s0;
-            re[1][k] += v * c1;
-            im[1][k] += v * s1;
+    // rows four at a time: every load of the four is issued before the sums
+    // that use them, in row order as before
+    constexpr int kRows = 4;
+    const int moments = L.moments;
+    for (int r0 = row_lo; r0 < row_hi; r0 += kRows) {
+        double c0[kRows], s0[kRows], c1[kRows], s1[kRows], v[kRows][5];
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) {
+            const int r = r0 + i;
+            const bool in = r < row_hi;
+            const double* d = deposit + (size_t)r * width + c;
+            const double* tc = table_c + (size_t)r * nfreq;
+            const double* ts = table_s + (size_t)r * nfreq;
+            c0[i] = in && first ? tc[lane] : 0.0;
+            s0[i] = in && first ? ts[lane] : 0.0;
+            c1[i] = in && second ? tc[lane + 64] : 0.0;
+            s1[i] = in && second ? ts[lane + 64] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) v[i][k] = in && k < moments ? d[k * p.channels] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) {
+            if (r0 + i >= row_hi) break;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                if (k >= moments) break;
+                re[0][k] += v[i][k] * c0[i];
+                im[0][k] += v[i][k] * s0[i];
+                re[1][k] += v[i][k] * c1[i];
+                im[1][k] += v[i][k] * s1[i];
+            }
         }
     }
     int slices = p.info[kInfoSlices], wide_count = p.info[kInfoWide];
