@@ -405,12 +405,31 @@ __global__ void __launch_bounds__(kThreads) compile_deposit(Params p, int rows, 
         double acc = 0.0;
         for (int l = 0; l < links; ++l) {
             int j = link[l][0], dt = link[l][1];
-            for (int q = range[l][0]; q < range[l][1]; ++q) {
-                int id = p.sorted_ids[q];
-                const NodeRecord& rec = p.records[id];
-                double wt = L.time_deposit == 1 ? (dt ? rec.tr : 1 - rec.tr) : 1.0;
-                double w = rec.wz[j] * wt;
-                acc += (w * rec.f[k]) * p.angular[(size_t)(id % count) * p.channels + c];
+            const int q_hi = range[l][1];
+            // nodes four at a time: their loads are issued before the sums,
+            // which run in node order as before
+            constexpr int kNodes = 4;
+            for (int q0 = range[l][0]; q0 < q_hi; q0 += kNodes) {
+                double wz[kNodes], tr[kNodes], f[kNodes], ang[kNodes];
+#pragma unroll
+                for (int i = 0; i < kNodes; ++i) {
+                    const int q = q0 + i;
+                    if (q < q_hi) {
+                        const int id = p.sorted_ids[q];
+                        const NodeRecord& rec = p.records[id];
+                        wz[i] = rec.wz[j];
+                        tr[i] = rec.tr;
+                        f[i] = rec.f[k];
+                        ang[i] = p.angular[(size_t)(id % count) * p.channels + c];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kNodes; ++i) {
+                    if (q0 + i >= q_hi) break;
+                    double wt = L.time_deposit == 1 ? (dt ? tr[i] : 1 - tr[i]) : 1.0;
+                    double w = wz[i] * wt;
+                    acc += (w * f[i]) * ang[i];
+                }
             }
         }
         deposit[(size_t)r * width + kc] = acc;
