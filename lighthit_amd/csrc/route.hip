@@ -29,15 +29,36 @@ __global__ void __launch_bounds__(kThreads) segment_bounds(const double* receive
     int d = blockIdx.x;
     double rx = receivers[3 * d], ry = receivers[3 * d + 1], rz = receivers[3 * d + 2];
     double bd = DBL_MAX, bt = DBL_MAX;
-    for (int i = threadIdx.x; i < n_el; i += kThreads) {
-        double ox = rx - starts[3 * i], oy = ry - starts[3 * i + 1], oz = rz - starts[3 * i + 2];
-        double ux = directions[3 * i], uy = directions[3 * i + 1], uz = directions[3 * i + 2];
-        double along = ox * ux + oy * uy + oz * uz;
-        along = fmin(fmax(along, 0.0), lengths[i]);
-        double dx = ox - along * ux, dy = oy - along * uy, dz = oz - along * uz;
-        double value = dx * dx + dy * dy + dz * dz;
-        bd = fmin(bd, value);
-        if (start_ns) bt = fmin(bt, start_ns[i] + flight * sqrt(value));
+    // four elements per thread at a time: every load is issued before the
+    // arithmetic (minima do not depend on the order)
+    constexpr int kBatch = 4;
+    for (int i0 = threadIdx.x; i0 < n_el; i0 += kBatch * kThreads) {
+        double sx[kBatch], sy[kBatch], sz[kBatch], ux[kBatch], uy[kBatch], uz[kBatch];
+        double len[kBatch], t0[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            const int i = i0 + b * kThreads;
+            const bool in = i < n_el;
+            sx[b] = in ? starts[3 * i] : 0.0;
+            sy[b] = in ? starts[3 * i + 1] : 0.0;
+            sz[b] = in ? starts[3 * i + 2] : 0.0;
+            ux[b] = in ? directions[3 * i] : 0.0;
+            uy[b] = in ? directions[3 * i + 1] : 0.0;
+            uz[b] = in ? directions[3 * i + 2] : 0.0;
+            len[b] = in ? lengths[i] : 0.0;
+            t0[b] = in && start_ns ? start_ns[i] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            if (i0 + b * kThreads >= n_el) break;
+            double ox = rx - sx[b], oy = ry - sy[b], oz = rz - sz[b];
+            double along = ox * ux[b] + oy * uy[b] + oz * uz[b];
+            along = fmin(fmax(along, 0.0), len[b]);
+            double dx = ox - along * ux[b], dy = oy - along * uy[b], dz = oz - along * uz[b];
+            double value = dx * dx + dy * dy + dz * dz;
+            bd = fmin(bd, value);
+            if (start_ns) bt = fmin(bt, t0[b] + flight * sqrt(value));
+        }
     }
     best_d[threadIdx.x] = bd;
     best_t[threadIdx.x] = bt;
