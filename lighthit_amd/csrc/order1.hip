@@ -1652,6 +1652,232 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
     return total;
 }
 
+// deposit_round for two wavelengths at once, one per half of the wave (lanes
+// 0-31 the low one, 32-63 the high one). Every half's events are in (panel,
+// bin) order; the flattened events carry their half in the key, so every
+// (wavelength, bin) is a contiguous run summed by one lane in panel order, as
+// in deposit_round. Returns the number of bin events of both halves.
+__device__ __forceinline__ int deposit_round_pair(const PanelEvents& ev, int lane,
+                                                  double* acc_lo, double* acc_hi,
+                                                  double* pre_lo, double* pre_hi, int* ev_key,
+                                                  double (*ev_val)[4], int* head_at) {
+    constexpr int kHalfBit = 1 << 30;
+    const int half = lane >> 5;
+    int mine = 0;
+    if (ev.kind == 1) mine = __popc(ev.mask);
+    else if (ev.kind == 2) mine = (ev.pre ? 0 : 1) + (ev.end ? 1 : 0);
+    int offset = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        int other = __shfl_up(offset, off, 64);
+        if (lane >= off) offset += other;
+    }
+    int n_events = __shfl(offset, 63, 64);
+    offset -= mine;
+    const int tag = half ? kHalfBit : 0;
+    if (ev.kind == 1) {
+#pragma unroll
+        for (int e = 0; e < kMaxDirect; ++e) {
+            if (!((ev.mask >> e) & 1)) continue;
+            ev_key[offset] = tag | (ev.k_lo + e);
+            ev_val[offset][0] = 0.0; ev_val[offset][1] = 0.0;
+            ev_val[offset][2] = 0.0; ev_val[offset][3] = ev.v[e];
+            ++offset;
+        }
+    } else if (ev.kind == 2) {
+        if (!ev.pre) {
+            ev_key[offset] = tag | ev.k_lo;
+            ev_val[offset][0] = ev.v[0]; ev_val[offset][1] = ev.v[1];
+            ev_val[offset][2] = ev.v[2]; ev_val[offset][3] = ev.v[3];
+            ++offset;
+        }
+        if (ev.end) {
+            ev_key[offset] = tag | ev.k_end;
+            ev_val[offset][0] = -ev.v[0]; ev_val[offset][1] = -ev.v[1];
+            ev_val[offset][2] = -ev.v[2]; ev_val[offset][3] = -ev.v_end;
+        }
+    }
+    wave_sync();
+    int heads = 0;
+    for (int c = 0; c < n_events; c += 64) {
+        int i = c + lane;
+        bool head = i < n_events && (i == 0 || ev_key[i] != ev_key[i - 1]);
+        unsigned long long ballot = __ballot(head);
+        if (head) head_at[heads + __popcll(ballot & ((1ULL << lane) - 1))] = i;
+        heads += __popcll(ballot);
+    }
+    wave_sync();
+    for (int h = lane; h < heads; h += 64) {
+        int i0 = head_at[h], i1 = h + 1 < heads ? head_at[h + 1] : n_events;
+        const int key = ev_key[i0];
+        double* a = ((key & kHalfBit) ? acc_hi : acc_lo) + 4 * (size_t)(key & (kHalfBit - 1));
+        double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+        for (int i = i0; i < i1; ++i) {
+            a0 += ev_val[i][0]; a1 += ev_val[i][1];
+            a2 += ev_val[i][2]; a3 += ev_val[i][3];
+        }
+        a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+    }
+    // window-front sums of the panels that start before the first edge, each
+    // half in panel order
+    unsigned long long pre = __ballot(ev.kind == 2 && ev.pre);
+    for (unsigned long long rest = pre; rest != 0; rest &= rest - 1) {
+        int q = __ffsll((long long)rest) - 1;
+        double a = __shfl(ev.v[0], q, 64), b = __shfl(ev.v[1], q, 64),
+               c = __shfl(ev.v[2], q, 64);
+        if (lane == 0) {
+            double* p = q < 32 ? pre_lo : pre_hi;
+            p[0] += a; p[1] += b; p[2] += c;
+        }
+    }
+    wave_sync();
+    return n_events;
+}
+
+__device__ __forceinline__ double half_sum(double v) {
+    for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// deposit_lambda for two wavelengths of one batch at once, one per half of
+// the wave (lam_hi < 0 or mus = 0 leaves a half idle): each half spreads its
+// panels over its 32 lanes, 32 at a time, and both halves share every
+// deposit round. Returns, in every lane, the charge of its half's wavelength.
+__device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int lam_hi,
+                                               const BatchHead& h, const double* path,
+                                               const double* g0, const double* g2,
+                                               double* acc_lo, double* acc_hi, double* pre_lo,
+                                               double* pre_hi, int lane, int* ev_key,
+                                               double (*ev_val)[4], int* head_at,
+                                               unsigned long long& count_panels,
+                                               unsigned long long& count_events) {
+    const int half = lane >> 5, sub = lane & 31;
+    const int lam = half ? lam_hi : lam_lo;
+    bool active = lam >= 0 && G.opt.mus[lam] != 0.0;
+    const int lam_s = active ? lam : lam_lo;   // a valid index for the idle half
+    int nb = G.opt.bins;
+    int n = h.n;
+    double r_b = h.r_b, t_b = h.t_b;
+    double v = G.opt.speed[lam_s];
+    double mut = G.opt.mut[lam_s];
+    double kappa = -mut * v;
+    // last as in deposit_lambda, per half
+    double limit = r_b + G.opt.tail_exponent / mut;
+    int within = 0;
+    for (int k0 = 0; k0 < n - 1; k0 += 32) {
+        int k = k0 + sub;
+        unsigned long long ballot = __ballot(active && k < n - 1 && path[k] <= limit);
+        within += __popcll(half ? (ballot >> 32) : (ballot & 0xffffffffULL));
+    }
+    int last = min(max(within, 2), n - 1);
+    if (last < 2) active = false;
+    if (!active) last = 0;   // an idle half reads node 0 only (every batch has one)
+    double spread = fmin(sqrt(12.0 * h.tau_var), (path[last] - r_b) / v);
+    double front_time = t_b + r_b / v;
+    double end_time = t_b + path[last] / v;
+    int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
+                  || end_time + 0.5 * spread <= G.opt.edges[0];
+    double t_base = outside ? 0.0 : t_b;
+    double e_emit = exp(kappa * t_base);
+    double h_scale = G.opt.mus[lam_s] * v / e_emit;
+    double w0 = G.opt.s0w[lam_s] * h_scale, w2 = G.opt.s2w[lam_s] * h_scale;
+    LambdaBins B;
+    B.inverse_width = G.opt.bin_inverse_width;
+    B.edges = G.opt.edges;
+    B.eedge = G.opt.eedge + (size_t)lam_s * (nb + 1);
+    B.acc = half ? acc_hi : acc_lo;
+    B.pre = half ? pre_hi : pre_lo;
+    B.nb = nb;
+    B.direct_bins = G.opt.direct_bins;
+    bool bins = active && G.compute_bins && !outside;
+    bool bins_any = __ballot(bins) != 0;
+    double h_front = w0 * g0[0] + w2 * g2[0];
+    double e_first = e_emit * exp(-mut * path[0]);
+    double e_last = e_emit * exp(-mut * path[last]);
+    int panels = active ? last / 2 + (last & 1) : 0;
+    int panels_max = max(__shfl(panels, 0, 64), __shfl(panels, 32, 64));
+    double total_lane = 0.0;
+    unsigned long long events_total = 0;
+    for (int base = 0; base < panels_max; base += 32) {
+        int i = base + sub;
+        PanelEvents ev;
+        ev.kind = 0;
+        if (i < panels) {
+            double t_a, t_e, alpha, beta, gamma, e_a, e_b;
+            int j;
+            if (i < last / 2) {
+                j = 2 * i;
+                t_a = t_base + path[j] / v;
+                double t_m = t_base + path[j + 1] / v;
+                t_e = t_base + path[j + 2] / v;
+                double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
+                double u_m = t_m - t_a, u_b = t_e - t_a;
+                double d1 = (h1 - h0) / u_m;
+                double d2 = ((h2 - h1) / (t_e - t_m) - d1) / u_b;
+                alpha = h0;
+                beta = d1 - d2 * u_m;
+                gamma = d2;
+                e_b = e_emit * exp(-mut * path[j + 2]);
+            } else {
+                j = last - 1;
+                double t_p = t_base + path[j - 1] / v;
+                t_a = t_base + path[j] / v;
+                t_e = t_base + path[j + 1] / v;
+                double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
+                double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
+                double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
+                double u_m = t_a - t_p, u_b = t_e - t_p;
+                double d1 = (h0 - hp) / u_m;
+                double d2 = ((h1 - h0) / (t_e - t_a) - d1) / u_b;
+                alpha = h0;
+                beta = d1 - d2 * u_m + 2.0 * d2 * u_m;
+                gamma = d2;
+                e_b = e_emit * exp(-mut * path[j + 1]);
+            }
+            e_a = j == 0 ? e_first : e_emit * exp(-mut * path[j]);
+            total_lane += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
+            if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
+        }
+        if (!bins_any) continue;
+        events_total += deposit_round_pair(ev, lane, acc_lo, acc_hi, pre_lo, pre_hi, ev_key,
+                                           ev_val, head_at);
+    }
+    double total = half_sum(total_lane);
+    double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
+    double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
+    total += exact;
+    // the front ramp and flat tail after every panel, as one more round
+    // (lanes 0 and 1 of each half)
+    if (bins_any) {
+        PanelEvents ev;
+        ev.kind = 0;
+        if (bins) {
+            if (spread > 0.0) {
+                double t_1 = t_f - 0.5 * spread, t_2 = t_f + 0.5 * spread;
+                double e_1 = exp(kappa * t_1), e_2 = exp(kappa * t_2);
+                double ramp = quad_integral(e_1, 0.0, 1.0 / spread, 0.0, spread, kappa, e_2);
+                double flat = t_end > t_2
+                                  ? quad_integral(e_2, 1.0, 0.0, 0.0, t_end - t_2, kappa,
+                                                  e_last)
+                                  : 0.0;
+                double height = ramp + flat != 0.0 ? exact / (ramp + flat) : 0.0;
+                if (sub == 0)
+                    panel_events(B, t_1, t_2, e_1, e_2, 0.0, height / spread, 0.0, kappa, ev);
+                else if (sub == 1 && t_end > t_2)
+                    panel_events(B, t_2, t_end, e_2, e_last, height, 0.0, 0.0, kappa, ev);
+            } else if (sub == 0 && t_end > t_f) {
+                panel_events(B, t_f, t_end, e_first, e_last, h_front, 0.0, 0.0, kappa, ev);
+            }
+        }
+        events_total += deposit_round_pair(ev, lane, acc_lo, acc_hi, pre_lo, pre_hi, ev_key,
+                                           ev_val, head_at);
+    }
+    count_panels += (unsigned long long)(__shfl(panels, 0, 64) + __shfl(panels, 32, 64));
+    count_events += events_total;
+    return active ? total : 0.0;
+}
+
 // The batch kernels run in one of two forms. The wide form (when the caller's
 // scratch holds every batch's path nodes): o1_batch_heads (one wave per
 // batch: moment means and node count), a scan of the node counts into the
@@ -1752,35 +1978,42 @@ __global__ void __launch_bounds__(kWideThreads) o1_batch_deposit(Group G, int gr
     if (tid < 2) s_counts[tid] = 0;
     __syncthreads();
     int nl = G.opt.bands, nb = G.opt.bins, S = G.segments;
+    int pairs = (nl + 1) / 2;
     long long gw = (long long)blockIdx.x * kWideWaves + wv;
     long long slots = (long long)G.om.work_count * S;
     unsigned long long count_panels = 0, count_events = 0;
-    if (wide_form(G, groups) && gw < slots * nl) {
-        int lam = (int)(gw % nl);
-        long long slot = gw / nl;
+    if (wide_form(G, groups) && gw < slots * pairs) {
+        int pair = (int)(gw % pairs);
+        long long slot = gw / pairs;
         int w = (int)(slot / S), seg = (int)(slot % S);
-        double* acc = G.part_acc + ((size_t)slot * nl + lam) * nb * 4;
-        double* pre = G.part_pre + (size_t)slot * nl * 3 + 3 * lam;
+        int lam_lo = 2 * pair, lam_hi = 2 * pair + 1 < nl ? 2 * pair + 1 : -1;
+        double* acc_lo = G.part_acc + ((size_t)slot * nl + lam_lo) * nb * 4;
+        double* acc_hi = lam_hi >= 0 ? acc_lo + (size_t)nb * 4 : acc_lo;
+        double* pre_lo = G.part_pre + (size_t)slot * nl * 3 + 3 * lam_lo;
+        double* pre_hi = lam_hi >= 0 ? pre_lo + 3 : pre_lo;
+        const int span = lam_hi >= 0 ? 2 : 1;
         if (G.compute_bins)
-            for (int i = lane; i < nb * 4; i += 64) acc[i] = 0.0;
-        if (lane < 3) pre[lane] = 0.0;
+            for (int i = lane; i < span * nb * 4; i += 64) acc_lo[i] = 0.0;
+        if (lane < 3 * span) pre_lo[lane] = 0.0;
         // the zeros are stored before any lane of this wave adds to them
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         wave_sync();
-        double charge = 0.0;
-        if (G.opt.mus[lam] != 0.0) {
+        double charge = 0.0;   // of this lane's half's wavelength
+        bool any = G.opt.mus[lam_lo] != 0.0 || (lam_hi >= 0 && G.opt.mus[lam_hi] != 0.0);
+        if (any) {
             for (int b = G.om_batches[w] + seg; b < G.om_batches[w + 1]; b += S) {
                 BatchHead h = G.heads[b];
                 if (h.n == 0) continue;
                 size_t off = (size_t)G.node_off[b];
-                charge += deposit_lambda(G, lam, h, G.pool_path + off, G.pool_g0 + off,
-                                         G.pool_g2 + off, acc, pre, lane, ev_bin_all[wv],
-                                         ev_val_all[wv], head_at_all[wv], count_panels,
-                                         count_events);
+                charge += deposit_pair(G, lam_lo, lam_hi, h, G.pool_path + off, G.pool_g0 + off,
+                                       G.pool_g2 + off, acc_lo, acc_hi, pre_lo, pre_hi, lane,
+                                       ev_bin_all[wv], ev_val_all[wv], head_at_all[wv],
+                                       count_panels, count_events);
                 wave_sync();
             }
         }
-        if (lane == 0) G.part_charge[slot * nl + lam] = charge;
+        if (lane == 0) G.part_charge[slot * nl + lam_lo] = charge;
+        if (lane == 32 && lam_hi >= 0) G.part_charge[slot * nl + lam_hi] = charge;
     }
     if (G.work_counts != nullptr) {
         if (lane == 0 && (count_panels || count_events)) {
@@ -2603,7 +2836,7 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     }
     if ((status = lh::check_launch("o1_batch_profile")) != LH_OK) return status;
     {
-        long long deposit_waves = (long long)work * SL.segments * optics->bands;
+        long long deposit_waves = (long long)work * SL.segments * ((optics->bands + 1) / 2);
         const unsigned grid_deposit = (unsigned)((deposit_waves + kWideWaves - 1) / kWideWaves);
         lh::KernelMark mark("o1_batch_deposit", s);
         o1_batch_deposit<<<grid_deposit, kWideThreads, 0, s>>>(G, groups);
