@@ -39,6 +39,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+O1_MEMBER_BYTES = 22 * 8 + 4
 
 # FP64 operations per unit of the order-1 batch kernel (each add, multiply,
 # divide, square root or exponential counted once):
@@ -382,7 +383,7 @@ def roofline_of(name, kernel_ms, workload, response, route, counts=None, launche
                 "algorithmic": f"48 B/element x {n_el} + 36 B/OM x {oms} + 8 B x {max(1, bins)} "
                                f"outputs/OM"}
     if name in ("o1_batch_deposit", "o1_batch_profile") and counts is not None:
-        terms, panels, events = (c / launches for c in counts)
+        terms, panels, events = (c / launches for c in counts[:3])
         if name == "o1_batch_profile":
             flops = O1_PROFILE_TERM_FLOPS * terms
             algorithmic = (f"per launch {terms:.4g} profile terms x {O1_PROFILE_TERM_FLOPS} "
@@ -396,8 +397,17 @@ def roofline_of(name, kernel_ms, workload, response, route, counts=None, launche
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "algorithmic": algorithmic,
                 "note": "FP64 vector work (no matrix form); priced against the FP64 peak"}
+    if name == "o1_group_sums" and counts is not None and len(counts) > 3:
+        members = counts[3] / launches
+        achieved = members * O1_MEMBER_BYTES / (kernel_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "algorithmic": f"per launch {members:.4g} group members x {O1_MEMBER_BYTES} B "
+                               "(the member's pencil fields, 22 FP64 values, and its 4-byte "
+                               "index; counted in the kernel)",
+                "note": "gathers in key order; the FP64 geometry per member is not priced"}
     if name == "o1_batches" and counts is not None:
-        terms, panels, events = (c / launches for c in counts)
+        terms, panels, events = (c / launches for c in counts[:3])
         flops = (O1_PROFILE_TERM_FLOPS * terms + O1_PANEL_FLOPS * panels
                  + O1_EVENT_FLOPS * events)
         achieved = flops / (kernel_ms * 1e-3) / 1e12

@@ -297,9 +297,10 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
  * default size, 0 for none): when every batch's path fits it, the batches run
  * as wide kernels (one wave per batch, then one per (OM, segment, wavelength));
  * otherwise one workgroup per (OM, segment) does it all. Both give the same
- * sums in the same order. Its optional work_counts (3 device
+ * sums in the same order. Its optional work_counts (4 device
  * counters, added to) receive the number of profile terms (path nodes x
- * groups), time panels and bin events it evaluated. Each pair shares its
+ * groups), time panels, bin events and group members ((pencil, OM) entries
+ * in groups) it evaluated. Each pair shares its
  * workspace. lh_order1_set_tables uploads the 24-term Taylor table of
  * I_k(z) (3 x 24) and the 48-node inverse-angle rule (tangent, weight sec^2).
  */

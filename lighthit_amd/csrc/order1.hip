@@ -293,20 +293,20 @@ __global__ void __launch_bounds__(kThreads) o1_row_pieces(Compress k, int runs,
     }
 }
 
+// Partial rows of a run added in piece order, one thread per (run, moment).
 __global__ void __launch_bounds__(kThreads) o1_row_combine(Compress k, int runs,
                                                            const int* piece_start,
                                                            const int* big_start,
                                                            const double* piece_rows) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= runs) return;
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)runs * kNmom) return;
+    int r = (int)(e / kNmom), m = (int)(e - (long long)r * kNmom);
     int n = piece_start[r + 1] - piece_start[r];
     if (n < 2) return;
     const double* in = piece_rows + (size_t)big_start[r] * kNmom;
-    for (int m = 0; m < kNmom; ++m) {
-        double total = in[m];
-        for (int i = 1; i < n; ++i) total += in[(size_t)i * kNmom + m];
-        k.rows[(size_t)r * kNmom + m] = total;
-    }
+    double total = in[m];
+    for (int i = 1; i < n; ++i) total += in[(size_t)i * kNmom + m];
+    k.rows[(size_t)r * kNmom + m] = total;
 }
 
 // Pencil from a moment row (finish_pencils, _prompt_numba.py:2282-2353).
@@ -467,7 +467,8 @@ struct Group {
     double* pool_g0;          // (pool_nodes) profile sums
     double* pool_g2;
     long long pool_nodes;     // nodes the pool holds (0: the single-kernel form)
-    unsigned long long* work_counts;  // optional: profile terms, panels, bin events
+    unsigned long long* work_counts;  // optional: profile terms, panels, bin events,
+                                      // group members
     double* prepared;         // (n_pen, 7)
     double* lambda_bins;      // (work, bands, bins)
     int* k_range;             // (work, 4) k1 min, k1 max, k0 min, k0 max
@@ -1011,6 +1012,8 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
     constexpr int nv = 2 * n_mono + 6;
     long long seg = (long long)G.slots * G.n_pen;
     long long members = G.member_off[groups];
+    if (G.work_counts != nullptr && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(G.work_counts + 3, (unsigned long long)members);
     long long tiles = (members + kPieceEntries - 1) / kPieceEntries;
     {
         long long t = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -1099,17 +1102,17 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
     }
 }
 
+// Pieces of a group added in piece order, one thread per (group, value).
 __global__ void __launch_bounds__(kThreads) o1_group_combine(Group G, int groups) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= groups) return;
-    int nv = 2 * G.opt.n_mono + 6;
+    const int nv = 2 * G.opt.n_mono + 6;
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)groups * nv) return;
+    int g = (int)(e / nv), m = (int)(e - (long long)g * nv);
     int first = G.piece_start[g], last = G.piece_start[g + 1];
     if (last - first == 1) return;   // written by o1_group_sums
-    for (int m = 0; m < nv; ++m) {
-        double total = G.piece_values[(size_t)first * nv + m];
-        for (int q = first + 1; q < last; ++q) total += G.piece_values[(size_t)q * nv + m];
-        G.values[(size_t)g * nv + m] = total;
-    }
+    double total = G.piece_values[(size_t)first * nv + m];
+    for (int q = first + 1; q < last; ++q) total += G.piece_values[(size_t)q * nv + m];
+    G.values[(size_t)g * nv + m] = total;
 }
 
 // Batches: consecutive groups of one OM with one key / theta_bins.
@@ -2602,8 +2605,8 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
                                                                           big_start, piece_rows);
         }
         if ((status = lh::check_launch("o1_row_pieces")) != LH_OK) return status;
-        o1_row_combine<<<blocks(runs), kThreads, 0, s>>>(k, runs, piece_start, big_start,
-                                                         piece_rows);
+        o1_row_combine<<<(unsigned)(((long long)runs * kNmom + kThreads - 1) / kThreads),
+                         kThreads, 0, s>>>(k, runs, piece_start, big_start, piece_rows);
         if ((status = lh::check_launch("o1_row_combine")) != LH_OK) return status;
     }
     if (grid->parent_face == 0) {
@@ -2806,7 +2809,9 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
             }
         }
         if ((status = lh::check_launch("o1_group_sums")) != LH_OK) return status;
-        o1_group_combine<<<blocks(groups), kThreads, 0, s>>>(G, groups);
+        o1_group_combine<<<(unsigned)(((long long)groups * (2 * optics->n_mono + 6)
+                                       + kThreads - 1) / kThreads),
+                           kThreads, 0, s>>>(G, groups);
         if ((status = lh::check_launch("o1_group_combine")) != LH_OK) return status;
     }
     o1_batch_flags<<<blocks(groups + 1), kThreads, 0, s>>>(G, groups, work);

@@ -392,11 +392,12 @@ _work_counts = {}
 
 def count_work(device="cuda", on=True):
     """Start (or stop) counting the grouped passes' work on ``device``: profile
-    terms, time panels and bin events (for cost models; off by default)."""
+    terms, time panels, bin events and group members (for cost models; off by
+    default)."""
     import torch
     key = str(torch.device(device))
     if on:
-        _work_counts[key] = torch.zeros(3, dtype=torch.int64, device=device)
+        _work_counts[key] = torch.zeros(4, dtype=torch.int64, device=device)
     else:
         _work_counts.pop(key, None)
 
