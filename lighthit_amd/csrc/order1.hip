@@ -247,6 +247,15 @@ __device__ inline double wave_total(double v) {
     return v;
 }
 
+// Block b of the first `used` blocks (dispatched to XCD b % 8 in turn)
+// renumbered so that every XCD's blocks cover one contiguous range.
+__device__ inline long long xcd_block(long long b, long long used) {
+    constexpr int kXcds = 8;
+    const long long per = used / kXcds, rem = used % kXcds;
+    const long long xcd = b % kXcds, k = b / kXcds;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
 constexpr int kRowPiece = 64;   // ring nodes per wave of the moment rows
 
 // Pieces of every run: ceil(nodes / kRowPiece); runs of more than one piece
@@ -267,8 +276,12 @@ __global__ void __launch_bounds__(kThreads) o1_row_pieces(Compress k, int runs,
                                                           const int* piece_start,
                                                           const int* big_start,
                                                           double* piece_rows) {
-    int q = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
+    // each XCD takes a contiguous range of pieces (neighbouring cells share
+    // elements), as in o1_group_sums
+    const long long used = ((long long)piece_start[runs] + kThreads / 64 - 1) / (kThreads / 64);
+    if ((long long)blockIdx.x >= used) return;
+    int q = (int)(xcd_block(blockIdx.x, used) * (kThreads / 64)) + (threadIdx.x >> 6);
     if (q >= piece_start[runs]) return;
     int low = 0, high = runs;
     while (high - low > 1) {
@@ -1016,7 +1029,13 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
         atomicAdd(G.work_counts + 3, (unsigned long long)members);
     long long tiles = (members + kPieceEntries - 1) / kPieceEntries;
     {
-        long long t = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+        // workgroups go to the 8 XCDs in turn; each XCD takes a contiguous
+        // eighth of the tiles, so neighbouring OMs (which share pencils)
+        // gather through one L2
+        const long long used = (tiles + kThreads / 64 - 1) / (kThreads / 64);
+        if ((long long)blockIdx.x >= used) return;
+        const long long b = xcd_block(blockIdx.x, used);
+        long long t = b * (kThreads / 64) + (threadIdx.x >> 6);
         if (t >= tiles) return;
         long long c0 = t * kPieceEntries;
         int low = 0, high = groups;     // member_off[low] <= c0 < member_off[high]
