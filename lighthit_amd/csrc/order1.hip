@@ -117,13 +117,19 @@ __global__ void __launch_bounds__(kThreads) o1_subsegments(Compress k) {
 }
 
 // Ring nodes of one sub-segment (compress_subsegments, _prompt_numba.py:1122-1190).
+// Ring nodes, one thread per node (subsegment s, ring index q): the ring
+// direction at angle offset + 2 pi q / R (evaluated directly rather than by
+// the reference's rotation recurrence: the same angles to the last bits), its
+// cell and pixel key. Consecutive threads write consecutive nodes.
 __global__ void __launch_bounds__(kThreads) o1_nodes(Compress k) {
-    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= k.grid.subsegments) return;
+    const int R = k.grid.ring_nodes, face = k.grid.pixel_face;
+    long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= k.grid.subsegments * (long long)R) return;
+    long long s = node / R;
+    int q = (int)(node - s * R);
     int i = k.sub_element[s], ia = k.sub_index[s];
     int e = k.ch.index ? k.ch.index[i] : i;
     const lh_order1_chords& c = k.ch;
-    int R = k.grid.ring_nodes, face = k.grid.pixel_face;
     double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
     double e1[3], e2[3];
     ring_basis(u, e1, e2);
@@ -142,24 +148,15 @@ __global__ void __launch_bounds__(kThreads) o1_nodes(Compress k) {
     }
     long long ck = (cell[0] * k.grid.dims[1] + cell[1]) * k.grid.dims[2] + cell[2];
     long long n_pix = 6LL * face * face;
-    double step_c = cos(2.0 * M_PI / R), step_s = sin(2.0 * M_PI / R);
-    double cp = cos(offset), sp = sin(offset);
-    long long base = s * R;
-    for (int q = 0; q < R; ++q) {
-        if (q > 0) {
-            double c2 = cp * step_c - sp * step_s;
-            double s2 = sp * step_c + cp * step_s;
-            cp = c2;
-            sp = s2;
-        }
-        double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
-        double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
-        double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
-        k.keys[base + q] = ck * n_pix + pixel_of(sx, sy, sz, face);
-        k.ids[base + q] = (int)(base + q);
-        k.cp[base + q] = cp;
-        k.sp[base + q] = sp;
-    }
+    double sp, cp;
+    sincos(offset + q * (2.0 * M_PI / R), &sp, &cp);
+    double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
+    double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
+    double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
+    k.keys[node] = ck * n_pix + pixel_of(sx, sy, sz, face);
+    k.ids[node] = (int)node;
+    k.cp[node] = cp;
+    k.sp[node] = sp;
 }
 
 __global__ void __launch_bounds__(kThreads) o1_run_flags(Compress k) {
@@ -2569,7 +2566,8 @@ int lh_order1_compress_keys(const lh_order1_chords* chords, const lh_order1_grid
     Compress k = make_compress(chords, grid, base, L);
     o1_subsegments<<<blocks(chords->count), kThreads, 0, s>>>(k);
     if ((status = lh::check_launch("o1_subsegments")) != LH_OK) return status;
-    o1_nodes<<<blocks(grid->subsegments), kThreads, 0, s>>>(k);
+    o1_nodes<<<(unsigned)((grid->subsegments * (long long)grid->ring_nodes + kThreads - 1)
+                          / kThreads), kThreads, 0, s>>>(k);
     if ((status = lh::check_launch("o1_nodes")) != LH_OK) return status;
     unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
                                  * 6ULL * grid->pixel_face * grid->pixel_face;
