@@ -266,12 +266,21 @@ __global__ void __launch_bounds__(kThreads) o1_row_counts(Compress k, int runs, 
     big[r] = c > 1 ? c : 0;
 }
 
+// The run of every piece, one thread per run writing its pieces.
+__global__ void __launch_bounds__(kThreads) o1_piece_runs(int runs, const int* piece_start,
+                                                          int* piece_run) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    for (int q = piece_start[r]; q < piece_start[r + 1]; ++q) piece_run[q] = r;
+}
+
 // Moment rows: one wave per piece of at most kRowPiece nodes of one run, a
 // fixed butterfly sum; a run of one piece writes its row, longer runs write
 // partial rows that o1_row_combine adds in piece order (deterministic).
 __global__ void __launch_bounds__(kThreads) o1_row_pieces(Compress k, int runs,
                                                           const int* piece_start,
                                                           const int* big_start,
+                                                          const int* piece_run,
                                                           double* piece_rows) {
     int lane = threadIdx.x & 63;
     // each XCD takes a contiguous range of pieces (neighbouring cells share
@@ -280,12 +289,7 @@ __global__ void __launch_bounds__(kThreads) o1_row_pieces(Compress k, int runs,
     if ((long long)blockIdx.x >= used) return;
     int q = (int)(xcd_block(blockIdx.x, used) * (kThreads / 64)) + (threadIdx.x >> 6);
     if (q >= piece_start[runs]) return;
-    int low = 0, high = runs;
-    while (high - low > 1) {
-        int middle = (low + high) >> 1;
-        if (piece_start[middle] <= q) low = middle; else high = middle;
-    }
-    int r = low, piece = q - piece_start[r];
+    int r = piece_run[q], piece = q - piece_start[r];
     int j0 = k.run_start[r] + piece * kRowPiece;
     int j1 = min(k.run_start[r + 1], j0 + kRowPiece);
     double v[kNmom];
@@ -459,6 +463,7 @@ struct Group {
     int* member_off;          // (groups + 1) first member of every group, numbered
                               // consecutively over the groups           caller scratch
     int* piece_start;         // (groups + 1) first piece of every group   caller scratch
+    int* tile_group;          // (tiles) group of every tile's first member  caller scratch
     double* piece_values;     // (pieces, nv) partial group sums          caller scratch
     int* om_groups;           // (work + 1) first group of every OM
     double* values;           // (groups, nv)            caller scratch
@@ -1007,6 +1012,17 @@ __global__ void __launch_bounds__(kThreads) o1_piece_counts(Group G, int groups,
     counts[g] = last / kPieceEntries - first / kPieceEntries + 1;
 }
 
+// The group of every tile's first member: tile t (members 64t ..) starts in
+// group g when member_off[g] <= 64t < member_off[g + 1]. One thread per group
+// writes its tiles, so the group sums need no search.
+__global__ void __launch_bounds__(kThreads) o1_tile_groups(Group G, int groups) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= groups) return;
+    long long t0 = ((long long)G.member_off[g] + kPieceEntries - 1) / kPieceEntries;
+    long long t1 = ((long long)G.member_off[g + 1] + kPieceEntries - 1) / kPieceEntries;
+    for (long long t = t0; t < t1; ++t) G.tile_group[t] = g;
+}
+
 // Group sums: one wave per tile of kPieceEntries consecutive members (of any
 // number of groups), a lane per member; a fixed-order segmented scan across
 // the lanes sums each group's members in the tile, and the group's last lane
@@ -1035,11 +1051,7 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
         long long t = b * (kThreads / 64) + (threadIdx.x >> 6);
         if (t >= tiles) return;
         long long c0 = t * kPieceEntries;
-        int low = 0, high = groups;     // member_off[low] <= c0 < member_off[high]
-        while (high - low > 1) {
-            int middle = (low + high) >> 1;
-            if (G.member_off[middle] <= c0) low = middle; else high = middle;
-        }
+        const int low = G.tile_group[t];   // member_off[low] <= c0 < member_off[low + 1]
         // the lane where each group of the tile starts holds the group's index;
         // a running maximum carries it to the group's other lanes
         wave_sync();
@@ -2245,7 +2257,7 @@ struct CompressLayout {
     size_t keys, ids, sorted_keys, sorted_ids, cp, sp, sub_element, sub_index, run_flag, run_scan;
     size_t run_start, run_key, rows, info, sort_temp, scan_temp, pkeys, pids, spkeys, spids,
         pflag, pscan, psort_temp, piece_count, piece_start, big_count, big_start, piece_rows,
-        total;
+        piece_run, total;
 };
 
 int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
@@ -2301,6 +2313,8 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
     L.big_start = take((runs_cap + 1) * sizeof(int));
     // pieces of runs longer than one piece: at most 2 nodes / kRowPiece
     L.piece_rows = take((size_t)(2 * nodes / kRowPiece + 1) * kNmom * sizeof(double));
+    // pieces <= runs + nodes / kRowPiece
+    L.piece_run = take((size_t)(runs_cap + nodes / kRowPiece + 1) * sizeof(int));
     L.sort_temp = take(sort_temp);
     L.scan_temp = take(scan_temp);
     L.psort_temp = take(psort_temp);
@@ -2417,8 +2431,8 @@ int segments_for(int work, int bands, int bins) {
 
 struct ScratchLayout {
     size_t values, batch_flag, batch_scan, batch_start, om_batches, part_acc, part_pre,
-        part_charge, piece_count, member_off, piece_start, piece_values, heads, node_n, node_off,
-        scan_temp, total;   // total: the fixed part; the node pool follows it
+        part_charge, piece_count, member_off, piece_start, tile_group, piece_values, heads, node_n,
+        node_off, scan_temp, total;   // total: the fixed part; the node pool follows it
     int segments;
     long long pieces;         // bound on the group-sum pieces
 };
@@ -2452,6 +2466,7 @@ int scratch_layout(int work, int n_pen, int slots, int bands, int bins, int n_mo
     L.piece_count = take((groups + 1) * sizeof(int));
     L.member_off = take((groups + 1) * sizeof(int));
     L.piece_start = take((groups + 1) * sizeof(int));
+    L.tile_group = take((size_t)((entries + kPieceEntries - 1) / kPieceEntries + 1) * sizeof(int));
     L.piece_values = take((size_t)std::max(L.pieces, 1LL) * (2 * n_mono + 6) * sizeof(double));
     L.heads = take((size_t)(groups + 1) * sizeof(BatchHead));
     L.node_n = take((size_t)(groups + 1) * sizeof(long long));
@@ -2607,6 +2622,7 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
         int* big = reinterpret_cast<int*>(base + L.big_count);
         int* big_start = reinterpret_cast<int*>(base + L.big_start);
         double* piece_rows = reinterpret_cast<double*>(base + L.piece_rows);
+        int* piece_run = reinterpret_cast<int*>(base + L.piece_run);
         o1_row_counts<<<blocks(runs + 1), kThreads, 0, s>>>(k, runs, counts, big);
         if ((status = lh::check_launch("o1_row_counts")) != LH_OK) return status;
         size_t scan_bytes = L.psort_temp - L.scan_temp;
@@ -2616,10 +2632,13 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
                                                      big_start, runs + 1, s));
         // pieces <= runs + nodes / kRowPiece
         long long bound = runs + nodes / kRowPiece + 1;
+        o1_piece_runs<<<blocks(runs), kThreads, 0, s>>>(runs, piece_start, piece_run);
+        if ((status = lh::check_launch("o1_piece_runs")) != LH_OK) return status;
         {
             lh::KernelMark mark("o1_row_pieces", s);
             o1_row_pieces<<<(unsigned)((bound + 3) / 4), kThreads, 0, s>>>(k, runs, piece_start,
-                                                                          big_start, piece_rows);
+                                                                          big_start, piece_run,
+                                                                          piece_rows);
         }
         if ((status = lh::check_launch("o1_row_pieces")) != LH_OK) return status;
         o1_row_combine<<<(unsigned)(((long long)runs * kNmom + kThreads - 1) / kThreads),
@@ -2789,6 +2808,7 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     G.segments = SL.segments;
     G.piece_start = reinterpret_cast<int*>(sc + SL.piece_start);
     G.member_off = reinterpret_cast<int*>(sc + SL.member_off);
+    G.tile_group = reinterpret_cast<int*>(sc + SL.tile_group);
     G.piece_values = reinterpret_cast<double*>(sc + SL.piece_values);
     G.heads = reinterpret_cast<BatchHead*>(sc + SL.heads);
     G.node_n = reinterpret_cast<long long*>(sc + SL.node_n);
@@ -2811,6 +2831,8 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
         if ((status = lh::check_launch("o1_piece_counts")) != LH_OK) return status;
         LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(sc + SL.scan_temp, scan_bytes, counts,
                                                      G.piece_start, groups + 1, s));
+        o1_tile_groups<<<blocks(groups), kThreads, 0, s>>>(G, groups);
+        if ((status = lh::check_launch("o1_tile_groups")) != LH_OK) return status;
         {
             lh::KernelMark mark("o1_group_sums", s);
             // one wave per tile of members; their count is on the device and
