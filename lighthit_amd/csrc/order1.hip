@@ -239,9 +239,30 @@ __device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
     v[22] = w0 * (py * pz + sub2 * u[1] * u[2]);
 }
 
-__device__ inline double wave_total(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// Wave sums of V <= 32 values per lane, reduce-scattered: every step pairs
+// lanes `o` apart, each keeps one half of its values and adds the partner's
+// copy of that half (32 values padded, offsets 32 .. 2: 16 + 8 + 4 + 2 + 1
+// shuffles, then one across lanes 1 apart). Lane l ends with the wave total of
+// value l >> 1 (both lanes of a pair hold it). A fixed order, as a butterfly
+// per value would be, with a fifth of its shuffles.
+template <int V>
+__device__ inline double wave_totals(const double (&v)[V], int lane) {
+    static_assert(V >= 1 && V <= 32, "at most 32 values");
+    double a[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) a[i] = i < V ? v[i] : 0.0;
+#pragma unroll
+    for (int step = 0; step < 5; ++step) {
+        const int o = 32 >> step, h = 16 >> step;
+        const bool upper = (lane & o) != 0;
+#pragma unroll
+        for (int i = 0; i < h; ++i) {
+            double send = upper ? a[i] : a[h + i];
+            double keep = upper ? a[h + i] : a[i];
+            a[i] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    return a[0] + __shfl_xor(a[0], 1, 64);
 }
 
 // Block b of the first `used` blocks (dispatched to XCD b % 8 in turn)
@@ -301,26 +322,30 @@ __global__ void __launch_bounds__(kThreads) o1_row_pieces(Compress k, int runs,
     bool single = piece_start[r + 1] - piece_start[r] == 1;
     double* out = single ? k.rows + (size_t)r * kNmom
                          : piece_rows + ((size_t)big_start[r] + piece) * kNmom;
-    for (int m = 0; m < kNmom; ++m) {
-        double total = wave_total(v[m]);
-        if (lane == 0) out[m] = total;
-    }
+    double total = wave_totals<kNmom>(v, lane);
+    if ((lane & 1) == 0 && (lane >> 1) < kNmom) out[lane >> 1] = total;
 }
 
-// Partial rows of a run added in piece order, one thread per (run, moment).
+// Partial rows of a run added in piece order, one thread per run (most runs
+// are one piece, written by o1_row_pieces, and return at once).
 __global__ void __launch_bounds__(kThreads) o1_row_combine(Compress k, int runs,
                                                            const int* piece_start,
                                                            const int* big_start,
                                                            const double* piece_rows) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)runs * kNmom) return;
-    int r = (int)(e / kNmom), m = (int)(e - (long long)r * kNmom);
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
     int n = piece_start[r + 1] - piece_start[r];
     if (n < 2) return;
     const double* in = piece_rows + (size_t)big_start[r] * kNmom;
-    double total = in[m];
-    for (int i = 1; i < n; ++i) total += in[(size_t)i * kNmom + m];
-    k.rows[(size_t)r * kNmom + m] = total;
+    double total[kNmom];
+#pragma unroll
+    for (int m = 0; m < kNmom; ++m) total[m] = in[m];
+    for (int i = 1; i < n; ++i) {
+#pragma unroll
+        for (int m = 0; m < kNmom; ++m) total[m] += in[(size_t)i * kNmom + m];
+    }
+#pragma unroll
+    for (int m = 0; m < kNmom; ++m) k.rows[(size_t)r * kNmom + m] = total[m];
 }
 
 // Pencil from a moment row (finish_pencils, _prompt_numba.py:2282-2353).
@@ -1130,14 +1155,22 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
     }
 }
 
-// Pieces of a group added in piece order, one thread per (group, value).
+// Pieces of a group added in piece order, one thread per (tile, value): a
+// group of more than one piece is combined by the tile after its first one
+// (the tile whose first member lies in the group one tile past the group's
+// first member's tile), so the launch covers tiles, not groups.
 __global__ void __launch_bounds__(kThreads) o1_group_combine(Group G, int groups) {
     const int nv = 2 * G.opt.n_mono + 6;
     long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)groups * nv) return;
-    int g = (int)(e / nv), m = (int)(e - (long long)g * nv);
+    const long long members = G.member_off[groups];
+    const long long tiles = (members + kPieceEntries - 1) / kPieceEntries;
+    if (e >= tiles * nv) return;
+    long long t = e / nv;
+    int m = (int)(e - t * nv);
+    if (t == 0) return;
+    int g = G.tile_group[t];
+    if (G.member_off[g] / kPieceEntries + 1 != t) return;
     int first = G.piece_start[g], last = G.piece_start[g + 1];
-    if (last - first == 1) return;   // written by o1_group_sums
     double total = G.piece_values[(size_t)first * nv + m];
     for (int q = first + 1; q < last; ++q) total += G.piece_values[(size_t)q * nv + m];
     G.values[(size_t)g * nv + m] = total;
@@ -2641,8 +2674,8 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
                                                                           piece_rows);
         }
         if ((status = lh::check_launch("o1_row_pieces")) != LH_OK) return status;
-        o1_row_combine<<<(unsigned)(((long long)runs * kNmom + kThreads - 1) / kThreads),
-                         kThreads, 0, s>>>(k, runs, piece_start, big_start, piece_rows);
+        o1_row_combine<<<blocks(runs), kThreads, 0, s>>>(k, runs, piece_start, big_start,
+                                                         piece_rows);
         if ((status = lh::check_launch("o1_row_combine")) != LH_OK) return status;
     }
     if (grid->parent_face == 0) {
@@ -2848,7 +2881,9 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
             }
         }
         if ((status = lh::check_launch("o1_group_sums")) != LH_OK) return status;
-        o1_group_combine<<<(unsigned)(((long long)groups * (2 * optics->n_mono + 6)
+        // one thread per (tile, value); the tiles are at most the entries / 64 + 1
+        long long entries = (long long)slots_of(pencils) * work * pencils->count;
+        o1_group_combine<<<(unsigned)(((entries / kPieceEntries + 1) * (2 * optics->n_mono + 6)
                                        + kThreads - 1) / kThreads),
                            kThreads, 0, s>>>(G, groups);
         if ((status = lh::check_launch("o1_group_combine")) != LH_OK) return status;
