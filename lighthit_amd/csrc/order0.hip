@@ -165,13 +165,20 @@ __device__ int candidate(const O0Params& P, int om, int i, int* glass) {
 
 // ------------------------------------------------------ node arithmetic
 
+// Bonnet's recurrence l P_l = (2l - 1) x P_{l-1} - (l - 1) P_{l-2} with its
+// two ratios tabulated (no division in the node loop).
+__constant__ float kBonnetA[kMaxAlpha] = {0.f, 0.f, 3.f / 2.f, 5.f / 3.f, 7.f / 4.f,
+                                          9.f / 5.f, 11.f / 6.f, 13.f / 7.f};
+__constant__ float kBonnetB[kMaxAlpha] = {0.f, 0.f, 1.f / 2.f, 2.f / 3.f, 3.f / 4.f,
+                                          4.f / 5.f, 5.f / 6.f, 6.f / 7.f};
+
 __device__ __forceinline__ float acceptance(const float* a, int n, float x) {
     // sum (2l+1) alpha_l P_l(x) / (4 pi), clipped at zero (_ballistic_interval_numba.py:21-37)
     float result = a[0];
     if (n > 1) result += 3.f * a[1] * x;
     float previous = 1.f, current = x;
     for (int ell = 2; ell < n; ++ell) {
-        const float value = ((2 * ell - 1) * x * current - (ell - 1) * previous) / ell;
+        const float value = kBonnetA[ell] * x * current - kBonnetB[ell] * previous;
         result += (2 * ell + 1) * a[ell] * value;
         previous = current;
         current = value;
@@ -346,13 +353,14 @@ __device__ float theta_node_value(const PanelTables& T, const Slot& S, int npane
         // hav(eps) - hav(gamma) = sin(theta) sin(psi) sin((half - phi)/2) sin((half + phi)/2)
         const float gap = sts * sin_small(0.5f * half * em) * sin_small(0.5f * half * ep);
         const float inside = fmaxf(gap * (rest - hav_g), 0.f);
-        const float d = S.D2R2 / (S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
+        const float d = __fdividef(S.D2R2,
+                                   S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
         // cos(phi) = 1 - 2 hav(phi), sin(phi) = 2 sin(phi/2) cos(phi/2)
         const float cp = 1.f - 2.f * hphi * hphi;
         const float sp = 2.f * hphi * sqrtf(fmaxf(0.f, 1.f - hphi * hphi));
         const float acc = acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across + sp * side))
                         + acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across - sp * side));
-        sum += half * ce * kEtaWeight[ie] * acc * expf(-mu * d);
+        sum += half * ce * kEtaWeight[ie] * acc * __expf(-mu * d);
     }
     return hw * wt * S.eps * ct * spectral * jac * sum;
 }
@@ -453,12 +461,13 @@ __device__ void node_bins(const PanelTables& T, const Slot& S, int npanel, int n
                 const float hav_g = p.hav_d + p.sts * hphi * hphi;
                 const float gap = p.sts * sin_small(0.5f * p.half * em) * sin_small(0.5f * p.half * ep);
                 const float inside = fmaxf(gap * (rest - hav_g), 0.f);
-                const float d = S.D2R2 / (S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
+                const float d = __fdividef(S.D2R2,
+                                           S.D * (1.f - 2.f * hav_g) + 2.f * S.D * sqrtf(inside));
                 const float cp = 1.f - 2.f * hphi * hphi;
                 const float sp = 2.f * hphi * sqrtf(fmaxf(0.f, 1.f - hphi * hphi));
                 const float acc = acceptance(T.alpha, nalpha, p.cth * axial + p.sth * (cp * across + sp * side))
                                 + acceptance(T.alpha, nalpha, p.cth * axial + p.sth * (cp * across - sp * side));
-                sum += p.half * ce * hwe * kGauss8w[ie] * acc * expf(-p.mu * d);
+                sum += p.half * ce * hwe * kGauss8w[ie] * acc * __expf(-p.mu * d);
             }
             atomicAdd(&wave_bins[bin], (double)(prefactor * sum));
         }
