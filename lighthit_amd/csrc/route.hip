@@ -17,62 +17,79 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__global__ void __launch_bounds__(kThreads) segment_bounds(const double* receivers,
-                                                           const double* starts,
-                                                           const double* directions,
-                                                           const double* lengths,
-                                                           const double* start_ns, int n_el,
-                                                           double flight, double* distance,
-                                                           double* earliest) {
-    __shared__ double best_d[kThreads];
-    __shared__ double best_t[kThreads];
-    int d = blockIdx.x;
-    double rx = receivers[3 * d], ry = receivers[3 * d + 1], rz = receivers[3 * d + 2];
-    double bd = DBL_MAX, bt = DBL_MAX;
-    // four elements per thread at a time: every load is issued before the
-    // arithmetic (minima do not depend on the order)
-    constexpr int kBatch = 4;
-    for (int i0 = threadIdx.x; i0 < n_el; i0 += kBatch * kThreads) {
-        double sx[kBatch], sy[kBatch], sz[kBatch], ux[kBatch], uy[kBatch], uz[kBatch];
-        double len[kBatch], t0[kBatch];
+constexpr int kBoundOms = 8;          // OMs per workgroup of segment_bounds
+constexpr int kBoundThreads = 512;
+
+// Nearest approach and earliest arrival of every OM over all elements: a
+// workgroup takes kBoundOms OMs, so each element it loads serves all of them
+// (the element list is read once per 8 OMs, not once per OM). Minima do not
+// depend on the order. An element whose start time is not below the running
+// earliest arrival cannot lower it (the flight term is >= 0), so its square
+// root is skipped.
+__global__ void __launch_bounds__(kBoundThreads) segment_bounds(const double* receivers,
+                                                                int n_receivers,
+                                                                const double* starts,
+                                                                const double* directions,
+                                                                const double* lengths,
+                                                                const double* start_ns, int n_el,
+                                                                double flight, double* distance,
+                                                                double* earliest) {
+    __shared__ double part[kBoundThreads / 64][2 * kBoundOms];
+    const int first = blockIdx.x * kBoundOms;
+    double rx[kBoundOms], ry[kBoundOms], rz[kBoundOms], bd[kBoundOms], bt[kBoundOms];
 #pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
-            const int i = i0 + b * kThreads;
-            const bool in = i < n_el;
-            sx[b] = in ? starts[3 * i] : 0.0;
-            sy[b] = in ? starts[3 * i + 1] : 0.0;
-            sz[b] = in ? starts[3 * i + 2] : 0.0;
-            ux[b] = in ? directions[3 * i] : 0.0;
-            uy[b] = in ? directions[3 * i + 1] : 0.0;
-            uz[b] = in ? directions[3 * i + 2] : 0.0;
-            len[b] = in ? lengths[i] : 0.0;
-            t0[b] = in && start_ns ? start_ns[i] : 0.0;
-        }
+    for (int k = 0; k < kBoundOms; ++k) {
+        const int d = min(first + k, n_receivers - 1);
+        rx[k] = receivers[3 * d];
+        ry[k] = receivers[3 * d + 1];
+        rz[k] = receivers[3 * d + 2];
+        bd[k] = DBL_MAX;
+        bt[k] = DBL_MAX;
+    }
+    for (int i = threadIdx.x; i < n_el; i += kBoundThreads) {
+        const double sx = starts[3 * i], sy = starts[3 * i + 1], sz = starts[3 * i + 2];
+        const double ux = directions[3 * i], uy = directions[3 * i + 1], uz = directions[3 * i + 2];
+        const double len = lengths[i];
+        const double t0 = start_ns ? start_ns[i] : 0.0;
 #pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
-            if (i0 + b * kThreads >= n_el) break;
-            double ox = rx - sx[b], oy = ry - sy[b], oz = rz - sz[b];
-            double along = ox * ux[b] + oy * uy[b] + oz * uz[b];
-            along = fmin(fmax(along, 0.0), len[b]);
-            double dx = ox - along * ux[b], dy = oy - along * uy[b], dz = oz - along * uz[b];
-            double value = dx * dx + dy * dy + dz * dz;
-            bd = fmin(bd, value);
-            if (start_ns) bt = fmin(bt, t0[b] + flight * sqrt(value));
+        for (int k = 0; k < kBoundOms; ++k) {
+            const double ox = rx[k] - sx, oy = ry[k] - sy, oz = rz[k] - sz;
+            double along = ox * ux + oy * uy + oz * uz;
+            along = fmin(fmax(along, 0.0), len);
+            const double dx = ox - along * ux, dy = oy - along * uy, dz = oz - along * uz;
+            const double value = dx * dx + dy * dy + dz * dz;
+            bd[k] = fmin(bd[k], value);
+            if (start_ns && t0 < bt[k]) bt[k] = fmin(bt[k], t0 + flight * sqrt(value));
         }
     }
-    best_d[threadIdx.x] = bd;
-    best_t[threadIdx.x] = bt;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kBoundOms; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            bd[k] = fmin(bd[k], __shfl_xor(bd[k], off, 64));
+            bt[k] = fmin(bt[k], __shfl_xor(bt[k], off, 64));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kBoundOms; ++k) {
+            part[wave][k] = bd[k];
+            part[wave][kBoundOms + k] = bt[k];
+        }
+    }
     __syncthreads();
-    for (int half = kThreads / 2; half > 0; half >>= 1) {
-        if (threadIdx.x < half) {
-            best_d[threadIdx.x] = fmin(best_d[threadIdx.x], best_d[threadIdx.x + half]);
-            best_t[threadIdx.x] = fmin(best_t[threadIdx.x], best_t[threadIdx.x + half]);
+    if (threadIdx.x < 2 * kBoundOms) {
+        double best = part[0][threadIdx.x];
+        for (int w = 1; w < kBoundThreads / 64; ++w) best = fmin(best, part[w][threadIdx.x]);
+        const int k = threadIdx.x % kBoundOms;
+        const int d = first + k;
+        if (d < n_receivers) {
+            if (threadIdx.x < kBoundOms) {
+                if (distance) distance[d] = sqrt(best);
+            } else if (earliest) {
+                earliest[d] = best;
+            }
         }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        if (distance) distance[d] = sqrt(best_d[0]);
-        if (earliest) earliest[d] = best_t[0];
     }
 }
 
@@ -166,8 +183,8 @@ extern "C" int lh_segment_bounds(const double* receivers, int n_receivers, const
     LH_REQUIRE(n_receivers >= 1 && n_segments >= 1, "need receivers and segments");
     LH_REQUIRE(earliest == nullptr || start_ns != nullptr, "earliest needs start_ns");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    segment_bounds<<<n_receivers, kThreads, 0, s>>>(receivers, starts, directions, lengths,
-                                                    start_ns, n_segments, flight, distance,
-                                                    earliest);
+    segment_bounds<<<(n_receivers + kBoundOms - 1) / kBoundOms, kBoundThreads, 0, s>>>(
+        receivers, n_receivers, starts, directions, lengths, start_ns, n_segments, flight,
+        distance, earliest);
     return lh::check_launch("segment_bounds");
 }
