@@ -90,6 +90,27 @@ def stream_handle(stream=None):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+def upload(arrays, device, dtype=None):
+    """Device copies of several host arrays of one dtype (default float64) in
+    one host-to-device copy: views into one buffer, each piece starting on a
+    256-byte boundary. Small per-event uploads then cost one copy, not one each."""
+    import numpy as np
+    import torch
+    dtype = torch.float64 if dtype is None else dtype
+    np_dtype = torch.empty((), dtype=dtype).numpy().dtype
+    arrays = [np.ascontiguousarray(a, dtype=np_dtype) for a in arrays]
+    align = max(1, 256 // np_dtype.itemsize)
+    offsets, at = [], 0
+    for a in arrays:
+        offsets.append(at)
+        at += -(-a.size // align) * align
+    host = np.zeros(max(at, 1), dtype=np_dtype)
+    for a, off in zip(arrays, offsets):
+        host[off:off + a.size] = a.ravel()
+    buf = torch.from_numpy(host).to(device)
+    return [buf.narrow(0, off, a.size).view(a.shape) for a, off in zip(arrays, offsets)]
+
+
 _workspaces = {}
 _workspaces_lock = threading.Lock()
 

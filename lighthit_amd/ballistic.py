@@ -107,10 +107,9 @@ class DeviceOMs:
         if detector.receiver_surface is None:
             raise ValueError("the first-entry GPU path needs detector.receiver_surface geometry")
         self.count = len(detector)
-        self._keep = [_dev(detector.positions_m, torch.float64, device),
-                      _dev(detector.looks, torch.float64, device),
-                      _dev(detector.effective_area_m2, torch.float64, device),
-                      _dev(detector.receiver_surface.radii(self.count), torch.float64, device)]
+        self._keep = _lib.upload([detector.positions_m, detector.looks,
+                                  detector.effective_area_m2,
+                                  detector.receiver_surface.radii(self.count)], device)
         self.struct = _Oms(*[t.data_ptr() for t in self._keep], self.count)
 
 
@@ -144,8 +143,8 @@ class DeviceElements:
         self.count = len(elements)
         names = ("start_m", "direction", "length_m", "coefficient0", "coefficient2", "beta",
                  "start_ns", "end_ns")
-        self._keep = [_dev(np.asarray(getattr(elements, n), float), torch.float64, device)
-                      for n in names]
+        # one host-to-device copy for every array
+        self._keep = _lib.upload([np.asarray(getattr(elements, n), float) for n in names], device)
         self.struct = _Elements(*[t.data_ptr() for t in self._keep], self.count)
 
 
@@ -206,8 +205,7 @@ class Order0:
             elements, self.device)
         oms = DeviceOMs(self.detector.subset(ids), self.device)
         out, ws = self.bins_device(device_elements, oms,
-                                   _dev(origins, torch.float64, self.device),
-                                   _dev(edges, torch.float64, self.device))
+                                   *_lib.upload([origins, edges], self.device))
         torch.cuda.synchronize()
         check_flags(ws)
         return out.cpu().numpy()

@@ -118,19 +118,20 @@ class DeviceChords:
         self.device = device
         self.host = elements
 
-        def up(values, dtype=torch.float64):
-            return torch.as_tensor(np.ascontiguousarray(values), dtype=dtype, device=device)
-        self.start_m = up(np.asarray(elements.start_m, float))
-        self.direction = up(np.asarray(elements.direction, float))
-        self.length_m = up(np.asarray(elements.length_m, float))
-        self.start_ns = up(np.asarray(elements.start_ns, float))
-        self.end_ns = up(np.asarray(elements.end_ns, float))
         spectral = hasattr(elements, "coefficient0")
-        self.coefficient0 = up(np.asarray(elements.coefficient0, float)) if spectral else None
-        self.coefficient2 = up(np.asarray(elements.coefficient2, float)) if spectral else None
-        self.beta = up(np.asarray(elements.beta, float)) if spectral else None
-        self.photons = None if spectral else up(np.asarray(elements.photons, float))
-        self.cone = None if spectral else up(np.asarray(elements.cone_cosine, float))
+        names = ["start_m", "direction", "length_m", "start_ns", "end_ns"]
+        names += (["coefficient0", "coefficient2", "beta"] if spectral
+                  else ["photons", "cone_cosine"])
+        # one host-to-device copy for every array
+        arrays = _lib.upload([np.asarray(getattr(elements, n), float) for n in names], device)
+        got = dict(zip(names, arrays))
+        self.start_m, self.direction = got["start_m"], got["direction"]
+        self.length_m, self.start_ns, self.end_ns = got["length_m"], got["start_ns"], got["end_ns"]
+        self.coefficient0 = got.get("coefficient0")
+        self.coefficient2 = got.get("coefficient2")
+        self.beta = got.get("beta")
+        self.photons = got.get("photons")
+        self.cone = got.get("cone_cosine")
 
 
 def _address(tensor):

@@ -223,9 +223,8 @@ class DeviceReceivers:
         self.local_looks = frame.rotate(np.atleast_2d(np.asarray(looks, float)))
         scale = np.ascontiguousarray(out_scale, float)
         self.count = len(positions)
-        self.d_positions = torch.as_tensor(np.ascontiguousarray(self.local_positions), device=device)
-        self.d_looks = torch.as_tensor(np.ascontiguousarray(self.local_looks), device=device)
-        self.d_scale = torch.as_tensor(scale, device=device)
+        self.d_positions, self.d_looks, self.d_scale = _lib.upload(
+            [self.local_positions, self.local_looks, scale], device)
         self.struct = _Receivers(self.d_positions.data_ptr(), self.d_looks.data_ptr(),
                                  self.d_scale.data_ptr(), self.count)
 

@@ -272,11 +272,9 @@ class DeviceOMs:
     def __init__(self, positions, looks, areas, origins, device="cuda"):
         import torch
 
-        def up(a):
-            return torch.as_tensor(np.ascontiguousarray(a, float), device=device)
         self.count = len(positions)
         self.device = device
-        self._keep = [up(positions), up(looks), up(areas), up(origins)]
+        self._keep = _lib.upload([positions, looks, areas, origins], device)
 
     def struct(self, work):
         import torch
