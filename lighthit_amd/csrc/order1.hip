@@ -474,6 +474,7 @@ struct Group {
     int slots;                // entries per (pencil, OM): 2 on the screen, else 1
     int key_bits;             // bits of the compact per-OM key; the OM sits above them
     long long key_mask;       // (1 << key_bits) - 1, also the compact sentinel
+    int key32;                // the keys fit 32 bits: stored and sorted as uint32
     long long theta_bins;
     double log_ratio;
     double mut_min, speed_max;
@@ -936,12 +937,24 @@ __global__ void __launch_bounds__(kThreads) o1_group_keys(Group G) {
     int n = pencil_entries(G, p, d, w, 0, e0, &key1);
     size_t at = ((size_t)w * G.n_pen + p) * G.slots;
     long long high = (long long)w << G.key_bits;
-    G.keys[at] = high | (n > 0 ? e0.key : G.key_mask);
-    G.vals[at] = 2 * p;
-    if (G.slots == 2) {
-        G.keys[at + 1] = high | (n > 1 ? key1 : G.key_mask);
-        G.vals[at + 1] = 2 * p + 1;
+    const long long k0 = high | (n > 0 ? e0.key : G.key_mask);
+    const long long k1 = high | (n > 1 ? key1 : G.key_mask);
+    if (G.key32) {
+        unsigned* keys = reinterpret_cast<unsigned*>(G.keys);
+        keys[at] = (unsigned)k0;
+        if (G.slots == 2) keys[at + 1] = (unsigned)k1;
+    } else {
+        G.keys[at] = k0;
+        if (G.slots == 2) G.keys[at + 1] = k1;
     }
+    G.vals[at] = 2 * p;
+    if (G.slots == 2) G.vals[at + 1] = 2 * p + 1;
+}
+
+// Sorted key j (stored as uint32 when the keys fit 32 bits).
+__device__ inline long long sorted_key(const Group& G, long long j) {
+    return G.key32 ? (long long)reinterpret_cast<const unsigned*>(G.sorted_keys)[j]
+                   : G.sorted_keys[j];
 }
 
 __device__ inline bool is_sentinel(const Group& G, long long key) {
@@ -953,8 +966,8 @@ __global__ void __launch_bounds__(kThreads) o1_group_flags(Group G, long long to
     if (j > total) return;
     if (j == total) { G.group_flag[j] = 0; return; }
     long long seg = (long long)G.slots * G.n_pen;
-    long long k = G.sorted_keys[j];
-    bool start = !is_sentinel(G, k) && (j % seg == 0 || G.sorted_keys[j - 1] != k);
+    long long k = sorted_key(G, j);
+    bool start = !is_sentinel(G, k) && (j % seg == 0 || sorted_key(G, j - 1) != k);
     G.group_flag[j] = start ? 1 : 0;
 }
 
@@ -970,8 +983,8 @@ __global__ void __launch_bounds__(kThreads) o1_group_starts(Group G, long long t
     }
     if (j % seg == 0) G.om_groups[j / seg] = G.group_scan[j];
     if (G.group_flag[j]) G.group_start[G.group_scan[j]] = (int)j;
-    long long k = G.sorted_keys[j];
-    if (!is_sentinel(G, k) && (j + 1 == total || (j + 1) % seg == 0 || G.sorted_keys[j + 1] != k))
+    long long k = sorted_key(G, j);
+    if (!is_sentinel(G, k) && (j + 1 == total || (j + 1) % seg == 0 || sorted_key(G, j + 1) != k))
         G.group_end[G.group_scan[j] + G.group_flag[j] - 1] = (int)(j + 1);
 }
 
@@ -1183,9 +1196,9 @@ __global__ void __launch_bounds__(kThreads) o1_batch_flags(Group G, int groups, 
     if (g == groups) { G.batch_flag[g] = 0; return; }
     long long seg = (long long)G.slots * G.n_pen;
     int w = (int)(G.group_start[g] / seg);
-    long long key = (G.sorted_keys[G.group_start[g]] & G.key_mask) / G.theta_bins;
+    long long key = (sorted_key(G, G.group_start[g]) & G.key_mask) / G.theta_bins;
     bool first = g == G.om_groups[w];
-    G.batch_flag[g] = (first || (G.sorted_keys[G.group_start[g - 1]] & G.key_mask)
+    G.batch_flag[g] = (first || (sorted_key(G, G.group_start[g - 1]) & G.key_mask)
                                         / G.theta_bins != key) ? 1 : 0;
 }
 
@@ -1442,7 +1455,7 @@ __device__ __forceinline__ BatchHead batch_head(const Group& G, int b, int lane)
     h.pad0 = h.pad1 = 0;
     h.n = h.n_in = h.n_out = 0;
     h.r_b = h.t_b = h.tau_var = h.sigma = h.u_win = h.u_end = 0.0;
-    h.backward = ((G.sorted_keys[G.group_start[start]] & G.key_mask) / G.theta_bins) % 2 == 1;
+    h.backward = ((sorted_key(G, G.group_start[start]) & G.key_mask) / G.theta_bins) % 2 == 1;
     double omega_sum = 0, r_sum = 0, t_sum = 0, tau_sum = 0, tau2_sum = 0, theta_small = M_PI;
     for (int q = start + lane; q < stop; q += 64) {
         const double* row = G.values + (size_t)q * nv;
@@ -2418,13 +2431,17 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
     LH_REQUIRE(entries < (1LL << 31) - 1,
                "%lld (pencil, OM) entries in one grouped call; pass fewer OMs per call",
                entries);
-    size_t sort_temp = 0, scan_temp = 0;
+    size_t sort_temp = 0, sort_temp32 = 0, scan_temp = 0;
     if (hipcub::DeviceRadixSort::SortPairs(
             nullptr, sort_temp, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
-            (int*)nullptr, (int*)nullptr, (int)entries, 0, 64, (hipStream_t)0) != hipSuccess) {
+            (int*)nullptr, (int*)nullptr, (int)entries, 0, 64, (hipStream_t)0) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairs(
+            nullptr, sort_temp32, (unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr,
+            (int*)nullptr, (int)entries, 0, 32, (hipStream_t)0) != hipSuccess) {
         lh::set_error("sort workspace query failed");
         return LH_ERR_HIP;
     }
+    sort_temp = std::max(sort_temp, sort_temp32);
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
                                          (int)(entries + 1), (hipStream_t)0) != hipSuccess) {
         lh::set_error("scan workspace query failed");
@@ -2769,6 +2786,9 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     G.key_mask = (1LL << G.key_bits) - 1;
     int end_bit = G.key_bits + bits_for((unsigned long long)std::max(work - 1, 0));
     LH_REQUIRE(end_bit <= 63, "grouped keys need %d bits; split the OM list", end_bit);
+    // keys of at most 32 bits are stored and sorted as uint32 (a third less
+    // data per sort pass)
+    G.key32 = end_bit <= 32;
     {
         lh::KernelMark mark("o1_group_keys", s);
         o1_group_keys<<<grid, kThreads, 0, s>>>(G);
@@ -2777,10 +2797,17 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     size_t sort_bytes = L.scan_temp - L.sort_temp;
     {
         lh::KernelMark mark("o1_sort", s);
-        LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
-            base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned long long*>(G.keys),
-            reinterpret_cast<unsigned long long*>(G.sorted_keys), G.vals, G.sorted_vals,
-            (int)total, 0, end_bit, s));
+        if (G.key32) {
+            LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+                base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned*>(G.keys),
+                reinterpret_cast<unsigned*>(G.sorted_keys), G.vals, G.sorted_vals, (int)total,
+                0, end_bit, s));
+        } else {
+            LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+                base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned long long*>(G.keys),
+                reinterpret_cast<unsigned long long*>(G.sorted_keys), G.vals, G.sorted_vals,
+                (int)total, 0, end_bit, s));
+        }
     }
     o1_group_flags<<<blocks(total + 1), kThreads, 0, s>>>(G, total);
     if ((status = lh::check_launch("o1_group_flags")) != LH_OK) return status;
@@ -2826,6 +2853,7 @@ int lh_order1_group_charge(const lh_order1_pencils* pencils, const lh_order1_oms
     Group G = make_group(pencils, oms, optics, merge, base, L);
     G.key_bits = key_bits;
     G.key_mask = (1LL << key_bits) - 1;
+    G.key32 = key_bits + bits_for((unsigned long long)std::max(work - 1, 0)) <= 32;
     G.compute_bins = compute_bins;
     G.work_counts = work_counts;
     G.charge = charge;
