@@ -79,6 +79,7 @@ struct Compress {
     int* ids;               // (nodes)
     long long* sorted_keys;
     int* sorted_ids;
+    int key32;              // keys below 2^32: stored and sorted as uint32
     double* cp;             // (nodes) ring azimuth cosine
     double* sp;             // (nodes) ring azimuth sine
     int* sub_element;       // (subsegments) unit element index of a sub-segment
@@ -153,17 +154,25 @@ __global__ void __launch_bounds__(kThreads) o1_nodes(Compress k) {
     double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
     double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
     double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
-    k.keys[node] = ck * n_pix + pixel_of(sx, sy, sz, face);
+    const long long key = ck * n_pix + pixel_of(sx, sy, sz, face);
+    if (k.key32) reinterpret_cast<unsigned*>(k.keys)[node] = (unsigned)key;
+    else k.keys[node] = key;
     k.ids[node] = (int)node;
     k.cp[node] = cp;
     k.sp[node] = sp;
+}
+
+// Sorted node key j (stored as uint32 when every key is below 2^32).
+__device__ inline long long node_key(const Compress& k, long long j) {
+    return k.key32 ? (long long)reinterpret_cast<const unsigned*>(k.sorted_keys)[j]
+                   : k.sorted_keys[j];
 }
 
 __global__ void __launch_bounds__(kThreads) o1_run_flags(Compress k) {
     long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j > k.nodes) return;
     if (j == k.nodes) { k.run_flag[j] = 0; return; }
-    k.run_flag[j] = (j == 0 || k.sorted_keys[j] != k.sorted_keys[j - 1]) ? 1 : 0;
+    k.run_flag[j] = (j == 0 || node_key(k, j) != node_key(k, j - 1)) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kThreads) o1_run_starts(Compress k) {
@@ -178,7 +187,7 @@ __global__ void __launch_bounds__(kThreads) o1_run_starts(Compress k) {
     if (k.run_flag[j]) {
         int r = k.run_scan[j];
         k.run_start[r] = (int)j;
-        k.run_key[r] = k.sorted_keys[j];
+        k.run_key[r] = node_key(k, j);
     }
 }
 
@@ -2312,13 +2321,18 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
     size_t sort_temp = 0, scan_temp = 0, psort_temp = 0;
     unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
                                  * 6ULL * grid->pixel_face * grid->pixel_face;
+    size_t sort_temp32 = 0;
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_temp, (long long*)nullptr,
                                            (long long*)nullptr, (int*)nullptr, (int*)nullptr,
                                            (int)nodes, 0, bits_for(max_key), (hipStream_t)0)
-        != hipSuccess) {
+            != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairs(nullptr, sort_temp32, (unsigned*)nullptr,
+                                           (unsigned*)nullptr, (int*)nullptr, (int*)nullptr,
+                                           (int)nodes, 0, 32, (hipStream_t)0) != hipSuccess) {
         lh::set_error("radix sort workspace query failed");
         return LH_ERR_HIP;
     }
+    sort_temp = std::max(sort_temp, sort_temp32);
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_temp, (int*)nullptr, (int*)nullptr,
                                          (int)(nodes + 1), (hipStream_t)0) != hipSuccess) {
         lh::set_error("scan workspace query failed");
@@ -2407,6 +2421,10 @@ Compress make_compress(const lh_order1_chords* ch, const lh_order1_grid* grid, c
     k.run_scan = reinterpret_cast<int*>(base + L.run_scan);
     k.run_start = reinterpret_cast<int*>(base + L.run_start);
     k.run_key = reinterpret_cast<long long*>(base + L.run_key);
+    const unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1]
+                                       * grid->dims[2] * 6ULL * grid->pixel_face
+                                       * grid->pixel_face;
+    k.key32 = bits_for(max_key) <= 32;
     k.rows = reinterpret_cast<double*>(base + L.rows);
     k.info = reinterpret_cast<int*>(base + L.info);
     return k;
@@ -2637,9 +2655,16 @@ int lh_order1_compress_keys(const lh_order1_chords* chords, const lh_order1_grid
     unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
                                  * 6ULL * grid->pixel_face * grid->pixel_face;
     size_t sort_bytes = L.scan_temp - L.sort_temp;
-    LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(base + L.sort_temp, sort_bytes, k.keys,
-                                                   k.sorted_keys, k.ids, k.sorted_ids, (int)nodes,
-                                                   0, bits_for(max_key), s));
+    if (k.key32) {
+        LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+            base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned*>(k.keys),
+            reinterpret_cast<unsigned*>(k.sorted_keys), k.ids, k.sorted_ids, (int)nodes, 0,
+            bits_for(max_key), s));
+    } else {
+        LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(base + L.sort_temp, sort_bytes, k.keys,
+                                                       k.sorted_keys, k.ids, k.sorted_ids,
+                                                       (int)nodes, 0, bits_for(max_key), s));
+    }
     o1_run_flags<<<blocks(nodes + 1), kThreads, 0, s>>>(k);
     if ((status = lh::check_launch("o1_run_flags")) != LH_OK) return status;
     size_t scan_bytes = L.psort_temp - L.scan_temp;
