@@ -80,6 +80,7 @@ struct Compress {
     long long* sorted_keys;
     int* sorted_ids;
     int key32;              // keys below 2^32: stored and sorted as uint32
+    double* sub_record;     // (subsegments, kSubRecord) per-sub-segment moment parts
     double* cp;             // (nodes) ring azimuth cosine
     double* sp;             // (nodes) ring azimuth sine
     int* sub_element;       // (subsegments) unit element index of a sub-segment
@@ -194,11 +195,19 @@ __global__ void __launch_bounds__(kThreads) o1_run_starts(Compress k) {
 // Moment row of one (cell, pixel) key: its nodes in enumeration order (:1192-1223).
 // Moment contributions of one ring node (the terms summed per pencil by
 // compress_ring_pencils, _prompt_numba.py:2246-2279).
-__device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
+// The per-sub-segment part of a ring node's moments: direction, ring basis,
+// cone cosine and sine, share weights, sub-point, time and the two spreads.
+// Every node of a sub-segment shares it, so it is evaluated once per
+// sub-segment (its divisions and square roots were once per node).
+constexpr int kSubRecord = 20;   // doubles per record: u e1 e2 p (3 each),
+                                 // cc cs w0 w2 t arc2 sub2, one spare
+
+__global__ void __launch_bounds__(kThreads) o1_sub_records(Compress k) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= k.grid.subsegments) return;
     const lh_order1_chords& c = k.ch;
     int R = k.grid.ring_nodes;
     double a_step = 0.5 * k.grid.cell_m;
-    long long s = node / R;
     int i = k.sub_element[s], ia = k.sub_index[s];
     int e = c.index ? c.index[i] : i;
     double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
@@ -209,12 +218,40 @@ __device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
     int n_a = n_sub(c.length_m[e], a_step);
     double share = 1.0 / ((double)n_a * R);
     double arc = 2.0 * M_PI * cs / R;
-    double arc2 = arc * arc / 12.0;
     double sl = c.length_m[e] / n_a;
-    double sub2 = sl * sl / 12.0;
-    double w0 = c.coefficient0[e] * share, w2 = c.coefficient2[e] * share;
     double p[3], t;
     sub_point(c, e, ia, n_a, p, &t);
+    double* out = k.sub_record + (size_t)s * kSubRecord;
+    for (int d = 0; d < 3; ++d) {
+        out[d] = u[d];
+        out[3 + d] = e1[d];
+        out[6 + d] = e2[d];
+        out[9 + d] = p[d];
+    }
+    out[12] = cc;
+    out[13] = cs;
+    out[14] = c.coefficient0[e] * share;
+    out[15] = c.coefficient2[e] * share;
+    out[16] = t;
+    out[17] = arc * arc / 12.0;
+    out[18] = sl * sl / 12.0;
+    out[19] = 0.0;
+}
+
+__device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
+    int R = k.grid.ring_nodes;
+    long long s = node / R;
+    const double* rec = k.sub_record + (size_t)s * kSubRecord;
+    double u[3], e1[3], e2[3], p[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        u[d] = rec[d];
+        e1[d] = rec[3 + d];
+        e2[d] = rec[6 + d];
+        p[d] = rec[9 + d];
+    }
+    const double cc = rec[12], cs = rec[13], w0 = rec[14], w2 = rec[15], t = rec[16];
+    const double arc2 = rec[17], sub2 = rec[18];
     double cp = k.cp[node], sp = k.sp[node];
     double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
     double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
@@ -2311,6 +2348,7 @@ int bits_for(unsigned long long value) {
 struct CompressLayout {
     size_t keys, ids, sorted_keys, sorted_ids, cp, sp, sub_element, sub_index, run_flag, run_scan;
     size_t run_start, run_key, rows, info, sort_temp, scan_temp, pkeys, pids, spkeys, spids,
+        sub_record,
         pflag, pscan, psort_temp, piece_count, piece_start, big_count, big_start, piece_rows,
         piece_run, total;
 };
@@ -2356,6 +2394,7 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
     L.sp = take(nodes * sizeof(double));
     L.sub_element = take(grid->subsegments * sizeof(int));
     L.sub_index = take(grid->subsegments * sizeof(int));
+    L.sub_record = take((size_t)grid->subsegments * kSubRecord * sizeof(double));
     L.run_flag = take((nodes + 1) * sizeof(int));
     L.run_scan = take((nodes + 1) * sizeof(int));
     L.run_start = take((nodes + 1) * sizeof(int));
@@ -2421,6 +2460,7 @@ Compress make_compress(const lh_order1_chords* ch, const lh_order1_grid* grid, c
     k.run_scan = reinterpret_cast<int*>(base + L.run_scan);
     k.run_start = reinterpret_cast<int*>(base + L.run_start);
     k.run_key = reinterpret_cast<long long*>(base + L.run_key);
+    k.sub_record = reinterpret_cast<double*>(base + L.sub_record);
     const unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1]
                                        * grid->dims[2] * 6ULL * grid->pixel_face
                                        * grid->pixel_face;
@@ -2709,6 +2749,8 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
         long long bound = runs + nodes / kRowPiece + 1;
         o1_piece_runs<<<blocks(runs), kThreads, 0, s>>>(runs, piece_start, piece_run);
         if ((status = lh::check_launch("o1_piece_runs")) != LH_OK) return status;
+        o1_sub_records<<<blocks(grid->subsegments), kThreads, 0, s>>>(k);
+        if ((status = lh::check_launch("o1_sub_records")) != LH_OK) return status;
         {
             lh::KernelMark mark("o1_row_pieces", s);
             o1_row_pieces<<<(unsigned)((bound + 3) / 4), kThreads, 0, s>>>(k, runs, piece_start,
