@@ -33,6 +33,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kNmom = 23;
+// Per-sub-segment record of the ring nodes (o1_sub_records), in doubles:
+// u e1 e2 p (3 each), cc cs w0 w2 t arc2 sub2, the ring's angle offset, the
+// cell index (its bits), one spare.
+constexpr int kSubRecord = 22;
 constexpr int kMaxMono = 15;        // acceptance degree <= 4
 constexpr int kMaxBands = 16;
 constexpr int kGauss = 48;
@@ -129,26 +133,16 @@ __global__ void __launch_bounds__(kThreads) o1_nodes(Compress k) {
     if (node >= k.grid.subsegments * (long long)R) return;
     long long s = node / R;
     int q = (int)(node - s * R);
-    int i = k.sub_element[s], ia = k.sub_index[s];
-    int e = k.ch.index ? k.ch.index[i] : i;
-    const lh_order1_chords& c = k.ch;
-    double u[3] = {c.direction[3 * e], c.direction[3 * e + 1], c.direction[3 * e + 2]};
-    double e1[3], e2[3];
-    ring_basis(u, e1, e2);
-    double cc = c.cone_cosine[e];
-    double cs = sqrt(fmax(1.0 - cc * cc, 0.0));
-    double a_step = 0.5 * k.grid.cell_m;
-    int n_a = n_sub(c.length_m[e], a_step);
-    const double golden = 0.6180339887498949;
-    double offset = fmod(i * golden, 1.0) * (2.0 * M_PI) / R;
-    double p[3], t;
-    sub_point(c, e, ia, n_a, p, &t);
-    long long cell[3];
+    // the sub-segment's direction, ring basis, cone, angle offset and cell
+    const double* rec = k.sub_record + (size_t)s * kSubRecord;
+    double u[3], e1[3], e2[3];
     for (int d = 0; d < 3; ++d) {
-        long long v = (long long)floor((p[d] - k.grid.origin[d]) / k.grid.cell_m);
-        cell[d] = min(max(v, 0LL), k.grid.dims[d] - 1);
+        u[d] = rec[d];
+        e1[d] = rec[3 + d];
+        e2[d] = rec[6 + d];
     }
-    long long ck = (cell[0] * k.grid.dims[1] + cell[1]) * k.grid.dims[2] + cell[2];
+    const double cc = rec[12], cs = rec[13], offset = rec[19];
+    const long long ck = __double_as_longlong(rec[20]);
     long long n_pix = 6LL * face * face;
     double sp, cp;
     sincos(offset + q * (2.0 * M_PI / R), &sp, &cp);
@@ -199,8 +193,6 @@ __global__ void __launch_bounds__(kThreads) o1_run_starts(Compress k) {
 // cone cosine and sine, share weights, sub-point, time and the two spreads.
 // Every node of a sub-segment shares it, so it is evaluated once per
 // sub-segment (its divisions and square roots were once per node).
-constexpr int kSubRecord = 20;   // doubles per record: u e1 e2 p (3 each),
-                                 // cc cs w0 w2 t arc2 sub2, one spare
 
 __global__ void __launch_bounds__(kThreads) o1_sub_records(Compress k) {
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,7 +227,16 @@ __global__ void __launch_bounds__(kThreads) o1_sub_records(Compress k) {
     out[16] = t;
     out[17] = arc * arc / 12.0;
     out[18] = sl * sl / 12.0;
-    out[19] = 0.0;
+    const double golden = 0.6180339887498949;
+    out[19] = fmod(i * golden, 1.0) * (2.0 * M_PI) / R;
+    long long cell[3];
+    for (int d = 0; d < 3; ++d) {
+        long long v = (long long)floor((p[d] - k.grid.origin[d]) / k.grid.cell_m);
+        cell[d] = min(max(v, 0LL), k.grid.dims[d] - 1);
+    }
+    out[20] = __longlong_as_double((cell[0] * k.grid.dims[1] + cell[1]) * k.grid.dims[2]
+                                   + cell[2]);
+    out[21] = 0.0;
 }
 
 __device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
@@ -2689,6 +2690,10 @@ int lh_order1_compress_keys(const lh_order1_chords* chords, const lh_order1_grid
     Compress k = make_compress(chords, grid, base, L);
     o1_subsegments<<<blocks(chords->count), kThreads, 0, s>>>(k);
     if ((status = lh::check_launch("o1_subsegments")) != LH_OK) return status;
+    // the per-sub-segment record, read by o1_nodes here and by the moment
+    // rows of lh_order1_compress_pencils (same workspace)
+    o1_sub_records<<<blocks(grid->subsegments), kThreads, 0, s>>>(k);
+    if ((status = lh::check_launch("o1_sub_records")) != LH_OK) return status;
     o1_nodes<<<(unsigned)((grid->subsegments * (long long)grid->ring_nodes + kThreads - 1)
                           / kThreads), kThreads, 0, s>>>(k);
     if ((status = lh::check_launch("o1_nodes")) != LH_OK) return status;
@@ -2749,8 +2754,6 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
         long long bound = runs + nodes / kRowPiece + 1;
         o1_piece_runs<<<blocks(runs), kThreads, 0, s>>>(runs, piece_start, piece_run);
         if ((status = lh::check_launch("o1_piece_runs")) != LH_OK) return status;
-        o1_sub_records<<<blocks(grid->subsegments), kThreads, 0, s>>>(k);
-        if ((status = lh::check_launch("o1_sub_records")) != LH_OK) return status;
         {
             lh::KernelMark mark("o1_row_pieces", s);
             o1_row_pieces<<<(unsigned)((bound + 3) / 4), kThreads, 0, s>>>(k, runs, piece_start,
