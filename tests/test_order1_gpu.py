@@ -150,3 +150,36 @@ def test_spectral_order1_with_global_screen_matches_oracle(device):
     _close(got[0], want[0])
     _close(got[1], want[1])
     assert abs(got[0].sum() - 50.68) < 0.01
+
+
+def test_grouped_keys_wider_than_32_bits_match_oracle(device, monkeypatch):
+    """A 1e-5 ns delay step makes the per-OM key space exceed 2^32: the grouped
+    pass then stores and sorts 64-bit keys (narrower keys go as uint32)."""
+    from lighthit_amd import _lib
+    from lighthit_amd.compile import DeviceChords
+    from lighthit_amd.order1 import (DeviceOMs, DeviceOptics, DeviceUnitChords, PromptSettings,
+                                     compress, grouped)
+    w, d, tables, el, origins, phase = _setup()
+    alpha = acceptance_alpha(d)
+    unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
+    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, PromptSettings(), device)
+    pencils = compress(unit, 0.5, 96, 192)
+    merge = (2e-3, 0.08, 1e-5, 0.02)
+    work = np.arange(len(d))
+    seen = []
+    call = _lib.call
+
+    def spy(name, *args):
+        if name == "lh_order1_group_charge":
+            seen.append(int(args[5]))      # the key bits of the compact per-OM keys
+        return call(name, *args)
+    monkeypatch.setattr(_lib, "call", spy)
+    charge, bins = grouped(pencils, oms, work, optics, merge)
+    assert seen and seen[0] + int(len(work) - 1).bit_length() > 32
+    want_pencils = oracle.compress(el, 1.0 / (el.beta * phase), 0.5, 96, 192)
+    want_charge, want_bins, _, _ = oracle.grouped(
+        d.positions_m, d.looks, d.effective_area_m2, origins, work, want_pencils, tables, alpha,
+        w.medium.g, w.edges_ns, oracle.PromptSettings(), merge, compute_bins=True)
+    _close(charge.cpu().numpy(), want_charge)
+    _close(bins.cpu().numpy(), want_bins)
