@@ -45,6 +45,23 @@ def test_fine_pencils_match_oracle(device):
         _close(got[key], want[key])
 
 
+def test_fine_pencils_with_wide_keys_match_oracle(device):
+    """5 cm cells make the (cell, pixel) key space exceed 2^32: the node sort
+    then runs on 64-bit keys (narrower keys go as uint32)."""
+    from lighthit_amd.compile import DeviceChords
+    from lighthit_amd.order1 import DeviceUnitChords, compress
+    w, d, tables, el, origins, phase = _setup()
+    unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
+    grid = unit.grid(0.05, 96, 24)
+    assert int(np.prod(list(grid.dims))) * 6 * 96 * 96 >= 2 ** 32
+    got = compress(unit, 0.05, 96, 24).host()
+    want = oracle.compress(el, 1.0 / (el.beta * phase), 0.05, 96, 24)
+    assert len(got["w0"]) == len(want["w0"])
+    for key in ("position", "direction", "w0", "w2", "time", "major", "var_major",
+                "var_minor", "pos_cov"):
+        _close(got[key], want[key])
+
+
 def test_screen_pencils_match_oracle(device):
     from lighthit_amd.compile import DeviceChords
     from lighthit_amd.order1 import DeviceUnitChords, compress
