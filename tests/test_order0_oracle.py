@@ -181,3 +181,47 @@ def test_bins_hold_the_charge_inside_a_wide_window(demo):
                                       edges=np.arange(-100., 2000., 5.))
     np.testing.assert_allclose(bins.sum(axis=1), charge, rtol=1e-10)
     assert np.all(charge > 0)
+
+
+def test_leading_model_reproduces_the_published_demonstration_total(demo):
+    """README.md:133 quotes 63.74 pe of order 0 for the demonstration shower: the
+    value of the leading finite-cone model (ballistic_interval.py:163-265), which
+    the oracle restates independently of its first-entry evaluator."""
+    w, optics, alpha = demo
+    d = w.detector
+    leading = order0.leading_interval_charge(d.positions_m, d.looks, d.effective_area_m2,
+                                             d.receiver_surface.radii(len(d)), w.elements,
+                                             optics, alpha)
+    assert abs(leading.sum() - 63.74) <= 0.005
+
+
+def test_leading_model_axial_limit_is_the_point_receiver_integral():
+    """Far from a small sphere, the window law integrates to one: the leading model
+    equals the point-receiver line integral of _integral (closed form in lambda for
+    a constant-phase medium: one Gauss node is exact)."""
+    beta, length = 1.0, 0.01
+    medium = SpectralMedium([400., 500.], [.02, .02], [.01, .01], [1.34, 1.34], [1.37, 1.37], g=.9)
+    rho = 40.0
+    n = 1.34
+    sine = np.sqrt(1 - 1 / (beta * n) ** 2)
+    cosine = 1 / (beta * n)
+    # OM on the cone of the element's midpoint
+    centre = np.array([rho, 0., length / 2 + rho * cosine / sine])
+    look = np.array([0., 0., 1.])
+    detector = DetectorArray([centre], [-look], .05, lambda x: .45 + .45 * x + .1 * x * x,
+                             lambda lam: np.full_like(lam, .25),
+                             receiver_surface=SphereSurface(1e-4))
+    alpha = acceptance_alpha(detector)
+    elements = _element(length, beta)
+    optics = prepare_optics(medium, detector)
+    optics = dict(optics, phase_nodes=optics["phase_nodes"] + 1e-12 * np.arange(
+        len(optics["phase_nodes"])))
+    got = order0.leading_interval_charge(detector.positions_m, detector.looks, .05, 1e-4,
+                                         elements, optics, alpha, rel_tol=1e-10)
+    # photons per nm per m: C0/lambda^2 (1 - 1/(beta n)^2); geometry 1/(2 pi rho sin)
+    lam = np.linspace(400., 500., 20001)
+    spectral = np.trapezoid(.25 * 1e8 * length * sine ** 2 / lam ** 2, lam)
+    distance = rho / sine
+    accept = (.45 + .45 * cosine + .1 * cosine ** 2)
+    expected = spectral / length * np.exp(-.03 * distance) * accept / (2 * np.pi * rho * sine) * .05
+    np.testing.assert_allclose(got, [expected], rtol=1e-6)
