@@ -203,6 +203,25 @@ int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
                      size_t workspace_bytes, void* stream);
 
 /*
+ * lh_order0_charge without a workspace argument and without any device
+ * allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The workspace lives in a fixed device array of the
+ * library, 64 MiB per GPU (lh_order0_charge_noalloc_max_bytes), held from the
+ * first call on a GPU until the process ends; a call whose
+ * lh_order0_workspace_bytes(oms->count, elements->count) exceeds it fails with
+ * LH_ERR_ARGUMENT. flags (device int, may be NULL) receives the flag word that
+ * lh_order0_charge leaves in its workspace. Every call on one GPU shares that
+ * array: calls that overlap on one GPU (concurrent streams or host threads)
+ * race and must be serialised by the caller. The result equals
+ * lh_order0_charge's bit for bit (same kernels).
+ */
+size_t lh_order0_charge_noalloc_max_bytes(void);
+int lh_order0_charge_noalloc(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                             const lh_order0_elements* elements, double* charge, int* flags,
+                             void* stream);
+
+/*
  * Order-0 arrival bins of the OMs in `oms` (normally the active subset).
  *
  * Replaces PreparedBallistic.bins(ids, origins, edges) (_surface_ballistic.py:

@@ -46,6 +46,13 @@ _lib.SIGNATURES["lh_order0_charge"] = (ctypes.c_int, [ctypes.POINTER(_Optics), c
                                                       ctypes.c_void_p, ctypes.c_size_t,
                                                       ctypes.c_void_p])
 
+_lib.SIGNATURES["lh_order0_charge_noalloc_max_bytes"] = (ctypes.c_size_t, [])
+_lib.SIGNATURES["lh_order0_charge_noalloc"] = (ctypes.c_int, [ctypes.POINTER(_Optics),
+                                                              ctypes.POINTER(_Oms),
+                                                              ctypes.POINTER(_Elements),
+                                                              ctypes.c_void_p, ctypes.c_void_p,
+                                                              ctypes.c_void_p])
+
 _lib.SIGNATURES["lh_order0_bins_workspace_bytes"] = (ctypes.c_size_t,
                                                      [ctypes.c_int, ctypes.c_int, ctypes.c_int])
 _lib.SIGNATURES["lh_order0_bins"] = (ctypes.c_int, [ctypes.POINTER(_Optics), ctypes.POINTER(_Oms),
@@ -174,6 +181,16 @@ class Order0:
                       ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
                       ctypes.c_size_t(ws.numel()), _lib.stream_handle(stream))
         return out, ws
+
+    def charge_noalloc_device(self, elements, out, flags, stream=None):
+        """Enqueue the charge through ``lh_order0_charge_noalloc`` into the caller's
+        ``out`` (OM,) float64 and ``flags`` (1,) int32 device tensors: no device
+        allocation, the library's fixed scratch instead of a workspace."""
+        _lib.call("lh_order0_charge_noalloc", ctypes.byref(self.optics.struct),
+                  ctypes.byref(self.oms.struct), ctypes.byref(elements.struct),
+                  ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(flags.data_ptr()),
+                  _lib.stream_handle(stream))
+        return out
 
     def bins_device(self, elements, oms, origins, edges, out=None, stream=None):
         """Enqueue the arrival bins of ``oms`` (a :class:`DeviceOMs`, normally the active

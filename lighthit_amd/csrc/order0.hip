@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace {
 
@@ -679,6 +680,11 @@ __global__ void order0_reset(int* flags) {
     if (threadIdx.x < 64) flags[threadIdx.x] = 0;
 }
 
+// Copies the flag word of a call to the caller's output.
+__global__ void order0_copy_flags(const int* flags, int* out) {
+    if (threadIdx.x == 0) out[0] = flags[0];
+}
+
 // Zeroes `count` doubles.
 __global__ void order0_zero(double* out, size_t count) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1065,4 +1071,83 @@ extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_om
     P.nbins = n_bins;
     const size_t lds = sizeof(double) * kWaves * (size_t)n_bins + sizeof(float) * (n_bins + 1);
     return run<true>(P, n_bins, bins, s, lds, "lh_order0_bins");
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation charge entry point: the per-call workspace of lh_order0_charge
+// lives in a fixed device array of this file. Every GPU holds its own copy of
+// the array (64 MiB) from the library's first use on it until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device.
+
+namespace {
+
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_address(void** out) {
+    int device = -1;
+    LH_HIP_CALL(hipGetDevice(&device));
+    LH_REQUIRE(device >= 0 && device < kMaxDevices,
+               "lh_order0_charge_noalloc: device %d outside 0..%d", device, kMaxDevices - 1);
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (g_scratch_ptr[device] == nullptr) {
+        void* ptr = nullptr;
+        LH_HIP_CALL(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        LH_REQUIRE(ptr != nullptr, "lh_order0_charge_noalloc: no device address for scratch");
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return LH_OK;
+}
+
+}  // namespace
+
+extern "C" size_t lh_order0_charge_noalloc_max_bytes(void) { return kNoallocScratchBytes; }
+
+extern "C" int lh_order0_charge_noalloc(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                                        const lh_order0_elements* elements, double* charge,
+                                        int* flags, void* stream) {
+    int status = check_descriptors(optics, oms, elements, "lh_order0_charge_noalloc");
+    if (status != LH_OK) return status;
+    const size_t need = lh_order0_workspace_bytes(oms->count, elements->count);
+    LH_REQUIRE(need <= kNoallocScratchBytes,
+               "lh_order0_charge_noalloc: %d OMs x %d elements need %zu bytes of scratch, "
+               "above its fixed %zu; call lh_order0_charge with a workspace instead",
+               oms->count, elements->count, need, kNoallocScratchBytes);
+    void* ws = nullptr;
+    status = scratch_address(&ws);
+    if (status != LH_OK) return status;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the flag words start at zero on every call (a kernel, not a memset)
+    hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(ws));
+    status = lh::check_launch("lh_order0_charge_noalloc (reset)");
+    if (status != LH_OK) return status;
+    if (oms->count > 0) {
+        if (elements->count == 0) {
+            hipLaunchKernelGGL(order0_zero, dim3((oms->count + 255) / 256), dim3(256), 0, s,
+                               charge, (size_t)oms->count);
+            status = lh::check_launch("lh_order0_charge_noalloc (zero)");
+        } else {
+            O0Params P = make_params(optics, oms, elements, ws, 1);
+            status = run<false>(P, 1, charge, s, 0, "lh_order0_charge_noalloc");
+        }
+        if (status != LH_OK) return status;
+    }
+    if (flags != nullptr) {
+        hipLaunchKernelGGL(order0_copy_flags, dim3(1), dim3(64), 0, s,
+                           static_cast<const int*>(ws), flags);
+        status = lh::check_launch("lh_order0_charge_noalloc (flags)");
+    }
+    return status;
 }
