@@ -173,11 +173,11 @@ def cpu_baseline(workload, kernel, route, response, fraction):
     seconds = {}
     with threadpool_limits(limits=1):
         began = time.perf_counter()
-        oracle_order0.first_entry(d.positions_m[work1], d.looks[work1],
+        oracle_order0.first_entry(d.positions_m[work1], (-d.orientations)[work1],
                                   d.effective_area_m2[work1], radii[work1], sample, optics,
                                   alpha, rel_tol=2e-6)
         if len(active):
-            oracle_order0.first_entry(d.positions_m[active], d.looks[active],
+            oracle_order0.first_entry(d.positions_m[active], (-d.orientations)[active],
                                       d.effective_area_m2[active], radii[active], sample,
                                       optics, alpha, origins=origins[active], edges=edges,
                                       rel_tol=2e-6)
@@ -185,7 +185,7 @@ def cpu_baseline(workload, kernel, route, response, fraction):
         began = time.perf_counter()
         spectral = oracle_order1.spectral_tables(kernel.medium, d, kernel.wavelength)
         charge = response.charge_components_pe
-        oracle_order1.spectral_order1(elements, d.positions_m, d.looks, d.effective_area_m2,
+        oracle_order1.spectral_order1(elements, d.positions_m, (-d.orientations), d.effective_area_m2,
                                       origins, work1, spectral, alpha, float(kernel.medium.g),
                                       edges, float(kernel.config.threshold_pe), charge[:, 0],
                                       2.0 * np.abs(charge[:, 2]), oracle_order1.PromptSettings())
@@ -200,7 +200,7 @@ def cpu_baseline(workload, kernel, route, response, fraction):
         seconds["compile"] = time.perf_counter() - began
         began = time.perf_counter()
         window = tuple(kernel.config.radial_range_m)
-        pos, looks, area = d.positions_m, d.looks, d.effective_area_m2
+        pos, looks, area = d.positions_m, (-d.orientations), d.effective_area_m2
         zero_sources = [copy.copy(s) for s in sources]
         for s in zero_sources:
             object.__setattr__(s, "channels", np.ascontiguousarray(s.channels[:, :, :1]))

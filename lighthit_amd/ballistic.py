@@ -14,6 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from .model import glass_radii, look_directions
 from .optics import prepare_optics
 
 
@@ -111,12 +112,13 @@ class DeviceOMs:
 
     def __init__(self, detector, device="cuda"):
         import torch
-        if detector.receiver_surface is None:
+        radii = glass_radii(detector)
+        if radii is None:
             raise ValueError("the first-entry GPU path needs detector.receiver_surface geometry")
-        self.count = len(detector)
-        self._keep = _lib.upload([detector.positions_m, detector.looks,
-                                  detector.effective_area_m2,
-                                  detector.receiver_surface.radii(self.count)], device)
+        self.count = len(radii)
+        area = np.broadcast_to(np.asarray(detector.effective_area_m2, float), (self.count,))
+        self._keep = _lib.upload([detector.positions_m, look_directions(detector), area, radii],
+                                 device)
         self.struct = _Oms(*[t.data_ptr() for t in self._keep], self.count)
 
 

@@ -1,213 +1,219 @@
-"""Medium, wavelength quadrature, detector and receiver contracts.
+"""Inputs of the route: medium, wavelength quadrature, OMs and their glass.
 
-Mirrors the reference's public input types — ``WavelengthQuadrature``,
-``SpectralMedium``, ``DetectorArray`` (``model.py:21-171``), ``SphereSurface``
-(``receiver.py:6-51``) and the acceptance Legendre coefficients
-(``cache.py:81-99``, ``directional.py:107-130``, ``transport.py:496-530``) —
-with the same validation and error messages.
+The route reads these objects only through the reference's attribute names
+(``SpectralMedium``, ``WavelengthQuadrature``, ``DetectorArray`` of
+``model.py:21-171``; ``SphereSurface`` of ``receiver.py:6-26``), so the
+reference's own objects can be handed to it unchanged. Two views are all the
+route takes of a detector beyond those attributes: :func:`look_directions`
+(``-orientations``, the photon direction an OM sees head-on, as the
+reference's routes form it at ``shower/api.py:271`` and ``prompt.py:530``) and
+:func:`glass_radii`.
+
+The classes below are small stand-ins with the same attribute set, for code
+without a reference installation (tests, benchmarks, synthetic workloads).
+They check what the route relies on (shapes, finiteness, signs) and nothing
+else.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field, replace
-from typing import Callable
-
 import numpy as np
-from numpy.polynomial.legendre import leggauss
+
+_MEDIUM_COLUMNS = ("absorption_per_m", "scattering_per_m", "phase_index", "group_index")
 
 
-def _axis(values, name, *, positive=False):
-    array = np.asarray(values, float)
-    if array.ndim != 1 or not len(array) or not np.isfinite(array).all():
-        raise ValueError(f"{name} must be a finite nonempty vector")
-    if positive and np.any(array <= 0):
-        raise ValueError(f"{name} must be positive")
+def look_directions(detector):
+    """Unit photon directions seen head-on by each OM: ``-orientations``, (N, 3)."""
+    orientation = np.asarray(detector.orientations, float)
+    norm = np.linalg.norm(orientation, axis=1, keepdims=True)
+    if np.any(norm <= 0):
+        raise ValueError("detector orientations must be nonzero")
+    return np.ascontiguousarray(-orientation / norm)
+
+
+def glass_radii(detector):
+    """Radius of every OM's glass sphere, or ``None`` when the detector has no surface."""
+    surface = getattr(detector, "receiver_surface", None)
+    if surface is None:
+        return None
+    return np.ascontiguousarray(surface.radii(len(detector.positions_m)), float)
+
+
+def _finite(values, what, *, shape=None, positive=False, nonnegative=False):
+    array = np.array(values, dtype=float)
+    if shape is not None and array.shape != shape:
+        raise ValueError(f"{what}: shape {array.shape}, expected {shape}")
+    if array.size == 0 or not np.all(np.isfinite(array)):
+        raise ValueError(f"{what} must be finite and nonempty")
+    if positive and not np.all(array > 0):
+        raise ValueError(f"{what} must be positive")
+    if nonnegative and not np.all(array >= 0):
+        raise ValueError(f"{what} must be nonnegative")
     return array
 
 
-@dataclass(frozen=True)
 class WavelengthQuadrature:
-    """Nodes and weights in nm (``model.py:21-44``)."""
-    wavelength_nm: np.ndarray
-    weight_nm: np.ndarray
+    """Wavelength nodes and weights, both in nm."""
 
-    def __post_init__(self):
-        wavelength = _axis(self.wavelength_nm, "wavelength_nm", positive=True)
-        weight = _axis(self.weight_nm, "weight_nm", positive=True)
-        if wavelength.shape != weight.shape or np.any(np.diff(wavelength) <= 0):
-            raise ValueError("quadrature wavelengths must increase and match weights")
-        object.__setattr__(self, "wavelength_nm", wavelength)
-        object.__setattr__(self, "weight_nm", weight)
+    def __init__(self, wavelength_nm, weight_nm):
+        nodes = _finite(wavelength_nm, "wavelength_nm", positive=True).ravel()
+        weights = _finite(weight_nm, "weight_nm", shape=nodes.shape, positive=True)
+        if len(nodes) > 1 and not np.all(nodes[1:] > nodes[:-1]):
+            raise ValueError("wavelength nodes must increase")
+        self.wavelength_nm, self.weight_nm = nodes, weights
 
     @classmethod
     def gauss_legendre(cls, low_nm, high_nm, nodes=9):
+        """Gauss-Legendre rule of ``nodes`` points on [low, high]. The nodes are
+        formed as low + half (x + 1) so that they equal the reference's to the
+        last bit (the band fold picks node bands by index)."""
+        count = int(nodes)
+        if count != nodes or count < 1 or isinstance(nodes, bool):
+            raise ValueError("nodes must be a positive integer")
         if not 0 < low_nm < high_nm:
             raise ValueError("require 0 < low_nm < high_nm")
-        if isinstance(nodes, bool) or not isinstance(nodes, (int, np.integer)) or nodes < 1:
-            raise ValueError("nodes must be a positive integer")
-        x, w = leggauss(int(nodes))
+        x, w = np.polynomial.legendre.leggauss(count)
         half = (high_nm - low_nm) / 2
         return cls(low_nm + half * (x + 1), half * w)
 
 
-@dataclass(frozen=True)
 class SpectralMedium:
-    """Optical coefficients tabulated on wavelength nodes, linear in between (``model.py:46-91``)."""
-    wavelength_nm: np.ndarray
-    absorption_per_m: np.ndarray
-    scattering_per_m: np.ndarray
-    phase_index: np.ndarray
-    group_index: np.ndarray
-    g: float = 0.9
-    provenance: str = "user-specified spectral medium"
+    """Water tabulated on wavelength nodes (absorption, scattering, phase and
+    group index), linear between nodes, with the HG asymmetry ``g``."""
 
-    def __post_init__(self):
-        wavelength = _axis(self.wavelength_nm, "wavelength_nm", positive=True)
-        if len(wavelength) < 2 or np.any(np.diff(wavelength) <= 0):
-            raise ValueError("wavelength_nm must contain at least two increasing nodes")
-        object.__setattr__(self, "wavelength_nm", wavelength)
-        for name in ("absorption_per_m", "scattering_per_m", "phase_index", "group_index"):
-            value = _axis(getattr(self, name), name, positive=name != "scattering_per_m")
-            if value.shape != wavelength.shape or (name == "scattering_per_m" and np.any(value < 0)):
-                raise ValueError(f"{name} must match wavelength_nm")
-            object.__setattr__(self, name, value)
-        if not np.isfinite(self.g) or not -1 < self.g < 1:
-            raise ValueError("HG requires finite -1 < g < 1")
+    def __init__(self, wavelength_nm, absorption_per_m, scattering_per_m, phase_index,
+                 group_index, g=0.9, provenance="user-specified spectral medium"):
+        nodes = _finite(wavelength_nm, "wavelength_nm", positive=True).ravel()
+        if len(nodes) < 2 or not np.all(np.diff(nodes) > 0):
+            raise ValueError("wavelength_nm needs two or more increasing nodes")
+        table = np.vstack([_finite(values, name, shape=nodes.shape,
+                                   nonnegative=name == "scattering_per_m",
+                                   positive=name != "scattering_per_m")
+                           for name, values in zip(_MEDIUM_COLUMNS, (
+                               absorption_per_m, scattering_per_m, phase_index, group_index))])
+        if not (np.isfinite(g) and abs(g) < 1):
+            raise ValueError("the HG asymmetry g must lie in (-1, 1)")
+        self.wavelength_nm = nodes
+        self._table = table
+        self.absorption_per_m, self.scattering_per_m, self.phase_index, self.group_index = table
+        self.g = float(g)
+        self.provenance = provenance
 
     @property
     def wavelength_range_nm(self):
         return float(self.wavelength_nm[0]), float(self.wavelength_nm[-1])
 
     def sample(self, wavelength_nm):
-        wavelength = np.asarray(wavelength_nm, float)
+        """Each column linearly interpolated at ``wavelength_nm`` (inside the table)."""
+        at = np.asarray(wavelength_nm, float)
         low, high = self.wavelength_range_nm
-        if not np.isfinite(wavelength).all() or np.any((wavelength < low) | (wavelength > high)):
+        if not np.all(np.isfinite(at)) or np.any(at < low) or np.any(at > high):
             raise ValueError(f"wavelength outside medium range [{low:g}, {high:g}] nm")
-
-        def interp(values):
-            return np.interp(wavelength, self.wavelength_nm, values)
-        return {"absorption_per_m": interp(self.absorption_per_m),
-                "scattering_per_m": interp(self.scattering_per_m),
-                "phase_index": interp(self.phase_index),
-                "group_index": interp(self.group_index)}
+        return {name: np.interp(at, self.wavelength_nm, row)
+                for name, row in zip(_MEDIUM_COLUMNS, self._table)}
 
 
-@dataclass(frozen=True)
 class SphereSurface:
-    """Spherical glass of one radius or a radius per OM (``receiver.py:6-26``)."""
-    radius_m: np.ndarray | float
+    """Glass sphere of every OM: one radius, or one per OM."""
 
-    def __post_init__(self):
-        radius = np.asarray(self.radius_m, float)
-        if radius.ndim > 1 or not np.isfinite(radius).all() or np.any(radius <= 0):
-            raise ValueError('surface radius_m must be finite and positive')
-        object.__setattr__(self, 'radius_m', radius.copy())
+    def __init__(self, radius_m):
+        radius = _finite(radius_m, "surface radius_m", positive=True)
+        if radius.ndim > 1:
+            raise ValueError("surface radius_m must be a scalar or one value per OM")
+        self.radius_m = radius
 
     def radii(self, count):
         return np.broadcast_to(self.radius_m, (count,))
 
     def subset(self, indices):
-        return self if self.radius_m.ndim == 0 else SphereSurface(self.radius_m[indices])
+        if self.radius_m.ndim == 0:
+            return self
+        return SphereSurface(self.radius_m[np.asarray(indices)])
 
 
-@dataclass(frozen=True)
 class DetectorArray:
-    """OMs with a common angular shape and spectral response (``model.py:94-171``).
+    """OMs: centres, orientations (towards the accepted hemisphere), effective
+    areas, a common angular and spectral response and optional glass spheres."""
 
-    ``orientations`` point towards the accepted hemisphere; the look direction
-    of the transport is ``-orientation``.
-    """
-    positions_m: np.ndarray
-    orientations: np.ndarray
-    effective_area_m2: np.ndarray | float
-    angular_acceptance: Callable[[np.ndarray], np.ndarray]
-    spectral_efficiency: Callable[[np.ndarray], np.ndarray]
-    identifiers: dict = field(default_factory=dict)
-    provenance: str = "user-specified detector"
-    receiver_surface: SphereSurface | None = None
-
-    def __post_init__(self):
-        positions = np.asarray(self.positions_m, float)
-        orientations = np.asarray(self.orientations, float)
-        if (positions.ndim != 2 or positions.shape[1] != 3 or not len(positions)
-                or not np.isfinite(positions).all() or orientations.shape != positions.shape
-                or not np.isfinite(orientations).all()):
-            raise ValueError("positions_m and orientations must be finite (N,3) arrays")
-        norm = np.linalg.norm(orientations, axis=1)
-        if np.any(norm <= 0):
+    def __init__(self, positions_m, orientations, effective_area_m2, angular_acceptance,
+                 spectral_efficiency, identifiers=None, provenance="user-specified detector",
+                 receiver_surface=None):
+        positions = _finite(positions_m, "positions_m")
+        if positions.ndim != 2 or positions.shape[1] != 3:
+            raise ValueError("positions_m must be an (N, 3) array")
+        count = len(positions)
+        axis = _finite(orientations, "orientations", shape=positions.shape)
+        length = np.linalg.norm(axis, axis=1, keepdims=True)
+        if np.any(length <= 0):
             raise ValueError("detector orientations must be nonzero")
-        orientations = orientations / norm[:, None]
-        area = np.broadcast_to(np.asarray(self.effective_area_m2, float), (len(positions),)).copy()
-        if not np.isfinite(area).all() or np.any(area <= 0):
-            raise ValueError("effective_area_m2 must be finite and positive")
-        if self.receiver_surface is not None:
-            if not isinstance(self.receiver_surface, SphereSurface):
-                raise ValueError('unsupported receiver_surface geometry')
-            self.receiver_surface.radii(len(positions))
-        for name, value in self.identifiers.items():
-            if np.asarray(value).shape != (len(positions),):
-                raise ValueError(f"identifier {name!r} must contain one value per OM")
-        probe = np.linspace(-1, 1, 17)
-        angular = np.asarray(self.angular_acceptance(probe), float)
-        spectral = np.asarray(self.spectral_efficiency(np.array([350., 450., 550.])), float)
-        if (angular.shape != probe.shape or not np.isfinite(angular).all()
-                or np.any(angular < 0) or spectral.shape != (3,)
-                or not np.isfinite(spectral).all() or np.any(spectral < 0)):
-            raise ValueError("detector response functions must return finite nonnegative arrays")
-        object.__setattr__(self, "positions_m", positions)
-        object.__setattr__(self, "orientations", orientations)
-        object.__setattr__(self, "effective_area_m2", area)
+        area = _finite(np.broadcast_to(np.asarray(effective_area_m2, float), (count,)),
+                       "effective_area_m2", positive=True)
+        labels = {key: np.asarray(value) for key, value in (identifiers or {}).items()}
+        if any(value.shape != (count,) for value in labels.values()):
+            raise ValueError("identifiers need one value per OM")
+        if receiver_surface is not None:
+            receiver_surface.radii(count)
+        self._set(positions, axis / length, area, angular_acceptance, spectral_efficiency,
+                  labels, provenance, receiver_surface)
+
+    def _set(self, positions, orientations, area, angular, spectral, labels, provenance,
+             surface):
+        self.positions_m, self.orientations, self.effective_area_m2 = positions, orientations, area
+        self.angular_acceptance, self.spectral_efficiency = angular, spectral
+        self.identifiers, self.provenance, self.receiver_surface = labels, provenance, surface
 
     def __len__(self):
         return len(self.positions_m)
 
     def subset(self, indices):
-        indices = np.asarray(indices)
-        return replace(self, positions_m=self.positions_m[indices],
-                       orientations=self.orientations[indices],
-                       effective_area_m2=self.effective_area_m2[indices],
-                       identifiers={k: np.asarray(v)[indices] for k, v in self.identifiers.items()},
-                       receiver_surface=(None if self.receiver_surface is None else
-                                         self.receiver_surface.subset(indices)))
+        """The OMs ``indices`` with their geometry, areas, labels and glass."""
+        pick = np.asarray(indices)
+        other = DetectorArray.__new__(DetectorArray)
+        other._set(self.positions_m[pick], self.orientations[pick], self.effective_area_m2[pick],
+                   self.angular_acceptance, self.spectral_efficiency,
+                   {key: value[pick] for key, value in self.identifiers.items()},
+                   self.provenance,
+                   None if self.receiver_surface is None else self.receiver_surface.subset(pick))
+        return other
 
     def spectral_weight(self, wavelength_nm):
+        """Spectral efficiency at ``wavelength_nm`` (finite and nonnegative)."""
         value = np.asarray(self.spectral_efficiency(np.asarray(wavelength_nm, float)), float)
-        if not np.isfinite(value).all() or np.any(value < 0):
+        if not np.all(np.isfinite(value)) or np.any(value < 0):
             raise ValueError("nonfinite or negative detector spectral response")
         return value
 
-    @property
-    def looks(self):
-        """Look directions of the transport (``-orientation``)."""
-        return np.ascontiguousarray(-self.orientations)
-
-
-def acceptance_coefficients(acceptance, degree, *, quadrature_order=2048):
-    """``alpha_l = 2 pi int A(x) P_l(x) dx`` (``cache.py:81-99``)."""
-    from scipy.special import eval_legendre, roots_legendre
-    x, w = roots_legendre(int(quadrature_order))
-    values = np.asarray(acceptance(x), dtype=float)
-    if values.shape != x.shape or not np.isfinite(values).all():
-        raise ValueError("acceptance must map an array of cosines to finite values")
-    if np.any(values < 0):
-        raise ValueError("acceptance must be nonnegative")
-    ell = np.arange(int(degree) + 1)
-    return 2 * np.pi * (eval_legendre(ell[:, None], x[None, :]) * (w * values)).sum(axis=1)
-
 
 def acceptance_alpha(detector, *, scattering_degree=8, acceptance_degree=None, tolerance=1e-10):
-    """Band-limited acceptance coefficients (``transport.py:496-530``, ``directional.py:107-130``)."""
+    """Legendre coefficients ``alpha_l = 2 pi int_{-1}^{1} A(x) P_l(x) dx`` of the OM
+    acceptance, truncated at its band limit (``TransportKernel.acceptance``,
+    ``transport.py:496-530``): the largest degree whose coefficient exceeds
+    ``tolerance`` of the largest, probed up to max(2 scattering_degree, 64) on a
+    2048-point Gauss rule; ``acceptance_degree`` forces the degree; a band limit
+    above ``scattering_degree`` is an error."""
     probe = max(2 * int(scattering_degree), 64)
-    alpha = acceptance_coefficients(detector.angular_acceptance, probe)
-    peak = float(np.max(np.abs(alpha)))
+    x, w = np.polynomial.legendre.leggauss(2048)
+    response = np.asarray(detector.angular_acceptance(x), float)
+    if response.shape != x.shape or not np.all(np.isfinite(response)):
+        raise ValueError("acceptance must map an array of cosines to finite values")
+    if np.any(response < 0):
+        raise ValueError("acceptance must be nonnegative")
+    alpha = 2 * np.pi * (np.polynomial.legendre.legvander(x, probe).T @ (w * response))
+    peak = float(np.abs(alpha).max())
     if peak <= 0:
         raise ValueError("acceptance is identically zero")
-    significant = np.flatnonzero(np.abs(alpha) > tolerance * peak)
-    bandwidth = int(significant[-1]) if len(significant) else 0
-    if bandwidth >= probe and acceptance_degree is None:
-        raise ValueError("the module acceptance is not band-limited")
+    above = np.abs(alpha) > tolerance * peak
+    limit = int(np.nonzero(above)[0].max())
     if acceptance_degree is not None:
-        bandwidth = int(acceptance_degree)
-    if bandwidth > scattering_degree:
-        raise ValueError(f"module acceptance needs degree {bandwidth}, above "
+        limit = int(acceptance_degree)
+    elif limit >= probe:
+        raise ValueError("the module acceptance is not band-limited: its Legendre coefficients "
+                         f"are still above the tolerance at degree {probe}")
+    if limit > scattering_degree:
+        raise ValueError(f"module acceptance needs degree {limit}, above "
                          f"scattering_degree={scattering_degree}")
-    return alpha[:bandwidth + 1].copy()
+    return alpha[:limit + 1].copy()
+
+
+def kernel_alpha(kernel):
+    """Acceptance coefficients of a route kernel (``kernel.acceptance()["alpha"]``)."""
+    return np.ascontiguousarray(kernel.acceptance()["alpha"], float)

@@ -19,6 +19,7 @@ import ctypes
 import json
 import threading
 import warnings
+from collections import OrderedDict
 from dataclasses import asdict, dataclass, field, replace
 from pathlib import Path
 from time import perf_counter, process_time
@@ -27,7 +28,7 @@ import numpy as np
 
 from . import _lib
 from .axial import AxisFrame, column_bounds
-from .model import WavelengthQuadrature, acceptance_alpha
+from .model import WavelengthQuadrature, acceptance_alpha, glass_radii, look_directions
 from .order1 import PromptSettings
 
 C_M_PER_NS = 0.299792458
@@ -94,12 +95,48 @@ class RouteKernel:
 
     def subset(self, receivers, threshold_pe=None):
         """The kernel on OMs ``receivers`` (``muon/universal_primary.py`` ``subset_kernel``)."""
-        other = RouteKernel(self.medium, self.detector.subset(np.asarray(receivers)),
-                            self.config if threshold_pe is None
-                            else replace(self.config, threshold_pe=float(threshold_pe)))
-        other.wavelength = self.wavelength
-        other._acceptance = self._acceptance
-        return other
+        return kernel_view(self, receivers=receivers, threshold_pe=threshold_pe)
+
+
+def kernel_view(kernel, *, receivers=None, threshold_pe=None, config=None):
+    """A :class:`RouteKernel` sharing ``kernel``'s medium, wavelength quadrature and
+    acceptance, on the OMs ``receivers`` (default all), with ``threshold_pe`` (and no
+    detail threshold) or a whole ``config`` replaced. ``kernel`` may be a
+    :class:`RouteKernel` or the reference's ``TransportKernel`` (the attributes
+    ``medium``, ``detector``, ``config``, ``wavelength`` and ``acceptance()``)."""
+    view = RouteKernel.__new__(RouteKernel)
+    view.medium = kernel.medium
+    view.detector = (kernel.detector if receivers is None
+                     else kernel.detector.subset(np.asarray(receivers)))
+    view.config = kernel.config if config is None else config
+    if threshold_pe is not None:
+        view.config = replace(view.config, threshold_pe=float(threshold_pe),
+                              detail_threshold_pe=None)
+    view.wavelength = kernel.wavelength
+    view._acceptance = kernel.acceptance()
+    return view
+
+
+def shower_kernel(kernel, data):
+    """The kernel with the route's frequency grids (``shower/api.py:92-112``): 65 table
+    frequencies and 321 readout frequencies over 0..1.2 /ns, with the data's
+    ``kernel_settings`` applied. The given kernel is returned when it already has
+    them; otherwise a view keeps its medium, detector, bins and threshold."""
+    config = kernel.config
+    settings = dict(getattr(data, "kernel_settings", None) or {})
+    if settings:
+        config = replace(config, **settings)
+    readout = getattr(config, "readout_omega_per_ns", None)
+    same = (np.array_equal(np.asarray(config.omega_per_ns), FREQUENCIES_PER_NS)
+            and readout is not None and np.array_equal(np.asarray(readout), READOUT_PER_NS))
+    if same and not settings:
+        return kernel
+    forced = dict(omega_per_ns=FREQUENCIES_PER_NS.copy(),
+                  readout_omega_per_ns=READOUT_PER_NS.copy())
+    rte_cache = getattr(data, "rte_cache", None)
+    if hasattr(config, "cache_policy") and rte_cache is not None:
+        forced.update(cache_directory=str(rte_cache), cache_policy="require")
+    return kernel_view(kernel, config=replace(config, **forced))
 
 
 @dataclass(frozen=True)
@@ -518,7 +555,7 @@ def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra
                                                                           float)
     if extra.shape != (n,) or not np.isfinite(extra).all():
         raise ValueError("selection_extra_charge must be finite per OM")
-    looks = detector.looks
+    looks = look_directions(detector)
     areas = np.asarray(detector.effective_area_m2, float)
     edges = np.asarray(kernel.config.relative_time_edges_ns, float)
     threshold = float(kernel.config.threshold_pe)
@@ -612,38 +649,85 @@ def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra
 
 # -------------------------------------------------------------------- route
 
+MANIFEST = "shower-data.json"
+ENVIRONMENT = "LIGHTHIT_SHOWER_DATA"
+
+
 @dataclass(frozen=True)
 class ShowerData:
-    """Tables of the route (``shower/data.py:26-82``): the order >=2 pack directory (or
-    an in-memory :class:`KernelPack`) and an optional prepared fold."""
+    """Tables of the route (``shower/data.py:26-100``): the RTE cache directory, the
+    order >=2 pack directory (or an in-memory :class:`KernelPack`), an optional
+    prepared fold, the manifest and its ``kernel_settings``. The route reads the
+    pack (and the fold); ``rte_cache`` is carried for the kernel's configuration
+    only, since the RTE tables enter the route through the pack."""
+    rte_cache: object
     ge2_pack: object
     ge2_band_fold: object = None
     manifest: Path | None = None
+    kernel_settings: dict = field(default_factory=dict)
 
     @classmethod
-    def of(cls, ge2_pack, ge2_band_fold=None):
-        return cls(ge2_pack, ge2_band_fold)
+    def of(cls, rte_cache, ge2_pack, ge2_band_fold=None):
+        """Tables given directly; paths must exist (``rte_cache`` may be None)."""
+        def resolve(item):
+            return item if item is None or not isinstance(item, (str, Path)) else \
+                Path(item).expanduser().resolve()
+        cache, pack = resolve(rte_cache), resolve(ge2_pack)
+        absent = [str(item) for item in (cache, pack) if isinstance(item, Path)
+                  and not item.exists()]
+        if absent:
+            raise FileNotFoundError("shower data incomplete: " + ", ".join(absent))
+        return cls(cache, pack, resolve(ge2_band_fold))
 
     @classmethod
     def load(cls, path):
-        """Read ``shower-data.json`` (``ge2_pack``, optional ``ge2_band_fold``)."""
+        """Read ``shower-data.json`` (or ``muon-data.json``) or a directory holding one."""
         path = Path(path).expanduser()
         if path.is_dir():
-            path = path / "shower-data.json"
+            path = path / MANIFEST if (path / MANIFEST).is_file() else path / "muon-data.json"
         entries = json.loads(path.read_text())
         if "ge2_pack" not in entries:
             raise ValueError(f"{path}: missing entries ['ge2_pack']")
         base = path.parent
-        fold = entries.get("ge2_band_fold")
-        return cls((base / entries["ge2_pack"]).resolve(),
-                   None if fold is None else (base / fold).resolve(), path.resolve())
+        cache, fold = entries.get("rte_cache"), entries.get("ge2_band_fold")
+        data = cls(None if cache is None else (base / cache).resolve(),
+                   (base / entries["ge2_pack"]).resolve(),
+                   None if fold is None else (base / fold).resolve(), path.resolve(),
+                   dict(entries.get("kernel_settings") or {}))
+        absent = [str(item) for item in (data.rte_cache, data.ge2_pack)
+                  if item is not None and not item.exists()]
+        if absent:
+            raise FileNotFoundError("shower data incomplete: " + ", ".join(absent))
+        return data
+
+    @classmethod
+    def find(cls, cache_directory=None):
+        """``$LIGHTHIT_SHOWER_DATA``, then a shower (or muon) manifest in or next to the
+        RTE cache directory (``shower/data.py:84-100``)."""
+        import os
+        places = []
+        if os.environ.get(ENVIRONMENT):
+            places.append(Path(os.environ[ENVIRONMENT]))
+        if cache_directory is not None:
+            cache = Path(cache_directory).expanduser()
+            for name in (MANIFEST, "muon-data.json"):
+                places += [cache / name, cache.parent / name]
+        for place in places:
+            if place.is_file() or (place.is_dir() and ((place / MANIFEST).is_file()
+                                                       or (place / "muon-data.json").is_file())):
+                return cls.load(place)
+        searched = ", ".join(str(item) for item in places) or "nothing (no cache directory)"
+        raise FileNotFoundError(f"no {MANIFEST} or muon-data.json found; searched {searched}")
 
 
 class ShowerTransport:
     """Transport for shower events on one detector (``shower/api.py:115-220``).
 
-    ``data`` is a :class:`ShowerData`, a manifest path, or a
-    :class:`lighthit_amd.fold.KernelPack`. The pack is folded onto the node
+    ``kernel`` is a :class:`RouteKernel` or the reference's ``TransportKernel``;
+    ``data`` is a :class:`ShowerData`, a manifest path, a
+    :class:`lighthit_amd.fold.KernelPack`, or None to search next to the
+    kernel's cache (:meth:`ShowerData.find`). The kernel is given the route's
+    frequency grids (:func:`shower_kernel`). The pack is folded onto the node
     bands once (on the GPU) and stays resident with the order-0 optics and
     the readout basis.
     """
@@ -651,13 +735,13 @@ class ShowerTransport:
     def __init__(self, kernel, data=None, *, config=None, threads=None, device="cuda"):
         from .fold import KernelPack
         if data is None:
-            raise ValueError("the shower route needs its order >=2 pack (ShowerData)")
-        if isinstance(data, KernelPack):
-            data = ShowerData(data)
+            data = ShowerData.find(getattr(kernel.config, "cache_directory", None))
+        elif isinstance(data, KernelPack):
+            data = ShowerData(None, data)
         elif not isinstance(data, ShowerData):
             data = ShowerData.load(data)
         self.data = data
-        self.kernel = kernel
+        self.kernel = shower_kernel(kernel, data)
         self.config = config or ShowerConfig()
         self.threads = threads
         self.device = device
@@ -665,7 +749,7 @@ class ShowerTransport:
         self._local = threading.local()   # per-thread cache of the last compile
         self._lock = threading.RLock()     # guards the lazily built resident parts
         self._order0 = None
-        self._readout = {}
+        self._readout = OrderedDict()
         self._window = None
 
     @property
@@ -745,32 +829,29 @@ class ShowerTransport:
                 self._window = (kernel.detector, WindowGeometry(kernel, self.device))
             return self._window[1]
 
+    #: readout bases a route keeps (the prompt grid and recent delayed-light shifts)
+    READOUT_CACHE = 8
+
     def readout(self, omega, edges):
-        from .readout import ReadoutBasis
-        key = (tuple(np.round(edges, 12)), len(omega))
-        with self._lock:
-            if key not in self._readout:
-                self._readout[key] = ReadoutBasis(omega, edges,
-                                                  self.kernel.config.readout_omega_per_ns,
-                                                  device=self.device)
-            return self._readout[key]
+        """The readout basis of (omega, edges, readout grid), kept per route in a
+        small least-recently-used cache (delayed light reads on shifted edges)."""
+        return _cached_readout(self._readout, self._lock, self.kernel, omega, edges,
+                               self.device, self.READOUT_CACHE)
 
     def __call__(self, source, *, progress=None, time_origin_ns=None, selection_extra_charge=None,
                  receivers=None):
         """Response for one event (``HybridTransportResponse``)."""
+        # detail threshold (transport.py:210-254): every selection decision at the
+        # lowered threshold, ``active`` at the original one -- also for a receiver
+        # subset, whose own threshold is zero
         kernel = self.kernel
-        detail = kernel.config.detail_threshold_pe
-        lowered = (detail is not None and receivers is None
-                   and detail < kernel.config.threshold_pe)
+        detail = getattr(kernel.config, "detail_threshold_pe", None)
+        lowered = detail is not None and detail < kernel.config.threshold_pe
         if lowered:
-            kernel = RouteKernel(kernel.medium, kernel.detector,
-                                 replace(kernel.config, threshold_pe=float(detail),
-                                         detail_threshold_pe=None))
-            kernel.wavelength = self.kernel.wavelength
-            kernel._acceptance = self.kernel._acceptance
+            kernel = kernel_view(kernel, threshold_pe=float(detail))
         order0 = self.order0()
         if receivers is not None:
-            kernel = kernel.subset(receivers, threshold_pe=0.0)
+            kernel = kernel_view(kernel, receivers=receivers, threshold_pe=0.0)
             order0 = None
         response = transport_shower(kernel, source, config=self.config, tables=self.tables(),
                                     time_origin_ns=time_origin_ns,
@@ -845,7 +926,7 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         elements = elements.subset(valid)
     efficiency = np.asarray(kernel.detector.spectral_weight(wavelengths), float)
     positions = np.asarray(kernel.detector.positions_m, float)
-    looks = kernel.detector.looks
+    looks = look_directions(kernel.detector)
     area = np.asarray(kernel.detector.effective_area_m2, float)
     alpha = np.ascontiguousarray(kernel.acceptance()["alpha"], float)
     low, cache_high = kernel.config.radial_range_m
@@ -1024,6 +1105,8 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         late_spectra.append((shift, values))
     timings["full_apply_seconds"] = perf_counter() - stage
     stage = perf_counter()
+    if fallback and not np.all(prompt.computed[active]):
+        raise ValueError("prompt did not compute every active OM")
     spectrum[0] = charge[:, 2] - late_charge
     components = np.zeros((n, len(edges) - 1, 3))
     if np.any(active):
@@ -1044,7 +1127,8 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
         components[ids, :, 2] = readout.apply(device_spectrum, device_origins,
                                               precision=readout_precision).double().cpu().numpy()
         for shift, values in late_spectra:
-            basis_shift = _readout_basis(kernel, omega, edges - shift, device)
+            basis_shift = (route.readout(omega, edges - shift) if route is not None else
+                           _readout_basis(kernel, omega, edges - shift, device))
             late = torch.as_tensor(np.ascontiguousarray(values[:, ids], np.complex64),
                                    device=device)
             components[ids, :, 2] += basis_shift.apply(
@@ -1118,13 +1202,35 @@ def _near_pack(tables, kernel, config, device):
     return DevicePack(bundle, device)
 
 
-_readout_cache = {}
+def _cached_readout(cache, lock, kernel, omega, edges, device, limit):
+    """A ReadoutBasis from ``cache`` (an OrderedDict guarded by ``lock``), keyed on the
+    exact bytes of the frequency grid, the readout grid and the bin edges."""
+    from .readout import ReadoutBasis
+    omega = np.ascontiguousarray(omega, float)
+    edges = np.ascontiguousarray(edges, float)
+    readout = getattr(kernel.config, "readout_omega_per_ns", None)
+    readout = None if readout is None else np.ascontiguousarray(readout, float)
+    key = (omega.tobytes(), edges.tobytes(), None if readout is None else readout.tobytes(),
+           str(device))
+    with lock:
+        basis = cache.get(key)
+        if basis is not None:
+            cache.move_to_end(key)
+            return basis
+    basis = ReadoutBasis(omega, edges, readout, device=device)
+    with lock:
+        cache[key] = basis
+        cache.move_to_end(key)
+        while len(cache) > limit:
+            cache.popitem(last=False)
+    return basis
+
+
+_route_free_readouts = OrderedDict()
+_route_free_lock = threading.Lock()
 
 
 def _readout_basis(kernel, omega, edges, device):
-    from .readout import ReadoutBasis
-    key = (tuple(np.round(edges, 12)), len(omega), str(device))
-    if key not in _readout_cache:
-        _readout_cache[key] = ReadoutBasis(omega, edges, kernel.config.readout_omega_per_ns,
-                                           device=device)
-    return _readout_cache[key]
+    """Readout basis for calls without a route object (small shared LRU cache)."""
+    return _cached_readout(_route_free_readouts, _route_free_lock, kernel, omega, edges, device,
+                           ShowerTransport.READOUT_CACHE)

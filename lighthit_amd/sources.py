@@ -1,10 +1,11 @@
 """Charged-step sources: the two-field Cherenkov elements and synthetic showers.
 
-Mirrors ``SpectralLightElements``, ``SourcePose`` and ``SyntheticShower``
-(``sources.py:12-58, 60-78, 110-262, 396-441``): the same fields,
-validation, placement (photon-weighted centroid and principal axis moved onto
-a pose) and the same seeded generator, so a given seed gives bit-identical
-elements.
+Stand-ins for ``SpectralLightElements``, ``SourcePose`` and ``SyntheticShower``
+(``sources.py:12-58, 60-78, 110-262, 396-441``) with the same fields, for
+code without a reference installation; the route takes the reference's own
+element objects as well. The placement (photon-weighted centroid and
+principal axis moved onto a pose) and the seeded generator follow the
+reference's steps, so a given seed gives bit-identical elements.
 """
 from __future__ import annotations
 
@@ -17,14 +18,17 @@ FINE_STRUCTURE = 7.2973525693e-3
 C_M_PER_NS = 0.299792458
 
 
-def _unit_vector(value, name):
-    vector = np.asarray(value, float)
-    if vector.shape != (3,) or not np.isfinite(vector).all():
-        raise ValueError(f"{name} must be a finite 3-vector")
-    norm = np.linalg.norm(vector)
-    if norm <= 0:
-        raise ValueError(f"{name} must be nonzero")
-    return vector / norm
+def _vector3(value, what, *, unit=False):
+    """A finite 3-vector (scaled to unit length with ``unit``)."""
+    vector = np.array(value, dtype=float)
+    if vector.shape != (3,) or not np.all(np.isfinite(vector)):
+        raise ValueError(f"{what} must be a finite 3-vector")
+    if not unit:
+        return vector
+    length = np.linalg.norm(vector)
+    if not length > 0:
+        raise ValueError(f"{what} must be nonzero")
+    return vector / length
 
 
 def _principal_axis(elements):
@@ -45,8 +49,8 @@ def _principal_axis(elements):
 
 
 def _rotation_between(source, target):
-    source = _unit_vector(source, "source direction")
-    target = _unit_vector(target, "target direction")
+    source = _vector3(source, "source direction", unit=True)
+    target = _vector3(target, "target direction", unit=True)
     cosine = float(np.clip(source @ target, -1.0, 1.0))
     if np.isclose(cosine, 1.0, atol=1e-14):
         return np.eye(3)
@@ -73,14 +77,10 @@ class SourcePose:
     time_ns: float = 0.0
 
     def __post_init__(self):
-        position = np.asarray(self.position_m, float).copy()
-        if position.shape != (3,) or not np.isfinite(position).all():
-            raise ValueError("position_m must be a finite 3-vector")
         if not np.isfinite(self.time_ns):
             raise ValueError("time_ns must be finite")
-        direction = _unit_vector(self.direction, "direction").copy()
-        object.__setattr__(self, "position_m", position)
-        object.__setattr__(self, "direction", direction)
+        object.__setattr__(self, "position_m", _vector3(self.position_m, "position_m"))
+        object.__setattr__(self, "direction", _vector3(self.direction, "direction", unit=True))
 
 
 @dataclass(frozen=True)
@@ -104,6 +104,10 @@ class LightField:
         return self.start_m + 0.5 * self.length_m[:, None] * self.direction
 
 
+_PER_ELEMENT = ("length_m", "coefficient0", "coefficient2", "beta", "start_ns", "end_ns",
+                "row_index", "uid")
+
+
 @dataclass(frozen=True)
 class SpectralLightElements:
     """Straight emitting chords with ``dN/dlambda = lambda^-2 (C0 + C2 / n^2)``."""
@@ -122,29 +126,30 @@ class SpectralLightElements:
     cone_model: str = "frozen"
 
     def __post_init__(self):
-        start = np.asarray(self.start_m, float)
-        direction = np.asarray(self.direction, float)
-        count = len(start)
-        if start.shape != (count, 3) or direction.shape != start.shape or not count:
+        # what the route relies on: (N, 3) geometry, one finite value per
+        # element, positive lengths and beta, C0 >= 0 >= C2
+        rows = np.array(self.start_m, dtype=float)
+        if rows.ndim != 2 or rows.shape[1:] != (3,) or len(rows) == 0:
             raise ValueError("start_m and direction must be nonempty (N,3) arrays")
-        arrays = {}
-        for name in ("length_m", "coefficient0", "coefficient2", "beta",
-                     "start_ns", "end_ns", "row_index", "uid"):
-            value = np.asarray(getattr(self, name))
-            if value.shape != (count,) or not np.isfinite(value.astype(float)).all():
+        axes = np.array(self.direction, dtype=float)
+        if axes.shape != rows.shape:
+            raise ValueError("start_m and direction must be nonempty (N,3) arrays")
+        columns = {name: np.asarray(getattr(self, name)) for name in _PER_ELEMENT}
+        for name, column in columns.items():
+            if column.shape != (len(rows),) or not np.all(np.isfinite(column.astype(float))):
                 raise ValueError(f"{name} must contain one finite value per element")
-            arrays[name] = value
-        norm = np.linalg.norm(direction, axis=1)
-        if (not np.isfinite(start).all() or np.any(norm <= 0)
-                or np.any(arrays["length_m"] <= 0) or np.any(arrays["coefficient0"] < 0)
-                or np.any(arrays["coefficient2"] > 0) or np.any(arrays["beta"] <= 0)):
+        length = np.linalg.norm(axes, axis=1)
+        signs_ok = (np.all(columns["length_m"] > 0) and np.all(columns["beta"] > 0)
+                    and np.all(columns["coefficient0"] >= 0)
+                    and np.all(columns["coefficient2"] <= 0))
+        if not (np.all(np.isfinite(rows)) and np.all(length > 0) and signs_ok):
             raise ValueError("invalid spectral element geometry or coefficients")
         if self.cone_model not in ("frozen", "spectral"):
             raise ValueError("cone_model must be frozen or spectral")
-        object.__setattr__(self, "start_m", start)
-        object.__setattr__(self, "direction", direction / norm[:, None])
-        for name, value in arrays.items():
-            object.__setattr__(self, name, value)
+        object.__setattr__(self, "start_m", rows)
+        object.__setattr__(self, "direction", axes / length[:, None])
+        for name, column in columns.items():
+            object.__setattr__(self, name, column)
 
     def __len__(self):
         return len(self.length_m)

@@ -105,14 +105,14 @@ def compile_axial_slices(field, degree, omega_per_ns, *, azimuthal_degree=4, cel
                          stencil_threshold_m=0.0, time_step_ns=0.02, time_deposit="linear",
                          element_order="auto", frame=None):
     """Axial-slice source of one light field (``_shower_basis.py:188-645``)."""
-    from lighthit_amd.axial import AxialSource, AxisFrame
+    from .host import AxialSource, axis_frame
     if not (cell_m > 0 and time_step_ns > 0 and stencil_m > 0 and stencil_threshold_m >= 0):
         raise ValueError("cell_m, time_step_ns and stencil_m must be positive")
     if axial_order not in (1, 3) or transverse not in ("collapse", "stencil"):
         raise ValueError("axial_order must be 1 or 3 and transverse collapse or stencil")
     if time_deposit not in ("linear", "nearest"):
         raise ValueError("the oracle restates the linear and nearest time lattices")
-    frame = frame or AxisFrame.of(field)
+    frame = frame or axis_frame(field)
     omega = np.atleast_1d(np.asarray(omega_per_ns, float))
     charge_only = len(omega) == 1 and omega[0] == 0
     photons = np.asarray(field.photons, float)

@@ -33,3 +33,45 @@ def near_event(om=20, elements=400, impact=(1.0, 0.3, 0.0), seed=5):
 def demonstration_route_only(device="cuda"):
     """The demonstration route alone (a picklable factory for worker processes)."""
     return demonstration_route(device)[3]
+
+
+def _joined(first, second):
+    from dataclasses import replace
+    names = ("start_m", "direction", "length_m", "coefficient0", "coefficient2", "beta",
+             "start_ns", "end_ns", "row_index", "uid")
+    return replace(first, **{name: np.concatenate([getattr(first, name), getattr(second, name)])
+                             for name in names})
+
+
+def small_shower(elements=300, seed=7, offset=(-12.0, 8.0, 0.0),
+                 direction=(0.35, -0.15, 0.925), track_m=300.0):
+    """A demonstration-like shower with fewer elements (spectral cones)."""
+    from dataclasses import replace
+
+    from lighthit_amd.sources import SourcePose, SyntheticShower
+    from lighthit_amd.synthetic import plain_small
+    w = plain_small()
+    centre = w.detector.positions_m.mean(axis=0)
+    pose = SourcePose(centre + np.asarray(offset, float), np.asarray(direction, float))
+    shower = SyntheticShower.gaussian(pose, charged_track_length_m=track_m, elements=elements,
+                                      seed=seed)
+    return replace(shower.elements, cone_model="spectral")
+
+
+def delayed_pair(elements=300, delay_ns=250.0):
+    """A shower and the same shower with a copy emitted ``delay_ns`` later (the
+    reference's delayed-light invariant, tests/test_shower_route.py:200-228)."""
+    from dataclasses import replace
+    first = small_shower(elements)
+    late = replace(first, start_ns=first.start_ns + delay_ns, end_ns=first.end_ns + delay_ns,
+                   row_index=first.row_index + len(first))
+    return first, _joined(first, late)
+
+
+def long_source(elements=300, separation_m=200.0):
+    """Two showers ``separation_m`` apart along the array's x axis: the source extent
+    exceeds the 150 m radial range, so the route takes per-OM (local) time origins."""
+    a = small_shower(elements, seed=7, offset=(-separation_m / 2, 8.0, 0.0))
+    b = small_shower(elements, seed=8, offset=(separation_m / 2, -8.0, 10.0),
+                     direction=(-0.30, 0.20, 0.93))
+    return _joined(a, b)

@@ -44,7 +44,7 @@ def test_route_with_a_near_pass_matches_the_oracle_route(device):
                          np.arange(65))
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        want = oracle_route.transport_shower(kernel, elements, bundle, route.tables()["fold"],
+        want = oracle_route.transport_shower(kernel, elements, bundle, None,
                                              config=route.config)
     assert len(want["near_oms"]) >= 1
     record = response.metadata["near_field"][0]

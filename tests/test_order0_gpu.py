@@ -18,7 +18,7 @@ def _compare(detector, medium, elements, alpha, floor=1e-6):
     from lighthit_amd.ballistic import Order0
     optics = prepare_optics(medium, detector)
     radii = detector.receiver_surface.radii(len(detector))
-    expected, _ = oracle.first_entry(detector.positions_m, detector.looks,
+    expected, _ = oracle.first_entry(detector.positions_m, (-detector.orientations),
                                      detector.effective_area_m2, radii, elements, optics, alpha,
                                      rel_tol=1e-10)
     got = Order0(medium, detector, alpha).charge(elements)
@@ -84,7 +84,7 @@ def _compare_bins(detector, medium, elements, alpha, ids, origins, edges, tolera
     optics = prepare_optics(medium, detector)
     sub = detector.subset(ids)
     charge, expected, uncertainty = oracle.first_entry(
-        sub.positions_m, sub.looks, sub.effective_area_m2, sub.receiver_surface.radii(len(sub)),
+        sub.positions_m, (-sub.orientations), sub.effective_area_m2, sub.receiver_surface.radii(len(sub)),
         elements, optics, alpha, origins=origins, edges=edges, rel_tol=1e-9, bin_tol=1e-6,
         return_uncertainty=True)
     assert np.all(uncertainty < 0.1 * tolerance * charge)

@@ -23,7 +23,7 @@ def _element(length, beta=1.0, coefficient=1e8):
 def _run(detector, medium, elements, alpha, **kw):
     optics = prepare_optics(medium, detector)
     radii = detector.receiver_surface.radii(len(detector))
-    return order0.first_entry(detector.positions_m, detector.looks, detector.effective_area_m2,
+    return order0.first_entry(detector.positions_m, (-detector.orientations), detector.effective_area_m2,
                               radii, elements, optics, alpha, **kw)
 
 
@@ -145,9 +145,9 @@ def test_cap_and_direct_ray_evaluators_agree_on_demo_event(demo):
     ids = np.array([4, 22, 35])
     sub = d.subset(ids)
     elements = w.elements.subset(np.arange(len(w.elements)) % 4 == 1)
-    cap, _ = order0.first_entry(sub.positions_m, sub.looks, sub.effective_area_m2, radii[ids],
+    cap, _ = order0.first_entry(sub.positions_m, (-sub.orientations), sub.effective_area_m2, radii[ids],
                                 elements, optics, alpha, rel_tol=1e-10)
-    rays, _ = order0.first_entry(sub.positions_m, sub.looks, sub.effective_area_m2, radii[ids],
+    rays, _ = order0.first_entry(sub.positions_m, (-sub.orientations), sub.effective_area_m2, radii[ids],
                                  elements, optics, alpha, rel_tol=1e-9, force_rays=True)
     assert np.all(cap > 0)
     np.testing.assert_allclose(rays, cap, rtol=1e-8)
@@ -167,8 +167,8 @@ def test_splitting_elements_leaves_charge_unchanged(demo):
                     end_ns=np.r_[tmid, e.end_ns], row_index=np.r_[e.row_index, e.row_index],
                     uid=np.r_[e.uid, e.uid])
     radii = d.receiver_surface.radii(len(d))
-    q1, _ = order0.first_entry(d.positions_m, d.looks, d.effective_area_m2, radii, e, optics, alpha)
-    q2, _ = order0.first_entry(d.positions_m, d.looks, d.effective_area_m2, radii, split, optics, alpha)
+    q1, _ = order0.first_entry(d.positions_m, (-d.orientations), d.effective_area_m2, radii, e, optics, alpha)
+    q2, _ = order0.first_entry(d.positions_m, (-d.orientations), d.effective_area_m2, radii, split, optics, alpha)
     np.testing.assert_allclose(q2, q1, rtol=1e-9)
 
 
@@ -176,7 +176,7 @@ def test_bins_hold_the_charge_inside_a_wide_window(demo):
     w, optics, alpha = demo
     d = w.detector.subset(np.array([16]))
     radii = d.receiver_surface.radii(len(d))
-    charge, bins = order0.first_entry(d.positions_m, d.looks, d.effective_area_m2, radii, w.elements,
+    charge, bins = order0.first_entry(d.positions_m, (-d.orientations), d.effective_area_m2, radii, w.elements,
                                       optics, alpha, origins=np.zeros(1),
                                       edges=np.arange(-100., 2000., 5.))
     np.testing.assert_allclose(bins.sum(axis=1), charge, rtol=1e-10)
@@ -189,7 +189,7 @@ def test_leading_model_reproduces_the_published_demonstration_total(demo):
     the oracle restates independently of its first-entry evaluator."""
     w, optics, alpha = demo
     d = w.detector
-    leading = order0.leading_interval_charge(d.positions_m, d.looks, d.effective_area_m2,
+    leading = order0.leading_interval_charge(d.positions_m, (-d.orientations), d.effective_area_m2,
                                              d.receiver_surface.radii(len(d)), w.elements,
                                              optics, alpha)
     assert abs(leading.sum() - 63.74) <= 0.005
@@ -216,7 +216,7 @@ def test_leading_model_axial_limit_is_the_point_receiver_integral():
     optics = prepare_optics(medium, detector)
     optics = dict(optics, phase_nodes=optics["phase_nodes"] + 1e-12 * np.arange(
         len(optics["phase_nodes"])))
-    got = order0.leading_interval_charge(detector.positions_m, detector.looks, .05, 1e-4,
+    got = order0.leading_interval_charge(detector.positions_m, (-detector.orientations), .05, 1e-4,
                                          elements, optics, alpha, rel_tol=1e-10)
     # photons per nm per m: C0/lambda^2 (1 - 1/(beta n)^2); geometry 1/(2 pi rho sin)
     lam = np.linspace(400., 500., 20001)

@@ -86,7 +86,7 @@ def test_grouped_charge_and_bins_match_oracle(device, screen, pool_nodes):
     alpha = acceptance_alpha(d)
     settings = PromptSettings()
     unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
-    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    oms = DeviceOMs(d.positions_m, (-d.orientations), d.effective_area_m2, origins, device)
     optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, settings, device)
     work = np.arange(len(d))
     cone = 1.0 / (el.beta * phase)
@@ -101,7 +101,7 @@ def test_grouped_charge_and_bins_match_oracle(device, screen, pool_nodes):
     charge, bins = grouped(pencils, oms, work, optics, merge, compute_bins=not screen,
                            pool_nodes=pool_nodes)
     want_charge, want_bins, _, _ = oracle.grouped(
-        d.positions_m, d.looks, d.effective_area_m2, origins, work, want_pencils, tables, alpha,
+        d.positions_m, (-d.orientations), d.effective_area_m2, origins, work, want_pencils, tables, alpha,
         w.medium.g, w.edges_ns, oracle.PromptSettings(), merge, compute_bins=not screen)
     _close(charge.cpu().numpy(), want_charge)
     if not screen:
@@ -116,7 +116,7 @@ def test_wide_and_single_kernel_batches_agree(device):
     w, d, tables, el, origins, phase = _setup()
     alpha = acceptance_alpha(d)
     unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
-    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    oms = DeviceOMs(d.positions_m, (-d.orientations), d.effective_area_m2, origins, device)
     optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, PromptSettings(), device)
     pencils = compress(unit, 0.5, 96, 192)
     merge = (2e-3, 0.08, 0.5, 0.02)
@@ -137,7 +137,7 @@ def test_grouped_in_parts_matches_one_call(device, monkeypatch):
     w, d, tables, el, origins, phase = _setup()
     alpha = acceptance_alpha(d)
     unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
-    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    oms = DeviceOMs(d.positions_m, (-d.orientations), d.effective_area_m2, origins, device)
     optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, PromptSettings(), device)
     pencils = compress(unit, 0.5, 96, 192)
     merge = (2e-3, 0.08, 0.5, 0.02)
@@ -157,11 +157,11 @@ def test_spectral_order1_with_global_screen_matches_oracle(device):
     rng = np.random.default_rng(3)
     q0 = rng.uniform(0.0, 0.02, len(d))
     extra = rng.uniform(0.0, 0.005, len(d))
-    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    oms = DeviceOMs(d.positions_m, (-d.orientations), d.effective_area_m2, origins, device)
     work = np.arange(len(d))
     got = spectral_order1(DeviceChords(el, device), oms, work, tables, alpha, w.medium.g,
                           w.edges_ns, 0.01, q0, extra)
-    want = oracle.spectral_order1(el, d.positions_m, d.looks, d.effective_area_m2, origins, work,
+    want = oracle.spectral_order1(el, d.positions_m, (-d.orientations), d.effective_area_m2, origins, work,
                                   tables, alpha, w.medium.g, w.edges_ns, 0.01, q0, extra)
     assert np.array_equal(got[2], want[2])
     _close(got[0], want[0])
@@ -179,7 +179,7 @@ def test_grouped_keys_wider_than_32_bits_match_oracle(device, monkeypatch):
     w, d, tables, el, origins, phase = _setup()
     alpha = acceptance_alpha(d)
     unit = DeviceUnitChords(DeviceChords(el, device), np.ones(len(el), bool), phase)
-    oms = DeviceOMs(d.positions_m, d.looks, d.effective_area_m2, origins, device)
+    oms = DeviceOMs(d.positions_m, (-d.orientations), d.effective_area_m2, origins, device)
     optics = DeviceOptics(tables, alpha, w.medium.g, w.edges_ns, PromptSettings(), device)
     pencils = compress(unit, 0.5, 96, 192)
     merge = (2e-3, 0.08, 1e-5, 0.02)
@@ -196,7 +196,7 @@ def test_grouped_keys_wider_than_32_bits_match_oracle(device, monkeypatch):
     assert seen and seen[0] + int(len(work) - 1).bit_length() > 32
     want_pencils = oracle.compress(el, 1.0 / (el.beta * phase), 0.5, 96, 192)
     want_charge, want_bins, _, _ = oracle.grouped(
-        d.positions_m, d.looks, d.effective_area_m2, origins, work, want_pencils, tables, alpha,
+        d.positions_m, (-d.orientations), d.effective_area_m2, origins, work, want_pencils, tables, alpha,
         w.medium.g, w.edges_ns, oracle.PromptSettings(), merge, compute_bins=True)
     _close(charge.cpu().numpy(), want_charge)
     _close(bins.cpu().numpy(), want_bins)

@@ -32,7 +32,7 @@ def test_demonstration_event_total_matches_the_published_value():
     origins = el.start_ns[first] + np.linalg.norm(d.positions_m - el.start_m[first],
                                                   axis=1) / (0.299792458 / fastest)
     q1, bins, level = oracle.spectral_order1(
-        el, d.positions_m, d.looks, d.effective_area_m2, origins, np.arange(len(d)), tables,
+        el, d.positions_m, (-d.orientations), d.effective_area_m2, origins, np.arange(len(d)), tables,
         acceptance_alpha(d), w.medium.g, w.edges_ns, w.threshold_pe, np.zeros(len(d)), None)
     assert abs(q1.sum() - 50.68) <= 0.005
     assert np.all(level == 2)

@@ -29,7 +29,7 @@ def forms(device):
         forms[form] = other(w.elements)
     bundle = pack.bundle(kernel.wavelength.wavelength_nm, kernel.config.source_degree,
                          np.arange(65))
-    want = oracle_route.transport_shower(kernel, w.elements, bundle, route.tables()["fold"],
+    want = oracle_route.transport_shower(kernel, w.elements, bundle, None,
                                          config=route.config)
     return forms, want
 

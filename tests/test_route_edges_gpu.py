@@ -22,7 +22,7 @@ def setup(device):
 def _oracle(setup, elements):
     from oracle import route as oracle_route
     w, kernel, route, bundle = setup
-    return oracle_route.transport_shower(kernel, elements, bundle, route.tables()["fold"],
+    return oracle_route.transport_shower(kernel, elements, bundle, None,
                                          config=route.config)
 
 

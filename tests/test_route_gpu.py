@@ -15,7 +15,7 @@ def demonstration(device):
     tables = route.tables()
     bundle = pack.bundle(kernel.wavelength.wavelength_nm, kernel.config.source_degree,
                          np.arange(65))
-    expected = oracle_route.transport_shower(kernel, w.elements, bundle, tables["fold"],
+    expected = oracle_route.transport_shower(kernel, w.elements, bundle, None,
                                              config=route.config)
     return w, route, response, expected
 

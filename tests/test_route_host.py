@@ -89,13 +89,13 @@ def test_oracle_route_orders_against_the_published_demonstration():
     d = w.detector
     kernel = RouteKernel(w.medium, w.detector, RouteConfig(relative_time_edges_ns=w.edges_ns))
     waves, indices, bundle = synthetic_pack(w.medium, kernel.wavelength)
-    fold = BandFold.of(kernel, rank=3)
-    out = oracle_route.transport_shower(kernel, w.elements, bundle, fold, config=ShowerConfig())
+    out = oracle_route.transport_shower(kernel, w.elements, bundle, config=ShowerConfig())
+    assert out["fold_bands"] == BandFold.of(kernel, rank=3).bands == (0, 1, 2)
     totals = out["charge"].sum(axis=0)
     assert abs(totals[1] - 50.68) <= 0.005
     optics = prepare_optics(w.medium, d)
     leading = oracle_order0.leading_interval_charge(
-        d.positions_m, d.looks, d.effective_area_m2, d.receiver_surface.radii(len(d)),
+        d.positions_m, (-d.orientations), d.effective_area_m2, d.receiver_surface.radii(len(d)),
         w.elements, optics, acceptance_alpha(d))
     assert abs(leading.sum() - 63.74) <= 0.005
     lit = leading > 1e-3
