@@ -61,30 +61,8 @@ TIMED_KERNELS = ("order0_pairs<false>", "order0_pairs<true>", "o1_batch_profile"
 
 def build_route(name):
     """Workload, route (tables resident) and the event's source."""
-    from lighthit_amd.fold import KernelPack
-    from lighthit_amd.route import (RouteConfig, RouteKernel, ShowerConfig, ShowerPrompt,
-                                    ShowerTransport)
-    from lighthit_amd.synthetic import (heavy_event, plain_detector_scale, plain_small,
-                                        synthetic_pack)
-    workload = {"detector": plain_detector_scale, "small": plain_small,
-                "heavy": heavy_event}[name]()
-    heavy = name == "heavy"
-    degree, azimuthal = (16, 4) if heavy else (8, 2)
-    kernel = RouteKernel(workload.medium, workload.detector,
-                         RouteConfig(relative_time_edges_ns=workload.edges_ns,
-                                     wavelength_nodes=len(workload.wavelength.wavelength_nm),
-                                     omega_per_ns=workload.omega_per_ns,
-                                     readout_omega_per_ns=workload.readout_omega_per_ns,
-                                     scattering_degree=degree, source_degree=degree,
-                                     azimuthal_degree=azimuthal))
-    pack = KernelPack()
-    pack.add(*synthetic_pack(workload.medium, kernel.wavelength, degree=degree,
-                             omega=workload.omega_per_ns))
-    # the order-1 selection radius: the reference's default guard (150 m)
-    # raises for these events; the heavy event's 10 km track needs 1000 m
-    config = ShowerConfig(prompt=ShowerPrompt(max_distance_m=1000.0 if heavy else 300.0))
-    route = ShowerTransport(kernel, pack, config=config)
-    return workload, kernel, route
+    from lighthit_amd.synthetic import workload_route
+    return workload_route(name)
 
 
 def route_factory(name):

@@ -160,22 +160,38 @@ def heavy_event(elements=1_000_000):
                     np.linspace(0., 1.2, 321), _shower(pose, 10_000.0, elements, 31))
 
 
-def serving_stream(detector, count=4096, *, seed=23, low=500, high=200_000, inflate_m=100.0,
-                   track_m=300.0):
-    """Events of the serving stream: element counts log-uniform in [low, high],
-    vertices uniform in the array's bounding box inflated by ``inflate_m``,
-    isotropic directions, all from ``seed``. Every event has a ``track_m``
-    charged track (the element count sets the work, not the brightness).
-    Yields ``count`` host element sets in order."""
+def serving_plan(detector, count=4096, *, seed=23, low=500, high=200_000, inflate_m=100.0):
+    """The draws of the serving stream without building its showers: per event
+    ``(elements, vertex, direction, shower_seed)``. Element counts are
+    log-uniform in [low, high], vertices uniform in the array's bounding box
+    inflated by ``inflate_m``, directions isotropic, all from ``seed``."""
     rng = np.random.default_rng(seed)
     lo = detector.positions_m.min(axis=0) - inflate_m
     hi = detector.positions_m.max(axis=0) + inflate_m
+    plan = []
     for _ in range(count):
         n = int(round(np.exp(rng.uniform(np.log(low), np.log(high)))))
         vertex = lo + rng.uniform(size=3) * (hi - lo)
         direction = rng.normal(size=3)
         direction /= np.linalg.norm(direction)
-        yield _shower(SourcePose(vertex, direction), track_m, n, int(rng.integers(2 ** 31)))
+        plan.append((n, vertex, direction, int(rng.integers(2 ** 31))))
+    return plan
+
+
+def serving_event(entry, track_m=300.0):
+    """The shower of one :func:`serving_plan` entry (a ``track_m`` charged track:
+    the element count sets the work, not the brightness)."""
+    n, vertex, direction, seed = entry
+    return _shower(SourcePose(vertex, direction), track_m, n, seed)
+
+
+def serving_stream(detector, count=4096, *, seed=23, low=500, high=200_000, inflate_m=100.0,
+                   track_m=300.0):
+    """Events of the serving stream (:func:`serving_plan`), built one at a time:
+    yields ``count`` host element sets in order."""
+    for entry in serving_plan(detector, count, seed=seed, low=low, high=high,
+                              inflate_m=inflate_m):
+        yield serving_event(entry, track_m)
 
 
 def poisson_arrivals(rate_per_s, count, seed=23):
@@ -230,3 +246,29 @@ def synthetic_pack(medium, quadrature, *, degree=8, acceptance_degree=2,
                   absorption=np.asarray(sample["absorption_per_m"], float),
                   speed=0.299792458 / np.asarray(sample["group_index"], float))
     return wavelengths, np.arange(len(omega)), bundle
+
+
+def workload_route(name, device="cuda"):
+    """Workload ``name`` ("detector", "small" or "heavy"), its route kernel and the
+    route (tables resident on ``device`` on first use): the configuration the
+    benchmark and the parity fixtures run. The order-1 selection radius is 300 m
+    (1000 m for the heavy event's 10 km track): the default guard of the radial
+    range (150 m) raises for these events."""
+    from .fold import KernelPack
+    from .route import RouteConfig, RouteKernel, ShowerConfig, ShowerPrompt, ShowerTransport
+    workload = {"detector": plain_detector_scale, "small": plain_small,
+                "heavy": heavy_event}[name]()
+    heavy = name == "heavy"
+    degree, azimuthal = (16, 4) if heavy else (8, 2)
+    kernel = RouteKernel(workload.medium, workload.detector,
+                         RouteConfig(relative_time_edges_ns=workload.edges_ns,
+                                     wavelength_nodes=len(workload.wavelength.wavelength_nm),
+                                     omega_per_ns=workload.omega_per_ns,
+                                     readout_omega_per_ns=workload.readout_omega_per_ns,
+                                     scattering_degree=degree, source_degree=degree,
+                                     azimuthal_degree=azimuthal))
+    pack = KernelPack()
+    pack.add(*synthetic_pack(workload.medium, kernel.wavelength, degree=degree,
+                             omega=workload.omega_per_ns))
+    config = ShowerConfig(prompt=ShowerPrompt(max_distance_m=1000.0 if heavy else 300.0))
+    return workload, kernel, ShowerTransport(kernel, pack, config=config, device=device)

@@ -709,8 +709,9 @@ int oracle_order0(const int64_t* pair_om, const int64_t* pair_el, int64_t npairs
                   const double* t0, const double* t1, int npanel, const double* panel_edges,
                   const double* phase, const double* group, const double* ext, const double* eff,
                   int effcols, const double* alpha, int nalpha, const double* origins,
-                  const double* edges, int nbins, double rel_tol, double bin_tol,
-                  int force_rays, double* charge, double* bins, double* bins_change) {
+                  const double* edges, int nbins, double rel_tol, double abs_tol,
+                  double bin_tol, int force_rays, double* charge, double* bins,
+                  double* bins_change) {
     Optics o = {npanel, panel_edges, phase, group, ext, eff, effcols};
     int regular_phase = 1;
     for (int k = 0; k < npanel; ++k)
@@ -764,8 +765,8 @@ int oracle_order0(const int64_t* pair_om, const int64_t* pair_el, int64_t npairs
                     memcpy(pb2, pb, sizeof(double) * nbins);
                 }
                 last_l1 = l1;
-                charge_converged = m > 0 && change <= rel_tol * fabs(value) + 1e-300;
-                if (charge_converged && l1 <= bin_tol * fabs(value) + 1e-300) {
+                charge_converged = m > 0 && change <= rel_tol * fabs(value) + abs_tol;
+                if (charge_converged && l1 <= bin_tol * fabs(value) + abs_tol) {
                     converged = 1;
                     break;
                 }

@@ -66,13 +66,15 @@ def candidates(positions, radii, elements, optics):
 
 
 def first_entry(positions, looks, areas, radii, elements, optics, alpha, *, origins=None,
-                edges=None, rel_tol=1e-11, bin_tol=5e-5, force_rays=False, pairs=None,
-                return_uncertainty=False):
+                edges=None, rel_tol=1e-11, abs_tol=1e-16, bin_tol=5e-5, force_rays=False,
+                pairs=None, return_uncertainty=False):
     """Charge (OM,) and bins (OM, bin) or None; looks are ``-orientations``.
 
-    Every pair's charge converges to ``rel_tol``; its bins to ``bin_tol`` of
-    its charge or, at the highest order, keep their last L1 change, summed per
-    OM into the bin uncertainty returned with ``return_uncertainty``.
+    Every pair's charge converges to ``rel_tol`` of itself plus ``abs_tol`` pe
+    (pairs of negligible light stop early); its bins to ``bin_tol`` of its
+    charge (plus ``abs_tol``) or, at the highest order, keep their last L1
+    change, summed per OM into the bin uncertainty returned with
+    ``return_uncertainty``.
     """
     positions = np.ascontiguousarray(positions, float)
     nom = len(positions)
@@ -105,7 +107,8 @@ def first_entry(positions, looks, areas, radii, elements, optics, alpha, *, orig
                ptr(opt["phase_nodes"]), ptr(opt["group_nodes"]), ptr(opt["extinction_nodes"]),
                ptr(eff), ctypes.c_int(eff.shape[1]), ptr(al), ctypes.c_int(len(al)),
                ptr(orig) if want_bins else None, ptr(edges_a) if want_bins else None,
-               ctypes.c_int(nbins), ctypes.c_double(rel_tol), ctypes.c_double(bin_tol),
+               ctypes.c_int(nbins), ctypes.c_double(rel_tol), ctypes.c_double(abs_tol),
+               ctypes.c_double(bin_tol),
                ctypes.c_int(int(force_rays)),
                ptr(charge), ptr(bins) if want_bins else None, ptr(change))
     del keep

@@ -1,0 +1,54 @@
+"""Writes detector_scale_route.npz: the oracle route on the detector-scale event.
+
+The event: 2304-OM array (8 clusters x 8 strings x 36 OMs), 50,000-element
+Gaussian shower with a 1500 m charged track from seed 11, 8 wavelength nodes,
+1 ns bins over -60..744 ns, order-1 selection radius 300 m, and the seeded
+synthetic order >=2 pack of ``lighthit_amd.synthetic.synthetic_pack`` (the same
+inputs as the benchmark). The node bands are the package's choice (the best
+subsets tie to rounding; the oracle checks the choice is optimal and computes
+the interpolation itself).
+
+Stored: charge per order (OM, 3), active, ge2 charge level, time origins, the
+order-1 level, bins per order of the active OMs (float32), the order >=2
+spectrum of the active OMs (complex64), the node bands and the workload's
+element count. Run from the repository root: ``python tests/golden/make_detector_scale_route.py``
+(several minutes on 8 cores; the oracle's C parts use OpenMP).
+"""
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+OUT = Path(__file__).resolve().parent / "detector_scale_route.npz"
+
+
+def main():
+    from lighthit_amd.fold import BandFold
+    from lighthit_amd.synthetic import workload_route
+    from oracle import route as oracle_route
+    workload, kernel, route = workload_route("detector", device="cpu")
+    pack = route.data.ge2_pack
+    bundle = pack.bundle(kernel.wavelength.wavelength_nm, int(kernel.config.source_degree),
+                         np.arange(65))
+    bands = BandFold.of(kernel, rank=3).bands
+    began = time.time()
+    out = oracle_route.transport_shower(kernel, workload.elements, bundle, bands,
+                                        config=route.config)
+    seconds = time.time() - began
+    active = np.flatnonzero(out["active"])
+    np.savez_compressed(
+        OUT, charge=out["charge"], active=out["active"], level=out["level"],
+        origins=out["origins"], order1_level=out["order1_level"], active_ids=active,
+        bins=out["bins"][active].astype(np.float32),
+        spectrum=out["spectrum"][:, active].astype(np.complex64),
+        bands=np.asarray(out["fold_bands"]), elements=np.int64(len(workload.elements)),
+        fallback=np.bool_(out["fallback"]), local_origin=np.bool_(out["local_origin"]))
+    print(f"{OUT.name}: {len(active)} active OMs, charge by order "
+          f"{out['charge'].sum(axis=0)}, {seconds:.0f} s")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,72 @@
+"""The serving stream of the detector-scale array (4096 events, element counts
+log-uniform in 500..200,000, seed 23): a fixed slice, holding the stream's
+first eight events, its largest (199,838 elements), its smallest (501) and a
+small bright one (event 46, 628 elements), served concurrently on three
+streams. Every result equals the plain call's bit for bit, and the small events
+with active OMs (7: 1355 elements, 46) equal the oracle route at the plain
+form's bars."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LARGEST, SMALLEST, SMALL_BRIGHT = 1061, 173, 46
+
+
+@pytest.fixture(scope="module")
+def served(device):
+    from lighthit_amd.serve import ShowerServer
+    from lighthit_amd.synthetic import serving_event, serving_plan, workload_route
+    workload, kernel, route = workload_route("detector", device=device)
+    plan = serving_plan(workload.detector)
+    assert plan[LARGEST][0] == max(p[0] for p in plan) == 199_838
+    assert plan[SMALLEST][0] == min(p[0] for p in plan) == 501
+    picks = list(range(8)) + [LARGEST, SMALLEST, SMALL_BRIGHT]
+    events = [serving_event(plan[i]) for i in picks]
+    print(f"serving slice: events {picks}, elements {[len(e) for e in events]}", flush=True)
+    with ShowerServer(route, workers=3, device=device) as server:
+        futures = [server.submit(e) for e in events]
+        mixed = [f.result(timeout=600) for f in futures]
+    print("serving slice: served, now alone", flush=True)
+    alone = [route(e) for e in events]
+    return workload, kernel, route, dict(zip(picks, events)), mixed, alone, futures
+
+
+def test_each_served_result_equals_the_plain_call(served):
+    workload, kernel, route, events, mixed, alone, futures = served
+    for m, a in zip(mixed, alone):
+        assert np.array_equal(m.active, a.active)
+        assert np.array_equal(m.charge_components_pe, a.charge_components_pe)
+        assert np.array_equal(m.components_pe, a.components_pe)
+        assert np.array_equal(m.spectrum_ge2_pe, a.spectrum_ge2_pe)
+        assert np.array_equal(m.time_origin_ns, a.time_origin_ns)
+    assert all(f.latency_s > 0 for f in futures)
+    assert sum(int(a.active.sum()) for a in alone) > 0
+
+
+@pytest.mark.parametrize("index", [7, SMALL_BRIGHT])
+def test_small_events_match_the_oracle_route(served, index):
+    from oracle import route as oracle_route
+    workload, kernel, route, events, mixed, alone, futures = served
+    tables = route.tables()
+    bundle = route.data.ge2_pack.bundle(kernel.wavelength.wavelength_nm,
+                                        int(kernel.config.source_degree), np.arange(65))
+    print(f"serving slice: oracle route of event {index}", flush=True)
+    want = oracle_route.transport_shower(kernel, events[index], bundle, tables["fold"].bands,
+                                         config=route.config)
+    got = alone[list(events).index(index)]
+    assert want["active"].sum() > 10
+    assert np.array_equal(got.active, want["active"])
+    for order, tolerance in ((0, 1e-5), (1, 1e-6), (2, 1e-5)):
+        ref = want["charge"][:, order]
+        scale = max(float(np.abs(ref).max()), 1e-300)
+        assert np.all(np.abs(got.charge_components_pe[:, order] - ref)
+                      <= tolerance * np.abs(ref) + 1e-7 * scale), order
+    ids = np.flatnonzero(want["active"])
+    # order-0 bins: the oracle converges each pair's bins to 5e-5 of its charge
+    # (bin_tol), so the bar on an OM's bin L1 is 2e-4 here
+    for order, tolerance in ((0, 2e-4), (1, 1e-6), (2, 2e-5)):
+        ref = want["bins"][ids, :, order]
+        scale = np.maximum(np.abs(ref).sum(axis=1), 1e-12)
+        err = np.abs(got.components_pe[ids, :, order] - ref).sum(axis=1) / scale
+        assert len(ids) == 0 or err.max() < tolerance, (order, err.max())
