@@ -110,8 +110,13 @@ typedef struct lh_ge2_pack {
     const double* omega;        /* (freq_stride) [rad/ns] */
     int bands, intervals, degree, acceptance_degree, freq_stride;
     int flight;                 /* 1: radial splines carry the flight phase */
-    int coefficient_format;     /* 0: complex64; 1: bfloat16 (re, im) pairs, 4 bytes each
-                                   (the precision form; FP32 accumulation either way) */
+    int coefficient_format;     /* 0: complex64 in the layout above (FP32 VALU contraction);
+                                   1: the precision form's MFMA layout: bfloat16 hi + lo,
+                                   (bands, intervals, degree+1, freq_stride, KQ,
+                                   [re hi, re lo, im hi, im lo], 4 powers), KQ = pairs
+                                   rounded up to a multiple of 4 with zeros; contracted
+                                   as hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x16_bf16
+                                   with FP32 accumulation */
 } lh_ge2_pack;
 
 typedef struct lh_ge2_source {
@@ -128,8 +133,13 @@ typedef struct lh_ge2_receivers {
     int count;
 } lh_ge2_receivers;
 
-/* Device workspace bytes lh_ge2_apply needs (0 when none). */
+/* Device workspace bytes lh_ge2_apply needs for a complex64 pack (0 when none). */
 size_t lh_ge2_workspace_bytes(int bands, int freq, int receivers, int points);
+
+/* Device workspace bytes of one lh_ge2_apply call with either coefficient format
+ * (the MFMA form keeps its pair lists, records and per-pair spectra there). */
+size_t lh_ge2_apply_workspace_bytes(const lh_ge2_pack* pack, const lh_ge2_source* source,
+                                    int receivers, int freq);
 
 /*
  * Descriptors are host structs holding device pointers. alpha (A+1) double

@@ -35,8 +35,9 @@ constexpr int kGroup = 8;     // points per degree-outer group in the wide kerne
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 struct Ge2Params {
-    const float2* coef;        // (bands, intervals, degree+1, pairs, 4, cstride) complex64
-    const unsigned* coef16;    // the same as bfloat16 (re, im) pairs: re low, im high half
+    const float2* coef;        // (bands, intervals, degree+1, pairs, 4, cstride) complex64, or
+                               // the bfloat16 hi + lo MFMA layout (coefficient_format 1)
+    const unsigned* coef16;    // unused
     const double* log_r;       // (intervals+1)
     const double* absorption;  // (bands)
     const double* speed;       // (bands)
@@ -60,15 +61,11 @@ struct Ge2Params {
 
 __device__ __forceinline__ double parity(int n) { return (n & 1) ? -1.0 : 1.0; }
 
-// Pack coefficient i: complex64 (F = 0) or a bfloat16 pair widened exactly (F = 1).
+// Pack coefficient i (complex64; F is kept for a second storage form).
 template <int F>
 __device__ __forceinline__ float2 coef_at(const Ge2Params& P, size_t i) {
-    if constexpr (F == 0) {
-        return P.coef[i];
-    } else {
-        const unsigned v = P.coef16[i];
-        return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
-    }
+    static_assert(F == 0, "complex64 coefficients only");
+    return P.coef[i];
 }
 
 __device__ __forceinline__ double ipow(double x, int k) {
@@ -519,6 +516,12 @@ __global__ __launch_bounds__(kThreads, 2) void ge2_wide(Ge2Params P) {
     }
 }
 
+// Zero spectrum (no source points).
+__global__ void ge2_zero_out(Ge2Params P) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < P.nrec * P.nfreq) P.out[idx] = make_float2(0.f, 0.f);
+}
+
 // out[w, d] = sum_band e^{i w ref_band} sum_split partial[d, split, band, w]
 __global__ void ge2_wide_reduce(Ge2Params P) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -701,6 +704,439 @@ int launch(Ge2Params P, hipStream_t stream) {
     return lh::check_launch("lh_ge2_apply (wide reduce)");
 }
 
+// ------------------------------------------------------------------ MFMA form
+//
+// The precision form of the multi-frequency apply. For one (OM d, point p) pair
+// in radial interval j and one degree l, the rotation of the point's channels
+// and the detector-side pair sums fold into one real matrix Q_l:
+//   sum_a rot_l[a] cs_l[a] = sum_b ch_l[b] H_l[b],   H_l[b] = sum_K Q_l[b, K] C_jl[K],
+//   Q_l[b, (k, pw)] = t^(3 - pw) (E_l[A + mu_k][b] pos_k + E_l[A - mu_k][b] neg_k),
+// with C_jl[K] (K = pair k x cubic power pw) the interval's spline coefficients
+// of every (band, frequency) column: a product whose right operand is shared
+// by every pair in interval j. Pairs are grouped by interval; a 16-row MFMA
+// tile holds 16 / R pairs, R the power of two >= 2M + 1 (rows b = -M..M of
+// each, then zero rows), so that every pair's rows fall on the same lanes of
+// the result whatever its place in the tile and its sums over b run in one
+// order. Columns are (band, frequency). Q and C are split into bfloat16 hi + lo and
+// hi*hi + hi*lo + lo*hi is accumulated in FP32 on v_mfma_f32_16x16x16_bf16
+// (the lo*lo term lies below the FP32 sum's rounding). The epilogue multiplies
+// by the point's channels, sums b and l in a fixed order, applies attenuation
+// and the FP64-reduced flight phase and writes the pair's spectrum; the last
+// kernel sums every OM's points in point order, so the result depends on
+// neither the grouping nor the scheduling.
+
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaChunk = 8;   // column tiles of 16 per pass
+
+// Device workspace of the MFMA form (laid out by mfma_layout).
+struct MfmaWork {
+    int* count;       // (intervals) pairs per interval
+    int* offset;      // (intervals + 1) first pair slot of each interval
+    int* cursor;      // (intervals) fill cursors
+    int* tile_off;    // (intervals + 1) first tile of each interval
+    int* totals;      // [pairs, tiles]
+    unsigned char* valid;   // (nrec, npoints) pair inside the radial window
+    int* list;        // (pairs) d * npoints + p, grouped by interval
+    float* records;   // (pairs, record_words)
+    float2* partial;  // (nrec, npoints, bands * nfreq)
+};
+
+__host__ __device__ inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Sizes (and, with base != nullptr, pointers) of the MFMA workspace.
+__host__ size_t mfma_layout(const Ge2Params& P, int words, char* base, MfmaWork* W) {
+    const size_t pairs = (size_t)P.nrec * P.npoints;
+    const int J = P.intervals;
+    size_t at = 0;
+    auto take = [&](size_t bytes) { const size_t here = at; at += align256(bytes); return here; };
+    const size_t o_count = take(sizeof(int) * J), o_offset = take(sizeof(int) * (J + 1));
+    const size_t o_cursor = take(sizeof(int) * J), o_tile = take(sizeof(int) * (J + 1));
+    const size_t o_totals = take(sizeof(int) * 2), o_valid = take(pairs);
+    const size_t o_list = take(sizeof(int) * pairs);
+    const size_t o_records = take(sizeof(float) * pairs * words);
+    const size_t o_partial = take(sizeof(float2) * pairs * P.bands * P.nfreq);
+    if (base != nullptr) {
+        W->count = reinterpret_cast<int*>(base + o_count);
+        W->offset = reinterpret_cast<int*>(base + o_offset);
+        W->cursor = reinterpret_cast<int*>(base + o_cursor);
+        W->tile_off = reinterpret_cast<int*>(base + o_tile);
+        W->totals = reinterpret_cast<int*>(base + o_totals);
+        W->valid = reinterpret_cast<unsigned char*>(base + o_valid);
+        W->list = reinterpret_cast<int*>(base + o_list);
+        W->records = reinterpret_cast<float*>(base + o_records);
+        W->partial = reinterpret_cast<float2*>(base + o_partial);
+    }
+    return at;
+}
+
+// Radial interval of one (OM, point) pair; false outside the window.
+__device__ bool pair_interval(const Ge2Params& P, int d, int p, int& j) {
+    const double dx = P.receivers[3 * d + 0] - P.points[3 * p + 0];
+    const double dy = P.receivers[3 * d + 1] - P.points[3 * p + 1];
+    const double dz = P.receivers[3 * d + 2] - P.points[3 * p + 2];
+    const double radius = sqrt(dx * dx + dy * dy + dz * dz);
+    if (radius < P.r_low || radius > P.r_high) return false;
+    const double logr = log(radius);
+    int lo = 0, hi = P.intervals + 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (P.log_r[mid] <= logr) lo = mid + 1; else hi = mid;
+    }
+    j = lo - 1;
+    j = j < 0 ? 0 : (j > P.intervals - 1 ? P.intervals - 1 : j);
+    return true;
+}
+
+// Clears the interval counters (a kernel, so the sequence can be captured).
+__global__ void ge2_mfma_init(Ge2Params P, MfmaWork W) {
+    for (int i = threadIdx.x; i < P.intervals; i += blockDim.x) {
+        W.count[i] = 0;
+        W.cursor[i] = 0;
+    }
+}
+
+__global__ void ge2_mfma_count(Ge2Params P, MfmaWork W) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)P.nrec * P.npoints) return;
+    const int d = (int)(idx / P.npoints), p = (int)(idx - (size_t)d * P.npoints);
+    int j = 0;
+    const bool inside = pair_interval(P, d, p, j);
+    W.valid[idx] = inside ? 1 : 0;
+    if (inside) atomicAdd(&W.count[j], 1);
+}
+
+// Pair and tile offsets of every interval (one thread: a few hundred intervals).
+__global__ void ge2_mfma_scan(Ge2Params P, MfmaWork W, int per_tile) {
+    if (threadIdx.x != 0) return;
+    int pairs = 0, tiles = 0;
+    for (int j = 0; j < P.intervals; ++j) {
+        W.offset[j] = pairs;
+        W.tile_off[j] = tiles;
+        pairs += W.count[j];
+        tiles += (W.count[j] + per_tile - 1) / per_tile;
+    }
+    W.offset[P.intervals] = pairs;
+    W.tile_off[P.intervals] = tiles;
+    W.totals[0] = pairs;
+    W.totals[1] = tiles;
+}
+
+// Pair list grouped by interval. The order inside an interval follows the
+// atomics; it changes which pairs share a tile, never a pair's result.
+__global__ void ge2_mfma_fill(Ge2Params P, MfmaWork W) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)P.nrec * P.npoints || !W.valid[idx]) return;
+    const int d = (int)(idx / P.npoints), p = (int)(idx - (size_t)d * P.npoints);
+    int j = 0;
+    pair_interval(P, d, p, j);
+    const int slot = W.offset[j] + atomicAdd(&W.cursor[j], 1);
+    W.list[slot] = (int)idx;
+}
+
+// FP64 geometry of every listed pair into its record (the wide kernel's LDS
+// record, in global memory): d * npoints + p, interval, offset t, radius,
+// detector pair factors, band scales and the rotation rows of every degree.
+template <int A, int M>
+__global__ void ge2_mfma_records(Ge2Params P, MfmaWork W) {
+    constexpr int NA = 2 * A + 1, NM = 2 * M + 1, NPAIR = (A + 1) * (A + 2) / 2;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= W.totals[0]) return;
+    const int idx = W.list[slot];
+    const int d = idx / P.npoints, p = idx - d * P.npoints;
+    PairGeometry<A, M> g;
+    g.compute(P, d, p);
+    float* rec = W.records + (size_t)slot * P.record_words;
+    int* irec = reinterpret_cast<int*>(rec);
+    irec[0] = idx;
+    irec[1] = g.j;
+    rec[2] = (float)g.t;
+    reinterpret_cast<double*>(rec)[2] = g.radius;
+#pragma unroll
+    for (int k = 0; k < NPAIR; ++k) {
+        rec[6 + 2 * k] = g.pos[k];
+        rec[7 + 2 * k] = g.neg[k];
+    }
+    float* scale = rec + 6 + 2 * NPAIR;
+    for (int lam = 0; lam < P.bands; ++lam)
+        scale[lam] = (float)(P.out_scale[(size_t)lam * P.nrec + d] * exp(-P.absorption[lam] * g.radius)
+                             / (4.0 * 3.14159265358979323846 * g.radius * g.radius));
+    float* E = scale + P.bands;
+    Rotation<A, M> rot;
+    rot.init(g.cosine, g.phi);
+    for (int l = 0; l <= P.degree; ++l) {
+        rot.step(l);
+        float El[NA][NM];
+        rot.rows(l, El);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NM; ++b) E[(l * NA + a) * NM + b] = El[a][b];
+    }
+}
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);   // nearest even
+}
+
+__device__ __forceinline__ float bf16_value(unsigned short h) {
+    return __uint_as_float((unsigned)h << 16);
+}
+
+// Canonical pair k -> (lambda, mu): lambda = 0..A, mu = 0..lambda.
+template <int A>
+__device__ __forceinline__ int pair_mu(int k) {
+    int lam = 0;
+    while ((lam + 1) * (lam + 2) / 2 <= k) ++lam;
+    return k - lam * (lam + 1) / 2;
+}
+
+template <int A, int M>
+__global__ __launch_bounds__(256) void ge2_mfma_apply(Ge2Params P, MfmaWork W,
+                                                       const shortx4* __restrict__ B) {
+    constexpr int NA = 2 * A + 1, NM = 2 * M + 1, NPAIR = (A + 1) * (A + 2) / 2;
+    constexpr int KQ = (NPAIR + 3) / 4 * 4, KSTEPS = KQ / 4;
+    constexpr int RPP = NM == 1 ? 1 : (NM <= 4 ? 4 : (NM <= 8 ? 8 : 16));   // rows per pair
+    constexpr int PPT = 16 / RPP;                // pairs per 16-row tile
+    constexpr int SLOTS = RPP == 1 ? 4 : 1;      // pairs one lane's four result rows hold
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int group = lane >> 4;                 // K quarter (A, B) / row quarter (D)
+    const int column = lane & 15;
+    const int ncol = P.bands * P.nfreq;
+    const int ntile_col = (ncol + 15) / 16;
+    const int total_tiles = W.totals[1];
+    const int J = P.intervals;
+    // operand A: this lane's row of the tile
+    const int arow_q = column / RPP, arow_b = column - arow_q * RPP;
+    // result rows of this lane: 4 * group + i
+    const int first_q = (4 * group) / RPP;
+
+    for (int tile = blockIdx.x * 4 + wave; tile < total_tiles; tile += gridDim.x * 4) {
+        // interval of the tile: the last j with tile_off[j] <= tile
+        int lo = 0, hi = J;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (W.tile_off[mid] <= tile) lo = mid; else hi = mid;
+        }
+        const int j = lo;
+        const int first = W.offset[j] + (tile - W.tile_off[j]) * PPT;
+        const int npairs = min(PPT, W.offset[j] + W.count[j] - first);
+        const bool arow_ok = arow_q < npairs;
+        const float* arec = W.records + (size_t)(first + (arow_ok ? arow_q : 0)) * P.record_words;
+        const float at = arec[2];
+        float tpow[4] = {at * at * at, at * at, at, 1.f};
+
+        for (int c0 = 0; c0 < ntile_col; c0 += kMfmaChunk) {
+            const int nct = min(kMfmaChunk, ntile_col - c0);
+            float2 tot[kMfmaChunk][SLOTS];
+#pragma unroll
+            for (int c = 0; c < kMfmaChunk; ++c)
+#pragma unroll
+                for (int s = 0; s < SLOTS; ++s) tot[c][s] = make_float2(0.f, 0.f);
+            int starts = 0;
+            for (int l = 0; l <= P.degree; ++l) {
+                const int top_p = l < A ? l : A;
+                const int keep = l < M ? l : M;
+                const int centre = starts + keep;
+                starts += 2 * (l < P.src_az ? l : P.src_az) + 1;
+                // Q_l fragments of this lane (row arow_q, b; K = 4 s + group, power i)
+                shortx4 qhi[KSTEPS], qlo[KSTEPS];
+#pragma unroll
+                for (int s = 0; s < KSTEPS; ++s) {
+                    const int k = 4 * s + group;
+                    float base = 0.f;
+                    const int bb = arow_b - M;
+                    if (arow_ok && arow_b < NM && k < NPAIR && bb >= -keep && bb <= keep) {
+                        const int mu = pair_mu<A>(k);
+                        if (mu <= top_p) {
+                            const float* E = arec + 6 + 2 * NPAIR + P.bands + l * NA * NM;
+                            base = E[(A + mu) * NM + arow_b] * arec[6 + 2 * k];
+                            if (mu) base += E[(A - mu) * NM + arow_b] * arec[7 + 2 * k];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float q = base * tpow[i];
+                        const unsigned short h = bf16_bits(q);
+                        qhi[s][i] = (short)h;
+                        qlo[s][i] = (short)bf16_bits(q - bf16_value(h));
+                    }
+                }
+                for (int c = 0; c < nct; ++c) {
+                    const int col = 16 * (c0 + c) + column;
+                    const bool col_ok = col < ncol;
+                    const int lam = col_ok ? col / P.nfreq : 0;
+                    const int w = col_ok ? col - lam * P.nfreq : 0;
+                    floatx4 hre = {0.f, 0.f, 0.f, 0.f}, him = {0.f, 0.f, 0.f, 0.f};
+                    const size_t bcol = ((((size_t)lam * J + j) * (P.degree + 1) + l) * P.cstride
+                                         + w) * KQ;
+#pragma unroll
+                    for (int s = 0; s < KSTEPS; ++s) {
+                        shortx4 b_re_hi = {0, 0, 0, 0}, b_re_lo = {0, 0, 0, 0};
+                        shortx4 b_im_hi = {0, 0, 0, 0}, b_im_lo = {0, 0, 0, 0};
+                        if (col_ok) {
+                            const shortx4* src = B + (bcol + 4 * s + group) * 4;
+                            b_re_hi = src[0];
+                            b_re_lo = src[1];
+                            b_im_hi = src[2];
+                            b_im_lo = src[3];
+                        }
+                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_re_hi, hre, 0, 0, 0);
+                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_re_lo, hre, 0, 0, 0);
+                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qlo[s], b_re_hi, hre, 0, 0, 0);
+                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_im_hi, him, 0, 0, 0);
+                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_im_lo, him, 0, 0, 0);
+                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qlo[s], b_im_hi, him, 0, 0, 0);
+                    }
+                    if (!col_ok) continue;
+                    // epilogue: H of this lane's rows times the point's channels
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int row = 4 * group + i;
+                        const int q = row / RPP, b = row - q * RPP;
+                        const int bb = b - M;
+                        if (q >= npairs || b >= NM || bb < -keep || bb > keep) continue;
+                        const float* rec = W.records + (size_t)(first + q) * P.record_words;
+                        const int p = reinterpret_cast<const int*>(rec)[0] % P.npoints;
+                        const float2 ch = P.channels[(((size_t)lam * P.npoints + p) * P.nchan
+                                                      + centre + bb) * P.chstride + w];
+                        const int slot = q - first_q;
+#pragma unroll
+                        for (int s = 0; s < SLOTS; ++s) {
+                            if (s != slot) continue;
+                            tot[c][s].x += ch.x * hre[i] - ch.y * him[i];
+                            tot[c][s].y += ch.x * him[i] + ch.y * hre[i];
+                        }
+                    }
+                }
+            }
+            // sum the four row quarters of every pair, then scale and phase
+            for (int c = 0; c < nct; ++c) {
+                const int col = 16 * (c0 + c) + column;
+                const bool col_ok = col < ncol;
+                const int lam = col_ok ? col / P.nfreq : 0;
+                const int w = col_ok ? col - lam * P.nfreq : 0;
+                for (int q = 0; q < PPT; ++q) {
+                    float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int s = 0; s < SLOTS; ++s)
+                        if (q - first_q == s) v = tot[c][s];
+                    v.x += __shfl_xor(v.x, 16, 64);
+                    v.y += __shfl_xor(v.y, 16, 64);
+                    v.x += __shfl_xor(v.x, 32, 64);
+                    v.y += __shfl_xor(v.y, 32, 64);
+                    if (group != 0 || !col_ok || q >= npairs) continue;
+                    const float* rec = W.records + (size_t)(first + q) * P.record_words;
+                    const int idx = reinterpret_cast<const int*>(rec)[0];
+                    const double radius = reinterpret_cast<const double*>(rec)[2];
+                    const float scale = rec[6 + 2 * NPAIR + lam];
+                    float2 sp = make_float2(scale, 0.f);
+                    if (P.flight) {
+                        const float2 ph = phase_of(P.omega[w] / P.speed[lam] * radius);
+                        sp = make_float2(scale * ph.x, scale * ph.y);
+                    }
+                    W.partial[(size_t)idx * ncol + col] = cmul(v, sp);
+                }
+            }
+        }
+    }
+}
+
+// out[w, d] = sum_band e^{i w ref_band} sum_p partial[d, p, band, w], points in order.
+__global__ void ge2_mfma_reduce(Ge2Params P, MfmaWork W) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.nrec * P.nfreq) return;
+    const int d = idx / P.nfreq, w = idx - d * P.nfreq;
+    const int ncol = P.bands * P.nfreq;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int lam = 0; lam < P.bands; ++lam) {
+        float2 band = make_float2(0.f, 0.f);
+        for (int p = 0; p < P.npoints; ++p) {
+            const size_t pair = (size_t)d * P.npoints + p;
+            if (!W.valid[pair]) continue;
+            const float2 v = W.partial[pair * ncol + lam * P.nfreq + w];
+            band.x += v.x;
+            band.y += v.y;
+        }
+        const float2 v = cmul(band, phase_of(P.omega[w] * P.band_ref[lam]));
+        sum.x += v.x;
+        sum.y += v.y;
+    }
+    P.out[(size_t)w * P.nrec + d] = sum;
+}
+
+template <int A, int M>
+int launch_mfma(Ge2Params P, const void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    constexpr int NM = 2 * M + 1;
+    P.record_words = Record<A, M>::words(P.degree, P.bands);
+    MfmaWork W{};
+    const size_t need = mfma_layout(P, P.record_words, nullptr, nullptr);
+    LH_REQUIRE(workspace != nullptr && workspace_bytes >= need,
+               "lh_ge2_apply: workspace of %zu bytes, %zu needed", workspace_bytes, need);
+    mfma_layout(P, P.record_words, static_cast<char*>(const_cast<void*>(workspace)), &W);
+    const size_t pairs = (size_t)P.nrec * P.npoints;
+    if (pairs == 0) {
+        const int total = P.nrec * P.nfreq;
+        hipLaunchKernelGGL(ge2_zero_out, dim3((total + 255) / 256), dim3(256), 0, stream, P);
+        return lh::check_launch("lh_ge2_apply (mfma zero)");
+    }
+    const unsigned grid_pairs = (unsigned)((pairs + 255) / 256);
+    hipLaunchKernelGGL(ge2_mfma_init, dim3(1), dim3(256), 0, stream, P, W);
+    hipLaunchKernelGGL(ge2_mfma_count, dim3(grid_pairs), dim3(256), 0, stream, P, W);
+    constexpr int RPP = NM == 1 ? 1 : (NM <= 4 ? 4 : (NM <= 8 ? 8 : 16));
+    hipLaunchKernelGGL(ge2_mfma_scan, dim3(1), dim3(64), 0, stream, P, W, 16 / RPP);
+    hipLaunchKernelGGL(ge2_mfma_fill, dim3(grid_pairs), dim3(256), 0, stream, P, W);
+    hipLaunchKernelGGL((ge2_mfma_records<A, M>), dim3(grid_pairs), dim3(256), 0, stream, P, W);
+    int status = lh::check_launch("lh_ge2_apply (mfma set-up)");
+    if (status != LH_OK) return status;
+    const size_t max_tiles = (pairs + 16 / RPP - 1) / (16 / RPP) + P.intervals;
+    const unsigned groups = (unsigned)std::min<size_t>((max_tiles + 3) / 4, 2048);
+    {
+        lh::KernelMark mark("ge2_mfma_apply", stream);
+        hipLaunchKernelGGL((ge2_mfma_apply<A, M>), dim3(groups), dim3(256), 0, stream, P, W,
+                           reinterpret_cast<const shortx4*>(P.coef));
+    }
+    status = lh::check_launch("lh_ge2_apply (mfma)");
+    if (status != LH_OK) return status;
+    const int total = P.nrec * P.nfreq;
+    hipLaunchKernelGGL(ge2_mfma_reduce, dim3((total + 255) / 256), dim3(256), 0, stream, P, W);
+    return lh::check_launch("lh_ge2_apply (mfma reduce)");
+}
+
+template <int A>
+int dispatch_mfma(int m, const Ge2Params& P, const void* ws, size_t bytes, hipStream_t s) {
+    switch (m) {
+        case 0: return launch_mfma<A, 0>(P, ws, bytes, s);
+        case 1: return launch_mfma<A, 1>(P, ws, bytes, s);
+        case 2: return launch_mfma<A, 2>(P, ws, bytes, s);
+        case 3: return launch_mfma<A, 3>(P, ws, bytes, s);
+        case 4: return launch_mfma<A, 4>(P, ws, bytes, s);
+        default: break;
+    }
+    lh::set_error("lh_ge2_apply: azimuthal degree %d not built (0..4)", m);
+    return LH_ERR_ARGUMENT;
+}
+
+// Records of the MFMA form for acceptance degree A and azimuthal degree m.
+int mfma_record_words(int A, int m, int degree, int bands) {
+    switch (A * 8 + m) {
+        case 16: return Record<2, 0>::words(degree, bands);
+        case 17: return Record<2, 1>::words(degree, bands);
+        case 18: return Record<2, 2>::words(degree, bands);
+        case 19: return Record<2, 3>::words(degree, bands);
+        case 20: return Record<2, 4>::words(degree, bands);
+        case 24: return Record<3, 0>::words(degree, bands);
+        case 25: return Record<3, 1>::words(degree, bands);
+        case 26: return Record<3, 2>::words(degree, bands);
+        case 27: return Record<3, 3>::words(degree, bands);
+        case 28: return Record<3, 4>::words(degree, bands);
+        default: return 0;
+    }
+}
+
 template <int A, int F>
 int dispatch_m(int m, const Ge2Params& P, hipStream_t s) {
     switch (m) {
@@ -721,6 +1157,23 @@ extern "C" size_t lh_ge2_workspace_bytes(int bands, int freq, int receivers, int
     if (freq <= 1 || receivers <= 0) return 0;
     const int nsplit = wide_splits(receivers, points);
     return nsplit > 1 ? sizeof(float2) * (size_t)receivers * nsplit * bands * freq : 0;
+}
+
+extern "C" size_t lh_ge2_apply_workspace_bytes(const lh_ge2_pack* pack,
+                                               const lh_ge2_source* source, int receivers,
+                                               int freq) {
+    if (pack == nullptr || source == nullptr || receivers <= 0) return 0;
+    if (pack->coefficient_format != 1)
+        return lh_ge2_workspace_bytes(pack->bands, freq, receivers, source->points);
+    Ge2Params P{};
+    P.nrec = receivers;
+    P.npoints = source->points;
+    P.intervals = pack->intervals;
+    P.bands = pack->bands;
+    P.nfreq = freq;
+    const int m = std::min(source->azimuthal_degree, pack->degree);
+    const int words = mfma_record_words(pack->acceptance_degree, m, pack->degree, pack->bands);
+    return words > 0 ? mfma_layout(P, words, nullptr, nullptr) : 0;
 }
 
 extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source,
@@ -750,12 +1203,12 @@ extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source
                    && out && (source->points == 0 || (source->points_m && source->channels_data
                                                       && source->reference_ns)),
                "lh_ge2_apply: null device pointer");
-    Ge2Params P{};
     LH_REQUIRE(pack->coefficient_format == 0 || pack->coefficient_format == 1,
-               "lh_ge2_apply: coefficient_format must be 0 (complex64) or 1 (bfloat16 pairs)");
-    const bool bf16 = pack->coefficient_format == 1;
-    P.coef = bf16 ? nullptr : static_cast<const float2*>(pack->coefficients);
-    P.coef16 = bf16 ? static_cast<const unsigned*>(pack->coefficients) : nullptr;
+               "lh_ge2_apply: coefficient_format must be 0 (complex64) or 1 (bfloat16 hi + lo, "
+               "MFMA layout)");
+    Ge2Params P{};
+    P.coef = static_cast<const float2*>(pack->coefficients);
+    P.coef16 = nullptr;
     P.log_r = pack->log_r;
     P.absorption = pack->absorption;
     P.speed = pack->speed;
@@ -781,15 +1234,25 @@ extern "C" int lh_ge2_apply(const lh_ge2_pack* pack, const lh_ge2_source* source
     P.nfreq = nfreq;
     P.r_low = radial_low;
     P.r_high = radial_high;
+    const int m = std::min(source->azimuthal_degree, pack->degree);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (pack->coefficient_format == 1) {
+        switch (pack->acceptance_degree) {
+            case 2: return dispatch_mfma<2>(m, P, workspace, workspace_bytes, s);
+            case 3: return dispatch_mfma<3>(m, P, workspace, workspace_bytes, s);
+            default: break;
+        }
+        lh::set_error("lh_ge2_apply: acceptance degree %d not built (2, 3)",
+                      pack->acceptance_degree);
+        return LH_ERR_ARGUMENT;
+    }
     const size_t need = lh_ge2_workspace_bytes(P.bands, nfreq, P.nrec, P.npoints);
     LH_REQUIRE(workspace_bytes >= need && (need == 0 || workspace != nullptr),
                "lh_ge2_apply: workspace of %zu bytes, %zu needed", workspace_bytes, need);
     P.partial = static_cast<float2*>(workspace);
-    const int m = std::min(source->azimuthal_degree, pack->degree);
-    hipStream_t s = static_cast<hipStream_t>(stream);
     switch (pack->acceptance_degree) {
-        case 2: return bf16 ? dispatch_m<2, 1>(m, P, s) : dispatch_m<2, 0>(m, P, s);
-        case 3: return bf16 ? dispatch_m<3, 1>(m, P, s) : dispatch_m<3, 0>(m, P, s);
+        case 2: return dispatch_m<2, 0>(m, P, s);
+        case 3: return dispatch_m<3, 0>(m, P, s);
         default: break;
     }
     lh::set_error("lh_ge2_apply: acceptance degree %d not built (2, 3)", pack->acceptance_degree);

@@ -44,6 +44,8 @@ _lib.SIGNATURES["lh_ge2_apply"] = (ctypes.c_int, [ctypes.POINTER(_Pack), ctypes.
                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                   ctypes.c_void_p])
 _lib.SIGNATURES["lh_ge2_workspace_bytes"] = (ctypes.c_size_t, [ctypes.c_int] * 4)
+_lib.SIGNATURES["lh_ge2_apply_workspace_bytes"] = (ctypes.c_size_t, [
+    ctypes.POINTER(_Pack), ctypes.POINTER(_Source), ctypes.c_int, ctypes.c_int])
 
 def _workspace(device, nbytes, stream=None):
     """A reusable device workspace of at least ``nbytes`` (one per device and
@@ -71,6 +73,35 @@ def workspace_bytes(bands, freq, receivers, points):
     return int(fn(int(bands), int(freq), int(receivers), int(points)))
 
 
+def apply_workspace_bytes(pack_struct, source_struct, receivers, freq):
+    """Device workspace bytes of one ``lh_ge2_apply`` call (either coefficient format)."""
+    lib = _lib.load()
+    fn = lib.lh_ge2_apply_workspace_bytes
+    fn.restype, fn.argtypes = _lib.SIGNATURES["lh_ge2_apply_workspace_bytes"]
+    return int(fn(ctypes.byref(pack_struct), ctypes.byref(source_struct), int(receivers),
+                  int(freq)))
+
+
+def split_mfma_layout(coefficients):
+    """The pack's coefficients (bands, intervals, degree+1, pairs, 4, frequency) complex64
+    as the right operand of the MFMA form: bfloat16 hi + lo (lo the bfloat16 rounding
+    of x - hi) laid out (bands, intervals, degree+1, frequency, pair, [re hi, re lo,
+    im hi, im lo], power), pairs padded with zeros to a multiple of four."""
+    import torch
+    x = torch.view_as_real(coefficients)                       # (.., pair, pw, w, 2)
+    x = x.permute(0, 1, 2, 5, 3, 6, 4).contiguous()           # (.., w, pair, ri, pw)
+    hi = x.to(torch.bfloat16)
+    lo = (x - hi.float()).to(torch.bfloat16)
+    parts = torch.stack([hi, lo], dim=-2)                      # (.., w, pair, ri, hl, pw)
+    npair = parts.shape[4]
+    padded = (npair + 3) // 4 * 4
+    if padded != npair:
+        shape = list(parts.shape)
+        shape[4] = padded - npair
+        parts = torch.cat([parts, parts.new_zeros(shape)], dim=4)
+    return parts.contiguous()
+
+
 def canonical_pairs(acceptance_degree):
     """(lambda, |mu|) pairs in the order of ``CouplingTable.build`` (``directional.py:164-168``)."""
     lam, mu = [], []
@@ -89,11 +120,11 @@ class DevicePack:
     """A folded ``Ge2KernelStore``-format bundle resident in device memory."""
 
     def __init__(self, bundle, device="cuda", coefficient_format="complex64"):
-        """``coefficient_format`` "bfloat16" keeps the coefficients as bfloat16
-        (re, im) pairs, half the bytes (the precision form)."""
+        """``coefficient_format`` "bf16_split" keeps the coefficients as bfloat16 hi + lo
+        in the layout of the MFMA form (the precision form's multi-frequency apply)."""
         import torch
-        if coefficient_format not in ("complex64", "bfloat16"):
-            raise ValueError("coefficient_format must be 'complex64' or 'bfloat16'")
+        if coefficient_format not in ("complex64", "bf16_split"):
+            raise ValueError("coefficient_format must be 'complex64' or 'bf16_split'")
         coefficients = bundle["coefficients"]
         resident = isinstance(coefficients, torch.Tensor)
         if not resident:
@@ -131,18 +162,15 @@ class DevicePack:
         self.d_speed = as_dev(self.speed, torch.float64)
         self.d_omega = as_dev(self.omega, torch.float64)
         self.coefficient_format = coefficient_format
-        width = 8
-        if coefficient_format == "bfloat16":
-            # (re, im) bfloat16 pairs: one 32-bit word per coefficient, re in the low half
-            self.d_coef = torch.view_as_real(self.d_coef).to(torch.bfloat16).contiguous()
-            width = 4
+        if coefficient_format == "bf16_split":
+            self.d_coef = split_mfma_layout(self.d_coef)
         self.struct = _Pack(self.d_coef.data_ptr(), self.d_log_r.data_ptr(),
                             self.d_absorption.data_ptr(), self.d_speed.data_ptr(),
                             self.d_omega.data_ptr(), bands, intervals, self.degree,
                             self.acceptance_degree, nfreq, int(self.flight),
-                            int(coefficient_format == "bfloat16"))
-        self.nbytes = int(np.prod(coefficients.shape)) * width
-        self._band_stride = intervals * ndeg * npair * 4 * nfreq * width
+                            int(coefficient_format == "bf16_split"))
+        self.nbytes = int(self.d_coef.numel() * self.d_coef.element_size())
+        self._band_stride = self.nbytes // bands
 
     def band(self, j):
         """Descriptor of node band ``j`` alone (for sources on different point sets)."""
@@ -251,7 +279,7 @@ def apply_device(pack, source, receivers, alpha, window, *, frequencies=None, ou
         out = torch.empty((nfreq, receivers.count), dtype=torch.complex64,
                           device=source.d_points.device)
     descriptor = pack.struct if band is None else pack.band(int(band))
-    need = workspace_bytes(descriptor.bands, nfreq, receivers.count, source.points)
+    need = apply_workspace_bytes(descriptor, source.struct, receivers.count, nfreq)
     ws = _workspace(source.d_points.device, need, stream) if need else None
     with _lib.stage("ge2_apply" if nfreq > 1 else "ge2_zero"):
         _lib.call("lh_ge2_apply", ctypes.byref(descriptor), ctypes.byref(source.struct),

@@ -182,15 +182,15 @@ class ShowerConfig:
     predictor_safety: float = 2.0
     prompt_delay_ns: float = 20.0
     delayed_cell_ns: float = 20.0
-    #: "fp32": the plain form; "mixed": the precision form (the readout's
-    #: inverse Fourier product in FP16 hi + lo on MFMA); "mixed_bf16": also
-    #: order >=2 from bfloat16 pack coefficients with FP32 accumulation (half
-    #: the pack's bytes; its order >=2 charges stray by a few parts in 10^3)
+    #: "fp32": the plain form; "mixed": the precision form: the multi-frequency
+    #: order >=2 applies as bfloat16 hi + lo products on MFMA and the readout's
+    #: inverse Fourier product as FP16 hi + lo on MFMA, FP32 accumulation both;
+    #: the zero-frequency charges that select OMs stay FP32
     precision: str = "fp32"
 
     def __post_init__(self):
-        if self.precision not in ("fp32", "mixed", "mixed_bf16"):
-            raise ValueError("precision must be 'fp32', 'mixed' or 'mixed_bf16'")
+        if self.precision not in ("fp32", "mixed"):
+            raise ValueError("precision must be 'fp32' or 'mixed'")
         if not 1 <= int(self.band_rank) <= 9 or int(self.fold_subdivide) < 1:
             raise ValueError("band_rank must lie in 1..9 and fold_subdivide be positive")
         if (not self.predictor_safety >= 1 or not self.predictor_cell_m > 0
@@ -800,10 +800,10 @@ class ShowerTransport:
             device_pack = DevicePack(folded, self.device)
             self._tables = dict(pack=pack, fold=fold, folded=folded, device_pack=device_pack,
                                 fold_seconds=perf_counter() - began)
-            if self.config.precision == "mixed_bf16":
-                # order >=2 from bfloat16 coefficients
-                self._tables["device_pack_bf16"] = DevicePack(folded, self.device,
-                                                              coefficient_format="bfloat16")
+            if self.config.precision == "mixed":
+                # the multi-frequency applies of the precision form (MFMA layout)
+                self._tables["device_pack_split"] = DevicePack(folded, self.device,
+                                                               coefficient_format="bf16_split")
         return self._tables
 
     def order0(self):
@@ -938,10 +938,11 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     if extra.shape != (n,) or not np.isfinite(extra).all():
         raise ValueError("selection_extra_charge must hold one finite value per OM")
     fold, folded, pack = tables["fold"], tables["folded"], tables["device_pack"]
-    if config.precision == "mixed_bf16":
+    full_pack = pack
+    if config.precision == "mixed":
         from .ge2 import DevicePack
-        pack = tables.get("device_pack_bf16") or DevicePack(folded, device,
-                                                            coefficient_format="bfloat16")
+        full_pack = tables.get("device_pack_split") or DevicePack(
+            folded, device, coefficient_format="bf16_split")
     if not np.allclose(folded["omega"], omega, rtol=0, atol=1e-12):
         raise ValueError("folded >=2 kernels and route frequencies differ")
     timings["setup_seconds"] = perf_counter() - stage
@@ -1094,13 +1095,13 @@ def transport_shower(kernel, source, *, config=None, tables, time_origin_ns=None
     spectrum = np.zeros((len(omega), n), complex)
     work = np.flatnonzero(active & (used > 0))
     if len(work):
-        spectrum[:, work] = apply_bands(pack, sources, positions[work], looks[work], alpha,
+        spectrum[:, work] = apply_bands(full_pack, sources, positions[work], looks[work], alpha,
                                         area[work], (low, high))
     late_spectra = []
     for shift, group in late_sources:
         values = np.zeros((len(omega), n), complex)
         if len(work):
-            values[:, work] = apply_bands(pack, group, positions[work], looks[work], alpha,
+            values[:, work] = apply_bands(full_pack, group, positions[work], looks[work], alpha,
                                           area[work], (low, high))
         late_spectra.append((shift, values))
     timings["full_apply_seconds"] = perf_counter() - stage
