@@ -728,13 +728,15 @@ int launch(Ge2Params P, hipStream_t stream) {
 typedef short shortx4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kMfmaChunk = 8;   // column tiles of 16 per pass
+constexpr int kMfmaChunk = 4;   // column tiles of 16 per pass
 
 // Device workspace of the MFMA form (laid out by mfma_layout).
 struct MfmaWork {
     int* count;       // (intervals) pairs per interval
     int* offset;      // (intervals + 1) first pair slot of each interval
-    int* cursor;      // (intervals) fill cursors
+    int* cursor;      // (intervals) unused
+    int* count_jd;    // (intervals, npoints) pairs per (interval, point)
+    int* offset_jd;   // (intervals, npoints) first slot of each (interval, point)
     int* tile_off;    // (intervals + 1) first tile of each interval
     int* totals;      // [pairs, tiles]
     unsigned char* valid;   // (nrec, npoints) pair inside the radial window
@@ -753,6 +755,8 @@ __host__ size_t mfma_layout(const Ge2Params& P, int words, char* base, MfmaWork*
     auto take = [&](size_t bytes) { const size_t here = at; at += align256(bytes); return here; };
     const size_t o_count = take(sizeof(int) * J), o_offset = take(sizeof(int) * (J + 1));
     const size_t o_cursor = take(sizeof(int) * J), o_tile = take(sizeof(int) * (J + 1));
+    const size_t o_count_jd = take(sizeof(int) * (size_t)J * P.npoints);
+    const size_t o_offset_jd = take(sizeof(int) * (size_t)J * P.npoints);
     const size_t o_totals = take(sizeof(int) * 2), o_valid = take(pairs);
     const size_t o_list = take(sizeof(int) * pairs);
     const size_t o_records = take(sizeof(float) * pairs * words);
@@ -761,6 +765,8 @@ __host__ size_t mfma_layout(const Ge2Params& P, int words, char* base, MfmaWork*
         W->count = reinterpret_cast<int*>(base + o_count);
         W->offset = reinterpret_cast<int*>(base + o_offset);
         W->cursor = reinterpret_cast<int*>(base + o_cursor);
+        W->count_jd = reinterpret_cast<int*>(base + o_count_jd);
+        W->offset_jd = reinterpret_cast<int*>(base + o_offset_jd);
         W->tile_off = reinterpret_cast<int*>(base + o_tile);
         W->totals = reinterpret_cast<int*>(base + o_totals);
         W->valid = reinterpret_cast<unsigned char*>(base + o_valid);
@@ -789,50 +795,85 @@ __device__ bool pair_interval(const Ge2Params& P, int d, int p, int& j) {
     return true;
 }
 
-// Clears the interval counters (a kernel, so the sequence can be captured).
-__global__ void ge2_mfma_init(Ge2Params P, MfmaWork W) {
-    for (int i = threadIdx.x; i < P.intervals; i += blockDim.x) {
-        W.count[i] = 0;
-        W.cursor[i] = 0;
-    }
-}
-
+// One workgroup per source point: the window flag of each OM and the point's
+// pairs per radial interval (LDS histogram), stored interval-major.
 __global__ void ge2_mfma_count(Ge2Params P, MfmaWork W) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)P.nrec * P.npoints) return;
-    const int d = (int)(idx / P.npoints), p = (int)(idx - (size_t)d * P.npoints);
-    int j = 0;
-    const bool inside = pair_interval(P, d, p, j);
-    W.valid[idx] = inside ? 1 : 0;
-    if (inside) atomicAdd(&W.count[j], 1);
-}
-
-// Pair and tile offsets of every interval (one thread: a few hundred intervals).
-__global__ void ge2_mfma_scan(Ge2Params P, MfmaWork W, int per_tile) {
-    if (threadIdx.x != 0) return;
-    int pairs = 0, tiles = 0;
-    for (int j = 0; j < P.intervals; ++j) {
-        W.offset[j] = pairs;
-        W.tile_off[j] = tiles;
-        pairs += W.count[j];
-        tiles += (W.count[j] + per_tile - 1) / per_tile;
+    extern __shared__ int hist[];
+    const int p = blockIdx.x;
+    for (int j = threadIdx.x; j < P.intervals; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    for (int d = threadIdx.x; d < P.nrec; d += blockDim.x) {
+        int j = 0;
+        const bool inside = pair_interval(P, d, p, j);
+        W.valid[(size_t)d * P.npoints + p] = inside ? 1 : 0;
+        if (inside) atomicAdd(&hist[j], 1);
     }
-    W.offset[P.intervals] = pairs;
-    W.tile_off[P.intervals] = tiles;
-    W.totals[0] = pairs;
-    W.totals[1] = tiles;
+    __syncthreads();
+    for (int j = threadIdx.x; j < P.intervals; j += blockDim.x)
+        W.count_jd[(size_t)j * P.npoints + p] = hist[j];
 }
 
-// Pair list grouped by interval. The order inside an interval follows the
-// atomics; it changes which pairs share a tile, never a pair's result.
+// Exclusive scan of the (interval, point) counts in interval-major order (one
+// workgroup of 1024 threads, each over a contiguous run), then the pair and
+// unit offsets of every interval.
+__global__ __launch_bounds__(1024) void ge2_mfma_scan(Ge2Params P, MfmaWork W, int per_unit) {
+    __shared__ int sums[1024];
+    const int n = P.intervals * P.npoints;
+    const int t = threadIdx.x;
+    const int run = (n + 1023) / 1024;
+    const int lo = min(n, t * run), hi = min(n, lo + run);
+    int local = 0;
+    for (int i = lo; i < hi; ++i) local += W.count_jd[i];
+    sums[t] = local;
+    __syncthreads();
+    for (int step = 1; step < 1024; step <<= 1) {   // inclusive Hillis-Steele scan
+        const int add = t >= step ? sums[t - step] : 0;
+        __syncthreads();
+        sums[t] += add;
+        __syncthreads();
+    }
+    int running = sums[t] - local;
+    for (int i = lo; i < hi; ++i) {
+        W.offset_jd[i] = running;
+        running += W.count_jd[i];
+    }
+    __syncthreads();
+    if (t == 0) {
+        int units = 0;
+        for (int j = 0; j < P.intervals; ++j) {
+            const int begin = W.offset_jd[(size_t)j * P.npoints];
+            const int end = j + 1 < P.intervals ? W.offset_jd[(size_t)(j + 1) * P.npoints]
+                                                : sums[1023];
+            W.offset[j] = begin;
+            W.count[j] = end - begin;
+            W.tile_off[j] = units;
+            units += (end - begin + per_unit - 1) / per_unit;
+        }
+        W.offset[P.intervals] = sums[1023];
+        W.tile_off[P.intervals] = units;
+        W.totals[0] = sums[1023];
+        W.totals[1] = units;
+    }
+}
+
+// One workgroup per source point: its pairs into the list, grouped by
+// interval, points in order inside an interval, so that neighbouring pairs
+// share the point's channels. The order of one point's OMs inside an interval
+// follows LDS atomics; it changes which pairs share a tile, never a pair's
+// result.
 __global__ void ge2_mfma_fill(Ge2Params P, MfmaWork W) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)P.nrec * P.npoints || !W.valid[idx]) return;
-    const int d = (int)(idx / P.npoints), p = (int)(idx - (size_t)d * P.npoints);
-    int j = 0;
-    pair_interval(P, d, p, j);
-    const int slot = W.offset[j] + atomicAdd(&W.cursor[j], 1);
-    W.list[slot] = (int)idx;
+    extern __shared__ int cursor[];
+    const int p = blockIdx.x;
+    for (int j = threadIdx.x; j < P.intervals; j += blockDim.x)
+        cursor[j] = W.offset_jd[(size_t)j * P.npoints + p];
+    __syncthreads();
+    for (int d = threadIdx.x; d < P.nrec; d += blockDim.x) {
+        const size_t pair = (size_t)d * P.npoints + p;
+        if (!W.valid[pair]) continue;
+        int j = 0;
+        pair_interval(P, d, p, j);
+        W.list[atomicAdd(&cursor[j], 1)] = (int)pair;
+    }
 }
 
 // FP64 geometry of every listed pair into its record (the wide kernel's LDS
@@ -876,13 +917,15 @@ __global__ void ge2_mfma_records(Ge2Params P, MfmaWork W) {
     }
 }
 
-__device__ __forceinline__ unsigned short bf16_bits(float x) {
-    const unsigned u = __float_as_uint(x);
-    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);   // nearest even
-}
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float bf16_value(unsigned short h) {
-    return __uint_as_float((unsigned)h << 16);
+// q = hi + lo, both bfloat16 rounded to nearest even (v_cvt_pk_bf16_f32); the
+// remainder of the pair is below 2^-16 of q.
+__device__ __forceinline__ void split_bf16(floatx4 q, shortx4& hi, shortx4& lo) {
+    const bf16x4 h = __builtin_convertvector(q, bf16x4);
+    const floatx4 rest = q - __builtin_convertvector(h, floatx4);
+    hi = __builtin_bit_cast(shortx4, h);
+    lo = __builtin_bit_cast(shortx4, __builtin_convertvector(rest, bf16x4));
 }
 
 // Canonical pair k -> (lambda, mu): lambda = 0..A, mu = 0..lambda.
@@ -893,13 +936,23 @@ __device__ __forceinline__ int pair_mu(int k) {
     return k - lam * (lam + 1) / 2;
 }
 
+// Rows per pair (a power of two >= 2M + 1), pairs per 16-row tile, tiles per
+// wave unit (they share every load of the right operand).
+template <int M> struct MfmaShape {
+    static constexpr int NM = 2 * M + 1;
+    static constexpr int RPP = NM == 1 ? 1 : (NM <= 4 ? 4 : (NM <= 8 ? 8 : 16));
+    static constexpr int PPT = 16 / RPP;
+    static constexpr int MT = 2;
+    static constexpr int UNIT = PPT * MT;
+};
+
 template <int A, int M>
-__global__ __launch_bounds__(256) void ge2_mfma_apply(Ge2Params P, MfmaWork W,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ge2_mfma_apply(Ge2Params P, MfmaWork W,
                                                        const shortx4* __restrict__ B) {
     constexpr int NA = 2 * A + 1, NM = 2 * M + 1, NPAIR = (A + 1) * (A + 2) / 2;
     constexpr int KQ = (NPAIR + 3) / 4 * 4, KSTEPS = KQ / 4;
-    constexpr int RPP = NM == 1 ? 1 : (NM <= 4 ? 4 : (NM <= 8 ? 8 : 16));   // rows per pair
-    constexpr int PPT = 16 / RPP;                // pairs per 16-row tile
+    using S = MfmaShape<M>;
+    constexpr int RPP = S::RPP, PPT = S::PPT, MT = S::MT, UNIT = S::UNIT;
     constexpr int SLOTS = RPP == 1 ? 4 : 1;      // pairs one lane's four result rows hold
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -907,129 +960,188 @@ __global__ __launch_bounds__(256) void ge2_mfma_apply(Ge2Params P, MfmaWork W,
     const int column = lane & 15;
     const int ncol = P.bands * P.nfreq;
     const int ntile_col = (ncol + 15) / 16;
-    const int total_tiles = W.totals[1];
+    const int nchunk = (ntile_col + kMfmaChunk - 1) / kMfmaChunk;
+    const int total_items = W.totals[1] * nchunk;
     const int J = P.intervals;
-    // operand A: this lane's row of the tile
-    const int arow_q = column / RPP, arow_b = column - arow_q * RPP;
-    // result rows of this lane: 4 * group + i
-    const int first_q = (4 * group) / RPP;
+    const int arow_q = column / RPP, arow_b = column - arow_q * RPP;   // operand A row
+    const int first_q = (4 * group) / RPP;                             // result rows
+    const size_t point_stride = (size_t)P.nchan * P.chstride;
+    // operand A of this lane: pair k = 4 s + group of row arow_b, read from
+    // clamped addresses and masked
+    const int arow_bc = arow_b < NM ? arow_b : NM - 1;
+    int kpos[KSTEPS], epos[KSTEPS], eneg[KSTEPS], kmu[KSTEPS];
+    bool kok[KSTEPS];
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) {
+        const int k = 4 * s + group;
+        kok[s] = k < NPAIR;
+        const int kc = kok[s] ? k : 0;
+        const int mu = pair_mu<A>(kc);
+        kmu[s] = mu;
+        kpos[s] = 6 + 2 * kc;
+        epos[s] = (A + mu) * NM + arow_bc;
+        eneg[s] = (A - mu) * NM + arow_bc;
+    }
+    const int e_first = 6 + 2 * NPAIR + P.bands;
 
-    for (int tile = blockIdx.x * 4 + wave; tile < total_tiles; tile += gridDim.x * 4) {
-        // interval of the tile: the last j with tile_off[j] <= tile
+    // A work item is one unit of pairs and one chunk of column tiles; the
+    // chunks of a unit go to neighbouring waves, which share its records and
+    // channels in cache.
+    for (int item = blockIdx.x * 4 + wave; item < total_items; item += gridDim.x * 4) {
+        const int unit = item / nchunk;
+        const int c0 = (item - unit * nchunk) * kMfmaChunk;
+        const int nct = min(kMfmaChunk, ntile_col - c0);
+        // interval of the unit: the last j with tile_off[j] <= unit
         int lo = 0, hi = J;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (W.tile_off[mid] <= tile) lo = mid; else hi = mid;
+            if (W.tile_off[mid] <= unit) lo = mid; else hi = mid;
         }
         const int j = lo;
-        const int first = W.offset[j] + (tile - W.tile_off[j]) * PPT;
-        const int npairs = min(PPT, W.offset[j] + W.count[j] - first);
-        const bool arow_ok = arow_q < npairs;
-        const float* arec = W.records + (size_t)(first + (arow_ok ? arow_q : 0)) * P.record_words;
-        const float at = arec[2];
-        float tpow[4] = {at * at * at, at * at, at, 1.f};
+        const int first = W.offset[j] + (unit - W.tile_off[j]) * UNIT;
+        const int npairs = min(UNIT, W.offset[j] + W.count[j] - first);
+        // operand A rows (one pair per tile and lane) and, for each result row
+        // of this lane, the offset of its point's channels (clamped to a valid
+        // address; rows without a pair or beyond 2M + 1 are masked)
+        const float* arec[MT];
+        bool arow_ok[MT];
+        floatx4 tpow[MT];
+        size_t chrow[MT][4];
+        bool row_ok[MT][4];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int qa = mt * PPT + arow_q;
+            arow_ok[mt] = qa < npairs && arow_b < NM;
+            arec[mt] = W.records + (size_t)(first + (qa < npairs ? qa : 0)) * P.record_words;
+            const float t = arec[mt][2];
+            tpow[mt] = floatx4{t * t * t, t * t, t, 1.f};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 4 * group + i;
+                const int q = mt * PPT + row / RPP;
+                const int b = row % RPP;
+                const bool ok = q < npairs && b < NM;
+                const int pt = ok ? W.list[first + q] % P.npoints : 0;
+                row_ok[mt][i] = ok;
+                chrow[mt][i] = (size_t)pt * point_stride;
+            }
+        }
 
-        for (int c0 = 0; c0 < ntile_col; c0 += kMfmaChunk) {
-            const int nct = min(kMfmaChunk, ntile_col - c0);
-            float2 tot[kMfmaChunk][SLOTS];
+        float2 tot[MT][kMfmaChunk][SLOTS];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int c = 0; c < kMfmaChunk; ++c)
 #pragma unroll
-                for (int s = 0; s < SLOTS; ++s) tot[c][s] = make_float2(0.f, 0.f);
-            int starts = 0;
-            for (int l = 0; l <= P.degree; ++l) {
-                const int top_p = l < A ? l : A;
-                const int keep = l < M ? l : M;
-                const int centre = starts + keep;
-                starts += 2 * (l < P.src_az ? l : P.src_az) + 1;
-                // Q_l fragments of this lane (row arow_q, b; K = 4 s + group, power i)
-                shortx4 qhi[KSTEPS], qlo[KSTEPS];
+                for (int s = 0; s < SLOTS; ++s) tot[mt][c][s] = make_float2(0.f, 0.f);
+        int starts = 0;
+        for (int l = 0; l <= P.degree; ++l) {
+            const int top_p = l < A ? l : A;
+            const int keep = l < M ? l : M;
+            const int centre = starts + keep;
+            starts += 2 * (l < P.src_az ? l : P.src_az) + 1;
+            // Q_l fragments (row arow of every tile; K = 4 s + group, power i)
+            const bool row_deg = arow_b - M >= -keep && arow_b - M <= keep;
+            shortx4 qhi[MT][KSTEPS], qlo[MT][KSTEPS];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const float* E = arec[mt] + e_first + l * NA * NM;
 #pragma unroll
                 for (int s = 0; s < KSTEPS; ++s) {
-                    const int k = 4 * s + group;
-                    float base = 0.f;
-                    const int bb = arow_b - M;
-                    if (arow_ok && arow_b < NM && k < NPAIR && bb >= -keep && bb <= keep) {
-                        const int mu = pair_mu<A>(k);
-                        if (mu <= top_p) {
-                            const float* E = arec + 6 + 2 * NPAIR + P.bands + l * NA * NM;
-                            base = E[(A + mu) * NM + arow_b] * arec[6 + 2 * k];
-                            if (mu) base += E[(A - mu) * NM + arow_b] * arec[7 + 2 * k];
-                        }
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float q = base * tpow[i];
-                        const unsigned short h = bf16_bits(q);
-                        qhi[s][i] = (short)h;
-                        qlo[s][i] = (short)bf16_bits(q - bf16_value(h));
-                    }
-                }
-                for (int c = 0; c < nct; ++c) {
-                    const int col = 16 * (c0 + c) + column;
-                    const bool col_ok = col < ncol;
-                    const int lam = col_ok ? col / P.nfreq : 0;
-                    const int w = col_ok ? col - lam * P.nfreq : 0;
-                    floatx4 hre = {0.f, 0.f, 0.f, 0.f}, him = {0.f, 0.f, 0.f, 0.f};
-                    const size_t bcol = ((((size_t)lam * J + j) * (P.degree + 1) + l) * P.cstride
-                                         + w) * KQ;
-#pragma unroll
-                    for (int s = 0; s < KSTEPS; ++s) {
-                        shortx4 b_re_hi = {0, 0, 0, 0}, b_re_lo = {0, 0, 0, 0};
-                        shortx4 b_im_hi = {0, 0, 0, 0}, b_im_lo = {0, 0, 0, 0};
-                        if (col_ok) {
-                            const shortx4* src = B + (bcol + 4 * s + group) * 4;
-                            b_re_hi = src[0];
-                            b_re_lo = src[1];
-                            b_im_hi = src[2];
-                            b_im_lo = src[3];
-                        }
-                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_re_hi, hre, 0, 0, 0);
-                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_re_lo, hre, 0, 0, 0);
-                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qlo[s], b_re_hi, hre, 0, 0, 0);
-                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_im_hi, him, 0, 0, 0);
-                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[s], b_im_lo, him, 0, 0, 0);
-                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qlo[s], b_im_hi, him, 0, 0, 0);
-                    }
-                    if (!col_ok) continue;
-                    // epilogue: H of this lane's rows times the point's channels
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int row = 4 * group + i;
-                        const int q = row / RPP, b = row - q * RPP;
-                        const int bb = b - M;
-                        if (q >= npairs || b >= NM || bb < -keep || bb > keep) continue;
-                        const float* rec = W.records + (size_t)(first + q) * P.record_words;
-                        const int p = reinterpret_cast<const int*>(rec)[0] % P.npoints;
-                        const float2 ch = P.channels[(((size_t)lam * P.npoints + p) * P.nchan
-                                                      + centre + bb) * P.chstride + w];
-                        const int slot = q - first_q;
-#pragma unroll
-                        for (int s = 0; s < SLOTS; ++s) {
-                            if (s != slot) continue;
-                            tot[c][s].x += ch.x * hre[i] - ch.y * him[i];
-                            tot[c][s].y += ch.x * him[i] + ch.y * hre[i];
-                        }
-                    }
+                    const float pos = E[epos[s]] * arec[mt][kpos[s]];
+                    const float neg = E[eneg[s]] * arec[mt][kpos[s] + 1];
+                    const bool ok = arow_ok[mt] && kok[s] && row_deg && kmu[s] <= top_p;
+                    const float base = ok ? (kmu[s] ? pos + neg : pos) : 0.f;
+                    split_bf16(base * tpow[mt], qhi[mt][s], qlo[mt][s]);
                 }
             }
-            // sum the four row quarters of every pair, then scale and phase
-            for (int c = 0; c < nct; ++c) {
+            // rows of this degree: |b - M| <= min(l, M); every row reads a
+            // channel of this degree's block (clamped), masked when unused
+            bool use[MT][4];
+            size_t chcol[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int bb = (4 * group + i) % RPP - M;
+                const int inside = bb < -keep ? -keep : (bb > keep ? keep : bb);
+                chcol[i] = (size_t)(centre + inside) * P.chstride;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    use[mt][i] = row_ok[mt][i] && bb >= -keep && bb <= keep;
+            }
+#pragma unroll
+            for (int c = 0; c < kMfmaChunk; ++c) {
+                if (c >= nct) break;
                 const int col = 16 * (c0 + c) + column;
                 const bool col_ok = col < ncol;
                 const int lam = col_ok ? col / P.nfreq : 0;
                 const int w = col_ok ? col - lam * P.nfreq : 0;
+                // channels of every result row, issued before the products
+                const float2* chl = P.channels + (size_t)lam * P.npoints * point_stride + w;
+                float2 ch[MT][4];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) ch[mt][i] = chl[chrow[mt][i] + chcol[i]];
+                // operand B: loaded once, used by every tile of the unit
+                shortx4 b_re_hi[KSTEPS], b_re_lo[KSTEPS], b_im_hi[KSTEPS], b_im_lo[KSTEPS];
+                const shortx4* bl = B + ((((((size_t)lam * J + j) * (P.degree + 1) + l) * P.cstride
+                                          + w) * KQ) + group) * 4;
+#pragma unroll
+                for (int s = 0; s < KSTEPS; ++s) {
+                    const shortx4* src = bl + 16 * s;
+                    b_re_hi[s] = src[0];
+                    b_re_lo[s] = src[1];
+                    b_im_hi[s] = src[2];
+                    b_im_lo[s] = src[3];
+                }
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    if (mt * PPT >= npairs) break;
+                    floatx4 hre = {0.f, 0.f, 0.f, 0.f}, him = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int s = 0; s < KSTEPS; ++s) {
+                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[mt][s], b_re_hi[s], hre, 0, 0, 0);
+                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[mt][s], b_im_hi[s], him, 0, 0, 0);
+                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[mt][s], b_re_lo[s], hre, 0, 0, 0);
+                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qhi[mt][s], b_im_lo[s], him, 0, 0, 0);
+                        hre = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qlo[mt][s], b_re_hi[s], hre, 0, 0, 0);
+                        him = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(qlo[mt][s], b_im_hi[s], him, 0, 0, 0);
+                    }
+                    // epilogue: H of this lane's rows times the point's channels
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int s = RPP == 1 ? i : 0;
+                        const float2 v = use[mt][i] ? ch[mt][i] : make_float2(0.f, 0.f);
+                        tot[mt][c][s].x += v.x * hre[i] - v.y * him[i];
+                        tot[mt][c][s].y += v.x * him[i] + v.y * hre[i];
+                    }
+                }
+            }
+        }
+        // sum the four row quarters of every pair, then scale and phase
+#pragma unroll
+        for (int c = 0; c < kMfmaChunk; ++c) {
+            if (c >= nct) break;
+            const int col = 16 * (c0 + c) + column;
+            const bool col_ok = col < ncol;
+            const int lam = col_ok ? col / P.nfreq : 0;
+            const int w = col_ok ? col - lam * P.nfreq : 0;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                if (mt * PPT >= npairs) break;
+#pragma unroll
                 for (int q = 0; q < PPT; ++q) {
                     float2 v = make_float2(0.f, 0.f);
 #pragma unroll
                     for (int s = 0; s < SLOTS; ++s)
-                        if (q - first_q == s) v = tot[c][s];
+                        if (q - first_q == s) v = tot[mt][c][s];
                     v.x += __shfl_xor(v.x, 16, 64);
                     v.y += __shfl_xor(v.y, 16, 64);
                     v.x += __shfl_xor(v.x, 32, 64);
                     v.y += __shfl_xor(v.y, 32, 64);
-                    if (group != 0 || !col_ok || q >= npairs) continue;
-                    const float* rec = W.records + (size_t)(first + q) * P.record_words;
+                    const int qq = mt * PPT + q;
+                    if (group != 0 || !col_ok || qq >= npairs) continue;
+                    const float* rec = W.records + (size_t)(first + qq) * P.record_words;
                     const int idx = reinterpret_cast<const int*>(rec)[0];
                     const double radius = reinterpret_cast<const double*>(rec)[2];
                     const float scale = rec[6 + 2 * NPAIR + lam];
@@ -1045,27 +1157,47 @@ __global__ __launch_bounds__(256) void ge2_mfma_apply(Ge2Params P, MfmaWork W,
     }
 }
 
-// out[w, d] = sum_band e^{i w ref_band} sum_p partial[d, p, band, w], points in order.
+// out[w, d] = sum_band e^{i w ref_band} sum_p partial[d, p, band, w], points in
+// order: one workgroup per OM, a thread per (band, frequency) column, then the
+// bands of every frequency in order.
 __global__ void ge2_mfma_reduce(Ge2Params P, MfmaWork W) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= P.nrec * P.nfreq) return;
-    const int d = idx / P.nfreq, w = idx - d * P.nfreq;
+    extern __shared__ float2 band_sum[];
+    const int d = blockIdx.x;
     const int ncol = P.bands * P.nfreq;
-    float2 sum = make_float2(0.f, 0.f);
-    for (int lam = 0; lam < P.bands; ++lam) {
-        float2 band = make_float2(0.f, 0.f);
-        for (int p = 0; p < P.npoints; ++p) {
-            const size_t pair = (size_t)d * P.npoints + p;
-            if (!W.valid[pair]) continue;
-            const float2 v = W.partial[pair * ncol + lam * P.nfreq + w];
-            band.x += v.x;
-            band.y += v.y;
+    const unsigned char* valid = W.valid + (size_t)d * P.npoints;
+    const float2* part = W.partial + (size_t)d * P.npoints * ncol;
+    for (int col = threadIdx.x; col < ncol; col += blockDim.x) {
+        float2 sum = make_float2(0.f, 0.f);
+        int p = 0;
+        for (; p + 4 <= P.npoints; p += 4) {
+            float2 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                v[k] = valid[p + k] ? part[(size_t)(p + k) * ncol + col] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                sum.x += v[k].x;
+                sum.y += v[k].y;
+            }
         }
-        const float2 v = cmul(band, phase_of(P.omega[w] * P.band_ref[lam]));
-        sum.x += v.x;
-        sum.y += v.y;
+        for (; p < P.npoints; ++p) {
+            if (!valid[p]) continue;
+            const float2 v = part[(size_t)p * ncol + col];
+            sum.x += v.x;
+            sum.y += v.y;
+        }
+        band_sum[col] = sum;
     }
-    P.out[(size_t)w * P.nrec + d] = sum;
+    __syncthreads();
+    for (int w = threadIdx.x; w < P.nfreq; w += blockDim.x) {
+        float2 sum = make_float2(0.f, 0.f);
+        for (int lam = 0; lam < P.bands; ++lam) {
+            const float2 v = cmul(band_sum[lam * P.nfreq + w], phase_of(P.omega[w] * P.band_ref[lam]));
+            sum.x += v.x;
+            sum.y += v.y;
+        }
+        P.out[(size_t)w * P.nrec + d] = sum;
+    }
 }
 
 template <int A, int M>
@@ -1084,16 +1216,16 @@ int launch_mfma(Ge2Params P, const void* workspace, size_t workspace_bytes, hipS
         return lh::check_launch("lh_ge2_apply (mfma zero)");
     }
     const unsigned grid_pairs = (unsigned)((pairs + 255) / 256);
-    hipLaunchKernelGGL(ge2_mfma_init, dim3(1), dim3(256), 0, stream, P, W);
-    hipLaunchKernelGGL(ge2_mfma_count, dim3(grid_pairs), dim3(256), 0, stream, P, W);
-    constexpr int RPP = NM == 1 ? 1 : (NM <= 4 ? 4 : (NM <= 8 ? 8 : 16));
-    hipLaunchKernelGGL(ge2_mfma_scan, dim3(1), dim3(64), 0, stream, P, W, 16 / RPP);
-    hipLaunchKernelGGL(ge2_mfma_fill, dim3(grid_pairs), dim3(256), 0, stream, P, W);
+    const size_t lds_j = sizeof(int) * (size_t)P.intervals;
+    hipLaunchKernelGGL(ge2_mfma_count, dim3(P.npoints), dim3(256), lds_j, stream, P, W);
+    hipLaunchKernelGGL(ge2_mfma_scan, dim3(1), dim3(1024), 0, stream, P, W, MfmaShape<M>::UNIT);
+    hipLaunchKernelGGL(ge2_mfma_fill, dim3(P.npoints), dim3(256), lds_j, stream, P, W);
     hipLaunchKernelGGL((ge2_mfma_records<A, M>), dim3(grid_pairs), dim3(256), 0, stream, P, W);
     int status = lh::check_launch("lh_ge2_apply (mfma set-up)");
     if (status != LH_OK) return status;
-    const size_t max_tiles = (pairs + 16 / RPP - 1) / (16 / RPP) + P.intervals;
-    const unsigned groups = (unsigned)std::min<size_t>((max_tiles + 3) / 4, 2048);
+    const size_t max_tiles = (pairs + MfmaShape<M>::UNIT - 1) / MfmaShape<M>::UNIT + P.intervals;
+    const size_t chunks = ((size_t)P.bands * P.nfreq + 16 * kMfmaChunk - 1) / (16 * kMfmaChunk);
+    const unsigned groups = (unsigned)std::min<size_t>((max_tiles * chunks + 3) / 4, 4096);
     {
         lh::KernelMark mark("ge2_mfma_apply", stream);
         hipLaunchKernelGGL((ge2_mfma_apply<A, M>), dim3(groups), dim3(256), 0, stream, P, W,
@@ -1102,7 +1234,8 @@ int launch_mfma(Ge2Params P, const void* workspace, size_t workspace_bytes, hipS
     status = lh::check_launch("lh_ge2_apply (mfma)");
     if (status != LH_OK) return status;
     const int total = P.nrec * P.nfreq;
-    hipLaunchKernelGGL(ge2_mfma_reduce, dim3((total + 255) / 256), dim3(256), 0, stream, P, W);
+    hipLaunchKernelGGL(ge2_mfma_reduce, dim3(P.nrec), dim3(256),
+                       sizeof(float2) * (size_t)P.bands * P.nfreq, stream, P, W);
     return lh::check_launch("lh_ge2_apply (mfma reduce)");
 }
 
