@@ -28,12 +28,48 @@ import numpy as np
 
 
 def footprint_bytes(source, oms):
-    """Conservative device bytes of one event: the source upload and compile,
-    the order-1 pencils and (pencil, OM) sort for up to ``oms`` OMs, and the
-    order >=2 spectra; a fixed 256 MiB covers the per-stream workspaces."""
+    """Prior estimate of one event's device bytes before any event has been
+    measured: the source upload and compile, the order-1 pencils and (pencil,
+    OM) sort for up to ``oms`` OMs, and the order >=2 spectra; a fixed 256 MiB
+    covers the per-stream workspaces."""
     elements = getattr(source, "elements", source)
     n = len(elements)
     return int(256 * 2 ** 20 + 2048 * n + 64 * n * min(oms, 512))
+
+
+class FootprintModel:
+    """Device bytes an event needs, learned from the events served so far.
+
+    Each worker process reports the peak device memory it allocated during an
+    event (the caching allocator's high-water mark above its level before the
+    call). The model keeps the smallest peak seen (the fixed part: workspaces
+    sized by the tables and the OM count) and the largest excess per element
+    above it, and admits an event of ``n`` elements as ``margin * (fixed +
+    per_element * n)``; before the first report it uses :func:`footprint_bytes`.
+    """
+
+    def __init__(self, oms, margin=1.25):
+        self.oms = int(oms)
+        self.margin = float(margin)
+        self.fixed = None
+        self.per_element = 0.0
+        self.samples = 0
+        self._lock = threading.Lock()
+
+    def need(self, elements):
+        n = len(elements)
+        with self._lock:
+            if self.fixed is None:
+                return footprint_bytes(elements, self.oms)
+            return int(self.margin * (self.fixed + self.per_element * n))
+
+    def record(self, elements, peak_bytes):
+        n = max(len(elements), 1)
+        with self._lock:
+            self.samples += 1
+            if self.fixed is None or peak_bytes < self.fixed:
+                self.fixed = float(peak_bytes)
+            self.per_element = max(self.per_element, (peak_bytes - self.fixed) / n)
 
 
 class ShowerServer:
@@ -216,15 +252,25 @@ def _process_main(factory, device, threads, tasks, results, slot, memory_fractio
             continue
         try:
             source = staged[payload.key] if isinstance(payload, StagedSource) else payload
+            retries = 0
+            base = 0
+            if device is not None:
+                torch.cuda.reset_peak_memory_stats()
+                base = torch.cuda.memory_allocated()
             try:
                 response = route(source, **call_kwargs)
             except out_of_memory:
+                # counted and reported: the admission model underestimated this event
                 from . import _lib
+                retries = 1
                 _lib.release_workspaces()
                 torch.cuda.empty_cache()
                 response = route(source, **call_kwargs)
             if drain is not None:
                 drain()
+            peak = (torch.cuda.max_memory_allocated() - base) if device is not None else 0
+            response.metadata["out_of_memory_retries"] = retries
+            response.metadata["device_peak_bytes"] = int(peak)
             results.put(("done", slot, key, _pack_response(response)))
         except BaseException as error:
             results.put(("error", slot, key, (type(error).__name__, str(error))))
@@ -237,10 +283,14 @@ class ProcessShowerServer:
     ``factory`` is a picklable callable that returns the route; each process
     calls it once, so every process holds the tables resident in HBM and uses
     at most ``memory_fraction`` of the device memory (default: an equal share
-    of 90% among the workers and the caller). An
-    event is sent to a free process when its estimated footprint fits the
-    memory budget beside the events already running; the response comes back
-    as arrays. ``submit`` returns a ``Future`` as ``ShowerServer.submit`` does.
+    of 90% among the workers and the caller). An event is sent to a free
+    process when its footprint (:class:`FootprintModel`, learned from the
+    workers' measured peaks) fits the budget (the workers' shares together)
+    beside the events already running; the response comes back as arrays.
+    ``stats`` counts served events, out-of-memory retries (an event that ran
+    out of its worker's share runs once more after the worker drops its
+    cached workspaces; the response's ``metadata["out_of_memory_retries"]``
+    says so) and the largest peak. ``submit`` returns a ``Future`` as ``ShowerServer.submit`` does.
     Sources travel as host element arrays (a ``ResidentShower`` is sent as its
     host elements), or are staged ahead in device memory with ``stage``.
     """
@@ -254,20 +304,24 @@ class ProcessShowerServer:
             raise ValueError("processes must be positive")
         self.detector = detector
         self.device = device
+        # each worker's share of the device memory (the caller keeps one share)
+        fraction = memory_fraction if memory_fraction is not None else 0.9 / (processes + 1)
         if memory_budget_bytes is None:
             if device is None:
                 raise ValueError("a host-only server needs memory_budget_bytes")
             import torch
-            free, _ = torch.cuda.mem_get_info(torch.device(device))
-            memory_budget_bytes = int(0.5 * free)
+            _, total = torch.cuda.mem_get_info(torch.device(device))
+            # every worker holds at most its share: events in flight together
+            # must fit the workers' shares together
+            memory_budget_bytes = int(fraction * total * processes)
         self.budget = int(memory_budget_bytes)
+        self.footprint = FootprintModel(len(detector))
+        self.stats = {"events": 0, "out_of_memory_retries": 0, "largest_peak_bytes": 0}
         share = int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
         threads = threads_per_process or max(1, share // (processes + 1))
         context = mp.get_context("spawn")
         self._results = context.Queue()
         self._tasks = [context.Queue() for _ in range(processes)]
-        # each worker's share of the device memory (the caller keeps one share)
-        fraction = memory_fraction if memory_fraction is not None else 0.9 / (processes + 1)
         self._processes = [context.Process(target=_process_main,
                                            args=(factory, device, threads, self._tasks[i],
                                                  self._results, i, fraction),
@@ -311,8 +365,7 @@ class ProcessShowerServer:
         future = Future()
         future.submitted = time.perf_counter()
         elements = source if isinstance(source, StagedSource) else _elements_of(source)
-        need = footprint_bytes(elements, len(self.detector))
-        self._waiting.put((future, elements, call_kwargs, need))
+        self._waiting.put((future, elements, call_kwargs, None))
         return future
 
     def stage(self, source):
@@ -353,7 +406,9 @@ class ProcessShowerServer:
             item = self._waiting.get()
             if item is None:
                 return
-            future, elements, call_kwargs, need = item
+            future, elements, call_kwargs, _ = item
+            # the model at dispatch time (it learns from the events finished so far)
+            need = self.footprint.need(elements)
             with self._lock:
                 self._idle = [i for i in self._idle if self._processes[i].is_alive()]
                 while not self._idle or (self._inflight > 0
@@ -370,7 +425,7 @@ class ProcessShowerServer:
                 self._inflight += need
                 key = self._next_key
                 self._next_key += 1
-                self._pending[key] = (future, need, slot)
+                self._pending[key] = (future, need, slot, len(elements))
             self._tasks[slot].put(("run", key, elements, call_kwargs))
 
     def _collect(self):
@@ -390,12 +445,20 @@ class ProcessShowerServer:
                     self._lock.notify_all()
                 continue
             with self._lock:
-                future, need, _ = self._pending.pop(key)
+                future, need, _, count = self._pending.pop(key)
                 self._inflight -= need
                 self._idle.append(slot)
                 self._lock.notify_all()
             future.latency_s = time.perf_counter() - future.submitted
             if kind == "done":
+                meta = payload["metadata"]
+                peak = int(meta.get("device_peak_bytes", 0))
+                if peak > 0:
+                    self.footprint.record(range(count), peak)
+                with self._lock:
+                    self.stats["events"] += 1
+                    self.stats["out_of_memory_retries"] += int(meta.get("out_of_memory_retries", 0))
+                    self.stats["largest_peak_bytes"] = max(self.stats["largest_peak_bytes"], peak)
                 future.set_result(_unpack_response(payload, self.detector))
             else:
                 name, message = payload
@@ -408,9 +471,9 @@ class ProcessShowerServer:
         the process is not replaced."""
         with self._lock:
             dead = {i for i, process in enumerate(self._processes) if not process.is_alive()}
-            lost = [key for key, (_, _, slot) in self._pending.items() if slot in dead]
+            lost = [key for key, (_, _, slot, _) in self._pending.items() if slot in dead]
             for key in lost:
-                future, need, slot = self._pending.pop(key)
+                future, need, slot, _ = self._pending.pop(key)
                 self._inflight -= need
                 future.latency_s = time.perf_counter() - future.submitted
                 future.set_exception(RuntimeError(f"worker process {slot} ended "

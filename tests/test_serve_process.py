@@ -67,8 +67,10 @@ def test_responses_round_trip_through_worker_processes():
         assert np.array_equal(r.charge_components_pe, want.charge_components_pe)
         assert np.array_equal(r.spectrum_ge2_pe, want.spectrum_ge2_pe)
         assert np.array_equal(r.active, want.active)
-        assert r.metadata == {"value": float(i + 1)}
+        assert r.metadata["value"] == float(i + 1)
+        assert r.metadata["out_of_memory_retries"] == 0
         assert futures[i].latency_s > 0
+    assert server.stats["events"] == 8 and server.stats["out_of_memory_retries"] == 0
 
 
 def test_worker_errors_reach_the_caller():
@@ -110,3 +112,16 @@ def test_staged_sources_run_from_every_worker():
         want = _Route()(_Elements(i + 1, i % OMS))
         assert np.array_equal(r.charge_components_pe, want.charge_components_pe)
         assert np.array_equal(r.components_pe, want.components_pe)
+
+
+def test_footprint_model_learns_from_measured_peaks():
+    model = serve.FootprintModel(oms=2304, margin=1.25)
+    prior = model.need(range(1000))
+    assert prior == serve.footprint_bytes(range(1000), 2304)
+    model.record(range(1000), 300 * 2 ** 20)       # a small event: the fixed part
+    model.record(range(100_000), 3300 * 2 ** 20)   # a large one: bytes per element
+    per_element = (3300 - 300) * 2 ** 20 / 100_000
+    assert model.need(range(50_000)) == int(1.25 * (300 * 2 ** 20 + per_element * 50_000))
+    # a later, larger per-element peak raises the estimate; it never falls
+    model.record(range(10_000), 1000 * 2 ** 20)
+    assert model.per_element >= per_element
