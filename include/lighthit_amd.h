@@ -249,6 +249,14 @@ int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
                    size_t workspace_bytes, void* stream);
 
 /*
+ * Diagnostics for the order-0 cost model; no counterpart in the reference.
+ * counter (one device unsigned 64-bit word, or NULL to stop counting): every
+ * later order-0 launch of the process adds the (theta, azimuth) quadrature
+ * nodes it evaluates. A process-wide setting, not for concurrent callers.
+ */
+int lh_order0_count_nodes(unsigned long long* counter);
+
+/*
  * Order >= 2 source: the axial-slice compile of one light field.
  *
  * Replaces compile_axial_slices (_shower_basis.py:188-645) as the shower route

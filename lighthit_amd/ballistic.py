@@ -62,8 +62,30 @@ _lib.SIGNATURES["lh_order0_bins"] = (ctypes.c_int, [ctypes.POINTER(_Optics), cty
                                                     ctypes.c_void_p, ctypes.c_size_t,
                                                     ctypes.c_void_p])
 
+_lib.SIGNATURES["lh_order0_count_nodes"] = (ctypes.c_int, [ctypes.c_void_p])
+
 FLAG_GLASS = 1
 FLAG_UNSUPPORTED = 2
+
+_node_counter = None
+
+
+def count_nodes(device="cuda", on=True):
+    """Start (or stop) counting the (theta, azimuth) quadrature nodes every
+    order-0 launch evaluates (for the cost model; off by default)."""
+    global _node_counter
+    import torch
+    if on:
+        _node_counter = torch.zeros(1, dtype=torch.int64, device=device)
+        _lib.call("lh_order0_count_nodes", ctypes.c_void_p(_node_counter.data_ptr()))
+    else:
+        _lib.call("lh_order0_count_nodes", ctypes.c_void_p(0))
+        _node_counter = None
+
+
+def node_count():
+    """Nodes counted since :func:`count_nodes` (None when not counting)."""
+    return None if _node_counter is None else int(_node_counter.item())
 
 
 def _dev(array, dtype, device):

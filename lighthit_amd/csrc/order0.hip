@@ -111,7 +111,12 @@ struct O0Params {
     const double* bin_edges;  // (nbins + 1)
     int nbins;
     int* flags;               // [0]: glass intersected, [1]: unsupported geometry
+    unsigned long long* node_count;   // optional: (theta, azimuth) nodes evaluated
 };
+
+// Device counter of evaluated quadrature nodes (null: not counted); set by
+// lh_order0_count_nodes and read by every later launch in the process.
+static unsigned long long* g_node_count = nullptr;
 
 // ---------------------------------------------------------- candidate test
 
@@ -863,6 +868,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const float c0 = (float)P.c0[i], c2 = (float)P.c2[i];
         const float inv_length = (float)(1.0 / L);
         float pair_sum = 0.f;
+        int evaluated = 0;
         for (int group = 0; group < slots; group += 64) {
             const Slot mine = (group + lane < slots)
                 ? make_slot(P, group + lane, pieces, scut, nscut, z, rho, L, R, beta, theta_major, norm,
@@ -872,6 +878,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
             for (int k0 = 0; k0 < active; k0 += 8) {
                 const Slot S = shuffle(mine, min(k0 + sub, 63));
                 if (k0 + sub < active && S.piece >= 0) {
+                    ++evaluated;
                     if (kBins)
                         slot_bins(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length, axial,
                                   across, side, xt, wt, (float)R, edges, P.nbins, wave_bins);
@@ -883,6 +890,10 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
             }
         }
         lane_sum += (double)pair_sum;
+        if (P.node_count != nullptr) {   // one theta node per call, kNe azimuth nodes each
+            for (int offset = 32; offset > 0; offset >>= 1) evaluated += __shfl_down(evaluated, offset);
+            if (lane == 0) atomicAdd(P.node_count, (unsigned long long)evaluated * kNe);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -980,6 +991,7 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     P.nchunk = (int)chunks_of(elements->count);
     P.flags = static_cast<int*>(workspace);
     P.live_count = P.flags + 1;
+    P.node_count = g_node_count;
     P.bounds = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
     P.partial = P.bounds + (size_t)P.nchunk * kBoundWords;
     P.live = reinterpret_cast<int*>(P.partial + (size_t)P.nom * P.nchunk * width);
@@ -1010,6 +1022,11 @@ int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const cha
 }
 
 }  // namespace
+
+extern "C" int lh_order0_count_nodes(unsigned long long* counter) {
+    g_node_count = counter;
+    return LH_OK;
+}
 
 extern "C" size_t lh_order0_bins_workspace_bytes(int n_oms, int n_elements, int n_bins) {
     const size_t nchunk = chunks_of(n_elements);

@@ -85,7 +85,9 @@ def spline_matrix(x, at):
         A[i, i] = 4 / h[i - 1] + 4 / h[i]
         A[i, i + 1] = 2 / h[i]
         B[i] = 6 * slope[i - 1] / h[i - 1] + 6 * slope[i] / h[i]
-    slopes = np.linalg.solve(A, B)                      # s = slopes @ y
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):                   # the same rounding in every process
+        slopes = np.linalg.solve(A, B)                  # s = slopes @ y
     index = np.clip(np.searchsorted(x, at, side="right") - 1, 0, n - 2)
     t = (at - x[index])[:, None]
     hi = h[index][:, None]
@@ -94,6 +96,16 @@ def spline_matrix(x, at):
     c2 = (3 * m - 2 * s0 - s1) / hi
     c3 = (s0 + s1 - 2 * m) / hi ** 2
     return eye[index] + s0 * t + c2 * t ** 2 + c3 * t ** 3
+
+
+def fixed_order_product(a, b):
+    """``a @ b`` summed over the inner index in order, one rank-1 update at a
+    time: the same bits whatever the BLAS library and its thread count (a
+    threaded matrix product rounds differently from one process to the next)."""
+    out = np.zeros((a.shape[0], b.shape[1]), np.result_type(a, b))
+    for k in range(a.shape[1]):
+        out += a[:, k, None] * b[None, k, :]
+    return out
 
 
 class ReadoutBasis:
@@ -117,7 +129,7 @@ class ReadoutBasis:
                 spline = spline_matrix(self.omega, readout)
                 grid = readout
         self.grid = grid
-        self.matrix = bin_kernel(grid, self.edges, sigma_ns) @ spline
+        self.matrix = fixed_order_product(bin_kernel(grid, self.edges, sigma_ns), spline)
         packed = np.concatenate([self.matrix.real, -self.matrix.imag], axis=1)
         self.device = torch.as_tensor(np.ascontiguousarray(packed, np.float32), device=device)
         self.omega_device = torch.as_tensor(self.omega, dtype=torch.float64, device=device)

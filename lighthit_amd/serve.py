@@ -342,10 +342,15 @@ class ProcessShowerServer:
         while ready < processes:
             remaining = start_timeout_s - (time.perf_counter() - started)
             try:
-                kind, slot, _, message = self._results.get(timeout=max(remaining, 0.1))
+                kind, slot, _, message = self._results.get(timeout=min(max(remaining, 0.1), 1.0))
             except queue.Empty:
-                self._terminate()
-                raise RuntimeError("worker processes did not start in time") from None
+                dead = [i for i, p in enumerate(self._processes) if not p.is_alive()]
+                if dead or remaining <= 0:
+                    codes = {i: self._processes[i].exitcode for i in dead}
+                    self._terminate()
+                    raise RuntimeError("worker processes did not start: "
+                                       + (f"exited {codes}" if dead else "not in time")) from None
+                continue
             if kind == "failed":
                 self._terminate()
                 raise RuntimeError(f"worker process {slot} failed to start: {message}")

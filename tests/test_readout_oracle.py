@@ -95,3 +95,21 @@ def test_folded_basis_equals_spline_then_integral():
     matrix = product.bin_kernel(readout, edges) @ product.spline_matrix(omega, readout)
     expected = oracle.route_readout(omega, spectrum, np.zeros(5), readout, edges)
     np.testing.assert_allclose((matrix @ spectrum).real.T, expected, rtol=0, atol=1e-12)
+
+
+def test_readout_basis_has_the_same_bits_at_any_thread_count():
+    """The folded basis M = K S feeds every process of the served form: it must
+    not depend on how many BLAS threads a process runs (a threaded matrix
+    product rounds differently), or the same event would get different bins
+    alone and served."""
+    from threadpoolctl import threadpool_limits
+    omega, edges = np.linspace(0., 1.2, 65), np.arange(-60., 745., 1.)
+    readout = np.linspace(0., 1.2, 321)
+    with threadpool_limits(limits=1):
+        one = product.ReadoutBasis(omega, edges, readout, device="cpu")
+    with threadpool_limits(limits=8):
+        many = product.ReadoutBasis(omega, edges, readout, device="cpu")
+    assert np.array_equal(one.matrix, many.matrix)
+    # and it is the matrix product
+    direct = product.bin_kernel(readout, edges) @ product.spline_matrix(omega, readout)
+    np.testing.assert_allclose(one.matrix, direct, rtol=0, atol=1e-14 * np.abs(direct).max())

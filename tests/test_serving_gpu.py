@@ -70,3 +70,33 @@ def test_small_events_match_the_oracle_route(served, index):
         scale = np.maximum(np.abs(ref).sum(axis=1), 1e-12)
         err = np.abs(got.components_pe[ids, :, order] - ref).sum(axis=1) / scale
         assert len(ids) == 0 or err.max() < tolerance, (order, err.max())
+
+
+def test_process_served_mixed_form_equals_the_plain_call(device):
+    """Stream events served by two worker processes in the precision form carry
+    the bits the plain call gives them in this process. Events 67 (100,771
+    elements) and 162 (3,755) have order >=2 bins that changed in the last bit
+    when the readout basis came from a threaded matrix product."""
+    import functools
+
+    from route_cases import detector_route_only
+
+    from lighthit_amd.serve import ProcessShowerServer
+    from lighthit_amd.synthetic import serving_event, serving_plan
+    route = detector_route_only("mixed", "cuda:0")
+    plan = serving_plan(route.kernel.detector)
+    picks = [0, 67, 162]
+    events = [serving_event(plan[i]) for i in picks]
+    alone = [route(e) for e in events]
+    print(f"mixed form: {picks} alone, now served by two processes", flush=True)
+    with ProcessShowerServer(functools.partial(detector_route_only, "mixed"), processes=2,
+                             detector=route.kernel.detector, device="cuda:0",
+                             threads_per_process=2) as server:
+        futures = [server.submit(e) for e in events]
+        served = [f.result(timeout=300) for f in futures]
+    for i, (a, s) in enumerate(zip(alone, served)):
+        assert np.array_equal(s.active, a.active), picks[i]
+        assert np.array_equal(s.charge_components_pe, a.charge_components_pe), picks[i]
+        assert np.array_equal(s.components_pe, a.components_pe), picks[i]
+        assert np.array_equal(s.spectrum_ge2_pe, a.spectrum_ge2_pe), picks[i]
+    assert sum(int(a.active.sum()) for a in alone) > 0
