@@ -986,8 +986,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
 
     // A work item is one unit of pairs and one chunk of column tiles; the
     // chunks of a unit go to neighbouring waves, which share its records and
-    // channels in cache.
-    for (int item = blockIdx.x * 4 + wave; item < total_items; item += gridDim.x * 4) {
+    // channels in cache. Workgroups are dealt to the 8 XCDs in turn (grid a
+    // multiple of 8): each XCD takes a contiguous eighth of the items, so the
+    // units of one radial interval, which read the same coefficients, meet in
+    // one L2.
+    const int xcd = blockIdx.x % 8, rank = blockIdx.x / 8, per_xcd = gridDim.x / 8;
+    const int first_item = (int)((long long)total_items * xcd / 8);
+    const int end_item = (int)((long long)total_items * (xcd + 1) / 8);
+    for (int item = first_item + rank * 4 + wave; item < end_item; item += per_xcd * 4) {
         const int unit = item / nchunk;
         const int c0 = (item - unit * nchunk) * kMfmaChunk;
         const int nct = min(kMfmaChunk, ntile_col - c0);
@@ -1225,7 +1231,9 @@ int launch_mfma(Ge2Params P, const void* workspace, size_t workspace_bytes, hipS
     if (status != LH_OK) return status;
     const size_t max_tiles = (pairs + MfmaShape<M>::UNIT - 1) / MfmaShape<M>::UNIT + P.intervals;
     const size_t chunks = ((size_t)P.bands * P.nfreq + 16 * kMfmaChunk - 1) / (16 * kMfmaChunk);
-    const unsigned groups = (unsigned)std::min<size_t>((max_tiles * chunks + 3) / 4, 4096);
+    // a multiple of 8 workgroups (the kernel deals items per XCD)
+    const size_t want = std::min<size_t>((max_tiles * chunks + 3) / 4, 4096);
+    const unsigned groups = (unsigned)std::max<size_t>(8, (want + 7) / 8 * 8);
     {
         lh::KernelMark mark("ge2_mfma_apply", stream);
         hipLaunchKernelGGL((ge2_mfma_apply<A, M>), dim3(groups), dim3(256), 0, stream, P, W,
