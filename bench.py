@@ -536,8 +536,7 @@ def counter_passes(workload_flag, precision, dominant, keep_dir=None):
         "fetch": ("FETCH_SIZE", True),
         "write": ("WRITE_SIZE", True),
         "engines": ("SQ_VALU_MFMA_BUSY_CYCLES SQ_ACTIVE_INST_VALU SQ_INSTS_VALU_MFMA_MOPS_BF16 "
-                    "SQ_INSTS_VALU_MFMA_MOPS_F16 SQ_INSTS_VALU_FLOPS_FP32 "
-                    "SQ_INSTS_VALU_FLOPS_FP64 GRBM_GUI_ACTIVE", False),
+                    "SQ_INSTS_VALU_MFMA_MOPS_F16 GRBM_GUI_ACTIVE", False),
     }
     tables = {}
     for tag, (counters, calibrate) in passes.items():
@@ -577,11 +576,11 @@ def counter_passes(workload_flag, precision, dominant, keep_dir=None):
             continue
         mean = {c: sum(e.get(c, 0.0) for e in rows) / len(rows) for c in rows[0]}
         active = (mean.get("GRBM_GUI_ACTIVE") or 0.0) / XCDS
+        # executed MFMA flops (MOPS x 512, padding included); busy fractions as
+        # rocprof's MfmaUtil and VALUBusy define them, per GPU clock of the dispatch
         entry = {"dispatches": len(rows),
-                 "mfma_flops_bf16": 512 * mean.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0),
-                 "mfma_flops_f16": 512 * mean.get("SQ_INSTS_VALU_MFMA_MOPS_F16", 0.0),
-                 "valu_flops_fp32": mean.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0),
-                 "valu_flops_fp64": mean.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0)}
+                 "mfma_flops_executed": 512 * (mean.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0)
+                                               + mean.get("SQ_INSTS_VALU_MFMA_MOPS_F16", 0.0))}
         if active:
             entry["mfma_busy"] = mean.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (active * SIMDS)
             entry["valu_busy"] = mean.get("SQ_ACTIVE_INST_VALU", 0.0) / (active * SIMDS // 4)
@@ -798,19 +797,26 @@ def main():
               file=sys.stderr, flush=True)
         stream = list(serving_stream(workload.detector, args.stream_events))
         stream_plain = plain_stream(served_route, stream)
-    # the worker processes get the device memory this process cached
-    from lighthit_amd import _lib as lib
-    lib.release_workspaces()
-    torch.cuda.empty_cache()
-    print(f"[bench] serving {args.steps} x {args.processes} events from {args.processes} "
-          f"processes ({args.precision})", file=sys.stderr, flush=True)
-    served = serve_processes(args.workload, args.precision, workload.detector, elements,
-                             args.processes, args.steps, args.warmup, stream, stream_plain,
-                             rates)
-    results = served.pop("results")
-    served["same_as_alone"] = bool(all(
-        np.array_equal(r.charge_components_pe, response.charge_components_pe)
-        and np.array_equal(r.components_pe, response.components_pe) for r in results))
+    served = None
+    if args.processes > 0:
+        # the worker processes get the device memory this process cached
+        from lighthit_amd import _lib as lib
+        lib.release_workspaces()
+        torch.cuda.empty_cache()
+        print(f"[bench] serving {args.steps} x {args.processes} events from {args.processes} "
+              f"processes ({args.precision})", file=sys.stderr, flush=True)
+        served = serve_processes(args.workload, args.precision, workload.detector, elements,
+                                 args.processes, args.steps, args.warmup, stream, stream_plain,
+                                 rates)
+        results = served.pop("results")
+        served["same_as_alone"] = bool(all(
+            np.array_equal(r.charge_components_pe, response.charge_components_pe)
+            and np.array_equal(r.components_pe, response.components_pe) for r in results))
+    else:
+        # plain form only (profiling runs): one step = one event
+        served = {"events_per_s": plain_served_form["events_per_s"],
+                  "ms_per_step": plain_served_form["ms_per_event"],
+                  "latency_ms_poisson": plain_served_form["latency_ms"]}
 
     kernels = plain_served_form["kernel_gpu_ms"]
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_event"]) if kernels else None
@@ -852,7 +858,8 @@ def main():
     value = served["events_per_s"]
     line = {
         "metric": "events/sec + p50/p99 latency at <=1e-3 charge error vs FP64; served vs plain GPU",
-        "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+        "value": value, "unit": "events/s", "n_gpus": 1,
+        "steps": args.steps if args.processes > 0 else args.plain_steps,
         "warmup": args.warmup, "ms_per_step": served["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16/fp16 hi+lo MFMA, fp32 accumulate; fp32/fp64 vector" if
@@ -862,8 +869,10 @@ def main():
                    "form": (f"served: {args.processes} worker processes on one GPU, each "
                             f"calling ShowerTransport(kernel, data)(shower) with precision "
                             f"'{args.precision}'; a step is a batch of {args.processes} events, "
-                            "sources resident in HBM"),
-                   "events_per_step": args.processes,
+                            "sources resident in HBM") if args.processes > 0 else
+                           (f"plain: one ShowerTransport call at a time, precision "
+                            f"'{args.precision}', sources resident in HBM"),
+                   "events_per_step": max(1, args.processes),
                    "oms": len(workload.detector), "elements": len(elements),
                    "wavelength_nodes": len(workload.wavelength.wavelength_nm),
                    "bins": len(workload.edges_ns) - 1,
@@ -873,7 +882,8 @@ def main():
                    "charge_pe_by_order": [float(x) for x in
                                           response.charge_components_pe.sum(axis=0)],
                    "latency_ms": dict(served["latency_ms_poisson"],
-                                      form="served, Poisson arrivals at 70% of throughput"),
+                                      form="served, Poisson arrivals at 70% of throughput"
+                                      if args.processes > 0 else "plain, one call at a time"),
                    "error_vs_fp64": precision,
                    "served": served,
                    "plain_fp32": plain_fp32, "plain_mixed": plain_mixed,
