@@ -59,6 +59,9 @@ class RouteConfig:
     azimuthal_degree: int = 2
     radial_range_m: tuple = (2.0, 150.0)
     radial_nodes: int = 48
+    k_max_per_m: float = 4.0
+    k_panel_per_m: float = 0.1
+    k_order: int = 6
     threshold_pe: float = 0.01
     detail_threshold_pe: float | None = None
 

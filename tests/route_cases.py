@@ -2,14 +2,20 @@
 import numpy as np
 
 
-def demonstration_route(device="cuda"):
+def demonstration_route(device="cuda", tables=False):
+    """The demonstration event's route; ``tables`` builds the order >=2 pack from
+    the RTE tables (lighthit_amd.tables) instead of the seeded synthetic pack."""
     from lighthit_amd.fold import KernelPack
     from lighthit_amd.route import RouteConfig, RouteKernel, ShowerTransport
     from lighthit_amd.synthetic import plain_small, synthetic_pack
     w = plain_small()
     kernel = RouteKernel(w.medium, w.detector, RouteConfig(relative_time_edges_ns=w.edges_ns))
     pack = KernelPack()
-    key = pack.add(*synthetic_pack(w.medium, kernel.wavelength))
+    if tables:
+        from lighthit_amd.tables import build_pack
+        pack.add(*build_pack(kernel))
+    else:
+        pack.add(*synthetic_pack(w.medium, kernel.wavelength))
     route = ShowerTransport(kernel, pack, device=device)
     return w, kernel, pack, route
 
