@@ -31,6 +31,8 @@
 // order, so the result does not depend on scheduling.
 #include "lh_common.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -46,7 +48,7 @@ constexpr int kMaxMajor = 12;      // medium nodes inside the band (+ ends)
 constexpr int kMaxPanels = 256;    // efficiency panels
 constexpr int kMaxAlpha = 8;
 constexpr double kC = 0.299792458;
-constexpr int kChunk = 512;        // elements per workgroup
+constexpr int kChunk = 512;        // most elements per (OM, chunk) item
 constexpr int kBoundWords = 12;
 constexpr size_t kPersistentGroups = 2048;  // workgroups of the pair kernel
 constexpr double kPi = 3.14159265358979323846;
@@ -100,10 +102,13 @@ struct O0Params {
     int nalpha;
     double phase_min, phase_max;
     int chunk, nchunk;
-    double* partial;          // (nom, nchunk)
+    double* partial;          // (live items, width): one row per live item
     double* bounds;           // (nchunk, kBoundWords) bounding sphere / direction cone
-    int* live;                // (nom * nchunk, 2) live (OM, chunk) items
-    int* live_count;
+    int* live;                // (nom * nchunk, 2) live (OM, chunk) items, in (OM, chunk) order
+    int* live_flag;           // (nom * nchunk + 1) 1 where the item may hold a pair
+    int* live_slot;           // (nom * nchunk + 1) exclusive scan of live_flag
+    int* live_count;          // = live_slot + nom * nchunk
+    int* range;               // (live items, 2) first and last bin an item wrote
     // bins only
     const double* start_ns;   // (nel)
     const double* end_ns;     // (nel)
@@ -696,20 +701,28 @@ __global__ void order0_zero(double* out, size_t count) {
     if (idx < count) out[idx] = 0.0;
 }
 
-// One thread per (OM, chunk): items that may hold a contributing pair are
-// appended to the live list; the partial sums of the others are zeroed here.
-__global__ void order0_cull(O0Params P, int width) {
+// One thread per (OM, chunk): flags the items that may hold a contributing
+// pair (the flag past the last item stays zero for the scan).
+__global__ void order0_cull(O0Params P) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)P.nom * P.nchunk) return;
-    const int om = (int)(idx / P.nchunk), chunk = (int)(idx - (size_t)om * P.nchunk);
-    if (chunk_culled(P, chunk, om)) {
-        double* out = P.partial + idx * width;
-        for (int b = 0; b < width; ++b) out[b] = 0.0;
+    const size_t items = (size_t)P.nom * P.nchunk;
+    if (idx > items) return;
+    if (idx == items) {
+        P.live_flag[idx] = 0;
         return;
     }
-    const int slot = atomicAdd(P.live_count, 1);
-    P.live[2 * slot] = om;
-    P.live[2 * slot + 1] = chunk;
+    const int om = (int)(idx / P.nchunk), chunk = (int)(idx - (size_t)om * P.nchunk);
+    P.live_flag[idx] = chunk_culled(P, chunk, om) ? 0 : 1;
+}
+
+// Live items in (OM, chunk) order at their scanned slots: the list, and so
+// every partial row, has the same order on every run.
+__global__ void order0_compact(O0Params P) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)P.nom * P.nchunk || !P.live_flag[idx]) return;
+    const int slot = P.live_slot[idx];
+    P.live[2 * slot] = (int)(idx / P.nchunk);
+    P.live[2 * slot + 1] = (int)(idx % P.nchunk);
 }
 
 template <bool kBins>
@@ -725,6 +738,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     __shared__ double wave_scut[kWaves][2 * kMaxMajor + 2];
     __shared__ int wave_nscut[kWaves];
     __shared__ int glass;
+    __shared__ int range_lo, range_hi;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -899,13 +913,35 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (kBins) {
-        // waves in a fixed order
+        // waves in a fixed order into the first histogram; only the bins from
+        // the first to the last nonzero one are written (the reduction reads
+        // the item's range)
+        if (tid == 0) {
+            range_lo = P.nbins;
+            range_hi = -1;
+        }
         __syncthreads();
-        double* out = P.partial + ((size_t)om * P.nchunk + chunk) * P.nbins;
+        int lo = P.nbins, hi = -1;
         for (int b = tid; b < P.nbins; b += kThreads) {
             double sum = 0.0;
             for (int w = 0; w < kWaves; ++w) sum += dynamic_lds[(size_t)w * P.nbins + b];
-            out[b] = sum;
+            dynamic_lds[b] = sum;
+            if (sum != 0.0) {
+                lo = min(lo, b);
+                hi = max(hi, b);
+            }
+        }
+        if (hi >= 0) {
+            atomicMin(&range_lo, lo);
+            atomicMax(&range_hi, hi);
+        }
+        __syncthreads();
+        const int first_bin = range_lo, last_bin = range_hi;
+        double* out = P.partial + (size_t)item * P.nbins;
+        for (int b = first_bin + tid; b <= last_bin; b += kThreads) out[b] = dynamic_lds[b];
+        if (tid == 0) {
+            P.range[2 * item] = first_bin;
+            P.range[2 * item + 1] = last_bin;
         }
         __syncthreads();
         continue;
@@ -917,19 +953,28 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     if (tid == 0) {
         double sum = 0.0;
         for (int w = 0; w < kWaves; ++w) sum += wave_sum[w];
-        P.partial[(size_t)om * P.nchunk + chunk] = sum;
+        P.partial[item] = sum;
     }
     __syncthreads();
     }
 }
 
-// out[om][b] = sum over chunks in order of partial[om][chunk][b] (width = bins or 1)
-__global__ void order0_reduce(const double* partial, int nom, int nchunk, int width, double* out) {
+// out[om][b] = sum over the OM's live items, in chunk order, of their
+// partial rows (width = bins or 1; a bin item adds only inside its range)
+__global__ void order0_reduce(O0Params P, int width, double* out) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)nom * width) return;
-    const size_t om = idx / width, b = idx - om * width;
+    if (idx >= (size_t)P.nom * width) return;
+    const size_t om = idx / width;
+    const int b = (int)(idx - om * width);
+    const int first = P.live_slot[om * P.nchunk], last = P.live_slot[(om + 1) * P.nchunk];
     double sum = 0.0;
-    for (int c = 0; c < nchunk; ++c) sum += partial[(om * nchunk + c) * width + b];
+    if (width == 1) {
+        for (int item = first; item < last; ++item) sum += P.partial[item];
+    } else {
+        for (int item = first; item < last; ++item)
+            if (b >= P.range[2 * item] && b <= P.range[2 * item + 1])
+                sum += P.partial[(size_t)item * width + b];
+    }
     out[idx] = sum;
 }
 
@@ -937,8 +982,50 @@ __global__ void order0_reduce(const double* partial, int nom, int nchunk, int wi
 
 namespace {
 
-size_t chunks_of(int n_elements) {
-    return n_elements > 0 ? (size_t)(n_elements + kChunk - 1) / kChunk : 1;
+// Elements per (OM, chunk) item, from the element count alone (so an OM's
+// result never depends on which other OMs share the call): small sources are
+// cut finer, so that few OMs near a short shower still give the chip several
+// hundred workgroups and the pairs of the nearest OM spread over many waves.
+// The bins launch (only the active OMs, each pair costlier) cuts small
+// sources finer still.
+int chunk_of(int n_elements, bool bins) {
+    const int chunk = n_elements >= 65536 ? kChunk : n_elements >= 16384 ? 256
+                    : n_elements >= 4096 ? 128 : n_elements >= 1024 ? 64 : 32;
+    return (bins && n_elements < 65536) ? std::max(16, chunk / 2) : chunk;
+}
+
+size_t chunks_of(int n_elements, bool bins) {
+    const int chunk = chunk_of(n_elements, bins);
+    return n_elements > 0 ? (size_t)(n_elements + chunk - 1) / chunk : 1;
+}
+
+size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+// Workspace of one call: flag words | chunk bounds | partial rows | live list
+// | live flags | live slots | bin ranges | scan scratch.
+struct O0Layout {
+    size_t nchunk, items, bounds, partial, live, flag, slot, range, scan, scan_bytes, total;
+};
+
+O0Layout layout_of(int n_oms, int n_elements, int width, bool bins) {
+    O0Layout L{};
+    L.nchunk = chunks_of(n_elements, bins);
+    L.items = (size_t)(n_oms > 0 ? n_oms : 0) * L.nchunk;
+    const size_t w = (size_t)(width > 0 ? width : 0);
+    L.bounds = 256;
+    L.partial = align256(L.bounds + sizeof(double) * L.nchunk * kBoundWords);
+    L.live = align256(L.partial + sizeof(double) * L.items * w);
+    L.flag = align256(L.live + 2 * sizeof(int) * L.items);
+    L.slot = align256(L.flag + sizeof(int) * (L.items + 1));
+    L.range = align256(L.slot + sizeof(int) * (L.items + 1));
+    L.scan = align256(L.range + (bins ? 2 * sizeof(int) * L.items : 0));
+    size_t scan_bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int*)nullptr, (int*)nullptr,
+                                         (int)(L.items + 1)) != hipSuccess)
+        scan_bytes = 0;
+    L.scan_bytes = std::max<size_t>(scan_bytes, 1);
+    L.total = align256(L.scan + L.scan_bytes);
+    return L;
 }
 
 int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms,
@@ -958,7 +1045,7 @@ int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms,
 }
 
 O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
-                     const lh_order0_elements* elements, void* workspace, int width) {
+                     const lh_order0_elements* elements, void* workspace, int width, bool bins) {
     O0Params P{};
     P.positions = oms->positions;
     P.looks = oms->looks;
@@ -987,25 +1074,38 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     P.nalpha = optics->acceptance_count;
     P.phase_min = optics->phase_min;
     P.phase_max = optics->phase_max;
-    P.chunk = kChunk;
-    P.nchunk = (int)chunks_of(elements->count);
+    const O0Layout L = layout_of(oms->count, elements->count, width, bins);
+    char* base = static_cast<char*>(workspace);
+    P.chunk = chunk_of(elements->count, bins);
+    P.nchunk = (int)L.nchunk;
     P.flags = static_cast<int*>(workspace);
-    P.live_count = P.flags + 1;
     P.node_count = g_node_count;
-    P.bounds = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
-    P.partial = P.bounds + (size_t)P.nchunk * kBoundWords;
-    P.live = reinterpret_cast<int*>(P.partial + (size_t)P.nom * P.nchunk * width);
+    P.bounds = reinterpret_cast<double*>(base + L.bounds);
+    P.partial = reinterpret_cast<double*>(base + L.partial);
+    P.live = reinterpret_cast<int*>(base + L.live);
+    P.live_flag = reinterpret_cast<int*>(base + L.flag);
+    P.live_slot = reinterpret_cast<int*>(base + L.slot);
+    P.live_count = P.live_slot + L.items;
+    P.range = bins ? reinterpret_cast<int*>(base + L.range) : nullptr;
     return P;
 }
 
 template <bool kBins>
-int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const char* name) {
+int run(O0Params P, int width, double* out, void* workspace, hipStream_t s, size_t lds,
+        const char* name) {
+    const O0Layout L = layout_of(P.nom, P.nel, width, kBins);
     hipLaunchKernelGGL(order0_chunk_bounds, dim3(P.nchunk), dim3(kThreads), 0, s, P);
     int status = lh::check_launch(name);
     if (status != LH_OK) return status;
-    const size_t items = (size_t)P.nom * P.nchunk;
-    hipLaunchKernelGGL(order0_cull, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, P,
-                       width);
+    const size_t items = L.items;
+    hipLaunchKernelGGL(order0_cull, dim3((unsigned)((items + 1 + 255) / 256)), dim3(256), 0, s, P);
+    status = lh::check_launch(name);
+    if (status != LH_OK) return status;
+    size_t scan_bytes = L.scan_bytes;
+    LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(static_cast<char*>(workspace) + L.scan,
+                                                 scan_bytes, P.live_flag, P.live_slot,
+                                                 (int)(items + 1), s));
+    hipLaunchKernelGGL(order0_compact, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, P);
     status = lh::check_launch(name);
     if (status != LH_OK) return status;
     const unsigned grid = (unsigned)std::min<size_t>(items, kPersistentGroups);
@@ -1017,7 +1117,7 @@ int run(O0Params P, int width, double* out, hipStream_t s, size_t lds, const cha
     if (status != LH_OK) return status;
     const size_t total = (size_t)P.nom * width;
     hipLaunchKernelGGL(order0_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       P.partial, P.nom, P.nchunk, width, out);
+                       P, width, out);
     return lh::check_launch(name);
 }
 
@@ -1029,14 +1129,11 @@ extern "C" int lh_order0_count_nodes(unsigned long long* counter) {
 }
 
 extern "C" size_t lh_order0_bins_workspace_bytes(int n_oms, int n_elements, int n_bins) {
-    const size_t nchunk = chunks_of(n_elements);
-    const size_t items = (size_t)(n_oms > 0 ? n_oms : 0) * nchunk;
-    return 256 + sizeof(double) * (nchunk * kBoundWords + items * (size_t)(n_bins > 0 ? n_bins : 0))
-           + 2 * sizeof(int) * items;
+    return layout_of(n_oms, n_elements, n_bins, true).total;
 }
 
 extern "C" size_t lh_order0_workspace_bytes(int n_oms, int n_elements) {
-    return lh_order0_bins_workspace_bytes(n_oms, n_elements, 1);
+    return layout_of(n_oms, n_elements, 1, false).total;
 }
 
 extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
@@ -1056,8 +1153,8 @@ extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_
                            (size_t)oms->count);
         return lh::check_launch("lh_order0_charge (zero)");
     }
-    O0Params P = make_params(optics, oms, elements, workspace, 1);
-    return run<false>(P, 1, charge, s, 0, "lh_order0_charge");
+    O0Params P = make_params(optics, oms, elements, workspace, 1, false);
+    return run<false>(P, 1, charge, workspace, s, 0, "lh_order0_charge");
 }
 
 extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
@@ -1082,12 +1179,12 @@ extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_om
                            bins, count);
         return lh::check_launch("lh_order0_bins (zero)");
     }
-    O0Params P = make_params(optics, oms, elements, workspace, n_bins);
+    O0Params P = make_params(optics, oms, elements, workspace, n_bins, true);
     P.origins = origins;
     P.bin_edges = bin_edges;
     P.nbins = n_bins;
     const size_t lds = sizeof(double) * kWaves * (size_t)n_bins + sizeof(float) * (n_bins + 1);
-    return run<true>(P, n_bins, bins, s, lds, "lh_order0_bins");
+    return run<true>(P, n_bins, bins, workspace, s, lds, "lh_order0_bins");
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,8 +1253,8 @@ extern "C" int lh_order0_charge_noalloc(const lh_order0_optics* optics, const lh
                                charge, (size_t)oms->count);
             status = lh::check_launch("lh_order0_charge_noalloc (zero)");
         } else {
-            O0Params P = make_params(optics, oms, elements, ws, 1);
-            status = run<false>(P, 1, charge, s, 0, "lh_order0_charge_noalloc");
+            O0Params P = make_params(optics, oms, elements, ws, 1, false);
+            status = run<false>(P, 1, charge, ws, s, 0, "lh_order0_charge_noalloc");
         }
         if (status != LH_OK) return status;
     }
