@@ -257,6 +257,16 @@ int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
 int lh_order0_count_nodes(unsigned long long* counter);
 
 /*
+ * Measurement only (no counterpart): the s rule of every later order-0 launch
+ * of the process. 0 (the default) chooses per pair: for the charge 3 Gauss
+ * nodes per s piece when the element is at most a tenth of the glass radius
+ * long and 6 otherwise, for the bins 6; 2, 3, 4 or 6 force that rule, so a
+ * caller can compare rule orders (the quadrature's error estimate). A
+ * process-wide setting, not for concurrent callers.
+ */
+int lh_order0_set_s_rule(int nodes);
+
+/*
  * Order >= 2 source: the axial-slice compile of one light field.
  *
  * Replaces compile_axial_slices (_shower_basis.py:188-645) as the shower route

@@ -63,6 +63,7 @@ _lib.SIGNATURES["lh_order0_bins"] = (ctypes.c_int, [ctypes.POINTER(_Optics), cty
                                                     ctypes.c_void_p])
 
 _lib.SIGNATURES["lh_order0_count_nodes"] = (ctypes.c_int, [ctypes.c_void_p])
+_lib.SIGNATURES["lh_order0_set_s_rule"] = (ctypes.c_int, [ctypes.c_int])
 
 FLAG_GLASS = 1
 FLAG_UNSUPPORTED = 2
@@ -86,6 +87,14 @@ def count_nodes(device="cuda", on=True):
 def node_count():
     """Nodes counted since :func:`count_nodes` (None when not counting)."""
     return None if _node_counter is None else int(_node_counter.item())
+
+
+def set_s_rule(nodes=0):
+    """Gauss nodes per s piece of every later order-0 launch: 0 (default) picks,
+    for the charge, 3 for elements at most a tenth of the glass radius long and
+    6 otherwise, and 6 for the bins; 2, 3, 4 or 6 force that rule (to compare
+    rule orders)."""
+    _lib.call("lh_order0_set_s_rule", int(nodes))
 
 
 def _dev(array, dtype, device):

@@ -41,7 +41,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int kNs = 6;             // s nodes per s piece
+constexpr int kMaxNs = 6;          // most s nodes per s piece
 constexpr int kNt = 8;             // theta nodes per medium panel
 constexpr int kNe = 8;             // azimuth nodes on [0, pi/2]
 constexpr int kMaxMajor = 12;      // medium nodes inside the band (+ ends)
@@ -61,11 +61,21 @@ __constant__ float kGauss8w[8] = {0.101228536290376f, 0.222381034453374f, 0.3137
                                   0.362683783378362f, 0.362683783378362f, 0.313706645877887f,
                                   0.222381034453374f, 0.101228536290376f};
 
-// 6-point Gauss-Legendre rule on [-1, 1]
-__constant__ double kGauss6x[6] = {-0.932469514203152, -0.661209386466265, -0.238619186083197,
-                                   0.238619186083197, 0.661209386466265, 0.932469514203152};
-__constant__ double kGauss6w[6] = {0.171324492379170, 0.360761573048139, 0.467913934572691,
-                                   0.467913934572691, 0.360761573048139, 0.171324492379170};
+// Gauss-Legendre rules on [-1, 1] of 2, 3, 4 and 6 points (the s rules),
+// stored one after the other from kGaussSOffset[n].
+__constant__ double kGaussSx[15] = {
+    -0.577350269189626, 0.577350269189626,
+    -0.774596669241483, 0.0, 0.774596669241483,
+    -0.861136311594053, -0.339981043584856, 0.339981043584856, 0.861136311594053,
+    -0.932469514203152, -0.661209386466264, -0.238619186083197, 0.238619186083197,
+    0.661209386466264, 0.932469514203152};
+__constant__ double kGaussSw[15] = {
+    1.0, 1.0,
+    0.555555555555556, 0.888888888888889, 0.555555555555556,
+    0.347854845137454, 0.652145154862546, 0.652145154862546, 0.347854845137454,
+    0.171324492379170, 0.360761573048139, 0.467913934572691, 0.467913934572691,
+    0.360761573048139, 0.171324492379170};
+__device__ __forceinline__ int gauss_s_offset(int n) { return n == 2 ? 0 : n == 3 ? 2 : n == 4 ? 5 : 9; }
 
 // Azimuth rule: eta = pi/4 (x + 1) on [0, pi/2] with the 8-point Gauss nodes;
 // sin, cos, 1 - sin (cancellation-free) and weight pi/4 w of every node.
@@ -117,11 +127,16 @@ struct O0Params {
     int nbins;
     int* flags;               // [0]: glass intersected, [1]: unsupported geometry
     unsigned long long* node_count;   // optional: (theta, azimuth) nodes evaluated
+    int s_rule;               // 0: s nodes per pair from its length; else that many
 };
 
 // Device counter of evaluated quadrature nodes (null: not counted); set by
 // lh_order0_count_nodes and read by every later launch in the process.
 static unsigned long long* g_node_count = nullptr;
+
+// s rule of later launches (0: chosen per pair; 2, 3, 4, 6: forced), set by
+// lh_order0_set_s_rule.
+static int g_s_rule = 0;
 
 // ---------------------------------------------------------- candidate test
 
@@ -222,20 +237,32 @@ struct Slot {
 
 // Set-up of slot `slot` of a pair: s node slot / pieces of the s pieces,
 // medium panel slot % pieces.
+// Nodes of the s rule on every s piece of a pair. The charge takes 3 when
+// the element is short against the glass radius (the cap then moves by at
+// most a tenth of its own radius along the element, and 3 nodes agree with 6
+// to the FP32 rounding of the node arithmetic), 6 otherwise; the bins keep 6
+// (a bin edge crossed inside a piece leaves a kink in s). `rule` > 0 forces
+// that many (2, 3, 4 or 6) for the convergence check.
+template <bool kBins>
+__device__ __forceinline__ int s_nodes(double L, double R, int rule) {
+    if (rule > 0) return rule;
+    return (!kBins && L <= 0.1 * R) ? 3 : kMaxNs;
+}
+
 __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double* scut, int nscut,
-                          double z, double rho, double L, double R, double beta,
+                          int ns, double z, double rho, double L, double R, double beta,
                           const double* theta_major, double norm, double t_start, double tau) {
     Slot out;
     out.piece = -1;
     const int snode = slot / pieces;
     const int piece = slot - snode * pieces;
-    const int spiece = snode / kNs;
-    const int j = snode - spiece * kNs;
+    const int spiece = snode / ns;
+    const int j = snode - spiece * ns + gauss_s_offset(ns);
     if (spiece >= nscut - 1) return out;
     const double sa = scut[spiece], sb = scut[spiece + 1];
     if (!(sb > sa)) return out;
-    // 6-point Gauss-Legendre on [sa, sb]
-    const double s = 0.5 * (sa + sb) + 0.5 * (sb - sa) * kGauss6x[j];
+    // ns-point Gauss-Legendre on [sa, sb]
+    const double s = 0.5 * (sa + sb) + 0.5 * (sb - sa) * kGaussSx[j];
     const double along = z - s;
     const double D = sqrt(rho * rho + along * along);
     const double psi = atan2(rho, along);
@@ -248,7 +275,7 @@ __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double*
     const double tb = xb >= 1. ? 0.5 * kPi : asin(xb);
     if (!(tb > ta)) return out;
     const double hs = sin(0.5 * eps);
-    out.weight = (float)(0.5 * (sb - sa) * kGauss6w[j] * norm);
+    out.weight = (float)(0.5 * (sb - sa) * kGaussSw[j] * norm);
     out.D = (float)D;
     out.D2R2 = (float)((D - R) * (D + R));
     out.eps = (float)eps;
@@ -878,14 +905,15 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const int nscut = wave_nscut[wave];
         const double* scut = wave_scut[wave];
         const double* theta_major = wave_theta[wave];
-        const int slots = (nscut - 1) * kNs * pieces;
+        const int ns = s_nodes<kBins>(L, R, P.s_rule);
+        const int slots = (nscut - 1) * ns * pieces;
         const float c0 = (float)P.c0[i], c2 = (float)P.c2[i];
         const float inv_length = (float)(1.0 / L);
         float pair_sum = 0.f;
         int evaluated = 0;
         for (int group = 0; group < slots; group += 64) {
             const Slot mine = (group + lane < slots)
-                ? make_slot(P, group + lane, pieces, scut, nscut, z, rho, L, R, beta, theta_major, norm,
+                ? make_slot(P, group + lane, pieces, scut, nscut, ns, z, rho, L, R, beta, theta_major, norm,
                             t_start, tau)
                 : Slot{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
             const int active = min(64, slots - group);
@@ -989,8 +1017,8 @@ namespace {
 // The bins launch (only the active OMs, each pair costlier) cuts small
 // sources finer still.
 int chunk_of(int n_elements, bool bins) {
-    const int chunk = n_elements >= 65536 ? kChunk : n_elements >= 16384 ? 256
-                    : n_elements >= 4096 ? 128 : n_elements >= 1024 ? 64 : 32;
+    const int chunk = n_elements >= 32768 ? kChunk : n_elements >= 8192 ? 256
+                    : n_elements >= 2048 ? 128 : n_elements >= 512 ? 64 : 32;
     return (bins && n_elements < 65536) ? std::max(16, chunk / 2) : chunk;
 }
 
@@ -1080,6 +1108,7 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     P.nchunk = (int)L.nchunk;
     P.flags = static_cast<int*>(workspace);
     P.node_count = g_node_count;
+    P.s_rule = g_s_rule;
     P.bounds = reinterpret_cast<double*>(base + L.bounds);
     P.partial = reinterpret_cast<double*>(base + L.partial);
     P.live = reinterpret_cast<int*>(base + L.live);
@@ -1125,6 +1154,13 @@ int run(O0Params P, int width, double* out, void* workspace, hipStream_t s, size
 
 extern "C" int lh_order0_count_nodes(unsigned long long* counter) {
     g_node_count = counter;
+    return LH_OK;
+}
+
+extern "C" int lh_order0_set_s_rule(int nodes) {
+    LH_REQUIRE(nodes == 0 || nodes == 2 || nodes == 3 || nodes == 4 || nodes == 6,
+               "lh_order0_set_s_rule: 0 (per pair), 2, 3, 4 or 6 nodes (got %d)", nodes);
+    g_s_rule = nodes;
     return LH_OK;
 }
 
