@@ -20,8 +20,9 @@ Workload (default): the detector-scale event: 2304 OMs (8 clusters x 8
 strings x 36 OMs), a 50,000-element Gaussian shower of seed 11 with a 1500 m
 charged track, 8 wavelength nodes, 1 ns bins over -60..745 ns, 65 RTE and 321
 readout frequencies, degree 8, azimuthal degree 2, radial window 2..150 m,
-order-1 selection radius 300 m. The order >=2 pack is the seeded synthetic
-pack of lighthit_amd.synthetic (the reference's pack needs its RTE solver).
+order-1 selection radius 300 m. The order >=2 pack is built from the RTE
+tables (lighthit_amd.tables, cached on disk between processes; ``--pack
+synthetic`` takes the seeded made-up pack instead).
 Sources are resident in HBM before the timed region. ``--gpus`` above 1
 exits: this build runs on one GPU.
 """
@@ -91,22 +92,24 @@ STAGE_KERNELS = {"order0": "order0_pairs<false>", "ge2_apply": "ge2_mfma_apply",
                  "readout": "readout_split_kernel"}
 
 
-def build_route(name, precision="fp32"):
-    """Workload, route (tables resident) in ``precision`` and the route kernel."""
+def build_route(name, precision="fp32", pack="rte"):
+    """Workload, route (tables resident) in ``precision`` and the route kernel;
+    ``pack`` "rte" (the order >=2 pack built from the RTE tables, cached on
+    disk) or "synthetic"."""
     from dataclasses import replace
 
     from lighthit_amd.route import ShowerTransport
     from lighthit_amd.synthetic import workload_route
-    workload, kernel, route = workload_route(name)
+    workload, kernel, route = workload_route(name, pack=pack)
     if precision != route.config.precision:
         route = ShowerTransport(route.kernel, route.data,
                                 config=replace(route.config, precision=precision))
     return workload, kernel, route
 
 
-def route_factory(name, precision="fp32"):
+def route_factory(name, precision="fp32", pack="rte"):
     """The route alone, built the same way in a serving worker process."""
-    return build_route(name, precision)[2]
+    return build_route(name, precision, pack)[2]
 
 
 # ----------------------------------------------------------- roofline models
@@ -518,7 +521,7 @@ def _counter_mean(table, name_part, counter):
     return sum(values) / len(values) if values else None
 
 
-def counter_passes(workload_flag, precision, dominant, keep_dir=None):
+def counter_passes(workload_flag, precision, dominant, keep_dir=None, pack="rte"):
     """rocprofv3 counter passes over one event of the plain route (a child
     process each): HBM bytes per launch of ``dominant`` (FETCH_SIZE and
     WRITE_SIZE, each scaled by what it reads for a plain 256 MiB copy) and
@@ -540,7 +543,8 @@ def counter_passes(workload_flag, precision, dominant, keep_dir=None):
     }
     tables = {}
     for tag, (counters, calibrate) in passes.items():
-        runs = [("bench", ["--profile-run", "--workload", workload_flag, "--precision", precision])]
+        runs = [("bench", ["--profile-run", "--workload", workload_flag, "--precision", precision,
+                           "--pack", pack])]
         if calibrate:
             runs.append(("copy", ["--copy-kernel-mib", "256"]))
         for kind, args in runs:
@@ -632,13 +636,14 @@ def time_plain(route, elements, steps, warmup, *, timers=True):
             "response": response}
 
 
-def precision_report(workload, response):
+def precision_report(workload, response, pack="rte"):
     """Charge error of ``response`` against the FP64 oracle route on the same
-    event (the committed detector-scale fixture) per order: max and median
-    relative error over the OMs whose FP64 charge of that order is above 1e-6
-    of the largest, the bins' L1 error over the active OMs, and whether the
-    active set is the same."""
-    path = os.path.join(ROOT, "tests", "golden", "detector_scale_route.npz")
+    event and pack (the committed detector-scale fixture) per order: max and
+    median relative error over the OMs whose FP64 charge of that order is above
+    1e-6 of the largest, the bins' L1 error over the active OMs, and whether
+    the active set is the same."""
+    path = os.path.join(ROOT, "tests", "golden", "detector_scale_route.npz" if pack == "synthetic"
+                        else f"detector_scale_route_{pack}.npz")
     if workload.name.split(":")[0] != "plain-detector-scale" or not os.path.exists(path):
         return None
     with np.load(path, allow_pickle=False) as f:
@@ -660,7 +665,7 @@ def precision_report(workload, response):
 
 
 def serve_processes(workload_flag, precision, detector, elements, processes, steps, warmup,
-                    stream=None, stream_plain=None, rates=()):
+                    stream=None, stream_plain=None, rates=(), pack="rte"):
     """``steps`` batches of ``processes`` events served by ``processes`` worker
     processes (sources staged in device memory before the timed region), then
     Poisson arrivals at 70% of that throughput, then the serving stream."""
@@ -668,7 +673,7 @@ def serve_processes(workload_flag, precision, detector, elements, processes, ste
 
     from lighthit_amd.serve import ProcessShowerServer, latency_summary
     events = steps * processes
-    with ProcessShowerServer(functools.partial(route_factory, workload_flag, precision),
+    with ProcessShowerServer(functools.partial(route_factory, workload_flag, precision, pack),
                              processes=processes, detector=detector,
                              device="cuda:0") as server:
         server.map([elements for _ in range(max(1, warmup) * processes)])
@@ -714,6 +719,9 @@ def main():
                         help="timed steps: batches of --processes served events")
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--workload", choices=["detector", "small", "heavy"], default="detector")
+    parser.add_argument("--pack", choices=["rte", "synthetic"], default="rte",
+                        help="order >=2 pack: built from the RTE tables (cached on disk) or the "
+                             "seeded synthetic one")
     parser.add_argument("--precision", choices=["mixed", "fp32"], default="mixed",
                         help="precision of the served form (the headline)")
     parser.add_argument("--processes", type=int, default=8,
@@ -759,15 +767,15 @@ def main():
     warnings.simplefilter("ignore", RuntimeWarning)
 
     if args.profile_run:
-        workload, kernel, route = build_route(args.workload, args.precision)
+        workload, kernel, route = build_route(args.workload, args.precision, args.pack)
         for _ in range(2):
             route(ResidentShower(workload.elements))
         torch.cuda.synchronize()
         print(json.dumps({"profile_run": workload.name}), flush=True)
         return
 
-    workload, kernel, route = build_route(args.workload, "fp32")
-    _, _, mixed = build_route(args.workload, "mixed")
+    workload, kernel, route = build_route(args.workload, "fp32", args.pack)
+    _, _, mixed = build_route(args.workload, "mixed", args.pack)
     served_route = mixed if args.precision == "mixed" else route
     elements = workload.elements
     print(f"[bench] {workload.name}: plain fp32, {args.plain_steps} events", file=sys.stderr,
@@ -787,7 +795,7 @@ def main():
     window = window_work(served_route, workload, response)
     order1_module.count_work("cuda", on=False)
     ballistic.count_nodes("cuda", on=False)
-    precision = precision_report(workload, response)
+    precision = precision_report(workload, response, args.pack)
 
     stream = stream_plain = None
     rates = [float(r) for r in args.stream_rates.split(",") if r]
@@ -807,7 +815,7 @@ def main():
               f"processes ({args.precision})", file=sys.stderr, flush=True)
         served = serve_processes(args.workload, args.precision, workload.detector, elements,
                                  args.processes, args.steps, args.warmup, stream, stream_plain,
-                                 rates)
+                                 rates, args.pack)
         results = served.pop("results")
         served["same_as_alone"] = bool(all(
             np.array_equal(r.charge_components_pe, response.charge_components_pe)
@@ -837,7 +845,7 @@ def main():
         counters, problem = (None, "not measured (--no-counters)")
         if not args.no_counters:
             counters, problem = counter_passes(args.workload, args.precision, dominant,
-                                               args.counter_dir)
+                                               args.counter_dir, args.pack)
         roofline["traffic"] = counters["traffic"] if counters else None
         roofline["traffic_note"] = counters["traffic_note"] if counters else problem
         if counters:
@@ -865,7 +873,7 @@ def main():
         "dtype": "bf16/fp16 hi+lo MFMA, fp32 accumulate; fp32/fp64 vector" if
                  args.precision == "mixed" else "fp32/fp64",
         "data": "synthetic",
-        "config": {"workload": workload.name,
+        "config": {"workload": workload.name, "order2_pack": args.pack,
                    "form": (f"served: {args.processes} worker processes on one GPU, each "
                             f"calling ShowerTransport(kernel, data)(shower) with precision "
                             f"'{args.precision}'; a step is a batch of {args.processes} events, "

@@ -248,12 +248,61 @@ def synthetic_pack(medium, quadrature, *, degree=8, acceptance_degree=2,
     return wavelengths, np.arange(len(omega)), bundle
 
 
-def workload_route(name, device="cuda"):
+def rte_pack(kernel, cache_directory=None):
+    """The order >=2 pack of ``kernel`` built from the RTE tables
+    (:func:`lighthit_amd.tables.build_pack`), kept on disk in the pack format
+    (``KernelPack.save``) under ``cache_directory`` (default: the
+    ``LIGHTHIT_AMD_PACK_CACHE`` environment variable, else ``lighthit_amd-packs``
+    in the temporary directory) and keyed by a digest of everything the build
+    reads, so later processes load it instead of building it again."""
+    import hashlib
+    import json
+    import os
+    import tempfile
+    from pathlib import Path
+
+    from .fold import KernelPack
+    from .tables import build_pack
+    c = kernel.config
+    wavelengths = np.asarray(kernel.wavelength.wavelength_nm, float)
+    samples = [kernel.medium.sample(float(w)) for w in wavelengths]
+    described = {
+        "wavelength_nm": wavelengths.tolist(),
+        "medium": [{k: float(np.asarray(v)) for k, v in sample.items()} for sample in samples],
+        "g": float(kernel.medium.g),
+        "acceptance_degree": int(kernel.acceptance()["degree"]),
+        "omega_per_ns": np.asarray(c.omega_per_ns, float).tolist(),
+        "radial_range_m": [float(x) for x in c.radial_range_m],
+        "radial_nodes": int(c.radial_nodes), "scattering_degree": int(c.scattering_degree),
+        "source_degree": int(c.source_degree), "k_max_per_m": float(c.k_max_per_m),
+        "k_panel_per_m": float(c.k_panel_per_m), "k_order": int(c.k_order)}
+    digest = hashlib.sha256(json.dumps(described, sort_keys=True).encode()).hexdigest()[:20]
+    root = Path(cache_directory or os.environ.get("LIGHTHIT_AMD_PACK_CACHE")
+                or Path(tempfile.gettempdir()) / "lighthit_amd-packs")
+    folder = root / f"ge2-pack-{digest}"
+    if (folder / "pack.json").exists():
+        return KernelPack.load(folder)
+    pack = KernelPack()
+    pack.add(*build_pack(kernel))
+    root.mkdir(parents=True, exist_ok=True)
+    partial = Path(tempfile.mkdtemp(prefix=folder.name + ".", dir=root))
+    pack.save(partial)
+    try:
+        os.replace(partial, folder)          # another process may have won the race
+    except OSError:
+        pass
+    return KernelPack.load(folder) if (folder / "pack.json").exists() else pack
+
+
+def workload_route(name, device="cuda", pack="synthetic"):
     """Workload ``name`` ("detector", "small" or "heavy"), its route kernel and the
     route (tables resident on ``device`` on first use): the configuration the
-    benchmark and the parity fixtures run. The order-1 selection radius is 300 m
-    (1000 m for the heavy event's 10 km track): the default guard of the radial
-    range (150 m) raises for these events."""
+    benchmark and the parity fixtures run. ``pack`` is "synthetic" (the seeded
+    made-up pack of :func:`synthetic_pack`, whose cost does not depend on its
+    values) or "rte" (the pack built from the RTE tables, :func:`rte_pack`).
+    The order-1 selection radius is 300 m (1000 m for the heavy event's 10 km
+    track): the default guard of the radial range (150 m) raises for these
+    events."""
     from .fold import KernelPack
     from .route import RouteConfig, RouteKernel, ShowerConfig, ShowerPrompt, ShowerTransport
     workload = {"detector": plain_detector_scale, "small": plain_small,
@@ -267,8 +316,13 @@ def workload_route(name, device="cuda"):
                                      readout_omega_per_ns=workload.readout_omega_per_ns,
                                      scattering_degree=degree, source_degree=degree,
                                      azimuthal_degree=azimuthal))
-    pack = KernelPack()
-    pack.add(*synthetic_pack(workload.medium, kernel.wavelength, degree=degree,
-                             omega=workload.omega_per_ns))
+    if pack == "rte":
+        tables = rte_pack(kernel)
+    elif pack == "synthetic":
+        tables = KernelPack()
+        tables.add(*synthetic_pack(workload.medium, kernel.wavelength, degree=degree,
+                                   omega=workload.omega_per_ns))
+    else:
+        raise ValueError(f"pack must be 'synthetic' or 'rte', not {pack!r}")
     config = ShowerConfig(prompt=ShowerPrompt(max_distance_m=1000.0 if heavy else 300.0))
-    return workload, kernel, ShowerTransport(kernel, pack, config=config, device=device)
+    return workload, kernel, ShowerTransport(kernel, tables, config=config, device=device)

@@ -3,16 +3,18 @@
 The event: 2304-OM array (8 clusters x 8 strings x 36 OMs), 50,000-element
 Gaussian shower with a 1500 m charged track from seed 11, 8 wavelength nodes,
 1 ns bins over -60..744 ns, order-1 selection radius 300 m, and the seeded
-synthetic order >=2 pack of ``lighthit_amd.synthetic.synthetic_pack`` (the same
-inputs as the benchmark). The node bands are the package's choice (the best
+synthetic order >=2 pack of ``lighthit_amd.synthetic.synthetic_pack``; with
+``--pack rte`` the pack built from the RTE tables
+(``lighthit_amd.synthetic.rte_pack``, the benchmark's) and the output
+detector_scale_route_rte.npz. The node bands are the package's choice (the best
 subsets tie to rounding; the oracle checks the choice is optimal and computes
 the interpolation itself).
 
 Stored: charge per order (OM, 3), active, ge2 charge level, time origins, the
 order-1 level, bins per order of the active OMs (float32), the order >=2
 spectrum of the active OMs (complex64), the node bands and the workload's
-element count. Run from the repository root: ``python tests/golden/make_detector_scale_route.py``
-(several minutes on 8 cores; the oracle's C parts use OpenMP).
+element count. Run from the repository root: ``python tests/golden/make_detector_scale_route.py
+[--pack rte]`` (several minutes on 8 cores; the oracle's C parts use OpenMP).
 """
 import sys
 import time
@@ -22,14 +24,17 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
-OUT = Path(__file__).resolve().parent / "detector_scale_route.npz"
+HERE = Path(__file__).resolve().parent
 
 
 def main():
     from lighthit_amd.fold import BandFold
     from lighthit_amd.synthetic import workload_route
     from oracle import route as oracle_route
-    workload, kernel, route = workload_route("detector", device="cpu")
+    pack_kind = sys.argv[sys.argv.index("--pack") + 1] if "--pack" in sys.argv else "synthetic"
+    OUT = HERE / ("detector_scale_route.npz" if pack_kind == "synthetic"
+                  else f"detector_scale_route_{pack_kind}.npz")
+    workload, kernel, route = workload_route("detector", device="cpu", pack=pack_kind)
     pack = route.data.ge2_pack
     bundle = pack.bundle(kernel.wavelength.wavelength_nm, int(kernel.config.source_degree),
                          np.arange(65))
