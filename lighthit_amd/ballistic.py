@@ -63,7 +63,7 @@ _lib.SIGNATURES["lh_order0_bins"] = (ctypes.c_int, [ctypes.POINTER(_Optics), cty
                                                     ctypes.c_void_p])
 
 _lib.SIGNATURES["lh_order0_count_nodes"] = (ctypes.c_int, [ctypes.c_void_p])
-_lib.SIGNATURES["lh_order0_set_s_rule"] = (ctypes.c_int, [ctypes.c_int])
+_lib.SIGNATURES["lh_order0_set_rules"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int])
 
 FLAG_GLASS = 1
 FLAG_UNSUPPORTED = 2
@@ -89,12 +89,15 @@ def node_count():
     return None if _node_counter is None else int(_node_counter.item())
 
 
-def set_s_rule(nodes=0):
-    """Gauss nodes per s piece of every later order-0 launch: 0 (default) picks,
-    for the charge, 3 for elements at most a tenth of the glass radius long and
-    6 otherwise, and 6 for the bins; 2, 3, 4 or 6 force that rule (to compare
-    rule orders)."""
-    _lib.call("lh_order0_set_s_rule", int(nodes))
+def set_rules(s_nodes=0, azimuth_nodes=0):
+    """Quadrature rules of every later order-0 launch. ``s_nodes`` 0 (default)
+    picks, for the charge, 3 Gauss nodes per s piece for elements at most a
+    tenth of the glass radius long and 6 otherwise, and 6 for the bins; 2, 3, 4
+    or 6 force that rule. ``azimuth_nodes`` 0 (default) picks, for the charge, 4
+    azimuth nodes when the glass radius is at most 1/20 of the OM's distance
+    from the element's line and 8 otherwise, and 8 for the bins; 4 or 8 force
+    that rule (to compare rule orders)."""
+    _lib.call("lh_order0_set_rules", int(s_nodes), int(azimuth_nodes))
 
 
 def _dev(array, dtype, device):

@@ -43,7 +43,7 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxNs = 6;          // most s nodes per s piece
 constexpr int kNt = 8;             // theta nodes per medium panel
-constexpr int kNe = 8;             // azimuth nodes on [0, pi/2]
+constexpr int kNe = 8;             // azimuth nodes on [0, pi/2] (4 for small caps)
 constexpr int kMaxMajor = 12;      // medium nodes inside the band (+ ends)
 constexpr int kMaxPanels = 256;    // efficiency panels
 constexpr int kMaxAlpha = 8;
@@ -83,6 +83,12 @@ __constant__ float kEtaSin[8] = {3.1183217838e-02f, 1.5901983477e-01f, 3.6408107
 __constant__ float kEtaCos[8] = {9.9951368521e-01f, 9.8727538820e-01f, 9.3136725806e-01f, 8.0130141766e-01f, 5.9826084449e-01f, 3.6408107699e-01f, 1.5901983477e-01f, 3.1183217838e-02f};
 __constant__ float kEtaOneMinus[8] = {9.6881678216e-01f, 8.4098016523e-01f, 6.3591892301e-01f, 4.0173915551e-01f, 1.9869858234e-01f, 6.8632741945e-02f, 1.2724611798e-02f, 4.8631478841e-04f};
 __constant__ float kEtaWeight[8] = {7.9504706486e-02f, 1.7465765603e-01f, 2.4638462352e-01f, 2.8485117736e-01f, 2.8485117736e-01f, 2.4638462352e-01f, 1.7465765603e-01f, 7.9504706486e-02f};
+// The same with the 4-point Gauss nodes (caps far smaller than the azimuth
+// scale of the acceptance and attenuation).
+__constant__ float kEta4Sin[4] = {1.0884720006e-01f, 4.9547160085e-01f, 8.6862413779e-01f, 9.9405849277e-01f};
+__constant__ float kEta4Cos[4] = {9.9405849277e-01f, 8.6862413779e-01f, 4.9547160085e-01f, 1.0884720006e-01f};
+__constant__ float kEta4OneMinus[4] = {8.9115279994e-01f, 5.0452839915e-01f, 1.3137586221e-01f, 5.9415072342e-03f};
+__constant__ float kEta4Weight[4] = {2.7320455650e-01f, 5.1219360690e-01f, 5.1219360690e-01f, 2.7320455650e-01f};
 
 struct O0Params {
     // OMs
@@ -128,15 +134,17 @@ struct O0Params {
     int* flags;               // [0]: glass intersected, [1]: unsupported geometry
     unsigned long long* node_count;   // optional: (theta, azimuth) nodes evaluated
     int s_rule;               // 0: s nodes per pair from its length; else that many
+    int azimuth_rule;         // 0: azimuth nodes per pair from its distance; else that many
 };
 
 // Device counter of evaluated quadrature nodes (null: not counted); set by
 // lh_order0_count_nodes and read by every later launch in the process.
 static unsigned long long* g_node_count = nullptr;
 
-// s rule of later launches (0: chosen per pair; 2, 3, 4, 6: forced), set by
-// lh_order0_set_s_rule.
+// Rules of later launches (0: chosen per pair; s: 2, 3, 4, 6 and azimuth: 4, 8
+// forced), set by lh_order0_set_rules.
 static int g_s_rule = 0;
+static int g_azimuth_rule = 0;
 
 // ---------------------------------------------------------- candidate test
 
@@ -249,6 +257,17 @@ __device__ __forceinline__ int s_nodes(double L, double R, int rule) {
     return (!kBins && L <= 0.1 * R) ? 3 : kMaxNs;
 }
 
+// Azimuth nodes of a pair's charge: 4 when the glass is at most 1/20 of the
+// OM's distance from the element's line (the cap's azimuth half-width is then
+// below ~0.08 rad, where the acceptance and attenuation are nearly linear and
+// 4 Gauss nodes in eta agree with 8 to below 1e-7), 8 otherwise; the bins
+// keep 8. `rule` > 0 forces 4 or 8.
+template <bool kBins>
+__device__ __forceinline__ int azimuth_nodes(double rho, double R, int rule) {
+    if (rule > 0) return rule;
+    return (!kBins && R <= 0.05 * rho) ? 4 : kNe;
+}
+
 __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double* scut, int nscut,
                           int ns, double z, double rho, double L, double R, double beta,
                           const double* theta_major, double norm, double t_start, double tau) {
@@ -344,6 +363,7 @@ __device__ __forceinline__ float sin_small(float x) {
 
 // Value of one slot at the lane's theta node, summed over the 8 azimuth
 // nodes (without the slot weight).
+template <int NE>
 __device__ float theta_node_value(const PanelTables& T, const Slot& S, int npanel, int nalpha,
                                   float beta, float c0, float c2, float inv_length, float axial,
                                   float across, float side, float xt, float wt) {
@@ -382,8 +402,10 @@ __device__ float theta_node_value(const PanelTables& T, const Slot& S, int npane
     const float rest = 1.f - S.hav_eps;
     float sum = 0.f;
 #pragma unroll
-    for (int ie = 0; ie < kNe; ++ie) {
-        const float se = kEtaSin[ie], ce = kEtaCos[ie], em = kEtaOneMinus[ie];
+    for (int ie = 0; ie < NE; ++ie) {
+        const float se = NE == 4 ? kEta4Sin[ie] : kEtaSin[ie];
+        const float ce = NE == 4 ? kEta4Cos[ie] : kEtaCos[ie];
+        const float em = NE == 4 ? kEta4OneMinus[ie] : kEtaOneMinus[ie];
         const float ep = 1.f + se;
         const float phi = half * se;
         const float hphi = sin_small(0.5f * phi);
@@ -398,7 +420,7 @@ __device__ float theta_node_value(const PanelTables& T, const Slot& S, int npane
         const float sp = 2.f * hphi * sqrtf(fmaxf(0.f, 1.f - hphi * hphi));
         const float acc = acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across + sp * side))
                         + acceptance(T.alpha, nalpha, cth * axial + sth * (cp * across - sp * side));
-        sum += half * ce * kEtaWeight[ie] * acc * __expf(-mu * d);
+        sum += half * ce * (NE == 4 ? kEta4Weight[ie] : kEtaWeight[ie]) * acc * __expf(-mu * d);
     }
     return hw * wt * S.eps * ct * spectral * jac * sum;
 }
@@ -558,13 +580,31 @@ __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int n
         const float Tedge = edges[e];
         for (int j = 0; j < 8 && ncut < kMaxCuts; ++j) {
             if ((tv[j] < Tedge) == (tv[j + 1] < Tedge)) continue;
+            // the crossing by the Illinois method (regula falsi whose kept end
+            // has its value halved when that end is kept twice in a row): the
+            // bracket [a, b] keeps the sign change, so every step stays inside
+            // the sub-interval; 8 steps reach the FP32 resolution of the times
             float a = tt[j], b = tt[j + 1];
-            const bool rising = tv[j] < Tedge;
-            for (int it = 0; it < 24; ++it) {
-                const float mid = 0.5f * (a + b);
-                if ((meridian_time(T, S, beta, mid) < Tedge) == rising) a = mid; else b = mid;
+            float fa = tv[j] - Tedge, fb = tv[j + 1] - Tedge;
+            float c = a;
+            int kept = 0;
+            for (int it = 0; it < 8; ++it) {
+                c = (a * fb - b * fa) / (fb - fa);
+                const float fc = meridian_time(T, S, beta, c) - Tedge;
+                if (fc == 0.f) break;
+                if ((fc < 0.f) == (fb < 0.f)) {
+                    b = c;
+                    fb = fc;
+                    if (kept == 1) fa *= 0.5f;
+                    kept = 1;
+                } else {
+                    a = c;
+                    fa = fc;
+                    if (kept == 2) fb *= 0.5f;
+                    kept = 2;
+                }
             }
-            cuts[ncut++] = 0.5f * (a + b);
+            cuts[ncut++] = c;
         }
         // rim crossing: n_g = (T - base) c / (D cos eps)
         if (T.dg[k] != 0.f && ncut < kMaxCuts) {
@@ -906,6 +946,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const double* scut = wave_scut[wave];
         const double* theta_major = wave_theta[wave];
         const int ns = s_nodes<kBins>(L, R, P.s_rule);
+        const int ne = azimuth_nodes<kBins>(rho, R, P.azimuth_rule);
         const int slots = (nscut - 1) * ns * pieces;
         const float c0 = (float)P.c0[i], c2 = (float)P.c2[i];
         const float inv_length = (float)(1.0 / L);
@@ -925,16 +966,18 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
                         slot_bins(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length, axial,
                                   across, side, xt, wt, (float)R, edges, P.nbins, wave_bins);
                     else
-                        pair_sum += S.weight * theta_node_value(T, S, P.npanel, P.nalpha, (float)beta,
-                                                                c0, c2, inv_length, axial, across,
-                                                                side, xt, wt);
+                        pair_sum += S.weight * (ne == 4
+                            ? theta_node_value<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
+                                                  inv_length, axial, across, side, xt, wt)
+                            : theta_node_value<kNe>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
+                                                    inv_length, axial, across, side, xt, wt));
                 }
             }
         }
         lane_sum += (double)pair_sum;
-        if (P.node_count != nullptr) {   // one theta node per call, kNe azimuth nodes each
+        if (P.node_count != nullptr) {   // one theta node per call, ne azimuth nodes each
             for (int offset = 32; offset > 0; offset >>= 1) evaluated += __shfl_down(evaluated, offset);
-            if (lane == 0) atomicAdd(P.node_count, (unsigned long long)evaluated * kNe);
+            if (lane == 0) atomicAdd(P.node_count, (unsigned long long)evaluated * ne);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1109,6 +1152,7 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     P.flags = static_cast<int*>(workspace);
     P.node_count = g_node_count;
     P.s_rule = g_s_rule;
+    P.azimuth_rule = g_azimuth_rule;
     P.bounds = reinterpret_cast<double*>(base + L.bounds);
     P.partial = reinterpret_cast<double*>(base + L.partial);
     P.live = reinterpret_cast<int*>(base + L.live);
@@ -1157,10 +1201,13 @@ extern "C" int lh_order0_count_nodes(unsigned long long* counter) {
     return LH_OK;
 }
 
-extern "C" int lh_order0_set_s_rule(int nodes) {
-    LH_REQUIRE(nodes == 0 || nodes == 2 || nodes == 3 || nodes == 4 || nodes == 6,
-               "lh_order0_set_s_rule: 0 (per pair), 2, 3, 4 or 6 nodes (got %d)", nodes);
-    g_s_rule = nodes;
+extern "C" int lh_order0_set_rules(int s_nodes, int azimuth_nodes) {
+    LH_REQUIRE(s_nodes == 0 || s_nodes == 2 || s_nodes == 3 || s_nodes == 4 || s_nodes == 6,
+               "lh_order0_set_rules: s nodes 0 (per pair), 2, 3, 4 or 6 (got %d)", s_nodes);
+    LH_REQUIRE(azimuth_nodes == 0 || azimuth_nodes == 4 || azimuth_nodes == 8,
+               "lh_order0_set_rules: azimuth nodes 0 (per pair), 4 or 8 (got %d)", azimuth_nodes);
+    g_s_rule = s_nodes;
+    g_azimuth_rule = azimuth_nodes;
     return LH_OK;
 }
 

@@ -4,7 +4,9 @@ fixture (tests/golden/make_detector_scale_route.py), at the bars of the plain
 form: charge per order within 1e-5 relative (order 1 1e-6) on every OM above
 threshold, the same ``active`` set except OMs within that tolerance of the
 threshold, and bins per order against each active OM's bin L1. The precision
-form ("mixed") must hold every active OM's charge within 1e-3 of the FP64 oracle."""
+form ("mixed") must hold every active OM's charge within 1e-3 of the FP64 oracle.
+Both order >=2 packs run: the seeded synthetic one and the one built from the
+RTE tables (the benchmark's), each against its own fixture."""
 from pathlib import Path
 
 import numpy as np
@@ -12,16 +14,21 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-FIXTURE = Path(__file__).resolve().parent / "golden" / "detector_scale_route.npz"
+GOLDEN = Path(__file__).resolve().parent / "golden"
+FIXTURES = {"synthetic": GOLDEN / "detector_scale_route.npz",
+            "rte": GOLDEN / "detector_scale_route_rte.npz"}
+ACTIVE = {"synthetic": 125, "rte": 38}
 
 
-@pytest.fixture(scope="module")
-def event(device):
+@pytest.fixture(scope="module", params=["synthetic", "rte"])
+def event(device, request):
     from lighthit_amd.synthetic import workload_route
-    want = dict(np.load(FIXTURE, allow_pickle=False))
-    workload, kernel, route = workload_route("detector", device=device)
+    pack = request.param
+    want = dict(np.load(FIXTURES[pack], allow_pickle=False))
+    want["pack"] = pack
+    print(f"detector scale: 2304 OMs, 50000 elements, plain form, {pack} pack", flush=True)
+    workload, kernel, route = workload_route("detector", device=device, pack=pack)
     assert len(workload.elements) == int(want["elements"])
-    print("detector scale: 2304 OMs, 50000 elements, plain form", flush=True)
     response = route(workload.elements)
     return workload, kernel, route, response, want
 
@@ -34,7 +41,7 @@ def test_active_set_and_levels(event):
     differ = response.active != want["active"]
     # only OMs within the charge tolerance of the threshold may differ
     assert np.all(np.abs(total[differ] - threshold) <= 1e-5 * threshold), np.flatnonzero(differ)
-    assert want["active"].sum() == 125
+    assert want["active"].sum() == ACTIVE[want["pack"]]
     assert np.array_equal(response.metadata["ge2_charge_level"], want["level"].tolist())
     assert np.array_equal(response.metadata["prompt_order1_level"], want["order1_level"].tolist())
     assert response.metadata["ge2_predictor"]["fallback_to_exact"] == bool(want["fallback"])

@@ -120,10 +120,11 @@ def test_fine_bins_split_chords(device):
                   acceptance_alpha(w.detector), ids, np.zeros(2), edges, 1e-3)
 
 
-def test_s_rule_convergence(device):
-    # Error estimates of the s rule on the demonstration event: the charge's
-    # per-pair rule (3 nodes for these 1 cm elements) against the 6-node rule
-    # on every OM, and the bins' 6-node rule against 4 nodes.
+def test_rule_convergence(device):
+    # Error estimates of the per-pair rules on the demonstration event: the
+    # charge's rules (3 s nodes for these 1 cm elements, 4 azimuth nodes past
+    # 20 glass radii) against 6 s and 8 azimuth nodes on every OM, and the
+    # bins' 6 s nodes against 4.
     from lighthit_amd import ballistic
     from lighthit_amd.ballistic import Order0
     from lighthit_amd.synthetic import plain_small
@@ -136,21 +137,25 @@ def test_s_rule_convergence(device):
     origins = e.start_ns[first] + np.linalg.norm(d.positions_m[ids] - e.start_m[first], axis=1) \
         / (0.299792458 / 1.367)
     try:
-        ballistic.set_s_rule(6)
-        q6 = order0.charge(e)
-        ballistic.set_s_rule(4)
+        ballistic.set_rules(6, 8)
+        q_full = order0.charge(e)
+        ballistic.set_rules(0, 8)
+        q_s = order0.charge(e)
+        ballistic.set_rules(4, 0)
         b4 = order0.bins(e, ids, origins, w.edges_ns)
-        ballistic.set_s_rule(0)
+        ballistic.set_rules(0, 0)
         q = order0.charge(e)
         b = order0.bins(e, ids, origins, w.edges_ns)
     finally:
-        ballistic.set_s_rule(0)
-    lit = q6 > 1e-6
-    err = np.abs(q[lit] / q6[lit] - 1)
-    l1 = np.abs(b - b4).sum(axis=1) / q6[ids]
-    print("charge: s rule 3 vs 6", err.max(), "; bins: s rule 6 vs 4, L1", l1.max())
+        ballistic.set_rules(0, 0)
+    lit = q_full > 1e-6
+    err = np.abs(q[lit] / q_full[lit] - 1)
+    err_s = np.abs(q_s[lit] / q_full[lit] - 1)
+    l1 = np.abs(b - b4).sum(axis=1) / q_full[ids]
+    print("charge: rules per pair vs 6 s x 8 azimuth nodes", err.max(), "(s rule alone",
+          err_s.max(), "); bins: s rule 6 vs 4, L1", l1.max())
     assert err.max() < 1e-6
-    assert not np.array_equal(q, q6)      # the two rules really differ
+    assert not np.array_equal(q, q_s) and not np.array_equal(q_s, q_full)   # rules differ
     # the bins' bar against the oracle is 1e-4 of the L1: the 4-node rule is
     # already within half of it, so the 6-node rule's own error is well below
     assert l1.max() < 5e-5
