@@ -261,10 +261,10 @@ int lh_order0_count_nodes(unsigned long long* counter);
  * order-0 launch of the process. s_nodes 0 (the default) chooses per pair:
  * for the charge 3 Gauss nodes per s piece when the element is at most a
  * tenth of the glass radius long and 6 otherwise, for the bins 6; 2, 3, 4 or
- * 6 force that rule. azimuth_nodes 0 (the default) chooses per pair: for the
- * charge 4 azimuth nodes when the glass radius is at most 1/20 of the OM's
- * distance from the element's line and 8 otherwise, for the bins 8; 4 or 8
- * force that rule. A caller compares rule orders with it (the quadrature's
+ * 6 force that rule. azimuth_nodes 0 (the default) chooses per pair: 4
+ * azimuth nodes (per azimuth piece of the bins) when the glass radius is at
+ * most 1/20 of the OM's distance from the element's line and 8 otherwise; 4
+ * or 8 force that rule. A caller compares rule orders with it (the quadrature's
  * error estimate). A process-wide setting, not for concurrent callers.
  */
 int lh_order0_set_rules(int s_nodes, int azimuth_nodes);

@@ -93,10 +93,10 @@ def set_rules(s_nodes=0, azimuth_nodes=0):
     """Quadrature rules of every later order-0 launch. ``s_nodes`` 0 (default)
     picks, for the charge, 3 Gauss nodes per s piece for elements at most a
     tenth of the glass radius long and 6 otherwise, and 6 for the bins; 2, 3, 4
-    or 6 force that rule. ``azimuth_nodes`` 0 (default) picks, for the charge, 4
-    azimuth nodes when the glass radius is at most 1/20 of the OM's distance
-    from the element's line and 8 otherwise, and 8 for the bins; 4 or 8 force
-    that rule (to compare rule orders)."""
+    or 6 force that rule. ``azimuth_nodes`` 0 (default) picks 4 azimuth nodes
+    (per azimuth piece of the bins) when the glass radius is at most 1/20 of the
+    OM's distance from the element's line and 8 otherwise; 4 or 8 force that
+    rule (to compare rule orders)."""
     _lib.call("lh_order0_set_rules", int(s_nodes), int(azimuth_nodes))
 
 

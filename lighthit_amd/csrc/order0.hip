@@ -57,6 +57,10 @@ constexpr double kPi = 3.14159265358979323846;
 __constant__ float kGauss8x[8] = {-0.960289856497536f, -0.796666477413627f, -0.525532409916329f,
                                   -0.183434642495650f, 0.183434642495650f, 0.525532409916329f,
                                   0.796666477413627f, 0.960289856497536f};
+__constant__ float kGauss4x[4] = {-0.861136311594053f, -0.339981043584856f, 0.339981043584856f,
+                                  0.861136311594053f};
+__constant__ float kGauss4w[4] = {0.347854845137454f, 0.652145154862546f, 0.652145154862546f,
+                                  0.347854845137454f};
 __constant__ float kGauss8w[8] = {0.101228536290376f, 0.222381034453374f, 0.313706645877887f,
                                   0.362683783378362f, 0.362683783378362f, 0.313706645877887f,
                                   0.222381034453374f, 0.101228536290376f};
@@ -257,15 +261,14 @@ __device__ __forceinline__ int s_nodes(double L, double R, int rule) {
     return (!kBins && L <= 0.1 * R) ? 3 : kMaxNs;
 }
 
-// Azimuth nodes of a pair's charge: 4 when the glass is at most 1/20 of the
-// OM's distance from the element's line (the cap's azimuth half-width is then
-// below ~0.08 rad, where the acceptance and attenuation are nearly linear and
-// 4 Gauss nodes in eta agree with 8 to below 1e-7), 8 otherwise; the bins
-// keep 8. `rule` > 0 forces 4 or 8.
-template <bool kBins>
+// Azimuth nodes of a pair (per azimuth piece for the bins): 4 when the glass
+// is at most 1/20 of the OM's distance from the element's line (the cap's
+// azimuth half-width is then below ~0.08 rad, where the acceptance and
+// attenuation are nearly linear and 4 Gauss nodes in eta agree with 8 to
+// below 1e-7), 8 otherwise. `rule` > 0 forces 4 or 8.
 __device__ __forceinline__ int azimuth_nodes(double rho, double R, int rule) {
     if (rule > 0) return rule;
-    return (!kBins && R <= 0.05 * rho) ? 4 : kNe;
+    return R <= 0.05 * rho ? 4 : kNe;
 }
 
 __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double* scut, int nscut,
@@ -474,6 +477,7 @@ __device__ __forceinline__ float meridian_time(const PanelTables& T, const Slot&
 // included): the azimuth chord [0, half] is split at every bin edge its
 // arrival time crosses (the time grows with the azimuth); each piece is
 // integrated with the 8-point rule and added to the bin it lies in.
+template <int NE>
 __device__ void node_bins(const PanelTables& T, const Slot& S, int npanel, int nalpha, float beta,
                           float c0, float c2, float inv_length, float axial, float across,
                           float side, float t, float wt_t, float R, const float* edges, int nbins,
@@ -512,9 +516,9 @@ __device__ void node_bins(const PanelTables& T, const Slot& S, int npanel, int n
             const float hwe = 0.5f * (eta_b - eta_a), mide = 0.5f * (eta_a + eta_b);
             float sum = 0.f;
 #pragma unroll
-            for (int ie = 0; ie < kNe; ++ie) {
+            for (int ie = 0; ie < NE; ++ie) {
                 float se, ce;
-                sincosf(mide + hwe * kGauss8x[ie], &se, &ce);
+                sincosf(mide + hwe * (NE == 4 ? kGauss4x[ie] : kGauss8x[ie]), &se, &ce);
                 const float em = ce * ce / (1.f + se), ep = 1.f + se;
                 const float phi = p.half * se;
                 const float hphi = sin_small(0.5f * phi);
@@ -527,7 +531,8 @@ __device__ void node_bins(const PanelTables& T, const Slot& S, int npanel, int n
                 const float sp = 2.f * hphi * sqrtf(fmaxf(0.f, 1.f - hphi * hphi));
                 const float acc = acceptance(T.alpha, nalpha, p.cth * axial + p.sth * (cp * across + sp * side))
                                 + acceptance(T.alpha, nalpha, p.cth * axial + p.sth * (cp * across - sp * side));
-                sum += p.half * ce * hwe * kGauss8w[ie] * acc * __expf(-p.mu * d);
+                sum += p.half * ce * hwe * (NE == 4 ? kGauss4w[ie] : kGauss8w[ie]) * acc
+                       * __expf(-p.mu * d);
             }
             atomicAdd(&wave_bins[bin], (double)(prefactor * sum));
         }
@@ -545,6 +550,7 @@ constexpr int kMaxCuts = 12;
 // crossing is where the group index takes one value: closed form); every
 // piece gets the lane's node on a sine map, which absorbs the square-root
 // behaviour of the bin weights at the cuts.
+template <int NE>
 __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int nalpha, float beta,
                           float c0, float c2, float inv_length, float axial, float across,
                           float side, float xt, float wt, float R, const float* edges, int nbins,
@@ -641,7 +647,7 @@ __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int n
         const float pa = cuts[piece], pb = cuts[piece + 1];
         if (!(pb > pa)) continue;
         const float hw = 0.5f * (pb - pa);
-        node_bins(T, S, npanel, nalpha, beta, c0, c2, inv_length, axial, across, side,
+        node_bins<NE>(T, S, npanel, nalpha, beta, c0, c2, inv_length, axial, across, side,
                   0.5f * (pa + pb) + hw * sxm, hw * cxm * 1.57079632679f * wt, R, edges, nbins,
                   wave_bins);
     }
@@ -946,7 +952,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const double* scut = wave_scut[wave];
         const double* theta_major = wave_theta[wave];
         const int ns = s_nodes<kBins>(L, R, P.s_rule);
-        const int ne = azimuth_nodes<kBins>(rho, R, P.azimuth_rule);
+        const int ne = azimuth_nodes(rho, R, P.azimuth_rule);
         const int slots = (nscut - 1) * ns * pieces;
         const float c0 = (float)P.c0[i], c2 = (float)P.c2[i];
         const float inv_length = (float)(1.0 / L);
@@ -963,8 +969,14 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
                 if (k0 + sub < active && S.piece >= 0) {
                     ++evaluated;
                     if (kBins)
-                        slot_bins(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length, axial,
-                                  across, side, xt, wt, (float)R, edges, P.nbins, wave_bins);
+                        if (ne == 4)
+                            slot_bins<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length,
+                                         axial, across, side, xt, wt, (float)R, edges, P.nbins,
+                                         wave_bins);
+                        else
+                            slot_bins<kNe>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
+                                           inv_length, axial, across, side, xt, wt, (float)R, edges,
+                                           P.nbins, wave_bins);
                     else
                         pair_sum += S.weight * (ne == 4
                             ? theta_node_value<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,

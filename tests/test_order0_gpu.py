@@ -123,8 +123,8 @@ def test_fine_bins_split_chords(device):
 def test_rule_convergence(device):
     # Error estimates of the per-pair rules on the demonstration event: the
     # charge's rules (3 s nodes for these 1 cm elements, 4 azimuth nodes past
-    # 20 glass radii) against 6 s and 8 azimuth nodes on every OM, and the
-    # bins' 6 s nodes against 4.
+    # 20 glass radii) against 6 s and 8 azimuth nodes on every OM; the bins'
+    # azimuth rule against 8 nodes and their 6 s nodes against 4.
     from lighthit_amd import ballistic
     from lighthit_amd.ballistic import Order0
     from lighthit_amd.synthetic import plain_small
@@ -143,6 +143,8 @@ def test_rule_convergence(device):
         q_s = order0.charge(e)
         ballistic.set_rules(4, 0)
         b4 = order0.bins(e, ids, origins, w.edges_ns)
+        ballistic.set_rules(0, 8)
+        b8 = order0.bins(e, ids, origins, w.edges_ns)
         ballistic.set_rules(0, 0)
         q = order0.charge(e)
         b = order0.bins(e, ids, origins, w.edges_ns)
@@ -152,8 +154,11 @@ def test_rule_convergence(device):
     err = np.abs(q[lit] / q_full[lit] - 1)
     err_s = np.abs(q_s[lit] / q_full[lit] - 1)
     l1 = np.abs(b - b4).sum(axis=1) / q_full[ids]
+    l1_azimuth = np.abs(b - b8).sum(axis=1) / q_full[ids]
     print("charge: rules per pair vs 6 s x 8 azimuth nodes", err.max(), "(s rule alone",
-          err_s.max(), "); bins: s rule 6 vs 4, L1", l1.max())
+          err_s.max(), "); bins: s rule 6 vs 4, L1", l1.max(), "; azimuth rule vs 8, L1",
+          l1_azimuth.max())
+    assert l1_azimuth.max() < 1e-6
     assert err.max() < 1e-6
     assert not np.array_equal(q, q_s) and not np.array_equal(q_s, q_full)   # rules differ
     # the bins' bar against the oracle is 1e-4 of the L1: the 4-node rule is
