@@ -50,6 +50,21 @@ int lh_timing_read(const char* kernel, double* total_ms, long long* launches);
 int lh_timing_reset(void);
 
 /*
+ * HIP graphs of a launch sequence (no counterpart in the reference: the
+ * serving form replays the fused order-0 launch from one). lh_graph_begin
+ * starts capturing `stream` (thread-local mode; not the null stream);
+ * every launch the calling thread then enqueues on it is recorded, not run;
+ * lh_graph_end stops the capture and instantiates it into *graph_exec;
+ * lh_graph_launch replays it on a stream; lh_graph_destroy frees it. Only
+ * sequences of kernel launches belong in a graph here (no memsets, no copies
+ * from pageable memory).
+ */
+int lh_graph_begin(void* stream);
+int lh_graph_end(void* stream, void** graph_exec);
+int lh_graph_launch(void* graph_exec, void* stream);
+int lh_graph_destroy(void* graph_exec);
+
+/*
  * Stage (c): order >=2 time bins of OMs from their spectra.
  *
  * Replaces, for the order >=2 column of the shower readout,
@@ -211,6 +226,34 @@ size_t lh_order0_workspace_bytes(int n_oms, int n_elements);
 int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
                      const lh_order0_elements* elements, double* charge, void* workspace,
                      size_t workspace_bytes, void* stream);
+
+/*
+ * Fused order-0 charge of several events on one set of OMs (the serving
+ * form): one persistent launch over the (event, OM, chunk) items of all of
+ * them, in place of one lh_order0_charge per event. The reference evaluates
+ * one event per call (PreparedBallistic.charge, _surface_ballistic.py:91-112,
+ * reached once per ShowerTransport call, shower/api.py:176-190); every
+ * event's charges equal its own lh_order0_charge result bit for bit.
+ *
+ * Two steps, so the second can be captured in a HIP graph once per capacity
+ * and replayed: lh_order0_batch_set writes the table of this batch's events
+ * (at most max_events, their element chunks at most max_chunks; outputs
+ * charges[k] (oms->count) double each) into the workspace with a kernel;
+ * lh_order0_charge_batch runs the launch chain, every grid sized by
+ * (oms->count, max_events, max_chunks) and every event size read from the
+ * table on the device. lh_order0_batch_chunks gives the chunks of a batch
+ * (from its element counts), lh_order0_batch_workspace_bytes the workspace of
+ * a capacity. The flag words of the workspace report as for lh_order0_charge,
+ * for the batch as a whole.
+ */
+size_t lh_order0_batch_workspace_bytes(int n_oms, int max_events, long long max_chunks);
+long long lh_order0_batch_chunks(const int* element_counts, int n_events);
+int lh_order0_batch_set(const lh_order0_oms* oms, const lh_order0_elements* events,
+                        double* const* charges, int n_events, int max_events, long long max_chunks,
+                        void* workspace, size_t workspace_bytes, void* stream);
+int lh_order0_charge_batch(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                           int max_events, long long max_chunks, void* workspace,
+                           size_t workspace_bytes, void* stream);
 
 /*
  * lh_order0_charge without a workspace argument and without any device

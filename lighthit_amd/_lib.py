@@ -31,6 +31,10 @@ SIGNATURES = {
     "lh_timing_enable": (_i, [_i]),
     "lh_timing_read": (_i, [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong)]),
     "lh_timing_reset": (_i, []),
+    "lh_graph_begin": (_i, [_p]),
+    "lh_graph_end": (_i, [_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "lh_graph_launch": (_i, [_p, _p]),
+    "lh_graph_destroy": (_i, [_p]),
 }
 
 _lib = None
@@ -233,3 +237,44 @@ class KernelTimer:
     @staticmethod
     def reset():
         call("lh_timing_reset")
+
+
+class Graph:
+    """A captured launch sequence of this library (``lh_graph_*``), replayed
+    with :meth:`replay` on torch's current stream of its device.
+
+    ``Graph.capture(enqueue, device)`` runs ``enqueue(stream)`` on a side
+    stream under capture (the enqueued launches are recorded, not run) and
+    instantiates the result. Unlike ``torch.cuda.graph`` it neither
+    synchronises the device nor empties torch's allocator cache, so a serving
+    process can capture while other work is in flight; ``enqueue`` must
+    allocate nothing (its buffers exist before the capture)."""
+
+    def __init__(self, handle, device):
+        self._handle = handle
+        self.device = device
+
+    @classmethod
+    def capture(cls, enqueue, device):
+        import torch
+        side = torch.cuda.Stream(device=device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        handle = ctypes.c_void_p()
+        raw = ctypes.c_void_p(side.cuda_stream)
+        call("lh_graph_begin", raw)
+        try:
+            enqueue(side)
+        finally:
+            call("lh_graph_end", raw, ctypes.byref(handle))
+        return cls(handle, device)
+
+    def replay(self, stream=None):
+        call("lh_graph_launch", self._handle, stream_handle(stream))
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle is not None and handle.value and _lib is not None:
+            try:
+                call("lh_graph_destroy", handle)
+            except Exception:
+                pass

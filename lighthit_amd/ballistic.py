@@ -64,6 +64,31 @@ _lib.SIGNATURES["lh_order0_bins"] = (ctypes.c_int, [ctypes.POINTER(_Optics), cty
 
 _lib.SIGNATURES["lh_order0_count_nodes"] = (ctypes.c_int, [ctypes.c_void_p])
 _lib.SIGNATURES["lh_order0_set_rules"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_int])
+_lib.SIGNATURES["lh_order0_batch_workspace_bytes"] = (ctypes.c_size_t, [
+    ctypes.c_int, ctypes.c_int, ctypes.c_longlong])
+_lib.SIGNATURES["lh_order0_batch_chunks"] = (ctypes.c_longlong, [
+    ctypes.POINTER(ctypes.c_int), ctypes.c_int])
+_lib.SIGNATURES["lh_order0_batch_set"] = (ctypes.c_int, [
+    ctypes.POINTER(_Oms), ctypes.POINTER(_Elements), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+    ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p])
+_lib.SIGNATURES["lh_order0_charge_batch"] = (ctypes.c_int, [
+    ctypes.POINTER(_Optics), ctypes.POINTER(_Oms), ctypes.c_int, ctypes.c_longlong,
+    ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p])
+
+
+def _query(name, *args):
+    lib = _lib.load()
+    fn = getattr(lib, name)
+    fn.restype, fn.argtypes = _lib.SIGNATURES[name]
+    return fn(*args)
+
+
+def _bucket(n, floor):
+    """The smallest power of two at least ``n`` and ``floor``."""
+    size = int(floor)
+    while size < n:
+        size *= 2
+    return size
 
 FLAG_GLASS = 1
 FLAG_UNSUPPORTED = 2
@@ -261,6 +286,93 @@ class Order0:
                                    *_lib.upload([origins, edges], self.device))
         torch.cuda.synchronize()
         check_flags(ws)
+        return out.cpu().numpy()
+
+    #: Largest batch (events, element chunks) replayed from a graph; larger ones
+    #: launch the same chain directly.
+    GRAPH_EVENTS, GRAPH_CHUNKS = 16, 4096
+
+    def charge_batch_device(self, events, *, graph=False, stream=None):
+        """Enqueue the charges of every OM for several events in one fused launch
+        (``lh_order0_charge_batch``: one persistent kernel over the (event, OM,
+        chunk) items of all of them); returns ((events, OM) charge tensor,
+        workspace). Each event's row equals its :meth:`charge_device` result bit
+        for bit (the same items, sums in the same order).
+
+        With ``graph`` the launch chain is captured once per capacity (events
+        rounded up to a power of two, element chunks to 256, 1024 or 4096) as a
+        HIP graph and replayed; only the small table of this batch's events is
+        written before the replay."""
+        import torch
+        events = list(events)
+        n = len(events)
+        if n == 0:
+            return torch.empty((0, self.oms.count), dtype=torch.float64, device=self.device), None
+        counts = (ctypes.c_int * n)(*[int(e.count) for e in events])
+        chunks = int(_query("lh_order0_batch_chunks", counts, n))
+        graph = graph and n <= self.GRAPH_EVENTS and chunks <= self.GRAPH_CHUNKS
+        if graph:
+            # capacities in coarse steps, so that few graphs serve every batch;
+            # all of them share one workspace of the largest capacity (a
+            # process replays them one at a time, in stream order)
+            max_events = _bucket(n, 2)
+            max_chunks = 256
+            while max_chunks < chunks:
+                max_chunks *= 4
+            if getattr(self, "_graph_ws", None) is None:
+                self._graph_ws = torch.empty(int(_query(
+                    "lh_order0_batch_workspace_bytes", self.oms.count, self.GRAPH_EVENTS,
+                    self.GRAPH_CHUNKS)), dtype=torch.uint8, device=self.device)
+                self._graphs = {}
+            ws = self._graph_ws
+            captured = self._graphs.get((max_events, max_chunks))
+        else:
+            max_events, max_chunks = n, chunks
+            need = int(_query("lh_order0_batch_workspace_bytes", self.oms.count, max_events,
+                              max_chunks))
+            ws = _lib.workspace("order0_batch", self.device, need, stream)
+        out = torch.empty((n, self.oms.count), dtype=torch.float64, device=self.device)
+        table = (_Elements * n)(*[e.struct for e in events])
+        outs = (ctypes.c_void_p * n)(*[out[k].data_ptr() for k in range(n)])
+        handle = _lib.stream_handle(stream)
+        with _lib.stage("order0_charge"):
+            _lib.call("lh_order0_batch_set", ctypes.byref(self.oms.struct), table, outs, n,
+                      max_events, max_chunks, ctypes.c_void_p(ws.data_ptr()),
+                      ctypes.c_size_t(ws.numel()), handle)
+            args = (ctypes.byref(self.optics.struct), ctypes.byref(self.oms.struct), max_events,
+                    max_chunks, ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()))
+            if not graph:
+                _lib.call("lh_order0_charge_batch", *args, handle)
+            else:
+                if captured is None:
+                    captured = self._capture(args, ws, stream)
+                captured[0].replay(stream)
+        return out, ws
+
+    def _capture(self, args, ws, stream):
+        """Capture the fused charge chain of one capacity as a HIP graph."""
+        if not hasattr(self, "_graphs"):
+            self._graphs = {}
+        graph = _lib.Graph.capture(
+            lambda side: _lib.call("lh_order0_charge_batch", *args, _lib.stream_handle(side)),
+            self.device)
+        self._graphs[(args[2], args[3])] = (graph, ws)
+        return self._graphs[(args[2], args[3])]
+
+    def charge_batch(self, events, *, graph=False):
+        """Charges per OM (numpy, one row per event) of several events in one fused
+        launch, raising the reference's errors (the event at fault is found by
+        evaluating the events one by one)."""
+        import torch
+        device_events = [e if isinstance(e, DeviceElements) else DeviceElements(e, self.device)
+                         for e in events]
+        out, ws = self.charge_batch_device(device_events, graph=graph)
+        torch.cuda.synchronize()
+        if ws is not None:
+            try:
+                check_flags(ws)
+            except ValueError:
+                return np.stack([self.charge(e) for e in device_events])
         return out.cpu().numpy()
 
     def charge(self, elements):

@@ -95,6 +95,37 @@ int lh_timing_reset(void) {
 }
 
 
+int lh_graph_begin(void* stream) {
+    LH_REQUIRE(stream != nullptr, "lh_graph_begin: capture needs a stream other than the null stream");
+    LH_HIP_CALL(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                      hipStreamCaptureModeThreadLocal));
+    return LH_OK;
+}
+
+int lh_graph_end(void* stream, void** graph_exec) {
+    LH_REQUIRE(stream != nullptr && graph_exec != nullptr, "lh_graph_end: null argument");
+    hipGraph_t graph = nullptr;
+    LH_HIP_CALL(hipStreamEndCapture(static_cast<hipStream_t>(stream), &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t made = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    LH_HIP_CALL(made);
+    *graph_exec = exec;
+    return LH_OK;
+}
+
+int lh_graph_launch(void* graph_exec, void* stream) {
+    LH_REQUIRE(graph_exec != nullptr, "lh_graph_launch: null graph");
+    LH_HIP_CALL(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec),
+                               static_cast<hipStream_t>(stream)));
+    return LH_OK;
+}
+
+int lh_graph_destroy(void* graph_exec) {
+    if (graph_exec != nullptr) LH_HIP_CALL(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+    return LH_OK;
+}
+
 const char* lh_last_error(void) { return lh::g_error; }
 
 void lh_clear_error(void) { lh::g_error[0] = '\0'; }

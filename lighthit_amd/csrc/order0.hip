@@ -94,6 +94,30 @@ __constant__ float kEta4Cos[4] = {9.9405849277e-01f, 8.6862413779e-01f, 4.954716
 __constant__ float kEta4OneMinus[4] = {8.9115279994e-01f, 5.0452839915e-01f, 1.3137586221e-01f, 5.9415072342e-03f};
 __constant__ float kEta4Weight[4] = {2.7320455650e-01f, 5.1219360690e-01f, 5.1219360690e-01f, 2.7320455650e-01f};
 
+// One event's elements and its place in a launch: its chunks are the
+// launch's chunks chunk_base .. chunk_base + nchunk - 1 and its (OM, chunk)
+// items item_base + om * nchunk + chunk. An event without elements has no
+// chunks; `out` (its charges) is null for padding entries of a table.
+struct O0Event {
+    const double* start;      // (nel, 3)
+    const double* dir;        // (nel, 3)
+    const double* length;
+    const double* c0;
+    const double* c2;
+    const double* beta;
+    const double* start_ns;   // (nel), bins only
+    const double* end_ns;     // (nel), bins only
+    double* out;              // (nom * width) result, or null
+    long long chunk_base, item_base;
+    int nel, chunk, nchunk;
+};
+
+constexpr int kEventsPerSet = 16;   // table entries written by one set-up launch
+
+struct O0EventSet {
+    O0Event e[kEventsPerSet];
+};
+
 struct O0Params {
     // OMs
     const double* positions;  // (nom, 3)
@@ -101,14 +125,10 @@ struct O0Params {
     const double* areas;      // (nom)
     const double* radii;      // (nom)
     int nom;
-    // elements
-    const double* start;      // (nel, 3)
-    const double* dir;        // (nel, 3)
-    const double* length;
-    const double* c0;
-    const double* c2;
-    const double* beta;
-    int nel;
+    // events (a table in the workspace) and the launch's capacities
+    const O0Event* events;
+    int nevents;
+    long long chunk_capacity, item_capacity;
     // optics panels (nm, phase, group, extinction at edges; cubic efficiency)
     const double* edges;
     const double* phase;
@@ -121,17 +141,14 @@ struct O0Params {
     const double* alpha;
     int nalpha;
     double phase_min, phase_max;
-    int chunk, nchunk;
     double* partial;          // (live items, width): one row per live item
     double* bounds;           // (nchunk, kBoundWords) bounding sphere / direction cone
-    int* live;                // (nom * nchunk, 2) live (OM, chunk) items, in (OM, chunk) order
-    int* live_flag;           // (nom * nchunk + 1) 1 where the item may hold a pair
-    int* live_slot;           // (nom * nchunk + 1) exclusive scan of live_flag
-    int* live_count;          // = live_slot + nom * nchunk
+    int* live;                // (items) live items (their launch index), in launch order
+    int* live_flag;           // (item capacity + 1) 1 where the item may hold a pair
+    int* live_slot;           // (item capacity + 1) exclusive scan of live_flag
+    int* live_count;          // = live_slot + item capacity
     int* range;               // (live items, 2) first and last bin an item wrote
-    // bins only
-    const double* start_ns;   // (nel)
-    const double* end_ns;     // (nel)
+    // bins only (one event)
     const double* origins;    // (nom) time origins
     const double* bin_edges;  // (nbins + 1)
     int nbins;
@@ -149,6 +166,28 @@ static unsigned long long* g_node_count = nullptr;
 // forced), set by lh_order0_set_rules.
 static int g_s_rule = 0;
 static int g_azimuth_rule = 0;
+
+// Launch totals of a table (the last entry ends the launch's chunks and items).
+__device__ __forceinline__ long long total_chunks(const O0Params& P) {
+    const O0Event& last = P.events[P.nevents - 1];
+    return last.chunk_base + last.nchunk;
+}
+__device__ __forceinline__ long long total_items(const O0Params& P) {
+    const O0Event& last = P.events[P.nevents - 1];
+    return last.item_base + (long long)P.nom * last.nchunk;
+}
+
+// The event of a launch chunk / item (the last entry starting at or before it).
+__device__ __forceinline__ int event_of_chunk(const O0Params& P, long long c) {
+    int e = 0;
+    while (e + 1 < P.nevents && P.events[e + 1].chunk_base <= c) ++e;
+    return e;
+}
+__device__ __forceinline__ int event_of_item(const O0Params& P, long long idx) {
+    int e = 0;
+    while (e + 1 < P.nevents && P.events[e + 1].item_base <= idx) ++e;
+    return e;
+}
 
 // ---------------------------------------------------------- candidate test
 
@@ -171,28 +210,28 @@ __device__ bool cone_can_reach_sphere(double z, double rho, double length, doubl
 
 // Candidate test of interval_candidates for a sphere (infinite reach) with the
 // support test of _support (:152-186). Returns 1 for a pair to integrate.
-__device__ int candidate(const O0Params& P, int om, int i, int* glass) {
-    if (P.c0[i] == 0. || P.beta[i] * P.phase_max <= 1.) return 0;
+__device__ int candidate(const O0Params& P, const O0Event& E, int om, int i, int* glass) {
+    if (E.c0[i] == 0. || E.beta[i] * P.phase_max <= 1.) return 0;
     const double R = P.radii[om];
-    const double dx = P.positions[3 * om] - P.start[3 * i];
-    const double dy = P.positions[3 * om + 1] - P.start[3 * i + 1];
-    const double dz = P.positions[3 * om + 2] - P.start[3 * i + 2];
-    const double ux = P.dir[3 * i], uy = P.dir[3 * i + 1], uz = P.dir[3 * i + 2];
+    const double dx = P.positions[3 * om] - E.start[3 * i];
+    const double dy = P.positions[3 * om + 1] - E.start[3 * i + 1];
+    const double dz = P.positions[3 * om + 2] - E.start[3 * i + 2];
+    const double ux = E.dir[3 * i], uy = E.dir[3 * i + 1], uz = E.dir[3 * i + 2];
     const double z = dx * ux + dy * uy + dz * uz;
     if (z <= 0. && z <= -R) return 0;
     const double cx = dy * uz - dz * uy, cy = dz * ux - dx * uz, cz = dx * uy - dy * ux;
     const double rho = sqrt(cx * cx + cy * cy + cz * cz);
-    const double L = P.length[i];
+    const double L = E.length[i];
     const double along = fmax(fmax(-z, z - L), 0.);
     if (rho != 0.) {
-        if (!cone_can_reach_sphere(z, rho, L, P.beta[i], R, P.phase_min, P.phase_max)) return 0;
+        if (!cone_can_reach_sphere(z, rho, L, E.beta[i], R, P.phase_min, P.phase_max)) return 0;
         const double quarter = 0.5 * kPi;
         const double psi0 = atan2(rho, z), psi1 = atan2(rho, z - L);
         const double a0 = asin(fmin(1., R / hypot(z, rho)));
         const double a1 = asin(fmin(1., R / hypot(z - L, rho)));
         const double low = psi0 - a0, high = psi1 + a1;
         if (low >= quarter) return 0;
-        const double b = P.beta[i];
+        const double b = E.beta[i];
         const double nmin = low <= 0. ? (1. + 1e-9) / b : fmax((1. + 1e-9) / b, 1. / (b * cos(low)));
         const double nmax = high >= quarter ? INFINITY : 1. / (b * cos(high));
         if (nmin > P.phase_max || nmax <= P.phase_min) return 0;
@@ -658,18 +697,21 @@ __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int n
 __global__ __launch_bounds__(kThreads) void order0_chunk_bounds(O0Params P) {
     __shared__ double red[kThreads][4];
     const int tid = threadIdx.x;
-    const int first = blockIdx.x * P.chunk;
-    const int last = min(P.nel, first + P.chunk);
+    const long long c = blockIdx.x;
+    if (c >= total_chunks(P)) return;
+    const O0Event E = P.events[event_of_chunk(P, c)];
+    const int first = (int)(c - E.chunk_base) * E.chunk;
+    const int last = min(E.nel, first + E.chunk);
     double acc[4] = {0., 0., 0., 0.};
     double dir[3] = {0., 0., 0.};
     for (int i = first + tid; i < last; i += kThreads) {
-        const double h = 0.5 * P.length[i];
-        acc[0] += P.start[3 * i] + h * P.dir[3 * i];
-        acc[1] += P.start[3 * i + 1] + h * P.dir[3 * i + 1];
-        acc[2] += P.start[3 * i + 2] + h * P.dir[3 * i + 2];
-        dir[0] += P.dir[3 * i];
-        dir[1] += P.dir[3 * i + 1];
-        dir[2] += P.dir[3 * i + 2];
+        const double h = 0.5 * E.length[i];
+        acc[0] += E.start[3 * i] + h * E.dir[3 * i];
+        acc[1] += E.start[3 * i + 1] + h * E.dir[3 * i + 1];
+        acc[2] += E.start[3 * i + 2] + h * E.dir[3 * i + 2];
+        dir[0] += E.dir[3 * i];
+        dir[1] += E.dir[3 * i + 1];
+        dir[2] += E.dir[3 * i + 2];
     }
     // centre
     for (int k = 0; k < 3; ++k) red[tid][k] = acc[k];
@@ -702,16 +744,16 @@ __global__ __launch_bounds__(kThreads) void order0_chunk_bounds(O0Params P) {
         cosmin = -1.;
     }
     for (int i = first + tid; i < last; i += kThreads) {
-        const double L = P.length[i];
+        const double L = E.length[i];
         for (int e = 0; e < 2; ++e) {
-            const double x = P.start[3 * i] + e * L * P.dir[3 * i] - cx;
-            const double y = P.start[3 * i + 1] + e * L * P.dir[3 * i + 1] - cy;
-            const double z = P.start[3 * i + 2] + e * L * P.dir[3 * i + 2] - cz;
+            const double x = E.start[3 * i] + e * L * E.dir[3 * i] - cx;
+            const double y = E.start[3 * i + 1] + e * L * E.dir[3 * i + 1] - cy;
+            const double z = E.start[3 * i + 2] + e * L * E.dir[3 * i + 2] - cz;
             radius = fmax(radius, sqrt(x * x + y * y + z * z));
         }
-        cosmin = fmin(cosmin, P.dir[3 * i] * ux + P.dir[3 * i + 1] * uy + P.dir[3 * i + 2] * uz);
-        bmin = fmin(bmin, P.beta[i]);
-        bmax = fmax(bmax, P.beta[i]);
+        cosmin = fmin(cosmin, E.dir[3 * i] * ux + E.dir[3 * i + 1] * uy + E.dir[3 * i + 2] * uz);
+        bmin = fmin(bmin, E.beta[i]);
+        bmax = fmax(bmax, E.beta[i]);
     }
     red[tid][0] = radius;
     red[tid][1] = cosmin;
@@ -728,7 +770,7 @@ __global__ __launch_bounds__(kThreads) void order0_chunk_bounds(O0Params P) {
         __syncthreads();
     }
     if (tid == 0) {
-        double* b = P.bounds + (size_t)blockIdx.x * kBoundWords;
+        double* b = P.bounds + (size_t)c * kBoundWords;
         b[0] = cx; b[1] = cy; b[2] = cz; b[3] = red[0][0] * (1.0 + 1e-12) + 1e-9;
         b[4] = ux; b[5] = uy; b[6] = uz; b[7] = red[0][1];
         b[8] = red[0][2]; b[9] = red[0][3];
@@ -758,9 +800,17 @@ __device__ bool chunk_culled(const O0Params& P, int chunk, int om) {
     return alpha - slack > theta_max || alpha + slack < theta_min;
 }
 
-// Clears the flag words (status flags, live-item count) of a call.
+// Clears the flag words (status flags) of a call.
 __global__ void order0_reset(int* flags) {
     if (threadIdx.x < 64) flags[threadIdx.x] = 0;
+}
+
+// Clears the flag words and writes `count` entries of the event table from
+// `first` on.
+__global__ void order0_set_events(int* flags, O0Event* table, int first, int count,
+                                  O0EventSet set) {
+    if (first == 0 && threadIdx.x < 64) flags[threadIdx.x] = 0;
+    if (threadIdx.x < count) table[first + threadIdx.x] = set.e[threadIdx.x];
 }
 
 // Copies the flag word of a call to the caller's output.
@@ -768,34 +818,28 @@ __global__ void order0_copy_flags(const int* flags, int* out) {
     if (threadIdx.x == 0) out[0] = flags[0];
 }
 
-// Zeroes `count` doubles.
-__global__ void order0_zero(double* out, size_t count) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < count) out[idx] = 0.0;
-}
-
-// One thread per (OM, chunk): flags the items that may hold a contributing
-// pair (the flag past the last item stays zero for the scan).
+// One thread per (event, OM, chunk) item of the launch's capacity: flags the
+// items that may hold a contributing pair (zero past the last item, for the
+// scan).
 __global__ void order0_cull(O0Params P) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t items = (size_t)P.nom * P.nchunk;
-    if (idx > items) return;
-    if (idx == items) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx > P.item_capacity) return;
+    if (idx >= total_items(P)) {
         P.live_flag[idx] = 0;
         return;
     }
-    const int om = (int)(idx / P.nchunk), chunk = (int)(idx - (size_t)om * P.nchunk);
-    P.live_flag[idx] = chunk_culled(P, chunk, om) ? 0 : 1;
+    const O0Event& E = P.events[event_of_item(P, idx)];
+    const long long local = idx - E.item_base;
+    const int om = (int)(local / E.nchunk), chunk = (int)(local - (long long)om * E.nchunk);
+    P.live_flag[idx] = chunk_culled(P, (int)(E.chunk_base + chunk), om) ? 0 : 1;
 }
 
-// Live items in (OM, chunk) order at their scanned slots: the list, and so
-// every partial row, has the same order on every run.
+// Live items in launch order (event, OM, chunk) at their scanned slots: the
+// list, and so every partial row, has the same order on every run.
 __global__ void order0_compact(O0Params P) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)P.nom * P.nchunk || !P.live_flag[idx]) return;
-    const int slot = P.live_slot[idx];
-    P.live[2 * slot] = (int)(idx / P.nchunk);
-    P.live[2 * slot + 1] = (int)(idx % P.nchunk);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P.item_capacity || !P.live_flag[idx]) return;
+    P.live[P.live_slot[idx]] = (int)idx;
 }
 
 template <bool kBins>
@@ -844,8 +888,10 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     // live (OM, chunk) items, strided over the persistent grid
     const int live = *P.live_count;
     for (int item = blockIdx.x; item < live; item += gridDim.x) {
-    const int om = P.live[2 * item];
-    const int chunk = P.live[2 * item + 1];
+    const long long idx = P.live[item];
+    const O0Event E = P.events[event_of_item(P, idx)];
+    const int om = (int)((idx - E.item_base) / E.nchunk);
+    const int chunk = (int)(idx - E.item_base - (long long)om * E.nchunk);
     if (tid == 0) {
         total_count = 0;
         glass = 0;
@@ -854,12 +900,12 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         for (int b = tid; b < kWaves * P.nbins; b += kThreads) dynamic_lds[b] = 0.0;
     __syncthreads();
     // candidate pairs of this chunk, compacted in element order
-    const int first = chunk * P.chunk;
-    const int last = min(P.nel, first + P.chunk);
+    const int first = chunk * E.chunk;
+    const int last = min(E.nel, first + E.chunk);
     int local_glass = 0;
     for (int base = first; base < last; base += kThreads) {
         const int i = base + tid;
-        const int pass = (i < last) ? candidate(P, om, i, &local_glass) : 0;
+        const int pass = (i < last) ? candidate(P, E, om, i, &local_glass) : 0;
         const unsigned long long ballot = __ballot(pass);
         const int before = __popcll(ballot & ((1ull << lane) - 1ull));
         if (lane == 0) wave_count[wave] = __popcll(ballot);
@@ -892,9 +938,9 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     double lane_sum = 0.0;
     for (int entry = wave; entry < count; entry += kWaves) {
         const int i = list[entry];
-        const double ux = P.dir[3 * i], uy = P.dir[3 * i + 1], uz = P.dir[3 * i + 2];
-        const double wx = pos[0] - P.start[3 * i], wy = pos[1] - P.start[3 * i + 1],
-                     wz = pos[2] - P.start[3 * i + 2];
+        const double ux = E.dir[3 * i], uy = E.dir[3 * i + 1], uz = E.dir[3 * i + 2];
+        const double wx = pos[0] - E.start[3 * i], wy = pos[1] - E.start[3 * i + 1],
+                     wz = pos[2] - E.start[3 * i + 2];
         const double z = wx * ux + wy * uy + wz * uz;
         const double tx = wx - z * ux, ty = wy - z * uy, tz = wz - z * uz;
         const double rho = sqrt(tx * tx + ty * ty + tz * tz);
@@ -907,9 +953,9 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const float axial = (float)(ux * look[0] + uy * look[1] + uz * look[2]);
         const float across = (float)(e1x * look[0] + e1y * look[1] + e1z * look[2]);
         const float side = (float)(e2x * look[0] + e2y * look[1] + e2z * look[2]);
-        const double L = P.length[i], beta = P.beta[i];
-        const double t_start = kBins ? P.start_ns[i] - P.origins[om] : 0.0;
-        const double tau = kBins ? (P.end_ns[i] - P.start_ns[i]) / L : 0.0;
+        const double L = E.length[i], beta = E.beta[i];
+        const double t_start = kBins ? E.start_ns[i] - P.origins[om] : 0.0;
+        const double tau = kBins ? (E.end_ns[i] - E.start_ns[i]) / L : 0.0;
         // cone angles of the medium panel edges (one lane each) and the s
         // pieces: [0, L] cut where a panel edge is tangent to the cap
         if (lane < P.nmajor) {
@@ -954,7 +1000,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const int ns = s_nodes<kBins>(L, R, P.s_rule);
         const int ne = azimuth_nodes(rho, R, P.azimuth_rule);
         const int slots = (nscut - 1) * ns * pieces;
-        const float c0 = (float)P.c0[i], c2 = (float)P.c2[i];
+        const float c0 = (float)E.c0[i], c2 = (float)E.c2[i];
         const float inv_length = (float)(1.0 / L);
         float pair_sum = 0.f;
         int evaluated = 0;
@@ -1042,14 +1088,19 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     }
 }
 
-// out[om][b] = sum over the OM's live items, in chunk order, of their
-// partial rows (width = bins or 1; a bin item adds only inside its range)
-__global__ void order0_reduce(O0Params P, int width, double* out) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)P.nom * width) return;
-    const size_t om = idx / width;
-    const int b = (int)(idx - om * width);
-    const int first = P.live_slot[om * P.nchunk], last = P.live_slot[(om + 1) * P.nchunk];
+// out[om][b] of every event = sum over the OM's live items, in chunk order,
+// of their partial rows (width = bins or 1; a bin item adds only inside its
+// range). An event without elements gets zeros; padding entries are skipped.
+__global__ void order0_reduce(O0Params P, int width) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per_event = (long long)P.nom * width;
+    if (idx >= per_event * P.nevents) return;
+    const O0Event& E = P.events[idx / per_event];
+    if (E.out == nullptr) return;
+    const long long rest = idx % per_event;
+    const int om = (int)(rest / width), b = (int)(rest - (long long)om * width);
+    const long long at = E.item_base + (long long)om * E.nchunk;
+    const int first = P.live_slot[at], last = P.live_slot[at + E.nchunk];
     double sum = 0.0;
     if (width == 1) {
         for (int item = first; item < last; ++item) sum += P.partial[item];
@@ -1058,7 +1109,7 @@ __global__ void order0_reduce(O0Params P, int width, double* out) {
             if (b >= P.range[2 * item] && b <= P.range[2 * item + 1])
                 sum += P.partial[(size_t)item * width + b];
     }
-    out[idx] = sum;
+    E.out[rest] = sum;
 }
 
 }  // namespace
@@ -1066,55 +1117,57 @@ __global__ void order0_reduce(O0Params P, int width, double* out) {
 namespace {
 
 // Elements per (OM, chunk) item, from the element count alone (so an OM's
-// result never depends on which other OMs share the call): small sources are
-// cut finer, so that few OMs near a short shower still give the chip several
-// hundred workgroups and the pairs of the nearest OM spread over many waves.
-// The bins launch (only the active OMs, each pair costlier) cuts small
-// sources finer still.
+// result never depends on which other OMs or events share the call): small
+// sources are cut finer, so that few OMs near a short shower still give the
+// chip several hundred workgroups and the pairs of the nearest OM spread over
+// many waves. The bins launch (only the active OMs, each pair costlier) cuts
+// small sources finer still.
 int chunk_of(int n_elements, bool bins) {
     const int chunk = n_elements >= 32768 ? kChunk : n_elements >= 8192 ? 256
                     : n_elements >= 2048 ? 128 : n_elements >= 512 ? 64 : 32;
     return (bins && n_elements < 65536) ? std::max(16, chunk / 2) : chunk;
 }
 
-size_t chunks_of(int n_elements, bool bins) {
+long long chunks_of(int n_elements, bool bins) {
     const int chunk = chunk_of(n_elements, bins);
-    return n_elements > 0 ? (size_t)(n_elements + chunk - 1) / chunk : 1;
+    return n_elements > 0 ? (n_elements + chunk - 1) / chunk : 0;
 }
 
 size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 
-// Workspace of one call: flag words | chunk bounds | partial rows | live list
-// | live flags | live slots | bin ranges | scan scratch.
+// Workspace of one launch: flag words | event table | chunk bounds | partial
+// rows | live list | live flags | live slots | bin ranges | scan scratch.
 struct O0Layout {
-    size_t nchunk, items, bounds, partial, live, flag, slot, range, scan, scan_bytes, total;
+    long long chunks, items;
+    size_t table, bounds, partial, live, flag, slot, range, scan, scan_bytes, total;
 };
 
-O0Layout layout_of(int n_oms, int n_elements, int width, bool bins) {
+O0Layout layout_of(int n_oms, int max_events, long long max_chunks, int width, bool bins) {
     O0Layout L{};
-    L.nchunk = chunks_of(n_elements, bins);
-    L.items = (size_t)(n_oms > 0 ? n_oms : 0) * L.nchunk;
+    L.chunks = std::max<long long>(max_chunks, 0);
+    L.items = (long long)(n_oms > 0 ? n_oms : 0) * L.chunks;
     const size_t w = (size_t)(width > 0 ? width : 0);
-    L.bounds = 256;
-    L.partial = align256(L.bounds + sizeof(double) * L.nchunk * kBoundWords);
-    L.live = align256(L.partial + sizeof(double) * L.items * w);
-    L.flag = align256(L.live + 2 * sizeof(int) * L.items);
-    L.slot = align256(L.flag + sizeof(int) * (L.items + 1));
-    L.range = align256(L.slot + sizeof(int) * (L.items + 1));
-    L.scan = align256(L.range + (bins ? 2 * sizeof(int) * L.items : 0));
+    const size_t items = (size_t)L.items;
+    L.table = 256;
+    L.bounds = align256(L.table + sizeof(O0Event) * (size_t)std::max(max_events, 1));
+    L.partial = align256(L.bounds + sizeof(double) * (size_t)L.chunks * kBoundWords);
+    L.live = align256(L.partial + sizeof(double) * items * w);
+    L.flag = align256(L.live + sizeof(int) * items);
+    L.slot = align256(L.flag + sizeof(int) * (items + 1));
+    L.range = align256(L.slot + sizeof(int) * (items + 1));
+    L.scan = align256(L.range + (bins ? 2 * sizeof(int) * items : 0));
     size_t scan_bytes = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int*)nullptr, (int*)nullptr,
-                                         (int)(L.items + 1)) != hipSuccess)
+                                         (int)(items + 1)) != hipSuccess)
         scan_bytes = 0;
     L.scan_bytes = std::max<size_t>(scan_bytes, 1);
     L.total = align256(L.scan + L.scan_bytes);
     return L;
 }
 
-int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms,
-                      const lh_order0_elements* elements, const char* name) {
-    LH_REQUIRE(optics && oms && elements, "%s: null descriptor", name);
-    LH_REQUIRE(oms->count >= 0 && elements->count >= 0, "%s: negative count", name);
+int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms, const char* name) {
+    LH_REQUIRE(optics && oms, "%s: null descriptor", name);
+    LH_REQUIRE(oms->count >= 0, "%s: negative count", name);
     LH_REQUIRE(oms->count <= 65535, "%s: at most 65535 OMs per launch", name);
     LH_REQUIRE(optics->panels >= 1 && optics->panels <= kMaxPanels,
                "%s: 1..%d optical panels supported", name, kMaxPanels);
@@ -1127,23 +1180,76 @@ int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms,
     return LH_OK;
 }
 
-O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
-                     const lh_order0_elements* elements, void* workspace, int width, bool bins) {
+int check_descriptors(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                      const lh_order0_elements* elements, const char* name) {
+    LH_REQUIRE(elements, "%s: null descriptor", name);
+    LH_REQUIRE(elements->count >= 0, "%s: negative count", name);
+    return check_descriptors(optics, oms, name);
+}
+
+// Table entries of `count` events (chunks and items laid out in event order),
+// padded to `max_events` with entries that own nothing.
+int event_table(const lh_order0_elements* elements, double* const* outs, int count,
+                int max_events, int n_oms, bool bins, long long max_chunks, O0Event* table,
+                const char* name) {
+    long long chunks = 0;
+    for (int k = 0; k < max_events; ++k) {
+        O0Event E{};
+        if (k < count) {
+            const lh_order0_elements& el = elements[k];
+            LH_REQUIRE(el.count >= 0, "%s: negative element count in event %d", name, k);
+            LH_REQUIRE(el.count == 0 || (el.start_m && el.direction && el.length_m
+                                         && el.coefficient0 && el.coefficient2 && el.beta),
+                       "%s: null element array in event %d", name, k);
+            E.start = el.start_m;
+            E.dir = el.direction;
+            E.length = el.length_m;
+            E.c0 = el.coefficient0;
+            E.c2 = el.coefficient2;
+            E.beta = el.beta;
+            E.start_ns = el.start_ns;
+            E.end_ns = el.end_ns;
+            E.out = outs[k];
+            LH_REQUIRE(E.out != nullptr || n_oms == 0, "%s: null output of event %d", name, k);
+            E.nel = el.count;
+            E.chunk = chunk_of(el.count, bins);
+            E.nchunk = (int)chunks_of(el.count, bins);
+        }
+        E.chunk_base = chunks;
+        E.item_base = chunks * n_oms;
+        chunks += E.nchunk;
+        table[k] = E;
+    }
+    LH_REQUIRE(chunks <= max_chunks, "%s: %lld element chunks, above the launch's %lld", name,
+               chunks, max_chunks);
+    return LH_OK;
+}
+
+// Writes the table into the workspace (and clears the flag words) with
+// kernels, never a host copy or a memset.
+int upload_table(const O0Event* table, int count, void* workspace, hipStream_t s,
+                 const char* name) {
+    O0Event* dst = reinterpret_cast<O0Event*>(static_cast<char*>(workspace) + 256);
+    for (int first = 0; first < std::max(count, 1); first += kEventsPerSet) {
+        O0EventSet set{};
+        const int n = std::min(kEventsPerSet, count - first);
+        for (int k = 0; k < n; ++k) set.e[k] = table[first + k];
+        hipLaunchKernelGGL(order0_set_events, dim3(1), dim3(64), 0, s,
+                           static_cast<int*>(workspace), dst, first, std::max(n, 0), set);
+        const int status = lh::check_launch(name);
+        if (status != LH_OK) return status;
+    }
+    return LH_OK;
+}
+
+O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms, void* workspace,
+                     int max_events, long long max_chunks, int width, bool bins) {
     O0Params P{};
     P.positions = oms->positions;
     P.looks = oms->looks;
     P.areas = oms->areas;
     P.radii = oms->radii;
     P.nom = oms->count;
-    P.start = elements->start_m;
-    P.dir = elements->direction;
-    P.length = elements->length_m;
-    P.c0 = elements->coefficient0;
-    P.c2 = elements->coefficient2;
-    P.beta = elements->beta;
-    P.start_ns = elements->start_ns;
-    P.end_ns = elements->end_ns;
-    P.nel = elements->count;
     P.edges = optics->panel_edges;
     P.phase = optics->phase_nodes;
     P.group = optics->group_nodes;
@@ -1157,10 +1263,12 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     P.nalpha = optics->acceptance_count;
     P.phase_min = optics->phase_min;
     P.phase_max = optics->phase_max;
-    const O0Layout L = layout_of(oms->count, elements->count, width, bins);
+    const O0Layout L = layout_of(oms->count, max_events, max_chunks, width, bins);
     char* base = static_cast<char*>(workspace);
-    P.chunk = chunk_of(elements->count, bins);
-    P.nchunk = (int)L.nchunk;
+    P.events = reinterpret_cast<const O0Event*>(base + L.table);
+    P.nevents = std::max(max_events, 1);
+    P.chunk_capacity = L.chunks;
+    P.item_capacity = L.items;
     P.flags = static_cast<int*>(workspace);
     P.node_count = g_node_count;
     P.s_rule = g_s_rule;
@@ -1175,35 +1283,63 @@ O0Params make_params(const lh_order0_optics* optics, const lh_order0_oms* oms,
     return P;
 }
 
+// The launch chain after the table is in place. Every grid is sized by the
+// launch's capacities (never by the events of one call), and every size the
+// kernels need is read from the table on the device: the chain can be
+// captured once per capacity and replayed for any events that fit.
 template <bool kBins>
-int run(O0Params P, int width, double* out, void* workspace, hipStream_t s, size_t lds,
-        const char* name) {
-    const O0Layout L = layout_of(P.nom, P.nel, width, kBins);
-    hipLaunchKernelGGL(order0_chunk_bounds, dim3(P.nchunk), dim3(kThreads), 0, s, P);
+int run(O0Params P, int width, void* workspace, hipStream_t s, size_t lds, const char* name) {
+    const O0Layout L = layout_of(P.nom, P.nevents, P.chunk_capacity, width, kBins);
+    if (L.chunks > 0) {
+        hipLaunchKernelGGL(order0_chunk_bounds, dim3((unsigned)L.chunks), dim3(kThreads), 0, s, P);
+        int status = lh::check_launch(name);
+        if (status != LH_OK) return status;
+    }
+    hipLaunchKernelGGL(order0_cull, dim3((unsigned)((L.items + 1 + 255) / 256)), dim3(256), 0, s,
+                       P);
     int status = lh::check_launch(name);
-    if (status != LH_OK) return status;
-    const size_t items = L.items;
-    hipLaunchKernelGGL(order0_cull, dim3((unsigned)((items + 1 + 255) / 256)), dim3(256), 0, s, P);
-    status = lh::check_launch(name);
     if (status != LH_OK) return status;
     size_t scan_bytes = L.scan_bytes;
     LH_HIP_CALL(hipcub::DeviceScan::ExclusiveSum(static_cast<char*>(workspace) + L.scan,
                                                  scan_bytes, P.live_flag, P.live_slot,
-                                                 (int)(items + 1), s));
-    hipLaunchKernelGGL(order0_compact, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, s, P);
-    status = lh::check_launch(name);
-    if (status != LH_OK) return status;
-    const unsigned grid = (unsigned)std::min<size_t>(items, kPersistentGroups);
-    {
+                                                 (int)(L.items + 1), s));
+    if (L.items > 0) {
+        hipLaunchKernelGGL(order0_compact, dim3((unsigned)((L.items + 255) / 256)), dim3(256), 0,
+                           s, P);
+        status = lh::check_launch(name);
+        if (status != LH_OK) return status;
+        const unsigned grid = (unsigned)std::min<long long>(L.items, kPersistentGroups);
         lh::KernelMark mark(kBins ? "order0_pairs<true>" : "order0_pairs<false>", s);
         hipLaunchKernelGGL((order0_pairs<kBins>), dim3(grid), dim3(kThreads), lds, s, P);
     }
     status = lh::check_launch(name);
     if (status != LH_OK) return status;
-    const size_t total = (size_t)P.nom * width;
-    hipLaunchKernelGGL(order0_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       P, width, out);
-    return lh::check_launch(name);
+    const long long total = (long long)P.nom * width * P.nevents;
+    if (total > 0) {
+        hipLaunchKernelGGL(order0_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                           P, width);
+        status = lh::check_launch(name);
+    }
+    return status;
+}
+
+// One event: its table entry, then the chain at its own capacity.
+template <bool kBins>
+int run_one(const lh_order0_optics* optics, const lh_order0_oms* oms,
+            const lh_order0_elements* elements, double* out, int width, void* workspace,
+            hipStream_t s, size_t lds, const double* origins, const double* bin_edges,
+            const char* name) {
+    O0Event table[1];
+    const long long chunks = chunks_of(elements->count, kBins);
+    int status = event_table(elements, &out, 1, 1, oms->count, kBins, chunks, table, name);
+    if (status != LH_OK) return status;
+    status = upload_table(table, 1, workspace, s, name);
+    if (status != LH_OK) return status;
+    O0Params P = make_params(optics, oms, workspace, 1, chunks, width, kBins);
+    P.origins = origins;
+    P.bin_edges = bin_edges;
+    P.nbins = kBins ? width : 0;
+    return run<kBins>(P, width, workspace, s, lds, name);
 }
 
 }  // namespace
@@ -1224,11 +1360,11 @@ extern "C" int lh_order0_set_rules(int s_nodes, int azimuth_nodes) {
 }
 
 extern "C" size_t lh_order0_bins_workspace_bytes(int n_oms, int n_elements, int n_bins) {
-    return layout_of(n_oms, n_elements, n_bins, true).total;
+    return layout_of(n_oms, 1, chunks_of(n_elements, true), n_bins, true).total;
 }
 
 extern "C" size_t lh_order0_workspace_bytes(int n_oms, int n_elements) {
-    return layout_of(n_oms, n_elements, 1, false).total;
+    return layout_of(n_oms, 1, chunks_of(n_elements, false), 1, false).total;
 }
 
 extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_oms* oms,
@@ -1240,16 +1376,63 @@ extern "C" int lh_order0_charge(const lh_order0_optics* optics, const lh_order0_
                "lh_order0_charge: workspace of %zu bytes, %zu needed", workspace_bytes,
                lh_order0_workspace_bytes(oms->count, elements->count));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(workspace));
-    status = lh::check_launch("lh_order0_charge (reset)");
-    if (status != LH_OK || oms->count == 0) return status;
-    if (elements->count == 0) {
-        hipLaunchKernelGGL(order0_zero, dim3((oms->count + 255) / 256), dim3(256), 0, s, charge,
-                           (size_t)oms->count);
-        return lh::check_launch("lh_order0_charge (zero)");
+    if (oms->count == 0) {
+        hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(workspace));
+        return lh::check_launch("lh_order0_charge (reset)");
     }
-    O0Params P = make_params(optics, oms, elements, workspace, 1, false);
-    return run<false>(P, 1, charge, workspace, s, 0, "lh_order0_charge");
+    return run_one<false>(optics, oms, elements, charge, 1, workspace, s, 0, nullptr, nullptr,
+                          "lh_order0_charge");
+}
+
+extern "C" size_t lh_order0_batch_workspace_bytes(int n_oms, int max_events, long long max_chunks) {
+    return layout_of(n_oms, max_events, max_chunks, 1, false).total;
+}
+
+extern "C" long long lh_order0_batch_chunks(const int* element_counts, int n_events) {
+    long long chunks = 0;
+    for (int k = 0; k < n_events; ++k) chunks += chunks_of(element_counts[k], false);
+    return chunks;
+}
+
+extern "C" int lh_order0_batch_set(const lh_order0_oms* oms, const lh_order0_elements* events,
+                                   double* const* charges, int n_events, int max_events,
+                                   long long max_chunks, void* workspace, size_t workspace_bytes,
+                                   void* stream) {
+    const char* name = "lh_order0_batch_set";
+    LH_REQUIRE(oms && events && charges, "%s: null argument", name);
+    LH_REQUIRE(n_events >= 0 && n_events <= max_events && max_events >= 1,
+               "%s: %d events for a launch of %d", name, n_events, max_events);
+    LH_REQUIRE(max_events <= 4096, "%s: at most 4096 events per launch", name);
+    LH_REQUIRE(oms->count >= 0 && oms->count <= 65535, "%s: 0..65535 OMs", name);
+    const size_t need = lh_order0_batch_workspace_bytes(oms->count, max_events, max_chunks);
+    LH_REQUIRE(workspace_bytes >= need, "%s: workspace of %zu bytes, %zu needed", name,
+               workspace_bytes, need);
+    O0Event* table = new O0Event[max_events];
+    int status = event_table(events, charges, n_events, max_events, oms->count, false, max_chunks,
+                             table, name);
+    if (status == LH_OK)
+        status = upload_table(table, max_events, workspace, static_cast<hipStream_t>(stream), name);
+    delete[] table;
+    return status;
+}
+
+extern "C" int lh_order0_charge_batch(const lh_order0_optics* optics, const lh_order0_oms* oms,
+                                      int max_events, long long max_chunks, void* workspace,
+                                      size_t workspace_bytes, void* stream) {
+    const char* name = "lh_order0_charge_batch";
+    int status = check_descriptors(optics, oms, name);
+    if (status != LH_OK) return status;
+    LH_REQUIRE(max_events >= 1 && max_events <= 4096 && max_chunks >= 0,
+               "%s: 1..4096 events and a nonnegative chunk capacity", name);
+    const size_t need = lh_order0_batch_workspace_bytes(oms->count, max_events, max_chunks);
+    LH_REQUIRE(workspace_bytes >= need, "%s: workspace of %zu bytes, %zu needed", name,
+               workspace_bytes, need);
+    LH_REQUIRE((long long)oms->count * max_chunks < (1LL << 31),
+               "%s: %lld (OM, chunk) items, above 2^31", name,
+               (long long)oms->count * max_chunks);
+    if (oms->count == 0) return LH_OK;
+    O0Params P = make_params(optics, oms, workspace, max_events, max_chunks, 1, false);
+    return run<false>(P, 1, workspace, static_cast<hipStream_t>(stream), 0, name);
 }
 
 extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_oms* oms,
@@ -1264,22 +1447,14 @@ extern "C" int lh_order0_bins(const lh_order0_optics* optics, const lh_order0_om
                "lh_order0_bins: workspace of %zu bytes, %zu needed", workspace_bytes,
                lh_order0_bins_workspace_bytes(oms->count, elements->count, n_bins));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(workspace));
-    status = lh::check_launch("lh_order0_bins (reset)");
-    if (status != LH_OK || oms->count == 0) return status;
-    LH_REQUIRE(origins && bin_edges && bins, "lh_order0_bins: null device pointer");
-    if (elements->count == 0) {
-        const size_t count = (size_t)oms->count * n_bins;
-        hipLaunchKernelGGL(order0_zero, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
-                           bins, count);
-        return lh::check_launch("lh_order0_bins (zero)");
+    if (oms->count == 0) {
+        hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(workspace));
+        return lh::check_launch("lh_order0_bins (reset)");
     }
-    O0Params P = make_params(optics, oms, elements, workspace, n_bins, true);
-    P.origins = origins;
-    P.bin_edges = bin_edges;
-    P.nbins = n_bins;
+    LH_REQUIRE(origins && bin_edges && bins, "lh_order0_bins: null device pointer");
     const size_t lds = sizeof(double) * kWaves * (size_t)n_bins + sizeof(float) * (n_bins + 1);
-    return run<true>(P, n_bins, bins, workspace, s, lds, "lh_order0_bins");
+    return run_one<true>(optics, oms, elements, bins, n_bins, workspace, s, lds, origins, bin_edges,
+                         "lh_order0_bins");
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,20 +1514,14 @@ extern "C" int lh_order0_charge_noalloc(const lh_order0_optics* optics, const lh
     if (status != LH_OK) return status;
     hipStream_t s = static_cast<hipStream_t>(stream);
     // the flag words start at zero on every call (a kernel, not a memset)
-    hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(ws));
-    status = lh::check_launch("lh_order0_charge_noalloc (reset)");
-    if (status != LH_OK) return status;
-    if (oms->count > 0) {
-        if (elements->count == 0) {
-            hipLaunchKernelGGL(order0_zero, dim3((oms->count + 255) / 256), dim3(256), 0, s,
-                               charge, (size_t)oms->count);
-            status = lh::check_launch("lh_order0_charge_noalloc (zero)");
-        } else {
-            O0Params P = make_params(optics, oms, elements, ws, 1, false);
-            status = run<false>(P, 1, charge, ws, s, 0, "lh_order0_charge_noalloc");
-        }
-        if (status != LH_OK) return status;
+    if (oms->count == 0) {
+        hipLaunchKernelGGL(order0_reset, dim3(1), dim3(64), 0, s, static_cast<int*>(ws));
+        status = lh::check_launch("lh_order0_charge_noalloc (reset)");
+    } else {
+        status = run_one<false>(optics, oms, elements, charge, 1, ws, s, 0, nullptr, nullptr,
+                                "lh_order0_charge_noalloc");
     }
+    if (status != LH_OK) return status;
     if (flags != nullptr) {
         hipLaunchKernelGGL(order0_copy_flags, dim3(1), dim3(64), 0, s,
                            static_cast<const int*>(ws), flags);

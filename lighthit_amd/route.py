@@ -844,6 +844,62 @@ class ShowerTransport:
     def __call__(self, source, *, progress=None, time_origin_ns=None, selection_extra_charge=None,
                  receivers=None):
         """Response for one event (``HybridTransportResponse``)."""
+        return self._call(source, time_origin_ns=time_origin_ns,
+                          selection_extra_charge=selection_extra_charge, receivers=receivers)
+
+    def batch(self, sources, *, graph=False, **call_kwargs):
+        """Responses of several events (the serving form's batch): the order-0
+        charges of all of them in one fused persistent launch
+        (``lh_order0_charge_batch``; with ``graph`` replayed from a HIP graph
+        captured once per capacity), the rest of each event as in a plain call.
+        Each response equals the plain call's bit for bit; an event that
+        raises gives its exception in the returned list in place of a response.
+        ``call_kwargs`` (``time_origin_ns``, ``selection_extra_charge``) apply
+        to every event; a ``receivers`` subset is not batched."""
+        import torch
+        from .ballistic import DeviceElements, check_flags
+        if call_kwargs.get("receivers") is not None:
+            raise ValueError("batched events take the route's whole detector")
+        residents = []
+        for source in sources:
+            try:
+                residents.append(source if isinstance(source, ResidentShower)
+                                 else ResidentShower(source, self.device))
+            except (TypeError, ValueError) as error:
+                residents.append(error)
+        fused = []
+        for resident in residents:
+            if isinstance(resident, ResidentShower):
+                try:
+                    fused.append(DeviceElements.from_chords(resident.chords))
+                except ValueError:
+                    pass        # the plain call raises it for this event
+        order0 = self.order0()
+        prefetched = {}
+        if fused:
+            out, ws = order0.charge_batch_device(fused, graph=graph)
+            torch.cuda.current_stream().synchronize()
+            try:
+                check_flags(ws)
+                host = out.cpu().numpy()
+                for row, elements in enumerate(fused):
+                    prefetched[(elements.struct.start_m, elements.count)] = host[row]
+            except ValueError:
+                pass            # the event at fault raises in its own call
+        order0 = _PrefetchedOrder0(order0, prefetched)
+        responses = []
+        for resident in residents:
+            if not isinstance(resident, ResidentShower):
+                responses.append(resident)
+                continue
+            try:
+                responses.append(self._call(resident, order0=order0, **call_kwargs))
+            except (ValueError, RuntimeError, TypeError) as error:
+                responses.append(error)
+        return responses
+
+    def _call(self, source, *, time_origin_ns=None, selection_extra_charge=None, receivers=None,
+              order0=None):
         # detail threshold (transport.py:210-254): every selection decision at the
         # lowered threshold, ``active`` at the original one -- also for a receiver
         # subset, whose own threshold is zero
@@ -852,7 +908,7 @@ class ShowerTransport:
         lowered = detail is not None and detail < kernel.config.threshold_pe
         if lowered:
             kernel = kernel_view(kernel, threshold_pe=float(detail))
-        order0 = self.order0()
+        order0 = self.order0() if order0 is None else order0
         if receivers is not None:
             kernel = kernel_view(kernel, receivers=receivers, threshold_pe=0.0)
             order0 = None
@@ -872,6 +928,24 @@ class ShowerTransport:
             response.metadata["detail_threshold_pe"] = float(detail)
             response.metadata["active_modules"] = int(np.count_nonzero(response.active))
         return response
+
+
+class _PrefetchedOrder0:
+    """The route's order-0 evaluator with the charges of some events already
+    computed (by one fused launch): ``charge`` of those events' device elements
+    returns them, everything else goes to the evaluator itself."""
+
+    def __init__(self, order0, prefetched):
+        self._order0 = order0
+        self._prefetched = prefetched
+
+    def charge(self, elements):
+        struct = getattr(elements, "struct", None)
+        got = None if struct is None else self._prefetched.get((struct.start_m, elements.count))
+        return got.copy() if got is not None else self._order0.charge(elements)
+
+    def __getattr__(self, name):
+        return getattr(self._order0, name)
 
 
 class ResidentShower:

@@ -41,17 +41,25 @@ class FootprintModel:
     """Device bytes an event needs, learned from the events served so far.
 
     Each worker process reports the peak device memory it allocated during an
-    event (the caching allocator's high-water mark above its level before the
-    call). The model keeps the smallest peak seen (the fixed part: workspaces
-    sized by the tables and the OM count) and the largest excess per element
-    above it, and admits an event of ``n`` elements as ``margin * (fixed +
-    per_element * n)``; before the first report it uses :func:`footprint_bytes`.
+    event (or a batch of events: the caching allocator's high-water mark above
+    its level before the call). An event's need is a fixed part (workspaces
+    sized by the tables and the OM count) plus a part that grows with its
+    elements: the model keeps the largest peak of small events (below
+    ``SMALL`` elements) and, from the larger ones, the largest excess per
+    element above the smallest peak seen; it admits an event of ``n``
+    elements as ``margin * max(small peak, smallest peak + per_element * n)``.
+    Small events alone would overstate the per-element part (their peaks are
+    mostly the fixed part), so only large ones set it. Before the first
+    report it uses :func:`footprint_bytes`.
     """
+
+    SMALL = 16384
 
     def __init__(self, oms, margin=1.25):
         self.oms = int(oms)
         self.margin = float(margin)
         self.fixed = None
+        self.small_peak = 0.0
         self.per_element = 0.0
         self.samples = 0
         self._lock = threading.Lock()
@@ -61,7 +69,7 @@ class FootprintModel:
         with self._lock:
             if self.fixed is None:
                 return footprint_bytes(elements, self.oms)
-            return int(self.margin * (self.fixed + self.per_element * n))
+            return int(self.margin * max(self.small_peak, self.fixed + self.per_element * n))
 
     def record(self, elements, peak_bytes):
         n = max(len(elements), 1)
@@ -69,7 +77,10 @@ class FootprintModel:
             self.samples += 1
             if self.fixed is None or peak_bytes < self.fixed:
                 self.fixed = float(peak_bytes)
-            self.per_element = max(self.per_element, (peak_bytes - self.fixed) / n)
+            if n < self.SMALL:
+                self.small_peak = max(self.small_peak, float(peak_bytes))
+            else:
+                self.per_element = max(self.per_element, (peak_bytes - self.fixed) / n)
 
 
 class ShowerServer:
@@ -208,7 +219,8 @@ class StagedSource:
         return self.count
 
 
-def _process_main(factory, device, threads, tasks, results, slot, memory_fraction=None):
+def _process_main(factory, device, threads, tasks, results, slot, memory_fraction=None,
+                  graphs=False):
     """One worker process: build the route, then answer tasks until ``None``.
     ``device`` None leaves the GPU alone (host-only routes). The process uses at
     most ``memory_fraction`` of the device memory; an event that runs out of it
@@ -250,30 +262,56 @@ def _process_main(factory, device, threads, tasks, results, slot, memory_fractio
         if kind == "release":
             staged.pop(key, None)
             continue
+        # a task is a batch of events (keys, payloads): one event runs as a plain
+        # call, several through route.batch (one fused order-0 launch for all)
+        keys, payloads = (key, payload) if kind == "batch" else ([key], [payload])
+        sent = set()
         try:
-            source = staged[payload.key] if isinstance(payload, StagedSource) else payload
+            sources = [staged[p.key] if isinstance(p, StagedSource) else p for p in payloads]
             retries = 0
             base = 0
             if device is not None:
                 torch.cuda.reset_peak_memory_stats()
                 base = torch.cuda.memory_allocated()
             try:
-                response = route(source, **call_kwargs)
+                if len(sources) == 1:
+                    responses = [route(sources[0], **call_kwargs)]
+                else:
+                    responses = route.batch(sources, graph=graphs, **call_kwargs)
+                    if any(isinstance(r, out_of_memory) for r in responses):
+                        raise next(r for r in responses if isinstance(r, out_of_memory))
             except out_of_memory:
-                # counted and reported: the admission model underestimated this event
+                # counted and reported: the admission model underestimated the
+                # events; each runs once more on its own
                 from . import _lib
                 retries = 1
                 _lib.release_workspaces()
                 torch.cuda.empty_cache()
-                response = route(source, **call_kwargs)
+                responses = []
+                for source in sources:
+                    try:
+                        responses.append(route(source, **call_kwargs))
+                    except (ValueError, RuntimeError, TypeError) as error:
+                        responses.append(error)
             if drain is not None:
                 drain()
             peak = (torch.cuda.max_memory_allocated() - base) if device is not None else 0
-            response.metadata["out_of_memory_retries"] = retries
-            response.metadata["device_peak_bytes"] = int(peak)
-            results.put(("done", slot, key, _pack_response(response)))
+            for k, response in zip(keys, responses):
+                if isinstance(response, BaseException):
+                    results.put(("error", slot, k, (type(response).__name__, str(response))))
+                else:
+                    response.metadata["out_of_memory_retries"] = retries
+                    response.metadata["device_peak_bytes"] = int(peak)
+                    response.metadata["batch_events"] = len(sources)
+                    response.metadata["batch_elements"] = int(sum(len(getattr(x, "elements", x))
+                                                                  for x in sources))
+                    results.put(("done", slot, k, _pack_response(response)))
+                sent.add(k)
         except BaseException as error:
-            results.put(("error", slot, key, (type(error).__name__, str(error))))
+            for k in keys:
+                if k not in sent:
+                    results.put(("error", slot, k, (type(error).__name__, str(error))))
+        results.put(("free", slot, None, None))
     limits.restore_original_limits()
 
 
@@ -293,17 +331,29 @@ class ProcessShowerServer:
     says so) and the largest peak. ``submit`` returns a ``Future`` as ``ShowerServer.submit`` does.
     Sources travel as host element arrays (a ``ResidentShower`` is sent as its
     host elements), or are staged ahead in device memory with ``stage``.
+
+    Continuous batching (``batch`` > 1): when a process is free, the event at
+    the head of the queue and the events already waiting behind it (at most
+    ``batch``, while they fit the memory budget, and only events without their
+    own call arguments) go to it as one task, which it runs through
+    ``route.batch`` (their order-0 charges in one fused persistent launch;
+    with ``graphs`` that launch is replayed from a HIP graph captured once per
+    capacity); events arriving meanwhile wait for the next free process. Each
+    response is the plain call's.
     """
 
     def __init__(self, factory, *, processes=2, detector, memory_budget_bytes=None,
                  device="cuda:0", threads_per_process=None, memory_fraction=None,
-                 start_timeout_s=600.0):
+                 start_timeout_s=600.0, batch=1, graphs=False):
         import multiprocessing as mp
         import os
         if processes < 1:
             raise ValueError("processes must be positive")
+        if batch < 1:
+            raise ValueError("batch must be positive")
         self.detector = detector
         self.device = device
+        self.batch = int(batch)
         # each worker's share of the device memory (the caller keeps one share)
         fraction = memory_fraction if memory_fraction is not None else 0.9 / (processes + 1)
         if memory_budget_bytes is None:
@@ -324,7 +374,7 @@ class ProcessShowerServer:
         self._tasks = [context.Queue() for _ in range(processes)]
         self._processes = [context.Process(target=_process_main,
                                            args=(factory, device, threads, self._tasks[i],
-                                                 self._results, i, fraction),
+                                                 self._results, i, fraction, graphs),
                                            name=f"shower-process-{i}", daemon=True)
                            for i in range(processes)]
         for process in self._processes:
@@ -407,31 +457,58 @@ class ProcessShowerServer:
         return [f.result() for f in futures]
 
     def _dispatch(self):
+        held = None
         while True:
-            item = self._waiting.get()
+            item = held if held is not None else self._waiting.get()
+            held = None
             if item is None:
                 return
-            future, elements, call_kwargs, _ = item
             # the model at dispatch time (it learns from the events finished so far)
-            need = self.footprint.need(elements)
+            group = [item + (self.footprint.need(item[1]),)]
             with self._lock:
                 self._idle = [i for i in self._idle if self._processes[i].is_alive()]
                 while not self._idle or (self._inflight > 0
-                                         and self._inflight + need > self.budget):
+                                         and self._inflight + group[0][4] > self.budget):
                     if not any(process.is_alive() for process in self._processes):
                         break
                     self._lock.wait(timeout=1.0)
                     self._idle = [i for i in self._idle if self._processes[i].is_alive()]
                 if not self._idle:
+                    future = item[0]
                     future.latency_s = time.perf_counter() - future.submitted
                     future.set_exception(RuntimeError("no worker process is running"))
                     continue
+                # continuous batching: events already waiting join this task
+                # while they fit the memory budget and the batch size
+                need = group[0][4]
+                while len(group) < self.batch:
+                    try:
+                        extra = self._waiting.get_nowait()
+                    except queue.Empty:
+                        break
+                    if extra is None:
+                        held = None
+                        self._waiting.put(None)
+                        break
+                    extra_need = self.footprint.need(extra[1])
+                    if (extra[2] or item[2]       # events with their own arguments run alone
+                            or self._inflight + need + extra_need > self.budget):
+                        held = extra
+                        break
+                    group.append(extra + (extra_need,))
+                    need += extra_need
                 slot = self._idle.pop(0)
                 self._inflight += need
-                key = self._next_key
-                self._next_key += 1
-                self._pending[key] = (future, need, slot, len(elements))
-            self._tasks[slot].put(("run", key, elements, call_kwargs))
+                keys = []
+                for future, elements, call_kwargs, _, event_need in group:
+                    key = self._next_key
+                    self._next_key += 1
+                    self._pending[key] = (future, event_need, slot, len(elements))
+                    keys.append(key)
+            if len(group) == 1:
+                self._tasks[slot].put(("run", keys[0], group[0][1], group[0][2]))
+            else:
+                self._tasks[slot].put(("batch", keys, [g[1] for g in group], group[0][2]))
 
     def _collect(self):
         errors = {"ValueError": ValueError, "RuntimeError": RuntimeError,
@@ -449,17 +526,22 @@ class ProcessShowerServer:
                     self._staging[key].append(payload)
                     self._lock.notify_all()
                 continue
+            if kind == "free":
+                with self._lock:
+                    self._idle.append(slot)
+                    self._lock.notify_all()
+                continue
             with self._lock:
                 future, need, _, count = self._pending.pop(key)
                 self._inflight -= need
-                self._idle.append(slot)
                 self._lock.notify_all()
             future.latency_s = time.perf_counter() - future.submitted
             if kind == "done":
                 meta = payload["metadata"]
                 peak = int(meta.get("device_peak_bytes", 0))
                 if peak > 0:
-                    self.footprint.record(range(count), peak)
+                    # a batch's peak belongs to all of its elements together
+                    self.footprint.record(range(int(meta.get("batch_elements", count))), peak)
                 with self._lock:
                     self.stats["events"] += 1
                     self.stats["out_of_memory_retries"] += int(meta.get("out_of_memory_retries", 0))

@@ -44,6 +44,16 @@ class _Route:
                                        {"value": value})
 
 
+    def batch(self, sources, graph=True, **call_kwargs):
+        out = []
+        for elements in sources:
+            try:
+                out.append(self(elements, **call_kwargs))
+            except ValueError as error:
+                out.append(error)
+        return out
+
+
 def _factory():
     return _Route()
 
@@ -122,6 +132,32 @@ def test_footprint_model_learns_from_measured_peaks():
     model.record(range(100_000), 3300 * 2 ** 20)   # a large one: bytes per element
     per_element = (3300 - 300) * 2 ** 20 / 100_000
     assert model.need(range(50_000)) == int(1.25 * (300 * 2 ** 20 + per_element * 50_000))
-    # a later, larger per-element peak raises the estimate; it never falls
+    # a small event's peak is mostly the fixed part: it raises the floor of
+    # the estimate, not the bytes per element
     model.record(range(10_000), 1000 * 2 ** 20)
-    assert model.per_element >= per_element
+    assert model.per_element == per_element
+    assert model.need(range(2000)) == int(1.25 * 1000 * 2 ** 20)
+    # a later, larger per-element peak of a large event raises the estimate
+    model.record(range(20_000), 2000 * 2 ** 20)
+    assert model.per_element > per_element
+
+
+def test_waiting_events_are_batched_into_one_task():
+    # one process, batches of up to 4: the events queued behind a running task
+    # go to the process together; every result is the plain call's, and an
+    # event that fails inside a batch fails alone
+    with serve.ProcessShowerServer(_factory, processes=1, detector=np.zeros((OMS, 3)),
+                                   device=None, memory_budget_bytes=10 ** 12,
+                                   threads_per_process=1, batch=4) as server:
+        events = [_Elements(i + 1, i % OMS) for i in range(9)] + [_Elements(-1, 0)]
+        futures = [server.submit(e) for e in events]
+        with pytest.raises(ValueError, match="negative"):
+            futures[-1].result(timeout=120)
+        out = [f.result(timeout=120) for f in futures[:-1]]
+    for event, r in zip(events, out):
+        want = _Route()(event)
+        assert np.array_equal(r.charge_components_pe, want.charge_components_pe)
+        assert np.array_equal(r.components_pe, want.components_pe)
+    sizes = [r.metadata["batch_events"] for r in out]
+    assert max(sizes) > 1 and max(sizes) <= 4
+    assert server.stats["events"] == 9

@@ -44,6 +44,43 @@ def test_each_served_result_equals_the_plain_call(served):
     assert sum(int(a.active.sum()) for a in alone) > 0
 
 
+def test_fused_order0_charge_equals_each_event(served):
+    """One fused launch over several events' (event, OM, chunk) items gives each
+    event its own launch's charges bit for bit, eager and replayed from a HIP
+    graph: each capacity is captured once and replayed with other eager work
+    and synchronisations between the replays, on batches that fill the
+    capacity and on batches with padding entries."""
+    import torch
+    from lighthit_amd.ballistic import DeviceElements
+    workload, kernel, route, events, mixed, alone, futures = served
+    order0 = route.order0()
+    device_events = [DeviceElements(e) for e in events.values()]
+    single = [order0.charge(e) for e in device_events]
+    eager = order0.charge_batch(device_events)
+    for k, q in enumerate(single):
+        assert np.array_equal(eager[k], q), k
+    subsets = [[0, 1, 2, 3], [4, 5, 6], [0, 1, 2, 3], [9, 10], [4, 5, 6], [0, 1, 2, 3]]
+    for subset in subsets:
+        got = order0.charge_batch([device_events[i] for i in subset], graph=True)
+        between = order0.charge(device_events[8])            # eager work between replays
+        torch.cuda.synchronize()
+        assert np.array_equal(between, single[8])
+        for row, i in enumerate(subset):
+            assert np.array_equal(got[row], single[i]), (subset, i)
+    assert len(order0._graphs) >= 2
+
+
+def test_route_batch_equals_the_plain_call(served):
+    workload, kernel, route, events, mixed, alone, futures = served
+    batched = route.batch(list(events.values()))
+    for b, a in zip(batched, alone):
+        assert not isinstance(b, Exception), b
+        assert np.array_equal(b.active, a.active)
+        assert np.array_equal(b.charge_components_pe, a.charge_components_pe)
+        assert np.array_equal(b.components_pe, a.components_pe)
+        assert np.array_equal(b.spectrum_ge2_pe, a.spectrum_ge2_pe)
+
+
 @pytest.mark.parametrize("index", [7, SMALL_BRIGHT])
 def test_small_events_match_the_oracle_route(served, index):
     from oracle import route as oracle_route
