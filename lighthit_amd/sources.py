@@ -201,6 +201,66 @@ class SpectralLightElements:
 
 
 @dataclass(frozen=True)
+class G4Shower:
+    """A Geant4 shower event read from the G4 step files (``sources.py:344-393``):
+    its spectral elements, the file and the event number."""
+    elements: SpectralLightElements
+    input_path: str = ""
+    event: int = 0
+
+    @property
+    def extent_m(self):
+        ends = np.vstack((self.elements.start_m, self.elements.start_m
+                          + self.elements.length_m[:, None] * self.elements.direction))
+        return float(np.linalg.norm(ends.max(axis=0) - ends.min(axis=0)))
+
+    def placed(self, pose: SourcePose):
+        return replace(self, elements=self.elements.placed(pose))
+
+    @classmethod
+    def from_hdf5(cls, path, event=0, *, reference_phase_index=1.35, max_elements=None):
+        """Elements of ``event_<event>/tracks`` (rows ``uid``, ``x_m``, ``y_m``,
+        ``z_m``, ``t_ns``, ``beta``, ``step_length_m``): every two consecutive
+        rows of one particle (same ``uid``) bound one step, emitted along the
+        chord between them with the mean of their beta, the Frank-Tamm yield of
+        the step's true path (the later row's ``step_length_m``) and the times
+        of the two rows; steps with zero chord or path, or below the Cherenkov
+        threshold of ``reference_phase_index``, are dropped; ``max_elements``
+        keeps the first steps only. The same steps, in the same order, as the
+        reference's reader."""
+        from pathlib import Path
+
+        from .hdf5 import read_dataset
+        path = Path(path)
+        rows = read_dataset(path, f"event_{int(event)}/tracks")
+        needed = {"uid", "x_m", "y_m", "z_m", "t_ns", "beta", "step_length_m"}
+        if rows.dtype.names is None or not needed.issubset(rows.dtype.names):
+            raise ValueError(f"G4 tracks must carry {sorted(needed)}")
+        uid = np.asarray(rows["uid"], np.int64)
+        later = np.flatnonzero(uid[1:] == uid[:-1]) + 1       # second row of each step
+        if max_elements is not None:
+            later = later[:int(max_elements)]
+        earlier = later - 1
+        xyz = np.column_stack([np.asarray(rows[k], float) for k in ("x_m", "y_m", "z_m")])
+        chord = xyz[later] - xyz[earlier]
+        length = np.linalg.norm(chord, axis=1)
+        beta_rows = np.asarray(rows["beta"], float)
+        beta = 0.5 * (beta_rows[earlier] + beta_rows[later])
+        path_m = np.asarray(rows["step_length_m"], float)[later]
+        kept = np.flatnonzero((length > 0) & (path_m > 0)
+                              & (beta * reference_phase_index > 1))
+        c0 = 2 * np.pi * FINE_STRUCTURE * 1e9 * path_m[kept]
+        times = np.asarray(rows["t_ns"], float)
+        elements = SpectralLightElements(
+            xyz[earlier][kept], chord[kept] / length[kept, None], length[kept], c0,
+            -c0 / beta[kept] ** 2, beta[kept], times[earlier][kept], times[later][kept],
+            later[kept].astype(np.int64), uid[later][kept], reference_phase_index,
+            {"file": str(path), "event": int(event), "rows": int(len(rows)),
+             "spectral_model": "two fields S0=lambda^-2, S2=lambda^-2*n_phase^-2"})
+        return cls(elements, str(path), int(event))
+
+
+@dataclass(frozen=True)
 class SyntheticShower:
     """Seeded synthetic axial cascade (``sources.py:396-441``)."""
     elements: SpectralLightElements
