@@ -192,6 +192,7 @@ class DeviceElements:
         cls._check(elements)
         self = cls.__new__(cls)
         self.count = len(elements)
+        self.chords = chords
         self._keep = [chords.start_m, chords.direction, chords.length_m, chords.coefficient0,
                       chords.coefficient2, chords.beta, chords.start_ns, chords.end_ns]
         self.struct = _Elements(*[t.data_ptr() for t in self._keep], self.count)
@@ -230,15 +231,18 @@ class Order0:
         need = _workspace_bytes(self.oms.count, n_elements)
         return _lib.workspace("order0_charge", self.device, need, stream)
 
-    def charge_device(self, elements, out=None, stream=None):
-        """Enqueue the charge of every OM; returns (charge tensor, workspace)."""
+    def charge_device(self, elements, out=None, stream=None, oms=None):
+        """Enqueue the charge of every OM (or of ``oms``, a :class:`DeviceOMs`);
+        returns (charge tensor, workspace)."""
         import torch
+        oms = self.oms if oms is None else oms
         if out is None:
-            out = torch.empty(self.oms.count, dtype=torch.float64, device=self.device)
-        ws = self.workspace(elements.count, stream)
+            out = torch.empty(oms.count, dtype=torch.float64, device=self.device)
+        ws = _lib.workspace("order0_charge", self.device,
+                            _workspace_bytes(oms.count, elements.count), stream)
         with _lib.stage("order0_charge"):
             _lib.call("lh_order0_charge", ctypes.byref(self.optics.struct),
-                      ctypes.byref(self.oms.struct), ctypes.byref(elements.struct),
+                      ctypes.byref(oms.struct), ctypes.byref(elements.struct),
                       ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
                       ctypes.c_size_t(ws.numel()), _lib.stream_handle(stream))
         return out, ws
@@ -292,7 +296,12 @@ class Order0:
     #: launch the same chain directly.
     GRAPH_EVENTS, GRAPH_CHUNKS = 16, 4096
 
-    def charge_batch_device(self, events, *, graph=False, stream=None):
+    def oms_of(self, ids):
+        """The :class:`DeviceOMs` of OMs ``ids`` of the detector (all OMs: None)."""
+        return self.oms if ids is None else DeviceOMs(self.detector.subset(np.asarray(ids)),
+                                                      self.device)
+
+    def charge_batch_device(self, events, *, graph=False, stream=None, oms=None):
         """Enqueue the charges of every OM for several events in one fused launch
         (``lh_order0_charge_batch``: one persistent kernel over the (event, OM,
         chunk) items of all of them); returns ((events, OM) charge tensor,
@@ -302,12 +311,15 @@ class Order0:
         With ``graph`` the launch chain is captured once per capacity (events
         rounded up to a power of two, element chunks to 256, 1024 or 4096) as a
         HIP graph and replayed; only the small table of this batch's events is
-        written before the replay."""
+        written before the replay. ``oms`` (a :class:`DeviceOMs`, default all)
+        evaluates those OMs only, without a graph."""
         import torch
         events = list(events)
         n = len(events)
+        graph = graph and oms is None
+        oms = self.oms if oms is None else oms
         if n == 0:
-            return torch.empty((0, self.oms.count), dtype=torch.float64, device=self.device), None
+            return torch.empty((0, oms.count), dtype=torch.float64, device=self.device), None
         counts = (ctypes.c_int * n)(*[int(e.count) for e in events])
         chunks = int(_query("lh_order0_batch_chunks", counts, n))
         graph = graph and n <= self.GRAPH_EVENTS and chunks <= self.GRAPH_CHUNKS
@@ -321,25 +333,25 @@ class Order0:
                 max_chunks *= 4
             if getattr(self, "_graph_ws", None) is None:
                 self._graph_ws = torch.empty(int(_query(
-                    "lh_order0_batch_workspace_bytes", self.oms.count, self.GRAPH_EVENTS,
+                    "lh_order0_batch_workspace_bytes", oms.count, self.GRAPH_EVENTS,
                     self.GRAPH_CHUNKS)), dtype=torch.uint8, device=self.device)
                 self._graphs = {}
             ws = self._graph_ws
             captured = self._graphs.get((max_events, max_chunks))
         else:
             max_events, max_chunks = n, chunks
-            need = int(_query("lh_order0_batch_workspace_bytes", self.oms.count, max_events,
+            need = int(_query("lh_order0_batch_workspace_bytes", oms.count, max_events,
                               max_chunks))
             ws = _lib.workspace("order0_batch", self.device, need, stream)
-        out = torch.empty((n, self.oms.count), dtype=torch.float64, device=self.device)
+        out = torch.empty((n, oms.count), dtype=torch.float64, device=self.device)
         table = (_Elements * n)(*[e.struct for e in events])
         outs = (ctypes.c_void_p * n)(*[out[k].data_ptr() for k in range(n)])
         handle = _lib.stream_handle(stream)
         with _lib.stage("order0_charge"):
-            _lib.call("lh_order0_batch_set", ctypes.byref(self.oms.struct), table, outs, n,
+            _lib.call("lh_order0_batch_set", ctypes.byref(oms.struct), table, outs, n,
                       max_events, max_chunks, ctypes.c_void_p(ws.data_ptr()),
                       ctypes.c_size_t(ws.numel()), handle)
-            args = (ctypes.byref(self.optics.struct), ctypes.byref(self.oms.struct), max_events,
+            args = (ctypes.byref(self.optics.struct), ctypes.byref(oms.struct), max_events,
                     max_chunks, ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()))
             if not graph:
                 _lib.call("lh_order0_charge_batch", *args, handle)
@@ -375,12 +387,13 @@ class Order0:
                 return np.stack([self.charge(e) for e in device_events])
         return out.cpu().numpy()
 
-    def charge(self, elements):
-        """Charge per OM (numpy), raising the reference's errors."""
+    def charge(self, elements, ids=None):
+        """Charge per OM (numpy), raising the reference's errors; with ``ids``
+        the charges of those OMs only (the others are not evaluated)."""
         import torch
         device_elements = elements if isinstance(elements, DeviceElements) else DeviceElements(
             elements, self.device)
-        out, ws = self.charge_device(device_elements)
+        out, ws = self.charge_device(device_elements, oms=self.oms_of(ids))
         torch.cuda.synchronize()
         check_flags(ws)
         return out.cpu().numpy()

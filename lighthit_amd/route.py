@@ -618,8 +618,9 @@ def transport_prompt(kernel, elements, settings, *, order0=None, selection_extra
     device_elements = DeviceElements.from_chords(chords)
     charge = np.zeros((n, 2))
     if len(work):
-        q0 = order0.charge(device_elements)
-        charge[work, 0] = q0[work]
+        # the OMs of the work set only, as the reference evaluates its subset
+        # detector (prompt.py:641-650); every OM's charge is the same either way
+        charge[work, 0] = order0.charge(device_elements, ids=work)
     timings["order0_charge_seconds"] = perf_counter() - stage
     # order 1
     stage = perf_counter()
@@ -876,8 +877,22 @@ class ShowerTransport:
                     pass        # the plain call raises it for this event
         order0 = self.order0()
         prefetched = {}
-        if fused:
-            out, ws = order0.charge_batch_device(fused, graph=graph)
+        # the OMs any event's order 0 evaluates: within the prompt's selection
+        # radius of its elements (the work sets of transport_prompt)
+        positions = np.ascontiguousarray(self.kernel.detector.positions_m, float)
+        reach = (float(self.kernel.config.radial_range_m[1])
+                 if self.config.prompt.max_distance_m is None
+                 else float(self.config.prompt.max_distance_m))
+        union = np.zeros(len(positions), bool)
+        for elements in fused:
+            chords = elements.chords
+            distance, _ = segment_bounds(positions, chords.host, None, self.device, chords)
+            union |= distance <= reach
+        ids = np.flatnonzero(union)
+        if fused and len(ids):
+            out, ws = order0.charge_batch_device(fused, graph=graph and len(ids) == len(union),
+                                                 oms=None if len(ids) == len(union)
+                                                 else order0.oms_of(ids))
             torch.cuda.current_stream().synchronize()
             try:
                 check_flags(ws)
@@ -886,7 +901,7 @@ class ShowerTransport:
                     prefetched[(elements.struct.start_m, elements.count)] = host[row]
             except ValueError:
                 pass            # the event at fault raises in its own call
-        order0 = _PrefetchedOrder0(order0, prefetched)
+        order0 = _PrefetchedOrder0(order0, prefetched, ids)
         responses = []
         for resident in residents:
             if not isinstance(resident, ResidentShower):
@@ -932,17 +947,26 @@ class ShowerTransport:
 
 class _PrefetchedOrder0:
     """The route's order-0 evaluator with the charges of some events already
-    computed (by one fused launch): ``charge`` of those events' device elements
-    returns them, everything else goes to the evaluator itself."""
+    computed (by one fused launch on a set of OMs): ``charge`` of those
+    events' device elements on OMs of that set returns them, everything else
+    goes to the evaluator itself."""
 
-    def __init__(self, order0, prefetched):
+    def __init__(self, order0, prefetched, ids):
         self._order0 = order0
         self._prefetched = prefetched
+        self._ids = ids
 
-    def charge(self, elements):
+    def charge(self, elements, ids=None):
         struct = getattr(elements, "struct", None)
         got = None if struct is None else self._prefetched.get((struct.start_m, elements.count))
-        return got.copy() if got is not None else self._order0.charge(elements)
+        if got is not None:
+            want = np.arange(len(self._order0.detector.positions_m)) if ids is None \
+                else np.asarray(ids)
+            at = np.searchsorted(self._ids, want)
+            if np.all(at < len(self._ids)) and np.array_equal(self._ids[np.minimum(
+                    at, len(self._ids) - 1)], want):
+                return got[at].copy()
+        return self._order0.charge(elements, ids=ids)
 
     def __getattr__(self, name):
         return getattr(self._order0, name)
