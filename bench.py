@@ -741,6 +741,9 @@ def main():
                         help="keep the counter passes' CSV output in this folder")
     parser.add_argument("--profile-run", action="store_true",
                         help="only run two events of the plain route (counter passes)")
+    parser.add_argument("--profile-stream", type=int, default=0,
+                        help="only run this many serving-stream events of the plain route, "
+                             "one call at a time (kernel traces of the stream)")
     parser.add_argument("--copy-kernel-mib", type=int, default=0,
                         help="only run a plain device copy of this size (HBM counter calibration)")
     args = parser.parse_args()
@@ -765,6 +768,14 @@ def main():
     from lighthit_amd import order1 as order1_module
     from lighthit_amd.route import ResidentShower
     warnings.simplefilter("ignore", RuntimeWarning)
+
+    if args.profile_stream:
+        from lighthit_amd.synthetic import serving_stream
+        workload, kernel, route = build_route(args.workload, args.precision, args.pack)
+        stream = list(serving_stream(workload.detector, args.profile_stream))
+        report = plain_stream(route, stream)["report"]
+        print(json.dumps({"profile_stream": len(stream), **report}), flush=True)
+        return
 
     if args.profile_run:
         workload, kernel, route = build_route(args.workload, args.precision, args.pack)
