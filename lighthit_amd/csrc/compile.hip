@@ -12,6 +12,7 @@
 // summed with e^{i omega tau dt} in FP64 and combined into the five-point
 // stencil (:563-598). All arithmetic is FP64; channels are written complex64.
 #include "lh_common.h"
+#include "lh_sort.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -585,7 +586,7 @@ int make_layout(const lh_compile_lattice* L, int count, Layout& out) {
     size_t span = (size_t)L->kmax * L->positions;
     int channels = channel_count(L->degree, std::min(L->degree, L->azimuthal_degree));
     size_t sort_temp = 0, scan_temp = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_temp, (unsigned*)nullptr,
+    if (lh::sort_pairs(nullptr, sort_temp, (unsigned*)nullptr,
                                            (unsigned*)nullptr, (int*)nullptr, (int*)nullptr,
                                            (int)total_nodes, 0, bits_for((long long)span),
                                            (hipStream_t)0) != hipSuccess) {
@@ -706,7 +707,7 @@ int lh_compile_layout(const lh_compile_field* field, const lh_compile_lattice* l
     if ((status = lh::check_launch("compile_nodes")) != LH_OK) return status;
     size_t sort_bytes = lay.scan_temp - lay.sort_temp;
     // keys are nonnegative: sorted as unsigned on their low bits
-    LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+    LH_HIP_CALL(lh::sort_pairs(
         base + lay.sort_temp, sort_bytes, reinterpret_cast<unsigned*>(p.keys),
         reinterpret_cast<unsigned*>(p.sorted_keys), p.ids, p.sorted_ids, total, 0,
         bits_for(p.span), s));

@@ -22,6 +22,7 @@
 //              a fixed order: deterministic, no atomics).
 // All arithmetic is FP64.
 #include "lh_common.h"
+#include "lh_sort.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -2361,11 +2362,11 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
     unsigned long long max_key = (unsigned long long)grid->dims[0] * grid->dims[1] * grid->dims[2]
                                  * 6ULL * grid->pixel_face * grid->pixel_face;
     size_t sort_temp32 = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_temp, (long long*)nullptr,
+    if (lh::sort_pairs(nullptr, sort_temp, (long long*)nullptr,
                                            (long long*)nullptr, (int*)nullptr, (int*)nullptr,
                                            (int)nodes, 0, bits_for(max_key), (hipStream_t)0)
             != hipSuccess ||
-        hipcub::DeviceRadixSort::SortPairs(nullptr, sort_temp32, (unsigned*)nullptr,
+        lh::sort_pairs(nullptr, sort_temp32, (unsigned*)nullptr,
                                            (unsigned*)nullptr, (int*)nullptr, (int*)nullptr,
                                            (int)nodes, 0, 32, (hipStream_t)0) != hipSuccess) {
         lh::set_error("radix sort workspace query failed");
@@ -2377,7 +2378,7 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
         lh::set_error("scan workspace query failed");
         return LH_ERR_HIP;
     }
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, psort_temp, (long long*)nullptr,
+    if (lh::sort_pairs(nullptr, psort_temp, (long long*)nullptr,
                                            (long long*)nullptr, (int*)nullptr, (int*)nullptr,
                                            (int)runs_cap, 0, bits_for(max_key),
                                            (hipStream_t)0) != hipSuccess) {
@@ -2491,10 +2492,10 @@ int grouped_layout(int work, int n_pen, int slots, int bands, int bins, GroupLay
                "%lld (pencil, OM) entries in one grouped call; pass fewer OMs per call",
                entries);
     size_t sort_temp = 0, sort_temp32 = 0, scan_temp = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(
+    if (lh::sort_pairs(
             nullptr, sort_temp, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
             (int*)nullptr, (int*)nullptr, (int)entries, 0, 64, (hipStream_t)0) != hipSuccess ||
-        hipcub::DeviceRadixSort::SortPairs(
+        lh::sort_pairs(
             nullptr, sort_temp32, (unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr,
             (int*)nullptr, (int)entries, 0, 32, (hipStream_t)0) != hipSuccess) {
         lh::set_error("sort workspace query failed");
@@ -2701,12 +2702,12 @@ int lh_order1_compress_keys(const lh_order1_chords* chords, const lh_order1_grid
                                  * 6ULL * grid->pixel_face * grid->pixel_face;
     size_t sort_bytes = L.scan_temp - L.sort_temp;
     if (k.key32) {
-        LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+        LH_HIP_CALL(lh::sort_pairs(
             base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned*>(k.keys),
             reinterpret_cast<unsigned*>(k.sorted_keys), k.ids, k.sorted_ids, (int)nodes, 0,
             bits_for(max_key), s));
     } else {
-        LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(base + L.sort_temp, sort_bytes, k.keys,
+        LH_HIP_CALL(lh::sort_pairs(base + L.sort_temp, sort_bytes, k.keys,
                                                        k.sorted_keys, k.ids, k.sorted_ids,
                                                        (int)nodes, 0, bits_for(max_key), s));
     }
@@ -2784,7 +2785,7 @@ int lh_order1_compress_pencils(const lh_order1_chords* chords, const lh_order1_g
                                  * 6ULL * grid->pixel_face * grid->pixel_face;
     size_t psort_bytes = L.total - L.psort_temp;
     // pencils are in fine-key order: a stable sort by parent key gives (parent, fine) order
-    LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(base + L.psort_temp, psort_bytes, pkeys,
+    LH_HIP_CALL(lh::sort_pairs(base + L.psort_temp, psort_bytes, pkeys,
                                                    spkeys, pids, spids, runs, 0,
                                                    bits_for(max_key), s));
     o1_parent_flags<<<blocks(runs + 1), kThreads, 0, s>>>(spkeys, runs, pflag);
@@ -2868,12 +2869,12 @@ int lh_order1_group_keys(const lh_order1_pencils* pencils, const lh_order1_oms* 
     {
         lh::KernelMark mark("o1_sort", s);
         if (G.key32) {
-            LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+            LH_HIP_CALL(lh::sort_pairs(
                 base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned*>(G.keys),
                 reinterpret_cast<unsigned*>(G.sorted_keys), G.vals, G.sorted_vals, (int)total,
                 0, end_bit, s));
         } else {
-            LH_HIP_CALL(hipcub::DeviceRadixSort::SortPairs(
+            LH_HIP_CALL(lh::sort_pairs(
                 base + L.sort_temp, sort_bytes, reinterpret_cast<unsigned long long*>(G.keys),
                 reinterpret_cast<unsigned long long*>(G.sorted_keys), G.vals, G.sorted_vals,
                 (int)total, 0, end_bit, s));
