@@ -583,27 +583,65 @@ __device__ void node_bins(const PanelTables& T, const Slot& S, int npanel, int n
 
 constexpr int kMaxCuts = 12;
 
+// The Illinois method (regula falsi whose kept end has its value halved when
+// that end is kept twice in a row) for the crossing of the centre meridian's
+// arrival time with `Tedge` on [a, b], where the bracket keeps the sign change:
+// every step stays inside the sub-interval; 8 steps reach the FP32 resolution
+// of the times.
+__device__ __forceinline__ float meridian_crossing(const PanelTables& T, const Slot& S, float beta,
+                                                   float Tedge, float a, float b, float fa,
+                                                   float fb) {
+    float c = a;
+    int kept = 0;
+    for (int it = 0; it < 8; ++it) {
+        c = (a * fb - b * fa) / (fb - fa);
+        const float fc = meridian_time(T, S, beta, c) - Tedge;
+        if (fc == 0.f) break;
+        if ((fc < 0.f) == (fb < 0.f)) {
+            b = c;
+            fb = fc;
+            if (kept == 1) fa *= 0.5f;
+            kept = 1;
+        } else {
+            a = c;
+            fa = fc;
+            if (kept == 2) fb *= 0.5f;
+            kept = 2;
+        }
+    }
+    return c;
+}
+
 // Bins of one slot at the lane's theta node. The slot's theta range is cut
 // where a bin edge crosses the centre meridian (sampled on 9 points, refined
-// by bisection) or the rim (there d_entry = D cos(eps) for every theta, so the
-// crossing is where the group index takes one value: closed form); every
-// piece gets the lane's node on a sine map, which absorbs the square-root
-// behaviour of the bin weights at the cuts.
+// by the Illinois method) or the rim (there d_entry = D cos(eps) for every
+// theta, so the crossing is where the group index takes one value: closed
+// form); every piece gets the lane's node on a sine map, which absorbs the
+// square-root behaviour of the bin weights at the cuts.
+//
+// The cuts depend on the slot only, so the 8 lanes of the slot's group (lanes
+// `group` .. `group` + 7, one theta node each, `it` = lane - group) find them
+// together: lane j evaluates sample j (every lane sample 8), the samples are
+// exchanged, every lane lists the same crossings in the same (edge, interval)
+// order, lane j refines crossings j, j + 8, ... and the refined cuts are
+// exchanged. Every lane then holds the cuts a lone lane would find (the same
+// expressions on the same values).
 template <int NE>
 __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int nalpha, float beta,
                           float c0, float c2, float inv_length, float axial, float across,
                           float side, float xt, float wt, float R, const float* edges, int nbins,
-                          double* wave_bins) {
+                          int it, int group, double* wave_bins) {
     const int k = S.piece;
     float cuts[kMaxCuts + 2];
     int ncut = 0;
     cuts[ncut++] = S.ta;
     float tv[9], tt[9];
     float lo_t = 3.4e38f, hi_t = -3.4e38f;
+    const float own = meridian_time(T, S, beta, S.ta + (S.tb - S.ta) * (it * 0.125f));
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         tt[j] = S.ta + (S.tb - S.ta) * (j * 0.125f);
-        tv[j] = meridian_time(T, S, beta, tt[j]);
+        tv[j] = j < 8 ? __shfl(own, group + j, 64) : meridian_time(T, S, beta, tt[j]);
         lo_t = fminf(lo_t, tv[j]);
         hi_t = fmaxf(hi_t, tv[j]);
     }
@@ -621,35 +659,16 @@ __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int n
         }
         e = a;
     }
+    // meridian crossings to refine, packed as (edge << 8) | (cut slot << 4) |
+    // interval j; rim cuts are set at once
+    int task[kMaxCuts];
+    int ntask = 0;
     for (; e <= nbins && edges[e] < hi_t && ncut < kMaxCuts; ++e) {
         const float Tedge = edges[e];
         for (int j = 0; j < 8 && ncut < kMaxCuts; ++j) {
             if ((tv[j] < Tedge) == (tv[j + 1] < Tedge)) continue;
-            // the crossing by the Illinois method (regula falsi whose kept end
-            // has its value halved when that end is kept twice in a row): the
-            // bracket [a, b] keeps the sign change, so every step stays inside
-            // the sub-interval; 8 steps reach the FP32 resolution of the times
-            float a = tt[j], b = tt[j + 1];
-            float fa = tv[j] - Tedge, fb = tv[j + 1] - Tedge;
-            float c = a;
-            int kept = 0;
-            for (int it = 0; it < 8; ++it) {
-                c = (a * fb - b * fa) / (fb - fa);
-                const float fc = meridian_time(T, S, beta, c) - Tedge;
-                if (fc == 0.f) break;
-                if ((fc < 0.f) == (fb < 0.f)) {
-                    b = c;
-                    fb = fc;
-                    if (kept == 1) fa *= 0.5f;
-                    kept = 1;
-                } else {
-                    a = c;
-                    fa = fc;
-                    if (kept == 2) fb *= 0.5f;
-                    kept = 2;
-                }
-            }
-            cuts[ncut++] = c;
+            task[ntask++] = (e << 8) | (ncut << 4) | j;
+            cuts[ncut++] = 0.f;
         }
         // rim crossing: n_g = (T - base) c / (D cos eps)
         if (T.dg[k] != 0.f && ncut < kMaxCuts) {
@@ -659,7 +678,7 @@ __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int n
                 // sin((th - th_a)/2) = (n - n_a) beta cos(th) cos(th_a) / (2 sin((th + th_a)/2))
                 const float dn = q * T.span[k] / T.dlam_dn[k];
                 float dth = 0.f;
-                for (int it = 0; it < 3; ++it) {
+                for (int iter = 0; iter < 3; ++iter) {
                     const float th = S.theta_a + dth;
                     dth = 2.f * asinf(dn * beta * cosf(th) * S.cos_theta_a
                                       / (2.f * sinf(S.theta_a + 0.5f * dth)));
@@ -671,6 +690,22 @@ __device__ void slot_bins(const PanelTables& T, const Slot& S, int npanel, int n
                 }
             }
         }
+    }
+    // lane `it` refines tasks it, it + 8 (at most kMaxCuts < 16 tasks)
+    float refined[2] = {0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int q = it + 8 * r;
+        if (q < ntask) {
+            const int j = task[q] & 7;
+            const float Tedge = edges[task[q] >> 8];
+            refined[r] = meridian_crossing(T, S, beta, Tedge, tt[j], tt[j + 1], tv[j] - Tedge,
+                                           tv[j + 1] - Tedge);
+        }
+    }
+    for (int q = 0; q < ntask; ++q) {
+        const float mine = q < 8 ? refined[0] : refined[1];
+        cuts[(task[q] >> 4) & 15] = __shfl(mine, group + (q & 7), 64);
     }
     cuts[ncut++] = S.tb;
     for (int a = 1; a < ncut; ++a)  // insertion sort (few entries)
@@ -1018,11 +1053,11 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
                         if (ne == 4)
                             slot_bins<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length,
                                          axial, across, side, xt, wt, (float)R, edges, P.nbins,
-                                         wave_bins);
+                                         it, lane - it, wave_bins);
                         else
                             slot_bins<kNe>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
                                            inv_length, axial, across, side, xt, wt, (float)R, edges,
-                                           P.nbins, wave_bins);
+                                           P.nbins, it, lane - it, wave_bins);
                     else
                         pair_sum += S.weight * (ne == 4
                             ? theta_node_value<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
