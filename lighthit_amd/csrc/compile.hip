@@ -373,7 +373,28 @@ __global__ void __launch_bounds__(kThreads) compile_row_keys(Params p) {
 
 // Deposit of one lattice row: sum over its base-key buckets in sorted order of
 // (wz * wt * moment) * P_l Y_lm (_shower_basis.py:482-555).
-__global__ void __launch_bounds__(kThreads) compile_deposit(Params p, int rows, double* deposit) {
+//
+// The bucket's nodes are staged through LDS in tiles of `tile` nodes: the
+// products (wz * wt) * moment of every node (5 per node) and its angular row
+// (channels values, coalesced), so each node's record and angular row are
+// read from memory once per row instead of once per (moment, channel) thread.
+// Every thread then adds its (moment, channel) values in node order with the
+// same rounding as a direct loop.
+constexpr int kDepositAcc = (5 * (kMaxDegree + 1) * (kMaxDegree + 1) + kThreads - 1) / kThreads;
+
+// Nodes per LDS tile of compile_deposit for `channels` angular channels.
+inline int deposit_tile(int channels) {
+    return std::max(8, std::min(64, 32768 / ((channels + 6) * 8)));
+}
+
+inline size_t deposit_lds_bytes(int channels) {
+    const int tile = deposit_tile(channels);
+    return (size_t)tile * (5 + channels) * sizeof(double) + (size_t)tile * sizeof(int);
+}
+
+__global__ void __launch_bounds__(kThreads) compile_deposit(Params p, int rows, int tile,
+                                                            double* deposit) {
+    extern __shared__ double stage[];   // (tile, 5) products, (tile, channels), (tile) ids
     __shared__ int range[8][2];
     __shared__ int link[8][2];   // shift index, tick offset
     int r = blockIdx.x;
@@ -399,41 +420,52 @@ __global__ void __launch_bounds__(kThreads) compile_deposit(Params p, int rows, 
         link[threadIdx.x][1] = dt;
     }
     __syncthreads();
-    int width = L.moments * p.channels;
-    int count = p.field.count;
-    for (int kc = threadIdx.x; kc < width; kc += kThreads) {
-        int k = kc / p.channels, c = kc - k * p.channels;
-        double acc = 0.0;
-        for (int l = 0; l < links; ++l) {
-            int j = link[l][0], dt = link[l][1];
-            const int q_hi = range[l][1];
-            // nodes four at a time: their loads are issued before the sums,
-            // which run in node order as before
-            constexpr int kNodes = 4;
-            for (int q0 = range[l][0]; q0 < q_hi; q0 += kNodes) {
-                double wz[kNodes], tr[kNodes], f[kNodes], ang[kNodes];
+    const int C = p.channels;
+    const int width = L.moments * C;
+    const int count = p.field.count;
+    double* s_wf = stage;
+    double* s_ang = stage + 5 * tile;
+    int* s_id = reinterpret_cast<int*>(stage + (size_t)(5 + C) * tile);
+    double acc[kDepositAcc];
 #pragma unroll
-                for (int i = 0; i < kNodes; ++i) {
-                    const int q = q0 + i;
-                    if (q < q_hi) {
-                        const int id = p.sorted_ids[q];
-                        const NodeRecord& rec = p.records[id];
-                        wz[i] = rec.wz[j];
-                        tr[i] = rec.tr;
-                        f[i] = rec.f[k];
-                        ang[i] = p.angular[(size_t)(id % count) * p.channels + c];
-                    }
-                }
+    for (int a = 0; a < kDepositAcc; ++a) acc[a] = 0.0;
+    for (int l = 0; l < links; ++l) {
+        const int j = link[l][0], dt = link[l][1];
+        const int q_hi = range[l][1];
+        for (int q0 = range[l][0]; q0 < q_hi; q0 += tile) {
+            const int n = min(tile, q_hi - q0);
+            __syncthreads();   // the previous tile has been read
+            if ((int)threadIdx.x < n) {
+                const int id = p.sorted_ids[q0 + threadIdx.x];
+                const NodeRecord& rec = p.records[id];
+                double wt = L.time_deposit == 1 ? (dt ? rec.tr : 1 - rec.tr) : 1.0;
+                double w = rec.wz[j] * wt;
 #pragma unroll
-                for (int i = 0; i < kNodes; ++i) {
-                    if (q0 + i >= q_hi) break;
-                    double wt = L.time_deposit == 1 ? (dt ? tr[i] : 1 - tr[i]) : 1.0;
-                    double w = wz[i] * wt;
-                    acc += (w * f[i]) * ang[i];
+                for (int k = 0; k < 5; ++k) s_wf[threadIdx.x * 5 + k] = w * rec.f[k];
+                s_id[threadIdx.x] = id % count;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < n * C; e += kThreads) {
+                const int i = e / C, c = e - i * C;
+                s_ang[e] = p.angular[(size_t)s_id[i] * C + c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < kDepositAcc; ++a) {
+                const int kc = threadIdx.x + a * kThreads;
+                if (kc < width) {
+                    const int k = kc / C, c = kc - k * C;
+                    double sum = acc[a];
+                    for (int i = 0; i < n; ++i) sum += s_wf[i * 5 + k] * s_ang[i * C + c];
+                    acc[a] = sum;
                 }
             }
         }
-        deposit[(size_t)r * width + kc] = acc;
+    }
+#pragma unroll
+    for (int a = 0; a < kDepositAcc; ++a) {
+        const int kc = threadIdx.x + a * kThreads;
+        if (kc < width) deposit[(size_t)r * width + kc] = acc[a];
     }
 }
 
@@ -740,7 +772,8 @@ int lh_compile_channels(const lh_compile_field* field, const lh_compile_lattice*
     hipStream_t s = static_cast<hipStream_t>(stream);
     {
         lh::KernelMark mark("compile_deposit", s);
-        compile_deposit<<<rows, kThreads, 0, s>>>(p, rows, deposit);
+        compile_deposit<<<rows, kThreads, deposit_lds_bytes(p.channels), s>>>(
+            p, rows, deposit_tile(p.channels), deposit);
     }
     if ((status = lh::check_launch("compile_deposit")) != LH_OK) return status;
     // the phase table follows the rows' deposits in the deposit buffer
