@@ -11,6 +11,7 @@
 //    scheduling.
 #include "lh_common.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace {
@@ -18,14 +19,38 @@ namespace {
 constexpr int kThreads = 256;
 
 constexpr int kBoundOms = 8;          // OMs per workgroup of segment_bounds
-constexpr int kBoundThreads = 512;
+constexpr int kBoundThreads = 256;
+constexpr int kBoundBlocks = 2048;    // target workgroups of one launch
+constexpr int kBoundMinElements = 1024;   // least elements of one split
 
-// Nearest approach and earliest arrival of every OM over all elements: a
-// workgroup takes kBoundOms OMs, so each element it loads serves all of them
-// (the element list is read once per 8 OMs, not once per OM). Minima do not
-// depend on the order. An element whose start time is not below the running
-// earliest arrival cannot lower it (the flight term is >= 0), so its square
-// root is skipped.
+// Orderable 64-bit encoding of a double (unsigned order = numeric order).
+__device__ inline unsigned long long order_bits(double x) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+
+__device__ inline double from_order_bits(unsigned long long b) {
+    return __longlong_as_double((long long)((b >> 63) ? (b & 0x7fffffffffffffffULL) : ~b));
+}
+
+// The outputs hold the running minima as order_bits encodings until
+// segment_bounds_finish decodes them; all ones lies above every encoding.
+__global__ void __launch_bounds__(kThreads) segment_bounds_init(int n_receivers,
+                                                                double* distance,
+                                                                double* earliest) {
+    const int d = blockIdx.x * kThreads + threadIdx.x;
+    if (d >= n_receivers) return;
+    if (distance) reinterpret_cast<unsigned long long*>(distance)[d] = ~0ULL;
+    if (earliest) reinterpret_cast<unsigned long long*>(earliest)[d] = ~0ULL;
+}
+
+// Nearest approach and earliest arrival of kBoundOms OMs over one split of
+// the elements (blockIdx.y of gridDim.y equal splits): each element a
+// workgroup loads serves all its OMs, and the splits keep the chip full when
+// the OMs are few. Minima do not depend on the order, so the workgroup minima
+// go to the outputs with atomic minima on their ordered encodings. An element
+// whose start time is not below the running earliest arrival cannot lower it
+// (the flight term is >= 0), so its square root is skipped.
 __global__ void __launch_bounds__(kBoundThreads) segment_bounds(const double* receivers,
                                                                 int n_receivers,
                                                                 const double* starts,
@@ -36,6 +61,8 @@ __global__ void __launch_bounds__(kBoundThreads) segment_bounds(const double* re
                                                                 double* earliest) {
     __shared__ double part[kBoundThreads / 64][2 * kBoundOms];
     const int first = blockIdx.x * kBoundOms;
+    const int per = (n_el + gridDim.y - 1) / gridDim.y;
+    const int lo = blockIdx.y * per, hi = min(n_el, lo + per);
     double rx[kBoundOms], ry[kBoundOms], rz[kBoundOms], bd[kBoundOms], bt[kBoundOms];
 #pragma unroll
     for (int k = 0; k < kBoundOms; ++k) {
@@ -46,7 +73,7 @@ __global__ void __launch_bounds__(kBoundThreads) segment_bounds(const double* re
         bd[k] = DBL_MAX;
         bt[k] = DBL_MAX;
     }
-    for (int i = threadIdx.x; i < n_el; i += kBoundThreads) {
+    for (int i = lo + threadIdx.x; i < hi; i += kBoundThreads) {
         const double sx = starts[3 * i], sy = starts[3 * i + 1], sz = starts[3 * i + 2];
         const double ux = directions[3 * i], uy = directions[3 * i + 1], uz = directions[3 * i + 2];
         const double len = lengths[i];
@@ -83,13 +110,27 @@ __global__ void __launch_bounds__(kBoundThreads) segment_bounds(const double* re
         for (int w = 1; w < kBoundThreads / 64; ++w) best = fmin(best, part[w][threadIdx.x]);
         const int k = threadIdx.x % kBoundOms;
         const int d = first + k;
-        if (d < n_receivers) {
-            if (threadIdx.x < kBoundOms) {
-                if (distance) distance[d] = sqrt(best);
-            } else if (earliest) {
-                earliest[d] = best;
-            }
+        if (d < n_receivers && best < DBL_MAX) {
+            double* out = threadIdx.x < kBoundOms ? distance : earliest;
+            if (out) atomicMin(reinterpret_cast<unsigned long long*>(out) + d, order_bits(best));
         }
+    }
+}
+
+// Decodes the minima: the distance is the square root of the nearest squared
+// distance.
+__global__ void __launch_bounds__(kThreads) segment_bounds_finish(int n_receivers,
+                                                                  double* distance,
+                                                                  double* earliest) {
+    const int d = blockIdx.x * kThreads + threadIdx.x;
+    if (d >= n_receivers) return;
+    if (distance) {
+        const unsigned long long b = reinterpret_cast<const unsigned long long*>(distance)[d];
+        distance[d] = sqrt(from_order_bits(b));
+    }
+    if (earliest) {
+        const unsigned long long b = reinterpret_cast<const unsigned long long*>(earliest)[d];
+        earliest[d] = from_order_bits(b);
     }
 }
 
@@ -183,8 +224,18 @@ extern "C" int lh_segment_bounds(const double* receivers, int n_receivers, const
     LH_REQUIRE(n_receivers >= 1 && n_segments >= 1, "need receivers and segments");
     LH_REQUIRE(earliest == nullptr || start_ns != nullptr, "earliest needs start_ns");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    segment_bounds<<<(n_receivers + kBoundOms - 1) / kBoundOms, kBoundThreads, 0, s>>>(
+    if (!distance && !earliest) return LH_OK;
+    const int groups = (n_receivers + kBoundOms - 1) / kBoundOms;
+    const int splits = std::max(1, std::min((kBoundBlocks + groups - 1) / groups,
+                                            n_segments / kBoundMinElements));
+    const unsigned flat = (unsigned)((n_receivers + kThreads - 1) / kThreads);
+    segment_bounds_init<<<flat, kThreads, 0, s>>>(n_receivers, distance, earliest);
+    int status = lh::check_launch("segment_bounds_init");
+    if (status != LH_OK) return status;
+    segment_bounds<<<dim3(groups, splits), kBoundThreads, 0, s>>>(
         receivers, n_receivers, starts, directions, lengths, start_ns, n_segments, flight,
         distance, earliest);
-    return lh::check_launch("segment_bounds");
+    if ((status = lh::check_launch("segment_bounds")) != LH_OK) return status;
+    segment_bounds_finish<<<flat, kThreads, 0, s>>>(n_receivers, distance, earliest);
+    return lh::check_launch("segment_bounds_finish");
 }
