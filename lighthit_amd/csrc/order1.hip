@@ -1117,11 +1117,16 @@ __global__ void __launch_bounds__(kThreads) o1_tile_groups(Group G, int groups) 
 // keep about twice the registers live.
 template <int P>
 __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups) {
-    __shared__ int head_all[kThreads / 64][kPieceEntries];
-    int lane = threadIdx.x & 63;
-    int* head = head_all[threadIdx.x >> 6];
     constexpr int n_mono = Monomials<P>::N;
     constexpr int nv = 2 * n_mono + 6;
+    // member rows, value-major with a padded stride: the lanes summing one
+    // piece's values read one member at a time from distinct banks
+    constexpr int kStride = kPieceEntries + 1;
+    __shared__ int head_all[kThreads / 64][kPieceEntries];
+    __shared__ int piece_lane_all[kThreads / 64][kPieceEntries + 1];
+    __shared__ double rows_all[kThreads / 64][nv * kStride];
+    int lane = threadIdx.x & 63;
+    int* head = head_all[threadIdx.x >> 6];
     long long seg = (long long)G.slots * G.n_pen;
     long long members = G.member_off[groups];
     if (G.work_counts != nullptr && blockIdx.x == 0 && threadIdx.x == 0)
@@ -1184,34 +1189,39 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
             row[2 * n_mono + 4] = omega * x.tau;
             row[2 * n_mono + 5] = omega * x.tau * x.tau;
         }
-        // inclusive segmented scan: lane i adds lane i - off when both lie in
-        // one group (groups are contiguous runs of lanes)
-        unsigned take = 0;
+        // the tile's pieces (maximal runs of one group among the valid lanes,
+        // which are a prefix of the wave) are summed from LDS: one lane per
+        // (piece, value) adds the piece's members in lane order
+        double* rows = rows_all[threadIdx.x >> 6];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            int other = __shfl_up(g, 1 << s, 64);
-            if (lane >= (1 << s) && other == g) take |= 1u << s;
+        for (int m = 0; m < nv; ++m) rows[m * kStride + lane] = row[m];
+        int* piece_lane = piece_lane_all[threadIdx.x >> 6];
+        int* piece_group = head;   // the heads have been read
+        const int prev = __shfl_up(g, 1, 64);
+        const bool start = valid && (lane == 0 || prev != g);
+        const unsigned long long starts = __ballot(start);
+        const int n_pieces = __popcll(starts);
+        const int n_valid = __popcll(__ballot(valid));
+        if (start) {
+            const int q = __popcll(starts & ((1ULL << lane) - 1ULL));
+            piece_lane[q] = lane;
+            piece_group[q] = g;
         }
-#pragma unroll
-        for (int m = 0; m < nv; ++m) {
-            double value = row[m];
-#pragma unroll
-            for (int s = 0; s < 6; ++s) {
-                double other = __shfl_up(value, 1 << s, 64);
-                if ((take >> s) & 1u) value += other;
-            }
-            row[m] = value;
-        }
-        int next = __shfl_down(g, 1, 64);
-        if (valid && (lane == 63 || next != g)) {
+        if (lane == 0) piece_lane[n_pieces] = n_valid;
+        wave_sync();
+        for (int task = lane; task < n_pieces * nv; task += 64) {
+            const int q = task / nv, m = task - q * nv;
+            const int a = piece_lane[q], b = piece_lane[q + 1];
+            const int gq = piece_group[q];
+            double sum = rows[m * kStride + a];
+            for (int i = a + 1; i < b; ++i) sum += rows[m * kStride + i];
             // a group within one tile is complete: its row goes straight to values
-            int first = G.piece_start[g];
-            double* out = G.piece_start[g + 1] - first == 1
-                              ? G.values + (size_t)g * nv
+            const int first = G.piece_start[gq];
+            double* out = G.piece_start[gq + 1] - first == 1
+                              ? G.values + (size_t)gq * nv
                               : G.piece_values
-                                    + (size_t)(first + (t - G.member_off[g] / kPieceEntries)) * nv;
-#pragma unroll
-            for (int m = 0; m < nv; ++m) out[m] = row[m];
+                                    + (size_t)(first + (t - G.member_off[gq] / kPieceEntries)) * nv;
+            out[m] = sum;
         }
     }
 }
