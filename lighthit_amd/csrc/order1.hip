@@ -1268,15 +1268,29 @@ __global__ void __launch_bounds__(kThreads) o1_batch_starts(Group G, int groups,
 
 // ---- batch core
 
+// The first N terms z^n / (n! (n + j + 1)) of i_j, j = 0, 1, 2 (Horner).
+template <int N>
+__device__ __forceinline__ void taylor_terms(double z, double* i0, double* i1, double* i2) {
+    double a = kTaylor[0][N - 1], b = kTaylor[1][N - 1], c = kTaylor[2][N - 1];
+#pragma unroll
+    for (int n = N - 2; n >= 0; --n) {
+        a = a * z + kTaylor[0][n];
+        b = b * z + kTaylor[1][n];
+        c = c * z + kTaylor[2][n];
+    }
+    *i0 = a; *i1 = b; *i2 = c;
+}
+
+// Inside |z| < 1 the series; the terms beyond the first N are below the FP64
+// rounding of the sum (>= 1/3) when |z| < 1/16 and N = 9 (remainder < 1e-17)
+// or |z| < 1/4 and N = 12 (< 1e-17), else all 24 are taken. The wave takes
+// the longest series any of its lanes needs, so the coefficients stay
+// uniform loads.
 __device__ void ifuncs(double z, double ez, double* i0, double* i1, double* i2) {
     if (fabs(z) < 1.0) {
-        double a = kTaylor[0][23], b = kTaylor[1][23], c = kTaylor[2][23];
-        for (int n = 22; n >= 0; --n) {
-            a = a * z + kTaylor[0][n];
-            b = b * z + kTaylor[1][n];
-            c = c * z + kTaylor[2][n];
-        }
-        *i0 = a; *i1 = b; *i2 = c;
+        if (__ballot(fabs(z) >= 0.25) != 0) taylor_terms<24>(z, i0, i1, i2);
+        else if (__ballot(fabs(z) >= 0.0625) != 0) taylor_terms<12>(z, i0, i1, i2);
+        else taylor_terms<9>(z, i0, i1, i2);
         return;
     }
     *i0 = (ez - 1.0) / z;
@@ -1670,6 +1684,7 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
     int n = h.n;
     double r_b = h.r_b, t_b = h.t_b;
     double v = G.opt.speed[lam];
+    const double inv_v = 1.0 / v;   // one division per batch, products per panel
     double mut = G.opt.mut[lam];
     double kappa = -mut * v;
     // last = the serial walk-back `while (last > 2 && path[last - 1] >
@@ -1683,9 +1698,9 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
     }
     int last = min(max(within, 2), n - 1);
     if (last < 2) return 0.0;
-    double spread = fmin(sqrt(12.0 * h.tau_var), (path[last] - r_b) / v);
-    double front_time = t_b + r_b / v;
-    double end_time = t_b + path[last] / v;
+    double spread = fmin(sqrt(12.0 * h.tau_var), (path[last] - r_b) * inv_v);
+    double front_time = t_b + r_b * inv_v;
+    double end_time = t_b + path[last] * inv_v;
     int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
                   || end_time + 0.5 * spread <= G.opt.edges[0];
     double t_base = outside ? 0.0 : t_b;
@@ -1718,9 +1733,9 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
             int j;
             if (i < last / 2) {
                 j = 2 * i;
-                t_a = t_base + path[j] / v;
-                double t_m = t_base + path[j + 1] / v;
-                t_e = t_base + path[j + 2] / v;
+                t_a = t_base + path[j] * inv_v;
+                double t_m = t_base + path[j + 1] * inv_v;
+                t_e = t_base + path[j + 2] * inv_v;
                 double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
                 double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
                 double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
@@ -1733,9 +1748,9 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
                 e_b = e_emit * exp(-mut * path[j + 2]);
             } else {
                 j = last - 1;
-                double t_p = t_base + path[j - 1] / v;
-                t_a = t_base + path[j] / v;
-                t_e = t_base + path[j + 1] / v;
+                double t_p = t_base + path[j - 1] * inv_v;
+                t_a = t_base + path[j] * inv_v;
+                t_e = t_base + path[j + 1] * inv_v;
                 double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
                 double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
                 double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
@@ -1757,7 +1772,7 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
         events_total += deposit_round(ev, lane, B, ev_bin, ev_val, head_at);
     }
     double total = wave_sum(total_lane);
-    double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
+    double t_f = t_base + path[0] * inv_v, t_end = t_base + path[last] * inv_v;
     double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
     total += exact;
     // the front ramp and flat tail after every panel, as one more round
@@ -1893,6 +1908,7 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
     int n = h.n;
     double r_b = h.r_b, t_b = h.t_b;
     double v = G.opt.speed[lam_s];
+    const double inv_v = 1.0 / v;   // one division per batch, products per panel
     double mut = G.opt.mut[lam_s];
     double kappa = -mut * v;
     // last as in deposit_lambda, per half
@@ -1906,9 +1922,9 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
     int last = min(max(within, 2), n - 1);
     if (last < 2) active = false;
     if (!active) last = 0;   // an idle half reads node 0 only (every batch has one)
-    double spread = fmin(sqrt(12.0 * h.tau_var), (path[last] - r_b) / v);
-    double front_time = t_b + r_b / v;
-    double end_time = t_b + path[last] / v;
+    double spread = fmin(sqrt(12.0 * h.tau_var), (path[last] - r_b) * inv_v);
+    double front_time = t_b + r_b * inv_v;
+    double end_time = t_b + path[last] * inv_v;
     int outside = front_time - 0.5 * spread >= G.opt.edges[nb]
                   || end_time + 0.5 * spread <= G.opt.edges[0];
     double t_base = outside ? 0.0 : t_b;
@@ -1941,9 +1957,9 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
             int j;
             if (i < last / 2) {
                 j = 2 * i;
-                t_a = t_base + path[j] / v;
-                double t_m = t_base + path[j + 1] / v;
-                t_e = t_base + path[j + 2] / v;
+                t_a = t_base + path[j] * inv_v;
+                double t_m = t_base + path[j + 1] * inv_v;
+                t_e = t_base + path[j + 2] * inv_v;
                 double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
                 double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
                 double h2 = (w0 * g0[j + 2] + w2 * g2[j + 2]) - h_front;
@@ -1956,9 +1972,9 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
                 e_b = e_emit * exp(-mut * path[j + 2]);
             } else {
                 j = last - 1;
-                double t_p = t_base + path[j - 1] / v;
-                t_a = t_base + path[j] / v;
-                t_e = t_base + path[j + 1] / v;
+                double t_p = t_base + path[j - 1] * inv_v;
+                t_a = t_base + path[j] * inv_v;
+                t_e = t_base + path[j + 1] * inv_v;
                 double hp = (w0 * g0[j - 1] + w2 * g2[j - 1]) - h_front;
                 double h0 = (w0 * g0[j] + w2 * g2[j]) - h_front;
                 double h1 = (w0 * g0[j + 1] + w2 * g2[j + 1]) - h_front;
@@ -1979,7 +1995,7 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
                                            ev_val, head_at);
     }
     double total = half_sum(total_lane);
-    double t_f = t_base + path[0] / v, t_end = t_base + path[last] / v;
+    double t_f = t_base + path[0] * inv_v, t_end = t_base + path[last] * inv_v;
     double exact = quad_integral(e_first, h_front, 0.0, 0.0, t_end - t_f, kappa, e_last);
     total += exact;
     // the front ramp and flat tail after every panel, as one more round
