@@ -353,24 +353,14 @@ __device__ Slot make_slot(const O0Params& P, int slot, int pieces, const double*
     return out;
 }
 
-__device__ __forceinline__ Slot shuffle(const Slot& s, int lane) {
-    Slot o;
-    o.weight = __shfl(s.weight, lane);
-    o.D = __shfl(s.D, lane);
-    o.D2R2 = __shfl(s.D2R2, lane);
-    o.eps = __shfl(s.eps, lane);
-    o.hav_eps = __shfl(s.hav_eps, lane);
-    o.sinpsi = __shfl(s.sinpsi, lane);
-    o.psi_theta_a = __shfl(s.psi_theta_a, lane);
-    o.theta_a = __shfl(s.theta_a, lane);
-    o.cos_theta_a = __shfl(s.cos_theta_a, lane);
-    o.ta = __shfl(s.ta, lane);
-    o.tb = __shfl(s.tb, lane);
-    o.base = __shfl(s.base, lane);
-    o.DmR = __shfl(s.DmR, lane);
-    o.piece = __shfl(s.piece, lane);
-    return o;
-}
+// A non-empty slot queued for evaluation with the constants of its pair.
+struct Task {
+    Slot S;
+    float beta, c0, c2, inv_length, axial, across, side;
+    int ne;          // azimuth nodes of the pair
+};
+
+constexpr int kTaskQueue = 72;   // at most 7 left over + one group of 64 slots
 
 struct PanelTables {
     // medium panels between consecutive major lines
@@ -889,6 +879,7 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     __shared__ double wave_roots[kWaves][2 * kMaxMajor];
     __shared__ double wave_scut[kWaves][2 * kMaxMajor + 2];
     __shared__ int wave_nscut[kWaves];
+    __shared__ Task wave_tasks[kWaves][kTaskQueue];
     __shared__ int glass;
     __shared__ int range_lo, range_hi;
     const int tid = threadIdx.x;
@@ -971,6 +962,38 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
     const double R = P.radii[om];
     const double norm = P.areas[om] / (kPi * R * R) / (2.0 * kPi);
     double lane_sum = 0.0;
+    unsigned long long lane_nodes = 0;
+    // non-empty slots of the wave's pairs, evaluated 8 at a time (one per
+    // group of 8 lanes, a theta node per lane) whatever pair they belong to
+    Task* tasks = wave_tasks[wave];
+    int queued = 0;
+    auto wave_fence = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto evaluate = [&](int first, int n) {
+        if (sub >= n) return;
+        const Task& t = tasks[first + sub];
+        lane_nodes += (unsigned long long)t.ne;
+        if (kBins) {
+            if (t.ne == 4)
+                slot_bins<4>(T, t.S, P.npanel, P.nalpha, t.beta, t.c0, t.c2, t.inv_length,
+                             t.axial, t.across, t.side, xt, wt, (float)R, edges, P.nbins, it,
+                             lane - it, wave_bins);
+            else
+                slot_bins<kNe>(T, t.S, P.npanel, P.nalpha, t.beta, t.c0, t.c2, t.inv_length,
+                               t.axial, t.across, t.side, xt, wt, (float)R, edges, P.nbins, it,
+                               lane - it, wave_bins);
+        } else {
+            const float value = t.ne == 4
+                ? theta_node_value<4>(T, t.S, P.npanel, P.nalpha, t.beta, t.c0, t.c2,
+                                      t.inv_length, t.axial, t.across, t.side, xt, wt)
+                : theta_node_value<kNe>(T, t.S, P.npanel, P.nalpha, t.beta, t.c0, t.c2,
+                                        t.inv_length, t.axial, t.across, t.side, xt, wt);
+            lane_sum += (double)(t.S.weight * value);
+        }
+    };
     for (int entry = wave; entry < count; entry += kWaves) {
         const int i = list[entry];
         const double ux = E.dir[3 * i], uy = E.dir[3 * i + 1], uz = E.dir[3 * i + 2];
@@ -1037,44 +1060,49 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
         const int slots = (nscut - 1) * ns * pieces;
         const float c0 = (float)E.c0[i], c2 = (float)E.c2[i];
         const float inv_length = (float)(1.0 / L);
-        float pair_sum = 0.f;
-        int evaluated = 0;
         for (int group = 0; group < slots; group += 64) {
             const Slot mine = (group + lane < slots)
                 ? make_slot(P, group + lane, pieces, scut, nscut, ns, z, rho, L, R, beta, theta_major, norm,
                             t_start, tau)
                 : Slot{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
-            const int active = min(64, slots - group);
-            for (int k0 = 0; k0 < active; k0 += 8) {
-                const Slot S = shuffle(mine, min(k0 + sub, 63));
-                if (k0 + sub < active && S.piece >= 0) {
-                    ++evaluated;
-                    if (kBins)
-                        if (ne == 4)
-                            slot_bins<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2, inv_length,
-                                         axial, across, side, xt, wt, (float)R, edges, P.nbins,
-                                         it, lane - it, wave_bins);
-                        else
-                            slot_bins<kNe>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
-                                           inv_length, axial, across, side, xt, wt, (float)R, edges,
-                                           P.nbins, it, lane - it, wave_bins);
-                    else
-                        pair_sum += S.weight * (ne == 4
-                            ? theta_node_value<4>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
-                                                  inv_length, axial, across, side, xt, wt)
-                            : theta_node_value<kNe>(T, S, P.npanel, P.nalpha, (float)beta, c0, c2,
-                                                    inv_length, axial, across, side, xt, wt));
-                }
+            // the group's non-empty slots (a cap meets few of the medium
+            // panels) join the wave's queue in slot order
+            const unsigned long long full = __ballot(mine.piece >= 0);
+            if (mine.piece >= 0) {
+                Task& t = tasks[queued + __popcll(full & ((1ULL << lane) - 1ULL))];
+                t.S = mine;
+                t.beta = (float)beta;
+                t.c0 = c0;
+                t.c2 = c2;
+                t.inv_length = inv_length;
+                t.axial = axial;
+                t.across = across;
+                t.side = side;
+                t.ne = ne;
             }
-        }
-        lane_sum += (double)pair_sum;
-        if (P.node_count != nullptr) {   // one theta node per call, ne azimuth nodes each
-            for (int offset = 32; offset > 0; offset >>= 1) evaluated += __shfl_down(evaluated, offset);
-            if (lane == 0) atomicAdd(P.node_count, (unsigned long long)evaluated * ne);
+            queued += __popcll(full);
+            wave_fence();
+            int done = 0;
+            for (; queued - done >= 8; done += 8) evaluate(done, 8);
+            // the rest (fewer than 8) moves to the front of the queue
+            const int left = queued - done;
+            if (done > 0 && left > 0) {
+                Task keep;
+                if (lane < left) keep = tasks[done + lane];
+                wave_fence();
+                if (lane < left) tasks[lane] = keep;
+            }
+            queued = left;
+            wave_fence();
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (queued > 0) evaluate(0, queued);
+    if (P.node_count != nullptr) {   // one theta node per lane, ne azimuth nodes each
+        for (int offset = 32; offset > 0; offset >>= 1) lane_nodes += __shfl_down(lane_nodes, offset);
+        if (lane == 0 && lane_nodes) atomicAdd(P.node_count, lane_nodes);
     }
     if (kBins) {
         // waves in a fixed order into the first histogram; only the bins from
