@@ -994,77 +994,90 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
             lane_sum += (double)(t.S.weight * value);
         }
     };
-    for (int entry = wave; entry < count; entry += kWaves) {
-        const int i = list[entry];
-        const double ux = E.dir[3 * i], uy = E.dir[3 * i + 1], uz = E.dir[3 * i + 2];
-        const double wx = pos[0] - E.start[3 * i], wy = pos[1] - E.start[3 * i + 1],
-                     wz = pos[2] - E.start[3 * i + 2];
-        const double z = wx * ux + wy * uy + wz * uz;
-        const double tx = wx - z * ux, ty = wy - z * uy, tz = wz - z * uz;
-        const double rho = sqrt(tx * tx + ty * ty + tz * tz);
-        if (!(rho > R)) {
-            if (lane == 0) atomicOr(&P.flags[0], 2);
-            continue;
-        }
-        const double e1x = tx / rho, e1y = ty / rho, e1z = tz / rho;
-        const double e2x = uy * e1z - uz * e1y, e2y = uz * e1x - ux * e1z, e2z = ux * e1y - uy * e1x;
-        const float axial = (float)(ux * look[0] + uy * look[1] + uz * look[2]);
-        const float across = (float)(e1x * look[0] + e1y * look[1] + e1z * look[2]);
-        const float side = (float)(e2x * look[0] + e2y * look[1] + e2z * look[2]);
-        const double L = E.length[i], beta = E.beta[i];
-        const double t_start = kBins ? E.start_ns[i] - P.origins[om] : 0.0;
-        const double tau = kBins ? (E.end_ns[i] - E.start_ns[i]) / L : 0.0;
-        // cone angles of the medium panel edges (one lane each) and the s
-        // pieces: [0, L] cut where a panel edge is tangent to the cap
-        if (lane < P.nmajor) {
-            const double c = 1.0 / (beta * P.phase[P.major[lane]]);
-            const double sn = sqrt(fmax(0.0, (1.0 - c) * (1.0 + c)));
-            wave_theta[wave][lane] = atan2(sn, c);
-            double r1 = -1.0, r2 = -1.0;
-            if (sn > 0.0) {
-                const double centre = z - rho * c / sn, spread = R / sn;
-                r1 = centre - spread;
-                r2 = centre + spread;
+    // One loop over the groups of 64 slots of the wave's pairs: a group joins
+    // the queue, then every full round of 8 queued slots is evaluated (all of
+    // them once the last pair is done), so the evaluation has one call site.
+    int entry = wave;           // the wave's next pair
+    int group = 0, slots = 0;   // the current pair's next group and its slots
+    double z = 0.0, rho = 0.0, L = 0.0, beta = 0.0, t_start = 0.0, tau = 0.0;
+    float axial = 0.f, across = 0.f, side = 0.f, c0 = 0.f, c2 = 0.f, inv_length = 0.f;
+    int ns = 0, ne = 0, nscut = 0;
+    const double* scut = wave_scut[wave];
+    const double* theta_major = wave_theta[wave];
+    for (;;) {
+        while (group >= slots && entry < count) {
+            const int i = list[entry];
+            entry += kWaves;
+            const double ux = E.dir[3 * i], uy = E.dir[3 * i + 1], uz = E.dir[3 * i + 2];
+            const double wx = pos[0] - E.start[3 * i], wy = pos[1] - E.start[3 * i + 1],
+                         wz = pos[2] - E.start[3 * i + 2];
+            z = wx * ux + wy * uy + wz * uz;
+            const double tx = wx - z * ux, ty = wy - z * uy, tz = wz - z * uz;
+            rho = sqrt(tx * tx + ty * ty + tz * tz);
+            if (!(rho > R)) {
+                if (lane == 0) atomicOr(&P.flags[0], 2);
+                continue;
             }
-            wave_roots[wave][2 * lane] = r1;
-            wave_roots[wave][2 * lane + 1] = r2;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane == 0) {
-            double* cut = wave_scut[wave];
-            int n = 0;
-            cut[n++] = 0.0;
-            for (int m = 0; m < 2 * P.nmajor; ++m) {
-                const double r = wave_roots[wave][m];
-                if (r > 0.0 && r < L) cut[n++] = r;
-            }
-            cut[n++] = L;
-            for (int a = 1; a < n; ++a)  // insertion sort (few entries)
-                for (int b = a; b > 0 && cut[b - 1] > cut[b]; --b) {
-                    const double tmp = cut[b];
-                    cut[b] = cut[b - 1];
-                    cut[b - 1] = tmp;
+            const double e1x = tx / rho, e1y = ty / rho, e1z = tz / rho;
+            const double e2x = uy * e1z - uz * e1y, e2y = uz * e1x - ux * e1z,
+                         e2z = ux * e1y - uy * e1x;
+            axial = (float)(ux * look[0] + uy * look[1] + uz * look[2]);
+            across = (float)(e1x * look[0] + e1y * look[1] + e1z * look[2]);
+            side = (float)(e2x * look[0] + e2y * look[1] + e2z * look[2]);
+            L = E.length[i];
+            beta = E.beta[i];
+            t_start = kBins ? E.start_ns[i] - P.origins[om] : 0.0;
+            tau = kBins ? (E.end_ns[i] - E.start_ns[i]) / L : 0.0;
+            // cone angles of the medium panel edges (one lane each) and the s
+            // pieces: [0, L] cut where a panel edge is tangent to the cap
+            if (lane < P.nmajor) {
+                const double c = 1.0 / (beta * P.phase[P.major[lane]]);
+                const double sn = sqrt(fmax(0.0, (1.0 - c) * (1.0 + c)));
+                wave_theta[wave][lane] = atan2(sn, c);
+                double r1 = -1.0, r2 = -1.0;
+                if (sn > 0.0) {
+                    const double centre = z - rho * c / sn, spread = R / sn;
+                    r1 = centre - spread;
+                    r2 = centre + spread;
                 }
-            wave_nscut[wave] = n;
+                wave_roots[wave][2 * lane] = r1;
+                wave_roots[wave][2 * lane + 1] = r2;
+            }
+            wave_fence();
+            if (lane == 0) {
+                double* cut = wave_scut[wave];
+                int n = 0;
+                cut[n++] = 0.0;
+                for (int m = 0; m < 2 * P.nmajor; ++m) {
+                    const double r = wave_roots[wave][m];
+                    if (r > 0.0 && r < L) cut[n++] = r;
+                }
+                cut[n++] = L;
+                for (int a = 1; a < n; ++a)  // insertion sort (few entries)
+                    for (int b = a; b > 0 && cut[b - 1] > cut[b]; --b) {
+                        const double tmp = cut[b];
+                        cut[b] = cut[b - 1];
+                        cut[b - 1] = tmp;
+                    }
+                wave_nscut[wave] = n;
+            }
+            wave_fence();
+            nscut = wave_nscut[wave];
+            ns = s_nodes<kBins>(L, R, P.s_rule);
+            ne = azimuth_nodes(rho, R, P.azimuth_rule);
+            slots = (nscut - 1) * ns * pieces;
+            group = 0;
+            c0 = (float)E.c0[i];
+            c2 = (float)E.c2[i];
+            inv_length = (float)(1.0 / L);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int nscut = wave_nscut[wave];
-        const double* scut = wave_scut[wave];
-        const double* theta_major = wave_theta[wave];
-        const int ns = s_nodes<kBins>(L, R, P.s_rule);
-        const int ne = azimuth_nodes(rho, R, P.azimuth_rule);
-        const int slots = (nscut - 1) * ns * pieces;
-        const float c0 = (float)E.c0[i], c2 = (float)E.c2[i];
-        const float inv_length = (float)(1.0 / L);
-        for (int group = 0; group < slots; group += 64) {
+        const bool finished = group >= slots;   // no pair left
+        if (!finished) {
             const Slot mine = (group + lane < slots)
-                ? make_slot(P, group + lane, pieces, scut, nscut, ns, z, rho, L, R, beta, theta_major, norm,
-                            t_start, tau)
+                ? make_slot(P, group + lane, pieces, scut, nscut, ns, z, rho, L, R, beta,
+                            theta_major, norm, t_start, tau)
                 : Slot{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, -1};
+            group += 64;
             // the group's non-empty slots (a cap meets few of the medium
             // panels) join the wave's queue in slot order
             const unsigned long long full = __ballot(mine.piece >= 0);
@@ -1081,25 +1094,23 @@ __global__ __launch_bounds__(kThreads, 2) void order0_pairs(O0Params P) {
                 t.ne = ne;
             }
             queued += __popcll(full);
-            wave_fence();
-            int done = 0;
-            for (; queued - done >= 8; done += 8) evaluate(done, 8);
-            // the rest (fewer than 8) moves to the front of the queue
-            const int left = queued - done;
-            if (done > 0 && left > 0) {
-                Task keep;
-                if (lane < left) keep = tasks[done + lane];
-                wave_fence();
-                if (lane < left) tasks[lane] = keep;
-            }
-            queued = left;
-            wave_fence();
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_fence();
+        int done = 0;
+        for (; queued - done >= 8 || (finished && queued > done); done += 8)
+            evaluate(done, min(8, queued - done));
+        // the rest (fewer than 8) moves to the front of the queue
+        const int left = max(queued - done, 0);
+        if (done > 0 && left > 0) {
+            Task keep;
+            if (lane < left) keep = tasks[done + lane];
+            wave_fence();
+            if (lane < left) tasks[lane] = keep;
+        }
+        queued = left;
+        wave_fence();
+        if (finished) break;
     }
-    if (queued > 0) evaluate(0, queued);
     if (P.node_count != nullptr) {   // one theta node per lane, ne azimuth nodes each
         for (int offset = 32; offset > 0; offset >>= 1) lane_nodes += __shfl_down(lane_nodes, offset);
         if (lane == 0 && lane_nodes) atomicAdd(P.node_count, lane_nodes);
