@@ -68,18 +68,18 @@ O1_MEMBER_BYTES = 22 * 8 + 4
 #   profile term (one path node x one group): scattering geometry 16, the two
 #     cubic acceptance polynomials 28, Henyey-Greenstein weight 8, sums 4;
 #   time panel: quadratic coefficients 30, two attenuation factors 4, the
-#     exponential-quadratic integral 154 (three 24-term Taylor series and the
-#     combination);
-#   bin event: one more exponential-quadratic integral over a bin piece 160.
+#     exponential-quadratic integral 64 (three 9-term Taylor series, the
+#     length FP64 resolves at the panel widths here, and the combination);
+#   bin event: one more exponential-quadratic integral over a bin piece 70.
 O1_PROFILE_TERM_FLOPS = 56
-O1_PANEL_FLOPS = 188
-O1_EVENT_FLOPS = 160
+O1_PANEL_FLOPS = 98
+O1_EVENT_FLOPS = 70
 # FP64 operations of one group member of the order-1 group sums: the pencil's
 # frame at the OM (distance, direction cosines, angle: ~40), its key (log,
 # floor, divisions: ~8), the shape factor or radial rule (~30), the weights
-# and moments at polynomial degree 1 (~18) and its share of the segmented
-# scan over the group (14 values x 6 steps: 84).
-O1_MEMBER_FLOPS = 180
+# and moments at polynomial degree 1 (~18) and its one add per value into its
+# group's sum (14).
+O1_MEMBER_FLOPS = 110
 
 #: Kernels the library times with HIP events on their stream.
 TIMED_KERNELS = ("order0_pairs<false>", "order0_pairs<true>", "o1_batch_profile",
