@@ -86,8 +86,6 @@ struct Compress {
     int* sorted_ids;
     int key32;              // keys below 2^32: stored and sorted as uint32
     double* sub_record;     // (subsegments, kSubRecord) per-sub-segment moment parts
-    double* cp;             // (nodes) ring azimuth cosine
-    double* sp;             // (nodes) ring azimuth sine
     int* sub_element;       // (subsegments) unit element index of a sub-segment
     int* sub_index;         // (subsegments) index of the sub-segment in its element
     int* run_flag;          // (nodes + 1)
@@ -154,8 +152,6 @@ __global__ void __launch_bounds__(kThreads) o1_nodes(Compress k) {
     if (k.key32) reinterpret_cast<unsigned*>(k.keys)[node] = (unsigned)key;
     else k.keys[node] = key;
     k.ids[node] = (int)node;
-    k.cp[node] = cp;
-    k.sp[node] = sp;
 }
 
 // Sorted node key j (stored as uint32 when every key is below 2^32).
@@ -254,7 +250,11 @@ __device__ void node_moments(const Compress& k, int node, double v[kNmom]) {
     }
     const double cc = rec[12], cs = rec[13], w0 = rec[14], w2 = rec[15], t = rec[16];
     const double arc2 = rec[17], sub2 = rec[18];
-    double cp = k.cp[node], sp = k.sp[node];
+    // the ring direction's angle, evaluated as o1_nodes evaluates it
+    const int q = (int)(node - s * R);
+    const double offset = rec[19];
+    double sp, cp;
+    sincos(offset + q * (2.0 * M_PI / R), &sp, &cp);
     double sx = cc * u[0] + cs * (cp * e1[0] + sp * e2[0]);
     double sy = cc * u[1] + cs * (cp * e1[1] + sp * e2[1]);
     double sz = cc * u[2] + cs * (cp * e1[2] + sp * e2[2]);
@@ -2374,7 +2374,7 @@ int bits_for(unsigned long long value) {
 // ---- compress layout
 
 struct CompressLayout {
-    size_t keys, ids, sorted_keys, sorted_ids, cp, sp, sub_element, sub_index, run_flag, run_scan;
+    size_t keys, ids, sorted_keys, sorted_ids, sub_element, sub_index, run_flag, run_scan;
     size_t run_start, run_key, rows, info, sort_temp, scan_temp, pkeys, pids, spkeys, spids,
         sub_record,
         pflag, pscan, psort_temp, piece_count, piece_start, big_count, big_start, piece_rows,
@@ -2418,8 +2418,6 @@ int compress_layout(const lh_order1_grid* grid, int count, long long runs_cap,
     L.ids = take(nodes * sizeof(int));
     L.sorted_keys = take(nodes * sizeof(long long));
     L.sorted_ids = take(nodes * sizeof(int));
-    L.cp = take(nodes * sizeof(double));
-    L.sp = take(nodes * sizeof(double));
     L.sub_element = take(grid->subsegments * sizeof(int));
     L.sub_index = take(grid->subsegments * sizeof(int));
     L.sub_record = take((size_t)grid->subsegments * kSubRecord * sizeof(double));
@@ -2480,8 +2478,6 @@ Compress make_compress(const lh_order1_chords* ch, const lh_order1_grid* grid, c
     k.ids = reinterpret_cast<int*>(base + L.ids);
     k.sorted_keys = reinterpret_cast<long long*>(base + L.sorted_keys);
     k.sorted_ids = reinterpret_cast<int*>(base + L.sorted_ids);
-    k.cp = reinterpret_cast<double*>(base + L.cp);
-    k.sp = reinterpret_cast<double*>(base + L.sp);
     k.sub_element = reinterpret_cast<int*>(base + L.sub_element);
     k.sub_index = reinterpret_cast<int*>(base + L.sub_index);
     k.run_flag = reinterpret_cast<int*>(base + L.run_flag);
