@@ -715,7 +715,7 @@ def serve_processes(workload_flag, precision, detector, elements, processes, ste
 def main():
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=20,
+    parser.add_argument("--steps", type=int, default=60,
                         help="timed steps: batches of --processes served events")
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--workload", choices=["detector", "small", "heavy"], default="detector")
