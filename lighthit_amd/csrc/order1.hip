@@ -1213,8 +1213,19 @@ __global__ void __launch_bounds__(kThreads, 4) o1_group_sums(Group G, int groups
             const int q = task / nv, m = task - q * nv;
             const int a = piece_lane[q], b = piece_lane[q + 1];
             const int gq = piece_group[q];
-            double sum = rows[m * kStride + a];
-            for (int i = a + 1; i < b; ++i) sum += rows[m * kStride + i];
+            // four running sums (the loads issue back to back), combined in a
+            // fixed order
+            const double* r = rows + m * kStride;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int i = a;
+            for (; i + 3 < b; i += 4) {
+                s0 += r[i];
+                s1 += r[i + 1];
+                s2 += r[i + 2];
+                s3 += r[i + 3];
+            }
+            for (; i < b; ++i) s0 += r[i];
+            const double sum = (s0 + s1) + (s2 + s3);
             // a group within one tile is complete: its row goes straight to values
             const int first = G.piece_start[gq];
             double* out = G.piece_start[gq + 1] - first == 1
