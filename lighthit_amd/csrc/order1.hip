@@ -1735,12 +1735,13 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
     int panels = last / 2 + (last & 1);
     double total_lane = 0.0;
     unsigned long long events_total = 0;
+    double carry = e_first;   // e_b of the panel before the round's first lane
     for (int base = 0; base < panels; base += 64) {
         int i = base + lane;
         PanelEvents ev;
         ev.kind = 0;
+        double t_a = 0.0, t_e = 0.0, alpha = 0.0, beta = 0.0, gamma = 0.0, e_b = 0.0;
         if (i < panels) {
-            double t_a, t_e, alpha, beta, gamma, e_a, e_b;
             int j;
             if (i < last / 2) {
                 j = 2 * i;
@@ -1773,9 +1774,14 @@ __device__ __forceinline__ double deposit_lambda(const Group& G, int lam, const 
                 gamma = d2;
                 e_b = e_emit * exp(-mut * path[j + 1]);
             }
-            // the serial loop carries e_a from the previous panel's e_b
-            // (the same expression at node j)
-            e_a = j == 0 ? e_first : e_emit * exp(-mut * path[j]);
+        }
+        // a panel starts at the node its predecessor ends at, so its e_a is the
+        // predecessor's e_b (the serial loop carries it the same way): from
+        // the lane before, or from the previous round's last lane
+        double before = __shfl_up(e_b, 1, 64);
+        double e_a = lane == 0 ? carry : before;
+        carry = __shfl(e_b, 63, 64);
+        if (i < panels) {
             total_lane += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
             if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
         }
@@ -1959,12 +1965,13 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
     int panels_max = max(__shfl(panels, 0, 64), __shfl(panels, 32, 64));
     double total_lane = 0.0;
     unsigned long long events_total = 0;
+    double carry = e_first;   // e_b of the panel before the half's round's first lane
     for (int base = 0; base < panels_max; base += 32) {
         int i = base + sub;
         PanelEvents ev;
         ev.kind = 0;
+        double t_a = 0.0, t_e = 0.0, alpha = 0.0, beta = 0.0, gamma = 0.0, e_b = 0.0;
         if (i < panels) {
-            double t_a, t_e, alpha, beta, gamma, e_a, e_b;
             int j;
             if (i < last / 2) {
                 j = 2 * i;
@@ -1997,7 +2004,12 @@ __device__ __forceinline__ double deposit_pair(const Group& G, int lam_lo, int l
                 gamma = d2;
                 e_b = e_emit * exp(-mut * path[j + 1]);
             }
-            e_a = j == 0 ? e_first : e_emit * exp(-mut * path[j]);
+        }
+        // e_a from the predecessor panel, as in deposit_lambda (per half)
+        double before = __shfl_up(e_b, 1, 64);
+        double e_a = sub == 0 ? carry : before;
+        carry = __shfl(e_b, (lane & 32) | 31, 64);
+        if (i < panels) {
             total_lane += quad_integral(e_a, alpha, beta, gamma, t_e - t_a, kappa, e_b);
             if (bins) panel_events(B, t_a, t_e, e_a, e_b, alpha, beta, gamma, kappa, ev);
         }
